@@ -1,0 +1,178 @@
+"""Pins the FP64 NUFFT oracle (``oracle/nufft.py``) without running MATLAB.
+
+Each test names the reference test or closed form it reproduces.  Tolerances
+are the reference's own (``@nufftOp/tests.m``) unless stated.
+"""
+import numpy as np
+import pytest
+
+from oracle.fftnc import fftnc, ifftnc
+from oracle.nufft import NufftPlan, grid_size, kb_beta, kb_nkern, nudft_centred
+
+
+# -- closed-form plan constants (nufftOp.m:146-156) -----------------------------
+@pytest.mark.parametrize("W,os,beta,nk", [
+    (4, 1.5, 7.8922854737, 2028),
+    (6, 1.5, 12.2481830039, 2840),
+    (6, 1.375, 11.6614092884, 3098),
+    (4, 1.375, 7.4868436367, 2212),
+    (8, 2.0, 18.6389397477, 2738),
+])
+def test_kb_constants(W, os, beta, nk):
+    assert abs(kb_beta(W, os) - beta) < 1e-9
+    assert kb_nkern(W, os) == nk
+
+
+@pytest.mark.parametrize("N,g15,g1375", [
+    (32, 48, 48), (36, 54, 50), (40, 60, 56), (64, 96, 90), (96, 144, 140),
+    (100, 150, 140), (110, 168, 160), (128, 192, 180), (160, 240, 224), (220, 336, 320),
+])
+def test_grid_size(N, g15, g1375):
+    assert grid_size([N], 1.5)[0] == g15
+    assert grid_size([N], 1.375)[0] == g1375
+
+
+def test_grid_sizes_batched_range():
+    gs = sorted({int(grid_size([n], 1.5)[0]) for n in range(96, 161, 2)})
+    assert gs == [144, 150, 160, 162, 168, 180, 192, 196, 200, 210, 216, 224, 240]
+
+
+def _cart_kloc(imN):
+    axes = [(np.arange(n) - n / 2) / n for n in imN]
+    g = np.meshgrid(*axes, indexing="ij")
+    return np.stack([a.reshape(-1, order="F") for a in g], axis=1)
+
+
+def _rand_c(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# -- @nufftOp/tests.m:10-37 adjoint, 1-D..3-D, non-square ------------------------
+@pytest.mark.parametrize("nD", [1, 2, 3])
+def test_adjoint(nD):
+    rng = np.random.default_rng(10 + nD)
+    imN = [32 + 4 * d for d in range(nD)] if nD < 3 else [12, 16, 20]
+    kloc = _cart_kloc(imN)
+    S = NufftPlan(imN, kloc, np.ones(kloc.shape[0]), 1.5, 6)
+    x = _rand_c(rng, imN) + rng.standard_normal()
+    y = _rand_c(rng, [kloc.shape[0]]) + rng.standard_normal()
+    d1 = np.vdot(x.reshape(-1, order="F"), S.adjoint(y).reshape(-1, order="F"))
+    d2 = np.vdot(S.forward(x).reshape(-1, order="F"), y)
+    assert abs(d1 - d2) / min(abs(d1), abs(d2)) < 1e-8
+
+
+# -- @nufftOp/tests.m:39-54 repetitions ------------------------------------------
+def test_repetitions_identical():
+    rng = np.random.default_rng(3)
+    imN = [32, 36]
+    kloc = rng.random((500, 2)) - 0.5
+    S = NufftPlan(imN, kloc, None, 1.5, 4)
+    x = _rand_c(rng, imN)
+    xr = np.repeat(np.repeat(x[..., None, None], 2, axis=2), 2, axis=3)
+    Sx = S.forward(xr)
+    assert np.max(np.abs(np.diff(Sx, axis=1))) < 1e-12
+    assert np.max(np.abs(np.diff(Sx, axis=2))) < 1e-12
+    y = _rand_c(rng, [500])
+    yr = np.repeat(np.repeat(y[:, None, None], 2, axis=1), 2, axis=2)
+    Sy = S.adjoint(yr)
+    assert np.max(np.abs(np.diff(Sy, axis=2))) < 1e-12
+
+
+# -- @nufftOp/tests.m:57-65 circular shift ---------------------------------------
+@pytest.mark.parametrize("nD", [1, 2])
+def test_circular_shift(nD):
+    rng = np.random.default_rng(5)
+    imN = [32, 36][:nD]
+    kloc = _cart_kloc(imN)
+    y = _rand_c(rng, [kloc.shape[0]]) + rng.standard_normal()
+    S1 = NufftPlan(imN, kloc, np.ones(kloc.shape[0]), 1.5, 6)
+    S2 = NufftPlan(imN, kloc + 0.25, np.ones(kloc.shape[0]), 1.5, 6)
+    a = np.abs(S1.adjoint(y)).reshape(-1)
+    b = np.abs(S2.adjoint(y)).reshape(-1)
+    assert abs(np.mean(a - b)) < 1e-5
+
+
+# -- @nufftOp/tests.m:67-87 PSF phase / flatness, and scaling across os, W --------
+@pytest.mark.parametrize("nD", [1, 2])
+def test_psf_and_scaling(nD):
+    imN = [32, 36][:nD]
+    kloc = _cart_kloc(imN)
+    dcf = np.ones(kloc.shape[0])
+    x = np.zeros(imN)
+    x[tuple(n // 2 for n in imN)] = 1
+    Sx1 = NufftPlan(imN, kloc, dcf, 1.5, 6).forward(x)
+    assert np.mean(np.abs(np.angle(Sx1))) < 1e-6
+    assert np.std(np.abs(Sx1)) / np.mean(np.abs(Sx1)) < 1e-4
+    Sx2 = NufftPlan(imN, kloc, dcf, 1.375, 6).forward(x)
+    Sx3 = NufftPlan(imN, kloc, dcf, 1.375, 4).forward(x)
+    tv = [np.mean(np.abs(s)) for s in (Sx1, Sx2, Sx3)]
+    assert np.max(np.abs(np.diff(tv))) < 1e-3
+
+
+# -- @nufftOp/tests.m:90-101 known answer: on-grid NUFFT == centred IDFT ----------
+@pytest.mark.parametrize("imN", [[32], [32, 36], [12, 16, 20]])
+def test_kat_vs_fft(imN):
+    rng = np.random.default_rng(7)
+    kloc = _cart_kloc(imN)
+    S = NufftPlan(imN, kloc, np.ones(kloc.shape[0]), 1.5, 6)
+    im = rng.random(imN)
+    k1 = S.forward(im).reshape(imN, order="F")
+    im1 = fftnc(k1)
+    tv = np.sqrt(np.sum(np.abs(im1 - im) ** 2)) / im.size
+    assert tv < 1e-4
+
+
+def test_kat_odd_half_size_sign():
+    """``fftShiftArray`` (nufftOp.m:262-271) carries exp(i*pi*N/2) per dim, so
+    for N/2 odd the on-grid forward is the centred IDFT times -1."""
+    rng = np.random.default_rng(8)
+    imN = [14, 16]
+    kloc = _cart_kloc(imN)
+    S = NufftPlan(imN, kloc, None, 1.5, 6)
+    im = rng.random(imN)
+    im1 = fftnc(S.forward(im).reshape(imN, order="F"))
+    assert np.sqrt(np.sum(np.abs(im1 + im) ** 2)) / im.size < 1e-4
+
+
+# -- brute-force NUDFT ("+" sign, centred, unitary) bounds the approximation -----
+@pytest.mark.parametrize("W,bound", [(4, 5e-3), (6, 2e-4)])
+def test_vs_nudft(W, bound):
+    rng = np.random.default_rng(11)
+    imN = [16, 20]
+    kloc = rng.random((300, 2)) - 0.5
+    x = _rand_c(rng, imN)
+    S = NufftPlan(imN, kloc, None, 1.5, W)
+    y = S.forward(x).reshape(-1)
+    yr = nudft_centred(imN, kloc, x).reshape(-1)
+    err = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    assert err < bound
+    # a sign flip or a half-grid centring mistake is far outside the bound
+    yneg = nudft_centred(imN, -kloc, x).reshape(-1)
+    assert np.linalg.norm(y - yneg) / np.linalg.norm(yneg) > 0.5
+
+
+def test_dcf_applied_once_each_way():
+    rng = np.random.default_rng(12)
+    imN = [16, 16]
+    kloc = rng.random((200, 2)) - 0.5
+    dcf = rng.random(200) + 0.5
+    S0 = NufftPlan(imN, kloc, None, 1.5, 4)
+    S1 = NufftPlan(imN, kloc, dcf, 1.5, 4)
+    x = _rand_c(rng, imN)
+    y = _rand_c(rng, [200])
+    assert np.allclose(S1.forward(x), dcf * S0.forward(x), rtol=1e-13, atol=1e-13)
+    assert np.allclose(S1.adjoint(y), S0.adjoint(dcf * y), rtol=1e-13, atol=1e-13)
+
+
+def test_ifftnc_fftnc_roundtrip_unitary():
+    rng = np.random.default_rng(13)
+    a = _rand_c(rng, [12, 18, 3])
+    for sh in (0, 1):
+        b = ifftnc(a, 2, sh)
+        assert np.allclose(fftnc(b, 2, sh), a, atol=1e-12)
+        assert abs(np.linalg.norm(b) - np.linalg.norm(a)) < 1e-10
+    # explicit pad: ifft with size at the end equals zero padding at the end
+    p = ifftnc(a[:, :, 0], 2, 0, 1, [16, 24])
+    z = np.zeros((16, 24), complex)
+    z[:12, :18] = a[:, :, 0]
+    assert np.allclose(p, np.fft.ifft2(z) * np.sqrt(16 * 24), atol=1e-13)

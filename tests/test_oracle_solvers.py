@@ -1,0 +1,192 @@
+"""Pins the FP64 solver / rcvr / mrSampFunc oracles on closed forms."""
+import numpy as np
+import pytest
+
+from oracle.mrsamp import mrSampFunc
+from oracle.rcvr import RcvrPlan
+from oracle.solvers import bfista, cgne, powermethod, softthresh
+
+
+def _rand_c(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+class Dense:
+    def __init__(self, A):
+        self.A = A
+
+    def forward(self, x):
+        return self.A @ x
+
+    def adjoint(self, y):
+        return self.A.conj().T @ y
+
+
+# -- rcvrOp (@rcvrOp/tests.m:19-31) --------------------------------------------
+@pytest.mark.parametrize("nD", [1, 2, 3])
+def test_rcvr_adjoint_and_loop(nD):
+    rng = np.random.default_rng(nD)
+    sz = (8,) * nD
+    R = RcvrPlan(_rand_c(rng, sz + (4,)))
+    x = _rand_c(rng, sz)
+    y = _rand_c(rng, sz + (4,))
+    d1 = np.vdot(x.reshape(-1), R.adjoint(y).reshape(-1))
+    d2 = np.vdot(R.forward(x).reshape(-1), y.reshape(-1))
+    assert abs(d1 - d2) / min(abs(d1), abs(d2)) < 1e-8
+    assert np.sum(np.abs(R.adjoint(y) - R.adjoint(y, doLoop=True))) < 1e-8
+
+
+# -- mrSampFunc (mrSampFunc.m:163-216) ------------------------------------------
+@pytest.mark.parametrize("vec_im,vec_da,tik", [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (1, 1, 1)])
+def test_mrsampfunc_adjoint(vec_im, vec_da, tik):
+    rng = np.random.default_rng(7)
+    N, NR = 16, 2
+    R = RcvrPlan(rng.random((N, N, NR)))
+    x = _rand_c(rng, (N, N))
+    mask = (rng.random((N, 1)) > 0.5).astype(float)
+    y = _rand_c(rng, (N, 1, NR))
+    opt = {"sumDims": [2]}
+    xa, ya = x, y
+    if vec_im:
+        opt["imNFull"] = [N, N]
+        xa = x.reshape(-1, order="F")
+    if vec_da:
+        opt["daNFull"] = list(y.shape)
+        ya = y.reshape(-1, order="F")
+    if tik:
+        opt["tikReg"] = 0.01
+        ya = np.concatenate([ya, xa])
+    Fx = mrSampFunc(xa, "notransp", mask, R, opt)
+    Fy = mrSampFunc(ya, "transp", mask, R, opt)
+    d1 = np.vdot(xa.reshape(-1, order="F"), Fy.reshape(-1, order="F"))
+    d2 = np.vdot(Fx.reshape(-1, order="F"), ya.reshape(-1, order="F"))
+    assert abs(d1 - d2) / min(abs(d1), abs(d2)) < 1e-8
+
+
+# -- cgne -------------------------------------------------------------------------
+def test_cgne_least_squares():
+    rng = np.random.default_rng(1)
+    A = _rand_c(rng, (40, 12))
+    b = _rand_c(rng, 40)
+    xls = np.linalg.lstsq(A, b, rcond=None)[0]
+    x, res, *_ , status = cgne(Dense(A), b, np.zeros(12, complex), 60,
+                               {"resThresh": 1e-24, "xChangeThresh": None})
+    assert np.linalg.norm(x - xls) / np.linalg.norm(xls) < 1e-10
+    assert status == 1
+
+
+def test_cgne_first_step_is_steepest_descent():
+    """``mod(0, expResN) == 0``: iteration 0 recomputes the residual and sets
+    beta = 0 (cgne.m:230-246), so x1 = x0 + alpha r0 with alpha = |r0|^2 / r0'Nr0."""
+    rng = np.random.default_rng(2)
+    A = _rand_c(rng, (30, 10))
+    b = _rand_c(rng, 30)
+    x0 = _rand_c(rng, 10)
+    Nop = A.conj().T @ A
+    r0 = A.conj().T @ b - Nop @ x0
+    alpha = np.vdot(r0, r0) / np.vdot(r0, Nop @ r0)
+    x, res, *_ = cgne(Dense(A), b, x0, 1, {"xChangeThresh": None})
+    assert np.allclose(x, x0 + alpha * r0, atol=1e-12)
+    # history: [|r0|^2, |r1|^2] with r1 the explicit residual
+    r1 = A.conj().T @ b - Nop @ x
+    assert np.allclose(res, [np.vdot(r0, r0), np.vdot(r1, r1)])
+
+
+def test_cgne_textbook_sequence():
+    """Iteration 0 is a steepest-descent step that also restarts (beta = 0,
+    cgne.m:230-246), so from x1 on the iterates are textbook CGNR started at x1:
+    an n-dimensional SPD system converges in 1 + n steps, not n."""
+    rng = np.random.default_rng(3)
+    A = _rand_c(rng, (20, 6))
+    b = _rand_c(rng, 20)
+    xls = np.linalg.lstsq(A, b, rcond=None)[0]
+    x, res, *_ = cgne(Dense(A), b, np.zeros(6, complex), 7, {"resThresh": None, "xChangeThresh": None})
+    assert np.linalg.norm(x - xls) / np.linalg.norm(xls) < 1e-8
+    # with only n = 6 iterations the restart leaves it short of the solution
+    x6, *_ = cgne(Dense(A), b, np.zeros(6, complex), 6, {"resThresh": None, "xChangeThresh": None})
+    assert np.linalg.norm(x6 - xls) / np.linalg.norm(xls) > 1e-6
+
+
+def test_cgne_default_x0_is_b_and_counts():
+    rng = np.random.default_rng(4)
+    A = _rand_c(rng, (25, 8))
+    b = _rand_c(rng, 25)
+    calls = {"n": 0, "t": 0}
+
+    def h(v, tr):
+        if tr == "notransp":
+            calls["n"] += 1
+            return A @ v
+        calls["t"] += 1
+        return A.conj().T @ v
+    x, res, _, xn, *_ , status = cgne(h, b, None, 5, {"resThresh": None, "xChangeThresh": None})
+    # x0 = A'b ; 5 iterations cost (n+2) forward and (n+3) adjoint applications
+    assert calls["n"] == 5 + 2 and calls["t"] == 5 + 3
+    assert np.isclose(xn[0], np.vdot(A.conj().T @ b, A.conj().T @ b))
+    assert status == 0 and len(res) == 6
+
+
+def test_cgne_status_codes():
+    rng = np.random.default_rng(5)
+    A = _rand_c(rng, (25, 8))
+    b = _rand_c(rng, 25)
+    z = np.zeros(8, complex)
+    # status 10: x change
+    assert cgne(Dense(A), b, z, 50, {"resThresh": None, "xChangeThresh": 1e3})[-1] == 10
+    # status 2: residual change
+    assert cgne(Dense(A), b, z, 50, {"resThresh": None, "xChangeThresh": None,
+                                     "resChangeThresh": 0.99})[-1] == 2
+    # status 4: residual increase (threshold 0 => every step counts)
+    out = cgne(Dense(A), b, z, 50, {"resThresh": None, "xChangeThresh": None,
+                                    "stopOnResInc": 2, "resIncThresh": 0.0})
+    assert out[-1] == 4 and len(out[1]) == 3
+    # status 3: discrepancy principle, then nseExtraIt more iterations
+    out = cgne(Dense(A), b, z, 50, {"resThresh": None, "xChangeThresh": None,
+                                    "noiseVar": 1e6, "nseExtraIt": 2})
+    # iteration 0 passes -> NitMax = 0 + 2, status 3; the loop then runs nit = 1
+    # and ends on NitMax with status 0 and a history of nit + 1 = 3 entries
+    assert out[-1] == 0 and len(out[1]) == 3
+
+
+def test_cgne_tikhonov_and_weights():
+    rng = np.random.default_rng(6)
+    A = _rand_c(rng, (30, 10))
+    b = _rand_c(rng, 30)
+    w = rng.random(30) + 0.5
+    gam = 0.3
+    x, *_ = cgne(Dense(A), b, np.zeros(10, complex), 100,
+                 {"resThresh": 1e-26, "xChangeThresh": None}, gam, Dense(np.eye(10)), w)
+    lhs = A.conj().T @ (w[:, None] * A) + gam * np.eye(10)
+    xs = np.linalg.solve(lhs, A.conj().T @ (w * b))
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+
+
+# -- powermethod (powermethod.m:77-95) -------------------------------------------
+def test_powermethod():
+    rng = np.random.default_rng(8)
+    A = _rand_c(rng, (8, 5))
+    e = np.max(np.linalg.eigvalsh(A.conj().T @ A))
+    ep = powermethod(Dense(A), rng.standard_normal(5))
+    assert abs(e - ep) / abs(e) < 1e-3
+    B = _rand_c(rng, (5, 8))
+    B = B.conj().T @ B
+    e2 = np.max(np.linalg.eigvalsh(B))
+    ep2 = powermethod(Dense(B), rng.standard_normal(8), None or 50, 0)
+    assert abs(e2 - ep2) / abs(e2) < 1e-3
+
+
+# -- bfista / softthresh -----------------------------------------------------------
+def test_softthresh_complex():
+    w = np.array([3 + 4j, 0.1, -2.0, 0])
+    out = softthresh(w, 1.0)
+    assert np.allclose(out, [(3 + 4j) * 4 / 5, 0, -1.0, 0])
+
+
+def test_bfista_identity_closed_form():
+    """argmin ||x-b||^2 + lam ||x||_1 = soft(b, lam/2)."""
+    rng = np.random.default_rng(9)
+    b = _rand_c(rng, 64)
+    lam = 0.8
+    eye = Dense(np.eye(64))
+    x, *_ = bfista(eye, b, eye, lam, np.zeros(64, complex), 500, {"maxEig": 1.0, "resThresh": 1e-14})
+    assert np.allclose(x, softthresh(b, lam / 2), atol=1e-8)
