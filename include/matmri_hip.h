@@ -1,0 +1,161 @@
+/*
+ * matmri_hip.h -- C ABI of the MI355X (gfx950) implementation of MatMRI's
+ * iterative-SENSE operator chain.
+ *
+ * Every entry point is plain C: opaque handles, plain pointers, 64-bit sizes,
+ * int return codes.  Complex data is interleaved (re, im) single precision
+ * ("c64") and column-major, exactly the memory layout of a MATLAB complex
+ * single array under the interleaved-complex MEX API.  Pointers passed with
+ * `on_device = 1` are device addresses (HBM); with `on_device = 0` they are
+ * host addresses and the call copies in and out and synchronises.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * MatMRI repository root):
+ *
+ *   mm_nufft_plan_create    nufftOp(imN,kloc,dcf,useGPU,os,kwidth)
+ *                           operators/@nufftOp/nufftOp.m:89-136 (+ doPrep :138-274)
+ *   mm_rcvr_plan_create     rcvrOp(maps,false)      operators/@rcvrOp/rcvrOp.m:47-94
+ *   mm_hio_plan_create      sampHighOrder(b0,sampTimes,phs_spha,phs_conc,phs_grid,b0mask,...)
+ *                           operators/@sampHighOrder/sampHighOrder.m:129-291, :496-588
+ *   mm_hio_set_phi_div      sampHighOrder.setPhiDiv  sampHighOrder.m:590-622
+ *   mm_sense_create         mrSampFunc(x,transp,samplingOp,R,opt)  operators/mrSampFunc.m:82-159
+ *   mm_op_apply             A*x / A'*y  (mtimes/ctranspose overloads:
+ *                           nufftOp.m:276-408, rcvrOp.m:96-159,
+ *                           sampHighOrder.m:293-391, mrSampFunc.m:82-159)
+ *   mm_cgne_solve           cgne(A,b,x0,NitMax,opt,gam,R,W)  iterativeSolvers/cgne.m:1-341
+ *
+ * `bfista`/`powermethod` (iterativeSolvers/bfista.m, powermethod.m) stay on
+ * the host side and call mm_op_apply; their regulariser R is host code.
+ */
+#ifndef MATMRI_HIP_H
+#define MATMRI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------ */
+#define MM_OK              0
+#define MM_ERR_ARG         1   /* bad argument / shape mismatch (MATLAB error('...')) */
+#define MM_ERR_OOM         2   /* device allocation failed (nufftOp.m:336-345 hint)    */
+#define MM_ERR_DEVICE      3   /* no usable gfx950 device                               */
+#define MM_ERR_HIP         4   /* other HIP runtime error                               */
+#define MM_ERR_UNSUPPORTED 5   /* valid in the reference, not implemented here          */
+
+typedef struct mm_ctx mm_ctx;   /* one device, one HIP stream; not thread-safe */
+typedef struct mm_op  mm_op;    /* a linear operator (NUFFT, coils, high-order, SENSE) */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* mm_last_error(void);
+/* Library version string. */
+const char* mm_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int mm_device_count(void);
+
+/* Context: `stream` is a hipStream_t to run on (NULL = the null stream). */
+int mm_ctx_create(int device, void* stream, mm_ctx** out);
+int mm_ctx_destroy(mm_ctx* ctx);
+int mm_ctx_sync(mm_ctx* ctx);
+
+/* ---- operators ---------------------------------------------------------- */
+/* Kaiser-Bessel gridding NUFFT over `nslice` independent 1-D or 2-D slices
+ * (nslice = 1 is nufftOp).  Slice s has image size imN[2s], imN[2s+1]
+ * (imN[2s+1] = 1 for 1-D), M[s] samples, kloc[s] = column-major [M x ndim]
+ * cycles/pixel in [-0.5, 0.5) (wrapped), dcf[s] = [M] or NULL.  `os` and
+ * `kwidth` follow nufftOp (kwidth even, <= 8).  Data of all slices is
+ * concatenated: images [V_s x B] then samples [M_s x B] per slice, where B is
+ * the number of transforms per slice passed to mm_op_apply as `nrep`. */
+int mm_nufft_plan_create(mm_ctx* ctx, int ndim, int nslice, const int64_t* imN,
+                         const int64_t* M, const double* const* kloc,
+                         const double* const* dcf, double os, int kwidth,
+                         mm_op** out);
+
+/* Coil sensitivities: maps = column-major [V x C] c64 (image dims flattened). */
+int mm_rcvr_plan_create(mm_ctx* ctx, int64_t V, int64_t C, const float* maps,
+                        mm_op** out);
+
+/* Coil sensitivities of `nslice` slices for a multi-slice SENSE operator:
+ * maps = concatenation over slices of column-major [V_s x C]. */
+int mm_rcvr_plan_create_multi(mm_ctx* ctx, int nslice, const int64_t* V, int64_t C,
+                              const float* maps, mm_op** out);
+
+/* Expanded-encoding operator.  x,y,z = voxel positions [V] (m); b0 = [V]
+ * rad/s or NULL; t = [K] s; h = column-major [Lh x K] (phs_spha);
+ * g = column-major [Lc x K] (phs_conc) or NULL with Lc = 0; mask = [V] or NULL
+ * (b0mask).  Phase terms, their masking and their order follow
+ * sampHighOrder.m:496-588; the B0 term is present when K > 1. */
+int mm_hio_plan_create(mm_ctx* ctx, int64_t V, int64_t K, int Lh, int Lc,
+                       const double* x, const double* y, const double* z,
+                       const double* b0, const double* t, const double* h,
+                       const double* g, const double* mask, mm_op** out);
+/* setPhiDiv: replaces the forward by (1i*exp(1i*Phi).*dPhi).' * x / sqrt(V)
+ * with dPhi from hd [Lh x K] and gd [Lc x K] (no B0 term); NULL,NULL clears. */
+int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd);
+
+/* SENSE encoding E = S o R with optional real data weights W ([M x C], used
+ * only by mm_cgne_solve and by mm_op_apply_normal). */
+int mm_sense_create(mm_ctx* ctx, mm_op* S, mm_op* R, const float* W, mm_op** out);
+
+/* Element counts (complex) per repetition of the operator input/output. */
+int mm_op_shape(const mm_op* op, int64_t* n_image, int64_t* n_data);
+/* Number of independent slices an operator spans (0 for NULL). */
+int mm_op_nslice(const mm_op* op);
+
+/* y = A x (adjoint = 0) or x = A' y (adjoint = 1) for `nrep` repetitions
+ * (column-major trailing dimension).  Out of place; c64 interleaved. */
+int mm_op_apply(mm_op* op, int adjoint, const void* in, void* out, int64_t nrep,
+                int on_device);
+/* x_out = A'(W .* (A x)) (the normal operator cgne iterates). */
+int mm_op_apply_normal(mm_op* op, const void* in, void* out, int64_t nrep,
+                       int on_device);
+
+/* ---- solver -------------------------------------------------------------- */
+/* Options of cgne.m:21-89.  A threshold < 0 means "empty" (disabled). */
+typedef struct mm_cg_opts {
+    double resThresh;        /* absolute |r|^2 (default 1e-10)          */
+    double resThreshRel;     /* |r|^2 < resThreshRel*|b|^2, b = E'(W.*y);
+                                not in cgne.m: relative-residual helper
+                                (default off; combined with resThresh by max) */
+    double resChangeThresh;  /* relative change of |r|^2 (default off)  */
+    double xChangeThresh;    /* |x - xold|^2 (default 1e-10)            */
+    int    expResN;          /* explicit residual period (default 100)  */
+    int    stopOnResInc;     /* default 0                               */
+    double resIncThresh;     /* default 1                               */
+    int    stopOnXdifInc;    /* default 0                               */
+    double stopThresh;       /* noise discrepancy |A' n|^2 (< 0: off)   */
+    int    nseExtraIt;       /* default 4                               */
+} mm_cg_opts;
+
+void mm_cg_opts_default(mm_cg_opts* o);
+
+/* cgne on E'(W.*E) x = E'(W.*y) for `nslice` slices at once (nslice must be
+ * the slice count of E).  y: data [sum_s M_s*C]; x0: image or NULL (NULL is
+ * cgne's default x0 = b); x: image out.  Per slice s the histories
+ * resSqAll/xnormAll/xdiffAll hold (nit_s+1) values at offset s*(NitMax+1);
+ * nit/status are per slice.  All buffers are host or device per on_device
+ * (histories, nit and status are always host). */
+int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax,
+                  const mm_cg_opts* opt, double* resSqAll, double* xnormAll,
+                  double* xdiffAll, int* nit, int* status, int on_device);
+
+/* Measurement hook (no counterpart in the reference): runs the operator's
+ * dominant kernel `iters` times back to back on the context stream between
+ * two HIP events (device pointers only) and returns the mean milliseconds per
+ * launch and the kernel's name. */
+int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nrep, int iters,
+                  double* ms_per_launch, char* kernel_name, int name_len);
+/* The hipStream_t the context runs on. */
+void* mm_ctx_stream(mm_ctx* ctx);
+
+/* Enable (1, default) or disable (0) hipGraph capture of the regular cgne
+ * iteration for operator E. */
+int mm_cg_set_graph(mm_op* E, int use_graph);
+
+int mm_op_destroy(mm_op* op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MATMRI_HIP_H */

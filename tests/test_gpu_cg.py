@@ -1,0 +1,104 @@
+"""GPU cgne (device loop, FP64 scalars) against the FP64 oracle cgne."""
+import numpy as np
+import pytest
+
+from conftest import load_pkg
+from oracle.mrsamp import mrSampFunc as o_mrSampFunc
+from oracle.nufft import NufftPlan
+from oracle.rcvr import RcvrPlan
+from oracle.solvers import cgne as o_cgne
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mm():
+    m = load_pkg()
+    if m.device_count() < 1:
+        pytest.fail("no GPU visible to libmatmri_hip")
+    return m
+
+
+def _problem(mm, N=64, M=6000, C=4, seed=5):
+    rng = np.random.default_rng(seed)
+    kloc = mm.workloads.spiral(M, N // 2)
+    maps = mm.workloads.gaussian_coils(N, N, C, rng)
+    ph = mm.workloads.ellipse_phantom(N, N, rng)
+    So, Ro = NufftPlan([N, N], kloc, None, 1.5, 4), RcvrPlan(maps)
+    y = mm.workloads.add_noise(o_mrSampFunc(ph, "notransp", So, Ro), rng)
+    Sg, Rg = mm.nufftOp([N, N], kloc, None, 1, 1.5, 4), mm.rcvrOp(maps)
+    return So, Ro, Sg, Rg, y, ph
+
+
+def test_cgne_matches_oracle(mm):
+    """Residual history and iterate of the device loop follow the FP64 loop
+    while the relative residual is above 1e-4 (FP32 vectors resolve it)."""
+    print("[case] cgne N=64 C=4 zero-init", flush=True)
+    So, Ro, Sg, Rg, y, ph = _problem(mm)
+    A = lambda x, tr: o_mrSampFunc(x, tr, So, Ro)
+    opt = {"resThresh": None, "xChangeThresh": None}
+    b = A(y, "transp")
+    bb = np.real(np.vdot(b, b))
+    x0 = np.zeros((64, 64), complex)
+    nit = 8
+    xo, ro, *_ = o_cgne(A, y, x0, nit, opt)
+    xg, rg, _, xng, xdg, _, st = mm.cgne(mm.mrSampHandle(Sg, Rg), y, x0, nit, opt)
+    ro = np.real(ro)
+    ok = ro > 1e-8 * bb
+    d = np.abs(rg - ro) / ro
+    print("  rel resid", np.sqrt(ro / bb), flush=True)
+    print("  resSq rel diff", d, flush=True)
+    assert len(rg) == nit + 1 and st == 0
+    assert np.all(d[ok] < 1e-3)
+    nrmse = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    print("  x nrmse", nrmse, flush=True)
+    assert nrmse < 1e-4
+
+
+def test_cgne_relative_threshold(mm):
+    """Stop at |r|^2 < 1e-8 |b|^2 (relative residual 1e-4) with cgne's default
+    x0 = b; same stopping iteration and NRMSE <= 1e-4 against the FP64 solve
+    stopped by the same rule."""
+    print("[case] cgne default x0, relative residual 1e-4", flush=True)
+    So, Ro, Sg, Rg, y, ph = _problem(mm, seed=6)
+    A = lambda x, tr: o_mrSampFunc(x, tr, So, Ro)
+    b = A(y, "transp")
+    bb = np.real(np.vdot(b, b))
+    xo, ro, *_, so = o_cgne(A, y, None, 80, {"resThresh": 1e-8 * bb, "xChangeThresh": None})
+    xg, rg, _, _, _, _, st = mm.cgne(mm.mrSampHandle(Sg, Rg), y, None, 80,
+                                     {"resThresh": None, "resThreshRel": 1e-8, "xChangeThresh": None})
+    nrmse = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    print(f"  oracle it {len(ro) - 1} status {so}; device it {len(rg) - 1} status {st}; nrmse {nrmse:.2e}",
+          flush=True)
+    assert so == 1 and st == 1
+    assert abs(len(rg) - len(ro)) <= 1
+    assert nrmse < 1e-4
+
+
+def test_cgne_multislice_equals_single(mm):
+    print("[case] cgne batched 3 slices vs single", flush=True)
+    rng = np.random.default_rng(9)
+    C = 4
+    dims = [(48, 3000), (64, 5000), (40, 2000)]
+    sl, mp, ys, singles = [], [], [], []
+    for N, M in dims:
+        kl = mm.workloads.spiral(M, N // 2)
+        maps = mm.workloads.gaussian_coils(N, N, C, rng)
+        ph = mm.workloads.ellipse_phantom(N, N, rng)
+        y = mm.workloads.add_noise(o_mrSampFunc(ph, "notransp", NufftPlan([N, N], kl, None, 1.5, 4),
+                                                RcvrPlan(maps)), rng)
+        sl.append(([N, N], kl, None))
+        mp.append(maps)
+        ys.append(y)
+        singles.append(mm.senseOp(mm.nufftOp([N, N], kl, None, 1, 1.5, 4), mm.rcvrOp(maps)))
+    E = mm.senseOp(mm.nufftOp.batch(sl, 1.5, 4), mm.rcvrOp.batch(mp))
+    ycat = np.concatenate([y.reshape(-1, order="F") for y in ys])
+    opt = {"resThresh": None, "resThreshRel": 1e-8, "xChangeThresh": None}
+    xb, rb, _, _, _, _, sb = mm.cgne(E, ycat, None, 60, opt)
+    off = 0
+    for (N, M), Es, y, r, s in zip(dims, singles, ys, rb, sb):
+        xs, rs, *_, ss = mm.cgne(Es, y, None, 60, opt)
+        e = np.linalg.norm(xb[off:off + N * N] - xs.reshape(-1, order="F")) / np.linalg.norm(xs)
+        print(f"  N={N} it {len(r)-1}/{len(rs)-1} status {s}/{ss} x diff {e:.2e}", flush=True)
+        assert s == 1 and ss == 1 and abs(len(r) - len(rs)) <= 1 and e < 1e-4
+        off += N * N

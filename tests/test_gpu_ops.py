@@ -1,0 +1,196 @@
+"""GPU parity of the device operators against the FP64 oracle.
+
+Tolerance: rel-L2 <= 1e-5 per forward/adjoint application (the per-application
+bound the port is held to).  Every case prints its name before it runs.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_pkg
+from oracle.highorder import HighOrderPlan
+from oracle.mrsamp import mrSampFunc as o_mrSampFunc
+from oracle.nufft import NufftPlan
+from oracle.rcvr import RcvrPlan
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a).reshape(-1, order="F")
+    b = np.asarray(b).reshape(-1, order="F")
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def crand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.fixture(scope="module")
+def mm():
+    m = load_pkg()
+    if m.device_count() < 1:
+        pytest.fail("no GPU visible to libmatmri_hip")
+    return m
+
+
+NUFFT_CASES = [
+    # imN, M, kwidth, os, dcf, nrep
+    ((32, 36), 500, 6, 1.5, False, 1),
+    ((64, 64), 3000, 4, 1.5, True, 3),
+    ((48, 40), 2000, 4, 1.375, False, 20),
+    ((64,), 300, 6, 1.5, True, 2),
+    ((96, 110), 5000, 8, 2.0, False, 1),
+]
+
+
+@pytest.mark.parametrize("imN,M,W,os_,use_dcf,nrep", NUFFT_CASES)
+def test_nufft_vs_oracle(mm, imN, M, W, os_, use_dcf, nrep):
+    print(f"[case] nufft imN={imN} M={M} W={W} os={os_} dcf={use_dcf} nrep={nrep}", flush=True)
+    rng = np.random.default_rng(7)
+    nd = len(imN)
+    kloc = rng.random((M, nd)) - 0.5
+    dcf = rng.random(M) + 0.5 if use_dcf else None
+    So = NufftPlan(list(imN), kloc, dcf, os_, W)
+    Sg = mm.nufftOp(list(imN), kloc, dcf, 1, os_, W)
+    x = crand(rng, imN + (nrep,))
+    y = crand(rng, (M, nrep))
+    e_f = rel(Sg * x, So.forward(x))
+    e_a = rel(Sg.H * y, So.adjoint(y))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+
+
+def test_nufft_spiral_128_full_size(mm):
+    """The 128^2 / 16384-sample / W=4 / os=1.5 spiral operator at full size."""
+    print("[case] nufft spiral 128x128 M=16384 W=4 8 reps", flush=True)
+    W = mm.workloads.nufft_problem(seed=1)
+    So = NufftPlan([128, 128], W["kloc"], None, 1.5, 4)
+    Sg = mm.nufftOp([128, 128], W["kloc"], None, 1, 1.5, 4)
+    rng = np.random.default_rng(1)
+    x = crand(rng, (128, 128, 8))
+    y = crand(rng, (16384, 8))
+    e_f, e_a = rel(Sg * x, So.forward(x)), rel(Sg.H * y, So.adjoint(y))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+
+
+def test_nufft_on_grid_known_answer(mm):
+    """@nufftOp/tests.m:90-101: on-grid kloc makes the forward a unitary
+    centred inverse DFT, fftnc(S*im) == im (tolerance 1e-4, L2/numel)."""
+    from oracle.fftnc import fftnc
+    print("[case] nufft on-grid KAT", flush=True)
+    imN = [32, 36]
+    axes = [(np.arange(n) - n / 2) / n for n in imN]
+    g = np.meshgrid(*axes, indexing="ij")
+    kloc = np.stack([a.reshape(-1, order="F") for a in g], axis=1)
+    Sg = mm.nufftOp(imN, kloc, None, 1, 1.5, 6)
+    rng = np.random.default_rng(3)
+    im = rng.random(imN)
+    k = (Sg * im).reshape(imN, order="F")
+    err = np.sqrt(np.sum(np.abs(fftnc(k, 2) - im) ** 2)) / im.size
+    assert err < 1e-4
+
+
+def test_sense_fused_vs_oracle(mm):
+    print("[case] sense nufft N=64 C=8", flush=True)
+    rng = np.random.default_rng(11)
+    N, M, C = 64, 4000, 8
+    kloc = rng.random((M, 2)) - 0.5
+    maps = crand(rng, (N, N, C))
+    So, Ro = NufftPlan([N, N], kloc, None, 1.5, 4), RcvrPlan(maps)
+    Sg, Rg = mm.nufftOp([N, N], kloc, None, 1, 1.5, 4), mm.rcvrOp(maps)
+    E = mm.senseOp(Sg, Rg)
+    x = crand(rng, (N, N))
+    y = crand(rng, (M, C))
+    e_f = rel(E * x, o_mrSampFunc(x, "notransp", So, Ro))
+    e_a = rel(E.H * y, o_mrSampFunc(y, "transp", So, Ro))
+    e_h = rel(mm.mrSampFunc(x, "notransp", Sg, Rg), o_mrSampFunc(x, "notransp", So, Ro))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e} handle {e_h:.2e}", flush=True)
+    assert max(e_f, e_a, e_h) < TOL
+    # rcvrOp alone (rcvrOp.m:145-157)
+    assert rel(Rg * x, Ro.forward(x)) < TOL
+    assert rel(Rg.H * (Ro.forward(x)), Ro.adjoint(Ro.forward(x))) < TOL
+
+
+def test_multislice_sense_vs_oracle(mm):
+    print("[case] multi-slice sense 3 slices", flush=True)
+    rng = np.random.default_rng(12)
+    C = 4
+    dims = [(48, 3000), (64, 5000), (40, 1500)]
+    sl, maps, orc = [], [], []
+    for N, M in dims:
+        kl = rng.random((M, 2)) - 0.5
+        mp = crand(rng, (N, N, C))
+        sl.append(([N, N], kl, None))
+        maps.append(mp)
+        orc.append((NufftPlan([N, N], kl, None, 1.5, 4), RcvrPlan(mp)))
+    S = mm.nufftOp.batch(sl, 1.5, 4)
+    R = mm.rcvrOp.batch(maps)
+    E = mm.senseOp(S, R)
+    xs = [crand(rng, (N, N)) for N, _ in dims]
+    ys = [crand(rng, (M, C)) for _, M in dims]
+    xcat = np.concatenate([x.reshape(-1, order="F") for x in xs])
+    ycat = np.concatenate([y.reshape(-1, order="F") for y in ys])
+    yg, xg = E * xcat, E.H * ycat
+    oy, ox = 0, 0
+    for (N, M), x, y, (So, Ro) in zip(dims, xs, ys, orc):
+        e_f = rel(yg[oy:oy + M * C], o_mrSampFunc(x, "notransp", So, Ro))
+        e_a = rel(xg[ox:ox + N * N], o_mrSampFunc(y, "transp", So, Ro))
+        print(f"  N={N} fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+        assert e_f < TOL and e_a < TOL
+        oy += M * C
+        ox += N * N
+
+
+def _hio_plans(mm, rng, N=12, K=300, conc=True, mask=False, Lh=16):
+    gx = (np.arange(N) - N / 2) * 0.01
+    X, Y = np.meshgrid(gx, gx, indexing="ij")
+    grid = {"x": X, "y": Y, "z": np.full_like(X, 0.02)}
+    b0 = rng.standard_normal((N, N)) * 300
+    t = np.arange(K) * 1e-5
+    hs = rng.standard_normal((Lh, K)) * np.array([5.0] + [300.0] * 3 + [3e3] * 5 + [3e4] * 7 + [1e5] * 20)[:Lh, None]
+    hc = rng.standard_normal((4, K)) * 1e3 if conc else None
+    mk = (rng.random((N, N)) > 0.3).astype(float) if mask else None
+    So = HighOrderPlan(b0, t, hs, hc, grid, mk)
+    Sg = mm.sampHighOrder(b0, t, hs, hc, grid, mk)
+    return So, Sg
+
+
+@pytest.mark.parametrize("conc,mask,Lh", [(False, False, 16), (True, True, 16), (True, False, 36)])
+def test_highorder_vs_oracle(mm, conc, mask, Lh):
+    print(f"[case] highorder conc={conc} mask={mask} Lh={Lh}", flush=True)
+    rng = np.random.default_rng(21)
+    So, Sg = _hio_plans(mm, rng, conc=conc, mask=mask, Lh=Lh)
+    x = crand(rng, (12, 12, 3))
+    y = crand(rng, (300, 3))
+    e_f, e_a = rel(Sg * x, So.forward(x)), rel(Sg.H * y, So.adjoint(y))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+
+
+def test_highorder_phidiv_vs_oracle(mm):
+    print("[case] highorder phiDiv", flush=True)
+    rng = np.random.default_rng(22)
+    So, Sg = _hio_plans(mm, rng)
+    dh = np.diff(np.concatenate([np.zeros((16, 1)), So.phs_spha], axis=1), axis=1)
+    dc = np.diff(np.concatenate([np.zeros((4, 1)), So.phs_conc], axis=1), axis=1)
+    So.set_phi_div(dh, dc)
+    Sg.setPhiDiv(dh, dc)
+    x = crand(rng, (12, 12))
+    assert rel(Sg * x, So.forward(x)) < TOL
+
+
+def test_highorder_sense_vs_oracle(mm):
+    print("[case] highorder sense C=6", flush=True)
+    rng = np.random.default_rng(23)
+    So, Sg = _hio_plans(mm, rng, N=16, K=400)
+    maps = crand(rng, (16, 16, 6))
+    Ro, Rg = RcvrPlan(maps), mm.rcvrOp(maps)
+    E = mm.senseOp(Sg, Rg)
+    x = crand(rng, (16, 16))
+    y = crand(rng, (400, 6))
+    e_f = rel(E * x, o_mrSampFunc(x, "notransp", So, Ro))
+    e_a = rel(E.H * y, o_mrSampFunc(y, "transp", So, Ro))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL

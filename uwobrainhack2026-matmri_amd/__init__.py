@@ -1,0 +1,15 @@
+"""MI355X (gfx950) implementation of MatMRI's iterative-SENSE operator chain.
+
+The operators and solvers keep the reference's MATLAB interfaces
+(``nufftOp``, ``rcvrOp``, ``sampHighOrder``, ``mrSampFunc``, ``cgne``,
+``bfista``); the arithmetic runs in hand-written HIP kernels behind the C ABI
+of ``include/matmri_hip.h`` (``libmatmri_hip.so`` in this directory).
+"""
+from ._lib import MMError, OutOfMemory, device_count, lib, LIB_PATH, EXPORTS  # noqa: F401
+from .operators import (LinearOp, mrSampFunc, mrSampHandle, nufftOp, rcvrOp,  # noqa: F401
+                        sampHighOrder, senseOp)
+from . import workloads  # noqa: F401
+from .solvers import bfista, cgne, powermethod, set_cg_graph, softthresh  # noqa: F401
+
+__all__ = ["nufftOp", "rcvrOp", "sampHighOrder", "senseOp", "mrSampFunc", "mrSampHandle", "cgne",
+           "bfista", "powermethod", "softthresh", "set_cg_graph", "device_count", "lib"]

@@ -1,0 +1,292 @@
+// extern "C" boundary of libmatmri_hip (declared in include/matmri_hip.h).
+// Every entry point converts exceptions into MM_ERR_* codes and keeps the
+// message for mm_last_error().
+#include <cstring>
+
+#include "cg.hpp"
+#include "hio.hpp"
+#include "ops.hpp"
+
+using namespace mm;
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        g_err.clear();
+        f();
+        return MM_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return MM_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return MM_ERR_HIP;
+    }
+}
+
+void set_device(mm_ctx* ctx) { HIPCHECK(hipSetDevice(ctx->device)); }
+
+struct SolverHolder {
+    mm_op* E;
+    std::unique_ptr<CgSolver> s;
+};
+// one cached solver per operator (its buffers and captured graph are reused)
+std::map<mm_op*, std::unique_ptr<CgSolver>>& solvers() {
+    static std::map<mm_op*, std::unique_ptr<CgSolver>> m;
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mm_last_error(void) { return g_err.c_str(); }
+
+const char* mm_version(void) { return "matmri-hip 0.1 (gfx950)"; }
+
+int mm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int mm_ctx_create(int device, void* stream, mm_ctx** out) {
+    return guarded([&] {
+        if (!out) fail(MM_ERR_ARG, "out is NULL");
+        *out = nullptr;
+        const int n = mm_device_count();
+        if (n <= 0) fail(MM_ERR_DEVICE, "no HIP device visible");
+        if (device < 0 || device >= n) fail(MM_ERR_DEVICE, "device index out of range");
+        HIPCHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            fail(MM_ERR_DEVICE, std::string("libmatmri_hip is built for gfx950, device is ") + prop.gcnArchName);
+        auto* c = new mm_ctx();
+        c->device = device;
+        if (stream) {
+            c->stream = (hipStream_t)stream;
+            c->own_stream = false;
+        } else {
+            HIPCHECK(hipStreamCreate(&c->stream));
+            c->own_stream = true;
+        }
+        *out = c;
+    });
+}
+
+int mm_ctx_destroy(mm_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int mm_ctx_sync(mm_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) fail(MM_ERR_ARG, "ctx is NULL");
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mm_nufft_plan_create(mm_ctx* ctx, int ndim, int nslice, const int64_t* imN, const int64_t* M,
+                         const double* const* kloc, const double* const* dcf, double os, int kwidth,
+                         mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out || !imN || !M || !kloc) fail(MM_ERR_ARG, "nufftOp: NULL argument");
+        set_device(ctx);
+        *out = new NufftOp(ctx, ndim, nslice, imN, M, kloc, dcf, os, kwidth);
+    });
+}
+
+int mm_rcvr_plan_create(mm_ctx* ctx, int64_t V, int64_t C, const float* maps, mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(MM_ERR_ARG, "rcvrOp: NULL argument");
+        set_device(ctx);
+        *out = new RcvrOp(ctx, 1, &V, C, maps);
+    });
+}
+
+int mm_rcvr_plan_create_multi(mm_ctx* ctx, int nslice, const int64_t* V, int64_t C,
+                              const float* maps, mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out || !V) fail(MM_ERR_ARG, "rcvrOp: NULL argument");
+        set_device(ctx);
+        *out = new RcvrOp(ctx, nslice, V, C, maps);
+    });
+}
+
+int mm_hio_plan_create(mm_ctx* ctx, int64_t V, int64_t K, int Lh, int Lc, const double* x,
+                       const double* y, const double* z, const double* b0, const double* t,
+                       const double* h, const double* g, const double* mask, mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(MM_ERR_ARG, "sampHighOrder: NULL argument");
+        set_device(ctx);
+        *out = new HioOp(ctx, V, K, Lh, Lc, x, y, z, b0, t, h, g, mask);
+    });
+}
+
+int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd) {
+    return guarded([&] {
+        auto* h = dynamic_cast<HioOp*>(op);
+        if (!h) fail(MM_ERR_ARG, "setPhiDiv: not a high-order operator");
+        set_device(op->ctx);
+        h->set_phi_div(hd, gd);
+    });
+}
+
+int mm_sense_create(mm_ctx* ctx, mm_op* S, mm_op* R, const float* W, mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(MM_ERR_ARG, "mrSampFunc: NULL argument");
+        set_device(ctx);
+        *out = new SenseOp(ctx, S, R, W);
+    });
+}
+
+int mm_op_shape(const mm_op* op, int64_t* n_image, int64_t* n_data) {
+    return guarded([&] {
+        if (!op) fail(MM_ERR_ARG, "op is NULL");
+        if (n_image) *n_image = op->n_image();
+        if (n_data) *n_data = op->n_data();
+    });
+}
+
+int mm_op_nslice(const mm_op* op) { return op ? op->nslice() : 0; }
+
+static void apply_common(mm_op* op, int mode, const void* in, void* out, int64_t nrep, int on_device) {
+    if (!op || !in || !out) fail(MM_ERR_ARG, "mtimes: NULL argument");
+    if (nrep < 1) fail(MM_ERR_ARG, "mtimes: nrep must be >= 1");
+    if (mode == 2 && op->W && nrep != 1) fail(MM_ERR_ARG, "normal operator with weights needs nrep = 1");
+    mm_ctx* ctx = op->ctx;
+    set_device(ctx);
+    const int64_t ni = op->n_image() * nrep, nd = op->n_data() * nrep;
+    const int64_t nin = mode == 1 ? nd : ni;
+    const int64_t nout = mode == 0 ? nd : ni;
+    const float2* din = (const float2*)in;
+    float2* dout = (float2*)out;
+    if (!on_device) {
+        ctx->stage_in.upload((const float2*)in, (size_t)nin, ctx->stream);
+        ctx->stage_out.ensure((size_t)nout);
+        din = ctx->stage_in.p;
+        dout = ctx->stage_out.p;
+    }
+    if (mode == 2) {
+        ctx->stage_aux.ensure((size_t)nd);
+        op->apply_normal(din, dout, nrep, ctx->stage_aux.p);
+    } else {
+        op->apply(mode, din, dout, nrep);
+    }
+    if (!on_device) {
+        HIPCHECK(hipMemcpyAsync(out, dout, sizeof(float2) * nout, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+int mm_op_apply(mm_op* op, int adjoint, const void* in, void* out, int64_t nrep, int on_device) {
+    return guarded([&] { apply_common(op, adjoint ? 1 : 0, in, out, nrep, on_device); });
+}
+
+int mm_op_apply_normal(mm_op* op, const void* in, void* out, int64_t nrep, int on_device) {
+    return guarded([&] { apply_common(op, 2, in, out, nrep, on_device); });
+}
+
+void mm_cg_opts_default(mm_cg_opts* o) {
+    if (!o) return;
+    o->resThresh = 1e-10;
+    o->resThreshRel = -1;
+    o->resChangeThresh = -1;
+    o->xChangeThresh = 1e-10;
+    o->expResN = 100;
+    o->stopOnResInc = 0;
+    o->resIncThresh = 1;
+    o->stopOnXdifInc = 0;
+    o->stopThresh = -1;
+    o->nseExtraIt = 4;
+}
+
+int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax, const mm_cg_opts* opt,
+                  double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status,
+                  int on_device) {
+    return guarded([&] {
+        if (!E || !y || !x) fail(MM_ERR_ARG, "cgne: NULL argument");
+        mm_cg_opts o;
+        mm_cg_opts_default(&o);
+        if (opt) o = *opt;
+        mm_ctx* ctx = E->ctx;
+        set_device(ctx);
+        auto& m = solvers();
+        auto it = m.find(E);
+        if (it == m.end()) it = m.emplace(E, std::make_unique<CgSolver>(E)).first;
+        CgSolver& s = *it->second;
+        const int64_t ni = E->n_image(), nd = E->n_data();
+        const float2 *dy = (const float2*)y, *dx0 = (const float2*)x0;
+        float2* dx = (float2*)x;
+        DevBuf<float2> hy, hx0, hx;
+        if (!on_device) {
+            hy.upload((const float2*)y, (size_t)nd, ctx->stream);
+            dy = hy.p;
+            if (x0) {
+                hx0.upload((const float2*)x0, (size_t)ni, ctx->stream);
+                dx0 = hx0.p;
+            }
+            hx.alloc((size_t)ni);
+            dx = hx.p;
+        }
+        s.solve(dy, dx0, dx, NitMax, o, resSqAll, xnormAll, xdiffAll, nit, status);
+        if (!on_device) {
+            HIPCHECK(hipMemcpyAsync(x, dx, sizeof(float2) * ni, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHECK(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+int mm_cg_set_graph(mm_op* E, int use_graph) {
+    return guarded([&] {
+        if (!E) fail(MM_ERR_ARG, "op is NULL");
+        auto& m = solvers();
+        auto it = m.find(E);
+        if (it == m.end()) it = m.emplace(E, std::make_unique<CgSolver>(E)).first;
+        it->second->use_graph_ = use_graph != 0;
+    });
+}
+
+int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nrep, int iters,
+                  double* ms_per_launch, char* kernel_name, int name_len) {
+    return guarded([&] {
+        if (!op || !in || !out || !ms_per_launch || iters < 1) fail(MM_ERR_ARG, "profile: bad argument");
+        set_device(op->ctx);
+        std::string name;
+        if (!op->profile_dominant(adjoint, (const float2*)in, (float2*)out, nrep, iters, ms_per_launch, &name))
+            fail(MM_ERR_UNSUPPORTED, "profile: operator has no dominant-kernel timer");
+        if (kernel_name && name_len > 0) {
+            std::strncpy(kernel_name, name.c_str(), (size_t)name_len - 1);
+            kernel_name[name_len - 1] = 0;
+        }
+    });
+}
+
+void* mm_ctx_stream(mm_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mm_op_destroy(mm_op* op) {
+    return guarded([&] {
+        if (!op) return;
+        (void)hipSetDevice(op->ctx->device);
+        (void)hipStreamSynchronize(op->ctx->stream);
+        solvers().erase(op);
+        delete op;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,499 @@
+// cgne on the device: CG on the normal equations E'(W.*E) x = E'(W.*y) for
+// one or many independent slices at once, following iterativeSolvers/cgne.m.
+//
+// Vectors are complex64 in HBM; every inner product is accumulated in FP64
+// (per-block partial sums reduced in a fixed order, so results do not depend
+// on scheduling) and the scalars alpha, beta, |r|^2 and the stopping logic
+// live in FP64 device memory.  The host never reads a scalar inside the
+// loop: a slice that has stopped sets its `done` flag and every later kernel
+// skips it; the host only polls the flags every few iterations to stop
+// launching.  The regular iteration (everything except the explicit-residual
+// restarts, cgne.m:230-246) is captured once into a hipGraph and replayed.
+#include <algorithm>
+#include <cstring>
+
+#include "cg.hpp"
+
+namespace mm {
+
+struct CgScal {
+    double rho_re, rho_im;       // res_sq_old
+    double al_re, al_im;         // alpha
+    double beta;                 // betaFact * res_sq_new / res_sq_old (real; cgne.m:328)
+    double stopThresh;           // < 0: off
+    double resThresh;            // effective absolute threshold (< 0: off)
+    int nit, nitmax, done, status;
+    int resIncs, xdifIncs, lastXdifInc, pad;
+};
+
+struct CgSeg {
+    int64_t off, len;   // vector segment of the slice
+    int blk0, nblk;     // partial-sum blocks of the slice
+};
+
+constexpr int kCgBlock = 256;
+constexpr int kCgMaxBlocksPerSlice = 64;
+
+__device__ __forceinline__ double2 warp_sum2(double2 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_down(v.x, o, 64);
+        v.y += __shfl_down(v.y, o, 64);
+    }
+    return v;
+}
+
+// Block reduction of NV double2 values; thread 0 gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(double2 (&v)[NV]) {
+    __shared__ double2 red[NV][kCgBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        v[k] = warp_sum2(v[k]);
+        if (lane == 0) red[k][wid] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double2 s = red[k][0];
+            for (int w = 1; w < kCgBlock / 64; ++w) {
+                s.x += red[k][w].x;
+                s.y += red[k][w].y;
+            }
+            v[k] = s;
+        }
+    }
+}
+
+// conj(a) * b accumulated in FP64
+__device__ __forceinline__ void acc_cdot(double2& s, float2 a, float2 b) {
+    s.x += (double)a.x * b.x + (double)a.y * b.y;
+    s.y += (double)a.x * b.y - (double)a.y * b.x;
+}
+
+// block -> (slice, index within slice) via the segment table
+__device__ __forceinline__ int find_slice(const CgSeg* seg, const int* blk2slice) {
+    return blk2slice[blockIdx.x];
+}
+
+// part[blk][0] = sum conj(p) q, part[blk][1] = sum conj(p) p
+__global__ __launch_bounds__(kCgBlock) void cg_dot_pq(const CgSeg* __restrict__ seg,
+                                                      const int* __restrict__ b2s,
+                                                      const CgScal* __restrict__ sc,
+                                                      const float2* __restrict__ p,
+                                                      const float2* __restrict__ q,
+                                                      double2* __restrict__ part) {
+    const int s = find_slice(seg, b2s);
+    if (sc[s].done) return;
+    const CgSeg g = seg[s];
+    const int bl = blockIdx.x - g.blk0;
+    double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
+        const float2 a = p[g.off + i], b = q[g.off + i];
+        acc_cdot(v[0], a, b);
+        acc_cdot(v[1], a, a);
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// alpha = rho / (p'Ap)  (cgne.m:228); keeps |p|^2 for |x - xold|^2
+__global__ void cg_alpha(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
+                         const double2* __restrict__ part, double* __restrict__ pp_out, int nslice) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslice || sc[s].done) return;
+    const CgSeg g = seg[s];
+    double2 pq = make_double2(0, 0);
+    double pp = 0;
+    for (int b = 0; b < g.nblk; ++b) {
+        pq.x += part[2 * (g.blk0 + b)].x;
+        pq.y += part[2 * (g.blk0 + b)].y;
+        pp += part[2 * (g.blk0 + b) + 1].x;
+    }
+    // complex division rho / pq
+    const double den = pq.x * pq.x + pq.y * pq.y;
+    const double rr = sc[s].rho_re, ri = sc[s].rho_im;
+    sc[s].al_re = (rr * pq.x + ri * pq.y) / den;
+    sc[s].al_im = (ri * pq.x - rr * pq.y) / den;
+    pp_out[s] = pp;
+}
+
+// x += alpha p; if `r_update`: r -= alpha q.  Partials: [0] = r'r, [1] = x'x.
+__global__ __launch_bounds__(kCgBlock) void cg_update(const CgSeg* __restrict__ seg,
+                                                      const int* __restrict__ b2s,
+                                                      const CgScal* __restrict__ sc,
+                                                      float2* __restrict__ x,
+                                                      const float2* __restrict__ p,
+                                                      float2* __restrict__ r,
+                                                      const float2* __restrict__ q, int r_update,
+                                                      double2* __restrict__ part) {
+    const int s = find_slice(seg, b2s);
+    if (sc[s].done) return;
+    const CgSeg g = seg[s];
+    const int bl = blockIdx.x - g.blk0;
+    const double are = sc[s].al_re, aim = sc[s].al_im;
+    double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
+        const int64_t k = g.off + i;
+        const float2 pv = p[k];
+        float2 xv = x[k];
+        xv.x = (float)(xv.x + (are * pv.x - aim * pv.y));
+        xv.y = (float)(xv.y + (are * pv.y + aim * pv.x));
+        x[k] = xv;
+        acc_cdot(v[1], xv, xv);
+        if (r_update) {
+            const float2 qv = q[k];
+            float2 rv = r[k];
+            rv.x = (float)(rv.x - (are * qv.x - aim * qv.y));
+            rv.y = (float)(rv.y - (are * qv.y + aim * qv.x));
+            r[k] = rv;
+            acc_cdot(v[0], rv, rv);
+        }
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// r = b - Nx (explicit residual, cgne.m:165-170 / :232-245); partial r'r
+// into part[..][0].  `force` ignores the done flags (initial residual).
+__global__ __launch_bounds__(kCgBlock) void cg_resid(const CgSeg* __restrict__ seg,
+                                                     const int* __restrict__ b2s,
+                                                     const CgScal* __restrict__ sc,
+                                                     const float2* __restrict__ b,
+                                                     const float2* __restrict__ nx,
+                                                     float2* __restrict__ r, float2* __restrict__ p,
+                                                     double2* __restrict__ part) {
+    const int s = find_slice(seg, b2s);
+    if (sc[s].done) return;
+    const CgSeg g = seg[s];
+    const int bl = blockIdx.x - g.blk0;
+    double2 v[1] = {make_double2(0, 0)};
+    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
+        const int64_t k = g.off + i;
+        const float2 bv = b[k], nv = nx[k];
+        const float2 rv = make_float2(bv.x - nv.x, bv.y - nv.y);
+        r[k] = rv;
+        if (p) p[k] = rv;
+        acc_cdot(v[0], rv, rv);
+    }
+    block_sum<1>(v);
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = v[0];
+}
+
+// |b|^2 (for the relative threshold) and x0 = b when requested
+__global__ __launch_bounds__(kCgBlock) void cg_norm_b(const CgSeg* __restrict__ seg,
+                                                      const int* __restrict__ b2s,
+                                                      const float2* __restrict__ b,
+                                                      float2* __restrict__ x, int copy,
+                                                      double2* __restrict__ part) {
+    const int s = find_slice(seg, b2s);
+    const CgSeg g = seg[s];
+    const int bl = blockIdx.x - g.blk0;
+    double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
+        const int64_t k = g.off + i;
+        const float2 bv = b[k];
+        acc_cdot(v[0], bv, bv);
+        const float2 xv = copy ? bv : x[k];
+        if (copy) x[k] = bv;
+        acc_cdot(v[1], xv, xv);
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// Initial scalars after r0 = b - N x0 (cgne.m:180-207).
+__global__ void cg_init(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
+                        const double2* __restrict__ part_b, const double2* __restrict__ part_r,
+                        double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
+                        int nslice, int nitmax, double resThresh, double resThreshRel,
+                        double stopThresh) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslice) return;
+    const CgSeg g = seg[s];
+    double bb = 0, xx = 0, rr = 0;
+    for (int b = 0; b < g.nblk; ++b) {
+        bb += part_b[2 * (g.blk0 + b)].x;
+        xx += part_b[2 * (g.blk0 + b) + 1].x;
+        rr += part_r[2 * (g.blk0 + b)].x;
+    }
+    CgScal c{};
+    c.rho_re = rr;
+    c.rho_im = 0;
+    c.nit = 0;
+    c.nitmax = nitmax;
+    c.stopThresh = stopThresh;
+    double th = resThresh;
+    if (resThreshRel >= 0) th = th >= 0 ? fmax(th, resThreshRel * bb) : resThreshRel * bb;
+    c.resThresh = th;
+    c.done = nitmax <= 0;
+    sc[s] = c;
+    const int64_t h = (int64_t)s * (nitmax + 1);
+    hres[h] = rr;
+    hxn[h] = xx;
+    hxd[h] = xx;
+}
+
+// res_sq_new, histories, stopping tests and beta (cgne.m:252-335).
+__global__ void cg_beta(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
+                        const double2* __restrict__ part, const double* __restrict__ pp,
+                        double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
+                        int nslice, int nitmax_alloc, int explicit_res, double xChangeThresh,
+                        double resChangeThresh, int stopOnResInc, double resIncThresh,
+                        int stopOnXdifInc, int nseExtraIt, int* __restrict__ done_host) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslice) return;
+    CgScal c = sc[s];
+    if (c.done) {
+        done_host[s] = 1;
+        return;
+    }
+    const CgSeg g = seg[s];
+    double rr = 0, xx = 0;
+    for (int b = 0; b < g.nblk; ++b) {
+        rr += part[2 * (g.blk0 + b)].x;
+        xx += part[2 * (g.blk0 + b) + 1].x;
+    }
+    const double a2 = c.al_re * c.al_re + c.al_im * c.al_im;
+    const double xd = a2 * pp[s];
+    const int64_t h = (int64_t)s * (nitmax_alloc + 1);
+    const int n = c.nit;
+    hres[h + n + 1] = rr;
+    hxn[h + n + 1] = xx;
+    hxd[h + n + 1] = xd;
+    int finished = 0;
+    if (xChangeThresh >= 0 && xd < xChangeThresh) { finished = 1; c.status = 10; }
+    if (c.resThresh >= 0 && rr < c.resThresh) { finished = 1; c.status = 1; }
+    if (resChangeThresh >= 0 && fabs((hres[h + n + 1] - hres[h + n]) / hres[h + n]) < resChangeThresh) {
+        finished = 1;
+        c.status = 2;
+    }
+    if (c.stopThresh >= 0 && rr < c.stopThresh) {
+        c.nitmax = n + nseExtraIt;
+        c.stopThresh = 0;
+        c.status = 3;
+    }
+    if (stopOnResInc > 0 && hres[h + n + 1] / hres[h + n] > resIncThresh) {
+        c.resIncs++;
+        if (c.resIncs >= stopOnResInc) { finished = 1; c.status = 4; }
+    }
+    if (stopOnXdifInc > 0 && hxd[h + n + 1] > hxd[h + n]) {
+        if (!c.lastXdifInc) {
+            c.xdifIncs++;
+            c.lastXdifInc = 1;
+            if (c.xdifIncs >= stopOnXdifInc) { finished = 1; c.status = 5; }
+        }
+    } else {
+        c.lastXdifInc = 0;
+    }
+    const double betaFact = explicit_res ? 0.0 : 1.0;
+    c.beta = betaFact * rr / c.rho_re;
+    c.rho_re = rr;
+    c.rho_im = 0;
+    c.nit = n + 1;
+    if (c.nit >= c.nitmax) { finished = 1; c.status = 0; }
+    // the p update of this iteration still runs (beta above); `done` takes
+    // effect from the next kernel that checks it
+    c.done = finished ? 2 : 0;
+    sc[s] = c;
+    done_host[s] = finished;
+}
+
+// p = r + beta p (cgne.m:328); a slice that finished this iteration (done == 2)
+// is promoted to done == 1.
+__global__ __launch_bounds__(kCgBlock) void cg_update_p(const CgSeg* __restrict__ seg,
+                                                        const int* __restrict__ b2s,
+                                                        CgScal* __restrict__ sc,
+                                                        float2* __restrict__ p,
+                                                        const float2* __restrict__ r) {
+    const int s = find_slice(seg, b2s);
+    const int dn = sc[s].done;
+    if (dn == 1) return;
+    const CgSeg g = seg[s];
+    const int bl = blockIdx.x - g.blk0;
+    if (dn == 2) {
+        if (bl == 0 && threadIdx.x == 0) sc[s].done = 1;
+        return;
+    }
+    const double be = sc[s].beta;
+    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
+        const int64_t k = g.off + i;
+        const float2 rv = r[k], pv = p[k];
+        p[k] = make_float2((float)(rv.x + be * pv.x), (float)(rv.y + be * pv.y));
+    }
+}
+
+__global__ void copy_rr_k(const double2* __restrict__ src, double2* __restrict__ dst, int nblk) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nblk) dst[2 * b] = src[2 * b];
+}
+
+__global__ void scale_copy_k(const float2* __restrict__ y, const float* __restrict__ w,
+                             float2* __restrict__ out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = make_float2(y[i].x * w[i], y[i].y * w[i]);
+}
+
+__global__ void zero_scal_k(CgScal* sc, int ns) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < ns) sc[s] = CgScal{};
+}
+
+// ---------------------------------------------------------------------------
+CgSolver::CgSolver(mm_op* E) : E_(E) {
+    const int ns = E->nslice();
+    std::vector<CgSeg> seg(ns);
+    std::vector<int> b2s;
+    int64_t tot = 0;
+    for (int s = 0; s < ns; ++s) {
+        const int64_t off = E->slice_image_off(s);
+        const int64_t end = s + 1 < ns ? E->slice_image_off(s + 1) : E->n_image();
+        seg[s].off = off;
+        seg[s].len = end - off;
+        const int64_t want = (seg[s].len + kCgBlock * 4 - 1) / (kCgBlock * 4);
+        seg[s].nblk = (int)std::max<int64_t>(1, std::min<int64_t>(want, kCgMaxBlocksPerSlice));
+        seg[s].blk0 = (int)b2s.size();
+        for (int b = 0; b < seg[s].nblk; ++b) b2s.push_back(s);
+        tot += seg[s].len;
+    }
+    nblk_ = (int)b2s.size();
+    n_ = E->n_image();
+    hipStream_t st = E->ctx->stream;
+    seg_.upload(seg, st);
+    b2s_.upload(b2s, st);
+    sc_.alloc(ns);
+    part_.alloc((size_t)2 * nblk_);
+    part2_.alloc((size_t)2 * nblk_);
+    pp_.alloc(ns);
+    b_.alloc(n_);
+    r_.alloc(n_);
+    p_.alloc(n_);
+    q_.alloc(n_);
+    tmp_.alloc(E->n_data());
+    HIPCHECK(hipHostMalloc((void**)&done_h_, sizeof(int) * ns, hipHostMallocDefault));
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+CgSolver::~CgSolver() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (done_h_) (void)hipHostFree(done_h_);
+}
+
+void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x) {
+    hipStream_t st = E_->ctx->stream;
+    const int ns = E_->nslice();
+    const int sb = (ns + 63) / 64;
+    E_->apply_normal(p_.p, q_.p, 1, tmp_.p);
+    hipLaunchKernelGGL(cg_dot_pq, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, q_.p,
+                       part_.p);
+    hipLaunchKernelGGL(cg_alpha, dim3(sb), dim3(64), 0, st, seg_.p, sc_.p, part_.p, pp_.p, ns);
+    hipLaunchKernelGGL(cg_update, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, x, p_.p,
+                       r_.p, q_.p, explicit_res ? 0 : 1, part_.p);
+    if (explicit_res) {
+        // r = b - N x; r'r into part2, x'x stays in part_[..][1]
+        E_->apply_normal(x, q_.p, 1, tmp_.p);
+        hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p,
+                           q_.p, r_.p, (float2*)nullptr, part2_.p);
+        hipLaunchKernelGGL(copy_rr_k, dim3((nblk_ + 255) / 256), dim3(256), 0, st, part2_.p, part_.p,
+                           nblk_);
+    }
+    hipLaunchKernelGGL(cg_beta, dim3(sb), dim3(64), 0, st, seg_.p, sc_.p, part_.p, pp_.p, hres_.p, hxn_.p,
+                       hxd_.p, ns, nitmax_, explicit_res, o.xChangeThresh, o.resChangeThresh,
+                       o.stopOnResInc, o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt, done_d_.p);
+    hipLaunchKernelGGL(cg_update_p, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, r_.p);
+    LAUNCHCHECK();
+}
+
+void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, const mm_cg_opts& o,
+                     double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status) {
+    hipStream_t st = E_->ctx->stream;
+    const int ns = E_->nslice();
+    const int sb = (ns + 63) / 64;
+    if (NitMax < 0) fail(MM_ERR_ARG, "cgne: NitMax must be >= 0");
+    if (NitMax != nitmax_) {
+        nitmax_ = NitMax;
+        hres_.alloc((size_t)ns * (NitMax + 1));
+        hxn_.alloc((size_t)ns * (NitMax + 1));
+        hxd_.alloc((size_t)ns * (NitMax + 1));
+        done_d_.alloc(ns);
+        if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+        if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    }
+    if (graph_x_ != x) {
+        if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+        if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    }
+    // b = E'(W.*y)  (cgne.m:136-140)
+    if (E_->W) {
+        hipLaunchKernelGGL(scale_copy_k, dim3(256), dim3(256), 0, st, y, E_->W, tmp_.p, E_->n_data());
+        E_->apply(1, tmp_.p, b_.p, 1);
+    } else {
+        E_->apply(1, y, b_.p, 1);
+    }
+    // x0 (cgne.m:143-145)
+    if (x0) HIPCHECK(hipMemcpyAsync(x, x0, sizeof(float2) * n_, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(cg_norm_b, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, b_.p, x,
+                       x0 ? 0 : 1, part2_.p);
+    // r = b - N x0, p = r  (cgne.m:165-180)
+    E_->apply_normal(x, q_.p, 1, tmp_.p);
+    hipLaunchKernelGGL(zero_scal_k, dim3(sb), dim3(64), 0, st, sc_.p, ns);
+    hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p, q_.p,
+                       r_.p, p_.p, part_.p);
+    hipLaunchKernelGGL(cg_init, dim3(sb), dim3(64), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
+                       hxn_.p, hxd_.p, ns, NitMax, o.resThresh, o.resThreshRel, o.stopThresh);
+    LAUNCHCHECK();
+    const int expResN = o.expResN > 0 ? o.expResN : 100;
+    const int poll = 4;
+    for (int it = 0; it < NitMax; ++it) {
+        const int explicit_res = (it % expResN) == 0;
+        if (explicit_res || !use_graph_) {
+            launch_iteration(explicit_res, o, x);
+        } else {
+            if (!graph_exec_) {
+                HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                launch_iteration(0, o, x);
+                HIPCHECK(hipStreamEndCapture(st, &graph_));
+                HIPCHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+                graph_x_ = x;
+            }
+            HIPCHECK(hipGraphLaunch(graph_exec_, st));
+        }
+        if ((it + 1) % poll == 0 && it + 1 < NitMax) {
+            HIPCHECK(hipMemcpyAsync(done_h_, done_d_.p, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            bool all = true;
+            for (int s = 0; s < ns; ++s) all &= done_h_[s] != 0;
+            if (all) break;
+        }
+    }
+    // results
+    std::vector<CgScal> hs(ns);
+    HIPCHECK(hipMemcpyAsync(hs.data(), sc_.p, sizeof(CgScal) * ns, hipMemcpyDeviceToHost, st));
+    const size_t hn = (size_t)ns * (NitMax + 1);
+    std::vector<double> a(hn), b(hn), c(hn);
+    HIPCHECK(hipMemcpyAsync(a.data(), hres_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(b.data(), hxn_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(c.data(), hxd_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int s = 0; s < ns; ++s) {
+        if (nit) nit[s] = hs[s].nit;
+        if (status) status[s] = hs[s].status;
+    }
+    if (resSqAll) std::memcpy(resSqAll, a.data(), sizeof(double) * hn);
+    if (xnormAll) std::memcpy(xnormAll, b.data(), sizeof(double) * hn);
+    if (xdiffAll) std::memcpy(xdiffAll, c.data(), sizeof(double) * hn);
+}
+
+}  // namespace mm

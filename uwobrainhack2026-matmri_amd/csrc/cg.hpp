@@ -1,0 +1,38 @@
+// Device cgne solver (declarations).
+#pragma once
+#include "common.hpp"
+
+namespace mm {
+
+struct CgScal;
+struct CgSeg;
+
+class CgSolver {
+   public:
+    explicit CgSolver(mm_op* E);
+    ~CgSolver();
+    // all vectors are device pointers; histories/nit/status are host
+    void solve(const float2* y, const float2* x0, float2* x, int NitMax, const mm_cg_opts& o,
+               double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status);
+    bool use_graph_ = true;
+
+   private:
+    void launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x);
+    mm_op* E_;
+    int64_t n_ = 0;
+    int nblk_ = 0;
+    int nitmax_ = -1;
+    DevBuf<CgSeg> seg_;
+    DevBuf<int> b2s_;
+    DevBuf<CgScal> sc_;
+    DevBuf<double2> part_, part2_;
+    DevBuf<double> pp_, hres_, hxn_, hxd_;
+    DevBuf<int> done_d_;
+    DevBuf<float2> b_, r_, p_, q_, tmp_;
+    int* done_h_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    float2* graph_x_ = nullptr;
+};
+
+}  // namespace mm

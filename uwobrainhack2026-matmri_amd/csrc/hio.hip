@@ -1,0 +1,509 @@
+// Expanded-encoding operator (sampHighOrder, direct model) for gfx950.
+//
+// Reference: operators/@sampHighOrder/sampHighOrder.m
+//   phase   Phi[v,k] = sum_n B_n(r_v) mu_n(v) h_n(t_k)       (prepForDirect :496-588;
+//                    + sum_n Cc_n(r_v) mu(v) g_n(t_k)          mu = b0mask for
+//                    + mu(v) b0(v) t_k  (K > 1 only)           harmonics n > 4)
+//   forward y = exp(+1i*Phi).' * x / sqrt(V)                   (:374-387)
+//   adjoint x = exp(-1i*Phi)   * y / sqrt(V)                   (:369-372, :387)
+//   phiDiv  y = (1i*exp(1i*Phi).*dPhi).' * x / sqrt(V)          (:375-382, :590-622)
+// Approaches 0, 2 and 3 (direct, segmented, segmented without precompute)
+// compute the same values; here the phase is never materialised: it is
+// regenerated per (voxel, sample) tile from O((V+K) r) factors.
+//
+// Phase precision.  |Phi| reaches 1e4 rad, where an FP32 ulp is 1e-3 rad,
+// so the contraction is done in FP64 and reduced modulo one revolution
+// before the FP32 sine/cosine.  The L phase terms are first compressed on
+// the host: B (V x L, FP64) = Q R by Gram-Schmidt with re-orthogonalisation,
+// dropping directions whose residual is below 1e-13 of the column norm
+// (on a 2-D slice the 16 third-order harmonics span only the 10 planar
+// monomials), and the coefficients become R*h/(2*pi) (revolutions).
+#include <algorithm>
+#include <cmath>
+
+#include "fft.hpp"
+#include "hio.hpp"
+
+namespace mm {
+
+constexpr int kHioMaxRank = 24;
+constexpr int kHioMaxDiv = 24;
+constexpr int kHioTile = 64;
+
+// computeHarmonics/basisFuncHarm.m:24-96 (n = 1..36, 1-based)
+static double basis_harm(double x, double y, double z, int n) {
+    const double r2 = x * x + y * y + z * z;
+    switch (n) {
+        case 1: return 1.0;
+        case 2: return x;
+        case 3: return y;
+        case 4: return z;
+        case 5: return x * y;
+        case 6: return z * y;
+        case 7: return 3 * z * z - r2;
+        case 8: return x * z;
+        case 9: return x * x - y * y;
+        case 10: return 3 * y * x * x - y * y * y;
+        case 11: return x * z * y;
+        case 12: return (5 * z * z - r2) * y;
+        case 13: return 5 * z * z * z - 3 * z * r2;
+        case 14: return (5 * z * z - r2) * x;
+        case 15: return x * x * z - y * y * z;
+        case 16: return x * x * x - 3 * x * y * y;
+        case 17: return (x * y) * (x * x - y * y);
+        case 18: return (y * z) * (3 * x * x - y * y);
+        case 19: return (x * y) * (7 * z * z - r2);
+        case 20: return y * (7 * z * z * z - 3 * z * r2);
+        case 21: return 35 * z * z * z * z - 30 * z * z * r2 + 3 * r2 * r2;
+        case 22: return x * (7 * z * z * z - 3 * z * r2);
+        case 23: return (x * x - y * y) * (7 * z * z - r2);
+        case 24: return x * (x * x - 3 * y * y) * z;
+        case 25: return x * x * (x * x - 3 * y * y) - y * y * (3 * x * x - y * y);
+        case 26: return y * (5 * x * x * x * x - 10 * x * x * y * y + y * y * y * y);
+        case 27: return x * y * z * (x * x - y * y);
+        case 28: return y * (r2 - 9 * z * z) * (y * y - 3 * x * x);
+        case 29: return x * y * z * (3 * z * z - r2);
+        case 30: return y * (r2 * r2 - 14 * r2 * z * z + 21 * z * z * z * z);
+        case 31: return 63 * std::pow(z, 5) - 70 * z * z * z * r2 + 15 * z * r2 * r2;
+        case 32: return x * (r2 * r2 - 14 * r2 * z * z + 21 * z * z * z * z);
+        case 33: return z * (3 * z * z - r2) * (x * x - y * y);
+        case 34: return x * (r2 - 9 * z * z) * (3 * y * y - x * x);
+        case 35: return z * (x * x * x * x - 6 * x * x * y * y + y * y * y * y);
+        case 36: return std::pow(x, 5) - 10 * x * x * x * y * y + 5 * x * std::pow(y, 4);
+        default: fail(MM_ERR_ARG, "Basis function undefined.");
+    }
+}
+
+// computeHarmonics/basisFuncConc.m:13-24
+static double basis_conc(double x, double y, double z, int n) {
+    switch (n) {
+        case 1: return z * z;
+        case 2: return x * x + y * y;
+        case 3: return x * z;
+        case 4: return y * z;
+        default: fail(MM_ERR_ARG, "Basis function undefined.");
+    }
+}
+
+HioOp::HioOp(mm_ctx* c, int64_t V, int64_t K, int Lh, int Lc, const double* x, const double* y,
+             const double* z, const double* b0, const double* t, const double* h, const double* g,
+             const double* mask)
+    : V_(V), K_(K) {
+    ctx = c;
+    if (V < 1 || K < 1) fail(MM_ERR_ARG, "sampHighOrder: empty image or k-space");
+    if (Lh < 0 || Lh > 36 || Lc < 0 || Lc > 4) fail(MM_ERR_ARG, "sampHighOrder: basis count out of range");
+    if (!x || !y || !z || !t || (Lh > 0 && !h) || (Lc > 0 && !g))
+        fail(MM_ERR_ARG, "sampHighOrder: NULL argument");
+    const bool hasB0 = K > 1 && b0 != nullptr;
+    Lh_ = Lh;
+    Lc_ = Lc;
+    Lnb_ = Lh + Lc;
+    L_ = Lnb_ + (hasB0 ? 1 : 0);
+    if (L_ == 0) fail(MM_ERR_ARG, "sampHighOrder: no phase terms");
+    // voxel basis B [V][L] with the reference's masking (sampHighOrder.m:560-586)
+    std::vector<double> B((size_t)V * L_);
+    for (int64_t v = 0; v < V; ++v) {
+        const double mu = mask ? mask[v] : 1.0;
+        double* row = &B[(size_t)v * L_];
+        for (int n = 1; n <= Lh; ++n) row[n - 1] = basis_harm(x[v], y[v], z[v], n) * (n > 4 ? mu : 1.0);
+        for (int n = 1; n <= Lc; ++n) row[Lh + n - 1] = basis_conc(x[v], y[v], z[v], n) * mu;
+        if (hasB0) row[Lnb_] = b0[v] * mu;
+    }
+    Bnb_host_.resize((size_t)V * std::max(Lnb_, 1));
+    for (int64_t v = 0; v < V; ++v)
+        for (int l = 0; l < Lnb_; ++l) Bnb_host_[(size_t)v * Lnb_ + l] = B[(size_t)v * L_ + l];
+    // coefficients H [L][K] in radians
+    std::vector<double> H((size_t)L_ * K);
+    for (int64_t k = 0; k < K; ++k) {
+        for (int l = 0; l < Lh; ++l) H[(size_t)l * K + k] = h[(size_t)k * Lh + l];
+        for (int l = 0; l < Lc; ++l) H[(size_t)(Lh + l) * K + k] = g[(size_t)k * Lc + l];
+        if (hasB0) H[(size_t)Lnb_ * K + k] = t[k];
+    }
+    // max |Phi| (FP64, reported for the precision study)
+    {
+        double mx = 0;
+        const int64_t sv = std::max<int64_t>(1, V / 4096), sk = std::max<int64_t>(1, K / 4096);
+        for (int64_t v = 0; v < V; v += sv)
+            for (int64_t k = 0; k < K; k += sk) {
+                double p = 0;
+                for (int l = 0; l < L_; ++l) p += B[(size_t)v * L_ + l] * H[(size_t)l * K + k];
+                mx = std::max(mx, std::fabs(p));
+            }
+        // include the last sample, where the trajectory phase usually peaks
+        for (int64_t v = 0; v < V; v += sv) {
+            double p = 0;
+            for (int l = 0; l < L_; ++l) p += B[(size_t)v * L_ + l] * H[(size_t)l * K + (K - 1)];
+            mx = std::max(mx, std::fabs(p));
+        }
+        maxphi_ = mx;
+    }
+    // Gram-Schmidt (twice) on the columns of B: B = Q R
+    std::vector<double> Q((size_t)V * L_, 0.0), R((size_t)L_ * L_, 0.0);
+    std::vector<int> keep;
+    std::vector<double> col(V);
+    for (int l = 0; l < L_; ++l) {
+        double n0 = 0;
+        for (int64_t v = 0; v < V; ++v) {
+            col[v] = B[(size_t)v * L_ + l];
+            n0 += col[v] * col[v];
+        }
+        n0 = std::sqrt(n0);
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t j = 0; j < keep.size(); ++j) {
+                double d = 0;
+                for (int64_t v = 0; v < V; ++v) d += Q[(size_t)v * L_ + j] * col[v];
+                R[j * L_ + l] += d;
+                for (int64_t v = 0; v < V; ++v) col[v] -= d * Q[(size_t)v * L_ + j];
+            }
+        double nr = 0;
+        for (int64_t v = 0; v < V; ++v) nr += col[v] * col[v];
+        nr = std::sqrt(nr);
+        if (n0 > 0 && nr > 1e-13 * n0) {
+            const size_t j = keep.size();
+            for (int64_t v = 0; v < V; ++v) Q[(size_t)v * L_ + j] = col[v] / nr;
+            R[j * L_ + l] = nr;
+            keep.push_back(l);
+        }
+    }
+    r_ = (int)keep.size();
+    if (r_ == 0) r_ = 1;  // all-zero phase: one zero column
+    std::vector<double> Qc((size_t)V * r_, 0.0), Hc((size_t)K * r_, 0.0);
+    for (int64_t v = 0; v < V; ++v)
+        for (int j = 0; j < (int)keep.size(); ++j) Qc[(size_t)v * r_ + j] = Q[(size_t)v * L_ + j];
+    const double inv2pi = 1.0 / (2.0 * M_PI);
+    for (int64_t k = 0; k < K; ++k)
+        for (int j = 0; j < (int)keep.size(); ++j) {
+            double s = 0;
+            for (int l = 0; l < L_; ++l) s += R[(size_t)j * L_ + l] * H[(size_t)l * K + k];
+            Hc[(size_t)k * r_ + j] = s * inv2pi;
+        }
+    Q_.upload(Qc, ctx->stream);
+    Hc_.upload(Hc, ctx->stream);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void HioOp::set_phi_div(const double* hd, const double* gd) {
+    if (!hd && !gd) {
+        phidiv_ = false;
+        return;
+    }
+    if (Lnb_ == 0) fail(MM_ERR_ARG, "setPhiDiv: no harmonic or concomitant terms");
+    if ((Lh_ > 0 && !hd) || (Lc_ > 0 && !gd)) fail(MM_ERR_ARG, "setPhiDiv: missing coefficients");
+    if (Lnb_ > kHioMaxDiv) fail(MM_ERR_UNSUPPORTED, "setPhiDiv: more than 24 terms");
+    // dPhi = prepForDirect(ksphaDiv, kconcDiv, 1): same basis and masking, no B0 term
+    std::vector<double> Hd((size_t)K_ * Lnb_);
+    for (int64_t k = 0; k < K_; ++k) {
+        for (int l = 0; l < Lh_; ++l) Hd[(size_t)k * Lnb_ + l] = hd[(size_t)k * Lh_ + l];
+        for (int l = 0; l < Lc_; ++l) Hd[(size_t)k * Lnb_ + Lh_ + l] = gd[(size_t)k * Lc_ + l];
+    }
+    Bnb_.upload(Bnb_host_, ctx->stream);
+    Hd_.upload(Hd, ctx->stream);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    phidiv_ = true;
+}
+
+// ---------------------------------------------------------------------------
+// device code
+
+__global__ void hio_prep_x(const float2* __restrict__ in, const float2* __restrict__ maps,
+                           float2* __restrict__ X, int64_t V, int ncol, int C, float scale) {
+    const int64_t n = V * ncol;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int col = (int)(i % ncol);
+        const int64_t v = i / ncol;
+        float2 val;
+        if (maps) {
+            const int rep = col / C, c = col % C;
+            val = cmulf(in[(int64_t)rep * V + v], maps[(int64_t)c * V + v]);
+        } else {
+            val = in[(int64_t)col * V + v];
+        }
+        X[i] = make_float2(val.x * scale, val.y * scale);
+    }
+}
+
+// Forward partial sums: part[split][k][col] = sum_{v in split} e^{i Phi} X[v][col]
+template <int NC, bool PHIDIV>
+__global__ __launch_bounds__(256) void hio_fwd_simple(const double* __restrict__ Q,
+                                                      const double* __restrict__ Hc, int r,
+                                                      int64_t V, int64_t K,
+                                                      const float2* __restrict__ X, int ncol,
+                                                      float2* __restrict__ part, int64_t vchunk,
+                                                      const double* __restrict__ Bnb,
+                                                      const double* __restrict__ Hd, int Lnb) {
+    __shared__ float2 xs[kHioTile][NC];
+    __shared__ double qs[kHioTile][kHioMaxRank];
+    __shared__ double bs[PHIDIV ? kHioTile : 1][PHIDIV ? kHioMaxDiv : 1];
+    double hd[PHIDIV ? kHioMaxDiv : 1];
+    if constexpr (PHIDIV) {
+#pragma unroll
+        for (int j = 0; j < kHioMaxDiv; ++j)
+            hd[j] = (j < Lnb && blockIdx.x * 256 + threadIdx.x < K) ? Hd[(blockIdx.x * 256 + threadIdx.x) * Lnb + j] : 0.0;
+    }
+    const int64_t k = blockIdx.x * 256 + threadIdx.x;
+    const int split = blockIdx.y;
+    const int col0 = blockIdx.z * NC;
+    const int64_t v0 = split * vchunk, v1 = min(V, v0 + vchunk);
+    double hk[kHioMaxRank];
+#pragma unroll
+    for (int j = 0; j < kHioMaxRank; ++j) hk[j] = (j < r && k < K) ? Hc[k * r + j] : 0.0;
+    float2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = make_float2(0.f, 0.f);
+    for (int64_t vb = v0; vb < v1; vb += kHioTile) {
+        const int nv = (int)min((int64_t)kHioTile, v1 - vb);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kHioTile * NC; i += 256) {
+            const int vv = i / NC, c = i % NC;
+            xs[vv][c] = (vv < nv && col0 + c < ncol) ? X[(vb + vv) * ncol + col0 + c] : make_float2(0.f, 0.f);
+        }
+        for (int i = threadIdx.x; i < kHioTile * kHioMaxRank; i += 256) {
+            const int vv = i / kHioMaxRank, j = i % kHioMaxRank;
+            qs[vv][j] = (vv < nv && j < r) ? Q[(vb + vv) * r + j] : 0.0;
+        }
+        if constexpr (PHIDIV) {
+            for (int i = threadIdx.x; i < kHioTile * kHioMaxDiv; i += 256) {
+                const int vv = i / kHioMaxDiv, j = i % kHioMaxDiv;
+                bs[vv][j] = (vv < nv && j < Lnb) ? Bnb[(vb + vv) * Lnb + j] : 0.0;
+            }
+        }
+        __syncthreads();
+        for (int vv = 0; vv < nv; ++vv) {
+            double ph = 0;
+#pragma unroll
+            for (int j = 0; j < kHioMaxRank; ++j)
+                if (j < r) ph = fma(qs[vv][j], hk[j], ph);
+            const float f = (float)(ph - rint(ph));
+            float s, c;
+            sincospif(2.f * f, &s, &c);
+            if constexpr (PHIDIV) {
+                // 1i * e^{i phi} * dPhi
+                double d = 0;
+#pragma unroll
+                for (int j = 0; j < kHioMaxDiv; ++j)
+                    if (j < Lnb) d = fma(bs[vv][j], hd[j], d);
+                const float df = (float)d;
+                const float c2 = -s * df, s2 = c * df;
+                c = c2;
+                s = s2;
+            }
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const float2 xv = xs[vv][q];
+                acc[q].x += c * xv.x - s * xv.y;
+                acc[q].y += c * xv.y + s * xv.x;
+            }
+        }
+    }
+    if (k < K) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q)
+            if (col0 + q < ncol) part[((int64_t)split * K + k) * ncol + col0 + q] = acc[q];
+    }
+}
+
+// Adjoint partial sums: part[split][v][col] = sum_{k in split} e^{-i Phi} Y[k][col]
+template <int NC>
+__global__ __launch_bounds__(256) void hio_adj_simple(const double* __restrict__ Q,
+                                                      const double* __restrict__ Hc, int r,
+                                                      int64_t V, int64_t K,
+                                                      const float2* __restrict__ Y, int ncol,
+                                                      float2* __restrict__ part, int64_t kchunk) {
+    __shared__ float2 ys[kHioTile][NC];
+    __shared__ double hs[kHioTile][kHioMaxRank];
+    const int64_t v = blockIdx.x * 256 + threadIdx.x;
+    const int split = blockIdx.y;
+    const int col0 = blockIdx.z * NC;
+    const int64_t k0 = split * kchunk, k1 = min(K, k0 + kchunk);
+    double qv[kHioMaxRank];
+#pragma unroll
+    for (int j = 0; j < kHioMaxRank; ++j) qv[j] = (j < r && v < V) ? Q[v * r + j] : 0.0;
+    float2 acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = make_float2(0.f, 0.f);
+    for (int64_t kb = k0; kb < k1; kb += kHioTile) {
+        const int nk = (int)min((int64_t)kHioTile, k1 - kb);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kHioTile * NC; i += 256) {
+            const int kk = i % kHioTile, c = i / kHioTile;   // k fastest: coalesced over k
+            ys[kk][c] = (kk < nk && col0 + c < ncol) ? Y[(int64_t)(col0 + c) * K + kb + kk]
+                                                      : make_float2(0.f, 0.f);
+        }
+        for (int i = threadIdx.x; i < kHioTile * kHioMaxRank; i += 256) {
+            const int kk = i / kHioMaxRank, j = i % kHioMaxRank;
+            hs[kk][j] = (kk < nk && j < r) ? Hc[(kb + kk) * r + j] : 0.0;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < nk; ++kk) {
+            double ph = 0;
+#pragma unroll
+            for (int j = 0; j < kHioMaxRank; ++j)
+                if (j < r) ph = fma(qv[j], hs[kk][j], ph);
+            const float f = (float)(ph - rint(ph));
+            float s, c;
+            sincospif(2.f * f, &s, &c);
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const float2 yv = ys[kk][q];
+                acc[q].x += c * yv.x + s * yv.y;   // e^{-i phi} * y
+                acc[q].y += c * yv.y - s * yv.x;
+            }
+        }
+    }
+    if (v < V) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q)
+            if (col0 + q < ncol) part[((int64_t)split * V + v) * ncol + col0 + q] = acc[q];
+    }
+}
+
+// y[col][k] = sum_split part[split][k][col] (* W)
+__global__ void hio_fwd_reduce(const float2* __restrict__ part, float2* __restrict__ y, int64_t K,
+                               int ncol, int nsplit, const float* __restrict__ W) {
+    const int64_t n = K * ncol;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = i % K;
+        const int col = (int)(i / K);
+        float2 s = make_float2(0.f, 0.f);
+        for (int p = 0; p < nsplit; ++p) {
+            const float2 q = part[((int64_t)p * K + k) * ncol + col];
+            s.x += q.x;
+            s.y += q.y;
+        }
+        if (W) {
+            s.x *= W[i];
+            s.y *= W[i];
+        }
+        y[i] = s;
+    }
+}
+
+// x = sum_split part / sqrt(V); with maps: coil combine per repetition
+__global__ void hio_adj_reduce(const float2* __restrict__ part, float2* __restrict__ x, int64_t V,
+                               int ncol, int nsplit, const float2* __restrict__ maps, int C,
+                               float scale) {
+    const int nout = maps ? ncol / C : ncol;
+    const int64_t n = V * nout;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i % V;
+        const int o = (int)(i / V);
+        float2 res = make_float2(0.f, 0.f);
+        const int cn = maps ? C : 1;
+        for (int c = 0; c < cn; ++c) {
+            const int col = maps ? o * C + c : o;
+            float2 s = make_float2(0.f, 0.f);
+            for (int p = 0; p < nsplit; ++p) {
+                const float2 q = part[((int64_t)p * V + v) * ncol + col];
+                s.x += q.x;
+                s.y += q.y;
+            }
+            if (maps) {
+                const float2 m = maps[(int64_t)c * V + v];
+                res.x += m.x * s.x + m.y * s.y;
+                res.y += m.x * s.y - m.y * s.x;
+            } else {
+                res = s;
+            }
+        }
+        x[i] = make_float2(res.x * scale, res.y * scale);
+    }
+}
+
+static int grid1(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+// Times `iters` launches of `launch` between two events on `st` (ms per launch).
+template <class F>
+static double time_launches(hipStream_t st, int iters, F&& launch) {
+    hipEvent_t a, b;
+    HIPCHECK(hipEventCreate(&a));
+    HIPCHECK(hipEventCreate(&b));
+    launch();  // warm-up
+    HIPCHECK(hipEventRecord(a, st));
+    for (int i = 0; i < iters; ++i) launch();
+    HIPCHECK(hipEventRecord(b, st));
+    HIPCHECK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / iters;
+}
+
+void HioOp::run(int adjoint, const float2* in, float2* out, int64_t ncol64, const float2* maps, int C,
+                const float* W, int prof_iters, double* prof_ms) {
+    if (r_ > kHioMaxRank) fail(MM_ERR_UNSUPPORTED, "sampHighOrder: phase rank above 24");
+    hipStream_t st = ctx->stream;
+    const int ncol = (int)ncol64;
+    const float scale = (float)(1.0 / std::sqrt((double)V_));
+    const int NC = 32;
+    const int ncg = (ncol + NC - 1) / NC;
+    if (!adjoint) {
+        X_.ensure((size_t)V_ * ncol);
+        hipLaunchKernelGGL(hio_prep_x, dim3(grid1(V_ * ncol)), dim3(256), 0, st, in, maps, X_.p, V_, ncol,
+                           C, scale);
+        const int64_t kb = (K_ + 255) / 256;
+        int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (2048 + kb * ncg - 1) / (kb * ncg)));
+        nsplit = (int)std::min<int64_t>(nsplit, (V_ + kHioTile - 1) / kHioTile);
+        const int64_t vchunk = (V_ + nsplit - 1) / nsplit;
+        part_.ensure((size_t)nsplit * K_ * ncol);
+        auto main = [&] {
+            if (phidiv_)
+                hipLaunchKernelGGL((hio_fwd_simple<NC, true>), dim3((unsigned)kb, nsplit, ncg), dim3(256), 0,
+                                   st, Q_.p, Hc_.p, r_, V_, K_, X_.p, ncol, part_.p, vchunk, Bnb_.p, Hd_.p,
+                                   Lnb_);
+            else
+                hipLaunchKernelGGL((hio_fwd_simple<NC, false>), dim3((unsigned)kb, nsplit, ncg), dim3(256), 0,
+                                   st, Q_.p, Hc_.p, r_, V_, K_, X_.p, ncol, part_.p, vchunk,
+                                   (const double*)nullptr, (const double*)nullptr, 0);
+        };
+        if (prof_iters > 0) *prof_ms = time_launches(st, prof_iters, main);
+        else main();
+        hipLaunchKernelGGL(hio_fwd_reduce, dim3(grid1(K_ * ncol)), dim3(256), 0, st, part_.p, out, K_, ncol,
+                           nsplit, W);
+    } else {
+        const int64_t vb = (V_ + 255) / 256;
+        int nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(64, (2048 + vb * ncg - 1) / (vb * ncg)));
+        nsplit = (int)std::min<int64_t>(nsplit, (K_ + kHioTile - 1) / kHioTile);
+        const int64_t kchunk = (K_ + nsplit - 1) / nsplit;
+        part_.ensure((size_t)nsplit * V_ * ncol);
+        auto main = [&] {
+            hipLaunchKernelGGL(hio_adj_simple<NC>, dim3((unsigned)vb, nsplit, ncg), dim3(256), 0, st, Q_.p,
+                               Hc_.p, r_, V_, K_, in, ncol, part_.p, kchunk);
+        };
+        if (prof_iters > 0) *prof_ms = time_launches(st, prof_iters, main);
+        else main();
+        hipLaunchKernelGGL(hio_adj_reduce, dim3(grid1(V_ * (maps ? ncol / C : ncol))), dim3(256), 0, st,
+                           part_.p, out, V_, ncol, nsplit, maps, C, scale);
+    }
+    LAUNCHCHECK();
+}
+
+bool HioOp::profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
+                             double* ms, std::string* name) {
+    run(adjoint, in, out, nrep, nullptr, 1, nullptr, iters, ms);
+    *name = adjoint ? "hio_adj_simple" : "hio_fwd_simple";
+    return true;
+}
+
+bool HioOp::profile_sense(int adjoint, const float2* in, float2* out, int64_t nrep, const float2* maps,
+                          int C, int iters, double* ms, std::string* name) {
+    run(adjoint, in, out, nrep * C, maps, C, nullptr, iters, ms);
+    *name = adjoint ? "hio_adj_simple" : "hio_fwd_simple";
+    return true;
+}
+
+void HioOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
+    if (nrep < 1) fail(MM_ERR_ARG, "sampHighOrder: nrep must be >= 1");
+    run(adjoint, in, out, nrep, nullptr, 1, nullptr);
+}
+
+void HioOp::apply_sense(int adjoint, const float2* in, float2* out, int64_t nrep, const float2* maps,
+                        int C, const float* Wt) {
+    if (nrep < 1) fail(MM_ERR_ARG, "sampHighOrder: nrep must be >= 1");
+    run(adjoint, in, out, nrep * C, maps, C, adjoint ? nullptr : Wt);
+}
+
+}  // namespace mm

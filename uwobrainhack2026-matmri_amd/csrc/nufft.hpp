@@ -1,0 +1,111 @@
+// Kaiser-Bessel gridding NUFFT on gfx950 (declarations shared with the SENSE
+// composer).  The plan follows nufftOp.doPrep (operators/@nufftOp/nufftOp.m:
+// 138-274); forward and adjoint follow nufftOp.mtimes (:354-394).
+#pragma once
+#include <map>
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace mm {
+
+// Per-slice constants seen by every NUFFT kernel.
+struct NuSliceDev {
+    int N1, N2, G1, G2, M, W1, W2;
+    int fac1, fac2;     // index into the FFT factor pool
+    int tw1, tw2;       // offset into the twiddle pool (float2)
+    int deap1, deap2;   // offset into the deapodisation pool (float)
+    int rpw, cpw;       // rows / columns per work-group in the FFT passes
+    int tile_first;     // first gridding tile of this slice in the tile pool
+    int pad0;
+    int64_t samp_off;   // first sample of this slice in the sample pools
+    int64_t tlist_off;  // first entry of this slice in the tile sample list
+    int64_t img_off;    // image-side offset (elements) for the current apply
+    int64_t maps_off;   // coil-map offset (elements)
+    int64_t dat_off;    // data-side offset (elements)
+    int64_t ws_off;     // workspace offset of chunk 0 (elements)
+    int64_t ws_chunk;   // workspace elements per chunk = G2*G1*BCT
+    int64_t part_off;   // partial-tile slot offset of chunk 0
+    int64_t part_chunk; // partial-tile elements per chunk
+    int64_t cnt_off;    // counter offset of chunk 0
+    int64_t cnt_chunk;  // counters per chunk
+};
+
+// One gridding work item: a chunk of the sample list of one tile.
+struct TileWork {
+    int t1, t2;         // tile coordinates (16x16 grid cells)
+    int start, count;   // range in the slice's tile sample list
+    int slot;           // partial slot (relative to the slice) or -1 if the tile is not split
+    int nparts;         // number of chunks of this tile
+    int first_slot;     // first partial slot of this tile
+    int counter;        // split-tile counter index (relative to the slice)
+};
+
+constexpr int kTile = 16;
+constexpr int kTileChunk = 512;
+
+class NufftOp : public mm_op {
+   public:
+    NufftOp(mm_ctx* ctx, int ndim, int nslice, const int64_t* imN, const int64_t* M,
+            const double* const* kloc, const double* const* dcf, double os, int kwidth);
+    int64_t n_image() const override { return V_total_; }
+    int64_t n_data() const override { return M_total_; }
+    int nslice() const override { return (int)sl_.size(); }
+    int64_t slice_image_off(int s) const override { return img_start_[s]; }
+    int64_t slice_data_off(int s) const override { return dat_start_[s]; }
+    void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
+    // SENSE-fused application: image side is one image per repetition,
+    // combined with `maps` ([V_s x C] per slice, concatenated); data side is
+    // [M_s x C x nrep] per slice.  W (nullable) multiplies the forward output.
+    void apply_sense(int adjoint, const float2* in, float2* out, int64_t nrep, const float2* maps,
+                     int C, const float* W);
+    int ndim() const { return ndim_; }
+    const std::vector<int64_t>& slice_V() const { return Vs_; }
+    const std::vector<int64_t>& slice_M() const { return Ms_; }
+
+    // host copies of the plan constants (exposed for tests)
+    double beta = 0;
+    int nkern = 0;
+
+   private:
+    struct Cfg {
+        int BCT = 1, sense = 0, C = 0;
+        int64_t nrep = 0;
+        std::vector<NuSliceDev> hs;
+        DevBuf<NuSliceDev> sl;
+        DevBuf<int4> w_rows, w_cols, w_deg, w_grid;
+        int n_rows = 0, n_cols = 0, n_deg = 0, n_grid = 0;
+        size_t lds_rows = 0, lds_cols = 0;
+        DevBuf<float2> ws, parts;
+        DevBuf<int> counters;
+    };
+    Cfg& config(int64_t nrep, int sense, int C);
+    void run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps, const float* W);
+
+    int ndim_ = 2;
+    int W_ = 4;
+    double os_ = 1.5;
+    std::vector<NuSliceDev> sl_;          // static part of each slice
+    std::vector<int64_t> Vs_, Ms_, img_start_, dat_start_;
+    int64_t V_total_ = 0, M_total_ = 0;
+    // tile work per slice: first index / count in tw_all_
+    std::vector<int> tw_first_, tw_count_, nslots_, ncounters_;
+    std::vector<TileWork> tw_host_;
+    // static device pools
+    DevBuf<FftFactors> facs_;
+    DevBuf<float2> twid_;
+    DevBuf<float> deap_;
+    DevBuf<uint32_t> sbase_;
+    DevBuf<float> swt_, sdcf_;
+    DevBuf<int> tlist_;
+    DevBuf<TileWork> tiles_;
+    std::vector<int64_t> tlist_off_;       // start of each slice's list in tlist_
+    std::map<std::pair<int64_t, int>, std::unique_ptr<Cfg>> cfgs_;
+};
+
+// Plan constants from the closed forms of nufftOp.m:146-182 (host, FP64).
+int nufft_grid_size(int64_t n, double os);
+double nufft_kb_beta(int kwidth, double os);
+int nufft_kb_nkern(int kwidth, double os);
+
+}  // namespace mm

@@ -1,0 +1,171 @@
+// Coil-sensitivity operator (rcvrOp) and the SENSE composer (mrSampFunc).
+//
+// rcvrOp maps path, operators/@rcvrOp/rcvrOp.m:145-157:
+//   forward  y(:,c) = x .* maps(:,c)
+//   adjoint  x = sum_c conj(maps(:,c)) .* y(:,c)
+// mrSampFunc, operators/mrSampFunc.m:82-159:  E = S o R,  E' = R' o S' (W
+// applied between them by the normal operator as cgne.m:165-170 does).
+#include "ops.hpp"
+
+namespace mm {
+
+void mm_op_default_normal(mm_op* op, const float2* in, float2* out, int64_t nrep, float2* tmp);
+
+__global__ void rcvr_fwd_k(const float2* __restrict__ x, const float2* __restrict__ maps,
+                           float2* __restrict__ y, int64_t V, int C, int64_t nrep) {
+    const int64_t n = V * C * nrep;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i % V;
+        const int64_t rest = i / V;
+        const int c = (int)(rest % C);
+        const int64_t r = rest / C;
+        y[i] = cmulf(x[r * V + v], maps[(int64_t)c * V + v]);
+    }
+}
+
+__global__ void rcvr_adj_k(const float2* __restrict__ y, const float2* __restrict__ maps,
+                           float2* __restrict__ x, int64_t V, int C, int64_t nrep) {
+    const int64_t n = V * nrep;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i % V;
+        const int64_t r = i / V;
+        float2 s = make_float2(0.f, 0.f);
+        for (int c = 0; c < C; ++c) {
+            const float2 m = maps[(int64_t)c * V + v];
+            const float2 q = y[(r * C + c) * V + v];
+            s.x += m.x * q.x + m.y * q.y;
+            s.y += m.x * q.y - m.y * q.x;
+        }
+        x[i] = s;
+    }
+}
+
+__global__ void scale_by_weights_k(float2* __restrict__ y, const float* __restrict__ w, int64_t n,
+                                   int64_t period) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float f = w[i % period];
+        y[i].x *= f;
+        y[i].y *= f;
+    }
+}
+
+static int grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+RcvrOp::RcvrOp(mm_ctx* c, int nslice, const int64_t* V, int64_t C, const float* maps) {
+    ctx = c;
+    if (nslice < 1 || C < 1) fail(MM_ERR_ARG, "rcvrOp: empty maps");
+    C_ = (int)C;
+    int64_t off = 0;
+    for (int s = 0; s < nslice; ++s) {
+        if (V[s] < 1) fail(MM_ERR_ARG, "rcvrOp: empty image");
+        Vs_.push_back(V[s]);
+        start_.push_back(off);
+        off += V[s];
+    }
+    V_total_ = off;
+    if (!maps) fail(MM_ERR_ARG, "rcvrOp: maps is NULL");
+    maps_.upload(reinterpret_cast<const float2*>(maps), (size_t)(off * C), ctx->stream);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void RcvrOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
+    hipStream_t st = ctx->stream;
+    for (size_t s = 0; s < Vs_.size(); ++s) {
+        const int64_t V = Vs_[s];
+        const float2* mp = maps_.p + start_[s] * C_;
+        const int64_t io = start_[s] * nrep, dof = start_[s] * C_ * nrep;
+        if (!adjoint) {
+            hipLaunchKernelGGL(rcvr_fwd_k, dim3(grid_for(V * C_ * nrep)), dim3(256), 0, st, in + io, mp,
+                               out + dof, V, C_, nrep);
+        } else {
+            hipLaunchKernelGGL(rcvr_adj_k, dim3(grid_for(V * nrep)), dim3(256), 0, st, in + dof, mp,
+                               out + io, V, C_, nrep);
+        }
+        LAUNCHCHECK();
+    }
+}
+
+SenseOp::SenseOp(mm_ctx* c, mm_op* S, mm_op* R, const float* Wh) : S_(S), R_(R) {
+    ctx = c;
+    if (!S || !R) fail(MM_ERR_ARG, "mrSampFunc: samplingOp and R are required");
+    auto* rc = dynamic_cast<RcvrOp*>(R);
+    if (!rc) fail(MM_ERR_ARG, "mrSampFunc: R must be a coil operator");
+    if (rc->n_image() != S->n_image() || rc->nslice() != S->nslice())
+        fail(MM_ERR_ARG, "mrSampFunc: image sizes of S and R differ");
+    for (int s = 0; s < S->nslice(); ++s)
+        if (rc->slice_image_off(s) != S->slice_image_off(s))
+            fail(MM_ERR_ARG, "mrSampFunc: slice layouts of S and R differ");
+    C_ = rc->coils();
+    nu_ = dynamic_cast<NufftOp*>(S);
+    if (nu_ && C_ > 16) nu_ = nullptr;  // unfused path
+    hio_ = dynamic_cast<HioOp*>(S);
+    if (Wh) {
+        Wd_.upload(Wh, (size_t)n_data(), ctx->stream);
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        W = Wd_.p;
+    }
+}
+
+void SenseOp::apply_impl(int adjoint, const float2* in, float2* out, int64_t nrep, const float* Wf) {
+    RcvrOp* rc = static_cast<RcvrOp*>(R_);
+    if (nu_) {
+        nu_->apply_sense(adjoint, in, out, nrep, rc->maps(), C_, Wf);
+        return;
+    }
+    if (hio_) {
+        hio_->apply_sense(adjoint, in, out, nrep, rc->maps(), C_, Wf);
+        return;
+    }
+    const int64_t nimgC = S_->n_image() * C_ * nrep;
+    tmp_.ensure((size_t)nimgC);
+    if (!adjoint) {
+        R_->apply(0, in, tmp_.p, nrep);
+        S_->apply(0, tmp_.p, out, nrep * C_);
+        if (Wf) {
+            hipLaunchKernelGGL(scale_by_weights_k, dim3(grid_for(n_data() * nrep)), dim3(256), 0,
+                               ctx->stream, out, Wf, n_data() * nrep, n_data());
+            LAUNCHCHECK();
+        }
+    } else {
+        S_->apply(1, in, tmp_.p, nrep * C_);
+        R_->apply(1, tmp_.p, out, nrep);
+    }
+}
+
+void SenseOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
+    apply_impl(adjoint, in, out, nrep, nullptr);
+}
+
+bool SenseOp::profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
+                               double* ms, std::string* name) {
+    RcvrOp* rc = static_cast<RcvrOp*>(R_);
+    if (hio_) return hio_->profile_sense(adjoint, in, out, nrep, rc->maps(), C_, iters, ms, name);
+    return false;
+}
+
+void SenseOp::apply_normal(const float2* in, float2* out, int64_t nrep, float2* tmp) {
+    apply_impl(0, in, tmp, nrep, W);
+    apply_impl(1, tmp, out, nrep, nullptr);
+}
+
+void mm_op_default_normal(mm_op* op, const float2* in, float2* out, int64_t nrep, float2* tmp) {
+    op->apply(0, in, tmp, nrep);
+    if (op->W) {
+        hipLaunchKernelGGL(scale_by_weights_k, dim3(grid_for(op->n_data() * nrep)), dim3(256), 0,
+                           op->ctx->stream, tmp, op->W, op->n_data() * nrep, op->n_data());
+        LAUNCHCHECK();
+    }
+    op->apply(1, tmp, out, nrep);
+}
+
+}  // namespace mm
+
+void mm_op::apply_normal(const float2* in, float2* out, int64_t nrep, float2* tmp) {
+    mm::mm_op_default_normal(this, in, out, nrep, tmp);
+}

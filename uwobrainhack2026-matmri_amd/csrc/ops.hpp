@@ -1,0 +1,55 @@
+// Coil operator and SENSE composer declarations.
+#pragma once
+#include "common.hpp"
+#include "fft.hpp"
+#include "hio.hpp"
+#include "nufft.hpp"
+
+namespace mm {
+
+class RcvrOp : public mm_op {
+   public:
+    RcvrOp(mm_ctx* ctx, int nslice, const int64_t* V, int64_t C, const float* maps);
+    int64_t n_image() const override { return V_total_; }
+    int64_t n_data() const override { return V_total_ * C_; }
+    int nslice() const override { return (int)Vs_.size(); }
+    int64_t slice_image_off(int s) const override { return start_[s]; }
+    int64_t slice_data_off(int s) const override { return start_[s] * C_; }
+    void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
+    int coils() const { return C_; }
+    const float2* maps() const { return maps_.p; }
+
+   private:
+    int C_ = 1;
+    std::vector<int64_t> Vs_, start_;
+    int64_t V_total_ = 0;
+    DevBuf<float2> maps_;
+};
+
+class SenseOp : public mm_op {
+   public:
+    SenseOp(mm_ctx* ctx, mm_op* S, mm_op* R, const float* W);
+    int64_t n_image() const override { return S_->n_image(); }
+    int64_t n_data() const override { return S_->n_data() * C_; }
+    int nslice() const override { return S_->nslice(); }
+    int64_t slice_image_off(int s) const override { return S_->slice_image_off(s); }
+    int64_t slice_data_off(int s) const override { return S_->slice_data_off(s) * C_; }
+    void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
+    void apply_normal(const float2* in, float2* out, int64_t nrep, float2* tmp) override;
+    bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
+                          double* ms, std::string* name) override;
+    mm_op* sampling() const { return S_; }
+    int coils() const { return C_; }
+
+   private:
+    void apply_impl(int adjoint, const float2* in, float2* out, int64_t nrep, const float* Wf);
+    mm_op* S_;
+    mm_op* R_;
+    int C_ = 1;
+    NufftOp* nu_ = nullptr;
+    HioOp* hio_ = nullptr;
+    DevBuf<float2> tmp_;
+    DevBuf<float> Wd_;
+};
+
+}  // namespace mm

@@ -1,0 +1,411 @@
+"""MATLAB-style operator classes over the gfx950 library.
+
+Mirrors the reference's class interfaces (paths relative to the MatMRI
+repository root):
+
+* :class:`nufftOp`       -- ``operators/@nufftOp/nufftOp.m`` (ctor ``:89``, ``mtimes`` ``:284``)
+* :class:`rcvrOp`        -- ``operators/@rcvrOp/rcvrOp.m`` (ctor ``:47``, ``mtimes`` ``:104``)
+* :class:`sampHighOrder` -- ``operators/@sampHighOrder/sampHighOrder.m`` (ctor ``:129``,
+  ``mtimes`` ``:293``, ``setPhiDiv`` ``:590``)
+* :func:`mrSampFunc`     -- ``operators/mrSampFunc.m``
+
+``A * x`` is ``mtimes``; ``A.H`` (or ``A.ctranspose()``) is MATLAB's ``A'``,
+a copy with the adjoint flag toggled that shares the device plan.
+
+Array conventions.  NumPy arrays are read and written with MATLAB's
+column-major meaning (``x.reshape(-1, order="F")``), so an image of size
+``imN`` is an array of shape ``imN + reps`` and the output lives on the host
+like a MATLAB CPU array.  Torch CUDA tensors (complex64) stay in HBM: their
+row-major shape is the column-major shape reversed, e.g. an image batch is
+``(reps, N2, N1)`` and data is ``(reps, C, M)``.
+"""
+import ctypes as C
+import weakref
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+
+try:  # torch is only needed for device-resident tensors
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+def _is_torch(x):
+    return torch is not None and isinstance(x, torch.Tensor)
+
+
+class _Plan:
+    """Owns one mm_op handle."""
+
+    def __init__(self, handle, keep=()):
+        self.h = handle
+        self._keep = keep  # plans this one references (kept alive)
+        self._fin = weakref.finalize(self, _destroy, handle.value)
+
+
+def _destroy(h):
+    try:
+        _lib.lib().mm_op_destroy(C.c_void_p(h))
+    except Exception:
+        pass
+
+
+class LinearOp:
+    """Common ``mtimes``/``ctranspose`` machinery."""
+
+    adjoint = 0
+
+    def _shapes(self):
+        ni, nd = C.c_int64(), C.c_int64()
+        check(_lib.lib().mm_op_shape(self._plan.h, C.byref(ni), C.byref(nd)))
+        return ni.value, nd.value
+
+    # MATLAB A' (ctranspose) and A.' (transpose, nufftOp.m:279-282)
+    def ctranspose(self):
+        c = object.__new__(type(self))
+        c.__dict__.update(self.__dict__)
+        c.adjoint = 1 - self.adjoint
+        return c
+
+    transpose = ctranspose
+
+    @property
+    def H(self):
+        return self.ctranspose()
+
+    def __mul__(self, x):
+        return self.mtimes(x)
+
+    def __matmul__(self, x):
+        return self.mtimes(x)
+
+    # shapes of one repetition on each side, column-major
+    def _in_shape(self):
+        return self.data_shape if self.adjoint else self.image_shape
+
+    def _out_shape(self):
+        return self.image_shape if self.adjoint else self.data_shape
+
+    def mtimes(self, x):
+        return _apply(self._plan.h, self.adjoint, x, self._in_shape(), self._out_shape(), self._shapes())
+
+    def normal(self, x):
+        """``A'*(A*x)`` in one call (the operator cgne iterates)."""
+        return _apply(self._plan.h, 2, x, self.image_shape, self.image_shape, self._shapes())
+
+
+def _apply(h, mode, x, in_shape, out_shape, shapes):
+    ni, nd = shapes
+    nin = nd if mode == 1 else ni
+    nout = ni if mode in (1, 2) else nd
+    L = _lib.lib()
+    if _is_torch(x):
+        if not x.is_cuda:
+            raise ValueError("torch tensors must be on the GPU")
+        xc = x.to(torch.complex64).contiguous()
+        if xc.numel() % nin:
+            raise ValueError(f"input has {xc.numel()} elements, not a multiple of {nin}")
+        nrep = xc.numel() // nin
+        y = torch.empty(nrep * nout, dtype=torch.complex64, device=x.device)
+        torch.cuda.current_stream(x.device).synchronize()
+        if mode == 2:
+            check(L.mm_op_apply_normal(h, ptr(xc), ptr(y), nrep, 1))
+        else:
+            check(L.mm_op_apply(h, mode, ptr(xc), ptr(y), nrep, 1))
+        check(L.mm_ctx_sync(_lib.context()))
+        lead = tuple(xc.shape[: xc.dim() - len(in_shape)]) if xc.dim() >= len(in_shape) else ()
+        shp = lead + tuple(reversed(out_shape)) if lead and int(np.prod(lead)) == nrep else \
+            ((nrep,) if nrep > 1 else ()) + tuple(reversed(out_shape))
+        return y.reshape(shp)
+    xa = np.asarray(x)
+    in_dtype = xa.dtype
+    xf = np.asfortranarray(xa).reshape(-1, order="F").astype(np.complex64)
+    if xf.size % nin:
+        raise ValueError(f"input has {xf.size} elements, not a multiple of {nin}")
+    nrep = xf.size // nin
+    y = np.empty(nrep * nout, dtype=np.complex64)
+    if mode == 2:
+        check(L.mm_op_apply_normal(h, xf.ctypes.data, y.ctypes.data, nrep, 0))
+    else:
+        check(L.mm_op_apply(h, mode, xf.ctypes.data, y.ctypes.data, nrep, 0))
+    reps = xa.shape[len(in_shape):] if xa.ndim >= len(in_shape) and \
+        int(np.prod(xa.shape[:len(in_shape)])) == nin else ((nrep,) if nrep > 1 else ())
+    y = y.reshape(tuple(out_shape) + tuple(reps), order="F")
+    # revert the class like nufftOp.m:404-406
+    if in_dtype == np.complex128 or in_dtype == np.float64:
+        y = y.astype(np.complex128)
+    return y
+
+
+class nufftOp(LinearOp):
+    """``nufftOp(imN, kloc, dcf, useGPU, os, kwidth)`` (nufftOp.m:89-136).
+
+    ``kloc`` is ``[M x nD]`` in cycles/pixel, ``[-0.5, 0.5)`` (wrapped).
+    ``kwidth`` defaults to 6 and ``os`` to 1.5 like the reference properties
+    (nufftOp.m:65-66).  Only the GPU path exists (``useGPU`` is accepted and
+    ignored); 1-D and 2-D images are implemented.
+    """
+
+    def __init__(self, imN, kloc, dcf=None, useGPU=1, os=1.5, kwidth=6):
+        imN = [int(n) for n in np.atleast_1d(imN)]
+        if len(imN) == 2 and imN[1] == 1:
+            imN = imN[:1]
+        nd = len(imN)
+        kloc = np.asarray(kloc, dtype=np.float64)
+        if kloc.ndim == 1:
+            kloc = kloc.reshape(-1, 1)
+        if kloc.shape[1] != nd:
+            raise ValueError("kloc must be [M x numel(imN)]")
+        self._build([(imN, kloc, dcf)], os, kwidth)
+
+    @classmethod
+    def batch(cls, slices, os=1.5, kwidth=6):
+        """Multi-slice plan: ``slices`` = list of ``(imN, kloc, dcf)``; data of
+        all slices is concatenated (images then samples, per slice)."""
+        obj = object.__new__(cls)
+        obj._build([([int(n) for n in np.atleast_1d(s[0])], np.asarray(s[1], np.float64),
+                     s[2] if len(s) > 2 else None) for s in slices], os, kwidth)
+        return obj
+
+    def _build(self, slices, os, kwidth):
+        nd = len(slices[0][0])
+        ns = len(slices)
+        imN = np.zeros(2 * ns, dtype=np.int64)
+        Ms = np.zeros(ns, dtype=np.int64)
+        self._kl = []
+        self._dc = []
+        for s, (n, kl, dc) in enumerate(slices):
+            imN[2 * s] = n[0]
+            imN[2 * s + 1] = n[1] if nd == 2 else 1
+            kl = np.asfortranarray(np.asarray(kl, np.float64).reshape(-1, nd))
+            Ms[s] = kl.shape[0]
+            self._kl.append(kl)
+            self._dc.append(None if dc is None else np.ascontiguousarray(np.asarray(dc, np.float64).reshape(-1)))
+        kp = (C.c_void_p * ns)(*[k.ctypes.data for k in self._kl])
+        dp = (C.c_void_p * ns)(*[(0 if d is None else d.ctypes.data) for d in self._dc]) \
+            if any(d is not None for d in self._dc) else None
+        h = C.c_void_p()
+        check(_lib.lib().mm_nufft_plan_create(
+            _lib.context(), nd, ns, imN.ctypes.data_as(C.POINTER(C.c_int64)),
+            Ms.ctypes.data_as(C.POINTER(C.c_int64)), kp, dp, float(os), int(kwidth), C.byref(h)))
+        self._plan = _Plan(h)
+        self.nslice = ns
+        self.imN = [list(s[0]) for s in slices]
+        self.M = [int(m) for m in Ms]
+        self.os = os
+        self.kwidth = kwidth
+        if ns == 1:
+            self.image_shape = tuple(self.imN[0])
+            self.data_shape = (self.M[0],)
+        else:
+            self.image_shape = (int(sum(np.prod(n) for n in self.imN)),)
+            self.data_shape = (int(sum(self.M)),)
+
+
+class rcvrOp(LinearOp):
+    """``rcvrOp(maps, isCalDat=false)`` (rcvrOp.m:47-94): ``maps`` has shape
+    ``imN + [C]``.  The calibration path (``isCalDat=true``) is not on the
+    accelerated path."""
+
+    def __init__(self, maps, isCalDat=False, apN=None, kalph=None):
+        if isCalDat:
+            raise NotImplementedError("rcvrOp calibration from low-resolution data is not implemented")
+        maps = np.asarray(maps)
+        self.imN = maps.shape[:-1]
+        self.C = maps.shape[-1]
+        V = int(np.prod(self.imN))
+        self._maps = np.asfortranarray(maps.astype(np.complex64)).reshape(-1, order="F")
+        h = C.c_void_p()
+        check(_lib.lib().mm_rcvr_plan_create(_lib.context(), V, self.C, self._maps.ctypes.data, C.byref(h)))
+        self._plan = _Plan(h)
+        self.image_shape = tuple(self.imN)
+        self.data_shape = tuple(self.imN) + (self.C,)
+        self.nslice = 1
+
+    @classmethod
+    def batch(cls, maps_list):
+        """Per-slice maps (each ``imN_s + [C]``, same C) for a multi-slice SENSE operator."""
+        obj = object.__new__(cls)
+        Cc = maps_list[0].shape[-1]
+        Vs = np.array([int(np.prod(m.shape[:-1])) for m in maps_list], dtype=np.int64)
+        obj._maps = np.concatenate([np.asfortranarray(np.asarray(m).astype(np.complex64)).reshape(-1, order="F")
+                                    for m in maps_list])
+        h = C.c_void_p()
+        check(_lib.lib().mm_rcvr_plan_create_multi(_lib.context(), len(maps_list),
+                                                   Vs.ctypes.data_as(C.POINTER(C.c_int64)), Cc,
+                                                   obj._maps.ctypes.data, C.byref(h)))
+        obj._plan = _Plan(h)
+        obj.C = Cc
+        obj.nslice = len(maps_list)
+        obj.image_shape = (int(Vs.sum()),)
+        obj.data_shape = (int(Vs.sum()) * Cc,)
+        return obj
+
+
+class sampHighOrder(LinearOp):
+    """``sampHighOrder(b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask,
+    useGPU, useSingle, approach, opt)`` (sampHighOrder.m:129-291).
+
+    ``b0`` is in rad/s with the image shape (or ``None``); ``sampTimes`` in s
+    with the k-space shape; ``phs_spha`` is ``[Nh, kshape]`` and
+    ``phs_conc`` ``[Nc, kshape]`` or ``None``; ``phs_grid`` a dict of
+    ``x, y, z`` position arrays (m) with the image shape.  Approaches 0, 2
+    and 3 give the same values and share one kernel; approach 1
+    (interpolation) is not implemented.
+    """
+
+    def __init__(self, b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask=None, useGPU=1,
+                 useSingle=0, approach=0, opt=None):
+        if approach == 1:
+            raise NotImplementedError("sampHighOrder approach 1 (interpolated) is not implemented")
+        gx = np.asarray(phs_grid["x"], np.float64)
+        self.imSize = gx.shape
+        V = gx.size
+        t = np.asarray(sampTimes, np.float64)
+        self.kSize = t.shape if t.ndim else (1,)
+        K = t.size
+        hs = np.asarray(phs_spha, np.float64)
+        Lh = hs.shape[0]
+        hs = np.ascontiguousarray(hs.reshape(Lh, -1, order="F").T)  # [K][Lh]
+        if hs.shape[0] != K:
+            raise ValueError("Dimension mismatch between phs_spha and sampTimes")
+        if phs_conc is None or np.size(phs_conc) == 0:
+            Lc, hc = 0, None
+        else:
+            hc = np.asarray(phs_conc, np.float64)
+            Lc = hc.shape[0]
+            hc = np.ascontiguousarray(hc.reshape(Lc, -1, order="F").T)
+            if hc.shape[0] != K:
+                raise ValueError("Dimension mismatch between phs_conc and sampTimes")
+        f = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1, order="F"))
+        if b0 is not None and np.size(b0) and np.shape(b0) != self.imSize:
+            raise ValueError("Size mismatch between b0 and phs_grid.x")
+        self._arr = [f(phs_grid["x"]), f(phs_grid["y"]), f(phs_grid["z"]),
+                     f(b0) if b0 is not None and np.size(b0) else None, f(t), hs, hc,
+                     f(b0mask) if b0mask is not None and np.size(b0mask) else None]
+        a = self._arr
+        h = C.c_void_p()
+        check(_lib.lib().mm_hio_plan_create(_lib.context(), V, K, Lh, Lc, ptr(a[0]), ptr(a[1]), ptr(a[2]),
+                                            ptr(a[3]), ptr(a[4]), ptr(a[5]), ptr(a[6]), ptr(a[7]),
+                                            C.byref(h)))
+        self._plan = _Plan(h)
+        self.Lh, self.Lc = Lh, Lc
+        self.image_shape = tuple(self.imSize)
+        self.data_shape = tuple(self.kSize)
+        self.nslice = 1
+
+    def setPhiDiv(self, ksphaDiv, kconcDiv=None):
+        """``setPhiDiv`` (sampHighOrder.m:590-622): the forward becomes
+        ``(1i*exp(1i*Phi).*dPhi).' * x / sqrt(V)``.  Returns the operator,
+        like the reference's value-class mutator."""
+        hd = np.ascontiguousarray(np.asarray(ksphaDiv, np.float64).reshape(self.Lh, -1, order="F").T)
+        gd = None
+        if self.Lc:
+            gd = np.ascontiguousarray(np.asarray(kconcDiv, np.float64).reshape(self.Lc, -1, order="F").T)
+        check(_lib.lib().mm_hio_set_phi_div(self._plan.h, ptr(hd), ptr(gd)))
+        return self
+
+
+class senseOp(LinearOp):
+    """E = S o R (mrSampFunc.m:97-107; adjoint :126-144) as one device
+    operator; W (data weights) is used by :meth:`normal` and by cgne."""
+
+    def __init__(self, S, R, W=None):
+        self.S, self.R = S, R
+        self._W = None if W is None else np.ascontiguousarray(
+            np.asarray(W, np.float32).reshape(-1, order="F"))
+        h = C.c_void_p()
+        check(_lib.lib().mm_sense_create(_lib.context(), S._plan.h, R._plan.h, ptr(self._W), C.byref(h)))
+        self._plan = _Plan(h, keep=(S._plan, R._plan))
+        self.nslice = getattr(S, "nslice", 1)
+        self.image_shape = S.image_shape
+        ni, nd = self._shapes()
+        if self.nslice == 1:
+            self.data_shape = tuple(S.data_shape) + (R.C,)
+        else:
+            self.data_shape = (nd,)
+
+
+_sense_cache = {}
+
+
+def _sense_for(S, R, W=None):
+    key = (id(S._plan), id(R._plan), None if W is None else id(W))
+    e = _sense_cache.get(key)
+    if e is None or e[1]() is None or e[2]() is None:
+        op = senseOp(S, R, W)
+        _sense_cache[key] = (op, weakref.ref(S._plan), weakref.ref(R._plan))
+        return op
+    return e[0]
+
+
+def mrSampFunc(x, transp, samplingOp, R=None, opt=None):
+    """``mrSampFunc(x, transp, samplingOp, R, opt)`` (mrSampFunc.m:1-161) for
+    device sampling operators.  Supported options: ``imNFull``/``daNFull``
+    (vectorisation), ``W``, ``tikReg``, ``hardReal``.  ``sumDims`` and
+    Cartesian masks are not part of the accelerated path."""
+    opt = dict(opt or {})
+    if not isinstance(samplingOp, LinearOp):
+        raise NotImplementedError("Cartesian-mask sampling is not implemented on the device path")
+    if opt.get("sumDims") is not None:
+        raise NotImplementedError("sumDims is not implemented on the device path")
+    imNFull, daNFull = opt.get("imNFull"), opt.get("daNFull")
+    tikReg, W = opt.get("tikReg"), opt.get("W")
+    hardReal = opt.get("hardReal", 0) or 0
+    if tikReg is not None and (imNFull is None or daNFull is None):
+        raise ValueError("Tikhonov regularization requires opt.imNFull and opt.daNFull")
+    E = _sense_for(samplingOp, R) if R is not None else samplingOp
+    xp = torch if _is_torch(x) else np
+    if transp == "notransp":
+        if hardReal:
+            x = xp.real(x) + 0j if xp is np else torch.real(x).to(torch.complex64)
+        if tikReg is not None:
+            t = np.atleast_1d(tikReg)
+            x_tik = t[0] * x.real + 1j * t[1] * x.imag if t.size == 2 else t[0] * x
+        if imNFull is not None and xp is np:
+            x = np.reshape(x, tuple(imNFull), order="F")
+        y = E.mtimes(x)
+        if daNFull is not None:
+            y = y.reshape(-1, order="F") if xp is np else y.reshape(-1)
+        if tikReg is not None:
+            y = xp.concatenate([y.reshape(-1, order="F") if xp is np else y.reshape(-1),
+                                x_tik.reshape(-1, order="F") if xp is np else x_tik.reshape(-1)]) \
+                if xp is np else torch.cat([y.reshape(-1), x_tik.reshape(-1)])
+        return y
+    if transp == "transp":
+        if tikReg is not None:
+            nda = int(np.prod(daNFull))
+            x_tik = x[nda:]
+            x = x[:nda]
+        if W is not None:
+            x = x * (np.asarray(W) if xp is np else torch.as_tensor(np.asarray(W), device=x.device))
+        if daNFull is not None and xp is np:
+            x = np.reshape(x, tuple(daNFull), order="F")
+        out = E.H.mtimes(x)
+        if imNFull is not None:
+            out = out.reshape(-1, order="F") if xp is np else out.reshape(-1)
+        if tikReg is not None:
+            t = np.atleast_1d(tikReg)
+            out = out + (t[0] * x_tik.real + 1j * t[1] * x_tik.imag if t.size == 2 else t[0] * x_tik)
+        if hardReal:
+            out = out.real + 0j if xp is np else torch.real(out).to(torch.complex64)
+        return out
+    raise ValueError("transp must be 'notransp' or 'transp'")
+
+
+class mrSampHandle:
+    """The handle ``@(x, transp) mrSampFunc(x, transp, S, R, opt)`` the demos
+    pass to cgne (demo_regSENSE.m:89-99).  cgne recognises it and runs the
+    whole solve on the device."""
+
+    def __init__(self, S, R, opt=None):
+        self.S, self.R, self.opt = S, R, dict(opt or {})
+
+    def __call__(self, x, transp):
+        return mrSampFunc(x, transp, self.S, self.R, self.opt)
+
+    def sense(self, W=None):
+        return _sense_for(self.S, self.R, W)
