@@ -146,6 +146,10 @@ int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax,
  * launch and the kernel's name. */
 int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nrep, int iters,
                   double* ms_per_launch, char* kernel_name, int name_len);
+/* Tuning knob (no counterpart in the reference): "hio_variant" (0..3: bit 0
+ * phase on FP64 MFMA, bit 1 hardware sine/cosine) and "hio_mfma" (0/1) of a
+ * high-order operator or a SENSE operator built on one. */
+int mm_op_set_option(mm_op* op, const char* key, int value);
 /* The hipStream_t the context runs on. */
 void* mm_ctx_stream(mm_ctx* ctx);
 

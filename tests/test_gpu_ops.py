@@ -157,11 +157,13 @@ def _hio_plans(mm, rng, N=12, K=300, conc=True, mask=False, Lh=16):
     return So, Sg
 
 
-@pytest.mark.parametrize("conc,mask,Lh", [(False, False, 16), (True, True, 16), (True, False, 36)])
-def test_highorder_vs_oracle(mm, conc, mask, Lh):
-    print(f"[case] highorder conc={conc} mask={mask} Lh={Lh}", flush=True)
+@pytest.mark.parametrize("conc,mask,Lh,variant", [(False, False, 16, 0), (True, True, 16, 0), (True, True, 16, 3),
+                                                  (True, False, 36, 0)])
+def test_highorder_vs_oracle(mm, conc, mask, Lh, variant):
+    print(f"[case] highorder conc={conc} mask={mask} Lh={Lh} variant={variant}", flush=True)
     rng = np.random.default_rng(21)
     So, Sg = _hio_plans(mm, rng, conc=conc, mask=mask, Lh=Lh)
+    Sg.set_option("hio_variant", variant)
     x = crand(rng, (12, 12, 3))
     y = crand(rng, (300, 3))
     e_f, e_a = rel(Sg * x, So.forward(x)), rel(Sg.H * y, So.adjoint(y))
@@ -190,6 +192,35 @@ def test_highorder_sense_vs_oracle(mm):
     E = mm.senseOp(Sg, Rg)
     x = crand(rng, (16, 16))
     y = crand(rng, (400, 6))
+    e_f = rel(E * x, o_mrSampFunc(x, "notransp", So, Ro))
+    e_a = rel(E.H * y, o_mrSampFunc(y, "transp", So, Ro))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+
+
+@pytest.mark.parametrize("f0,variant", [(0.0, 0), (70e3, 0), (70e3, 1), (70e3, 2), (70e3, 3)])
+def test_highorder_full_grid_precision(mm, f0, variant):
+    """220x220 grid, 32 coils, 16 harmonics + B0 of the high-order workload on
+    1200 of its 24000 time points (spread over the whole readout, so the late
+    samples with the largest phase are included); with the 70 kHz zeroth-order
+    term max|Phi| exceeds 1e4 rad.  rel-L2 <= 1e-5 against the FP64 oracle."""
+    print(f"[case] highorder 220^2 C=32 K=1200/24000 f0={f0} variant={variant}", flush=True)
+    P = mm.workloads.highorder_problem(seed=2, f0_hz=f0)
+    sel = np.linspace(0, P["K"] - 1, 1200).round().astype(int)
+    t, h = P["t"][sel], P["h"][:, sel]
+    So = HighOrderPlan(P["b0"], t, h, None, P["grid"])
+    ph = So.phase(tsl=slice(len(sel) - 8, len(sel)))
+    maxphi = np.max(np.abs(ph))
+    print(f"  max|Phi| over the last samples {maxphi:.3e} rad", flush=True)
+    if f0:
+        assert maxphi > 1e4
+    Sg = mm.sampHighOrder(P["b0"], t, h, None, P["grid"])
+    Sg.set_option("hio_variant", variant)
+    Ro, Rg = RcvrPlan(P["maps"]), mm.rcvrOp(P["maps"])
+    E = mm.senseOp(Sg, Rg)
+    rng = np.random.default_rng(3)
+    x = crand(rng, (220, 220))
+    y = crand(rng, (1200, 32))
     e_f = rel(E * x, o_mrSampFunc(x, "notransp", So, Ro))
     e_a = rel(E.H * y, o_mrSampFunc(y, "transp", So, Ro))
     print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)

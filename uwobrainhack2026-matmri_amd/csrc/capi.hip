@@ -277,6 +277,15 @@ int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nre
     });
 }
 
+int mm_op_set_option(mm_op* op, const char* key, int value) {
+    return guarded([&] {
+        if (!op || !key) fail(MM_ERR_ARG, "set_option: NULL argument");
+        if (!op->set_option(key, value)) fail(MM_ERR_ARG, std::string("set_option: unknown key ") + key);
+        // captured solver graphs hold the old kernels
+        solvers().erase(op);
+    });
+}
+
 void* mm_ctx_stream(mm_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int mm_op_destroy(mm_op* op) {

@@ -104,6 +104,8 @@ struct mm_op {
     // optional: accumulate per-slice partial FP64 dot products of
     // conj(in) . out into `dot` (nslice x 2 doubles, already zeroed) during
     // apply_normal.  Returns false if the operator cannot fuse it.
+    // Tuning knobs (kernel variants); false if the key is unknown.
+    virtual bool set_option(const std::string& key, int value) { (void)key; (void)value; return false; }
     // Time the operator's dominant kernel: `iters` back-to-back launches on
     // the context stream between two HIP events.  Returns ms per launch and
     // the kernel's name; false if the operator has no single dominant kernel.

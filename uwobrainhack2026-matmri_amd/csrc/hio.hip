@@ -23,6 +23,7 @@
 
 #include "fft.hpp"
 #include "hio.hpp"
+#include "hio_mfma.hpp"
 
 namespace mm {
 
@@ -167,18 +168,22 @@ HioOp::HioOp(mm_ctx* c, int64_t V, int64_t K, int Lh, int Lc, const double* x, c
     }
     r_ = (int)keep.size();
     if (r_ == 0) r_ = 1;  // all-zero phase: one zero column
-    std::vector<double> Qc((size_t)V * r_, 0.0), Hc((size_t)K * r_, 0.0);
+    rp_ = ((r_ + 3) / 4) * 4;
+    Vp_ = (V + 15) / 16 * 16;
+    Kp_ = (K + 15) / 16 * 16;
+    std::vector<double> Qc((size_t)Vp_ * rp_, 0.0), Hc((size_t)Kp_ * rp_, 0.0);
     for (int64_t v = 0; v < V; ++v)
-        for (int j = 0; j < (int)keep.size(); ++j) Qc[(size_t)v * r_ + j] = Q[(size_t)v * L_ + j];
+        for (int j = 0; j < (int)keep.size(); ++j) Qc[(size_t)v * rp_ + j] = Q[(size_t)v * L_ + j];
     const double inv2pi = 1.0 / (2.0 * M_PI);
     for (int64_t k = 0; k < K; ++k)
         for (int j = 0; j < (int)keep.size(); ++j) {
             double s = 0;
             for (int l = 0; l < L_; ++l) s += R[(size_t)j * L_ + l] * H[(size_t)l * K + k];
-            Hc[(size_t)k * r_ + j] = s * inv2pi;
+            Hc[(size_t)k * rp_ + j] = s * inv2pi;
         }
     Q_.upload(Qc, ctx->stream);
     Hc_.upload(Hc, ctx->stream);
+    amax_.alloc(1);
     HIPCHECK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -434,9 +439,13 @@ static double time_launches(hipStream_t st, int iters, F&& launch) {
 
 void HioOp::run(int adjoint, const float2* in, float2* out, int64_t ncol64, const float2* maps, int C,
                 const float* W, int prof_iters, double* prof_ms) {
-    if (r_ > kHioMaxRank) fail(MM_ERR_UNSUPPORTED, "sampHighOrder: phase rank above 24");
+    if (rp_ > kHioMaxRank) fail(MM_ERR_UNSUPPORTED, "sampHighOrder: phase rank above 24");
     hipStream_t st = ctx->stream;
     const int ncol = (int)ncol64;
+    if (!phidiv_ && rp_ <= 16 && (!maps || C <= 32) && use_mfma_) {
+        run_mfma(adjoint, in, out, ncol, maps, C, W, prof_iters, prof_ms);
+        return;
+    }
     const float scale = (float)(1.0 / std::sqrt((double)V_));
     const int NC = 32;
     const int ncg = (ncol + NC - 1) / NC;
@@ -452,11 +461,11 @@ void HioOp::run(int adjoint, const float2* in, float2* out, int64_t ncol64, cons
         auto main = [&] {
             if (phidiv_)
                 hipLaunchKernelGGL((hio_fwd_simple<NC, true>), dim3((unsigned)kb, nsplit, ncg), dim3(256), 0,
-                                   st, Q_.p, Hc_.p, r_, V_, K_, X_.p, ncol, part_.p, vchunk, Bnb_.p, Hd_.p,
+                                   st, Q_.p, Hc_.p, rp_, V_, K_, X_.p, ncol, part_.p, vchunk, Bnb_.p, Hd_.p,
                                    Lnb_);
             else
                 hipLaunchKernelGGL((hio_fwd_simple<NC, false>), dim3((unsigned)kb, nsplit, ncg), dim3(256), 0,
-                                   st, Q_.p, Hc_.p, r_, V_, K_, X_.p, ncol, part_.p, vchunk,
+                                   st, Q_.p, Hc_.p, rp_, V_, K_, X_.p, ncol, part_.p, vchunk,
                                    (const double*)nullptr, (const double*)nullptr, 0);
         };
         if (prof_iters > 0) *prof_ms = time_launches(st, prof_iters, main);
@@ -471,7 +480,7 @@ void HioOp::run(int adjoint, const float2* in, float2* out, int64_t ncol64, cons
         part_.ensure((size_t)nsplit * V_ * ncol);
         auto main = [&] {
             hipLaunchKernelGGL(hio_adj_simple<NC>, dim3((unsigned)vb, nsplit, ncg), dim3(256), 0, st, Q_.p,
-                               Hc_.p, r_, V_, K_, in, ncol, part_.p, kchunk);
+                               Hc_.p, rp_, V_, K_, in, ncol, part_.p, kchunk);
         };
         if (prof_iters > 0) *prof_ms = time_launches(st, prof_iters, main);
         else main();
@@ -481,17 +490,49 @@ void HioOp::run(int adjoint, const float2* in, float2* out, int64_t ncol64, cons
     LAUNCHCHECK();
 }
 
+void HioOp::run_mfma(int adjoint, const float2* in, float2* out, int ncol, const float2* maps, int C,
+                     const float* W, int prof_iters, double* prof_ms) {
+    hipStream_t st = ctx->stream;
+    HioMfmaCall c{};
+    c.rp = rp_;
+    c.nrows = (int)(adjoint ? V_ : K_);
+    c.nred = adjoint ? K_ : V_;
+    c.nblk = (int)((c.nred + 15) / 16);
+    c.rowfac = adjoint ? Q_.p : Hc_.p;
+    c.redfac = adjoint ? Hc_.p : Q_.p;
+    c.mode = adjoint ? 2 : (maps ? 0 : 1);
+    c.in = in;
+    c.maps = maps;
+    c.C = C;
+    c.post = (float)(1.0 / std::sqrt((double)V_));
+    c.W = adjoint ? nullptr : W;
+    c.out = out;
+    c.amax = amax_.p;
+    c.variant = variant_;
+    const int gwmax = maps ? C * (32 / C) : 32;
+    hio_mfma_split(c.nrows, c.nblk, &c.nsplit, &c.bps);
+    Bs_.ensure(hio_mfma_bs_bytes(c.nblk, std::min(gwmax, ncol)));
+    mpart_.ensure((size_t)c.nsplit * c.nrows * 64);
+    c.Bs = Bs_.p;
+    c.part = mpart_.p;
+    for (int col0 = 0; col0 < ncol; col0 += gwmax) {
+        c.col0 = col0;
+        c.gw = std::min(gwmax, ncol - col0);
+        hio_mfma_apply(st, c, col0 == 0 ? prof_iters : 0, prof_ms);
+    }
+}
+
 bool HioOp::profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
                              double* ms, std::string* name) {
     run(adjoint, in, out, nrep, nullptr, 1, nullptr, iters, ms);
-    *name = adjoint ? "hio_adj_simple" : "hio_fwd_simple";
+    *name = kernel_name(adjoint);
     return true;
 }
 
 bool HioOp::profile_sense(int adjoint, const float2* in, float2* out, int64_t nrep, const float2* maps,
                           int C, int iters, double* ms, std::string* name) {
     run(adjoint, in, out, nrep * C, maps, C, nullptr, iters, ms);
-    *name = adjoint ? "hio_adj_simple" : "hio_fwd_simple";
+    *name = kernel_name(adjoint);
     return true;
 }
 

@@ -19,6 +19,11 @@ class HioOp : public mm_op {
     void set_phi_div(const double* hd, const double* gd);
     bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
                           double* ms, std::string* name) override;
+    bool set_option(const std::string& key, int value) override {
+        if (key == "hio_variant") { variant_ = value & 3; return true; }
+        if (key == "hio_mfma") { use_mfma_ = value != 0; return true; }
+        return false;
+    }
     bool profile_sense(int adjoint, const float2* in, float2* out, int64_t nrep, const float2* maps,
                        int C, int iters, double* ms, std::string* name);
     int phase_terms() const { return L_; }
@@ -26,12 +31,24 @@ class HioOp : public mm_op {
     // max |Phi| in radians over the plan (host FP64, computed at creation)
     double max_abs_phase() const { return maxphi_; }
 
+    // the MFMA path is used when the padded rank is <= 16 and there is no phiDiv
+    bool use_mfma_ = true;
+    int variant_ = 3;  // MFMA kernel variant (see HioMfmaCall::variant)
+    const char* kernel_name(int adjoint) const {
+        if (!phidiv_ && rp_ <= 16 && use_mfma_) return "hio_mfma_gemm";
+        return adjoint ? "hio_adj_simple" : "hio_fwd_simple";
+    }
+
    private:
+    void run_mfma(int adjoint, const float2* in, float2* out, int ncol, const float2* maps, int C,
+                  const float* W, int prof_iters, double* prof_ms);
     void run(int adjoint, const float2* in, float2* out, int64_t ncol, const float2* maps, int C,
              const float* W, int prof_iters = 0, double* prof_ms = nullptr);
     int64_t V_, K_;
     int L_ = 0;      // phase terms before compression
     int r_ = 0;      // rank after compression
+    int rp_ = 4;     // rank padded to a multiple of 4 (row stride of Q_/Hc_)
+    int64_t Vp_ = 0, Kp_ = 0;   // row counts padded to a multiple of 16
     int Lnb_ = 0;    // terms without the B0 column (phiDiv basis)
     int Lh_ = 0, Lc_ = 0;
     double maxphi_ = 0;
@@ -42,6 +59,9 @@ class HioOp : public mm_op {
     std::vector<double> Bnb_host_;   // [V][Lnb]
     DevBuf<double> Bnb_, Hd_;        // [V][Lnb], [K][Lnb]
     DevBuf<float2> X_, part_;
+    DevBuf<unsigned char> Bs_;
+    DevBuf<float> mpart_;
+    DevBuf<unsigned> amax_;
 };
 
 }  // namespace mm

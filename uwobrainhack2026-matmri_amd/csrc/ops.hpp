@@ -38,6 +38,7 @@ class SenseOp : public mm_op {
     void apply_normal(const float2* in, float2* out, int64_t nrep, float2* tmp) override;
     bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
                           double* ms, std::string* name) override;
+    bool set_option(const std::string& key, int value) override { return S_->set_option(key, value); }
     mm_op* sampling() const { return S_; }
     int coils() const { return C_; }
 

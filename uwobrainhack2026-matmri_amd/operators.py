@@ -92,6 +92,11 @@ class LinearOp:
     def mtimes(self, x):
         return _apply(self._plan.h, self.adjoint, x, self._in_shape(), self._out_shape(), self._shapes())
 
+    def set_option(self, key, value):
+        """Kernel tuning knob (see mm_op_set_option)."""
+        check(_lib.lib().mm_op_set_option(self._plan.h, key.encode(), int(value)))
+        return self
+
     def normal(self, x):
         """``A'*(A*x)`` in one call (the operator cgne iterates)."""
         return _apply(self._plan.h, 2, x, self.image_shape, self.image_shape, self._shapes())
