@@ -108,19 +108,25 @@ __host__ __device__ inline void dft_small(float2 (&v)[R], float sgn) {
     }
 }
 
-// One Stockham pass over T interleaved transforms (T = 1 << log2T).
-// tw[t] = exp(+2*pi*i*t/n), t < n.
+// i % p for i < 512 via a 20-bit fixed-point reciprocal pinv = ceil(2^20/p)
+// (the rounding error i*(pinv - 2^20/p)/2^20 < 512/2^20 stays below 1/p).
+__host__ __device__ inline int mod_small(int i, int p, unsigned pinv) {
+    const int q = (int)(((unsigned)i * pinv) >> 20);
+    return i - q * p;
+}
+
+// One butterfly of a Stockham pass over T interleaved transforms
+// (T = 1 << log2T); tw[t] = exp(+2*pi*i*t/n), t < n; step = n/(p*R).
 template <int R>
 __host__ __device__ inline void stockham_butterfly(const float2* __restrict__ src,
                                                    float2* __restrict__ dst, int n, int log2T,
-                                                   int p, const float2* __restrict__ tw, float sgn,
-                                                   int w) {
+                                                   int p, unsigned pinv, int step,
+                                                   const float2* __restrict__ tw, float sgn, int w) {
     const int T = 1 << log2T;
     const int nb = n / R;
     const int t = w & (T - 1);
     const int i = w >> log2T;
-    const int k = i % p;
-    const int step = n / (p * R);
+    const int k = mod_small(i, p, pinv);
     float2 v[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = src[(i + j * nb) * T + t];
@@ -147,13 +153,15 @@ __device__ inline float2* fft_lds(float2* a, float2* b, const FftFactors& f, int
     for (int s = 0; s < f.nf; ++s) {
         const int R = f.r[s];
         const int total = (n / R) * T;
+        const unsigned pinv = ((1u << 20) + (unsigned)p - 1) / (unsigned)p;
+        const int step = n / (p * R);
         for (int w = threadIdx.x; w < total; w += blockDim.x) {
             switch (R) {
-                case 4: stockham_butterfly<4>(a, b, n, log2T, p, tw, sgn, w); break;
-                case 2: stockham_butterfly<2>(a, b, n, log2T, p, tw, sgn, w); break;
-                case 3: stockham_butterfly<3>(a, b, n, log2T, p, tw, sgn, w); break;
-                case 5: stockham_butterfly<5>(a, b, n, log2T, p, tw, sgn, w); break;
-                default: stockham_butterfly<7>(a, b, n, log2T, p, tw, sgn, w); break;
+                case 4: stockham_butterfly<4>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 2: stockham_butterfly<2>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 3: stockham_butterfly<3>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 5: stockham_butterfly<5>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                default: stockham_butterfly<7>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
             }
         }
         __syncthreads();

@@ -133,6 +133,7 @@ __host__ __device__ static inline int ilog2(int v) {
 // ---------------------------------------------------------------------------
 // device code
 struct NuRun {
+    int ylayout;        // 1: data side in the transposed [M][BCT] layout (yT)
     int sense;          // 1: image side is one image combined with coil maps
     int C;              // coils (sense) -- lanes per chunk
     int64_t B;          // transforms per slice (plain)
@@ -273,8 +274,27 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     }
 }
 
-template <int BCT>
-__global__ __launch_bounds__(256) void nu_degrid(const NuSliceDev* __restrict__ sl,
+// Kernel weights of one sample: W1 values for dim 1, W2 for dim 2.  With a
+// compile-time width (WT > 0, 2-D) they come in as 16-byte vector loads.
+template <int WT>
+__device__ __forceinline__ void load_weights(const float* __restrict__ wt, int W1, int W2, float* w1,
+                                             float* w2) {
+    if constexpr (WT == 4) {
+        const float4 a = *reinterpret_cast<const float4*>(wt);
+        const float4 b = *reinterpret_cast<const float4*>(wt + 4);
+        w1[0] = a.x; w1[1] = a.y; w1[2] = a.z; w1[3] = a.w;
+        w2[0] = b.x; w2[1] = b.y; w2[2] = b.z; w2[3] = b.w;
+    } else {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            w1[s] = s < W1 ? wt[s] : 0.f;
+            w2[s] = s < W2 ? wt[W1 + s] : 0.f;
+        }
+    }
+}
+
+template <int BCT, int WT>
+__global__ __launch_bounds__(256, 2) void nu_degrid(const NuSliceDev* __restrict__ sl,
                                                  const int4* __restrict__ work,
                                                  const uint32_t* __restrict__ sbase,
                                                  const float* __restrict__ swt,
@@ -286,26 +306,26 @@ __global__ __launch_bounds__(256) void nu_degrid(const NuSliceDev* __restrict__ 
     const int ch = wk.y;
     const int m = wk.z + threadIdx.x;
     if (m >= S.M) return;
-    const int G1 = S.G1, G2 = S.G2, W1 = S.W1, W2 = S.W2;
+    const int G1 = S.G1, G2 = S.G2;
+    const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2;
     const int64_t sm = S.samp_off + m;
     const uint32_t base = sbase[sm];
     const int g1_0 = base & 0xffff, g2_0 = base >> 16;
-    const float* wt = swt + sm * (W1 + W2);
     float w1[8], w2[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        w1[s] = s < W1 ? wt[s] : 0.f;
-        w2[s] = s < W2 ? wt[W1 + s] : 0.f;
-    }
+    load_weights<WT>(swt + sm * (W1 + W2), W1, W2, w1, w2);
     float2 acc[BCT];
 #pragma unroll
     for (int b = 0; b < BCT; ++b) acc[b] = make_float2(0.f, 0.f);
     const float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    for (int s2 = 0; s2 < W2; ++s2) {
+#pragma unroll 1
+    for (int s2 = 0; s2 < (WT > 0 ? WT : 8); ++s2) {
+        if (s2 >= W2) break;
         int g2 = g2_0 + s2;
         if (g2 >= G2) g2 -= G2;
         const float2* zrow = zg + (int64_t)g2 * G1 * BCT;
-        for (int s1 = 0; s1 < W1; ++s1) {
+#pragma unroll
+        for (int s1 = 0; s1 < (WT > 0 ? WT : 8); ++s1) {
+            if (s1 >= W1) break;
             int g1 = g1_0 + s1;
             if (g1 >= G1) g1 -= G1;
             const float w = w1[s1] * w2[s2];
@@ -329,6 +349,17 @@ __global__ __launch_bounds__(256) void nu_degrid(const NuSliceDev* __restrict__ 
     const float d = sdcf[sm];
     const int lanes = lanes_of(run, BCT, ch);
     const int64_t stride = chunk_stride(run, BCT);
+    if (run.ylayout) {
+        // transposed layout: the BCT values of a sample are contiguous
+        float2* yt = y + S.yt_off + ((int64_t)ch * S.M + m) * BCT;
+#pragma unroll
+        for (int b = 0; b < BCT; ++b) {
+            float f = b < lanes ? d : 0.f;
+            if (run.W && b < lanes) f *= run.W[S.dat_off + ((int64_t)ch * stride + b) * S.M + m];
+            yt[b] = make_float2(acc[b].x * f, acc[b].y * f);
+        }
+        return;
+    }
 #pragma unroll
     for (int b = 0; b < BCT; ++b) {
         if (b < lanes) {
@@ -340,79 +371,281 @@ __global__ __launch_bounds__(256) void nu_degrid(const NuSliceDev* __restrict__ 
     }
 }
 
+// Standard data layout -> transposed [M][BCT] layout (LDS transpose of 256
+// samples x BCT lanes per work-group; lanes beyond the valid count are 0).
 template <int BCT>
-__global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl,
-                                               const int4* __restrict__ work,
-                                               const TileWork* __restrict__ tiles,
-                                               const int* __restrict__ tlist,
-                                               const uint32_t* __restrict__ sbase,
-                                               const float* __restrict__ swt,
-                                               const float* __restrict__ sdcf,
-                                               const float2* __restrict__ y, float2* __restrict__ ws,
-                                               float2* __restrict__ parts, int* __restrict__ counters,
-                                               NuRun run) {
-    __shared__ float acc[kTile * kTile * BCT * 2];
-    __shared__ int s_last;
+__global__ __launch_bounds__(256) void nu_transpose_y(const NuSliceDev* __restrict__ sl,
+                                                      const int4* __restrict__ work,
+                                                      const float2* __restrict__ y,
+                                                      float2* __restrict__ yt, NuRun run) {
+    __shared__ float2 buf[256 * BCT];
     const int4 wk = work[blockIdx.x];
     const NuSliceDev& S = sl[wk.x];
-    const int ch = wk.y;
-    const TileWork tw = tiles[wk.z];
-    const int G1 = S.G1, G2 = S.G2, W1 = S.W1, W2 = S.W2, M = S.M;
-    for (int i = threadIdx.x; i < kTile * kTile * BCT * 2; i += blockDim.x) acc[i] = 0.f;
+    const int ch = wk.y, m0 = wk.z;
+    const int n = min(256, S.M - m0);
+    const int lanes = lanes_of(run, BCT, ch);
+    const int64_t stride = chunk_stride(run, BCT);
+    for (int b = 0; b < BCT; ++b) {
+        const int t = threadIdx.x;
+        buf[t * BCT + b] = (t < n && b < lanes) ? y[S.dat_off + ((int64_t)ch * stride + b) * S.M + m0 + t]
+                                                : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2* dst = yt + S.yt_off + ((int64_t)ch * S.M + m0) * BCT;
+    for (int i = threadIdx.x; i < n * BCT; i += blockDim.x) dst[i] = buf[i];
+}
+
+// Gather gridding: each thread owns (cell, transform) pairs of rows
+// [r0, r1) of one tile and sums, in a fixed order, the samples of the W x W
+// base-cell bins that reach the cell -- no atomics, deterministic.  The data
+// side is read in the transposed layout, so the BCT lanes of a cell read one
+// contiguous BCT*8-byte row per sample.
+constexpr int kMaxBins = (kTile + 7) * (kTile + 7) + 1;
+
+// entries of a gathered tile that fit the LDS staging area (32 KB)
+template <int BCT>
+constexpr int gather_max_entries(int W) {
+    return (32 * 1024) / (8 * W + 8 * BCT);
+}
+
+template <int BCT, int WT>
+__device__ __forceinline__ void grid_gather_rows(const NuSliceDev& S, int ch, const GatherWork g,
+                                                 const int* __restrict__ glist,
+                                                 const int* __restrict__ gbin,
+                                                 const float* __restrict__ swt,
+                                                 const float* __restrict__ sdcf,
+                                                 const float2* __restrict__ yt,
+                                                 float2* __restrict__ ws, const NuRun& run, int* sb,
+                                                 float* stage) {
+    const int W = WT > 0 ? WT : S.W1;
+    const int NB1 = kTile + W - 1;
+    const int nbins = NB1 * NB1 + 1;
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) sb[i] = gbin[g.bin_off + i];
+    __syncthreads();
+    const int n = sb[nbins - 1];   // entries of the tile (fits the staging area)
+    const int* list = glist + g.ent_off;
+    const float2* ych = yt + S.yt_off + (int64_t)ch * S.M * BCT;
+    // stage: per entry 2W weights (dim-1 weights times dcf) and its BCT data values
+    float* sw = stage;                                          // [n][2W]
+    float2* sy = reinterpret_cast<float2*>(stage + 2 * W * n);  // [n][BCT]
+    for (int i = threadIdx.x; i < n * 2 * W; i += blockDim.x) {
+        const int e = i / (2 * W), q = i % (2 * W);
+        const int64_t sm = S.samp_off + list[e];
+        float w = swt[sm * (2 * W) + q];
+        if (q < W) w *= sdcf[sm];
+        sw[i] = w;
+    }
+    for (int i = threadIdx.x; i < n * BCT; i += blockDim.x) {
+        const int e = i / BCT, b = i & (BCT - 1);
+        sy[i] = ych[(int64_t)list[e] * BCT + b];
+    }
+    __syncthreads();
+    const int G1 = S.G1, G2 = S.G2;
+    const int lanes = lanes_of(run, BCT, ch);
+    float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
+    const int pairs = (g.r1 - g.r0) * kTile * BCT;
+    for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
+        const int b = p & (BCT - 1);
+        const int cell = p / BCT;
+        const int l1 = cell % kTile, l2 = g.r0 + cell / kTile;
+        const int g1 = g.t1 * kTile + l1, g2 = g.t2 * kTile + l2;
+        if (g1 >= G1 || g2 >= G2) continue;
+        float2 acc = make_float2(0.f, 0.f);
+        if (b < lanes) {
+            for (int dy = 0; dy < W; ++dy) {
+                const int s2 = W - 1 - dy;
+                const int rowb = (l2 + dy) * NB1 + l1;
+                for (int dx = 0; dx < W; ++dx) {
+                    const int s1 = W - 1 - dx;
+                    const int e1 = sb[rowb + dx + 1];
+                    for (int e = sb[rowb + dx]; e < e1; ++e) {
+                        const float w = sw[e * 2 * W + s1] * sw[e * 2 * W + W + s2];
+                        const float2 v = sy[e * BCT + b];
+                        acc.x = fmaf(w, v.x, acc.x);
+                        acc.y = fmaf(w, v.y, acc.y);
+                    }
+                }
+            }
+        }
+        zg[((int64_t)g2 * G1 + g1) * BCT + b] = acc;
+    }
+}
+
+template <int BCT, int WT>
+__device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, const TileWork tw,
+                                                  const int* __restrict__ tlist,
+                                                  const uint32_t* __restrict__ sbase,
+                                                  const float* __restrict__ swt,
+                                                  const float* __restrict__ sdcf,
+                                                  const float2* __restrict__ y, float2* __restrict__ ws,
+                                                  float2* __restrict__ parts, int* __restrict__ counters,
+                                                  const NuRun& run, float* accr, float* acci, int& s_last) {
+    constexpr int CELLS = kTile * kTile * BCT;
+    const int G1 = S.G1, G2 = S.G2, M = S.M;
+    const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2;
+    for (int i = threadIdx.x; i < CELLS; i += blockDim.x) {
+        accr[i] = 0.f;
+        acci[i] = 0.f;
+    }
     __syncthreads();
     const int t1_0 = tw.t1 * kTile, t2_0 = tw.t2 * kTile;
     const int lanes = lanes_of(run, BCT, ch);
     const int64_t stride = chunk_stride(run, BCT);
     const int* list = tlist + S.tlist_off + tw.start;
-    const int total = tw.count * BCT;
+    if constexpr (WT == 4) {
+        // Each thread owns one transform b and a contiguous run of the list,
+        // which is sorted by base cell: the 4x4 footprint of consecutive
+        // samples with the same base is summed in registers and added to the
+        // LDS tile once per distinct base.  In the dense centre of a
+        // variable-density trajectory hundreds of samples share a base, so
+        // this removes almost all same-address LDS atomics.
+        const int b = threadIdx.x & (BCT - 1);
+        const int SPB = 256 / BCT;
+        const int per = (tw.count + SPB - 1) / SPB;
+        const int j0 = (threadIdx.x / BCT) * per;
+        const int j1 = min(tw.count, j0 + per);
+        if (b < lanes && j0 < j1) {
+            float2 acc[4][4];
+            uint32_t cur = sbase[S.samp_off + list[j0]];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[a][c] = make_float2(0.f, 0.f);
+            auto flush = [&](uint32_t base) {
+                int l1[4], l2[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int g = (int)(base & 0xffff) + q;
+                    g -= g >= G1 ? G1 : 0;
+                    l1[q] = g - t1_0;
+                    int h = (int)(base >> 16) + q;
+                    h -= h >= G2 ? G2 : 0;
+                    l2[q] = h - t2_0;
+                }
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+                    if ((unsigned)l2[s2] >= (unsigned)kTile) continue;
+#pragma unroll
+                    for (int s1 = 0; s1 < 4; ++s1) {
+                        if ((unsigned)l1[s1] >= (unsigned)kTile) continue;
+                        const int c = (l2[s2] * kTile + l1[s1]) * BCT + b;
+                        atomicAdd(&accr[c], acc[s2][s1].x);
+                        atomicAdd(&acci[c], acc[s2][s1].y);
+                    }
+                }
+            };
+            for (int j = j0; j < j1; j += 4) {
+                // loads of four samples are issued before any of them is used
+                int mq[4];
+                uint32_t bq[4];
+                float dq[4];
+                float2 vq[4];
+                float4 wa[4], wb[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) mq[q] = j + q < j1 ? list[j + q] : -1;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (mq[q] < 0) continue;
+                    const int64_t sm = S.samp_off + mq[q];
+                    bq[q] = sbase[sm];
+                    dq[q] = sdcf[sm];
+                    vq[q] = run.ylayout ? y[S.yt_off + ((int64_t)ch * M + mq[q]) * BCT + b]
+                                        : y[S.dat_off + ((int64_t)ch * stride + b) * M + mq[q]];
+                    wa[q] = *reinterpret_cast<const float4*>(swt + sm * 8);
+                    wb[q] = *reinterpret_cast<const float4*>(swt + sm * 8 + 4);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (mq[q] < 0) continue;
+                    if (bq[q] != cur) {
+                        flush(cur);
+#pragma unroll
+                        for (int a = 0; a < 4; ++a)
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) acc[a][c] = make_float2(0.f, 0.f);
+                        cur = bq[q];
+                    }
+                    const float w1[4] = {wa[q].x, wa[q].y, wa[q].z, wa[q].w};
+                    const float w2[4] = {wb[q].x, wb[q].y, wb[q].z, wb[q].w};
+                    const float vx0 = vq[q].x * dq[q], vy0 = vq[q].y * dq[q];
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) {
+                        const float vx = w2[s2] * vx0, vy = w2[s2] * vy0;
+#pragma unroll
+                        for (int s1 = 0; s1 < 4; ++s1) {
+                            acc[s2][s1].x = fmaf(w1[s1], vx, acc[s2][s1].x);
+                            acc[s2][s1].y = fmaf(w1[s1], vy, acc[s2][s1].y);
+                        }
+                    }
+                }
+            }
+            flush(cur);
+        }
+    } else {
+    // Samples that are adjacent in the list are adjacent on the trajectory and
+    // hit the same cells; the lanes of one pass take samples Q apart instead
+    // (a transposed walk of the SPB x Q list), so that the LDS atomics of a
+    // wave rarely target the same address.
+    const int SPB = 256 / BCT;
+    const int Q = (tw.count + SPB - 1) / SPB;
+    const int total = SPB * Q * BCT;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int b = idx & (BCT - 1);
-        if (b >= lanes) continue;
-        const int m = list[idx / BCT];
+        const int k = idx / BCT;
+        const int kk = (k % SPB) * Q + k / SPB;
+        if (b >= lanes || kk >= tw.count) continue;
+        const int m = list[kk];
         const int64_t sm = S.samp_off + m;
-        float2 v = y[S.dat_off + ((int64_t)ch * stride + b) * M + m];
+        const uint32_t base = sbase[sm];
         const float d = sdcf[sm];
+        float2 v = run.ylayout ? y[S.yt_off + ((int64_t)ch * M + m) * BCT + b]
+                               : y[S.dat_off + ((int64_t)ch * stride + b) * M + m];
         v.x *= d;
         v.y *= d;
-        const uint32_t base = sbase[sm];
-        const int g1_0 = base & 0xffff, g2_0 = base >> 16;
-        const float* wt = swt + sm * (W1 + W2);
-        for (int s2 = 0; s2 < W2; ++s2) {
-            int g2 = g2_0 + s2;
-            if (g2 >= G2) g2 -= G2;
-            const int l2 = g2 - t2_0;
-            if (l2 < 0 || l2 >= kTile) continue;
-            const float w2 = wt[W1 + s2];
-            for (int s1 = 0; s1 < W1; ++s1) {
-                int g1 = g1_0 + s1;
-                if (g1 >= G1) g1 -= G1;
-                const int l1 = g1 - t1_0;
-                if (l1 < 0 || l1 >= kTile) continue;
-                const float w = wt[s1] * w2;
-                float* a = acc + ((l2 * kTile + l1) * BCT + b) * 2;
-                atomicAdd(a, w * v.x);
-                atomicAdd(a + 1, w * v.y);
+        float w1[8], w2[8];
+        load_weights<WT>(swt + sm * (W1 + W2), W1, W2, w1, w2);
+        // tile-local tap coordinates (the footprint may wrap around the grid)
+        int l1[8], l2[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            int g = (int)(base & 0xffff) + s;
+            g -= g >= G1 ? G1 : 0;
+            l1[s] = g - t1_0;
+            int h = (int)(base >> 16) + s;
+            h -= h >= G2 ? G2 : 0;
+            l2[s] = h - t2_0;
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) {
+            if (s2 >= W2 || (unsigned)l2[s2] >= (unsigned)kTile) continue;
+            const float wy = w2[s2];
+#pragma unroll
+            for (int s1 = 0; s1 < 8; ++s1) {
+                if (s1 >= W1 || (unsigned)l1[s1] >= (unsigned)kTile) continue;
+                const float w = w1[s1] * wy;
+                const int c = (l2[s2] * kTile + l1[s1]) * BCT + b;
+                atomicAdd(&accr[c], w * v.x);
+                atomicAdd(&acci[c], w * v.y);
             }
         }
     }
+    }
     __syncthreads();
     float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    const int cells = kTile * kTile * BCT;
     if (tw.slot < 0) {
-        for (int i = threadIdx.x; i < cells; i += blockDim.x) {
+        for (int i = threadIdx.x; i < CELLS; i += blockDim.x) {
             const int b = i & (BCT - 1);
             const int l1 = (i / BCT) % kTile, l2 = i / (BCT * kTile);
             const int g1 = t1_0 + l1, g2 = t2_0 + l2;
-            if (g1 < G1 && g2 < G2)
-                zg[((int64_t)g2 * G1 + g1) * BCT + b] = make_float2(acc[2 * i], acc[2 * i + 1]);
+            if (g1 < G1 && g2 < G2) zg[((int64_t)g2 * G1 + g1) * BCT + b] = make_float2(accr[i], acci[i]);
         }
         return;
     }
     // split tile: publish this part, the last arriving work-group sums all
     // parts in slot order and writes the tile
     float2* pbase = parts + S.part_off + (int64_t)ch * S.part_chunk;
-    float2* mine = pbase + (int64_t)tw.slot * cells;
-    for (int i = threadIdx.x; i < cells; i += blockDim.x) mine[i] = make_float2(acc[2 * i], acc[2 * i + 1]);
+    float2* mine = pbase + (int64_t)tw.slot * CELLS;
+    for (int i = threadIdx.x; i < CELLS; i += blockDim.x) mine[i] = make_float2(accr[i], acci[i]);
     __threadfence();   // every wave's partial is visible device-wide before the count
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -424,18 +657,50 @@ __global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    for (int i = threadIdx.x; i < cells; i += blockDim.x) {
-        float2 s = make_float2(0.f, 0.f);
+    for (int i = threadIdx.x; i < CELLS; i += blockDim.x) {
+        float2 sacc = make_float2(0.f, 0.f);
         for (int p = 0; p < tw.nparts; ++p) {
-            const float2 q = pbase[(int64_t)(tw.first_slot + p) * cells + i];
-            s.x += q.x;
-            s.y += q.y;
+            const float2 q = pbase[(int64_t)(tw.first_slot + p) * CELLS + i];
+            sacc.x += q.x;
+            sacc.y += q.y;
         }
         const int b = i & (BCT - 1);
         const int l1 = (i / BCT) % kTile, l2 = i / (BCT * kTile);
         const int g1 = t1_0 + l1, g2 = t2_0 + l2;
-        if (g1 < G1 && g2 < G2) zg[((int64_t)g2 * G1 + g1) * BCT + b] = s;
+        if (g1 < G1 && g2 < G2) zg[((int64_t)g2 * G1 + g1) * BCT + b] = sacc;
     }
+}
+
+// Gridding: one launch over two kinds of work items (int4.w): 1 = a chunk of
+// a dense tile (register-run scatter into LDS), 0 = cell rows of a sparse
+// tile (atomic-free gather).  Each tile is handled by one kind only.
+template <int BCT, int WT>
+__global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl,
+                                               const int4* __restrict__ work,
+                                               const TileWork* __restrict__ tiles,
+                                               const int* __restrict__ tlist,
+                                               const GatherWork* __restrict__ gwk,
+                                               const int* __restrict__ glist,
+                                               const int* __restrict__ gbin,
+                                               const uint32_t* __restrict__ sbase,
+                                               const float* __restrict__ swt,
+                                               const float* __restrict__ sdcf,
+                                               const float2* __restrict__ y, float2* __restrict__ ws,
+                                               float2* __restrict__ parts, int* __restrict__ counters,
+                                               NuRun run) {
+    constexpr int CELLS = kTile * kTile * BCT;
+    // scatter: two CELLS-float planes; gather: 32 KB entry staging
+    constexpr int AREA = 2 * CELLS > 8192 ? 2 * CELLS : 8192;
+    __shared__ float area[AREA];
+    __shared__ int sb[kMaxBins];
+    __shared__ int s_last;
+    const int4 wk = work[blockIdx.x];
+    const NuSliceDev& S = sl[wk.x];
+    if (wk.w == 0)
+        grid_gather_rows<BCT, WT>(S, wk.y, gwk[wk.z], glist, gbin, swt, sdcf, y, ws, run, sb, area);
+    else
+        grid_scatter_tile<BCT, WT>(S, wk.y, tiles[wk.z], tlist, sbase, swt, sdcf, y, ws, parts, counters, run,
+                                   area, area + CELLS, s_last);
 }
 
 template <int BCT>
@@ -548,7 +813,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     std::vector<float> deap;
     std::vector<uint32_t> sbase;
     std::vector<float> swt, sdcf;
-    std::vector<int> tlist;
+    std::vector<int> tlist, glist_h, gbin_h;
     int64_t voff = 0, moff = 0;
     for (int s = 0; s < nslice; ++s) {
         NuSliceDev d{};
@@ -650,9 +915,27 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         tw_first_.push_back((int)tw_host_.size());
         int nslots = 0, ncnt = 0;
         int64_t lstart = 0;
+        // A tile is gridded by the register-run scatter when its samples pile
+        // up on few base cells (dense centre of a variable-density
+        // trajectory), otherwise by the atomic-free gather.
+        const bool can_gather = ndim == 2 && d.G1 >= 24 && d.G2 >= 24;
+        std::vector<char> dense((size_t)nt1 * nt2, 1);
         for (int t2 = 0; t2 < nt2; ++t2)
             for (int t1 = 0; t1 < nt1; ++t1) {
-                const auto& L = lists[(size_t)t2 * nt1 + t1];
+                auto& L = lists[(size_t)t2 * nt1 + t1];
+                // by base cell: runs of samples with the same footprint are
+                // accumulated in registers by the gridding kernel
+                std::stable_sort(L.begin(), L.end(), [&](int a, int b) {
+                    return sbase[(size_t)d.samp_off + a] < sbase[(size_t)d.samp_off + b];
+                });
+                int distinct = 0;
+                for (size_t q = 0; q < L.size(); ++q)
+                    distinct += (q == 0 || sbase[(size_t)d.samp_off + L[q]] != sbase[(size_t)d.samp_off + L[q - 1]]);
+                const int gmax = std::min(gather_max_entries<16>(kwidth), 4096);
+                const bool dn = !can_gather || (L.size() >= 64 && (int)L.size() >= 3 * distinct) ||
+                                (int)L.size() > gmax;
+                dense[(size_t)t2 * nt1 + t1] = dn;
+                if (!dn) continue;
                 const int cnt = (int)L.size();
                 const int np = std::max(1, (cnt + kTileChunk - 1) / kTileChunk);
                 const int first = np > 1 ? nslots : -1;
@@ -675,6 +958,54 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                 lstart += cnt;
             }
         tw_count_.push_back((int)tw_host_.size() - tw_first_.back());
+        // gather bins: every (sample, touched tile) pair goes to the bin of its
+        // base cell relative to the tile origin
+        if (can_gather) {
+            const int NB1 = kTile + kwidth - 1;
+            gw_first_.push_back((int)gw_host_.size());
+            for (int t2 = 0; t2 < nt2; ++t2)
+                for (int t1 = 0; t1 < nt1; ++t1) {
+                    if (dense[(size_t)t2 * nt1 + t1]) continue;
+                    const auto& L = lists[(size_t)t2 * nt1 + t1];
+                    std::vector<std::pair<int, int>> eb;  // (bin, sample)
+                    eb.reserve(L.size());
+                    for (int m : L) {
+                        // bin = base cell - tile origin + (W-1), taken modulo G
+                        const int r1p = ((g0[0][m] - t1 * kTile + (kwidth - 1)) % d.G1 + d.G1) % d.G1;
+                        const int r2p = ((g0[1][m] - t2 * kTile + (kwidth - 1)) % d.G2 + d.G2) % d.G2;
+                        if (r1p >= NB1 || r2p >= NB1) continue;  // footprint misses the tile
+                        eb.emplace_back(r2p * NB1 + r1p, m);
+                    }
+                    std::stable_sort(eb.begin(), eb.end(),
+                                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                                         return a.first < b.first;
+                                     });
+                    GatherWork gwk{};
+                    gwk.t1 = t1;
+                    gwk.t2 = t2;
+                    gwk.ent_off = (int64_t)glist_h.size();
+                    gwk.bin_off = (int64_t)gbin_h.size();
+                    std::vector<int> cnt(NB1 * NB1 + 1, 0);
+                    for (auto& q : eb) cnt[q.first + 1]++;
+                    for (int i = 0; i < NB1 * NB1; ++i) cnt[i + 1] += cnt[i];
+                    gbin_h.insert(gbin_h.end(), cnt.begin(), cnt.end());
+                    for (auto& q : eb) glist_h.push_back(q.second);
+                    // split heavy tiles by cell rows (no merge needed)
+                    const int n = (int)eb.size();
+                    const int nsplit = std::min(kTile, std::max(1, (n + 47) / 48));
+                    const int rows = (kTile + nsplit - 1) / nsplit;
+                    for (int r = 0; r < kTile; r += rows) {
+                        gwk.r0 = r;
+                        gwk.r1 = std::min(kTile, r + rows);
+                        gw_host_.push_back(gwk);
+                        gw_cost_.push_back((int64_t)n * (gwk.r1 - gwk.r0) / kTile + 16);
+                    }
+                }
+            gw_count_.push_back((int)gw_host_.size() - gw_first_.back());
+        } else {
+            gw_first_.push_back((int)gw_host_.size());
+            gw_count_.push_back(0);
+        }
         nslots_.push_back(nslots);
         ncounters_.push_back(ncnt);
         d.tile_first = tw_first_.back();
@@ -696,7 +1027,12 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     swt_.upload(swt, st);
     sdcf_.upload(sdcf, st);
     tlist_.upload(tlist.empty() ? std::vector<int>{0} : tlist, st);
-    tiles_.upload(tw_host_, st);
+    tiles_.upload(tw_host_.empty() ? std::vector<TileWork>{TileWork{}} : tw_host_, st);
+    // the transposed data layout is used when any tile is gathered
+    gather_ = !gw_host_.empty();
+    gwork_.upload(gw_host_.empty() ? std::vector<GatherWork>{GatherWork{}} : gw_host_, st);
+    glist_.upload(glist_h.empty() ? std::vector<int>{0} : glist_h, st);
+    gbin_.upload(gbin_h.empty() ? std::vector<int>{0} : gbin_h, st);
     HIPCHECK(hipStreamSynchronize(st));
 }
 
@@ -723,8 +1059,9 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     const int BCT = c.BCT;
     const int ns = (int)sl_.size();
     c.hs = sl_;
-    int64_t wsoff = 0, poff = 0, coff = 0;
-    std::vector<int4> wr, wc, wd, wg;
+    int64_t wsoff = 0, poff = 0, coff = 0, ytoff = 0;
+    std::vector<int4> wr, wc, wd, wg, wga;
+    std::vector<int64_t> wga_cost;
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
         // rows / columns per work-group: aim for 8-16 interleaved transforms
@@ -755,17 +1092,33 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         d.cnt_chunk = ncounters_[s];
         d.cnt_off = coff;
         coff += d.cnt_chunk * nchunk;
+        d.yt_off = ytoff;
+        ytoff += (int64_t)d.M * BCT * nchunk;
         for (int ch = 0; ch < nchunk; ++ch) {
             for (int r = 0; r < d.N2; r += rpw) wr.push_back(make_int4(s, ch, r, 0));
             if (d.G2 > 1)
                 for (int q = 0; q < d.G1; q += cpw) wc.push_back(make_int4(s, ch, q, 0));
             for (int m = 0; m < d.M; m += 256) wd.push_back(make_int4(s, ch, m, 0));
-            for (int t = 0; t < tw_count_[s]; ++t)
-                wg.push_back(make_int4(s, ch, tw_first_[s] + t, tw_host_[tw_first_[s] + t].count));
+            // gridding items: .w = 1 scatter chunk, 0 gather rows
+            for (int t = 0; t < tw_count_[s]; ++t) {
+                wg.push_back(make_int4(s, ch, tw_first_[s] + t, 1));
+                wga_cost.push_back(tw_host_[tw_first_[s] + t].count + 16);
+            }
+            for (int t = 0; t < gw_count_[s]; ++t) {
+                wg.push_back(make_int4(s, ch, gw_first_[s] + t, 0));
+                wga_cost.push_back(gw_cost_[gw_first_[s] + t]);
+            }
         }
     }
     // heaviest gridding items first
-    std::stable_sort(wg.begin(), wg.end(), [](const int4& a, const int4& b) { return a.w > b.w; });
+    {
+        std::vector<size_t> ord(wg.size());
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wga_cost[a] > wga_cost[b]; });
+        std::vector<int4> sorted(wg.size());
+        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wg[ord[i]];
+        wg.swap(sorted);
+    }
     hipStream_t st = ctx->stream;
     c.sl.upload(c.hs, st);
     c.w_rows.upload(wr, st);
@@ -776,6 +1129,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.n_cols = (int)wc.size();
     c.n_deg = (int)wd.size();
     c.n_grid = (int)wg.size();
+    if (gather_) c.yT.alloc(std::max<int64_t>(ytoff, 1));
     c.ws.alloc(std::max<int64_t>(wsoff, 1));
     c.parts.alloc(std::max<int64_t>(poff, 1));
     c.counters.alloc(std::max<int64_t>(coff, 1));
@@ -786,12 +1140,8 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
 }
 
 template <int BCT>
-static void launch_all(int adjoint, hipStream_t st, const NuRun& run, const float2* in, float2* out,
-                       const NuSliceDev* sl, int ndim, const int4* w_rows, int n_rows, size_t lds_rows,
-                       const int4* w_cols, int n_cols, size_t lds_cols, const int4* w_deg, int n_deg,
-                       const int4* w_grid, int n_grid, const FftFactors* facs, const float2* twid,
-                       const float* deap, const uint32_t* sbase, const float* swt, const float* sdcf,
-                       const TileWork* tiles, const int* tlist, float2* ws, float2* parts, int* counters) {
+void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, float2* out, int mode) {
+    hipStream_t st = ctx->stream;
     static bool attr_set = false;
     if (!attr_set) {
         // allow FFT passes above 64 KB of LDS where the device permits it
@@ -804,66 +1154,109 @@ static void launch_all(int adjoint, hipStream_t st, const NuRun& run, const floa
         (void)hipGetLastError();
         attr_set = true;
     }
+    const bool wt4 = ndim_ == 2 && W_ == 4;
+    const NuSliceDev* sl = c.sl.p;
+    auto forward = [&](const float2* x, float2* y, NuRun r) {
+        hipLaunchKernelGGL(nu_fwd_rows<BCT>, dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
+                           twid_.p, deap_.p, x, c.ws.p, r);
+        LAUNCHCHECK();
+        if (ndim_ == 2 && c.n_cols > 0) {
+            hipLaunchKernelGGL(nu_cols<BCT>, dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
+                               twid_.p, c.ws.p, 0);
+            LAUNCHCHECK();
+        }
+        if (wt4)
+            hipLaunchKernelGGL((nu_degrid<BCT, 4>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, sbase_.p,
+                               swt_.p, sdcf_.p, (const float2*)c.ws.p, y, r);
+        else
+            hipLaunchKernelGGL((nu_degrid<BCT, 0>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, sbase_.p,
+                               swt_.p, sdcf_.p, (const float2*)c.ws.p, y, r);
+        LAUNCHCHECK();
+    };
+    auto adjoint_from = [&](const float2* y, float2* x, NuRun r) {
+        if (wt4)
+            hipLaunchKernelGGL((nu_grid<BCT, 4>), dim3(c.n_grid), dim3(256), 0, st, sl, c.w_grid.p, tiles_.p,
+                               tlist_.p, gwork_.p, glist_.p, gbin_.p, sbase_.p, swt_.p, sdcf_.p, y, c.ws.p,
+                               c.parts.p, c.counters.p, r);
+        else
+            hipLaunchKernelGGL((nu_grid<BCT, 0>), dim3(c.n_grid), dim3(256), 0, st, sl, c.w_grid.p, tiles_.p,
+                               tlist_.p, gwork_.p, glist_.p, gbin_.p, sbase_.p, swt_.p, sdcf_.p, y, c.ws.p,
+                               c.parts.p, c.counters.p, r);
+        LAUNCHCHECK();
+        if (ndim_ == 2 && c.n_cols > 0) {
+            hipLaunchKernelGGL(nu_cols<BCT>, dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
+                               twid_.p, c.ws.p, 1);
+            LAUNCHCHECK();
+        }
+        hipLaunchKernelGGL(nu_adj_rows<BCT>, dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
+                           twid_.p, deap_.p, (const float2*)c.ws.p, x, r);
+        LAUNCHCHECK();
+    };
+    if (mode == 1) {
+        NuRun rf = run;
+        rf.ylayout = gather_ ? 1 : 0;
+        float2* ybuf = gather_ ? c.yT.p : c.ydata.p;
+        forward(in, ybuf, rf);
+        NuRun ra = run;
+        ra.W = nullptr;
+        ra.ylayout = rf.ylayout;
+        adjoint_from(ybuf, out, ra);
+        return;
+    }
     if (!adjoint) {
-        hipLaunchKernelGGL(nu_fwd_rows<BCT>, dim3(n_rows), dim3(256), lds_rows, st, sl, w_rows, facs,
-                           twid, deap, in, ws, run);
+        NuRun rf = run;
+        rf.ylayout = 0;
+        forward(in, out, rf);
+        return;
+    }
+    NuRun ra = run;
+    ra.W = nullptr;
+    if (gather_) {
+        ra.ylayout = 0;
+        hipLaunchKernelGGL(nu_transpose_y<BCT>, dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, in, c.yT.p, ra);
         LAUNCHCHECK();
-        if (ndim == 2 && n_cols > 0) {
-            hipLaunchKernelGGL(nu_cols<BCT>, dim3(n_cols), dim3(256), lds_cols, st, sl, w_cols, facs,
-                               twid, ws, 0);
-            LAUNCHCHECK();
-        }
-        hipLaunchKernelGGL(nu_degrid<BCT>, dim3(n_deg), dim3(256), 0, st, sl, w_deg, sbase, swt, sdcf,
-                           (const float2*)ws, out, run);
-        LAUNCHCHECK();
+        ra.ylayout = 1;
+        adjoint_from(c.yT.p, out, ra);
     } else {
-        hipLaunchKernelGGL(nu_grid<BCT>, dim3(n_grid), dim3(256), 0, st, sl, w_grid, tiles, tlist,
-                           sbase, swt, sdcf, in, ws, parts, counters, run);
-        LAUNCHCHECK();
-        if (ndim == 2 && n_cols > 0) {
-            hipLaunchKernelGGL(nu_cols<BCT>, dim3(n_cols), dim3(256), lds_cols, st, sl, w_cols, facs,
-                               twid, ws, 1);
-            LAUNCHCHECK();
-        }
-        hipLaunchKernelGGL(nu_adj_rows<BCT>, dim3(n_rows), dim3(256), lds_rows, st, sl, w_rows, facs,
-                           twid, deap, (const float2*)ws, out, run);
-        LAUNCHCHECK();
+        ra.ylayout = 0;
+        adjoint_from(in, out, ra);
     }
 }
 
-void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps,
-                  const float* W) {
+void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps, const float* W,
+                  int mode) {
     NuRun r;
+    r.ylayout = 0;
     r.sense = c.sense;
     r.C = c.C;
     r.B = c.nrep;
     r.maps = maps;
-    r.W = adjoint ? nullptr : W;
-    hipStream_t st = ctx->stream;
-#define MM_NU_LAUNCH(BC)                                                                          \
-    launch_all<BC>(adjoint, st, r, in, out, c.sl.p, ndim_, c.w_rows.p, c.n_rows, c.lds_rows,      \
-                   c.w_cols.p, c.n_cols, c.lds_cols, c.w_deg.p, c.n_deg, c.w_grid.p, c.n_grid,    \
-                   facs_.p, twid_.p, deap_.p, sbase_.p, swt_.p, sdcf_.p, tiles_.p, tlist_.p,       \
-                   c.ws.p, c.parts.p, c.counters.p)
+    r.W = W;
+    if (mode == 1 && !gather_) c.ydata.ensure((size_t)(c.sense ? M_total_ * c.C * c.nrep : M_total_ * c.nrep));
     switch (c.BCT) {
-        case 1: MM_NU_LAUNCH(1); break;
-        case 2: MM_NU_LAUNCH(2); break;
-        case 4: MM_NU_LAUNCH(4); break;
-        case 8: MM_NU_LAUNCH(8); break;
-        default: MM_NU_LAUNCH(16); break;
+        case 1: launch<1>(c, adjoint, r, in, out, mode); break;
+        case 2: launch<2>(c, adjoint, r, in, out, mode); break;
+        case 4: launch<4>(c, adjoint, r, in, out, mode); break;
+        case 8: launch<8>(c, adjoint, r, in, out, mode); break;
+        default: launch<16>(c, adjoint, r, in, out, mode); break;
     }
-#undef MM_NU_LAUNCH
 }
 
 void NufftOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
-    run(config(nrep, 0, 0), adjoint, in, out, nullptr, nullptr);
+    run(config(nrep, 0, 0), adjoint, in, out, nullptr, nullptr, 0);
 }
 
 void NufftOp::apply_sense(int adjoint, const float2* in, float2* out, int64_t nrep,
                           const float2* maps, int C, const float* Wt) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
-    run(config(nrep, 1, C), adjoint, in, out, maps, Wt);
+    run(config(nrep, 1, C), adjoint, in, out, maps, adjoint ? nullptr : Wt);
+}
+
+void NufftOp::apply_sense_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
+                                 const float* Wt) {
+    if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
+    run(config(nrep, 1, C), 0, in, out, maps, Wt, 1);
 }
 
 }  // namespace mm

@@ -29,6 +29,16 @@ struct NuSliceDev {
     int64_t part_chunk; // partial-tile elements per chunk
     int64_t cnt_off;    // counter offset of chunk 0
     int64_t cnt_chunk;  // counters per chunk
+    int64_t yt_off;     // transposed data ([M][BCT] per chunk) offset of chunk 0
+};
+
+// One gather-gridding work item: rows [r0, r1) of the 16x16 cells of tile
+// (t1, t2).  The tile's samples are bucketed by base cell relative to the
+// tile origin into (16+W-1)^2 bins (CSR offsets at bin_off).
+struct GatherWork {
+    int t1, t2, r0, r1;
+    int64_t ent_off;    // first entry of the tile in the gather list
+    int64_t bin_off;    // first bin offset of the tile
 };
 
 // One gridding work item: a chunk of the sample list of one tile.
@@ -59,6 +69,11 @@ class NufftOp : public mm_op {
     // [M_s x C x nrep] per slice.  W (nullable) multiplies the forward output.
     void apply_sense(int adjoint, const float2* in, float2* out, int64_t nrep, const float2* maps,
                      int C, const float* W);
+    // x_out = E'(W .* E x) for the SENSE-fused operator; the data side stays in
+    // the transposed [M][coil] layout between the two halves.
+    void apply_sense_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
+                            const float* W);
+    bool gather() const { return gather_; }
     int ndim() const { return ndim_; }
     const std::vector<int64_t>& slice_V() const { return Vs_; }
     const std::vector<int64_t>& slice_M() const { return Ms_; }
@@ -76,11 +91,19 @@ class NufftOp : public mm_op {
         DevBuf<int4> w_rows, w_cols, w_deg, w_grid;
         int n_rows = 0, n_cols = 0, n_deg = 0, n_grid = 0;
         size_t lds_rows = 0, lds_cols = 0;
-        DevBuf<float2> ws, parts;
+        DevBuf<float2> ws, parts, yT, ydata;
         DevBuf<int> counters;
+        DevBuf<int4> w_gath;
+        int n_gath = 0;
     };
     Cfg& config(int64_t nrep, int sense, int C);
-    void run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps, const float* W);
+    // mode 0: the requested direction with the standard data layout;
+    // mode 1: the normal operator (forward writes the transposed data layout
+    // that the gathering adjoint reads)
+    void run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps, const float* W,
+             int mode = 0);
+    template <int BCT>
+    void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
 
     int ndim_ = 2;
     int W_ = 4;
@@ -101,6 +124,13 @@ class NufftOp : public mm_op {
     DevBuf<TileWork> tiles_;
     std::vector<int64_t> tlist_off_;       // start of each slice's list in tlist_
     std::map<std::pair<int64_t, int>, std::unique_ptr<Cfg>> cfgs_;
+    // gather gridding (2-D, every grid side >= 24)
+    bool gather_ = false;
+    std::vector<GatherWork> gw_host_;
+    std::vector<int> gw_first_, gw_count_;
+    std::vector<int64_t> gw_cost_;
+    DevBuf<GatherWork> gwork_;
+    DevBuf<int> glist_, gbin_;
 };
 
 // Plan constants from the closed forms of nufftOp.m:146-182 (host, FP64).

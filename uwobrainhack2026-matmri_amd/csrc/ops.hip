@@ -150,6 +150,10 @@ bool SenseOp::profile_dominant(int adjoint, const float2* in, float2* out, int64
 }
 
 void SenseOp::apply_normal(const float2* in, float2* out, int64_t nrep, float2* tmp) {
+    if (nu_) {
+        nu_->apply_sense_normal(in, out, nrep, static_cast<RcvrOp*>(R_)->maps(), C_, W);
+        return;
+    }
     apply_impl(0, in, tmp, nrep, W);
     apply_impl(1, tmp, out, nrep, nullptr);
 }
