@@ -1,8 +1,19 @@
 #!/usr/bin/env python3
 """CG-SENSE solves/s on one MI355X.
 
-Workload (default, ``--workload highorder``): the 2-D expanded-encoding
-problem -- 220x220 complex64 image, 32 Gaussian-ring coils, 24000 spiral
+Workload (default, ``--workload warmstart``): the learned warm start against
+zero-init CG on 100 held-out 2-D spiral SENSE problems (128x128 complex64
+image, 8 Gaussian-ring coils with random phases, 16384-sample variable-density
+spiral, Kaiser-Bessel W=4 os=1.5, random 10-ellipse phantom, noise at SNR 40).
+One step = the 100 problems solved to relative residual |r|/|b| = 1e-4 in one
+multi-slice solve (one launch per stage for all problems, hipGraph per
+iteration): the density-compensated adjoint images, the network's x0, and
+cgne from x0 with the zero-init fallback.  The zero-init solve of the same
+batch is timed the same way and reported beside it.  Network weights are the
+ones trained on the device by ``python -m uwobrainhack2026-matmri_amd.warmstart``
+(20000 problems whose seeds precede the held-out ones).
+
+``--workload highorder``: the 2-D expanded-encoding problem -- 220x220 complex64 image, 32 Gaussian-ring coils, 24000 spiral
 samples at 1 us, 16 real solid harmonics up to 3rd order on a 0.2 m FOV with
 a seeded smooth drift, a smooth +-100 Hz B0 map; data = E x_true + complex
 Gaussian noise at SNR 40.  One step = one cgne solve from a zero initial
@@ -36,7 +47,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["highorder", "nufft", "batch"], default="highorder")
+    p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch"], default="warmstart")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--hio-variant", type=int, default=None, help="high-order MFMA kernel variant 0..3")
@@ -203,6 +214,177 @@ def cpu_baseline_nufft(P, seconds):
                       f"scaled to one 20-iteration solve ({apps} applications)"}
 
 
+def nufft_stage_bytes(V, G, M, C):
+    """Algorithmic HBM bytes of each NUFFT SENSE stage for one slice and one
+    application (compulsory reads and writes of complex64 vectors; plan
+    constants such as interpolation weights are not counted)."""
+    return {
+        "nu_fwd_rows": 8.0 * (V + V * C + G * C),     # image, maps in; padded grid out
+        "nu_cols": 2 * 8.0 * G * C,                   # one read + one write of the grid
+        "nu_degrid": 8.0 * (G * C + M + M * C),       # grid, k-space locations in; samples out
+        "nu_grid": 8.0 * (M * C + M + G * C),         # samples, locations in; grid out
+        "nu_adj_rows": 8.0 * (G * C + V * C + V),     # grid, maps in; image out
+        "nu_transpose_y": 2 * 8.0 * M * C,
+    }
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/pmc_<kernel>_<workload>.md, line 'bytes_per_dispatch: N'),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.md")
+    try:
+        for line in open(path):
+            if line.startswith("bytes_per_dispatch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def cpu_baseline_warmstart(fam, host, prob, seconds):
+    """FP64 NumPy oracle zero-init cgne to relative residual 1e-4 on as many
+    of the held-out problems as fit the time budget (the reference CPU
+    path has no learned warm start)."""
+    import numpy as np
+    from oracle.mrsamp import mrSampFunc
+    from oracle.nufft import NufftPlan
+    from oracle.rcvr import RcvrPlan
+    from oracle.solvers import cgne as o_cgne
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    N, M, C = fam.p["N"], fam.p["M"], fam.p["C"]
+    So = NufftPlan([N, N], fam.kloc, None, fam.p["os"], fam.p["kwidth"])
+    Y = prob["y"].reshape(fam.batch, C, M).cpu().numpy().astype(np.complex128)
+    t_tot, n, its = 0.0, 0, []
+    while t_tot < seconds and n < fam.batch:
+        Ro = RcvrPlan(host[0][n])
+        y = Y[n].T
+        A = lambda x, tr: mrSampFunc(x, tr, So, Ro)
+        t0 = time.perf_counter()
+        b = A(y, "transp")
+        x, res, *_ = o_cgne(A, y, np.zeros((N, N), complex), 200,
+                            {"resThresh": 1e-8 * float(np.real(np.vdot(b, b))), "xChangeThresh": None})
+        t_tot += time.perf_counter() - t0
+        its.append(len(res) - 1)
+        n += 1
+    return {"value": n / t_tot, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 NumPy oracle zero-init cgne to rel. resid 1e-4 on {n} of the held-out "
+                      f"problems ({np.mean(its):.1f} iterations on average), {t_tot:.1f} s"}
+
+
+def run_warmstart(a, mm, torch, dev):
+    import ctypes as C
+    import numpy as np
+    ws = mm.warmstart
+    net = mm.WarmStartNet.load()
+    P = 100
+    fam = ws.ProblemFamily(P)
+    N, M, Cc = fam.p["N"], fam.p["M"], fam.p["C"]
+    seeds = list(range(ws.TRAIN_SEED0 + ws.HELDOUT_OFFSET, ws.TRAIN_SEED0 + ws.HELDOUT_OFFSET + P))
+    host = fam.host_batch(seeds)
+    prob = fam.make(seeds, host)
+    E, R, y = prob["E"], prob["R"], prob["y"]
+    NitMax, rel = 200, 1e-4
+    zeros = torch.zeros((P, N, N), dtype=torch.complex64, device=dev)
+
+    def warm_step():
+        xin = ws.input_image(E, R, y, fam.dcf_t).reshape(P, N, N)
+        x0 = ws.warm_x0(net, xin)
+        return ws.solve(E, y, x0, rel, NitMax, fallback=True)
+
+    def zero_step():
+        return ws.solve(E, y, zeros, rel, NitMax)
+
+    res = {}
+    for name, fn in (("zero", zero_step), ("warm", warm_step)):
+        for _ in range(a.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nits, fbs = [], 0
+        for _ in range(a.steps):
+            x, nit, st, fb, hist = fn()
+            nits.append(nit)
+            fbs += int(fb.sum())
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        nits = np.concatenate(nits)
+        res[name] = dict(el=el, nit=nits, fallbacks=fbs, status=np.asarray(st))
+    # relative residuals against |b| (the zero-init history starts at |b|^2)
+    _, _, _, _, hz = zero_step()
+    b2 = np.array([h[0] for h in hz])
+    _, _, _, _, hw = warm_step()
+    rel_w = np.sqrt(np.array([h[-1] for h in hw]) / b2)
+    # network inference alone
+    xin = ws.input_image(E, R, y, fam.dcf_t).reshape(P, N, N)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        ws.warm_x0(net, xin)
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - t0) / 10
+    # dominant kernel of the normal operator, HIP events on the library stream
+    ms = C.c_double()
+    name = C.create_string_buffer(64)
+    xi = torch.zeros(P * N * N, dtype=torch.complex64, device=dev)
+    xo = torch.empty_like(xi)
+    L = mm.lib()
+    rc = L.mm_op_profile(E._plan.h, 2, xi.data_ptr(), xo.data_ptr(), 1, 20, C.byref(ms), name, 64)
+    stages = {}
+    if rc == 0:
+        k = 0
+        while True:
+            sm, sl, sn = C.c_double(), C.c_int(), C.create_string_buffer(64)
+            if L.mm_op_profile_stage(E._plan.h, k, C.byref(sm), C.byref(sl), sn, 64) != 0:
+                break
+            stages[sn.value.decode()] = {"ms_per_launch": sm.value, "launches_per_application": sl.value}
+            k += 1
+    zw, zz = res["warm"], res["zero"]
+    cfg = {"workload": "warmstart: 100 held-out 2-D spiral SENSE problems per step (128x128 c64, 8 coils, "
+                       "M=16384 VD spiral, KB W=4 os=1.5, SNR 40), one multi-slice solve to rel. resid 1e-4; "
+                       "x0 from the learned 6x(3x3 conv, 32 ch) warm start with zero-init fallback",
+           "problems_per_step": P, "image": [N, N], "coils": Cc, "samples": M, "rel_resid": rel,
+           "max_iterations": NitMax, "network": "6 layers 3x3 conv, 2-32-32-32-32-32-2, residual skip, "
+                                                "bf16 weights, fp32 accumulation"}
+    out = {"metric": "CG-SENSE solves/s at rel. resid 1e-4, learned warm start vs zero-init",
+           "value": P * a.steps / zw["el"], "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": 1e3 * zw["el"] / a.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+           "data": "synthetic (seeded phantoms, coils, trajectory, noise); network trained on the device "
+                   "from CG solutions of 20000 synthetic problems",
+           "config": cfg,
+           "zero_init": {"value": P * a.steps / zz["el"], "unit": "solves/s",
+                         "ms_per_step": 1e3 * zz["el"] / a.steps,
+                         "iterations_mean": float(zz["nit"].mean()), "iterations_max": int(zz["nit"].max())},
+           "warm_start": {"iterations_mean": float(zw["nit"].mean()), "iterations_max": int(zw["nit"].max()),
+                          "fallbacks": zw["fallbacks"], "inference_ms_per_step": 1e3 * t_inf},
+           "iteration_ratio_zero_over_warm": float(zz["nit"].mean() / zw["nit"].mean()),
+           "speedup_warm_over_zero": float(zz["el"] / zw["el"]),
+           "rel_resid_final_max": float(rel_w.max()),
+           "status_last": {str(int(k)): int(v) for k, v in zip(*np.unique(zw["status"], return_counts=True))}}
+    if rc == 0 and ms.value > 0:
+        kname = name.value.decode()
+        G = 192 * 192
+        per = nufft_stage_bytes(N * N, G, M, Cc).get(kname)
+        if per is not None:
+            byts = per * P
+            ach = byts / (ms.value * 1e-3) / 1e9
+            traffic = read_traffic(kname, "warmstart")
+            out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": ach / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname,
+                               "ms_per_launch": ms.value, "algorithmic_bytes_per_launch": byts}
+        out["normal_operator_stages"] = stages
+    if not a.no_cpu:
+        cb = cpu_baseline_warmstart(fam, host, prob, a.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -215,6 +397,10 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
     mm._lib.context(0, stream.cuda_stream)
+    if a.workload == "warmstart":
+        with torch.cuda.stream(stream):
+            run_warmstart(a, mm, torch, dev)
+        return
     with torch.cuda.stream(stream):
         W = {"highorder": highorder_setup, "nufft": nufft_setup, "batch": batch_setup}[a.workload](mm, torch, dev)
         E, y, x0, NitMax = W["E"], W["y"], W["x0"], W["NitMax"]

@@ -26,6 +26,9 @@
  *
  * `bfista`/`powermethod` (iterativeSolvers/bfista.m, powermethod.m) stay on
  * the host side and call mm_op_apply; their regulariser R is host code.
+ *
+ *   mm_wsnet_*              no counterpart: the learned warm start that
+ *                           produces cgne's x0 argument (cgne.m:1, :143-145)
  */
 #ifndef MATMRI_HIP_H
 #define MATMRI_HIP_H
@@ -46,6 +49,7 @@ extern "C" {
 
 typedef struct mm_ctx mm_ctx;   /* one device, one HIP stream; not thread-safe */
 typedef struct mm_op  mm_op;    /* a linear operator (NUFFT, coils, high-order, SENSE) */
+typedef struct mm_wsnet mm_wsnet; /* learned warm-start network (one image size) */
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* mm_last_error(void);
@@ -126,6 +130,10 @@ typedef struct mm_cg_opts {
     int    stopOnXdifInc;    /* default 0                               */
     double stopThresh;       /* noise discrepancy |A' n|^2 (< 0: off)   */
     int    nseExtraIt;       /* default 4                               */
+    int    x0Fallback;       /* not in cgne.m: with x0 given, a slice whose
+                                initial |b - N x0|^2 exceeds |b|^2 (the
+                                residual of x0 = 0) starts from x0 = 0
+                                instead (default 0)                     */
 } mm_cg_opts;
 
 void mm_cg_opts_default(mm_cg_opts* o);
@@ -140,12 +148,21 @@ int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax,
                   const mm_cg_opts* opt, double* resSqAll, double* xnormAll,
                   double* xdiffAll, int* nit, int* status, int on_device);
 
-/* Measurement hook (no counterpart in the reference): runs the operator's
- * dominant kernel `iters` times back to back on the context stream between
- * two HIP events (device pointers only) and returns the mean milliseconds per
- * launch and the kernel's name. */
+/* Measurement hook (no counterpart in the reference): times the operator's
+ * dominant kernel with HIP events on the context stream (device pointers
+ * only) and returns the mean milliseconds per launch and the kernel's name.
+ * adjoint = 0/1: the forward/adjoint kernel of a high-order SENSE operator,
+ * launched `iters` times back to back; adjoint = 2: `iters` applications of
+ * the normal operator E'(W.*E x) of a NUFFT SENSE operator (x: image, out:
+ * image) with an event after every kernel -- the dominant kernel is the one
+ * with the largest time per application. */
 int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nrep, int iters,
                   double* ms_per_launch, char* kernel_name, int name_len);
+/* Stage `idx` of the last adjoint = 2 profile of `op`: its mean
+ * milliseconds per launch, launches per application and kernel name;
+ * MM_ERR_ARG past the last stage. */
+int mm_op_profile_stage(mm_op* op, int idx, double* ms_per_launch, int* launches, char* kernel_name,
+                        int name_len);
 /* Tuning knob (no counterpart in the reference): "hio_variant" (0..3: bit 0
  * phase on FP64 MFMA, bit 1 hardware sine/cosine) and "hio_mfma" (0/1) of a
  * high-order operator or a SENSE operator built on one. */
@@ -156,8 +173,38 @@ void* mm_ctx_stream(mm_ctx* ctx);
 /* Enable (1, default) or disable (0) hipGraph capture of the regular cgne
  * iteration for operator E. */
 int mm_cg_set_graph(mm_op* E, int use_graph);
+/* Per slice (nslice ints): 1 where the last mm_cgne_solve on E replaced x0
+ * by 0 (mm_cg_opts.x0Fallback), else 0. */
+int mm_cg_last_fallback(mm_op* E, int* fellback);
 
 int mm_op_destroy(mm_op* op);
+
+/* ---- learned warm start (no counterpart in the reference) ---------------- */
+/* A 6-layer 3x3 convolutional network (2 -> 32 -> 32 -> 32 -> 32 -> 32 -> 2
+ * channels, ReLU between layers, residual skip) on complex N1 x N2 images
+ * (column-major, c64), computing x0 = x + s * net(x / s) with
+ * s = sqrt(mean |x|^2) per image; bf16 weights, FP32 accumulation.  Its input
+ * is the density-compensated adjoint image R'F'(dcf .* y); its output is the
+ * x0 passed to mm_cgne_solve.  Parameters are a flat FP32 vector: for each
+ * layer W[cout][cin][3][3] then b[cout] (38178 values).  A new network has
+ * all parameters zero, i.e. x0 = x. */
+int mm_wsnet_create(mm_ctx* ctx, int64_t N1, int64_t N2, mm_wsnet** out);
+int mm_wsnet_num_params(const mm_wsnet* net, int64_t* n);
+/* Sets the FP32 master parameters and resets the optimiser state. */
+int mm_wsnet_set_params(mm_wsnet* net, const float* theta, int on_device);
+int mm_wsnet_get_params(mm_wsnet* net, float* theta, int on_device);
+/* y = network(x) for `batch` images [batch][N1*N2]. */
+int mm_wsnet_forward(mm_wsnet* net, const void* x, void* y, int64_t batch, int on_device);
+/* Training loss mean((x/s + z - t/s)^2) over `batch` (input, target) image
+ * pairs and its gradient w.r.t. the parameters (`grad`: host or device per
+ * on_device, or NULL; `loss` may be NULL). */
+int mm_wsnet_loss_grad(mm_wsnet* net, const void* x, const void* target, int64_t batch,
+                       double* loss, float* grad, int on_device);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that gradient.  With
+ * loss == NULL the call does not synchronise with the device. */
+int mm_wsnet_train_step(mm_wsnet* net, const void* x, const void* target, int64_t batch,
+                        float lr, double* loss, int on_device);
+int mm_wsnet_destroy(mm_wsnet* net);
 
 #ifdef __cplusplus
 }

@@ -11,5 +11,15 @@ from .operators import (LinearOp, mrSampFunc, mrSampHandle, nufftOp, rcvrOp,  # 
 from . import workloads  # noqa: F401
 from .solvers import bfista, cgne, powermethod, set_cg_graph, softthresh  # noqa: F401
 
+
+def __getattr__(name):
+    # the warm-start module is loaded on first use (it is also runnable with
+    # ``python -m <package>.warmstart``)
+    if name in ("warmstart", "WarmStartNet"):
+        import importlib
+        mod = importlib.import_module(__name__ + ".warmstart")
+        return mod if name == "warmstart" else mod.WarmStartNet
+    raise AttributeError(name)
+
 __all__ = ["nufftOp", "rcvrOp", "sampHighOrder", "senseOp", "mrSampFunc", "mrSampHandle", "cgne",
-           "bfista", "powermethod", "softthresh", "set_cg_graph", "device_count", "lib"]
+           "bfista", "powermethod", "softthresh", "set_cg_graph", "device_count", "lib", "WarmStartNet"]

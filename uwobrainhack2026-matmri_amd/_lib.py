@@ -19,7 +19,10 @@ EXPORTS = [
     "mm_ctx_sync", "mm_nufft_plan_create", "mm_rcvr_plan_create", "mm_rcvr_plan_create_multi",
     "mm_hio_plan_create", "mm_hio_set_phi_div", "mm_sense_create", "mm_op_shape", "mm_op_nslice",
     "mm_op_apply", "mm_op_apply_normal", "mm_cg_opts_default", "mm_cgne_solve", "mm_cg_set_graph",
-    "mm_op_profile", "mm_op_set_option", "mm_ctx_stream", "mm_op_destroy",
+    "mm_op_profile", "mm_op_set_option", "mm_ctx_stream", "mm_op_destroy", "mm_cg_last_fallback",
+    "mm_op_profile_stage",
+    "mm_wsnet_create", "mm_wsnet_num_params", "mm_wsnet_set_params", "mm_wsnet_get_params",
+    "mm_wsnet_forward", "mm_wsnet_loss_grad", "mm_wsnet_train_step", "mm_wsnet_destroy",
 ]
 
 
@@ -38,7 +41,7 @@ class CgOpts(C.Structure):
         ("resThresh", C.c_double), ("resThreshRel", C.c_double), ("resChangeThresh", C.c_double),
         ("xChangeThresh", C.c_double), ("expResN", C.c_int), ("stopOnResInc", C.c_int),
         ("resIncThresh", C.c_double), ("stopOnXdifInc", C.c_int), ("stopThresh", C.c_double),
-        ("nseExtraIt", C.c_int),
+        ("nseExtraIt", C.c_int), ("x0Fallback", C.c_int),
     ]
 
 
@@ -77,11 +80,21 @@ def lib():
     L.mm_cg_opts_default.restype = None
     L.mm_cgne_solve.argtypes = [vp, vp, vp, vp, i32, P(CgOpts), vp, vp, vp, vp, vp, i32]
     L.mm_cg_set_graph.argtypes = [vp, i32]
+    L.mm_cg_last_fallback.argtypes = [vp, vp]
     L.mm_op_destroy.argtypes = [vp]
     L.mm_op_profile.argtypes = [vp, i32, vp, vp, i64, i32, P(dbl), C.c_char_p, i32]
     L.mm_op_set_option.argtypes = [vp, C.c_char_p, i32]
+    L.mm_op_profile_stage.argtypes = [vp, i32, P(dbl), P(i32), C.c_char_p, i32]
     L.mm_ctx_stream.argtypes = [vp]
     L.mm_ctx_stream.restype = vp
+    L.mm_wsnet_create.argtypes = [vp, i64, i64, P(vp)]
+    L.mm_wsnet_num_params.argtypes = [vp, P(i64)]
+    L.mm_wsnet_set_params.argtypes = [vp, vp, i32]
+    L.mm_wsnet_get_params.argtypes = [vp, vp, i32]
+    L.mm_wsnet_forward.argtypes = [vp, vp, vp, i64, i32]
+    L.mm_wsnet_loss_grad.argtypes = [vp, vp, vp, i64, P(dbl), vp, i32]
+    L.mm_wsnet_train_step.argtypes = [vp, vp, vp, i64, C.c_float, P(dbl), i32]
+    L.mm_wsnet_destroy.argtypes = [vp]
     _lib = L
     return L
 

@@ -6,6 +6,7 @@
 #include "cg.hpp"
 #include "hio.hpp"
 #include "ops.hpp"
+#include "wsnet.hpp"
 
 using namespace mm;
 
@@ -214,6 +215,7 @@ void mm_cg_opts_default(mm_cg_opts* o) {
     o->stopOnXdifInc = 0;
     o->stopThresh = -1;
     o->nseExtraIt = 4;
+    o->x0Fallback = 0;
 }
 
 int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax, const mm_cg_opts* opt,
@@ -262,6 +264,19 @@ int mm_cg_set_graph(mm_op* E, int use_graph) {
     });
 }
 
+int mm_cg_last_fallback(mm_op* E, int* fellback) {
+    return guarded([&] {
+        if (!E || !fellback) fail(MM_ERR_ARG, "cgne: NULL argument");
+        auto& m = solvers();
+        auto it = m.find(E);
+        const int ns = E->nslice();
+        for (int s = 0; s < ns; ++s) fellback[s] = 0;
+        if (it == m.end()) return;
+        const auto& v = it->second->last_fallback_;
+        for (int s = 0; s < ns && s < (int)v.size(); ++s) fellback[s] = v[s];
+    });
+}
+
 int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nrep, int iters,
                   double* ms_per_launch, char* kernel_name, int name_len) {
     return guarded([&] {
@@ -272,6 +287,22 @@ int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nre
             fail(MM_ERR_UNSUPPORTED, "profile: operator has no dominant-kernel timer");
         if (kernel_name && name_len > 0) {
             std::strncpy(kernel_name, name.c_str(), (size_t)name_len - 1);
+            kernel_name[name_len - 1] = 0;
+        }
+    });
+}
+
+int mm_op_profile_stage(mm_op* op, int idx, double* ms_per_launch, int* launches, char* kernel_name,
+                        int name_len) {
+    return guarded([&] {
+        auto* se = dynamic_cast<SenseOp*>(op);
+        if (!se) fail(MM_ERR_ARG, "profile: not a SENSE operator");
+        if (idx < 0 || idx >= (int)se->stages_.size()) fail(MM_ERR_ARG, "profile: no such stage");
+        const auto& e = se->stages_[(size_t)idx];
+        if (ms_per_launch) *ms_per_launch = e.second.first;
+        if (launches) *launches = e.second.second;
+        if (kernel_name && name_len > 0) {
+            std::strncpy(kernel_name, e.first.c_str(), (size_t)name_len - 1);
             kernel_name[name_len - 1] = 0;
         }
     });
@@ -295,6 +326,133 @@ int mm_op_destroy(mm_op* op) {
         (void)hipStreamSynchronize(op->ctx->stream);
         solvers().erase(op);
         delete op;
+    });
+}
+
+// ---- learned warm start ----------------------------------------------------
+
+int mm_wsnet_create(mm_ctx* ctx, int64_t N1, int64_t N2, mm_wsnet** out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(MM_ERR_ARG, "warm-start net: NULL argument");
+        *out = nullptr;
+        set_device(ctx);
+        *out = new mm_wsnet(ctx, N1, N2);
+    });
+}
+
+int mm_wsnet_num_params(const mm_wsnet* net, int64_t* n) {
+    return guarded([&] {
+        if (!net || !n) fail(MM_ERR_ARG, "warm-start net: NULL argument");
+        *n = net->num_params();
+    });
+}
+
+int mm_wsnet_set_params(mm_wsnet* net, const float* theta, int on_device) {
+    return guarded([&] {
+        if (!net || !theta) fail(MM_ERR_ARG, "warm-start net: NULL argument");
+        set_device(net->ctx);
+        net->set_params(theta, on_device != 0);
+    });
+}
+
+int mm_wsnet_get_params(mm_wsnet* net, float* theta, int on_device) {
+    return guarded([&] {
+        if (!net || !theta) fail(MM_ERR_ARG, "warm-start net: NULL argument");
+        set_device(net->ctx);
+        net->get_params(theta, on_device != 0);
+    });
+}
+
+namespace {
+// host-pointer calls: stage n_img images per pointer in device memory
+struct WsStage {
+    DevBuf<float2> x, t, y;
+};
+}  // namespace
+
+int mm_wsnet_forward(mm_wsnet* net, const void* x, void* y, int64_t batch, int on_device) {
+    return guarded([&] {
+        if (!net || !x || !y) fail(MM_ERR_ARG, "warm-start net: NULL argument");
+        if (batch < 1) fail(MM_ERR_ARG, "warm-start net: batch must be >= 1");
+        mm_ctx* ctx = net->ctx;
+        set_device(ctx);
+        int64_t ni = 0;
+        const float2* dx = (const float2*)x;
+        float2* dy = (float2*)y;
+        WsStage st;
+        if (!on_device) {
+            ni = batch * net->n_pixels();
+            st.x.upload((const float2*)x, (size_t)ni, ctx->stream);
+            st.y.alloc((size_t)ni);
+            dx = st.x.p;
+            dy = st.y.p;
+        }
+        net->forward(dx, dy, batch);
+        if (!on_device) {
+            HIPCHECK(hipMemcpyAsync(y, dy, sizeof(float2) * ni, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHECK(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+int mm_wsnet_loss_grad(mm_wsnet* net, const void* x, const void* target, int64_t batch, double* loss,
+                       float* grad, int on_device) {
+    return guarded([&] {
+        if (!net || !x || !target) fail(MM_ERR_ARG, "warm-start net: NULL argument");
+        if (batch < 1) fail(MM_ERR_ARG, "warm-start net: batch must be >= 1");
+        mm_ctx* ctx = net->ctx;
+        set_device(ctx);
+        const float2* dx = (const float2*)x;
+        const float2* dt = (const float2*)target;
+        WsStage st;
+        if (!on_device) {
+            const int64_t ni = batch * net->n_pixels();
+            st.x.upload((const float2*)x, (size_t)ni, ctx->stream);
+            st.t.upload((const float2*)target, (size_t)ni, ctx->stream);
+            dx = st.x.p;
+            dt = st.t.p;
+        }
+        const double L = net->loss_grad(dx, dt, batch, loss != nullptr || !on_device);
+        if (loss) *loss = L;
+        if (grad) {
+            HIPCHECK(hipMemcpyAsync(grad, net->grad(), sizeof(float) * net->num_params(),
+                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (!on_device) HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mm_wsnet_train_step(mm_wsnet* net, const void* x, const void* target, int64_t batch, float lr,
+                        double* loss, int on_device) {
+    return guarded([&] {
+        if (!net || !x || !target) fail(MM_ERR_ARG, "warm-start net: NULL argument");
+        if (batch < 1) fail(MM_ERR_ARG, "warm-start net: batch must be >= 1");
+        if (!(lr > 0.f)) fail(MM_ERR_ARG, "warm-start net: learning rate must be > 0");
+        mm_ctx* ctx = net->ctx;
+        set_device(ctx);
+        const float2* dx = (const float2*)x;
+        const float2* dt = (const float2*)target;
+        WsStage st;
+        if (!on_device) {
+            const int64_t ni = batch * net->n_pixels();
+            st.x.upload((const float2*)x, (size_t)ni, ctx->stream);
+            st.t.upload((const float2*)target, (size_t)ni, ctx->stream);
+            dx = st.x.p;
+            dt = st.t.p;
+        }
+        const double L = net->loss_grad(dx, dt, batch, loss != nullptr);
+        net->adam_step(lr);
+        if (loss) *loss = L;
+        if (!on_device) HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mm_wsnet_destroy(mm_wsnet* net) {
+    return guarded([&] {
+        if (!net) return;
+        (void)hipSetDevice(net->ctx->device);
+        (void)hipStreamSynchronize(net->ctx->stream);
+        delete net;
     });
 }
 

@@ -23,7 +23,7 @@ struct CgScal {
     double stopThresh;           // < 0: off
     double resThresh;            // effective absolute threshold (< 0: off)
     int nit, nitmax, done, status;
-    int resIncs, xdifIncs, lastXdifInc, pad;
+    int resIncs, xdifIncs, lastXdifInc, fallback;  // fallback: x0 replaced by 0
 };
 
 struct CgSeg {
@@ -217,7 +217,7 @@ __global__ void cg_init(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                         const double2* __restrict__ part_b, const double2* __restrict__ part_r,
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
                         int nslice, int nitmax, double resThresh, double resThreshRel,
-                        double stopThresh) {
+                        double stopThresh, int x0fallback) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslice) return;
     const CgSeg g = seg[s];
@@ -228,6 +228,13 @@ __global__ void cg_init(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
         rr += part_r[2 * (g.blk0 + b)].x;
     }
     CgScal c{};
+    // warm-start fallback: a start worse than x0 = 0 (whose residual is b)
+    // is replaced by x0 = 0 (cg_fallback_k resets x, r and p)
+    if (x0fallback && rr > bb) {
+        c.fallback = 1;
+        rr = bb;
+        xx = 0;
+    }
     c.rho_re = rr;
     c.rho_im = 0;
     c.nit = 0;
@@ -242,6 +249,25 @@ __global__ void cg_init(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
     hres[h] = rr;
     hxn[h] = xx;
     hxd[h] = xx;
+}
+
+// x = 0, r = p = b for the slices cg_init marked as falling back
+__global__ __launch_bounds__(kCgBlock) void cg_fallback_k(const CgSeg* __restrict__ seg,
+                                                          const int* __restrict__ b2s,
+                                                          const CgScal* __restrict__ sc,
+                                                          const float2* __restrict__ b, float2* __restrict__ x,
+                                                          float2* __restrict__ r, float2* __restrict__ p) {
+    const int s = find_slice(seg, b2s);
+    if (!sc[s].fallback) return;
+    const CgSeg g = seg[s];
+    const int bl = blockIdx.x - g.blk0;
+    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
+        const int64_t k = g.off + i;
+        const float2 bv = b[k];
+        x[k] = make_float2(0.f, 0.f);
+        r[k] = bv;
+        p[k] = bv;
+    }
 }
 
 // res_sq_new, histories, stopping tests and beta (cgne.m:252-335).
@@ -452,8 +478,14 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p, q_.p,
                        r_.p, p_.p, part_.p);
     hipLaunchKernelGGL(cg_init, dim3(sb), dim3(64), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
-                       hxn_.p, hxd_.p, ns, NitMax, o.resThresh, o.resThreshRel, o.stopThresh);
+                       hxn_.p, hxd_.p, ns, NitMax, o.resThresh, o.resThreshRel, o.stopThresh,
+                       (x0 && o.x0Fallback) ? 1 : 0);
     LAUNCHCHECK();
+    if (x0 && o.x0Fallback) {
+        hipLaunchKernelGGL(cg_fallback_k, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p, x,
+                           r_.p, p_.p);
+        LAUNCHCHECK();
+    }
     const int expResN = o.expResN > 0 ? o.expResN : 100;
     const int poll = 4;
     for (int it = 0; it < NitMax; ++it) {
@@ -487,7 +519,9 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     HIPCHECK(hipMemcpyAsync(b.data(), hxn_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(c.data(), hxd_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    last_fallback_.assign(ns, 0);
     for (int s = 0; s < ns; ++s) {
+        last_fallback_[s] = hs[s].fallback;
         if (nit) nit[s] = hs[s].nit;
         if (status) status[s] = hs[s].status;
     }

@@ -15,6 +15,8 @@ class CgSolver {
     void solve(const float2* y, const float2* x0, float2* x, int NitMax, const mm_cg_opts& o,
                double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status);
     bool use_graph_ = true;
+    // per slice: 1 if the last solve replaced x0 by 0 (mm_cg_opts.x0Fallback)
+    std::vector<int> last_fallback_;
 
    private:
     void launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x);

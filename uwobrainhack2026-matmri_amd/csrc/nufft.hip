@@ -1160,10 +1160,12 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         hipLaunchKernelGGL(nu_fwd_rows<BCT>, dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
                            twid_.p, deap_.p, x, c.ws.p, r);
         LAUNCHCHECK();
+        mark("nu_fwd_rows");
         if (ndim_ == 2 && c.n_cols > 0) {
             hipLaunchKernelGGL(nu_cols<BCT>, dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
                                twid_.p, c.ws.p, 0);
             LAUNCHCHECK();
+            mark("nu_cols");
         }
         if (wt4)
             hipLaunchKernelGGL((nu_degrid<BCT, 4>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, sbase_.p,
@@ -1172,6 +1174,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
             hipLaunchKernelGGL((nu_degrid<BCT, 0>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, sbase_.p,
                                swt_.p, sdcf_.p, (const float2*)c.ws.p, y, r);
         LAUNCHCHECK();
+        mark("nu_degrid");
     };
     auto adjoint_from = [&](const float2* y, float2* x, NuRun r) {
         if (wt4)
@@ -1183,14 +1186,17 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                tlist_.p, gwork_.p, glist_.p, gbin_.p, sbase_.p, swt_.p, sdcf_.p, y, c.ws.p,
                                c.parts.p, c.counters.p, r);
         LAUNCHCHECK();
+        mark("nu_grid");
         if (ndim_ == 2 && c.n_cols > 0) {
             hipLaunchKernelGGL(nu_cols<BCT>, dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
                                twid_.p, c.ws.p, 1);
             LAUNCHCHECK();
+            mark("nu_cols");
         }
         hipLaunchKernelGGL(nu_adj_rows<BCT>, dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
                            twid_.p, deap_.p, (const float2*)c.ws.p, x, r);
         LAUNCHCHECK();
+        mark("nu_adj_rows");
     };
     if (mode == 1) {
         NuRun rf = run;
@@ -1215,6 +1221,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         ra.ylayout = 0;
         hipLaunchKernelGGL(nu_transpose_y<BCT>, dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, in, c.yT.p, ra);
         LAUNCHCHECK();
+        mark("nu_transpose_y");
         ra.ylayout = 1;
         adjoint_from(c.yT.p, out, ra);
     } else {
@@ -1257,6 +1264,57 @@ void NufftOp::apply_sense_normal(const float2* in, float2* out, int64_t nrep, co
                                  const float* Wt) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
     run(config(nrep, 1, C), 0, in, out, maps, Wt, 1);
+}
+
+void NufftOp::mark(const char* name) {
+    if (!prof_) return;
+    Prof& p = *prof_;
+    if (p.k + 1 >= (int)p.ev.size()) return;
+    HIPCHECK(hipEventRecord(p.ev[p.k + 1], ctx->stream));
+    p.names[p.k] = name;
+    ++p.k;
+}
+
+void NufftOp::profile_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
+                             const float* Wt, int iters,
+                             std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel) {
+    if (nrep < 1 || iters < 1) fail(MM_ERR_ARG, "profile: bad argument");
+    Cfg& c = config(nrep, maps ? 1 : 0, maps ? C : 0);
+    // one untimed application (plan set-up, first-launch costs)
+    run(c, 0, in, out, maps, Wt, 1);
+    Prof p;
+    p.ev.resize(16);
+    p.names.resize(16);
+    for (auto& e : p.ev) HIPCHECK(hipEventCreate(&e));
+    std::map<std::string, std::pair<double, int>> acc;
+    int launches_per_app = 0;
+    try {
+        for (int it = 0; it < iters; ++it) {
+            p.k = 0;
+            HIPCHECK(hipEventRecord(p.ev[0], ctx->stream));
+            prof_ = &p;
+            run(c, 0, in, out, maps, Wt, 1);
+            prof_ = nullptr;
+            HIPCHECK(hipEventSynchronize(p.ev[p.k]));
+            launches_per_app = p.k;
+            for (int j = 0; j < p.k; ++j) {
+                float ms = 0.f;
+                HIPCHECK(hipEventElapsedTime(&ms, p.ev[j], p.ev[j + 1]));
+                auto& a = acc[p.names[j]];
+                a.first += ms;
+                a.second += 1;
+            }
+        }
+    } catch (...) {
+        prof_ = nullptr;
+        for (auto& e : p.ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : p.ev) (void)hipEventDestroy(e);
+    (void)launches_per_app;
+    per_kernel->clear();
+    for (auto& kv : acc)
+        per_kernel->push_back({kv.first, {kv.second.first / kv.second.second, kv.second.second / iters}});
 }
 
 }  // namespace mm

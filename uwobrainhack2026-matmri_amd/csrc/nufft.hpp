@@ -73,6 +73,12 @@ class NufftOp : public mm_op {
     // the transposed [M][coil] layout between the two halves.
     void apply_sense_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                             const float* W);
+    // Per-kernel time of the normal operator E'(W.*E x): `iters`
+    // applications on the context stream with a HIP event after every
+    // kernel; per kernel name: (ms per launch, launches per application).
+    void profile_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
+                        const float* W, int iters,
+                        std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
     bool gather() const { return gather_; }
     int ndim() const { return ndim_; }
     const std::vector<int64_t>& slice_V() const { return Vs_; }
@@ -104,6 +110,14 @@ class NufftOp : public mm_op {
              int mode = 0);
     template <int BCT>
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
+    // event marks for profile_normal (inactive unless prof_ is set)
+    struct Prof {
+        std::vector<hipEvent_t> ev;
+        std::vector<const char*> names;
+        int k = 0;
+    };
+    Prof* prof_ = nullptr;
+    void mark(const char* name);
 
     int ndim_ = 2;
     int W_ = 4;

@@ -145,7 +145,23 @@ void SenseOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
 bool SenseOp::profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
                                double* ms, std::string* name) {
     RcvrOp* rc = static_cast<RcvrOp*>(R_);
-    if (hio_) return hio_->profile_sense(adjoint, in, out, nrep, rc->maps(), C_, iters, ms, name);
+    if (hio_ && adjoint != 2) return hio_->profile_sense(adjoint, in, out, nrep, rc->maps(), C_, iters, ms, name);
+    if (nu_ && adjoint == 2) {
+        // the dominant kernel of the normal operator cgne iterates
+        std::vector<std::pair<std::string, std::pair<double, int>>> k;
+        nu_->profile_normal(in, out, nrep, rc->maps(), C_, W, iters, &k);
+        double best = -1;
+        for (auto& e : k) {
+            const double tot = e.second.first * e.second.second;
+            if (tot > best) {
+                best = tot;
+                *ms = e.second.first;
+                *name = e.first;
+            }
+        }
+        stages_ = k;
+        return best >= 0;
+    }
     return false;
 }
 

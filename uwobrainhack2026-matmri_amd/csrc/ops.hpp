@@ -41,6 +41,8 @@ class SenseOp : public mm_op {
     bool set_option(const std::string& key, int value) override { return S_->set_option(key, value); }
     mm_op* sampling() const { return S_; }
     int coils() const { return C_; }
+    // per-kernel times of the last normal-operator profile
+    std::vector<std::pair<std::string, std::pair<double, int>>> stages_;
 
    private:
     void apply_impl(int adjoint, const float2* in, float2* out, int64_t nrep, const float* Wf);

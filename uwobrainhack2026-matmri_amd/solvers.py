@@ -46,6 +46,7 @@ def _cg_opts(opt, NitMax):
     o.resIncThresh = float(opt.get("resIncThresh", 1.0))
     o.stopOnXdifInc = int(opt.get("stopOnXdifInc", 0))
     o.nseExtraIt = int(opt.get("nseExtraIt", 4))
+    o.x0Fallback = int(bool(opt.get("x0Fallback", 0)))
     o.stopThresh = -1.0
     return o
 
