@@ -38,7 +38,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP16_MFMA_TFLOPS = 2500.0   # MI355X dense fp16 matrix peak (the high-order GEMM runs on it)
 PEAK_HBM_GBS = 8000.0
 
 
@@ -47,7 +47,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch"], default="warmstart")
+    p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch", "copy"], default="warmstart",
+                   help="copy: a plain 1 GiB device copy (HBM counter calibration, not a solve)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--hio-variant", type=int, default=None, help="high-order MFMA kernel variant 0..3")
@@ -242,6 +243,34 @@ def read_traffic(kernel, workload):
     return None
 
 
+def oracle_nrmse_warmstart(fam, host, prob, xw, xz, nprob=3):
+    """NRMSE of the device warm-start and zero-init solutions (rel. resid
+    1e-4) against the FP64 oracle solution run far past it (untimed)."""
+    import numpy as np
+    from oracle.mrsamp import mrSampFunc
+    from oracle.nufft import NufftPlan
+    from oracle.rcvr import RcvrPlan
+    from oracle.solvers import cgne as o_cgne
+    N, M, C = fam.p["N"], fam.p["M"], fam.p["C"]
+    So = NufftPlan([N, N], fam.kloc, None, fam.p["os"], fam.p["kwidth"])
+    Y = prob["y"].reshape(fam.batch, C, M)[:nprob].cpu().numpy().astype(np.complex128)
+    XW = xw.reshape(fam.batch, N, N)[:nprob].cpu().numpy()
+    XZ = xz.reshape(fam.batch, N, N)[:nprob].cpu().numpy()
+    ew, ez = [], []
+    for n in range(nprob):
+        Ro = RcvrPlan(host[0][n])
+        A = lambda x, tr: mrSampFunc(x, tr, So, Ro)
+        y = Y[n].T
+        b = A(y, "transp")
+        xo, *_ = o_cgne(A, y, np.zeros((N, N), complex), 150,
+                        {"resThresh": 1e-14 * float(np.real(np.vdot(b, b))), "xChangeThresh": None})
+        no = np.linalg.norm(xo)
+        ew.append(float(np.linalg.norm(XW[n].T - xo) / no))
+        ez.append(float(np.linalg.norm(XZ[n].T - xo) / no))
+    return {"warm": float(np.mean(ew)), "zero": float(np.mean(ez)), "problems": nprob,
+            "reference": "FP64 NumPy oracle cgne, zero init, 150 iterations (|r|/|b| ~ 1e-5)"}
+
+
 def cpu_baseline_warmstart(fam, host, prob, seconds):
     """FP64 NumPy oracle zero-init cgne to relative residual 1e-4 on as many
     of the held-out problems as fit the time budget (the reference CPU
@@ -313,7 +342,7 @@ def run_warmstart(a, mm, torch, dev):
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         nits = np.concatenate(nits)
-        res[name] = dict(el=el, nit=nits, fallbacks=fbs, status=np.asarray(st))
+        res[name] = dict(el=el, nit=nits, fallbacks=fbs, status=np.asarray(st), x=x)
     # relative residuals against |b| (the zero-init history starts at |b|^2)
     _, _, _, _, hz = zero_step()
     b2 = np.array([h[0] for h in hz])
@@ -379,10 +408,34 @@ def run_warmstart(a, mm, torch, dev):
                                "ms_per_launch": ms.value, "algorithmic_bytes_per_launch": byts}
         out["normal_operator_stages"] = stages
     if not a.no_cpu:
+        out["nrmse_vs_fp64_oracle"] = oracle_nrmse_warmstart(fam, host, prob, zw["x"], zz["x"])
         cb = cpu_baseline_warmstart(fam, host, prob, a.cpu_seconds)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
     print(json.dumps(out), flush=True)
+
+
+def run_copy(a, torch, dev):
+    """Plain device-to-device copy of 1 GiB: the known-size reference that
+    scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of the profiles."""
+    n = 1 << 28
+    x = torch.empty(n, dtype=torch.float32, device=dev).uniform_()
+    y = torch.empty_like(x)
+    for _ in range(max(1, a.warmup)):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nb = 4 * n
+    print(json.dumps({"metric": "device copy bandwidth (HBM counter calibration)", "value": 2 * nb * a.steps / el / 1e9,
+                      "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                      "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "copy: 1 GiB device-to-device", "bytes_read": nb,
+                                 "bytes_written": nb}}), flush=True)
 
 
 def main():
@@ -397,6 +450,10 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
     mm._lib.context(0, stream.cuda_stream)
+    if a.workload == "copy":
+        with torch.cuda.stream(stream):
+            run_copy(a, torch, dev)
+        return
     if a.workload == "warmstart":
         with torch.cuda.stream(stream):
             run_warmstart(a, mm, torch, dev)
@@ -439,10 +496,15 @@ def main():
            "status_last": st if W["kind"] != "batch" else {str(k): st.count(k) for k in set(st)}}
     if kms:
         if W["kind"] == "highorder":
+            # algorithmic flops of one application (phase contraction + complex
+            # MAC); the kernel runs them on fp16 MFMA with split operands, which
+            # issues three fp16 products per FP32-accurate product
             ach = W["flops"] / (kms * 1e-3) / 1e12
-            out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                               "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
-                               "kernel": name.value.decode(), "ms_per_launch": kms}
+            out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                               "frac": ach / PEAK_FP16_MFMA_TFLOPS,
+                               "traffic": read_traffic(name.value.decode(), "highorder"),
+                               "kernel": name.value.decode(), "ms_per_launch": kms,
+                               "mfma_products_per_flop": 3}
     if not a.no_cpu:
         cb = {"highorder": cpu_baseline_highorder, "nufft": cpu_baseline_nufft,
               "batch": cpu_baseline_batch}[W["kind"]](W["P"], a.cpu_seconds)

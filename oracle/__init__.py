@@ -17,9 +17,9 @@ repository root):
 * ``mrsamp``     -- ``operators/mrSampFunc.m:82-159``
 * ``solvers``    -- ``iterativeSolvers/cgne.m``, ``bfista.m``,
   ``powermethod.m``, ``general/softthresh.m``
+* ``warmstart``  -- the learned warm-start network (no reference counterpart;
+  defines the function the device network computes)
 
 How each function is pinned (known-answer tests, closed forms, invariants) is
 listed in the docstring of each module and exercised by ``tests/test_oracle_*``.
-The compiled C restatement of the high-order operator and of ``cgne`` lives in
-``oracle/c/`` (used as the multi-threaded CPU baseline).
 """

@@ -206,7 +206,7 @@ def test_trained_warm_start_gain(mm):
     spiral SENSE problems) against zero-init cgne on 100 held-out problems:
     fewer iterations to relative residual 1e-4 (1.6x measured on 1000), no
     fallback, and -- on one problem -- a solution as close to the converged
-    FP64 oracle solution as the zero-init one."""
+    FP64 oracle solution as the zero-init one (NRMSE 0.127 vs 0.133 measured)."""
     import torch
     from oracle.nufft import NufftPlan
     from oracle.rcvr import RcvrPlan
@@ -241,5 +241,8 @@ def test_trained_warm_start_gain(mm):
     ez = np.linalg.norm(xz.reshape(100, N, N)[0].cpu().numpy().T - xo) / xo_n
     print(f"  NRMSE vs converged FP64 oracle: warm {ew:.3e} zero {ez:.3e} "
           f"(oracle rel resid {np.sqrt(np.real(ro[-1]) / np.real(ro[0])):.1e})", flush=True)
-    assert ew < 3e-2 and ez < 3e-2
-    assert ew < 1.5 * ez + 2e-3
+    # at relative residual 1e-4 both are still ~13% from the fully converged
+    # (noise-amplifying, unregularised) solution; the warm start must match
+    # the zero-init solve there, not degrade it
+    assert ez < 0.3
+    assert ew <= 1.1 * ez + 1e-3
