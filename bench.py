@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--hio-variant", type=int, default=None, help="high-order MFMA kernel variant 0..3")
+    p.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                   help="tuning option of the sampling operator (mm_op_set_option), e.g. nu_degrid_t=0")
     return p.parse_args()
 
 
@@ -312,6 +314,9 @@ def run_warmstart(a, mm, torch, dev):
     net = mm.WarmStartNet.load()
     P = 100
     fam = ws.ProblemFamily(P)
+    for kv in a.set:
+        k, v = kv.split("=")
+        fam.S.set_option(k, int(v))
     N, M, Cc = fam.p["N"], fam.p["M"], fam.p["C"]
     seeds = list(range(ws.TRAIN_SEED0 + ws.HELDOUT_OFFSET, ws.TRAIN_SEED0 + ws.HELDOUT_OFFSET + P))
     host = fam.host_batch(seeds)
@@ -463,6 +468,9 @@ def main():
         E, y, x0, NitMax = W["E"], W["y"], W["x0"], W["NitMax"]
         if a.hio_variant is not None and W["kind"] == "highorder":
             E.set_option("hio_variant", a.hio_variant)
+        for kv in a.set:
+            k, v = kv.split("=")
+            W["S"].set_option(k, int(v))
         opt = {"resThresh": None, "resThreshRel": 1e-8, "xChangeThresh": None}
         b = E.H * y
         bb = float(torch.sum(torch.abs(b.reshape(-1).to(torch.complex128)) ** 2))

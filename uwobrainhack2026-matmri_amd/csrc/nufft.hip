@@ -22,9 +22,13 @@
 //   nu_fwd_rows   coil multiply + deapod + wrap/zero-pad + FFT along g1
 //   nu_cols       FFT along g2 (fwd: zero-pad from N2 rows; adj: crop to N2)
 //   nu_degrid     gather of W^2 taps per sample (dcf, optional weights fused)
-//   nu_grid       LDS-tiled gridding; a tile is owned by one work-group and
-//                 written once, heavy tiles are split and merged by the
-//                 last-arriving work-group (no global float atomics)
+//   nu_grid_csr   2-D gridding as an output-stationary gather: one lane per
+//                 (cell, transform) sums, in sample order, the CSR row of
+//                 (sample, w1*w2*dcf) pairs that reach the cell -- every cell
+//                 written once, no atomics, deterministic
+//   nu_grid       1-D gridding: LDS-tiled scatter (float atomics in LDS),
+//                 heavy tiles split and merged by the last-arriving
+//                 work-group
 //   nu_adj_rows   FFT along g1 + crop + deapod + conj-coil combine
 #include <algorithm>
 #include <cmath>
@@ -293,6 +297,67 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ wt, int W
     }
 }
 
+// Degridding with BCT lanes per sample (one transform each): a load
+// instruction reads BCT*8 contiguous bytes for each of 64/BCT samples, each
+// lane keeps one accumulator.  Taps are summed in the same order as
+// nu_degrid.  Work item = (slice, chunk, first sample) covering 256 samples.
+template <int BCT, int WT>
+__global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict__ sl,
+                                                   const int4* __restrict__ work,
+                                                   const uint32_t* __restrict__ sbase,
+                                                   const float* __restrict__ swt,
+                                                   const float* __restrict__ sdcf,
+                                                   const float2* __restrict__ ws,
+                                                   float2* __restrict__ y, NuRun run) {
+    constexpr int SPP = 256 / BCT;
+    const int4 wk = work[blockIdx.x];
+    const NuSliceDev& S = sl[wk.x];
+    const int ch = wk.y;
+    const int b = threadIdx.x & (BCT - 1);
+    const int G1 = S.G1, G2 = S.G2;
+    const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2;
+    const int lanes = lanes_of(run, BCT, ch);
+    const int64_t stride = chunk_stride(run, BCT);
+    const float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b;
+    const int mend = min(S.M, wk.z + 256);
+    for (int m = wk.z + (int)(threadIdx.x / BCT); m < mend; m += SPP) {
+        const int64_t sm = S.samp_off + m;
+        const uint32_t base = sbase[sm];
+        const int g1_0 = base & 0xffff, g2_0 = base >> 16;
+        float w1[8], w2[8];
+        load_weights<WT>(swt + sm * (W1 + W2), W1, W2, w1, w2);
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int s2 = 0; s2 < (WT > 0 ? WT : 8); ++s2) {
+            if (s2 >= W2) break;
+            int g2 = g2_0 + s2;
+            if (g2 >= G2) g2 -= G2;
+            const float2* zrow = zg + (int64_t)g2 * G1 * BCT;
+#pragma unroll
+            for (int s1 = 0; s1 < (WT > 0 ? WT : 8); ++s1) {
+                if (s1 >= W1) break;
+                int g1 = g1_0 + s1;
+                if (g1 >= G1) g1 -= G1;
+                const float w = w1[s1] * w2[s2];
+                const float2 q = zrow[(int64_t)g1 * BCT];
+                acc.x += w * q.x;
+                acc.y += w * q.y;
+            }
+        }
+        const float d = sdcf[sm];
+        if (run.ylayout) {
+            float f = b < lanes ? d : 0.f;
+            if (run.W && b < lanes) f *= run.W[S.dat_off + ((int64_t)ch * stride + b) * S.M + m];
+            y[S.yt_off + ((int64_t)ch * S.M + m) * BCT + b] = make_float2(acc.x * f, acc.y * f);
+        } else if (b < lanes) {
+            const int64_t o = S.dat_off + ((int64_t)ch * stride + b) * S.M + m;
+            float f = d;
+            if (run.W) f *= run.W[o];
+            y[o] = make_float2(acc.x * f, acc.y * f);
+        }
+    }
+}
+
 template <int BCT, int WT>
 __global__ __launch_bounds__(256, 2) void nu_degrid(const NuSliceDev* __restrict__ sl,
                                                  const int4* __restrict__ work,
@@ -401,6 +466,8 @@ __global__ __launch_bounds__(256) void nu_transpose_y(const NuSliceDev* __restri
 // side is read in the transposed layout, so the BCT lanes of a cell read one
 // contiguous BCT*8-byte row per sample.
 constexpr int kMaxBins = (kTile + 7) * (kTile + 7) + 1;
+// CSR gridding: entries per work item (lanes x entries stays balanced)
+constexpr int kCsrItemEntries = 2048;
 
 // entries of a gathered tile that fit the LDS staging area (32 KB)
 template <int BCT>
@@ -671,6 +738,86 @@ __device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, c
     }
 }
 
+// CSR gridding: work item = (slice, chunk, first cell, cell count | log2(P)
+// << 24).  Lanes are (cell, part, transform) triples, BCT consecutive lanes
+// per (cell, part), so the BCT values a sample holds for the chunk are one
+// contiguous read of the transposed data [M][BCT].  With P > 1 (rows of the
+// dense k-space centre) each of the P parts sums a contiguous 1/P of the
+// row and the parts are added in order through LDS; every cell's sum has a
+// fixed order for a given plan.
+template <int BCT>
+__global__ __launch_bounds__(256) void nu_grid_csr(const NuSliceDev* __restrict__ sl,
+                                                   const int4* __restrict__ work,
+                                                   const int* __restrict__ rowptr,
+                                                   const int2* __restrict__ ent,
+                                                   const float2* __restrict__ yt, float2* __restrict__ ws,
+                                                   NuRun run) {
+    constexpr int NG = 256 / BCT;  // lane groups
+    __shared__ float2 red[256];
+    const int4 wk = work[blockIdx.x];
+    const NuSliceDev& S = sl[wk.x];
+    const int ch = wk.y;
+    const int lp = wk.w >> 24;
+    const int P = 1 << lp;
+    const int cpi = NG >> lp;      // cells per pass
+    const int b = threadIdx.x & (BCT - 1);
+    const int grp = threadIdx.x / BCT;
+    const int part = grp & (P - 1);
+    const int cl = grp >> lp;
+    const int lanes = lanes_of(run, BCT, ch);
+    const int* rp = rowptr + S.csr_off;
+    const int2* en = ent + S.ent_off;
+    const float2* ych = yt + S.yt_off + (int64_t)ch * S.M * BCT + b;
+    float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b;
+    const int cend = wk.z + (wk.w & 0xFFFFFF);
+    for (int cbase = wk.z; cbase < cend; cbase += cpi) {
+        const int cell = cbase + cl;
+        float2 acc = make_float2(0.f, 0.f);
+        if (cell < cend && b < lanes) {
+            const int r0 = rp[cell], len = rp[cell + 1] - r0;
+            int e = r0 + (int)(((int64_t)len * part) >> lp);
+            const int e1 = r0 + (int)(((int64_t)len * (part + 1)) >> lp);
+            for (; e + 4 <= e1; e += 4) {
+                int2 q[4];
+                float2 v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) q[i] = en[e + i];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = ych[(int64_t)q[i].x * BCT];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float w = __int_as_float(q[i].y);
+                    acc.x = fmaf(w, v[i].x, acc.x);
+                    acc.y = fmaf(w, v[i].y, acc.y);
+                }
+            }
+            for (; e < e1; ++e) {
+                const int2 q = en[e];
+                const float2 v = ych[(int64_t)q.x * BCT];
+                const float w = __int_as_float(q.y);
+                acc.x = fmaf(w, v.x, acc.x);
+                acc.y = fmaf(w, v.y, acc.y);
+            }
+        }
+        if (P == 1) {
+            if (cell < cend) zg[(int64_t)cell * BCT] = acc;
+            continue;
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (part == 0 && cell < cend) {
+            float2 t = acc;
+            for (int q = 1; q < P; ++q) {
+                const float2 u = red[threadIdx.x + q * BCT];
+                t.x += u.x;
+                t.y += u.y;
+            }
+            zg[(int64_t)cell * BCT] = t;
+        }
+        __syncthreads();
+    }
+}
+
 // Gridding: one launch over two kinds of work items (int4.w): 1 = a chunk of
 // a dense tile (register-run scatter into LDS), 0 = cell rows of a sparse
 // tile (atomic-free gather).  Each tile is handled by one kind only.
@@ -814,6 +961,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     std::vector<uint32_t> sbase;
     std::vector<float> swt, sdcf;
     std::vector<int> tlist, glist_h, gbin_h;
+    std::vector<int2> csr_ent_h;
     int64_t voff = 0, moff = 0;
     for (int s = 0; s < nslice; ++s) {
         NuSliceDev d{};
@@ -885,6 +1033,65 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
             if (ndim == 1) wdst[(size_t)m * Ws + d.W1] = 1.f;
             sbase.push_back((uint32_t)g0[0][m] | ((uint32_t)g0[1][m] << 16));
             sdcf.push_back(dc ? (float)dc[m] : 1.f);
+        }
+        if (ndim == 2) {
+            // CSR gridding: row pointers (per cell) and (sample, weight) entries
+            const int64_t ncell = (int64_t)d.G1 * d.G2;
+            const int64_t nent = (int64_t)d.M * d.W1 * d.W2;
+            if (nent > (int64_t)INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples in one slice");
+            d.csr_off = (int64_t)csr_ptr_host_.size();
+            d.ent_off = (int64_t)csr_ent_h.size();
+            std::vector<int> cnt((size_t)ncell + 1, 0);
+            for (int m = 0; m < d.M; ++m)
+                for (int s2 = 0; s2 < d.W2; ++s2) {
+                    int g2 = g0[1][m] + s2;
+                    g2 -= g2 >= d.G2 ? d.G2 : 0;
+                    for (int s1 = 0; s1 < d.W1; ++s1) {
+                        int g1 = g0[0][m] + s1;
+                        g1 -= g1 >= d.G1 ? d.G1 : 0;
+                        cnt[(size_t)g2 * d.G1 + g1 + 1]++;
+                    }
+                }
+            for (int64_t i = 0; i < ncell; ++i) cnt[(size_t)i + 1] += cnt[(size_t)i];
+            csr_ptr_host_.insert(csr_ptr_host_.end(), cnt.begin(), cnt.end());
+            std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+            const size_t e0 = csr_ent_h.size();
+            csr_ent_h.resize(e0 + (size_t)nent);
+            const float* wrow = swt.data() + (size_t)d.samp_off * Ws;
+            for (int m = 0; m < d.M; ++m) {
+                const double dm = dc ? dc[m] : 1.0;
+                for (int s2 = 0; s2 < d.W2; ++s2) {
+                    int g2 = g0[1][m] + s2;
+                    g2 -= g2 >= d.G2 ? d.G2 : 0;
+                    const double w2 = wrow[(size_t)m * Ws + d.W1 + s2];
+                    for (int s1 = 0; s1 < d.W1; ++s1) {
+                        int g1 = g0[0][m] + s1;
+                        g1 -= g1 >= d.G1 ? d.G1 : 0;
+                        const float w = (float)(wrow[(size_t)m * Ws + s1] * w2 * dm);
+                        int2 q;
+                        q.x = m;
+                        q.y = __builtin_bit_cast(int, w);
+                        csr_ent_h[e0 + (size_t)pos[(size_t)g2 * d.G1 + g1]++] = q;
+                    }
+                }
+            }
+            tlist_off_.push_back((int64_t)tlist.size());
+            d.tlist_off = tlist_off_.back();
+            tw_first_.push_back((int)tw_host_.size());
+            tw_count_.push_back(0);
+            gw_first_.push_back((int)gw_host_.size());
+            gw_count_.push_back(0);
+            nslots_.push_back(0);
+            ncounters_.push_back(0);
+            d.tile_first = tw_first_.back();
+            sl_.push_back(d);
+            Vs_.push_back(n1 * n2);
+            Ms_.push_back(d.M);
+            img_start_.push_back(voff);
+            dat_start_.push_back(moff);
+            voff += n1 * n2;
+            moff += d.M;
+            continue;
         }
         // gridding tiles: each sample joins every tile its footprint touches
         const int nt1 = (d.G1 + kTile - 1) / kTile, nt2 = (d.G2 + kTile - 1) / kTile;
@@ -1029,7 +1236,12 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     tlist_.upload(tlist.empty() ? std::vector<int>{0} : tlist, st);
     tiles_.upload(tw_host_.empty() ? std::vector<TileWork>{TileWork{}} : tw_host_, st);
     // the transposed data layout is used when any tile is gathered
-    gather_ = !gw_host_.empty();
+    csr_ = ndim == 2;
+    gather_ = csr_ || !gw_host_.empty();
+    if (csr_) {
+        csr_ptr_.upload(csr_ptr_host_, st);
+        csr_ent_.upload(csr_ent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : csr_ent_h, st);
+    }
     gwork_.upload(gw_host_.empty() ? std::vector<GatherWork>{GatherWork{}} : gw_host_, st);
     glist_.upload(glist_h.empty() ? std::vector<int>{0} : glist_h, st);
     gbin_.upload(gbin_h.empty() ? std::vector<int>{0} : gbin_h, st);
@@ -1060,8 +1272,8 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     const int ns = (int)sl_.size();
     c.hs = sl_;
     int64_t wsoff = 0, poff = 0, coff = 0, ytoff = 0;
-    std::vector<int4> wr, wc, wd, wg, wga;
-    std::vector<int64_t> wga_cost;
+    std::vector<int4> wr, wc, wd, wg, wga, wcsr;
+    std::vector<int64_t> wga_cost, wcsr_cost;
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
         // rows / columns per work-group: aim for 8-16 interleaved transforms
@@ -1108,7 +1320,56 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
                 wg.push_back(make_int4(s, ch, gw_first_[s] + t, 0));
                 wga_cost.push_back(gw_cost_[gw_first_[s] + t]);
             }
+            if (csr_) {
+                // blocks of NG = 256/BCT cells; a block whose longest row
+                // exceeds 64 entries gets P lane groups per cell (P parts of
+                // each row) and is cut into items of NG/P cells; light blocks
+                // are merged into items of up to 4 blocks and
+                // kCsrItemEntries entries
+                const int NG = 256 / BCT;
+                const int64_t ncell = (int64_t)d.G1 * d.G2;
+                const int* rp = csr_ptr_host_.data() + d.csr_off;
+                auto block_P = [&](int64_t c0) {
+                    const int64_t c1 = std::min<int64_t>(ncell, c0 + NG);
+                    int mx = 0;
+                    for (int64_t q = c0; q < c1; ++q) mx = std::max(mx, rp[q + 1] - rp[q]);
+                    int lp = 0;
+                    while ((1 << lp) < NG && mx > 64 * (1 << lp)) ++lp;
+                    return lp;
+                };
+                int64_t c0 = 0;
+                while (c0 < ncell) {
+                    const int lp = block_P(c0);
+                    if (lp > 0) {
+                        const int64_t c1 = std::min<int64_t>(ncell, c0 + NG);
+                        const int cpi = NG >> lp;
+                        for (int64_t q = c0; q < c1; q += cpi) {
+                            const int64_t qe = std::min<int64_t>(c1, q + cpi);
+                            wcsr.push_back(make_int4(s, ch, (int)q, (int)(qe - q) | (lp << 24)));
+                            wcsr_cost.push_back((int64_t)(rp[qe] - rp[q]) * 4);
+                        }
+                        c0 = c1;
+                        continue;
+                    }
+                    int64_t c1 = std::min<int64_t>(ncell, c0 + NG);
+                    while (c1 < ncell && c1 - c0 < 4 * NG && block_P(c1) == 0 &&
+                           rp[std::min<int64_t>(ncell, c1 + NG)] - rp[c0] <= kCsrItemEntries)
+                        c1 = std::min<int64_t>(ncell, c1 + NG);
+                    wcsr.push_back(make_int4(s, ch, (int)c0, (int)(c1 - c0)));
+                    wcsr_cost.push_back(rp[c1] - rp[c0]);
+                    c0 = c1;
+                }
+            }
         }
+    }
+    {
+        // heaviest CSR items first
+        std::vector<size_t> ord(wcsr.size());
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wcsr_cost[a] > wcsr_cost[b]; });
+        std::vector<int4> sorted(wcsr.size());
+        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wcsr[ord[i]];
+        wcsr.swap(sorted);
     }
     // heaviest gridding items first
     {
@@ -1124,7 +1385,9 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.w_rows.upload(wr, st);
     c.w_cols.upload(wc, st);
     c.w_deg.upload(wd, st);
-    c.w_grid.upload(wg, st);
+    c.w_grid.upload(wg.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wg, st);
+    c.w_csr.upload(wcsr.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wcsr, st);
+    c.n_csr = (int)wcsr.size();
     c.n_rows = (int)wr.size();
     c.n_cols = (int)wc.size();
     c.n_deg = (int)wd.size();
@@ -1167,7 +1430,14 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
             LAUNCHCHECK();
             mark("nu_cols");
         }
-        if (wt4)
+        if (BCT >= 2 && degrid_t_) {
+            if (wt4)
+                hipLaunchKernelGGL((nu_degrid_t<BCT, 4>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p,
+                                   sbase_.p, swt_.p, sdcf_.p, (const float2*)c.ws.p, y, r);
+            else
+                hipLaunchKernelGGL((nu_degrid_t<BCT, 0>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p,
+                                   sbase_.p, swt_.p, sdcf_.p, (const float2*)c.ws.p, y, r);
+        } else if (wt4)
             hipLaunchKernelGGL((nu_degrid<BCT, 4>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, sbase_.p,
                                swt_.p, sdcf_.p, (const float2*)c.ws.p, y, r);
         else
@@ -1177,7 +1447,10 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         mark("nu_degrid");
     };
     auto adjoint_from = [&](const float2* y, float2* x, NuRun r) {
-        if (wt4)
+        if (csr_)
+            hipLaunchKernelGGL(nu_grid_csr<BCT>, dim3(c.n_csr), dim3(256), 0, st, sl, c.w_csr.p, csr_ptr_.p,
+                               csr_ent_.p, y, c.ws.p, r);
+        else if (wt4)
             hipLaunchKernelGGL((nu_grid<BCT, 4>), dim3(c.n_grid), dim3(256), 0, st, sl, c.w_grid.p, tiles_.p,
                                tlist_.p, gwork_.p, glist_.p, gbin_.p, sbase_.p, swt_.p, sdcf_.p, y, c.ws.p,
                                c.parts.p, c.counters.p, r);

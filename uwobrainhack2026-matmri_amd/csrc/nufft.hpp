@@ -30,6 +30,8 @@ struct NuSliceDev {
     int64_t cnt_off;    // counter offset of chunk 0
     int64_t cnt_chunk;  // counters per chunk
     int64_t yt_off;     // transposed data ([M][BCT] per chunk) offset of chunk 0
+    int64_t csr_off;    // first row pointer of this slice (CSR gridding), G1*G2 + 1 of them
+    int64_t ent_off;    // first CSR entry of this slice
 };
 
 // One gather-gridding work item: rows [r0, r1) of the 16x16 cells of tile
@@ -80,6 +82,13 @@ class NufftOp : public mm_op {
                         const float* W, int iters,
                         std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
     bool gather() const { return gather_; }
+    bool set_option(const std::string& key, int value) override {
+        if (key == "nu_degrid_t") {
+            degrid_t_ = value != 0;
+            return true;
+        }
+        return false;
+    }
     int ndim() const { return ndim_; }
     const std::vector<int64_t>& slice_V() const { return Vs_; }
     const std::vector<int64_t>& slice_M() const { return Ms_; }
@@ -101,6 +110,8 @@ class NufftOp : public mm_op {
         DevBuf<int> counters;
         DevBuf<int4> w_gath;
         int n_gath = 0;
+        DevBuf<int4> w_csr;      // CSR gridding items: (slice, chunk, first cell, cells)
+        int n_csr = 0;
     };
     Cfg& config(int64_t nrep, int sense, int C);
     // mode 0: the requested direction with the standard data layout;
@@ -138,6 +149,13 @@ class NufftOp : public mm_op {
     DevBuf<TileWork> tiles_;
     std::vector<int64_t> tlist_off_;       // start of each slice's list in tlist_
     std::map<std::pair<int64_t, int>, std::unique_ptr<Cfg>> cfgs_;
+    // CSR gridding (2-D): per cell the (sample, weight) pairs that reach it,
+    // in sample order; row pointers are relative to the slice's ent_off
+    bool csr_ = false;
+    bool degrid_t_ = true;   // BCT lanes per sample in degridding (option "nu_degrid_t")
+    std::vector<int> csr_ptr_host_;
+    DevBuf<int> csr_ptr_;
+    DevBuf<int2> csr_ent_;
     // gather gridding (2-D, every grid side >= 24)
     bool gather_ = false;
     std::vector<GatherWork> gw_host_;
