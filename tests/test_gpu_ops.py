@@ -41,6 +41,9 @@ NUFFT_CASES = [
     ((48, 40), 2000, 4, 1.375, False, 20),
     ((64,), 300, 6, 1.5, True, 2),
     ((96, 110), 5000, 8, 2.0, False, 1),
+    # grid sides above 1024 (1080 x 144; 1-D 2100): FFT index arithmetic at large n
+    ((704, 96), 6000, 4, 1.5, False, 2),
+    ((1400,), 800, 6, 1.5, True, 3),
 ]
 
 

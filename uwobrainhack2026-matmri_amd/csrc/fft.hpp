@@ -108,16 +108,37 @@ __host__ __device__ inline void dft_small(float2 (&v)[R], float sgn) {
     }
 }
 
-// i % p for i < 512 via a 20-bit fixed-point reciprocal pinv = ceil(2^20/p)
-// (the rounding error i*(pinv - 2^20/p)/2^20 < 512/2^20 stays below 1/p).
-__host__ __device__ inline int mod_small(int i, int p, unsigned pinv) {
-    const int q = (int)(((unsigned)i * pinv) >> 20);
-    return i - q * p;
+// i % p via a fixed-point reciprocal.  Small transforms (n <= 1024, so every
+// index i < 512 and p <= 512): pinv = ceil(2^20/p), quotient (i*pinv) >> 20,
+// exact because the rounding error i*(pinv - 2^20/p)/2^20 < 512/2^20 stays
+// below 1/p.  Large transforms (BIG): pinv = ceil(2^32/p) (0 for p = 1) and
+// the high half of the 64-bit product, exact while i*p < 2^32 (every side up
+// to 65535).  The small form is the one the common kernels are compiled with
+// (it keeps their register count low).
+template <bool BIG>
+__host__ __device__ inline unsigned mod_small_inv(int p) {
+    if constexpr (BIG)
+        return p <= 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned long long)p - 1) / (unsigned long long)p);
+    else
+        return ((1u << 20) + (unsigned)p - 1) / (unsigned)p;
 }
+template <bool BIG>
+__host__ __device__ inline int mod_small(int i, int p, unsigned pinv) {
+    if constexpr (BIG) {
+        const unsigned q = (unsigned)(((unsigned long long)(unsigned)i * (unsigned long long)pinv) >> 32);
+        const int r = i - (int)q * p;
+        return p == 1 ? 0 : r;
+    } else {
+        const int q = (int)(((unsigned)i * pinv) >> 20);
+        return i - q * p;
+    }
+}
+// largest transform length the small form handles
+constexpr int kFftSmallMax = 1024;
 
 // One butterfly of a Stockham pass over T interleaved transforms
 // (T = 1 << log2T); tw[t] = exp(+2*pi*i*t/n), t < n; step = n/(p*R).
-template <int R>
+template <int R, bool BIG = false>
 __host__ __device__ inline void stockham_butterfly(const float2* __restrict__ src,
                                                    float2* __restrict__ dst, int n, int log2T,
                                                    int p, unsigned pinv, int step,
@@ -126,7 +147,7 @@ __host__ __device__ inline void stockham_butterfly(const float2* __restrict__ sr
     const int nb = n / R;
     const int t = w & (T - 1);
     const int i = w >> log2T;
-    const int k = mod_small(i, p, pinv);
+    const int k = mod_small<BIG>(i, p, pinv);
     float2 v[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) v[j] = src[(i + j * nb) * T + t];
@@ -145,6 +166,7 @@ __host__ __device__ inline void stockham_butterfly(const float2* __restrict__ sr
 // Full transform in LDS, ping-ponging between a and b.  All threads of the
 // block take part; returns the buffer holding the result.  The caller must
 // __syncthreads() after filling `a`.
+template <bool BIG = false>
 __device__ inline float2* fft_lds(float2* a, float2* b, const FftFactors& f, int log2T,
                                   const float2* tw, float sgn) {
     const int n = f.n;
@@ -153,15 +175,15 @@ __device__ inline float2* fft_lds(float2* a, float2* b, const FftFactors& f, int
     for (int s = 0; s < f.nf; ++s) {
         const int R = f.r[s];
         const int total = (n / R) * T;
-        const unsigned pinv = ((1u << 20) + (unsigned)p - 1) / (unsigned)p;
+        const unsigned pinv = mod_small_inv<BIG>(p);
         const int step = n / (p * R);
         for (int w = threadIdx.x; w < total; w += blockDim.x) {
             switch (R) {
-                case 4: stockham_butterfly<4>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                case 2: stockham_butterfly<2>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                case 3: stockham_butterfly<3>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                case 5: stockham_butterfly<5>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                default: stockham_butterfly<7>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 4: stockham_butterfly<4, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 2: stockham_butterfly<2, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 3: stockham_butterfly<3, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 5: stockham_butterfly<5, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                default: stockham_butterfly<7, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
             }
         }
         __syncthreads();

@@ -153,7 +153,7 @@ __device__ __forceinline__ int lanes_of(const NuRun& r, int BCT, int ch) {
 
 __device__ __forceinline__ int64_t chunk_stride(const NuRun& r, int BCT) { return r.sense ? r.C : BCT; }
 
-template <int BCT>
+template <int BCT, bool BIG>
 __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
         A[(pos * rpw + r) * BCT + b] = make_float2(val.x * f, val.y * f);
     }
     __syncthreads();
-    const float2* R = fft_lds(A, Bb, facs[S.fac1], log2T, tw, 1.f);
+    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac1], log2T, tw, 1.f);
     float2* out = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int tot2 = T * G1;
     for (int idx = threadIdx.x; idx < tot2; idx += blockDim.x) {
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
 // FFT along g2 for `cpw` columns.  Forward (adj = 0): input rows 0..N2-1 are
 // placed at (i2 - N2/2) mod G2, output G2 rows.  Adjoint: input G2 rows,
 // output rows i2 < N2 read from (i2 - N2/2) mod G2.  In place in `ws`.
-template <int BCT>
+template <int BCT, bool BIG>
 __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl,
                                                const int4* __restrict__ work,
                                                const FftFactors* __restrict__ facs,
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
         }
     }
     __syncthreads();
-    const float2* R = fft_lds(A, Bb, facs[S.fac2], log2T, tw, adj ? -1.f : 1.f);
+    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac2], log2T, tw, adj ? -1.f : 1.f);
     if (!adj) {
         const int total = G2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl
                                    area, area + CELLS, s_last);
 }
 
-template <int BCT>
+template <int BCT, bool BIG>
 __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
@@ -881,7 +881,7 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
             i2 < N2 ? g[((int64_t)i2 * G1 + g1) * BCT + b] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    const float2* R = fft_lds(A, Bb, facs[S.fac1], log2T, tw, -1.f);
+    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac1], log2T, tw, -1.f);
     const int lanes = lanes_of(run, BCT, ch);
     if (run.sense) {
         const int tot = rpw * N1;
@@ -1226,6 +1226,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     }
     V_total_ = voff;
     M_total_ = moff;
+    for (const auto& q : sl_) big_ |= q.G1 > kFftSmallMax || q.G2 > kFftSmallMax;
     hipStream_t st = ctx->stream;
     facs_.upload(facs, st);
     twid_.upload(twid, st);
@@ -1278,9 +1279,9 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         NuSliceDev& d = c.hs[s];
         // rows / columns per work-group: aim for 8-16 interleaved transforms
         // within 64 KB of LDS
-        int rpw = std::max(1, 8 / BCT);
+        int rpw = std::max(1, rows_T_ / BCT);
         while (rpw > 1 && fft_lds_bytes(d.G1, rpw * BCT) > 65536) rpw >>= 1;
-        int cpw = std::max(1, 16 / BCT);
+        int cpw = std::max(1, cols_T_ / BCT);
         while (cpw > 1 && fft_lds_bytes(d.G2, cpw * BCT) > 65536) cpw >>= 1;
         d.rpw = rpw;
         d.cpw = cpw;
@@ -1402,17 +1403,17 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     return c;
 }
 
-template <int BCT>
+template <int BCT, bool BIG>
 void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, float2* out, int mode) {
     hipStream_t st = ctx->stream;
     static bool attr_set = false;
     if (!attr_set) {
         // allow FFT passes above 64 KB of LDS where the device permits it
-        (void)hipFuncSetAttribute((const void*)nu_fwd_rows<BCT>,
+        (void)hipFuncSetAttribute((const void*)nu_fwd_rows<BCT, BIG>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)nu_adj_rows<BCT>,
+        (void)hipFuncSetAttribute((const void*)nu_adj_rows<BCT, BIG>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)nu_cols<BCT>,
+        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, BIG>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
         attr_set = true;
@@ -1420,12 +1421,12 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     const bool wt4 = ndim_ == 2 && W_ == 4;
     const NuSliceDev* sl = c.sl.p;
     auto forward = [&](const float2* x, float2* y, NuRun r) {
-        hipLaunchKernelGGL(nu_fwd_rows<BCT>, dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
+        hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
                            twid_.p, deap_.p, x, c.ws.p, r);
         LAUNCHCHECK();
         mark("nu_fwd_rows");
         if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL(nu_cols<BCT>, dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
+            hipLaunchKernelGGL((nu_cols<BCT, BIG>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
                                twid_.p, c.ws.p, 0);
             LAUNCHCHECK();
             mark("nu_cols");
@@ -1461,12 +1462,12 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_grid");
         if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL(nu_cols<BCT>, dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
+            hipLaunchKernelGGL((nu_cols<BCT, BIG>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
                                twid_.p, c.ws.p, 1);
             LAUNCHCHECK();
             mark("nu_cols");
         }
-        hipLaunchKernelGGL(nu_adj_rows<BCT>, dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
+        hipLaunchKernelGGL((nu_adj_rows<BCT, BIG>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
                            twid_.p, deap_.p, (const float2*)c.ws.p, x, r);
         LAUNCHCHECK();
         mark("nu_adj_rows");
@@ -1513,12 +1514,22 @@ void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const floa
     r.maps = maps;
     r.W = W;
     if (mode == 1 && !gather_) c.ydata.ensure((size_t)(c.sense ? M_total_ * c.C * c.nrep : M_total_ * c.nrep));
+    if (big_) {
+        switch (c.BCT) {
+            case 1: launch<1, true>(c, adjoint, r, in, out, mode); break;
+            case 2: launch<2, true>(c, adjoint, r, in, out, mode); break;
+            case 4: launch<4, true>(c, adjoint, r, in, out, mode); break;
+            case 8: launch<8, true>(c, adjoint, r, in, out, mode); break;
+            default: launch<16, true>(c, adjoint, r, in, out, mode); break;
+        }
+        return;
+    }
     switch (c.BCT) {
-        case 1: launch<1>(c, adjoint, r, in, out, mode); break;
-        case 2: launch<2>(c, adjoint, r, in, out, mode); break;
-        case 4: launch<4>(c, adjoint, r, in, out, mode); break;
-        case 8: launch<8>(c, adjoint, r, in, out, mode); break;
-        default: launch<16>(c, adjoint, r, in, out, mode); break;
+        case 1: launch<1, false>(c, adjoint, r, in, out, mode); break;
+        case 2: launch<2, false>(c, adjoint, r, in, out, mode); break;
+        case 4: launch<4, false>(c, adjoint, r, in, out, mode); break;
+        case 8: launch<8, false>(c, adjoint, r, in, out, mode); break;
+        default: launch<16, false>(c, adjoint, r, in, out, mode); break;
     }
 }
 

@@ -81,14 +81,23 @@ class NufftOp : public mm_op {
     void profile_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                         const float* W, int iters,
                         std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
-    bool gather() const { return gather_; }
+    // tuning: transforms per work-group of the row / column FFT passes
+    // (options "nu_rows_T", "nu_cols_T"; powers of two, default 8 / 8),
+    // degridding variant ("nu_degrid_t": 1 = one lane per (sample, coil))
     bool set_option(const std::string& key, int value) override {
+        if (key == "nu_rows_T" || key == "nu_cols_T") {
+            if (value < 1 || value > 64 || (value & (value - 1))) return false;
+            (key == "nu_rows_T" ? rows_T_ : cols_T_) = value;
+            cfgs_.clear();   // work tables and LDS sizes depend on it
+            return true;
+        }
         if (key == "nu_degrid_t") {
             degrid_t_ = value != 0;
             return true;
         }
         return false;
     }
+    bool gather() const { return gather_; }
     int ndim() const { return ndim_; }
     const std::vector<int64_t>& slice_V() const { return Vs_; }
     const std::vector<int64_t>& slice_M() const { return Ms_; }
@@ -119,8 +128,9 @@ class NufftOp : public mm_op {
     // that the gathering adjoint reads)
     void run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps, const float* W,
              int mode = 0);
-    template <int BCT>
+    template <int BCT, bool BIG>
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
+    bool big_ = false;   // a grid side above kFftSmallMax: FFT kernels with the wide modulo
     // event marks for profile_normal (inactive unless prof_ is set)
     struct Prof {
         std::vector<hipEvent_t> ev;
@@ -153,6 +163,7 @@ class NufftOp : public mm_op {
     // in sample order; row pointers are relative to the slice's ent_off
     bool csr_ = false;
     bool degrid_t_ = true;   // BCT lanes per sample in degridding (option "nu_degrid_t")
+    int rows_T_ = 8, cols_T_ = 8;
     std::vector<int> csr_ptr_host_;
     DevBuf<int> csr_ptr_;
     DevBuf<int2> csr_ent_;
