@@ -146,6 +146,39 @@ def test_multislice_sense_vs_oracle(mm):
         ox += N * N
 
 
+def test_multislice_shared_trajectory(mm):
+    """Slices with the same trajectory share their plan data (sample weights,
+    CSR gridding rows): slices 0 and 2 share one spiral, slice 1 has its own,
+    one with a dcf.  Each slice equals the oracle and its single-slice plan."""
+    print("[case] multi-slice sense, shared trajectories", flush=True)
+    rng = np.random.default_rng(21)
+    C, N = 4, 48
+    kl_a = mm.workloads.spiral(3000, 24)
+    kl_b = rng.random((2500, 2)) - 0.5
+    dcf_b = rng.random(2500) + 0.5
+    sl = [([N, N], kl_a, None), ([N, N], kl_b, dcf_b), ([N, N], kl_a.copy(), None), ([N, N], kl_b.copy(), dcf_b.copy())]
+    maps = [crand(rng, (N, N, C)) for _ in sl]
+    E = mm.senseOp(mm.nufftOp.batch(sl, 1.5, 4), mm.rcvrOp.batch(maps))
+    xs = [crand(rng, (N, N)) for _ in sl]
+    ys = [crand(rng, (s[1].shape[0], C)) for s in sl]
+    yg = E * np.concatenate([x.reshape(-1, order="F") for x in xs])
+    xg = E.H * np.concatenate([y.reshape(-1, order="F") for y in ys])
+    oy = ox = 0
+    for (imN, kl, dc), mp, x, y in zip(sl, maps, xs, ys):
+        M = kl.shape[0]
+        So, Ro = NufftPlan(imN, kl, dc, 1.5, 4), RcvrPlan(mp)
+        Es = mm.senseOp(mm.nufftOp(imN, kl, dc, 1, 1.5, 4), mm.rcvrOp(mp))
+        e_f = rel(yg[oy:oy + M * C], o_mrSampFunc(x, "notransp", So, Ro))
+        e_a = rel(xg[ox:ox + N * N], o_mrSampFunc(y, "transp", So, Ro))
+        d_f = rel(yg[oy:oy + M * C], (Es * x).reshape(-1, order="F"))
+        d_a = rel(xg[ox:ox + N * N], (Es.H * y).reshape(-1, order="F"))
+        print(f"  M={M} fwd {e_f:.2e} adj {e_a:.2e}; vs single-slice plan {d_f:.1e} {d_a:.1e}", flush=True)
+        assert e_f < TOL and e_a < TOL
+        assert d_f < 1e-6 and d_a < 1e-6
+        oy += M * C
+        ox += N * N
+
+
 def _hio_plans(mm, rng, N=12, K=300, conc=True, mask=False, Lh=16):
     gx = (np.arange(N) - N / 2) * 0.01
     X, Y = np.meshgrid(gx, gx, indexing="ij")

@@ -31,6 +31,7 @@
 //                 work-group
 //   nu_adj_rows   FFT along g1 + crop + deapod + conj-coil combine
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <numeric>
 
@@ -61,6 +62,20 @@ double nufft_kb_beta(int kwidth, double os) {
 int nufft_kb_nkern(int kwidth, double os) {
     const double precision = 1e-6;
     return (int)std::ceil(0.5 * (kwidth + 1) * std::sqrt(0.37 / (os * os) / precision)) * 2;
+}
+
+// FNV-1a over the bytes of a trajectory (and its dcf): finds slices that can
+// share their sample data.
+static uint64_t traj_hash(const double* kl, const double* dc, int M, int ndim) {
+    uint64_t h = 1469598103934665603ull;
+    auto eat = [&](const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    };
+    eat(kl, sizeof(double) * (size_t)M * ndim);
+    if (dc) eat(dc, sizeof(double) * (size_t)M);
+    h ^= (uint64_t)M * 0x9E3779B97F4A7C15ull;
+    return h;
 }
 
 // Modified Bessel function I0 by its power series (converges for all x;
@@ -215,12 +230,13 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
 // FFT along g2 for `cpw` columns.  Forward (adj = 0): input rows 0..N2-1 are
 // placed at (i2 - N2/2) mod G2, output G2 rows.  Adjoint: input G2 rows,
 // output rows i2 < N2 read from (i2 - N2/2) mod G2.  In place in `ws`.
-template <int BCT, bool BIG>
+template <int BCT, bool BIG, int ADJ>
 __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl,
                                                const int4* __restrict__ work,
                                                const FftFactors* __restrict__ facs,
                                                const float2* __restrict__ twp,
-                                               float2* __restrict__ ws, int adj) {
+                                               float2* __restrict__ ws) {
+    constexpr int adj = ADJ;
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
     const NuSliceDev& S = sl[wk.x];
@@ -256,7 +272,7 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
         }
     }
     __syncthreads();
-    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac2], log2T, tw, adj ? -1.f : 1.f);
+    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac2], log2T, tw, ADJ ? -1.f : 1.f);
     if (!adj) {
         const int total = G2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
@@ -962,6 +978,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     std::vector<float> swt, sdcf;
     std::vector<int> tlist, glist_h, gbin_h;
     std::vector<int2> csr_ent_h;
+    std::map<uint64_t, int> traj_seen;   // trajectory hash -> first slice with it
     int64_t voff = 0, moff = 0;
     for (int s = 0; s < nslice; ++s) {
         NuSliceDev d{};
@@ -998,10 +1015,50 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         } else {
             deap.push_back(1.f);
         }
-        // samples (nufftOp.m:187-246)
-        d.samp_off = (int64_t)sbase.size();
         const double* kl = kloc[s];
         const double* dc = dcf ? dcf[s] : nullptr;
+        if (ndim == 2) {
+            // a 2-D slice with the same size and the same trajectory and dcf
+            // as an earlier one shares its sample data and CSR gridding rows
+            const uint64_t h = traj_hash(kl, dc, d.M, ndim) ^ ((uint64_t)d.N1 << 40) ^ ((uint64_t)d.N2 << 20);
+            auto it = traj_seen.find(h);
+            int twin = -1;
+            if (it != traj_seen.end()) {
+                const int t = it->second;
+                const NuSliceDev& e = sl_[t];
+                if (e.N1 == d.N1 && e.N2 == d.N2 && e.M == d.M &&
+                    std::memcmp(kloc[t], kl, sizeof(double) * (size_t)d.M * ndim) == 0 &&
+                    ((dcf ? dcf[t] : nullptr) == nullptr) == (dc == nullptr) &&
+                    (dc == nullptr || std::memcmp(dcf[t], dc, sizeof(double) * (size_t)d.M) == 0))
+                    twin = t;
+            }
+            if (twin >= 0) {
+                const NuSliceDev& e = sl_[twin];
+                d.samp_off = e.samp_off;
+                d.csr_off = e.csr_off;
+                d.ent_off = e.ent_off;
+                d.tlist_off = e.tlist_off;
+                tlist_off_.push_back(e.tlist_off);
+                tw_first_.push_back((int)tw_host_.size());
+                tw_count_.push_back(0);
+                gw_first_.push_back((int)gw_host_.size());
+                gw_count_.push_back(0);
+                nslots_.push_back(0);
+                ncounters_.push_back(0);
+                d.tile_first = tw_first_.back();
+                sl_.push_back(d);
+                Vs_.push_back(n1 * n2);
+                Ms_.push_back(d.M);
+                img_start_.push_back(voff);
+                dat_start_.push_back(moff);
+                voff += n1 * n2;
+                moff += d.M;
+                continue;
+            }
+            traj_seen.emplace(h, s);
+        }
+        // samples (nufftOp.m:187-246)
+        d.samp_off = (int64_t)sbase.size();
         const int Ws = d.W1 + d.W2;
         swt.resize(swt.size() + (size_t)d.M * Ws);
         float* wdst = swt.data() + (size_t)d.samp_off * Ws;
@@ -1413,7 +1470,9 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)nu_adj_rows<BCT, BIG>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, BIG>,
+        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, BIG, 0>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, BIG, 1>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
         attr_set = true;
@@ -1426,8 +1485,8 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_fwd_rows");
         if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
-                               twid_.p, c.ws.p, 0);
+            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
+                               facs_.p, twid_.p, c.ws.p);
             LAUNCHCHECK();
             mark("nu_cols");
         }
@@ -1462,8 +1521,8 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_grid");
         if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p, facs_.p,
-                               twid_.p, c.ws.p, 1);
+            hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
+                               facs_.p, twid_.p, c.ws.p);
             LAUNCHCHECK();
             mark("nu_cols");
         }
