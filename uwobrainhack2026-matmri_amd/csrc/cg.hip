@@ -488,6 +488,13 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     }
     const int expResN = o.expResN > 0 ? o.expResN : 100;
     const int poll = 4;
+    // finished slices are left out of the operator from the next iteration on
+    HIPCHECK(hipMemsetAsync(done_d_.p, 0, sizeof(int) * ns, st));
+    struct SkipGuard {
+        mm_op* E;
+        ~SkipGuard() { E->set_skip(nullptr); }
+    } guard{E_};
+    E_->set_skip(done_d_.p);
     for (int it = 0; it < NitMax; ++it) {
         const int explicit_res = (it % expResN) == 0;
         if (explicit_res || !use_graph_) {

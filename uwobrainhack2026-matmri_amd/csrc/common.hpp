@@ -104,6 +104,10 @@ struct mm_op {
     // optional: accumulate per-slice partial FP64 dot products of
     // conj(in) . out into `dot` (nslice x 2 doubles, already zeroed) during
     // apply_normal.  Returns false if the operator cannot fuse it.
+    // Per-slice skip flags (device, nslice ints) for apply_normal: slices
+    // whose flag is nonzero may be left out (their output is not defined).
+    // The solver installs its "slice finished" flags here; NULL = none.
+    virtual void set_skip(const int* flags) { (void)flags; }
     // Tuning knobs (kernel variants); false if the key is unknown.
     virtual bool set_option(const std::string& key, int value) { (void)key; (void)value; return false; }
     // Time the operator's dominant kernel: `iters` back-to-back launches on

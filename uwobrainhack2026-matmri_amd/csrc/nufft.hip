@@ -158,7 +158,10 @@ struct NuRun {
     int64_t B;          // transforms per slice (plain)
     const float2* maps;
     const float* W;     // optional forward-output weights (data layout)
+    const int* skip;    // per-slice flags: nonzero = leave the slice out (normal operator)
 };
+
+__device__ __forceinline__ bool skipped(const NuRun& r, int slice) { return r.skip && r.skip[slice]; }
 
 __device__ __forceinline__ int lanes_of(const NuRun& r, int BCT, int ch) {
     if (r.sense) return r.C;
@@ -178,6 +181,7 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
                                                    float2* __restrict__ ws, NuRun run) {
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
+    if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y, r0 = wk.z;
     const int N1 = S.N1, N2 = S.N2, G1 = S.G1, rpw = S.rpw;
@@ -235,10 +239,11 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
                                                const int4* __restrict__ work,
                                                const FftFactors* __restrict__ facs,
                                                const float2* __restrict__ twp,
-                                               float2* __restrict__ ws) {
+                                               float2* __restrict__ ws, const int* __restrict__ skip) {
     constexpr int adj = ADJ;
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
+    if (skip && skip[wk.x]) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y, c0 = wk.z;
     const int N2 = S.N2, G1 = S.G1, G2 = S.G2, cpw = S.cpw;
@@ -327,6 +332,7 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
                                                    float2* __restrict__ y, NuRun run) {
     constexpr int SPP = 256 / BCT;
     const int4 wk = work[blockIdx.x];
+    if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y;
     const int b = threadIdx.x & (BCT - 1);
@@ -383,6 +389,7 @@ __global__ __launch_bounds__(256, 2) void nu_degrid(const NuSliceDev* __restrict
                                                  const float2* __restrict__ ws,
                                                  float2* __restrict__ y, NuRun run) {
     const int4 wk = work[blockIdx.x];
+    if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y;
     const int m = wk.z + threadIdx.x;
@@ -771,6 +778,7 @@ __global__ __launch_bounds__(256) void nu_grid_csr(const NuSliceDev* __restrict_
     constexpr int NG = 256 / BCT;  // lane groups
     __shared__ float2 red[256];
     const int4 wk = work[blockIdx.x];
+    if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y;
     const int lp = wk.w >> 24;
@@ -876,6 +884,7 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
                                                    float2* __restrict__ x, NuRun run) {
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
+    if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y, r0 = wk.z;
     const int N1 = S.N1, N2 = S.N2, G1 = S.G1, rpw = S.rpw;
@@ -1486,7 +1495,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         mark("nu_fwd_rows");
         if (ndim_ == 2 && c.n_cols > 0) {
             hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
-                               facs_.p, twid_.p, c.ws.p);
+                               facs_.p, twid_.p, c.ws.p, r.skip);
             LAUNCHCHECK();
             mark("nu_cols");
         }
@@ -1522,7 +1531,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         mark("nu_grid");
         if (ndim_ == 2 && c.n_cols > 0) {
             hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
-                               facs_.p, twid_.p, c.ws.p);
+                               facs_.p, twid_.p, c.ws.p, r.skip);
             LAUNCHCHECK();
             mark("nu_cols");
         }
@@ -1572,6 +1581,7 @@ void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const floa
     r.B = c.nrep;
     r.maps = maps;
     r.W = W;
+    r.skip = mode == 1 ? skip_ : nullptr;
     if (mode == 1 && !gather_) c.ydata.ensure((size_t)(c.sense ? M_total_ * c.C * c.nrep : M_total_ * c.nrep));
     if (big_) {
         switch (c.BCT) {

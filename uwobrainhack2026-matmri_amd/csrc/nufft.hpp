@@ -97,6 +97,7 @@ class NufftOp : public mm_op {
         }
         return false;
     }
+    void set_skip(const int* flags) override { skip_ = flags; }
     bool gather() const { return gather_; }
     int ndim() const { return ndim_; }
     const std::vector<int64_t>& slice_V() const { return Vs_; }
@@ -130,7 +131,8 @@ class NufftOp : public mm_op {
              int mode = 0);
     template <int BCT, bool BIG>
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
-    bool big_ = false;   // a grid side above kFftSmallMax: FFT kernels with the wide modulo
+    bool big_ = false;
+    const int* skip_ = nullptr;   // per-slice skip flags of the normal operator   // a grid side above kFftSmallMax: FFT kernels with the wide modulo
     // event marks for profile_normal (inactive unless prof_ is set)
     struct Prof {
         std::vector<hipEvent_t> ev;
