@@ -273,6 +273,43 @@ def oracle_nrmse_warmstart(fam, host, prob, xw, xz, nprob=3):
             "reference": "FP64 NumPy oracle cgne, zero init, 150 iterations (|r|/|b| ~ 1e-5)"}
 
 
+def nufft_roofline(mm, E, dev, slices, workload, iters=20):
+    """Dominant kernel of a NUFFT SENSE normal operator (HIP events on the
+    library stream, every stage timed) and its HBM roofline: algorithmic
+    bytes = per-slice stage formula summed over `slices` [(N, M, C)]."""
+    import ctypes as C
+    import torch
+    ms = C.c_double()
+    name = C.create_string_buffer(64)
+    ni = int(sum(n * n for n, _, _ in slices))
+    xi = torch.zeros(ni, dtype=torch.complex64, device=dev)
+    xo = torch.empty_like(xi)
+    L = mm.lib()
+    rc = L.mm_op_profile(E._plan.h, 2, xi.data_ptr(), xo.data_ptr(), 1, iters, C.byref(ms), name, 64)
+    if rc != 0 or ms.value <= 0:
+        return None, {}
+    stages = {}
+    k = 0
+    while True:
+        sm, sl, sn = C.c_double(), C.c_int(), C.create_string_buffer(64)
+        if L.mm_op_profile_stage(E._plan.h, k, C.byref(sm), C.byref(sl), sn, 64) != 0:
+            break
+        stages[sn.value.decode()] = {"ms_per_launch": sm.value, "launches_per_application": sl.value}
+        k += 1
+    kname = name.value.decode()
+    byts = 0.0
+    for n, m, c in slices:
+        g = mm.workloads.grid_size(n, 1.5)
+        byts += nufft_stage_bytes(n * n, g * g, m, c).get(kname, 0.0)
+    if byts <= 0:
+        return None, stages
+    ach = byts / (ms.value * 1e-3) / 1e9
+    kern_file = {"nu_grid": "nu_grid"}.get(kname, kname)
+    return ({"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+             "traffic": read_traffic(kern_file, workload), "kernel": kname, "ms_per_launch": ms.value,
+             "algorithmic_bytes_per_launch": byts}, stages)
+
+
 def cpu_baseline_warmstart(fam, host, prob, seconds):
     """FP64 NumPy oracle zero-init cgne to relative residual 1e-4 on as many
     of the held-out problems as fit the time budget (the reference CPU
@@ -361,22 +398,7 @@ def run_warmstart(a, mm, torch, dev):
         ws.warm_x0(net, xin)
     torch.cuda.synchronize()
     t_inf = (time.perf_counter() - t0) / 10
-    # dominant kernel of the normal operator, HIP events on the library stream
-    ms = C.c_double()
-    name = C.create_string_buffer(64)
-    xi = torch.zeros(P * N * N, dtype=torch.complex64, device=dev)
-    xo = torch.empty_like(xi)
-    L = mm.lib()
-    rc = L.mm_op_profile(E._plan.h, 2, xi.data_ptr(), xo.data_ptr(), 1, 20, C.byref(ms), name, 64)
-    stages = {}
-    if rc == 0:
-        k = 0
-        while True:
-            sm, sl, sn = C.c_double(), C.c_int(), C.create_string_buffer(64)
-            if L.mm_op_profile_stage(E._plan.h, k, C.byref(sm), C.byref(sl), sn, 64) != 0:
-                break
-            stages[sn.value.decode()] = {"ms_per_launch": sm.value, "launches_per_application": sl.value}
-            k += 1
+    roof, stages = nufft_roofline(mm, E, dev, [(N, M, Cc)] * P, "warmstart")
     zw, zz = res["warm"], res["zero"]
     cfg = {"workload": "warmstart: 100 held-out 2-D spiral SENSE problems per step (128x128 c64, 8 coils, "
                        "M=16384 VD spiral, KB W=4 os=1.5, SNR 40), one multi-slice solve to rel. resid 1e-4; "
@@ -400,18 +422,9 @@ def run_warmstart(a, mm, torch, dev):
            "speedup_warm_over_zero": float(zz["el"] / zw["el"]),
            "rel_resid_final_max": float(rel_w.max()),
            "status_last": {str(int(k)): int(v) for k, v in zip(*np.unique(zw["status"], return_counts=True))}}
-    if rc == 0 and ms.value > 0:
-        kname = name.value.decode()
-        G = 192 * 192
-        per = nufft_stage_bytes(N * N, G, M, Cc).get(kname)
-        if per is not None:
-            byts = per * P
-            ach = byts / (ms.value * 1e-3) / 1e9
-            traffic = read_traffic(kname, "warmstart")
-            out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                               "frac": ach / PEAK_HBM_GBS, "traffic": traffic, "kernel": kname,
-                               "ms_per_launch": ms.value, "algorithmic_bytes_per_launch": byts}
-        out["normal_operator_stages"] = stages
+    if roof:
+        out["roofline"] = roof
+    out["normal_operator_stages"] = stages
     if not a.no_cpu:
         out["nrmse_vs_fp64_oracle"] = oracle_nrmse_warmstart(fam, host, prob, zw["x"], zz["x"])
         cb = cpu_baseline_warmstart(fam, host, prob, a.cpu_seconds)
@@ -494,6 +507,11 @@ def main():
         yout = torch.empty_like(y.reshape(-1))
         rc = mm.lib().mm_op_profile(E._plan.h, 0, xin.data_ptr(), yout.data_ptr(), 1, 10, C.byref(ms), name, 64)
         kms = ms.value if rc == 0 else None
+    if W["kind"] in ("nufft", "batch"):
+        with torch.cuda.stream(stream):
+            sl = ([(p["N"], p["M"], 16) for p in W["P"]] if W["kind"] == "batch"
+                  else [(W["P"]["N"], W["P"]["M"], W["P"]["C"])])
+            roof_nu, stages_nu = nufft_roofline(mm, E, dev, sl, W["kind"])
     sps = W.get("solves_per_step", 1)
     out = {"metric": "CG-SENSE solves/s at rel. resid 1e-4, learned warm start vs zero-init",
            "value": sps * a.steps / el, "unit": "solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -513,6 +531,10 @@ def main():
                                "traffic": read_traffic(name.value.decode(), "highorder"),
                                "kernel": name.value.decode(), "ms_per_launch": kms,
                                "mfma_products_per_flop": 3}
+    if W["kind"] in ("nufft", "batch"):
+        if roof_nu:
+            out["roofline"] = roof_nu
+        out["normal_operator_stages"] = stages_nu
     if not a.no_cpu:
         cb = {"highorder": cpu_baseline_highorder, "nufft": cpu_baseline_nufft,
               "batch": cpu_baseline_batch}[W["kind"]](W["P"], a.cpu_seconds)

@@ -176,10 +176,10 @@ def _sense_problem(mm, N=64, M=6000, C=4, seed=5):
 def test_cg_x0_fallback(mm):
     """A start worse than x0 = 0 falls back to it: flagged, its history
     starts at |b|^2 like the zero-init solve, and the solve lands in the
-    FP32 band of the zero-init one; a good start is kept.  (b = E'y is not
-    bit-reproducible between calls -- gridding of dense tiles accumulates
-    with LDS float atomics -- so the two solves agree to FP32 rounding,
-    which CG then amplifies.)"""
+    FP32 band of the zero-init one; a good start is kept.  (|b|^2, which the
+    fallback takes as its residual, and the zero-init |r0|^2 are the same
+    sum computed by two separately compiled kernels, so they can differ in
+    the last bits, which FP32 CG then amplifies.)"""
     import torch
     print("[case] cgne x0 fallback N=64 C=4", flush=True)
     E, y, xt = _sense_problem(mm)

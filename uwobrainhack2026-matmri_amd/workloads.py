@@ -22,6 +22,23 @@ import numpy as np
 HARM_ORDER = np.array([0] + [1] * 3 + [2] * 5 + [3] * 7 + [4] * 9 + [5] * 11)
 
 
+def grid_size(n, os=1.5):
+    """Oversampled grid side: 2*ceil(n*os/2), raised by 2 until its largest
+    prime factor is at most 7 (the rule of nufftOp.m:146-152)."""
+    g = int(np.ceil(n * os / 2.0)) * 2
+
+    def maxp(v):
+        best, p = 1, 2
+        while p * p <= v:
+            while v % p == 0:
+                best, v = p, v // p
+            p += 1
+        return max(best, v) if v > 1 else best
+    while maxp(g) > 7:
+        g += 2
+    return g
+
+
 def spiral(M, n_turns=64, dens_exp=1.5, kmax=0.5, wrap=True):
     """[M x 2] cycles/pixel; sample m at tau = m/(M-1).  With `wrap` a
     coordinate at +0.5 is moved to -0.5 (the same grid point)."""
