@@ -183,9 +183,10 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
     const int4 wk = work[blockIdx.x];
     if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
-    const int ch = wk.y, r0 = wk.z;
-    const int N1 = S.N1, N2 = S.N2, G1 = S.G1, rpw = S.rpw;
-    const int T = rpw * BCT, log2T = ilog2(T);
+    const int ch = wk.y, r0 = wk.z, b0 = wk.w;
+    const int N1 = S.N1, N2 = S.N2, G1 = S.G1, rpw = S.frpw;
+    const int LL = 1 << S.lfl;           // transforms (coils) of this work-group: b0 .. b0+LL-1
+    const int T = rpw * LL, log2T = ilog2(T);
     const int64_t V = (int64_t)N1 * N2;
     float2* A = lds;
     float2* Bb = lds + G1 * T;
@@ -198,8 +199,9 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int i1 = idx % N1;
         const int rest = idx / N1;
-        const int b = rest & (BCT - 1);
-        const int r = rest / BCT;
+        const int bl = rest & (LL - 1);
+        const int b = b0 + bl;
+        const int r = rest >> S.lfl;
         const int i2 = r0 + r;
         if (i2 >= N2 || b >= lanes) continue;
         const int64_t v = (int64_t)i2 * N1 + i1;
@@ -214,20 +216,20 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
         const float f = deap[S.deap1 + i1] * deap[S.deap2 + i2];
         int pos = i1 - N1 / 2;
         if (pos < 0) pos += G1;
-        A[(pos * rpw + r) * BCT + b] = make_float2(val.x * f, val.y * f);
+        A[(pos * rpw + r) * LL + bl] = make_float2(val.x * f, val.y * f);
     }
     __syncthreads();
     const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac1], log2T, tw, 1.f);
     float2* out = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int tot2 = T * G1;
     for (int idx = threadIdx.x; idx < tot2; idx += blockDim.x) {
-        const int b = idx & (BCT - 1);
-        const int rest = idx / BCT;
+        const int bl = idx & (LL - 1);
+        const int rest = idx >> S.lfl;
         const int g1 = rest % G1;
         const int r = rest / G1;
         const int i2 = r0 + r;
         if (i2 >= N2) continue;
-        out[((int64_t)i2 * G1 + g1) * BCT + b] = R[(g1 * rpw + r) * BCT + b];
+        out[((int64_t)i2 * G1 + g1) * BCT + b0 + bl] = R[(g1 * rpw + r) * LL + bl];
     }
 }
 
@@ -245,9 +247,11 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     const int4 wk = work[blockIdx.x];
     if (skip && skip[wk.x]) return;
     const NuSliceDev& S = sl[wk.x];
-    const int ch = wk.y, c0 = wk.z;
+    const int ch = wk.y, c0 = wk.z, b0 = wk.w;
     const int N2 = S.N2, G1 = S.G1, G2 = S.G2, cpw = S.cpw;
-    const int T = cpw * BCT, log2T = ilog2(T);
+    const int lcl = S.lcl;                // transforms (coils) b0 .. b0 + 2^lcl - 1 of `cpw` columns
+    const int T = cpw << lcl, log2T = ilog2(T);
+    const int CL = 1 << lcl;
     float2* A = lds;
     float2* Bb = lds + G2 * T;
     float2* tw = lds + 2 * G2 * T;
@@ -261,19 +265,20 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
             const int i2 = idx >> log2T;
-            const int col = t / BCT;
+            const int col = t >> lcl;
             if (col >= ncol) continue;
             int pos = i2 - N2 / 2;
             if (pos < 0) pos += G2;
-            A[pos * T + t] = g[((int64_t)i2 * G1 + c0) * BCT + t];
+            A[pos * T + t] = g[((int64_t)i2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))];
         }
     } else {
         const int total = G2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
             const int g2 = idx >> log2T;
-            const int col = t / BCT;
-            A[idx] = col < ncol ? g[((int64_t)g2 * G1 + c0) * BCT + t] : make_float2(0.f, 0.f);
+            const int col = t >> lcl;
+            A[idx] = col < ncol ? g[((int64_t)g2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))]
+                                : make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
@@ -283,18 +288,20 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
             const int g2 = idx >> log2T;
-            if (t / BCT >= ncol) continue;
-            g[((int64_t)g2 * G1 + c0) * BCT + t] = R[idx];
+            const int col = t >> lcl;
+            if (col >= ncol) continue;
+            g[((int64_t)g2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))] = R[idx];
         }
     } else {
         const int total = N2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
             const int i2 = idx >> log2T;
-            if (t / BCT >= ncol) continue;
+            const int col = t >> lcl;
+            if (col >= ncol) continue;
             int pos = i2 - N2 / 2;
             if (pos < 0) pos += G2;
-            g[((int64_t)i2 * G1 + c0) * BCT + t] = R[pos * T + t];
+            g[((int64_t)i2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))] = R[pos * T + t];
         }
     }
 }
@@ -1339,20 +1346,31 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     const int ns = (int)sl_.size();
     c.hs = sl_;
     int64_t wsoff = 0, poff = 0, coff = 0, ytoff = 0;
-    std::vector<int4> wr, wc, wd, wg, wga, wcsr;
+    std::vector<int4> wr, wfr, wc, wd, wg, wga, wcsr;
     std::vector<int64_t> wga_cost, wcsr_cost;
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
         // rows / columns per work-group: aim for 8-16 interleaved transforms
         // within 64 KB of LDS
+        // adjoint rows combine the coils: all BCT transforms in one work-group
         int rpw = std::max(1, rows_T_ / BCT);
         while (rpw > 1 && fft_lds_bytes(d.G1, rpw * BCT) > 65536) rpw >>= 1;
+        // forward rows: all BCT transforms per work-group as well (splitting
+        // the coils of a row leaves too little work per FFT pass); columns:
+        // min(BCT, target) transforms of one chunk per work-group (a chunk
+        // of 16 coils is split in two at target 8)
+        const int fl = BCT;
+        const int frpw = rpw;
+        const int cl = std::min(BCT, cols_T_);
         int cpw = std::max(1, cols_T_ / BCT);
-        while (cpw > 1 && fft_lds_bytes(d.G2, cpw * BCT) > 65536) cpw >>= 1;
+        while (cpw > 1 && fft_lds_bytes(d.G2, cpw * cl) > 65536) cpw >>= 1;
         d.rpw = rpw;
+        d.frpw = frpw;
+        d.lfl = ilog2(fl);
         d.cpw = cpw;
-        c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, rpw * BCT));
-        c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, cpw * BCT));
+        d.lcl = ilog2(cl);
+        c.lds_rows = std::max({c.lds_rows, fft_lds_bytes(d.G1, rpw * BCT), fft_lds_bytes(d.G1, frpw * fl)});
+        c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, cpw * cl));
         if (sense) {
             d.img_off = img_start_[s] * nrep;
             d.maps_off = img_start_[s] * C;
@@ -1375,8 +1393,11 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         ytoff += (int64_t)d.M * BCT * nchunk;
         for (int ch = 0; ch < nchunk; ++ch) {
             for (int r = 0; r < d.N2; r += rpw) wr.push_back(make_int4(s, ch, r, 0));
+            for (int r = 0; r < d.N2; r += frpw)
+                for (int b0 = 0; b0 < BCT; b0 += fl) wfr.push_back(make_int4(s, ch, r, b0));
             if (d.G2 > 1)
-                for (int q = 0; q < d.G1; q += cpw) wc.push_back(make_int4(s, ch, q, 0));
+                for (int q = 0; q < d.G1; q += cpw)
+                    for (int b0 = 0; b0 < BCT; b0 += cl) wc.push_back(make_int4(s, ch, q, b0));
             for (int m = 0; m < d.M; m += 256) wd.push_back(make_int4(s, ch, m, 0));
             // gridding items: .w = 1 scatter chunk, 0 gather rows
             for (int t = 0; t < tw_count_[s]; ++t) {
@@ -1450,6 +1471,8 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     hipStream_t st = ctx->stream;
     c.sl.upload(c.hs, st);
     c.w_rows.upload(wr, st);
+    c.w_frows.upload(wfr, st);
+    c.n_frows = (int)wfr.size();
     c.w_cols.upload(wc, st);
     c.w_deg.upload(wd, st);
     c.w_grid.upload(wg.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wg, st);
@@ -1489,7 +1512,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     const bool wt4 = ndim_ == 2 && W_ == 4;
     const NuSliceDev* sl = c.sl.p;
     auto forward = [&](const float2* x, float2* y, NuRun r) {
-        hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
+        hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(c.n_frows), dim3(256), c.lds_rows, st, sl, c.w_frows.p, facs_.p,
                            twid_.p, deap_.p, x, c.ws.p, r);
         LAUNCHCHECK();
         mark("nu_fwd_rows");

@@ -17,7 +17,9 @@ struct NuSliceDev {
     int deap1, deap2;   // offset into the deapodisation pool (float)
     int rpw, cpw;       // rows / columns per work-group in the FFT passes
     int tile_first;     // first gridding tile of this slice in the tile pool
-    int pad0;
+    int frpw;           // rows per work-group of the forward row FFT
+    int lfl, lcl;       // log2 transforms (coils) per work-group: forward rows, columns
+    int pad1;
     int64_t samp_off;   // first sample of this slice in the sample pools
     int64_t tlist_off;  // first entry of this slice in the tile sample list
     int64_t img_off;    // image-side offset (elements) for the current apply
@@ -113,8 +115,8 @@ class NufftOp : public mm_op {
         int64_t nrep = 0;
         std::vector<NuSliceDev> hs;
         DevBuf<NuSliceDev> sl;
-        DevBuf<int4> w_rows, w_cols, w_deg, w_grid;
-        int n_rows = 0, n_cols = 0, n_deg = 0, n_grid = 0;
+        DevBuf<int4> w_rows, w_frows, w_cols, w_deg, w_grid;
+        int n_rows = 0, n_frows = 0, n_cols = 0, n_deg = 0, n_grid = 0;
         size_t lds_rows = 0, lds_cols = 0;
         DevBuf<float2> ws, parts, yT, ydata;
         DevBuf<int> counters;
