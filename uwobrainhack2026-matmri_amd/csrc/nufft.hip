@@ -347,7 +347,9 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
     const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2;
     const int lanes = lanes_of(run, BCT, ch);
     const int64_t stride = chunk_stride(run, BCT);
-    const float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b;
+    // uniform base of the chunk's grid; a chunk is far below 2^31 elements,
+    // so taps are addressed with 32-bit element offsets from it
+    const float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int mend = min(S.M, wk.z + 256);
     for (int m = wk.z + (int)(threadIdx.x / BCT); m < mend; m += SPP) {
         const int64_t sm = S.samp_off + m;
@@ -356,19 +358,24 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
         float w1[8], w2[8];
         load_weights<WT>(swt + sm * (W1 + W2), W1, W2, w1, w2);
         float2 acc = make_float2(0.f, 0.f);
+        int c1[8];
+#pragma unroll
+        for (int s1 = 0; s1 < (WT > 0 ? WT : 8); ++s1) {
+            int g1 = g1_0 + s1;
+            if (g1 >= G1) g1 -= G1;
+            c1[s1] = g1 * BCT + b;
+        }
 #pragma unroll
         for (int s2 = 0; s2 < (WT > 0 ? WT : 8); ++s2) {
             if (s2 >= W2) break;
             int g2 = g2_0 + s2;
             if (g2 >= G2) g2 -= G2;
-            const float2* zrow = zg + (int64_t)g2 * G1 * BCT;
+            const int row = g2 * G1 * BCT;
 #pragma unroll
             for (int s1 = 0; s1 < (WT > 0 ? WT : 8); ++s1) {
                 if (s1 >= W1) break;
-                int g1 = g1_0 + s1;
-                if (g1 >= G1) g1 -= G1;
                 const float w = w1[s1] * w2[s2];
-                const float2 q = zrow[(int64_t)g1 * BCT];
+                const float2 q = zg[(unsigned)(row + c1[s1])];
                 acc.x += w * q.x;
                 acc.y += w * q.y;
             }
@@ -498,6 +505,9 @@ __global__ __launch_bounds__(256) void nu_transpose_y(const NuSliceDev* __restri
 constexpr int kMaxBins = (kTile + 7) * (kTile + 7) + 1;
 // CSR gridding: entries per work item (lanes x entries stays balanced)
 constexpr int kCsrItemEntries = 2048;
+// accelerator complex dies (each with its own L2); work-groups are dealt to
+// them round-robin by workgroup index
+constexpr int kXcds = 8;
 
 // entries of a gathered tile that fit the LDS staging area (32 KB)
 template <int BCT>
@@ -1450,7 +1460,33 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
             }
         }
     }
-    {
+    if (xcd_order_) {
+        // XCD-aware order: work-group i runs on XCD i % kXcds, so the items of
+        // slice s are placed at positions = s (mod kXcds): one slice's data
+        // stays in one XCD's L2.  Within an XCD, slice by slice, each
+        // slice's items heaviest first.
+        std::vector<std::vector<size_t>> q(kXcds);
+        std::vector<size_t> ord(wcsr.size());
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+            if (wcsr[a].x != wcsr[b].x) return wcsr[a].x < wcsr[b].x;
+            if (wcsr[a].y != wcsr[b].y) return wcsr[a].y < wcsr[b].y;
+            return wcsr_cost[a] > wcsr_cost[b];
+        });
+        for (size_t i : ord) q[(size_t)(wcsr[i].x + (int64_t)wcsr[i].y * 3) % kXcds].push_back(i);
+        std::vector<int4> sorted;
+        sorted.reserve(wcsr.size());
+        std::vector<size_t> pos(kXcds, 0);
+        for (bool any = true; any;) {
+            any = false;
+            for (int x = 0; x < kXcds; ++x)
+                if (pos[x] < q[x].size()) {
+                    sorted.push_back(wcsr[q[x][pos[x]++]]);
+                    any = true;
+                }
+        }
+        wcsr.swap(sorted);
+    } else {
         // heaviest CSR items first
         std::vector<size_t> ord(wcsr.size());
         std::iota(ord.begin(), ord.end(), 0);

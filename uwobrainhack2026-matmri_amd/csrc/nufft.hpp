@@ -97,6 +97,11 @@ class NufftOp : public mm_op {
             degrid_t_ = value != 0;
             return true;
         }
+        if (key == "nu_xcd_order") {
+            xcd_order_ = value != 0;
+            cfgs_.clear();
+            return true;
+        }
         return false;
     }
     void set_skip(const int* flags) override { skip_ = flags; }
@@ -168,6 +173,10 @@ class NufftOp : public mm_op {
     bool csr_ = false;
     bool degrid_t_ = true;   // BCT lanes per sample in degridding (option "nu_degrid_t")
     int rows_T_ = 8, cols_T_ = 8;
+    // XCD-aware order of the gridding items (option "nu_xcd_order"); off: on
+    // the MI355X it measured slower than the heaviest-first order (318 vs 297 us
+    // per launch for 100 slices x 8 coils)
+    bool xcd_order_ = false;
     std::vector<int> csr_ptr_host_;
     DevBuf<int> csr_ptr_;
     DevBuf<int2> csr_ent_;
