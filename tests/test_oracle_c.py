@@ -1,0 +1,64 @@
+"""The C/OpenMP restatement (oracle/c/mmref.c, the CPU baseline) against the
+NumPy oracle: per application to 1e-12, cgne histories and iterates to 1e-8
+(the OpenMP reductions sum in a thread-count dependent order, and 12 CG
+steps over five decades of residual amplify that)."""
+import numpy as np
+import pytest
+
+from conftest import load_pkg
+from oracle import cref
+from oracle.mrsamp import mrSampFunc
+from oracle.nufft import NufftPlan
+from oracle.rcvr import RcvrPlan
+from oracle.solvers import cgne as o_cgne
+
+
+@pytest.fixture(scope="module")
+def built():
+    cref.build()
+    return cref
+
+
+def _problem(N, M, C, seed, dcf=False):
+    mm = load_pkg()
+    rng = np.random.default_rng(seed)
+    kloc = mm.workloads.spiral(M, N // 2)
+    maps = mm.workloads.gaussian_coils(N, N, C, rng)
+    ph = mm.workloads.ellipse_phantom(N, N, rng)
+    d = mm.workloads.spiral_dcf(M, N // 2) if dcf else None
+    return kloc, maps, ph, d, rng
+
+
+@pytest.mark.parametrize("N,M,C,dcf", [(32, 1500, 4, False), (48, 3000, 3, True), (40, 2200, 2, False)])
+def test_forward_adjoint_match_numpy_oracle(built, N, M, C, dcf):
+    kloc, maps, ph, d, rng = _problem(N, M, C, 3, dcf)
+    So, Ro = NufftPlan([N, N], kloc, d, 1.5, 4), RcvrPlan(maps)
+    E = cref.SenseRef(N, kloc, d, maps)
+    x = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    y = rng.standard_normal((M, C)) + 1j * rng.standard_normal((M, C))
+    yo = mrSampFunc(x, "notransp", So, Ro).reshape(M, C, order="F")
+    xo = mrSampFunc(y, "transp", So, Ro).reshape(N, N, order="F")
+    ef = np.linalg.norm(E.forward(x) - yo) / np.linalg.norm(yo)
+    ea = np.linalg.norm(E.adjoint(y) - xo) / np.linalg.norm(xo)
+    assert ef < 1e-12 and ea < 1e-12, (ef, ea)
+
+
+def test_cgne_matches_numpy_oracle(built):
+    N, M, C = 32, 1500, 4
+    kloc, maps, ph, _, rng = _problem(N, M, C, 5)
+    So, Ro = NufftPlan([N, N], kloc, None, 1.5, 4), RcvrPlan(maps)
+    A = lambda v, tr: mrSampFunc(v, tr, So, Ro)
+    y = mrSampFunc(ph, "notransp", So, Ro)
+    E = cref.SenseRef(N, kloc, None, maps)
+    x0 = np.zeros((N, N), complex)
+    xo, ro, *_ = o_cgne(A, y, x0, 12, {"resThresh": None, "xChangeThresh": None, "expResN": 5})
+    xc, rc, nit = E.cgne(y.reshape(M, C, order="F"), x0, 12, expResN=5)
+    assert nit == 12
+    assert np.allclose(rc, np.real(ro), rtol=1e-8, atol=0)
+    assert np.linalg.norm(xc - xo) / np.linalg.norm(xo) < 1e-8
+    # default x0 = b and a relative threshold stop
+    xo2, ro2, *_ = o_cgne(A, y, None, 50, {"resThresh": 1e-6 * np.real(np.vdot(A(y, "transp"), A(y, "transp"))),
+                                           "xChangeThresh": None})
+    xc2, rc2, nit2 = E.cgne(y.reshape(M, C, order="F"), None, 50, resThreshRel=1e-6)
+    assert nit2 == len(ro2) - 1
+    assert np.linalg.norm(xc2 - xo2) / np.linalg.norm(xo2) < 1e-8
