@@ -248,16 +248,20 @@ def cpu_baseline_nufft(P, seconds):
                       f"scaled to one 20-iteration solve ({apps} applications)"}
 
 
-def nufft_stage_bytes(V, G, M, C):
+def nufft_stage_bytes(N1, N2, G1, G2, M, C):
     """Algorithmic HBM bytes of each NUFFT SENSE stage for one slice and one
-    application (compulsory reads and writes of complex64 vectors; plan
-    constants such as interpolation weights are not counted)."""
+    application: compulsory reads and writes of complex64 vectors (8 bytes a
+    value; plan constants such as interpolation weights are not counted).
+    The forward row pass writes only the N2 image rows of the padded grid and
+    the column passes read N2 rows and write G2 (forward) or the reverse
+    (adjoint)."""
+    V, G = N1 * N2, G1 * G2
     return {
-        "nu_fwd_rows": 8.0 * (V + V * C + G * C),     # image, maps in; padded grid out
-        "nu_cols": 2 * 8.0 * G * C,                   # one read + one write of the grid
-        "nu_degrid": 8.0 * (G * C + M + M * C),       # grid, k-space locations in; samples out
-        "nu_grid": 8.0 * (M * C + M + G * C),         # samples, locations in; grid out
-        "nu_adj_rows": 8.0 * (G * C + V * C + V),     # grid, maps in; image out
+        "nu_fwd_rows": 8.0 * (V + V * C + G1 * N2 * C),   # image, maps in; N2 grid rows out
+        "nu_cols": 8.0 * G1 * (N2 + G2) * C,              # N2 rows in, G2 out (or the reverse)
+        "nu_degrid": 8.0 * (G * C + M + M * C),           # grid, k-space locations in; samples out
+        "nu_grid": 8.0 * (M * C + M + G * C),             # samples, locations in; grid out
+        "nu_adj_rows": 8.0 * (G1 * N2 * C + V * C + V),   # N2 grid rows, maps in; image out
         "nu_transpose_y": 2 * 8.0 * M * C,
     }
 
@@ -331,7 +335,7 @@ def nufft_roofline(mm, E, dev, slices, workload, iters=20):
     byts = 0.0
     for n, m, c in slices:
         g = mm.workloads.grid_size(n, 1.5)
-        byts += nufft_stage_bytes(n * n, g * g, m, c).get(kname, 0.0)
+        byts += nufft_stage_bytes(n, n, g, g, m, c).get(kname, 0.0)
     if byts <= 0:
         return None, stages
     ach = byts / (ms.value * 1e-3) / 1e9

@@ -58,6 +58,24 @@ struct TileWork {
 constexpr int kTile = 16;
 constexpr int kTileChunk = 512;
 
+// One rectangle of grid cells for 2-D CSR gridding.  kind 0: the union of
+// the samples that reach the rectangle (sorted, at most kStageSamples) and
+// the cells' (local sample, weight) rows (at most kStageEntries entries)
+// are staged in LDS; kind 1: the cells are summed straight from the global
+// CSR rows (dense k-space centre).  Offsets are absolute in the pools.
+struct GridTile {
+    int g1, g2, w, h;    // first cell and extent (cells, g1 fastest)
+    int kind;
+    int n_stage, n_ent;  // staged samples / entries (kind 0)
+    int stage_off;       // first sample id in the stage pool
+    int ptr_off;         // first of the w*h+1 row pointers (relative to ent_off)
+    int ent_off;         // first (local sample, weight) entry
+    int max_row;         // longest cell row of the rectangle
+    int pad;
+};
+constexpr int kStageSamples = 256;
+constexpr int kStageEntries = 2048;
+
 class NufftOp : public mm_op {
    public:
     NufftOp(mm_ctx* ctx, int ndim, int nslice, const int64_t* imN, const int64_t* M,
@@ -85,7 +103,9 @@ class NufftOp : public mm_op {
                         std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
     // tuning: transforms per work-group of the row / column FFT passes
     // (options "nu_rows_T", "nu_cols_T"; powers of two, default 8 / 8),
-    // degridding variant ("nu_degrid_t": 1 = one lane per (sample, coil))
+    // degridding variant ("nu_degrid_t": 1 = one lane per (sample, coil)),
+    // 2-D gridding from LDS-staged rectangles ("nu_grid_stage": 1) or from
+    // the global CSR rows only (0)
     bool set_option(const std::string& key, int value) override {
         if (key == "nu_rows_T" || key == "nu_cols_T") {
             if (value < 1 || value > 64 || (value & (value - 1))) return false;
@@ -95,6 +115,10 @@ class NufftOp : public mm_op {
         }
         if (key == "nu_degrid_t") {
             degrid_t_ = value != 0;
+            return true;
+        }
+        if (key == "nu_grid_stage") {
+            grid_stage_ = value != 0;
             return true;
         }
         if (key == "nu_xcd_order") {
@@ -129,6 +153,10 @@ class NufftOp : public mm_op {
         int n_gath = 0;
         DevBuf<int4> w_csr;      // CSR gridding items: (slice, chunk, first cell, cells)
         int n_csr = 0;
+        // staged CSR gridding items: (slice, chunk, tile, lp | c0 << 8 | cells << 16)
+        DevBuf<int4> w_gt;
+        int n_gt = 0;
+        size_t lds_gt = 0;
     };
     Cfg& config(int64_t nrep, int sense, int C);
     // mode 0: the requested direction with the standard data layout;
@@ -180,6 +208,15 @@ class NufftOp : public mm_op {
     std::vector<int> csr_ptr_host_;
     DevBuf<int> csr_ptr_;
     DevBuf<int2> csr_ent_;
+    // staged CSR gridding (option "nu_grid_stage", default on): rectangles
+    // of each distinct 2-D trajectory; slices that share a trajectory share
+    // its rectangles
+    bool grid_stage_ = true;
+    std::vector<GridTile> gtile_host_;
+    std::vector<int> gtile_first_, gtile_count_;   // per slice
+    DevBuf<GridTile> gtile_;
+    DevBuf<int> gstage_, gptr_;
+    DevBuf<int2> gent_;
     // gather gridding (2-D, every grid side >= 24)
     bool gather_ = false;
     std::vector<GatherWork> gw_host_;
