@@ -1,7 +1,9 @@
 """ctypes binding of libmatmri_hip.so (the C ABI in include/matmri_hip.h).
 
-The library is loaded from this package directory.  There is no fallback:
-if it is missing or no gfx950 device is visible, the operators raise.
+The library is loaded from this package directory (or from the path in the
+environment variable MATMRI_HIP_LIB, for a library built elsewhere).  There is
+no fallback: if it is missing or no gfx950 device is visible, the operators
+raise.
 """
 import ctypes as C
 import os
@@ -9,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmatmri_hip.so")
+LIB_PATH = os.environ.get("MATMRI_HIP_LIB") or os.path.join(HERE, "libmatmri_hip.so")
 
 MM_OK, MM_ERR_ARG, MM_ERR_OOM, MM_ERR_DEVICE, MM_ERR_HIP, MM_ERR_UNSUPPORTED = range(6)
 

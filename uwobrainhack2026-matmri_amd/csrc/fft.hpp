@@ -1,15 +1,19 @@
-// Mixed-radix (2,3,4,5,7) Stockham FFT on data resident in LDS.
+// Mixed-radix (8,4,2,3,5,7) Stockham FFT on data resident in LDS.
 //
-// Layout: element (position p, transform t) of `T` interleaved transforms of
-// length n lives at buf[p*T + t], so consecutive lanes work on consecutive
-// transforms (conflict-free LDS access).  Each pass of radix R reads R
-// inputs spaced n/R apart, applies the twiddles exp(sgn*2*pi*i*j*k/(P*R)),
-// does a length-R DFT and writes the R outputs spaced P apart (P = product of
-// the radices already done), which leaves the result in natural order after
-// the last pass.  sgn = +1 is MATLAB's ifft kernel (unscaled), sgn = -1 its
-// fft kernel.
+// Layout: element (position q, transform t) of `T` interleaved transforms of
+// length n lives at buf[q*T + t], so consecutive lanes work on consecutive
+// transforms (conflict-free LDS access).  A pass of radix R with P = the
+// product of the radices already done reads R inputs spaced n/R apart,
+// multiplies input j by exp(sgn*2*pi*i*j*k/(P*R)) (k = butterfly index mod
+// P), does a length-R DFT and writes the R outputs spaced P apart, which
+// leaves the result in natural order after the last pass.  sgn = +1 is
+// MATLAB's ifft kernel (unscaled), sgn = -1 its fft kernel.  T and sgn are
+// compile-time constants of the kernels, so every index is a shift, a mask or
+// a multiply-add.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 namespace mm {
 
@@ -21,13 +25,15 @@ struct FftFactors {
     int r[kMaxFactors] = {};
 };
 
-// Radix order: 4s first, then 2, 3, 5, 7.  Returns false when n has a prime
-// factor above 7 (the grid-size rule of nufftOp.m:146-152 never produces one).
+// Radix order: 8s first, then 4, 2, 3, 5, 7.  Returns false when n has a
+// prime factor above 7 (the grid-size rule of nufftOp.m:146-152 never
+// produces one).
 inline bool fft_factorize(int n, FftFactors& f) {
     f = FftFactors();
     f.n = n;
     int m = n;
-    while (m % 4 == 0) { f.r[f.nf++] = 4; m /= 4; }
+    while (m % 8 == 0 && f.nf < kMaxFactors) { f.r[f.nf++] = 8; m /= 8; }
+    while (m % 4 == 0 && f.nf < kMaxFactors) { f.r[f.nf++] = 4; m /= 4; }
     const int primes[4] = {2, 3, 5, 7};
     for (int pi = 0; pi < 4; ++pi) {
         while (m % primes[pi] == 0) {
@@ -39,6 +45,24 @@ inline bool fft_factorize(int n, FftFactors& f) {
     return m == 1;
 }
 
+// Pass twiddles of one transform length, n - 1 values: pass s (P = product
+// of the earlier radices, radix R) holds exp(+2*pi*i*j*k/(P*R)) for k < P,
+// j = 1..R-1 at offset (P - 1) + k*(R - 1) + (j - 1).  Pass 0 (P = 1) needs
+// none.  Host side, FP64 angles rounded once.
+template <class Push>
+inline void fft_pass_twiddles(const FftFactors& f, Push push) {
+    int p = 1;
+    for (int s = 0; s < f.nf; ++s) {
+        const int R = f.r[s];
+        for (int k = 0; k < p; ++k)
+            for (int j = 1; j < R; ++j) {
+                const double a = 2.0 * 3.14159265358979323846 * (double)j * (double)k / ((double)p * R);
+                push((float)std::cos(a), (float)std::sin(a));
+            }
+        p *= R;
+    }
+}
+
 __host__ __device__ inline float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -46,7 +70,26 @@ __host__ __device__ inline float2 cmulf(float2 a, float2 b) {
 // In-register length-R DFT: X_q = sum_j v_j exp(sgn*2*pi*i*j*q/R).
 template <int R>
 __host__ __device__ inline void dft_small(float2 (&v)[R], float sgn) {
-    if constexpr (R == 2) {
+    if constexpr (R == 8) {
+        // two length-4 DFTs (even / odd inputs) and the w8^q factors,
+        // w8 = exp(sgn*2*pi*i/8) = (1 + sgn*i)/sqrt(2)
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft_small<4>(e, sgn);
+        dft_small<4>(o, sgn);
+        const float h = 0.70710678118654752f;
+        const float2 o1 = make_float2(h * (o[1].x - sgn * o[1].y), h * (o[1].y + sgn * o[1].x));
+        const float2 o2 = make_float2(-sgn * o[2].y, sgn * o[2].x);
+        const float2 o3 = make_float2(h * (-o[3].x - sgn * o[3].y), h * (-o[3].y + sgn * o[3].x));
+        v[0] = make_float2(e[0].x + o[0].x, e[0].y + o[0].y);
+        v[4] = make_float2(e[0].x - o[0].x, e[0].y - o[0].y);
+        v[1] = make_float2(e[1].x + o1.x, e[1].y + o1.y);
+        v[5] = make_float2(e[1].x - o1.x, e[1].y - o1.y);
+        v[2] = make_float2(e[2].x + o2.x, e[2].y + o2.y);
+        v[6] = make_float2(e[2].x - o2.x, e[2].y - o2.y);
+        v[3] = make_float2(e[3].x + o3.x, e[3].y + o3.y);
+        v[7] = make_float2(e[3].x - o3.x, e[3].y - o3.y);
+    } else if constexpr (R == 2) {
         float2 a = v[0], b = v[1];
         v[0] = make_float2(a.x + b.x, a.y + b.y);
         v[1] = make_float2(a.x - b.x, a.y - b.y);
@@ -136,54 +179,74 @@ __host__ __device__ inline int mod_small(int i, int p, unsigned pinv) {
 // largest transform length the small form handles
 constexpr int kFftSmallMax = 1024;
 
-// One butterfly of a Stockham pass over T interleaved transforms
-// (T = 1 << log2T); tw[t] = exp(+2*pi*i*t/n), t < n; step = n/(p*R).
-template <int R, bool BIG = false>
-__host__ __device__ inline void stockham_butterfly(const float2* __restrict__ src,
-                                                   float2* __restrict__ dst, int n, int log2T,
-                                                   int p, unsigned pinv, int step,
-                                                   const float2* __restrict__ tw, float sgn, int w) {
-    const int T = 1 << log2T;
-    const int nb = n / R;
+// One butterfly of a Stockham pass of radix R over T interleaved transforms
+// (w = butterfly index; nb = n/R; p = product of the earlier radices, pinv
+// its mod_small reciprocal; twp = this pass's twiddles).  TW = false for the
+// first pass (p = 1: no twiddles).
+template <int R, int T, int SGN, bool TW, bool BIG>
+__host__ __device__ __forceinline__ void fft_butterfly(const float2* __restrict__ src, float2* __restrict__ dst, int nb,
+                                              int p, unsigned pinv, const float2* __restrict__ twp, int w) {
+    constexpr int LT = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6;
+    static_assert((1 << LT) == T, "T must be a power of two <= 64");
+    const int i = w >> LT;
     const int t = w & (T - 1);
-    const int i = w >> log2T;
-    const int k = mod_small<BIG>(i, p, pinv);
     float2 v[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = src[(i + j * nb) * T + t];
+    for (int j = 0; j < R; ++j) v[j] = src[w + j * nb * T];
+    int k = 0;
+    if constexpr (TW) {
+        k = mod_small<BIG>(i, p, pinv);
+        const float2* c = twp + k * (R - 1);
 #pragma unroll
-    for (int j = 1; j < R; ++j) {
-        float2 c = tw[j * k * step];
-        c.y *= sgn;
-        v[j] = cmulf(v[j], c);
+        for (int j = 1; j < R; ++j) {
+            const float2 cj = c[j - 1];
+            const float cy = SGN > 0 ? cj.y : -cj.y;
+            v[j] = make_float2(v[j].x * cj.x - v[j].y * cy, v[j].x * cy + v[j].y * cj.x);
+        }
     }
-    dft_small<R>(v, sgn);
-    const int o = (i - k) * R + k;
+    dft_small<R>(v, (float)SGN);
+    float2* d = dst + ((i - k) * R + k) * T + t;
 #pragma unroll
-    for (int j = 0; j < R; ++j) dst[(o + j * p) * T + t] = v[j];
+    for (int j = 0; j < R; ++j) d[j * p * T] = v[j];
 }
 
-// Full transform in LDS, ping-ponging between a and b.  All threads of the
-// block take part; returns the buffer holding the result.  The caller must
-// __syncthreads() after filling `a`.
-template <bool BIG = false>
-__device__ inline float2* fft_lds(float2* a, float2* b, const FftFactors& f, int log2T,
-                                  const float2* tw, float sgn) {
+template <int R, int T, int SGN, bool TW, bool BIG>
+__device__ __forceinline__ void fft_pass(const float2* __restrict__ a, float2* __restrict__ b, int n, int p,
+                                         const float2* __restrict__ twp) {
+    const int nb = n / R;
+    const int total = nb * T;
+    const unsigned pinv = TW ? mod_small_inv<BIG>(p) : 0u;
+    for (int w = threadIdx.x; w < total; w += blockDim.x) fft_butterfly<R, T, SGN, TW, BIG>(a, b, nb, p, pinv, twp, w);
+}
+
+// Full transform in LDS, ping-ponging between a and b; tw = the pass
+// twiddles of fft_pass_twiddles.  All threads of the block take part;
+// returns the buffer holding the result.  The caller must __syncthreads()
+// after filling `a` (and tw).
+template <int T, int SGN, bool BIG = false>
+__device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactors& f, const float2* tw) {
     const int n = f.n;
-    const int T = 1 << log2T;
     int p = 1;
     for (int s = 0; s < f.nf; ++s) {
         const int R = f.r[s];
-        const int total = (n / R) * T;
-        const unsigned pinv = mod_small_inv<BIG>(p);
-        const int step = n / (p * R);
-        for (int w = threadIdx.x; w < total; w += blockDim.x) {
+        const float2* twp = tw + (p - 1);
+        if (s == 0) {
             switch (R) {
-                case 4: stockham_butterfly<4, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                case 2: stockham_butterfly<2, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                case 3: stockham_butterfly<3, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                case 5: stockham_butterfly<5, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
-                default: stockham_butterfly<7, BIG>(a, b, n, log2T, p, pinv, step, tw, sgn, w); break;
+                case 8: fft_pass<8, T, SGN, false, BIG>(a, b, n, p, twp); break;
+                case 4: fft_pass<4, T, SGN, false, BIG>(a, b, n, p, twp); break;
+                case 2: fft_pass<2, T, SGN, false, BIG>(a, b, n, p, twp); break;
+                case 3: fft_pass<3, T, SGN, false, BIG>(a, b, n, p, twp); break;
+                case 5: fft_pass<5, T, SGN, false, BIG>(a, b, n, p, twp); break;
+                default: fft_pass<7, T, SGN, false, BIG>(a, b, n, p, twp); break;
+            }
+        } else {
+            switch (R) {
+                case 8: fft_pass<8, T, SGN, true, BIG>(a, b, n, p, twp); break;
+                case 4: fft_pass<4, T, SGN, true, BIG>(a, b, n, p, twp); break;
+                case 2: fft_pass<2, T, SGN, true, BIG>(a, b, n, p, twp); break;
+                case 3: fft_pass<3, T, SGN, true, BIG>(a, b, n, p, twp); break;
+                case 5: fft_pass<5, T, SGN, true, BIG>(a, b, n, p, twp); break;
+                default: fft_pass<7, T, SGN, true, BIG>(a, b, n, p, twp); break;
             }
         }
         __syncthreads();

@@ -246,6 +246,20 @@ __device__ __forceinline__ int lanes_of(const NuRun& r, int BCT, int ch) {
 
 __device__ __forceinline__ int64_t chunk_stride(const NuRun& r, int BCT) { return r.sense ? r.C : BCT; }
 
+// Transforms per work-group of the FFT passes (compile-time): the row
+// passes take all BCT transforms of a chunk for max(1, 8/BCT) rows, the
+// column passes 8 transforms (8/CL columns of CL = min(BCT, 8) transforms).
+// Grids with a side above kFftSmallMax (BIG) take one row / one column per
+// work-group so that the two LDS buffers fit.
+__host__ __device__ constexpr int clog2(int v) { return v <= 1 ? 0 : 1 + clog2(v / 2); }
+__host__ __device__ constexpr int rows_T(int BCT, bool BIG) { return BIG || BCT >= 8 ? BCT : 8; }
+__host__ __device__ constexpr int cols_CL(int BCT) { return BCT < 8 ? BCT : 8; }
+__host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_CL(BCT) : 8; }
+
+// Forward rows: coil multiply + deapodisation, image written wrapped into
+// the zero-padded row (positions (i1 - N1/2) mod G1), FFT along g1.  Lane
+// index = (position, row, transform) with the transform fastest.  Offsets
+// inside a slice are 32-bit (the plan checks V*BCT and G1*G2*BCT < 2^31).
 template <int BCT, bool BIG>
 __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
@@ -254,129 +268,117 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ x,
                                                    float2* __restrict__ ws, NuRun run) {
+    constexpr int T = rows_T(BCT, BIG), LT = clog2(T), LB = clog2(BCT);
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
     if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
-    const int ch = wk.y, r0 = wk.z, b0 = wk.w;
-    const int N1 = S.N1, N2 = S.N2, G1 = S.G1, rpw = S.frpw;
-    const int LL = 1 << S.lfl;           // transforms (coils) of this work-group: b0 .. b0+LL-1
-    const int T = rpw * LL, log2T = ilog2(T);
-    const int64_t V = (int64_t)N1 * N2;
+    const int ch = wk.y, r0 = wk.z;
+    const int N1 = S.N1, N2 = S.N2, G1 = S.G1;
+    const int V = N1 * N2;
     float2* A = lds;
     float2* Bb = lds + G1 * T;
     float2* tw = lds + 2 * G1 * T;
-    for (int i = threadIdx.x; i < G1; i += blockDim.x) tw[i] = twp[S.tw1 + i];
-    for (int i = threadIdx.x; i < G1 * T; i += blockDim.x) A[i] = make_float2(0.f, 0.f);
-    __syncthreads();
+    for (int i = threadIdx.x; i < G1 - 1; i += blockDim.x) tw[i] = twp[S.tw1 + i];
+    for (int i = (N1 - N1 / 2) * T + threadIdx.x; i < (G1 - N1 / 2) * T; i += blockDim.x)
+        A[i] = make_float2(0.f, 0.f);
     const int lanes = lanes_of(run, BCT, ch);
-    const int total = T * N1;
+    const float* d1 = deap + S.deap1;
+    const float* d2 = deap + S.deap2;
+    const float2* xs = x + S.img_off + (run.sense ? (int64_t)ch * V : (int64_t)ch * BCT * V);
+    const float2* mp = run.maps + S.maps_off;
+    const int total = N1 * T;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int i1 = idx % N1;
-        const int rest = idx / N1;
-        const int bl = rest & (LL - 1);
-        const int b = b0 + bl;
-        const int r = rest >> S.lfl;
-        const int i2 = r0 + r;
-        if (i2 >= N2 || b >= lanes) continue;
-        const int64_t v = (int64_t)i2 * N1 + i1;
-        float2 val;
-        if (run.sense) {
-            const float2 im = x[S.img_off + (int64_t)ch * V + v];
-            const float2 mp = run.maps[S.maps_off + (int64_t)b * V + v];
-            val = cmulf(im, mp);
-        } else {
-            val = x[S.img_off + ((int64_t)ch * BCT + b) * V + v];
+        const int t = idx & (T - 1);
+        const int i1 = idx >> LT;
+        const int bl = t & (BCT - 1);
+        const int i2 = r0 + (t >> LB);
+        float2 val = make_float2(0.f, 0.f);
+        if (i2 < N2 && bl < lanes) {
+            const int v = i2 * N1 + i1;
+            val = run.sense ? cmulf(xs[v], mp[bl * V + v]) : xs[bl * V + v];
+            const float f = d1[i1] * d2[i2];
+            val = make_float2(val.x * f, val.y * f);
         }
-        const float f = deap[S.deap1 + i1] * deap[S.deap2 + i2];
         int pos = i1 - N1 / 2;
         if (pos < 0) pos += G1;
-        A[(pos * rpw + r) * LL + bl] = make_float2(val.x * f, val.y * f);
+        A[pos * T + t] = val;
     }
     __syncthreads();
-    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac1], log2T, tw, 1.f);
+    const float2* R = fft_lds<T, 1, BIG>(A, Bb, facs[S.fac1], tw);
     float2* out = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    const int tot2 = T * G1;
+    const int tot2 = G1 * T;
     for (int idx = threadIdx.x; idx < tot2; idx += blockDim.x) {
-        const int bl = idx & (LL - 1);
-        const int rest = idx >> S.lfl;
-        const int g1 = rest % G1;
-        const int r = rest / G1;
-        const int i2 = r0 + r;
+        const int t = idx & (T - 1);
+        const int i2 = r0 + (t >> LB);
         if (i2 >= N2) continue;
-        out[((int64_t)i2 * G1 + g1) * BCT + b0 + bl] = R[(g1 * rpw + r) * LL + bl];
+        out[(i2 * G1 + (idx >> LT)) * BCT + (t & (BCT - 1))] = R[idx];
     }
 }
 
-// FFT along g2 for `cpw` columns.  Forward (adj = 0): input rows 0..N2-1 are
-// placed at (i2 - N2/2) mod G2, output G2 rows.  Adjoint: input G2 rows,
-// output rows i2 < N2 read from (i2 - N2/2) mod G2.  In place in `ws`.
+// FFT along g2 for 8/CL columns of CL transforms.  Forward (adj = 0): input
+// rows 0..N2-1 are placed at (i2 - N2/2) mod G2, output G2 rows.  Adjoint:
+// input G2 rows, output rows i2 < N2 read from (i2 - N2/2) mod G2.  In place
+// in `ws`.
 template <int BCT, bool BIG, int ADJ>
 __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl,
                                                const int4* __restrict__ work,
                                                const FftFactors* __restrict__ facs,
                                                const float2* __restrict__ twp,
                                                float2* __restrict__ ws, const int* __restrict__ skip) {
-    constexpr int adj = ADJ;
+    constexpr int T = cols_T(BCT, BIG), LT = clog2(T), CL = cols_CL(BCT), LCL = clog2(CL), CPW = T / CL;
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
     if (skip && skip[wk.x]) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y, c0 = wk.z, b0 = wk.w;
-    const int N2 = S.N2, G1 = S.G1, G2 = S.G2, cpw = S.cpw;
-    const int lcl = S.lcl;                // transforms (coils) b0 .. b0 + 2^lcl - 1 of `cpw` columns
-    const int T = cpw << lcl, log2T = ilog2(T);
-    const int CL = 1 << lcl;
+    const int N2 = S.N2, G1 = S.G1, G2 = S.G2;
     float2* A = lds;
     float2* Bb = lds + G2 * T;
     float2* tw = lds + 2 * G2 * T;
-    float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    for (int i = threadIdx.x; i < G2; i += blockDim.x) tw[i] = twp[S.tw2 + i];
-    const int ncol = min(cpw, G1 - c0);
-    if (!adj) {
-        for (int i = threadIdx.x; i < G2 * T; i += blockDim.x) A[i] = make_float2(0.f, 0.f);
-        __syncthreads();
+    float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b0;
+    for (int i = threadIdx.x; i < G2 - 1; i += blockDim.x) tw[i] = twp[S.tw2 + i];
+    const int ncol = min(CPW, G1 - c0);
+    if (!ADJ) {
+        for (int i = (N2 - N2 / 2) * T + threadIdx.x; i < (G2 - N2 / 2) * T; i += blockDim.x)
+            A[i] = make_float2(0.f, 0.f);
         const int total = N2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
-            const int i2 = idx >> log2T;
-            const int col = t >> lcl;
-            if (col >= ncol) continue;
+            const int i2 = idx >> LT;
+            const int col = t >> LCL;
             int pos = i2 - N2 / 2;
             if (pos < 0) pos += G2;
-            A[pos * T + t] = g[((int64_t)i2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))];
+            A[pos * T + t] = col < ncol ? g[(i2 * G1 + c0 + col) * BCT + (t & (CL - 1))] : make_float2(0.f, 0.f);
         }
     } else {
         const int total = G2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
-            const int g2 = idx >> log2T;
-            const int col = t >> lcl;
-            A[idx] = col < ncol ? g[((int64_t)g2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))]
-                                : make_float2(0.f, 0.f);
+            const int col = t >> LCL;
+            A[idx] = col < ncol ? g[((idx >> LT) * G1 + c0 + col) * BCT + (t & (CL - 1))] : make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
-    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac2], log2T, tw, ADJ ? -1.f : 1.f);
-    if (!adj) {
+    const float2* R = fft_lds<T, ADJ ? -1 : 1, BIG>(A, Bb, facs[S.fac2], tw);
+    if (!ADJ) {
         const int total = G2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
-            const int g2 = idx >> log2T;
-            const int col = t >> lcl;
+            const int col = t >> LCL;
             if (col >= ncol) continue;
-            g[((int64_t)g2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))] = R[idx];
+            g[((idx >> LT) * G1 + c0 + col) * BCT + (t & (CL - 1))] = R[idx];
         }
     } else {
         const int total = N2 * T;
         for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
             const int t = idx & (T - 1);
-            const int i2 = idx >> log2T;
-            const int col = t >> lcl;
+            const int i2 = idx >> LT;
+            const int col = t >> LCL;
             if (col >= ncol) continue;
             int pos = i2 - N2 / 2;
             if (pos < 0) pos += G2;
-            g[((int64_t)i2 * G1 + c0 + col) * BCT + b0 + (t & (CL - 1))] = R[pos * T + t];
+            g[(i2 * G1 + c0 + col) * BCT + (t & (CL - 1))] = R[pos * T + t];
         }
     }
 }
@@ -1124,6 +1126,10 @@ __global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl
                                    area, area + CELLS, s_last);
 }
 
+// Adjoint rows: FFT along g1, crop to N1 positions, deapodisation and (SENSE)
+// the conjugate-coil combine.  In the combine two lanes share an output
+// pixel, each summing half of the coils; the halves are added through a
+// lane shuffle.
 template <int BCT, bool BIG>
 __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
@@ -1132,68 +1138,83 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ ws,
                                                    float2* __restrict__ x, NuRun run) {
+    constexpr int T = rows_T(BCT, BIG), LT = clog2(T), LB = clog2(BCT), RPW = T / BCT, LR = clog2(RPW);
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
     if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y, r0 = wk.z;
-    const int N1 = S.N1, N2 = S.N2, G1 = S.G1, rpw = S.rpw;
-    const int T = rpw * BCT, log2T = ilog2(T);
-    const int64_t V = (int64_t)N1 * N2;
+    const int N1 = S.N1, N2 = S.N2, G1 = S.G1;
+    const int V = N1 * N2;
     float2* A = lds;
     float2* Bb = lds + G1 * T;
     float2* tw = lds + 2 * G1 * T;
-    for (int i = threadIdx.x; i < G1; i += blockDim.x) tw[i] = twp[S.tw1 + i];
+    for (int i = threadIdx.x; i < G1 - 1; i += blockDim.x) tw[i] = twp[S.tw1 + i];
     const float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    const int total = T * G1;
+    const int total = G1 * T;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int b = idx & (BCT - 1);
-        const int rest = idx / BCT;
-        const int g1 = rest % G1;
-        const int r = rest / G1;
-        const int i2 = r0 + r;
-        A[(g1 * rpw + r) * BCT + b] =
-            i2 < N2 ? g[((int64_t)i2 * G1 + g1) * BCT + b] : make_float2(0.f, 0.f);
+        const int t = idx & (T - 1);
+        const int i2 = r0 + (t >> LB);
+        A[idx] = i2 < N2 ? g[(i2 * G1 + (idx >> LT)) * BCT + (t & (BCT - 1))] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    const float2* R = fft_lds<BIG>(A, Bb, facs[S.fac1], log2T, tw, -1.f);
+    const float2* R = fft_lds<T, -1, BIG>(A, Bb, facs[S.fac1], tw);
     const int lanes = lanes_of(run, BCT, ch);
+    const float* d1 = deap + S.deap1;
+    const float* d2 = deap + S.deap2;
     if (run.sense) {
-        const int tot = rpw * N1;
-        for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-            const int i1 = idx % N1;
-            const int r = idx / N1;
+        constexpr int H = BCT >= 2 ? 2 : 1, HB = BCT / H;
+        const float2* mp = run.maps + S.maps_off;
+        float2* xo = x + S.img_off + (int64_t)ch * V;
+        const int tot = N1 * RPW * H;
+        // whole waves run every iteration (the shuffle needs both lanes)
+        for (int idx0 = 0; idx0 < tot; idx0 += blockDim.x) {
+            const int idx = idx0 + threadIdx.x;
+            const int h = idx & (H - 1);
+            const int r = (idx / H) & (RPW - 1);
+            const int i1 = (idx / H) >> LR;
             const int i2 = r0 + r;
-            if (i2 >= N2) continue;
+            const bool live = idx < tot && i2 < N2;
+            float2 s = make_float2(0.f, 0.f);
             int pos = i1 - N1 / 2;
             if (pos < 0) pos += G1;
-            const int64_t v = (int64_t)i2 * N1 + i1;
-            const float2* src = R + (pos * rpw + r) * BCT;
-            float2 s = make_float2(0.f, 0.f);
-            for (int b = 0; b < lanes; ++b) {
-                const float2 mp = run.maps[S.maps_off + (int64_t)b * V + v];
-                const float2 q = src[b];
-                s.x += mp.x * q.x + mp.y * q.y;   // conj(maps) * q
-                s.y += mp.x * q.y - mp.y * q.x;
+            if (live) {
+                const int v = i2 * N1 + i1;
+                const float2* src = R + pos * T + r * BCT;
+#pragma unroll
+                for (int q = 0; q < HB; ++q) {
+                    const int b = h * HB + q;
+                    if (b < lanes) {
+                        const float2 m = mp[b * V + v];
+                        const float2 u = src[b];
+                        s.x += m.x * u.x + m.y * u.y;   // conj(maps) * u
+                        s.y += m.x * u.y - m.y * u.x;
+                    }
+                }
             }
-            const float f = deap[S.deap1 + i1] * deap[S.deap2 + i2];
-            x[S.img_off + (int64_t)ch * V + v] = make_float2(s.x * f, s.y * f);
+            if constexpr (H == 2) {
+                s.x += __shfl_xor(s.x, 1);
+                s.y += __shfl_xor(s.y, 1);
+            }
+            if (live && h == 0) {
+                const float f = d1[i1] * d2[i2];
+                xo[i2 * N1 + i1] = make_float2(s.x * f, s.y * f);
+            }
         }
     } else {
-        const int tot = T * N1;
+        const int tot = N1 * T;
+        float2* xo = x + S.img_off + (int64_t)ch * BCT * V;
         for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-            const int i1 = idx % N1;
-            const int rest = idx / N1;
-            const int b = rest & (BCT - 1);
-            const int r = rest / BCT;
-            const int i2 = r0 + r;
+            const int t = idx & (T - 1);
+            const int i1 = idx >> LT;
+            const int b = t & (BCT - 1);
+            const int i2 = r0 + (t >> LB);
             if (i2 >= N2 || b >= lanes) continue;
             int pos = i1 - N1 / 2;
             if (pos < 0) pos += G1;
-            const float2 q = R[(pos * rpw + r) * BCT + b];
-            const float f = deap[S.deap1 + i1] * deap[S.deap2 + i2];
-            x[S.img_off + ((int64_t)ch * BCT + b) * V + (int64_t)i2 * N1 + i1] =
-                make_float2(q.x * f, q.y * f);
+            const float2 q = R[pos * T + t];
+            const float f = d1[i1] * d2[i2];
+            xo[b * V + i2 * N1 + i1] = make_float2(q.x * f, q.y * f);
         }
     }
 }
@@ -1227,8 +1248,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         if (!fft_factorize(G, f)) fail(MM_ERR_ARG, "nufftOp: grid size with a prime factor > 7");
         std::pair<int, int> r((int)facs.size(), (int)twid.size());
         facs.push_back(f);
-        for (int t = 0; t < G; ++t)
-            twid.push_back(make_float2((float)std::cos(2 * M_PI * t / G), (float)std::sin(2 * M_PI * t / G)));
+        fft_pass_twiddles(f, [&](float c, float sn) { twid.push_back(make_float2(c, sn)); });
         fft_of[G] = r;
         return r;
     };
@@ -1617,27 +1637,25 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.lds_gt = 256 * sizeof(float2);   // the part sums of dense items
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
-        // rows / columns per work-group: aim for 8-16 interleaved transforms
-        // within 64 KB of LDS
-        // adjoint rows combine the coils: all BCT transforms in one work-group
-        int rpw = std::max(1, rows_T_ / BCT);
-        while (rpw > 1 && fft_lds_bytes(d.G1, rpw * BCT) > 65536) rpw >>= 1;
-        // forward rows: all BCT transforms per work-group as well (splitting
-        // the coils of a row leaves too little work per FFT pass); columns:
-        // min(BCT, target) transforms of one chunk per work-group (a chunk
-        // of 16 coils is split in two at target 8)
-        const int fl = BCT;
-        const int frpw = rpw;
-        const int cl = std::min(BCT, cols_T_);
-        int cpw = std::max(1, cols_T_ / BCT);
-        while (cpw > 1 && fft_lds_bytes(d.G2, cpw * cl) > 65536) cpw >>= 1;
+        // rows / columns per work-group (the kernels' compile-time tiling):
+        // rows take all BCT transforms of max(1, 8/BCT) rows, columns 8
+        // transforms (8/CL columns of CL = min(BCT, 8) transforms)
+        const int TR = big_ ? rows_T(BCT, true) : rows_T(BCT, false);
+        const int TC = big_ ? cols_T(BCT, true) : cols_T(BCT, false);
+        const int rpw = TR / BCT;
+        const int cl = cols_CL(BCT);
+        const int cpw = TC / cl;
         d.rpw = rpw;
-        d.frpw = frpw;
-        d.lfl = ilog2(fl);
+        d.frpw = rpw;
+        d.lfl = ilog2(BCT);
         d.cpw = cpw;
         d.lcl = ilog2(cl);
-        c.lds_rows = std::max({c.lds_rows, fft_lds_bytes(d.G1, rpw * BCT), fft_lds_bytes(d.G1, frpw * fl)});
-        c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, cpw * cl));
+        c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR));
+        c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, TC));
+        if (c.lds_rows > 160 * 1024 || c.lds_cols > 160 * 1024)
+            fail(MM_ERR_UNSUPPORTED, "nufftOp: grid side too large for the LDS FFT passes");
+        if ((int64_t)d.N1 * d.N2 * BCT >= INT32_MAX || (int64_t)d.G1 * d.G2 * BCT >= INT32_MAX)
+            fail(MM_ERR_UNSUPPORTED, "nufftOp: slice too large for 32-bit offsets");
         if (sense) {
             d.img_off = img_start_[s] * nrep;
             d.maps_off = img_start_[s] * C;
@@ -1660,8 +1678,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         ytoff += (int64_t)d.M * BCT * nchunk;
         for (int ch = 0; ch < nchunk; ++ch) {
             for (int r = 0; r < d.N2; r += rpw) wr.push_back(make_int4(s, ch, r, 0));
-            for (int r = 0; r < d.N2; r += frpw)
-                for (int b0 = 0; b0 < BCT; b0 += fl) wfr.push_back(make_int4(s, ch, r, b0));
+            for (int r = 0; r < d.N2; r += rpw) wfr.push_back(make_int4(s, ch, r, 0));
             if (d.G2 > 1)
                 for (int q = 0; q < d.G1; q += cpw)
                     for (int b0 = 0; b0 < BCT; b0 += cl) wc.push_back(make_int4(s, ch, q, b0));

@@ -101,18 +101,10 @@ class NufftOp : public mm_op {
     void profile_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                         const float* W, int iters,
                         std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
-    // tuning: transforms per work-group of the row / column FFT passes
-    // (options "nu_rows_T", "nu_cols_T"; powers of two, default 8 / 8),
-    // degridding variant ("nu_degrid_t": 1 = one lane per (sample, coil)),
+    // tuning: degridding variant ("nu_degrid_t": 1 = one lane per (sample, coil)),
     // 2-D gridding from LDS-staged rectangles ("nu_grid_stage": 1) or from
     // the global CSR rows only (0)
     bool set_option(const std::string& key, int value) override {
-        if (key == "nu_rows_T" || key == "nu_cols_T") {
-            if (value < 1 || value > 64 || (value & (value - 1))) return false;
-            (key == "nu_rows_T" ? rows_T_ : cols_T_) = value;
-            cfgs_.clear();   // work tables and LDS sizes depend on it
-            return true;
-        }
         if (key == "nu_degrid_t") {
             degrid_t_ = value != 0;
             return true;
@@ -200,7 +192,6 @@ class NufftOp : public mm_op {
     // in sample order; row pointers are relative to the slice's ent_off
     bool csr_ = false;
     bool degrid_t_ = true;   // BCT lanes per sample in degridding (option "nu_degrid_t")
-    int rows_T_ = 8, cols_T_ = 8;
     // XCD-aware order of the gridding items (option "nu_xcd_order"); off: on
     // the MI355X it measured slower than the heaviest-first order (318 vs 297 us
     // per launch for 100 slices x 8 coils)
