@@ -67,30 +67,63 @@ int nufft_kb_nkern(int kwidth, double os) {
 // Rectangles of the staged 2-D gridding for one trajectory, given its CSR
 // rows (rp: G1*G2+1 pointers from 0, en: (sample, weight) entries): 16x16
 // tiles, cut into quadrants while the union of their samples exceeds
-// kStageSamples or their entries kStageEntries; a rectangle of at most 4x4
-// cells that still exceeds them is summed from the global rows (kind 1).
+// kStageSamples or their entries kStageEntries.  A rectangle of at most 4x4
+// cells that still exceeds them (dense k-space centre) is cut into chunks of
+// consecutive samples of its sorted union, each within both limits (kind 2).
 // Staged rows keep the sample order of the global rows (local sample ids are
-// ranks in the sorted union).
+// ranks in the sorted union, or in the chunk).  npcells / ndense count the
+// cells of the partial blocks and the dense rectangles of the trajectory.
 static void build_grid_tiles(int G1, int G2, int M, const int* rp, const int2* en,
                              std::vector<GridTile>& tiles, std::vector<int>& stage,
-                             std::vector<int>& ptr, std::vector<int2>& lent) {
+                             std::vector<int>& ptr, std::vector<int2>& lent, int& npcells, int& ndense) {
     std::vector<int> stamp((size_t)M, -1), loc((size_t)M, 0), uni;
     std::vector<int4> todo;
     for (int a2 = G2 - ((G2 - 1) % 16 + 1); a2 >= 0; a2 -= 16)
         for (int a1 = G1 - ((G1 - 1) % 16 + 1); a1 >= 0; a1 -= 16)
             todo.push_back(make_int4(a1, a2, std::min(16, G1 - a1), std::min(16, G2 - a2)));
+    npcells = 0;
+    ndense = 0;
     int id = 0;
+    // appends a staged tile over the samples uni[u0, u1) (sorted) of rectangle r
+    auto emit = [&](const int4& r, int kind, int u0, int u1) {
+        GridTile t{};
+        t.g1 = r.x;
+        t.g2 = r.y;
+        t.w = r.z;
+        t.h = r.w;
+        t.kind = kind;
+        t.n_stage = u1 - u0;
+        t.stage_off = (int)stage.size();
+        t.ptr_off = (int)ptr.size();
+        t.ent_off = (int)lent.size();
+        for (int i = u0; i < u1; ++i) loc[(size_t)uni[i]] = i - u0;
+        stage.insert(stage.end(), uni.begin() + u0, uni.begin() + u1);
+        const int mlo = uni[u0], mhi = uni[u1 - 1];
+        int run = 0;
+        ptr.push_back(0);
+        for (int y = r.y; y < r.y + r.w; ++y)
+            for (int x = r.x; x < r.x + r.z; ++x) {
+                const int c = y * G1 + x;
+                for (int q = rp[c]; q < rp[c + 1]; ++q) {
+                    const int m = en[q].x;
+                    if (m < mlo || m > mhi) continue;
+                    lent.push_back(make_int2(loc[(size_t)m], en[q].y));
+                    ++run;
+                }
+                ptr.push_back(run);
+            }
+        t.n_ent = run;
+        tiles.push_back(t);
+    };
     while (!todo.empty()) {
         const int4 r = todo.back();
         todo.pop_back();
         uni.clear();
         int64_t ne = 0;
-        int mx = 0;
         for (int y = r.y; y < r.y + r.w; ++y)
             for (int x = r.x; x < r.x + r.z; ++x) {
                 const int c = y * G1 + x;
                 ne += rp[c + 1] - rp[c];
-                mx = std::max(mx, rp[c + 1] - rp[c]);
                 for (int q = rp[c]; q < rp[c + 1]; ++q)
                     if (stamp[en[q].x] != id) {
                         stamp[en[q].x] = id;
@@ -98,35 +131,50 @@ static void build_grid_tiles(int G1, int G2, int M, const int* rp, const int2* e
                     }
             }
         ++id;
-        GridTile t{};
-        t.g1 = r.x;
-        t.g2 = r.y;
-        t.w = r.z;
-        t.h = r.w;
-        t.max_row = mx;
-        t.n_ent = (int)std::min<int64_t>(ne, INT32_MAX);
+        std::sort(uni.begin(), uni.end());
         if ((int)uni.size() <= kStageSamples && ne <= kStageEntries) {
-            std::sort(uni.begin(), uni.end());
+            if (!uni.empty()) emit(r, 0, 0, (int)uni.size());
+            else {
+                GridTile t{};   // no samples: the cells are written as zeros
+                t.g1 = r.x;
+                t.g2 = r.y;
+                t.w = r.z;
+                t.h = r.w;
+                t.ptr_off = (int)ptr.size();
+                t.ent_off = (int)lent.size();
+                for (int i = 0; i <= r.z * r.w; ++i) ptr.push_back(0);
+                tiles.push_back(t);
+            }
+        } else if (r.z <= 4 && r.w <= 4) {
+            // entries per sample inside the rectangle
+            std::vector<int> cnt(uni.size(), 0);
             for (int i = 0; i < (int)uni.size(); ++i) loc[(size_t)uni[i]] = i;
-            t.kind = 0;
-            t.n_stage = (int)uni.size();
-            t.stage_off = (int)stage.size();
-            t.ptr_off = (int)ptr.size();
-            t.ent_off = (int)lent.size();
-            stage.insert(stage.end(), uni.begin(), uni.end());
-            int run = 0;
-            ptr.push_back(0);
             for (int y = r.y; y < r.y + r.w; ++y)
                 for (int x = r.x; x < r.x + r.z; ++x) {
                     const int c = y * G1 + x;
-                    for (int q = rp[c]; q < rp[c + 1]; ++q) lent.push_back(make_int2(loc[(size_t)en[q].x], en[q].y));
-                    run += rp[c + 1] - rp[c];
-                    ptr.push_back(run);
+                    for (int q = rp[c]; q < rp[c + 1]; ++q) cnt[(size_t)loc[(size_t)en[q].x]]++;
                 }
-            tiles.push_back(t);
-        } else if (r.z <= 4 && r.w <= 4) {
-            t.kind = 1;
-            tiles.push_back(t);
+            const size_t first = tiles.size();
+            int u0 = 0, cs = 0, ce = 0;
+            for (int i = 0; i < (int)uni.size(); ++i) {
+                if (cs + 1 > kStageSamples || ce + cnt[(size_t)i] > kStageEntries) {
+                    emit(r, 2, u0, i);
+                    u0 = i;
+                    cs = ce = 0;
+                }
+                ++cs;
+                ce += cnt[(size_t)i];
+            }
+            emit(r, 2, u0, (int)uni.size());
+            const int np = (int)(tiles.size() - first);
+            for (int k = 0; k < np; ++k) {
+                GridTile& t = tiles[first + (size_t)k];
+                t.pofs0 = npcells;
+                t.pofs = npcells + k * r.z * r.w;
+                t.nparts = np;
+            }
+            npcells += np * r.z * r.w;
+            ++ndense;
         } else {
             const int hw = (r.z + 1) / 2, hh = (r.w + 1) / 2;
             if (r.z > hw && r.w > hh) todo.push_back(make_int4(r.x + hw, r.y + hh, r.z - hw, r.w - hh));
@@ -936,161 +984,122 @@ __global__ __launch_bounds__(256) void nu_grid_csr(const NuSliceDev* __restrict_
     }
 }
 
-// Staged CSR gridding: work item = (slice, chunk, rectangle, lp | c0 << 8 |
-// cells << 16).  Kind 0 rectangles first copy, with every lane, the
-// transposed data of their samples ([n_stage][BCT]), their (local sample,
-// weight) rows and row pointers into LDS; each (cell, transform) lane then
-// sums its row from LDS in sample order -- the same products in the same
-// order as a sum over the global CSR row, so the result equals the
-// unstaged gather bit for bit.  Kind 1 rectangles (dense k-space centre)
-// are summed from the global rows: cells [c0, c0 + cells) of the rectangle
-// in one pass, P = 2^lp lane groups per cell each taking a contiguous 1/P
-// of the row, the parts added in order through LDS.
+// Staged CSR gridding: one work item per rectangle (or chunk of a dense
+// rectangle) of one (slice, chunk).  The work-group first copies, with every
+// lane, the transposed data of its samples ([n_stage][BCT]), its (local
+// sample, weight) rows and row pointers into LDS; each (cell, transform) lane
+// then sums its row from LDS in sample order -- the products and the order
+// of a sum over the global CSR row.  Kind 0 writes the grid cells; kind 2
+// (a chunk of a dense rectangle) writes a partial block, which nu_grid_merge
+// adds in chunk order into the grid (deterministic for a given plan).
 template <int BCT>
-__global__ __launch_bounds__(256) void nu_grid_tile(const NuSliceDev* __restrict__ sl,
-                                                    const int4* __restrict__ work,
-                                                    const GridTile* __restrict__ tiles,
+__global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__ work,
                                                     const int* __restrict__ stage,
                                                     const int* __restrict__ tptr,
                                                     const int2* __restrict__ tent,
-                                                    const int* __restrict__ rowptr,
-                                                    const int2* __restrict__ ent,
                                                     const float2* __restrict__ yt, float2* __restrict__ ws,
-                                                    NuRun run) {
+                                                    float2* __restrict__ parts, const int* __restrict__ skip) {
     constexpr int NG = 256 / BCT;  // lane groups
     extern __shared__ float2 lds[];
-    const int4 wk = work[blockIdx.x];
-    if (skipped(run, wk.x)) return;
-    const NuSliceDev& S = sl[wk.x];
-    const GridTile T = tiles[wk.z];
-    const int ch = wk.y;
+    // the item carries every offset: one scalar load before the data loads
+    const GridItem T = work[blockIdx.x];
+    if (skip && skip[T.slice]) return;
     const int b = threadIdx.x & (BCT - 1);
     const int grp = threadIdx.x / BCT;
-    const int lanes = lanes_of(run, BCT, ch);
-    const float2* ych = yt + S.yt_off + (int64_t)ch * S.M * BCT + b;
-    float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b;
-    if (T.kind == 0) {
-        const int cells = T.w * T.h;
-        const int nsv = T.n_stage * BCT;
-        float2* sv = lds;                              // [n_stage][BCT]
-        int2* se = (int2*)(lds + nsv);                 // [n_ent]
-        int* sp = (int*)(se + T.n_ent);                // [cells + 1]
-        const int* sid = stage + T.stage_off;
-        // element i of sv is transform (i mod BCT) = b of sample i / BCT
-        for (int i0 = threadIdx.x; i0 < nsv; i0 += 4 * 256) {
-            int m[4];
+    const int lanes = T.lanes;
+    const float2* ych = yt + T.yt + b;
+    const int cells = T.w * T.h;
+    const int nsv = T.n_stage * BCT;
+    float2* sv = lds;                              // [n_stage][BCT]
+    int2* se = (int2*)(lds + nsv);                 // [n_ent]
+    int* sp = (int*)(se + T.n_ent);                // [cells + 1]
+    const int* sid = stage + T.stage_off;
+    // element i of sv is transform (i mod BCT) = b of sample i / BCT
+    for (int i0 = threadIdx.x; i0 < nsv; i0 += 4 * 256) {
+        int m[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int i = i0 + u * 256;
-                m[u] = i < nsv ? sid[i / BCT] : 0;
-            }
-            float2 v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = ych[(int64_t)m[u] * BCT];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (i0 + u * 256 < nsv) sv[i0 + u * 256] = v[u];
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * 256;
+            m[u] = i < nsv ? sid[i / BCT] : 0;
         }
-        const int2* te = tent + T.ent_off;
-        for (int i0 = threadIdx.x; i0 < T.n_ent; i0 += 4 * 256) {
-            int2 q[4];
+        float2 v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = i0 + u * 256 < T.n_ent ? te[i0 + u * 256] : make_int2(0, 0);
+        for (int u = 0; u < 4; ++u) v[u] = ych[(int64_t)m[u] * BCT];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (i0 + u * 256 < T.n_ent) se[i0 + u * 256] = q[u];
-        }
-        const int* tp = tptr + T.ptr_off;
-        for (int i = threadIdx.x; i <= cells; i += 256) sp[i] = tp[i];
-        __syncthreads();
-        for (int c = grp; c < cells; c += NG) {
-            float2 acc = make_float2(0.f, 0.f);
-            if (b < lanes) {
-                int e = sp[c];
-                const int e1 = sp[c + 1];
-                for (; e + 4 <= e1; e += 4) {
-                    int2 q[4];
-                    float2 v[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) q[i] = se[e + i];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) v[i] = sv[q[i].x * BCT + b];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const float w = __int_as_float(q[i].y);
-                        acc.x = fmaf(w, v[i].x, acc.x);
-                        acc.y = fmaf(w, v[i].y, acc.y);
-                    }
-                }
-                for (; e < e1; ++e) {
-                    const int2 q = se[e];
-                    const float2 v = sv[q.x * BCT + b];
-                    const float w = __int_as_float(q.y);
-                    acc.x = fmaf(w, v.x, acc.x);
-                    acc.y = fmaf(w, v.y, acc.y);
-                }
-            }
-            const int r = c / T.w;
-            zg[((int64_t)(T.g2 + r) * S.G1 + T.g1 + (c - r * T.w)) * BCT] = acc;
-        }
-        return;
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * 256 < nsv) sv[i0 + u * 256] = v[u];
     }
-    // kind 1: one pass over cells [c0, c0 + ncell) of the rectangle
-    float2* red = lds;   // [256]
-    const int lp = wk.w & 0xFF;
-    const int P = 1 << lp;
-    const int c0 = (wk.w >> 8) & 0xFF;
-    const int ncell = wk.w >> 16;
-    const int part = grp & (P - 1);
-    const int cl = grp >> lp;
-    const int* rp = rowptr + S.csr_off;
-    const int2* en = ent + S.ent_off;
-    int cell = -1;
-    if (cl < ncell) {
-        const int cr = (c0 + cl) / T.w;
-        cell = (T.g2 + cr) * S.G1 + T.g1 + (c0 + cl - cr * T.w);
-    }
-    float2 acc = make_float2(0.f, 0.f);
-    if (cell >= 0 && b < lanes) {
-        const int r0 = rp[cell], len = rp[cell + 1] - r0;
-        int e = r0 + (int)(((int64_t)len * part) >> lp);
-        const int e1 = r0 + (int)(((int64_t)len * (part + 1)) >> lp);
-        for (; e + 4 <= e1; e += 4) {
-            int2 q[4];
-            float2 v[4];
+    const int2* te = tent + T.ent_off;
+    for (int i0 = threadIdx.x; i0 < T.n_ent; i0 += 4 * 256) {
+        int2 q[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) q[i] = en[e + i];
+        for (int u = 0; u < 4; ++u) q[u] = i0 + u * 256 < T.n_ent ? te[i0 + u * 256] : make_int2(0, 0);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = ych[(int64_t)q[i].x * BCT];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float w = __int_as_float(q[i].y);
-                acc.x = fmaf(w, v[i].x, acc.x);
-                acc.y = fmaf(w, v[i].y, acc.y);
-            }
-        }
-        for (; e < e1; ++e) {
-            const int2 q = en[e];
-            const float2 v = ych[(int64_t)q.x * BCT];
-            const float w = __int_as_float(q.y);
-            acc.x = fmaf(w, v.x, acc.x);
-            acc.y = fmaf(w, v.y, acc.y);
-        }
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * 256 < T.n_ent) se[i0 + u * 256] = q[u];
     }
-    if (P == 1) {
-        if (cell >= 0) zg[(int64_t)cell * BCT] = acc;
-        return;
-    }
-    red[threadIdx.x] = acc;
+    const int* tp = tptr + T.ptr_off;
+    for (int i = threadIdx.x; i <= cells; i += 256) sp[i] = tp[i];
     __syncthreads();
-    if (part == 0 && cell >= 0) {
-        float2 t = acc;
-        for (int q = 1; q < P; ++q) {
-            const float2 u = red[threadIdx.x + q * BCT];
+    float2* zg = ws + T.zg + b;
+    float2* pb = parts + T.part + (int64_t)T.pofs * BCT + b;   // kind 2: this chunk's block
+    for (int c = grp; c < cells; c += NG) {
+        float2 acc = make_float2(0.f, 0.f);
+        if (b < lanes) {
+            int e = sp[c];
+            const int e1 = sp[c + 1];
+            for (; e + 4 <= e1; e += 4) {
+                int2 q[4];
+                float2 v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) q[i] = se[e + i];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = sv[q[i].x * BCT + b];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float w = __int_as_float(q[i].y);
+                    acc.x = fmaf(w, v[i].x, acc.x);
+                    acc.y = fmaf(w, v[i].y, acc.y);
+                }
+            }
+            for (; e < e1; ++e) {
+                const int2 q = se[e];
+                const float2 v = sv[q.x * BCT + b];
+                const float w = __int_as_float(q.y);
+                acc.x = fmaf(w, v.x, acc.x);
+                acc.y = fmaf(w, v.y, acc.y);
+            }
+        }
+        if (T.kind == 0) {
+            const int r = c / T.w;
+            zg[((int64_t)(T.g2 + r) * T.G1 + T.g1 + (c - r * T.w)) * BCT] = acc;
+        } else {
+            pb[c * BCT] = acc;
+        }
+    }
+}
+
+// Sum of the partial blocks of each dense rectangle in chunk order (one item
+// per rectangle and (slice, chunk): its first chunk's record), written to the
+// grid.
+template <int BCT>
+__global__ __launch_bounds__(256) void nu_grid_merge(const GridItem* __restrict__ work,
+                                                     const float2* __restrict__ parts, float2* __restrict__ ws,
+                                                     const int* __restrict__ skip) {
+    const GridItem T = work[blockIdx.x];
+    if (skip && skip[T.slice]) return;
+    const float2* p0 = parts + T.part + (int64_t)T.pofs0 * BCT;
+    const int nv = T.w * T.h * BCT;
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        float2 t = make_float2(0.f, 0.f);
+        for (int k = 0; k < T.nparts; ++k) {
+            const float2 u = p0[(int64_t)k * nv + i];
             t.x += u.x;
             t.y += u.y;
         }
-        zg[(int64_t)cell * BCT] = t;
+        const int c = i / BCT;
+        const int r = c / T.w;
+        ws[T.zg + ((int64_t)(T.g2 + r) * T.G1 + T.g1 + (c - r * T.w)) * BCT + (i & (BCT - 1))] = t;
     }
 }
 
@@ -1320,6 +1329,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                 d.ent_off = e.ent_off;
                 gtile_first_.push_back(gtile_first_[twin]);
                 gtile_count_.push_back(gtile_count_[twin]);
+                gpcells_.push_back(gpcells_[twin]);
                 d.tlist_off = e.tlist_off;
                 tlist_off_.push_back(e.tlist_off);
                 tw_first_.push_back((int)tw_host_.size());
@@ -1416,17 +1426,22 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                 }
             }
             gtile_first_.push_back((int)gtile_host_.size());
+            int npc = 0, nde = 0;
             build_grid_tiles(d.G1, d.G2, d.M, cnt.data(), csr_ent_h.data() + e0, gtile_host_, gstage_h, gptr_h,
-                             gent_h);
+                             gent_h, npc, nde);
             gtile_count_.push_back((int)gtile_host_.size() - gtile_first_.back());
+            gpcells_.push_back(npc);
             {
-                // the rectangles tile the grid and hold every entry once
+                // the rectangles (chunks counted once) tile the grid and hold
+                // every entry once
                 int64_t cells = 0, ents = 0;
                 for (int t = gtile_first_.back(); t < (int)gtile_host_.size(); ++t) {
-                    cells += (int64_t)gtile_host_[(size_t)t].w * gtile_host_[(size_t)t].h;
-                    ents += gtile_host_[(size_t)t].n_ent;
+                    const GridTile& q = gtile_host_[(size_t)t];
+                    if (q.kind == 0 || q.pofs == q.pofs0) cells += (int64_t)q.w * q.h;
+                    ents += q.n_ent;
                 }
-                if (cells != ncell || ents != nent) fail(MM_ERR_HIP, "nufftOp: internal: gridding rectangles do not tile the grid");
+                if (cells != ncell || ents != nent)
+                    fail(MM_ERR_HIP, "nufftOp: internal: gridding rectangles do not tile the grid");
             }
             tlist_off_.push_back((int64_t)tlist.size());
             d.tlist_off = tlist_off_.back();
@@ -1448,6 +1463,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         }
         gtile_first_.push_back((int)gtile_host_.size());
         gtile_count_.push_back(0);
+        gpcells_.push_back(0);
         // gridding tiles: each sample joins every tile its footprint touches
         const int nt1 = (d.G1 + kTile - 1) / kTile, nt2 = (d.G2 + kTile - 1) / kTile;
         std::vector<std::vector<int>> lists((size_t)nt1 * nt2);
@@ -1597,7 +1613,6 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     if (csr_) {
         csr_ptr_.upload(csr_ptr_host_, st);
         csr_ent_.upload(csr_ent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : csr_ent_h, st);
-        gtile_.upload(gtile_host_.empty() ? std::vector<GridTile>(1) : gtile_host_, st);
         gstage_.upload(gstage_h.empty() ? std::vector<int>{0} : gstage_h, st);
         gptr_.upload(gptr_h.empty() ? std::vector<int>{0} : gptr_h, st);
         gent_.upload(gent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : gent_h, st);
@@ -1632,9 +1647,10 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     const int ns = (int)sl_.size();
     c.hs = sl_;
     int64_t wsoff = 0, poff = 0, coff = 0, ytoff = 0;
-    std::vector<int4> wr, wfr, wc, wd, wg, wga, wcsr, wgt;
+    std::vector<int4> wr, wfr, wc, wd, wg, wga, wcsr;
+    std::vector<GridItem> wgt, wgm;
     std::vector<int64_t> wga_cost, wcsr_cost, wgt_cost;
-    c.lds_gt = 256 * sizeof(float2);   // the part sums of dense items
+    c.lds_gt = 16;
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
         // rows / columns per work-group (the kernels' compile-time tiling):
@@ -1668,10 +1684,10 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         d.ws_chunk = (int64_t)d.G1 * d.G2 * BCT;
         d.ws_off = wsoff;
         wsoff += d.ws_chunk * nchunk;
-        d.part_chunk = (int64_t)nslots_[s] * kTile * kTile * BCT;
+        d.part_chunk = csr_ ? (int64_t)gpcells_[s] * BCT : (int64_t)nslots_[s] * kTile * kTile * BCT;
         d.part_off = poff;
         poff += d.part_chunk * nchunk;
-        d.cnt_chunk = ncounters_[s];
+        d.cnt_chunk = csr_ ? 0 : ncounters_[s];
         d.cnt_off = coff;
         coff += d.cnt_chunk * nchunk;
         d.yt_off = ytoff;
@@ -1731,31 +1747,35 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
                     wcsr_cost.push_back(rp[c1] - rp[c0]);
                     c0 = c1;
                 }
-                // staged rectangles: one item each; dense rectangles: one
-                // pass of NG/P cells per item, P from the longest row
+                // staged rectangles and chunks of dense rectangles: one item each
                 for (int t = gtile_first_[s]; t < gtile_first_[s] + gtile_count_[s]; ++t) {
                     const GridTile& T = gtile_host_[(size_t)t];
-                    if (T.kind == 0) {
-                        wgt.push_back(make_int4(s, ch, t, 0));
-                        wgt_cost.push_back((int64_t)T.n_ent + 2 * (int64_t)T.n_stage);
-                        c.lds_gt = std::max(c.lds_gt, (size_t)T.n_stage * BCT * sizeof(float2) +
-                                                          (size_t)T.n_ent * sizeof(int2) +
-                                                          (size_t)(T.w * T.h + 1) * sizeof(int));
-                        continue;
-                    }
-                    int lp = 0;
-                    while ((1 << lp) < NG && T.max_row > 64 * (1 << lp)) ++lp;
-                    const int cpi = NG >> lp, ncl = T.w * T.h;
-                    for (int q = 0; q < ncl; q += cpi) {
-                        const int qn = std::min(cpi, ncl - q);
-                        int64_t ne = 0;
-                        for (int k = q; k < q + qn; ++k) {
-                            const int64_t cell = (int64_t)(T.g2 + k / T.w) * d.G1 + T.g1 + k % T.w;
-                            ne += rp[cell + 1] - rp[cell];
-                        }
-                        wgt.push_back(make_int4(s, ch, t, lp | (q << 8) | (qn << 16)));
-                        wgt_cost.push_back(4 * ne);
-                    }
+                    GridItem it{};
+                    it.yt = d.yt_off + (int64_t)ch * d.M * BCT;
+                    it.zg = d.ws_off + (int64_t)ch * d.ws_chunk;
+                    it.part = d.part_off + (int64_t)ch * d.part_chunk;
+                    it.slice = s;
+                    it.G1 = d.G1;
+                    it.kind = T.kind;
+                    it.lanes = sense ? C : (int)std::min<int64_t>(BCT, nrep - (int64_t)ch * BCT);
+                    it.g1 = T.g1;
+                    it.g2 = T.g2;
+                    it.w = T.w;
+                    it.h = T.h;
+                    it.stage_off = T.stage_off;
+                    it.n_stage = T.n_stage;
+                    it.ptr_off = T.ptr_off;
+                    it.ent_off = T.ent_off;
+                    it.n_ent = T.n_ent;
+                    it.pofs = T.pofs;
+                    it.pofs0 = T.pofs0;
+                    it.nparts = T.nparts;
+                    wgt.push_back(it);
+                    wgt_cost.push_back((int64_t)T.n_ent + 2 * (int64_t)T.n_stage);
+                    if (T.kind == 2 && T.pofs == T.pofs0) wgm.push_back(it);
+                    c.lds_gt = std::max(c.lds_gt, (size_t)T.n_stage * BCT * sizeof(float2) +
+                                                      (size_t)T.n_ent * sizeof(int2) +
+                                                      (size_t)(T.w * T.h + 1) * sizeof(int));
                 }
             }
         }
@@ -1800,7 +1820,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         std::vector<size_t> ord(wgt.size());
         std::iota(ord.begin(), ord.end(), 0);
         std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wgt_cost[a] > wgt_cost[b]; });
-        std::vector<int4> sorted(wgt.size());
+        std::vector<GridItem> sorted(wgt.size());
         for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wgt[ord[i]];
         wgt.swap(sorted);
     }
@@ -1823,8 +1843,10 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.w_grid.upload(wg.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wg, st);
     c.w_csr.upload(wcsr.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wcsr, st);
     c.n_csr = (int)wcsr.size();
-    c.w_gt.upload(wgt.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wgt, st);
+    c.w_gt.upload(wgt.empty() ? std::vector<GridItem>(1) : wgt, st);
     c.n_gt = (int)wgt.size();
+    c.w_gm.upload(wgm.empty() ? std::vector<GridItem>(1) : wgm, st);
+    c.n_gm = (int)wgm.size();
     c.lds_gt = (c.lds_gt + 15) & ~(size_t)15;
     c.n_rows = (int)wr.size();
     c.n_cols = (int)wc.size();
@@ -1889,8 +1911,13 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     auto adjoint_from = [&](const float2* y, float2* x, NuRun r) {
         if (csr_ && grid_stage_) {
             if (c.n_gt > 0)
-                hipLaunchKernelGGL(nu_grid_tile<BCT>, dim3(c.n_gt), dim3(256), c.lds_gt, st, sl, c.w_gt.p, gtile_.p,
-                                   gstage_.p, gptr_.p, gent_.p, csr_ptr_.p, csr_ent_.p, y, c.ws.p, r);
+                hipLaunchKernelGGL(nu_grid_tile<BCT>, dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p, gstage_.p,
+                                   gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
+            if (c.n_gm > 0) {
+                LAUNCHCHECK();
+                hipLaunchKernelGGL(nu_grid_merge<BCT>, dim3(c.n_gm), dim3(256), 0, st, c.w_gm.p, c.parts.p,
+                                   c.ws.p, r.skip);
+            }
         } else if (csr_)
             hipLaunchKernelGGL(nu_grid_csr<BCT>, dim3(c.n_csr), dim3(256), 0, st, sl, c.w_csr.p, csr_ptr_.p,
                                csr_ent_.p, y, c.ws.p, r);

@@ -58,23 +58,39 @@ struct TileWork {
 constexpr int kTile = 16;
 constexpr int kTileChunk = 512;
 
-// One rectangle of grid cells for 2-D CSR gridding.  kind 0: the union of
-// the samples that reach the rectangle (sorted, at most kStageSamples) and
-// the cells' (local sample, weight) rows (at most kStageEntries entries)
-// are staged in LDS; kind 1: the cells are summed straight from the global
-// CSR rows (dense k-space centre).  Offsets are absolute in the pools.
+// One rectangle of grid cells for 2-D CSR gridding, with the union of the
+// samples that reach it (sorted, at most kStageSamples) and the cells'
+// (local sample, weight) rows (at most kStageEntries entries), staged in
+// LDS by its work-group.  kind 0: the whole rectangle, written to the grid;
+// kind 2: one chunk of the samples of a dense rectangle (k-space centre),
+// written to a partial block; a second launch sums the blocks in chunk order
+// into the grid.  Offsets are absolute in the pools; partial offsets (pofs,
+// pofs0, in cells) are relative to the slice.
 struct GridTile {
     int g1, g2, w, h;    // first cell and extent (cells, g1 fastest)
     int kind;
-    int n_stage, n_ent;  // staged samples / entries (kind 0)
+    int n_stage, n_ent;  // staged samples / entries
     int stage_off;       // first sample id in the stage pool
     int ptr_off;         // first of the w*h+1 row pointers (relative to ent_off)
     int ent_off;         // first (local sample, weight) entry
-    int max_row;         // longest cell row of the rectangle
-    int pad;
+    int pofs, pofs0;     // kind 2: this chunk's partial block, the rectangle's first
+    int nparts;          // kind 2: chunks of the rectangle
+    int pad0;
 };
-constexpr int kStageSamples = 256;
-constexpr int kStageEntries = 2048;
+// One work item of the staged gridding: a rectangle of one (slice, chunk)
+// with every offset its work-group needs, so that one scalar load precedes
+// the data loads.
+struct GridItem {
+    int64_t yt;          // transposed data of the (slice, chunk) in yT
+    int64_t zg;          // grid of the (slice, chunk) in the workspace
+    int64_t part;        // partial blocks of the (slice, chunk) (kind 2)
+    int slice, G1, kind, lanes;
+    int g1, g2, w, h;
+    int stage_off, n_stage, ptr_off, ent_off;
+    int n_ent, pofs, pofs0, nparts;
+};
+constexpr int kStageSamples = 128;
+constexpr int kStageEntries = 1024;
 
 class NufftOp : public mm_op {
    public:
@@ -146,8 +162,8 @@ class NufftOp : public mm_op {
         DevBuf<int4> w_csr;      // CSR gridding items: (slice, chunk, first cell, cells)
         int n_csr = 0;
         // staged CSR gridding items: (slice, chunk, tile, lp | c0 << 8 | cells << 16)
-        DevBuf<int4> w_gt;
-        int n_gt = 0;
+        DevBuf<GridItem> w_gt, w_gm;   // staged items; merges of dense rectangles
+        int n_gt = 0, n_gm = 0;
         size_t lds_gt = 0;
     };
     Cfg& config(int64_t nrep, int sense, int C);
@@ -205,7 +221,7 @@ class NufftOp : public mm_op {
     bool grid_stage_ = true;
     std::vector<GridTile> gtile_host_;
     std::vector<int> gtile_first_, gtile_count_;   // per slice
-    DevBuf<GridTile> gtile_;
+    std::vector<int> gpcells_;                     // per slice: cells of the partial blocks
     DevBuf<int> gstage_, gptr_;
     DevBuf<int2> gent_;
     // gather gridding (2-D, every grid side >= 24)
