@@ -519,6 +519,100 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
     }
 }
 
+// Degridding from an LDS-staged grid region (2-D): work item = (slice,
+// chunk, sample block); a block holds up to kDegridBlock samples whose first
+// tap lies in one kDegridTile x kDegridTile tile, in sample order, with
+// their first taps, weights, dcf and ids stored contiguously in block order
+// (plan time).  The work-group copies the block's sample data and the tile's
+// (kDegridTile + W - 1)^2 cells (wrapped) into LDS; each (sample, transform)
+// lane then sums its W^2 taps from LDS in the order and with the products of
+// nu_degrid_t, so the two give identical results.
+template <int BCT, int WT>
+__global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restrict__ sl,
+                                                      const int4* __restrict__ work,
+                                                      const int4* __restrict__ blocks,
+                                                      const int* __restrict__ bid,
+                                                      const uint32_t* __restrict__ bbase,
+                                                      const float* __restrict__ bwt,
+                                                      const float* __restrict__ bdcf,
+                                                      const float2* __restrict__ ws,
+                                                      float2* __restrict__ y, NuRun run) {
+    constexpr int SPP = 256 / BCT;
+    extern __shared__ float2 lds[];
+    const int4 wk = work[blockIdx.x];
+    if (skipped(run, wk.x)) return;
+    const NuSliceDev& S = sl[wk.x];
+    const int4 blk = blocks[wk.z];   // (t1 | t2 << 16, first block-order entry, samples, 0)
+    const int ch = wk.y;
+    const int b = threadIdx.x & (BCT - 1);
+    const int G1 = S.G1, G2 = S.G2;
+    const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2, WS = W1 + W2;
+    const int RW = kDegridTile + (WT > 0 ? WT : (W1 > W2 ? W1 : W2)) - 1;   // staged side
+    const int t1 = blk.x & 0xffff, t2 = blk.x >> 16;
+    const int n = blk.z;
+    const int lanes = lanes_of(run, BCT, ch);
+    const int64_t stride = chunk_stride(run, BCT);
+    const float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
+    // LDS: grid cells [RW*RW][BCT], then per sample: id, first taps, dcf, weights
+    const int nst = RW * RW * BCT;
+    int* s_id = (int*)(lds + nst);
+    uint32_t* s_base = (uint32_t*)(s_id + kDegridBlock);
+    float* s_dcf = (float*)(s_base + kDegridBlock);
+    float* s_w = s_dcf + kDegridBlock;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s_id[i] = bid[blk.y + i];
+        s_base[i] = bbase[blk.y + i];
+        s_dcf[i] = bdcf[blk.y + i];
+    }
+    for (int i = threadIdx.x; i < n * WS; i += blockDim.x) s_w[i] = bwt[(int64_t)blk.y * WS + i];
+    for (int i = threadIdx.x; i < nst; i += blockDim.x) {
+        const int cl = i / BCT;
+        const int l2 = cl / RW, l1 = cl - l2 * RW;
+        int g1 = t1 + l1, g2 = t2 + l2;
+        while (g1 >= G1) g1 -= G1;
+        while (g2 >= G2) g2 -= G2;
+        lds[i] = zg[(g2 * G1 + g1) * BCT + (i & (BCT - 1))];
+    }
+    __syncthreads();
+    for (int k = (int)(threadIdx.x / BCT); k < n; k += SPP) {
+        const int m = s_id[k];
+        const uint32_t base = s_base[k];
+        const int l1_0 = (int)(base & 0xffff) - t1, l2_0 = (int)(base >> 16) - t2;
+        float w1[8], w2[8];
+        const float* wk8 = s_w + k * WS;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            w1[q] = q < W1 ? wk8[q] : 0.f;
+            w2[q] = q < W2 ? wk8[W1 + q] : 0.f;
+        }
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int s2 = 0; s2 < (WT > 0 ? WT : 8); ++s2) {
+            if (s2 >= W2) break;
+            const float2* row = lds + ((l2_0 + s2) * RW + l1_0) * BCT + b;
+#pragma unroll
+            for (int s1 = 0; s1 < (WT > 0 ? WT : 8); ++s1) {
+                if (s1 >= W1) break;
+                const float w = w1[s1] * w2[s2];
+                const float2 q = row[s1 * BCT];
+                acc.x += w * q.x;
+                acc.y += w * q.y;
+            }
+        }
+        const float d = s_dcf[k];
+        if (run.ylayout) {
+            float f = b < lanes ? d : 0.f;
+            if (run.W && b < lanes) f *= run.W[S.dat_off + ((int64_t)ch * stride + b) * S.M + m];
+            y[S.yt_off + ((int64_t)ch * S.M + m) * BCT + b] = make_float2(acc.x * f, acc.y * f);
+        } else if (b < lanes) {
+            const int64_t o = S.dat_off + ((int64_t)ch * stride + b) * S.M + m;
+            float f = d;
+            if (run.W) f *= run.W[o];
+            y[o] = make_float2(acc.x * f, acc.y * f);
+        }
+    }
+}
+
 template <int BCT, int WT>
 __global__ __launch_bounds__(256, 2) void nu_degrid(const NuSliceDev* __restrict__ sl,
                                                  const int4* __restrict__ work,
@@ -1266,7 +1360,9 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     std::vector<float> swt, sdcf;
     std::vector<int> tlist, glist_h, gbin_h;
     std::vector<int2> csr_ent_h;
-    std::vector<int> gstage_h, gptr_h;
+    std::vector<int> gstage_h, gptr_h, dsamp_h;
+    std::vector<uint32_t> dbase_h;
+    std::vector<float> ddcf_h, dwt_h;
     std::vector<int2> gent_h;
     std::map<uint64_t, int> traj_seen;   // trajectory hash -> first slice with it
     int64_t voff = 0, moff = 0;
@@ -1329,6 +1425,8 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                 d.ent_off = e.ent_off;
                 gtile_first_.push_back(gtile_first_[twin]);
                 gtile_count_.push_back(gtile_count_[twin]);
+                dblk_first_.push_back(dblk_first_[twin]);
+                dblk_count_.push_back(dblk_count_[twin]);
                 gpcells_.push_back(gpcells_[twin]);
                 d.tlist_off = e.tlist_off;
                 tlist_off_.push_back(e.tlist_off);
@@ -1425,6 +1523,37 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                     }
                 }
             }
+            {
+                // degridding blocks: samples bucketed by the kDegridTile^2
+                // tile of their first tap, in sample order, cut into blocks of
+                // at most kDegridBlock; ids, first taps, weights and dcf are
+                // copied in that order
+                const int ntx = (d.G1 + kDegridTile - 1) / kDegridTile;
+                const int nty = (d.G2 + kDegridTile - 1) / kDegridTile;
+                std::vector<int> tcnt((size_t)ntx * nty + 1, 0);
+                auto tile_of = [&](int m) { return (g0[1][m] / kDegridTile) * ntx + g0[0][m] / kDegridTile; };
+                for (int m = 0; m < d.M; ++m) tcnt[(size_t)tile_of(m) + 1]++;
+                for (size_t i = 0; i + 1 < tcnt.size(); ++i) tcnt[i + 1] += tcnt[i];
+                const int64_t base = (int64_t)dsamp_h.size();
+                dsamp_h.resize(dsamp_h.size() + (size_t)d.M);
+                std::vector<int> pos(tcnt.begin(), tcnt.end() - 1);
+                for (int m = 0; m < d.M; ++m) dsamp_h[(size_t)(base + pos[(size_t)tile_of(m)]++)] = m;
+                for (int i = 0; i < d.M; ++i) {
+                    const int m = dsamp_h[(size_t)(base + i)];
+                    dbase_h.push_back(sbase[(size_t)(d.samp_off + m)]);
+                    ddcf_h.push_back(sdcf[(size_t)(d.samp_off + m)]);
+                    for (int q = 0; q < Ws; ++q) dwt_h.push_back(swt[(size_t)(d.samp_off + m) * Ws + q]);
+                }
+                dblk_first_.push_back((int)dblk_host_.size());
+                for (int t = 0; t < ntx * nty; ++t)
+                    for (int k = tcnt[(size_t)t]; k < tcnt[(size_t)t + 1]; k += kDegridBlock) {
+                        const int nb = std::min(kDegridBlock, tcnt[(size_t)t + 1] - k);
+                        dblk_host_.push_back(make_int4(((t % ntx) * kDegridTile) | (((t / ntx) * kDegridTile) << 16),
+                                                       (int)(base + k), nb, 0));
+                    }
+                dblk_count_.push_back((int)dblk_host_.size() - dblk_first_.back());
+                if (dwt_h.size() > (size_t)INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples");
+            }
             gtile_first_.push_back((int)gtile_host_.size());
             int npc = 0, nde = 0;
             build_grid_tiles(d.G1, d.G2, d.M, cnt.data(), csr_ent_h.data() + e0, gtile_host_, gstage_h, gptr_h,
@@ -1464,6 +1593,8 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         gtile_first_.push_back((int)gtile_host_.size());
         gtile_count_.push_back(0);
         gpcells_.push_back(0);
+        dblk_first_.push_back((int)dblk_host_.size());
+        dblk_count_.push_back(0);
         // gridding tiles: each sample joins every tile its footprint touches
         const int nt1 = (d.G1 + kTile - 1) / kTile, nt2 = (d.G2 + kTile - 1) / kTile;
         std::vector<std::vector<int>> lists((size_t)nt1 * nt2);
@@ -1616,6 +1747,11 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         gstage_.upload(gstage_h.empty() ? std::vector<int>{0} : gstage_h, st);
         gptr_.upload(gptr_h.empty() ? std::vector<int>{0} : gptr_h, st);
         gent_.upload(gent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : gent_h, st);
+        dblk_.upload(dblk_host_.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : dblk_host_, st);
+        dsamp_.upload(dsamp_h.empty() ? std::vector<int>{0} : dsamp_h, st);
+        dbase_.upload(dbase_h.empty() ? std::vector<uint32_t>{0u} : dbase_h, st);
+        ddcf_.upload(ddcf_h.empty() ? std::vector<float>{0.f} : ddcf_h, st);
+        dwt_.upload(dwt_h.empty() ? std::vector<float>{0.f} : dwt_h, st);
     }
     gwork_.upload(gw_host_.empty() ? std::vector<GatherWork>{GatherWork{}} : gw_host_, st);
     glist_.upload(glist_h.empty() ? std::vector<int>{0} : glist_h, st);
@@ -1649,6 +1785,8 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     int64_t wsoff = 0, poff = 0, coff = 0, ytoff = 0;
     std::vector<int4> wr, wfr, wc, wd, wg, wga, wcsr;
     std::vector<GridItem> wgt, wgm;
+    std::vector<int4> wdb;
+    std::vector<int64_t> wdb_cost;
     std::vector<int64_t> wga_cost, wcsr_cost, wgt_cost;
     c.lds_gt = 16;
     for (int s = 0; s < ns; ++s) {
@@ -1699,6 +1837,11 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
                 for (int q = 0; q < d.G1; q += cpw)
                     for (int b0 = 0; b0 < BCT; b0 += cl) wc.push_back(make_int4(s, ch, q, b0));
             for (int m = 0; m < d.M; m += 256) wd.push_back(make_int4(s, ch, m, 0));
+            if (ndim_ == 2)
+                for (int k = dblk_first_[s]; k < dblk_first_[s] + dblk_count_[s]; ++k) {
+                    wdb.push_back(make_int4(s, ch, k, 0));
+                    wdb_cost.push_back(dblk_host_[(size_t)k].z);
+                }
             // gridding items: .w = 1 scatter chunk, 0 gather rows
             for (int t = 0; t < tw_count_[s]; ++t) {
                 wg.push_back(make_int4(s, ch, tw_first_[s] + t, 1));
@@ -1815,6 +1958,15 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wcsr[ord[i]];
         wcsr.swap(sorted);
     }
+    // largest degridding blocks first
+    {
+        std::vector<size_t> ord(wdb.size());
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wdb_cost[a] > wdb_cost[b]; });
+        std::vector<int4> sorted(wdb.size());
+        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wdb[ord[i]];
+        wdb.swap(sorted);
+    }
     // heaviest staged gridding items first
     {
         std::vector<size_t> ord(wgt.size());
@@ -1851,6 +2003,10 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.n_rows = (int)wr.size();
     c.n_cols = (int)wc.size();
     c.n_deg = (int)wd.size();
+    c.w_db.upload(wdb.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wdb, st);
+    c.n_db = (int)wdb.size();
+    c.lds_db = (size_t)(kDegridTile + W_ - 1) * (kDegridTile + W_ - 1) * BCT * sizeof(float2) +
+               (size_t)kDegridBlock * (3 + 2 * W_) * sizeof(float);
     c.n_grid = (int)wg.size();
     if (gather_) c.yT.alloc(std::max<int64_t>(ytoff, 1));
     c.ws.alloc(std::max<int64_t>(wsoff, 1));
@@ -1876,6 +2032,10 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)nu_cols<BCT, BIG, 1>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)nu_degrid_tile<BCT, 4>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)nu_degrid_tile<BCT, 0>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
         attr_set = true;
     }
@@ -1892,7 +2052,14 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
             LAUNCHCHECK();
             mark("nu_cols");
         }
-        if (BCT >= 2 && degrid_t_) {
+        if (ndim_ == 2 && degrid_stage_ && c.n_db > 0) {
+            if (wt4)
+                hipLaunchKernelGGL((nu_degrid_tile<BCT, 4>), dim3(c.n_db), dim3(256), c.lds_db, st, sl, c.w_db.p,
+                                   dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y, r);
+            else
+                hipLaunchKernelGGL((nu_degrid_tile<BCT, 0>), dim3(c.n_db), dim3(256), c.lds_db, st, sl, c.w_db.p,
+                                   dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y, r);
+        } else if (BCT >= 2 && degrid_t_) {
             if (wt4)
                 hipLaunchKernelGGL((nu_degrid_t<BCT, 4>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p,
                                    sbase_.p, swt_.p, sdcf_.p, (const float2*)c.ws.p, y, r);

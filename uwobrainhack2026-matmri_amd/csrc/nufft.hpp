@@ -89,6 +89,8 @@ struct GridItem {
     int stage_off, n_stage, ptr_off, ent_off;
     int n_ent, pofs, pofs0, nparts;
 };
+constexpr int kDegridTile = 8;      // tile side of the staged degridding
+constexpr int kDegridBlock = 64;    // samples per staged degridding block
 constexpr int kStageSamples = 128;
 constexpr int kStageEntries = 1024;
 
@@ -119,10 +121,15 @@ class NufftOp : public mm_op {
                         std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
     // tuning: degridding variant ("nu_degrid_t": 1 = one lane per (sample, coil)),
     // 2-D gridding from LDS-staged rectangles ("nu_grid_stage": 1) or from
-    // the global CSR rows only (0)
+    // the global CSR rows only (0), 2-D degridding from LDS-staged tiles
+    // ("nu_degrid_stage": 1) or from the grid in global memory (0)
     bool set_option(const std::string& key, int value) override {
         if (key == "nu_degrid_t") {
             degrid_t_ = value != 0;
+            return true;
+        }
+        if (key == "nu_degrid_stage") {
+            degrid_stage_ = value != 0;
             return true;
         }
         if (key == "nu_grid_stage") {
@@ -162,6 +169,9 @@ class NufftOp : public mm_op {
         DevBuf<int4> w_csr;      // CSR gridding items: (slice, chunk, first cell, cells)
         int n_csr = 0;
         // staged CSR gridding items: (slice, chunk, tile, lp | c0 << 8 | cells << 16)
+        DevBuf<int4> w_db;             // staged degridding: (slice, chunk, block)
+        int n_db = 0;
+        size_t lds_db = 0;
         DevBuf<GridItem> w_gt, w_gm;   // staged items; merges of dense rectangles
         int n_gt = 0, n_gm = 0;
         size_t lds_gt = 0;
@@ -219,6 +229,16 @@ class NufftOp : public mm_op {
     // of each distinct 2-D trajectory; slices that share a trajectory share
     // its rectangles
     bool grid_stage_ = true;
+    // staged degridding (option "nu_degrid_stage", default on): sample
+    // blocks of each distinct 2-D trajectory, (tile | tile2 << 16, first
+    // entry of dsamp_, samples, 0)
+    bool degrid_stage_ = true;
+    std::vector<int4> dblk_host_;
+    std::vector<int> dblk_first_, dblk_count_;   // per slice
+    DevBuf<int4> dblk_;
+    DevBuf<int> dsamp_;          // sample ids in block order
+    DevBuf<uint32_t> dbase_;     // first taps in block order
+    DevBuf<float> ddcf_, dwt_;   // dcf, kernel weights (W1 + W2 a sample) in block order
     std::vector<GridTile> gtile_host_;
     std::vector<int> gtile_first_, gtile_count_;   // per slice
     std::vector<int> gpcells_;                     // per slice: cells of the partial blocks
