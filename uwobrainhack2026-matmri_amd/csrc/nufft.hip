@@ -336,21 +336,34 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
     const float2* xs = x + S.img_off + (run.sense ? (int64_t)ch * V : (int64_t)ch * BCT * V);
     const float2* mp = run.maps + S.maps_off;
     const int total = N1 * T;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int t = idx & (T - 1);
-        const int i1 = idx >> LT;
-        const int bl = t & (BCT - 1);
-        const int i2 = r0 + (t >> LB);
-        float2 val = make_float2(0.f, 0.f);
-        if (i2 < N2 && bl < lanes) {
-            const int v = i2 * N1 + i1;
-            val = run.sense ? cmulf(xs[v], mp[bl * V + v]) : xs[bl * V + v];
-            const float f = d1[i1] * d2[i2];
-            val = make_float2(val.x * f, val.y * f);
+    // four elements per thread in flight: every load of a round is issued
+    // before the first LDS store
+    for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
+        float2 val[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = i0 + u * 256;
+            const int t = idx & (T - 1);
+            const int i1 = idx >> LT;
+            const int bl = t & (BCT - 1);
+            const int i2 = r0 + (t >> LB);
+            val[u] = make_float2(0.f, 0.f);
+            if (idx < total && i2 < N2 && bl < lanes) {
+                const int v = i2 * N1 + i1;
+                const float2 q = run.sense ? cmulf(xs[v], mp[bl * V + v]) : xs[bl * V + v];
+                const float f = d1[i1] * d2[i2];
+                val[u] = make_float2(q.x * f, q.y * f);
+            }
         }
-        int pos = i1 - N1 / 2;
-        if (pos < 0) pos += G1;
-        A[pos * T + t] = val;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = i0 + u * 256;
+            if (idx >= total) break;
+            const int i1 = idx >> LT;
+            int pos = i1 - N1 / 2;
+            if (pos < 0) pos += G1;
+            A[pos * T + (idx & (T - 1))] = val[u];
+        }
     }
     __syncthreads();
     const float2* R = fft_lds<T, 1, BIG>(A, Bb, facs[S.fac1], tw);
@@ -391,20 +404,40 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
         for (int i = (N2 - N2 / 2) * T + threadIdx.x; i < (G2 - N2 / 2) * T; i += blockDim.x)
             A[i] = make_float2(0.f, 0.f);
         const int total = N2 * T;
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            const int t = idx & (T - 1);
-            const int i2 = idx >> LT;
-            const int col = t >> LCL;
-            int pos = i2 - N2 / 2;
-            if (pos < 0) pos += G2;
-            A[pos * T + t] = col < ncol ? g[(i2 * G1 + c0 + col) * BCT + (t & (CL - 1))] : make_float2(0.f, 0.f);
+        for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
+            float2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = i0 + u * 256;
+                const int t = idx & (T - 1);
+                const int col = t >> LCL;
+                v[u] = idx < total && col < ncol ? g[((idx >> LT) * G1 + c0 + col) * BCT + (t & (CL - 1))]
+                                                 : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = i0 + u * 256;
+                if (idx >= total) break;
+                int pos = (idx >> LT) - N2 / 2;
+                if (pos < 0) pos += G2;
+                A[pos * T + (idx & (T - 1))] = v[u];
+            }
         }
     } else {
         const int total = G2 * T;
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-            const int t = idx & (T - 1);
-            const int col = t >> LCL;
-            A[idx] = col < ncol ? g[((idx >> LT) * G1 + c0 + col) * BCT + (t & (CL - 1))] : make_float2(0.f, 0.f);
+        for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
+            float2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = i0 + u * 256;
+                const int t = idx & (T - 1);
+                const int col = t >> LCL;
+                v[u] = idx < total && col < ncol ? g[((idx >> LT) * G1 + c0 + col) * BCT + (t & (CL - 1))]
+                                                 : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * 256 < total) A[i0 + u * 256] = v[u];
         }
     }
     __syncthreads();
@@ -565,13 +598,21 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
         s_dcf[i] = bdcf[blk.y + i];
     }
     for (int i = threadIdx.x; i < n * WS; i += blockDim.x) s_w[i] = bwt[(int64_t)blk.y * WS + i];
-    for (int i = threadIdx.x; i < nst; i += blockDim.x) {
-        const int cl = i / BCT;
-        const int l2 = cl / RW, l1 = cl - l2 * RW;
-        int g1 = t1 + l1, g2 = t2 + l2;
-        while (g1 >= G1) g1 -= G1;
-        while (g2 >= G2) g2 -= G2;
-        lds[i] = zg[(g2 * G1 + g1) * BCT + (i & (BCT - 1))];
+    for (int i0 = threadIdx.x; i0 < nst; i0 += 4 * 256) {
+        float2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = min(i0 + u * 256, nst - 1);
+            const int cl = i / BCT;
+            const int l2 = cl / RW, l1 = cl - l2 * RW;
+            int g1 = t1 + l1, g2 = t2 + l2;
+            while (g1 >= G1) g1 -= G1;
+            while (g2 >= G2) g2 -= G2;
+            v[u] = zg[(g2 * G1 + g1) * BCT + (i & (BCT - 1))];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * 256 < nst) lds[i0 + u * 256] = v[u];
     }
     __syncthreads();
     for (int k = (int)(threadIdx.x / BCT); k < n; k += SPP) {
@@ -1186,10 +1227,17 @@ __global__ __launch_bounds__(256) void nu_grid_merge(const GridItem* __restrict_
     const int nv = T.w * T.h * BCT;
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
         float2 t = make_float2(0.f, 0.f);
-        for (int k = 0; k < T.nparts; ++k) {
-            const float2 u = p0[(int64_t)k * nv + i];
-            t.x += u.x;
-            t.y += u.y;
+        for (int k0 = 0; k0 < T.nparts; k0 += 8) {
+            float2 u[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                u[k] = k0 + k < T.nparts ? p0[(int64_t)(k0 + k) * nv + i] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (k0 + k < T.nparts) {
+                    t.x += u[k].x;
+                    t.y += u[k].y;
+                }
         }
         const int c = i / BCT;
         const int r = c / T.w;
@@ -1255,10 +1303,19 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
     for (int i = threadIdx.x; i < G1 - 1; i += blockDim.x) tw[i] = twp[S.tw1 + i];
     const float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int total = G1 * T;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int t = idx & (T - 1);
-        const int i2 = r0 + (t >> LB);
-        A[idx] = i2 < N2 ? g[(i2 * G1 + (idx >> LT)) * BCT + (t & (BCT - 1))] : make_float2(0.f, 0.f);
+    for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
+        float2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = i0 + u * 256;
+            const int t = idx & (T - 1);
+            const int i2 = r0 + (t >> LB);
+            v[u] = idx < total && i2 < N2 ? g[(i2 * G1 + (idx >> LT)) * BCT + (t & (BCT - 1))]
+                                          : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + u * 256 < total) A[i0 + u * 256] = v[u];
     }
     __syncthreads();
     const float2* R = fft_lds<T, -1, BIG>(A, Bb, facs[S.fac1], tw);
