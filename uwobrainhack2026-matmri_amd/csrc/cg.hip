@@ -32,6 +32,9 @@ struct CgSeg {
 };
 
 constexpr int kCgBlock = 256;
+// threads of the per-slice scalar kernels (cg_alpha, cg_beta, cg_init,
+// zero_scal_k); a compile-time constant, as blockDim.x would be a memory load
+constexpr int kCgScalBlock = 64;
 constexpr int kCgMaxBlocksPerSlice = 64;
 
 __device__ __forceinline__ double2 warp_sum2(double2 v) {
@@ -102,9 +105,9 @@ __global__ __launch_bounds__(kCgBlock) void cg_dot_pq(const CgSeg* __restrict__ 
 }
 
 // alpha = rho / (p'Ap)  (cgne.m:228); keeps |p|^2 for |x - xold|^2
-__global__ void cg_alpha(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
+__global__ __launch_bounds__(kCgScalBlock) void cg_alpha(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                          const double2* __restrict__ part, double* __restrict__ pp_out, int nslice) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * kCgScalBlock + threadIdx.x;
     if (s >= nslice || sc[s].done) return;
     const CgSeg g = seg[s];
     double2 pq = make_double2(0, 0);
@@ -213,12 +216,12 @@ __global__ __launch_bounds__(kCgBlock) void cg_norm_b(const CgSeg* __restrict__ 
 }
 
 // Initial scalars after r0 = b - N x0 (cgne.m:180-207).
-__global__ void cg_init(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
+__global__ __launch_bounds__(kCgScalBlock) void cg_init(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                         const double2* __restrict__ part_b, const double2* __restrict__ part_r,
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
                         int nslice, int nitmax, double resThresh, double resThreshRel,
                         double stopThresh, int x0fallback) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * kCgScalBlock + threadIdx.x;
     if (s >= nslice) return;
     const CgSeg g = seg[s];
     double bb = 0, xx = 0, rr = 0;
@@ -271,13 +274,13 @@ __global__ __launch_bounds__(kCgBlock) void cg_fallback_k(const CgSeg* __restric
 }
 
 // res_sq_new, histories, stopping tests and beta (cgne.m:252-335).
-__global__ void cg_beta(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
+__global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                         const double2* __restrict__ part, const double* __restrict__ pp,
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
                         int nslice, int nitmax_alloc, int explicit_res, double xChangeThresh,
                         double resChangeThresh, int stopOnResInc, double resIncThresh,
                         int stopOnXdifInc, int nseExtraIt, int* __restrict__ done_host) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * kCgScalBlock + threadIdx.x;
     if (s >= nslice) return;
     CgScal c = sc[s];
     if (c.done) {
@@ -359,20 +362,19 @@ __global__ __launch_bounds__(kCgBlock) void cg_update_p(const CgSeg* __restrict_
     }
 }
 
-__global__ void copy_rr_k(const double2* __restrict__ src, double2* __restrict__ dst, int nblk) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kCgBlock) void copy_rr_k(const double2* __restrict__ src, double2* __restrict__ dst, int nblk) {
+    const int b = blockIdx.x * kCgBlock + threadIdx.x;
     if (b < nblk) dst[2 * b] = src[2 * b];
 }
 
-__global__ void scale_copy_k(const float2* __restrict__ y, const float* __restrict__ w,
+__global__ __launch_bounds__(kCgBlock) void scale_copy_k(const float2* __restrict__ y, const float* __restrict__ w,
                              float2* __restrict__ out, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock)
         out[i] = make_float2(y[i].x * w[i], y[i].y * w[i]);
 }
 
-__global__ void zero_scal_k(CgScal* sc, int ns) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kCgScalBlock) void zero_scal_k(CgScal* sc, int ns) {
+    const int s = blockIdx.x * kCgScalBlock + threadIdx.x;
     if (s < ns) sc[s] = CgScal{};
 }
 
@@ -420,11 +422,11 @@ CgSolver::~CgSolver() {
 void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x) {
     hipStream_t st = E_->ctx->stream;
     const int ns = E_->nslice();
-    const int sb = (ns + 63) / 64;
+    const int sb = (ns + kCgScalBlock - 1) / kCgScalBlock;
     E_->apply_normal(p_.p, q_.p, 1, tmp_.p);
     hipLaunchKernelGGL(cg_dot_pq, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, q_.p,
                        part_.p);
-    hipLaunchKernelGGL(cg_alpha, dim3(sb), dim3(64), 0, st, seg_.p, sc_.p, part_.p, pp_.p, ns);
+    hipLaunchKernelGGL(cg_alpha, dim3(sb), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, ns);
     hipLaunchKernelGGL(cg_update, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, x, p_.p,
                        r_.p, q_.p, explicit_res ? 0 : 1, part_.p);
     if (explicit_res) {
@@ -435,7 +437,7 @@ void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x
         hipLaunchKernelGGL(copy_rr_k, dim3((nblk_ + 255) / 256), dim3(256), 0, st, part2_.p, part_.p,
                            nblk_);
     }
-    hipLaunchKernelGGL(cg_beta, dim3(sb), dim3(64), 0, st, seg_.p, sc_.p, part_.p, pp_.p, hres_.p, hxn_.p,
+    hipLaunchKernelGGL(cg_beta, dim3(sb), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, hres_.p, hxn_.p,
                        hxd_.p, ns, nitmax_, explicit_res, o.xChangeThresh, o.resChangeThresh,
                        o.stopOnResInc, o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt, done_d_.p);
     hipLaunchKernelGGL(cg_update_p, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, r_.p);
@@ -446,7 +448,7 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
                      double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status) {
     hipStream_t st = E_->ctx->stream;
     const int ns = E_->nslice();
-    const int sb = (ns + 63) / 64;
+    const int sb = (ns + kCgScalBlock - 1) / kCgScalBlock;
     if (NitMax < 0) fail(MM_ERR_ARG, "cgne: NitMax must be >= 0");
     if (NitMax != nitmax_) {
         nitmax_ = NitMax;
@@ -474,10 +476,10 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
                        x0 ? 0 : 1, part2_.p);
     // r = b - N x0, p = r  (cgne.m:165-180)
     E_->apply_normal(x, q_.p, 1, tmp_.p);
-    hipLaunchKernelGGL(zero_scal_k, dim3(sb), dim3(64), 0, st, sc_.p, ns);
+    hipLaunchKernelGGL(zero_scal_k, dim3(sb), dim3(kCgScalBlock), 0, st, sc_.p, ns);
     hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p, q_.p,
                        r_.p, p_.p, part_.p);
-    hipLaunchKernelGGL(cg_init, dim3(sb), dim3(64), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
+    hipLaunchKernelGGL(cg_init, dim3(sb), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
                        hxn_.p, hxd_.p, ns, NitMax, o.resThresh, o.resThreshRel, o.stopThresh,
                        (x0 && o.x0Fallback) ? 1 : 0);
     LAUNCHCHECK();

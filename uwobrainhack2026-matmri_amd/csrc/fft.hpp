@@ -19,6 +19,10 @@ namespace mm {
 
 constexpr int kMaxFactors = 12;
 
+// threads of every work-group that runs these transforms (a compile-time
+// constant: blockDim.x would be a memory load)
+constexpr int kFftThreads = 256;
+
 struct FftFactors {
     int n = 1;
     int nf = 0;
@@ -216,7 +220,7 @@ __device__ __forceinline__ void fft_pass(const float2* __restrict__ a, float2* _
     const int nb = n / R;
     const int total = nb * T;
     const unsigned pinv = TW ? mod_small_inv<BIG>(p) : 0u;
-    for (int w = threadIdx.x; w < total; w += blockDim.x) fft_butterfly<R, T, SGN, TW, BIG>(a, b, nb, p, pinv, twp, w);
+    for (int w = threadIdx.x; w < total; w += kFftThreads) fft_butterfly<R, T, SGN, TW, BIG>(a, b, nb, p, pinv, twp, w);
 }
 
 // Full transform in LDS, ping-ponging between a and b; tw = the pass

@@ -327,8 +327,8 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
     float2* A = lds;
     float2* Bb = lds + G1 * T;
     float2* tw = lds + 2 * G1 * T;
-    for (int i = threadIdx.x; i < G1 - 1; i += blockDim.x) tw[i] = twp[S.tw1 + i];
-    for (int i = (N1 - N1 / 2) * T + threadIdx.x; i < (G1 - N1 / 2) * T; i += blockDim.x)
+    for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
+    for (int i = (N1 - N1 / 2) * T + threadIdx.x; i < (G1 - N1 / 2) * T; i += kFftThreads)
         A[i] = make_float2(0.f, 0.f);
     const int lanes = lanes_of(run, BCT, ch);
     const float* d1 = deap + S.deap1;
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
     const float2* R = fft_lds<T, 1, BIG>(A, Bb, facs[S.fac1], tw);
     float2* out = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int tot2 = G1 * T;
-    for (int idx = threadIdx.x; idx < tot2; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < tot2; idx += kFftThreads) {
         const int t = idx & (T - 1);
         const int i2 = r0 + (t >> LB);
         if (i2 >= N2) continue;
@@ -398,10 +398,10 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     float2* Bb = lds + G2 * T;
     float2* tw = lds + 2 * G2 * T;
     float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b0;
-    for (int i = threadIdx.x; i < G2 - 1; i += blockDim.x) tw[i] = twp[S.tw2 + i];
+    for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
     const int ncol = min(CPW, G1 - c0);
     if (!ADJ) {
-        for (int i = (N2 - N2 / 2) * T + threadIdx.x; i < (G2 - N2 / 2) * T; i += blockDim.x)
+        for (int i = (N2 - N2 / 2) * T + threadIdx.x; i < (G2 - N2 / 2) * T; i += kFftThreads)
             A[i] = make_float2(0.f, 0.f);
         const int total = N2 * T;
         for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     const float2* R = fft_lds<T, ADJ ? -1 : 1, BIG>(A, Bb, facs[S.fac2], tw);
     if (!ADJ) {
         const int total = G2 * T;
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
             const int t = idx & (T - 1);
             const int col = t >> LCL;
             if (col >= ncol) continue;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
         }
     } else {
         const int total = N2 * T;
-        for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
             const int t = idx & (T - 1);
             const int i2 = idx >> LT;
             const int col = t >> LCL;
@@ -592,12 +592,12 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
     uint32_t* s_base = (uint32_t*)(s_id + kDegridBlock);
     float* s_dcf = (float*)(s_base + kDegridBlock);
     float* s_w = s_dcf + kDegridBlock;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = threadIdx.x; i < n; i += kFftThreads) {
         s_id[i] = bid[blk.y + i];
         s_base[i] = bbase[blk.y + i];
         s_dcf[i] = bdcf[blk.y + i];
     }
-    for (int i = threadIdx.x; i < n * WS; i += blockDim.x) s_w[i] = bwt[(int64_t)blk.y * WS + i];
+    for (int i = threadIdx.x; i < n * WS; i += kFftThreads) s_w[i] = bwt[(int64_t)blk.y * WS + i];
     for (int i0 = threadIdx.x; i0 < nst; i0 += 4 * 256) {
         float2 v[4];
 #pragma unroll
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(256) void nu_transpose_y(const NuSliceDev* __restri
     }
     __syncthreads();
     float2* dst = yt + S.yt_off + ((int64_t)ch * S.M + m0) * BCT;
-    for (int i = threadIdx.x; i < n * BCT; i += blockDim.x) dst[i] = buf[i];
+    for (int i = threadIdx.x; i < n * BCT; i += kFftThreads) dst[i] = buf[i];
 }
 
 // Gather gridding: each thread owns (cell, transform) pairs of rows
@@ -787,7 +787,7 @@ __device__ __forceinline__ void grid_gather_rows(const NuSliceDev& S, int ch, co
     const int W = WT > 0 ? WT : S.W1;
     const int NB1 = kTile + W - 1;
     const int nbins = NB1 * NB1 + 1;
-    for (int i = threadIdx.x; i < nbins; i += blockDim.x) sb[i] = gbin[g.bin_off + i];
+    for (int i = threadIdx.x; i < nbins; i += kFftThreads) sb[i] = gbin[g.bin_off + i];
     __syncthreads();
     const int n = sb[nbins - 1];   // entries of the tile (fits the staging area)
     const int* list = glist + g.ent_off;
@@ -795,14 +795,14 @@ __device__ __forceinline__ void grid_gather_rows(const NuSliceDev& S, int ch, co
     // stage: per entry 2W weights (dim-1 weights times dcf) and its BCT data values
     float* sw = stage;                                          // [n][2W]
     float2* sy = reinterpret_cast<float2*>(stage + 2 * W * n);  // [n][BCT]
-    for (int i = threadIdx.x; i < n * 2 * W; i += blockDim.x) {
+    for (int i = threadIdx.x; i < n * 2 * W; i += kFftThreads) {
         const int e = i / (2 * W), q = i % (2 * W);
         const int64_t sm = S.samp_off + list[e];
         float w = swt[sm * (2 * W) + q];
         if (q < W) w *= sdcf[sm];
         sw[i] = w;
     }
-    for (int i = threadIdx.x; i < n * BCT; i += blockDim.x) {
+    for (int i = threadIdx.x; i < n * BCT; i += kFftThreads) {
         const int e = i / BCT, b = i & (BCT - 1);
         sy[i] = ych[(int64_t)list[e] * BCT + b];
     }
@@ -811,7 +811,7 @@ __device__ __forceinline__ void grid_gather_rows(const NuSliceDev& S, int ch, co
     const int lanes = lanes_of(run, BCT, ch);
     float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int pairs = (g.r1 - g.r0) * kTile * BCT;
-    for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
+    for (int p = threadIdx.x; p < pairs; p += kFftThreads) {
         const int b = p & (BCT - 1);
         const int cell = p / BCT;
         const int l1 = cell % kTile, l2 = g.r0 + cell / kTile;
@@ -850,7 +850,7 @@ __device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, c
     constexpr int CELLS = kTile * kTile * BCT;
     const int G1 = S.G1, G2 = S.G2, M = S.M;
     const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2;
-    for (int i = threadIdx.x; i < CELLS; i += blockDim.x) {
+    for (int i = threadIdx.x; i < CELLS; i += kFftThreads) {
         accr[i] = 0.f;
         acci[i] = 0.f;
     }
@@ -956,7 +956,7 @@ __device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, c
     const int SPB = 256 / BCT;
     const int Q = (tw.count + SPB - 1) / SPB;
     const int total = SPB * Q * BCT;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
         const int b = idx & (BCT - 1);
         const int k = idx / BCT;
         const int kk = (k % SPB) * Q + k / SPB;
@@ -1000,7 +1000,7 @@ __device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, c
     __syncthreads();
     float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     if (tw.slot < 0) {
-        for (int i = threadIdx.x; i < CELLS; i += blockDim.x) {
+        for (int i = threadIdx.x; i < CELLS; i += kFftThreads) {
             const int b = i & (BCT - 1);
             const int l1 = (i / BCT) % kTile, l2 = i / (BCT * kTile);
             const int g1 = t1_0 + l1, g2 = t2_0 + l2;
@@ -1012,7 +1012,7 @@ __device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, c
     // parts in slot order and writes the tile
     float2* pbase = parts + S.part_off + (int64_t)ch * S.part_chunk;
     float2* mine = pbase + (int64_t)tw.slot * CELLS;
-    for (int i = threadIdx.x; i < CELLS; i += blockDim.x) mine[i] = make_float2(accr[i], acci[i]);
+    for (int i = threadIdx.x; i < CELLS; i += kFftThreads) mine[i] = make_float2(accr[i], acci[i]);
     __threadfence();   // every wave's partial is visible device-wide before the count
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1024,7 +1024,7 @@ __device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, c
     __syncthreads();
     if (!s_last) return;
     __threadfence();
-    for (int i = threadIdx.x; i < CELLS; i += blockDim.x) {
+    for (int i = threadIdx.x; i < CELLS; i += kFftThreads) {
         float2 sacc = make_float2(0.f, 0.f);
         for (int p = 0; p < tw.nparts; ++p) {
             const float2 q = pbase[(int64_t)(tw.first_slot + p) * CELLS + i];
@@ -1225,7 +1225,7 @@ __global__ __launch_bounds__(256) void nu_grid_merge(const GridItem* __restrict_
     if (skip && skip[T.slice]) return;
     const float2* p0 = parts + T.part + (int64_t)T.pofs0 * BCT;
     const int nv = T.w * T.h * BCT;
-    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    for (int i = threadIdx.x; i < nv; i += kFftThreads) {
         float2 t = make_float2(0.f, 0.f);
         for (int k0 = 0; k0 < T.nparts; k0 += 8) {
             float2 u[8];
@@ -1300,7 +1300,7 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
     float2* A = lds;
     float2* Bb = lds + G1 * T;
     float2* tw = lds + 2 * G1 * T;
-    for (int i = threadIdx.x; i < G1 - 1; i += blockDim.x) tw[i] = twp[S.tw1 + i];
+    for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     const float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int total = G1 * T;
     for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
@@ -1328,7 +1328,7 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
         float2* xo = x + S.img_off + (int64_t)ch * V;
         const int tot = N1 * RPW * H;
         // whole waves run every iteration (the shuffle needs both lanes)
-        for (int idx0 = 0; idx0 < tot; idx0 += blockDim.x) {
+        for (int idx0 = 0; idx0 < tot; idx0 += kFftThreads) {
             const int idx = idx0 + threadIdx.x;
             const int h = idx & (H - 1);
             const int r = (idx / H) & (RPW - 1);
@@ -1364,7 +1364,7 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
     } else {
         const int tot = N1 * T;
         float2* xo = x + S.img_off + (int64_t)ch * BCT * V;
-        for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
             const int t = idx & (T - 1);
             const int i1 = idx >> LT;
             const int b = t & (BCT - 1);

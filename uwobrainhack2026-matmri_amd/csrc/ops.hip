@@ -11,11 +11,11 @@ namespace mm {
 
 void mm_op_default_normal(mm_op* op, const float2* in, float2* out, int64_t nrep, float2* tmp);
 
-__global__ void rcvr_fwd_k(const float2* __restrict__ x, const float2* __restrict__ maps,
+__global__ __launch_bounds__(256) void rcvr_fwd_k(const float2* __restrict__ x, const float2* __restrict__ maps,
                            float2* __restrict__ y, int64_t V, int C, int64_t nrep) {
     const int64_t n = V * C * nrep;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
         const int64_t v = i % V;
         const int64_t rest = i / V;
         const int c = (int)(rest % C);
@@ -24,11 +24,11 @@ __global__ void rcvr_fwd_k(const float2* __restrict__ x, const float2* __restric
     }
 }
 
-__global__ void rcvr_adj_k(const float2* __restrict__ y, const float2* __restrict__ maps,
+__global__ __launch_bounds__(256) void rcvr_adj_k(const float2* __restrict__ y, const float2* __restrict__ maps,
                            float2* __restrict__ x, int64_t V, int C, int64_t nrep) {
     const int64_t n = V * nrep;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
         const int64_t v = i % V;
         const int64_t r = i / V;
         float2 s = make_float2(0.f, 0.f);
@@ -42,10 +42,10 @@ __global__ void rcvr_adj_k(const float2* __restrict__ y, const float2* __restric
     }
 }
 
-__global__ void scale_by_weights_k(float2* __restrict__ y, const float* __restrict__ w, int64_t n,
+__global__ __launch_bounds__(256) void scale_by_weights_k(float2* __restrict__ y, const float* __restrict__ w, int64_t n,
                                    int64_t period) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
         const float f = w[i % period];
         y[i].x *= f;
         y[i].y *= f;
