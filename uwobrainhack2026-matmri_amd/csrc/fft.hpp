@@ -223,10 +223,69 @@ __device__ __forceinline__ void fft_pass(const float2* __restrict__ a, float2* _
     for (int w = threadIdx.x; w < total; w += kFftThreads) fft_butterfly<R, T, SGN, TW, BIG>(a, b, nb, p, pinv, twp, w);
 }
 
-// Full transform in LDS, ping-ponging between a and b; tw = the pass
-// twiddles of fft_pass_twiddles.  All threads of the block take part;
-// returns the buffer holding the result.  The caller must __syncthreads()
-// after filling `a` (and tw).
+// Butterflies a thread holds in registers in a single-buffer pass of radix
+// R (8 complex values at most).
+__host__ __device__ constexpr int fft_inplace_maxb(int R) { return R > 8 ? 1 : 8 / R; }
+
+// Whether every pass of the transform (length n, T interleaved transforms)
+// can run in a single buffer: each thread's butterflies fit its registers.
+inline bool fft_inplace_ok(const FftFactors& f, int T) {
+    for (int s = 0; s < f.nf; ++s)
+        if ((f.n / f.r[s]) * T > fft_inplace_maxb(f.r[s]) * kFftThreads) return false;
+    return true;
+}
+
+// Single-buffer Stockham pass: every thread reads the inputs of its
+// butterflies (at most fft_inplace_maxb(R)) into registers, the block
+// synchronises, and the outputs overwrite the same buffer.
+template <int R, int T, int SGN, bool TW, bool BIG>
+__device__ __forceinline__ void fft_pass_inplace(float2* __restrict__ a, int n, int p,
+                                                 const float2* __restrict__ twp) {
+    constexpr int MAXB = fft_inplace_maxb(R);
+    constexpr int LT = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6;
+    const int nb = n / R;
+    const int total = nb * T;
+    const unsigned pinv = TW ? mod_small_inv<BIG>(p) : 0u;
+    float2 v[MAXB][R];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int w = threadIdx.x + q * kFftThreads;
+        if (w < total) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[q][j] = a[w + j * nb * T];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int w = threadIdx.x + q * kFftThreads;
+        if (w < total) {
+            const int i = w >> LT;
+            const int t = w & (T - 1);
+            int k = 0;
+            if constexpr (TW) {
+                k = mod_small<BIG>(i, p, pinv);
+                const float2* c = twp + k * (R - 1);
+#pragma unroll
+                for (int j = 1; j < R; ++j) {
+                    const float2 cj = c[j - 1];
+                    const float cy = SGN > 0 ? cj.y : -cj.y;
+                    v[q][j] = make_float2(v[q][j].x * cj.x - v[q][j].y * cy, v[q][j].x * cy + v[q][j].y * cj.x);
+                }
+            }
+            dft_small<R>(v[q], (float)SGN);
+            float2* d = a + ((i - k) * R + k) * T + t;
+#pragma unroll
+            for (int j = 0; j < R; ++j) d[j * p * T] = v[q][j];
+        }
+    }
+}
+
+// Full transform in LDS; tw = the pass twiddles of fft_pass_twiddles.  With
+// b == nullptr every pass runs in the single buffer a (the caller checked
+// fft_inplace_ok), else the passes ping-pong between a and b.  All threads
+// of the block take part; returns the buffer holding the result.  The
+// caller must __syncthreads() after filling `a` (and tw).
 template <int T, int SGN, bool BIG = false>
 __device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactors& f, const float2* tw) {
     const int n = f.n;
@@ -234,6 +293,30 @@ __device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactor
     for (int s = 0; s < f.nf; ++s) {
         const int R = f.r[s];
         const float2* twp = tw + (p - 1);
+        if (b == nullptr) {
+            if (s == 0) {
+                switch (R) {
+                    case 8: fft_pass_inplace<8, T, SGN, false, BIG>(a, n, p, twp); break;
+                    case 4: fft_pass_inplace<4, T, SGN, false, BIG>(a, n, p, twp); break;
+                    case 2: fft_pass_inplace<2, T, SGN, false, BIG>(a, n, p, twp); break;
+                    case 3: fft_pass_inplace<3, T, SGN, false, BIG>(a, n, p, twp); break;
+                    case 5: fft_pass_inplace<5, T, SGN, false, BIG>(a, n, p, twp); break;
+                    default: fft_pass_inplace<7, T, SGN, false, BIG>(a, n, p, twp); break;
+                }
+            } else {
+                switch (R) {
+                    case 8: fft_pass_inplace<8, T, SGN, true, BIG>(a, n, p, twp); break;
+                    case 4: fft_pass_inplace<4, T, SGN, true, BIG>(a, n, p, twp); break;
+                    case 2: fft_pass_inplace<2, T, SGN, true, BIG>(a, n, p, twp); break;
+                    case 3: fft_pass_inplace<3, T, SGN, true, BIG>(a, n, p, twp); break;
+                    case 5: fft_pass_inplace<5, T, SGN, true, BIG>(a, n, p, twp); break;
+                    default: fft_pass_inplace<7, T, SGN, true, BIG>(a, n, p, twp); break;
+                }
+            }
+            __syncthreads();
+            p *= R;
+            continue;
+        }
         if (s == 0) {
             switch (R) {
                 case 8: fft_pass<8, T, SGN, false, BIG>(a, b, n, p, twp); break;

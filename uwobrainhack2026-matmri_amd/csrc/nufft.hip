@@ -309,7 +309,7 @@ __host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_
 // index = (position, row, transform) with the transform fastest.  Offsets
 // inside a slice are 32-bit (the plan checks V*BCT and G1*G2*BCT < 2^31).
 template <int BCT, bool BIG>
-__global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
                                                    const float2* __restrict__ twp,
@@ -324,9 +324,10 @@ __global__ __launch_bounds__(256) void nu_fwd_rows(const NuSliceDev* __restrict_
     const int ch = wk.y, r0 = wk.z;
     const int N1 = S.N1, N2 = S.N2, G1 = S.G1;
     const int V = N1 * N2;
+    const bool ip = S.fip & 1;   // single-buffer row FFTs
     float2* A = lds;
-    float2* Bb = lds + G1 * T;
-    float2* tw = lds + 2 * G1 * T;
+    float2* Bb = ip ? nullptr : lds + G1 * T;
+    float2* tw = lds + (ip ? 1 : 2) * G1 * T;
     for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     for (int i = (N1 - N1 / 2) * T + threadIdx.x; i < (G1 - N1 / 2) * T; i += kFftThreads)
         A[i] = make_float2(0.f, 0.f);
@@ -394,9 +395,10 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     const NuSliceDev& S = sl[wk.x];
     const int ch = wk.y, c0 = wk.z, b0 = wk.w;
     const int N2 = S.N2, G1 = S.G1, G2 = S.G2;
+    const bool ip = (S.fip >> 1) & 1;   // single-buffer column FFTs
     float2* A = lds;
-    float2* Bb = lds + G2 * T;
-    float2* tw = lds + 2 * G2 * T;
+    float2* Bb = ip ? nullptr : lds + G2 * T;
+    float2* tw = lds + (ip ? 1 : 2) * G2 * T;
     float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b0;
     for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
     const int ncol = min(CPW, G1 - c0);
@@ -1282,7 +1284,7 @@ __global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl
 // pixel, each summing half of the coils; the halves are added through a
 // lane shuffle.
 template <int BCT, bool BIG>
-__global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
                                                    const float2* __restrict__ twp,
@@ -1297,9 +1299,10 @@ __global__ __launch_bounds__(256) void nu_adj_rows(const NuSliceDev* __restrict_
     const int ch = wk.y, r0 = wk.z;
     const int N1 = S.N1, N2 = S.N2, G1 = S.G1;
     const int V = N1 * N2;
+    const bool ip = S.fip & 1;   // single-buffer row FFTs
     float2* A = lds;
-    float2* Bb = lds + G1 * T;
-    float2* tw = lds + 2 * G1 * T;
+    float2* Bb = ip ? nullptr : lds + G1 * T;
+    float2* tw = lds + (ip ? 1 : 2) * G1 * T;
     for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     const float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int total = G1 * T;
@@ -1816,7 +1819,11 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     HIPCHECK(hipStreamSynchronize(st));
 }
 
-static size_t fft_lds_bytes(int G, int T) { return (size_t)(2 * G * T + G) * sizeof(float2); }
+// LDS of one FFT work-group: one (single-buffer passes) or two buffers of
+// G*T values and the G-1 pass twiddles
+static size_t fft_lds_bytes(int G, int T, bool inplace) {
+    return (size_t)((inplace ? 1 : 2) * G * T + G) * sizeof(float2);
+}
 
 NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     auto key = std::make_pair(nrep, sense ? C : 0);
@@ -1861,8 +1868,15 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         d.lfl = ilog2(BCT);
         d.cpw = cpw;
         d.lcl = ilog2(cl);
-        c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR));
-        c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, TC));
+        {
+            FftFactors f1, f2;
+            fft_factorize(d.G1, f1);
+            fft_factorize(d.G2, f2);
+            const bool ip1 = fft_inplace_ok(f1, TR), ip2 = fft_inplace_ok(f2, TC);
+            d.fip = (ip1 ? 1 : 0) | (ip2 ? 2 : 0);
+            c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR, ip1));
+            c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, TC, ip2));
+        }
         if (c.lds_rows > 160 * 1024 || c.lds_cols > 160 * 1024)
             fail(MM_ERR_UNSUPPORTED, "nufftOp: grid side too large for the LDS FFT passes");
         if ((int64_t)d.N1 * d.N2 * BCT >= INT32_MAX || (int64_t)d.G1 * d.G2 * BCT >= INT32_MAX)
@@ -2015,24 +2029,36 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wcsr[ord[i]];
         wcsr.swap(sorted);
     }
-    // largest degridding blocks first
-    {
-        std::vector<size_t> ord(wdb.size());
+    // Order of the staged items: heaviest first; with the XCD-aware order
+    // (option "nu_xcd_order") the items of (slice, chunk) q go to the
+    // positions = q (mod kXcds), heaviest first within each XCD, so a slice's
+    // data stays in one XCD's L2.
+    auto order = [&](auto& items, const std::vector<int64_t>& cost, auto key) {
+        std::vector<size_t> ord(items.size());
         std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wdb_cost[a] > wdb_cost[b]; });
-        std::vector<int4> sorted(wdb.size());
-        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wdb[ord[i]];
-        wdb.swap(sorted);
-    }
-    // heaviest staged gridding items first
-    {
-        std::vector<size_t> ord(wgt.size());
-        std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wgt_cost[a] > wgt_cost[b]; });
-        std::vector<GridItem> sorted(wgt.size());
-        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wgt[ord[i]];
-        wgt.swap(sorted);
-    }
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        std::vector<size_t> pick;
+        if (xcd_order_) {
+            std::vector<std::vector<size_t>> q(kXcds);
+            for (size_t i : ord) q[(size_t)(key(items[i]) % kXcds)].push_back(i);
+            std::vector<size_t> pos(kXcds, 0);
+            for (bool any = true; any;) {
+                any = false;
+                for (int x = 0; x < kXcds; ++x)
+                    if (pos[x] < q[x].size()) {
+                        pick.push_back(q[x][pos[x]++]);
+                        any = true;
+                    }
+            }
+        } else {
+            pick = ord;
+        }
+        std::remove_reference_t<decltype(items)> sorted(items.size());
+        for (size_t i = 0; i < pick.size(); ++i) sorted[i] = items[pick[i]];
+        items.swap(sorted);
+    };
+    order(wdb, wdb_cost, [&](const int4& w) { return (int64_t)w.x * 131 + w.y; });
+    order(wgt, wgt_cost, [&](const GridItem& g) { return (int64_t)g.slice * 131 + (g.zg - c.hs[g.slice].ws_off) / std::max<int64_t>(1, c.hs[g.slice].ws_chunk); });
     // heaviest gridding items first
     {
         std::vector<size_t> ord(wg.size());
@@ -2047,6 +2073,24 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.w_rows.upload(wr, st);
     c.w_frows.upload(wfr, st);
     c.n_frows = (int)wfr.size();
+    if (pair_cols_) {
+        // neighbouring columns share cache lines (a column touches 64 of each
+        // 128 bytes): place items 2k and 2k + 1 of every 16 at positions k and
+        // k + 8, which the dispatcher sends to the same XCD at about the same
+        // time, so each line is fetched into one L2
+        std::vector<int4> r(wc.size());
+        for (size_t j = 0; j < wc.size(); ++j) {
+            const size_t q = j / 16, rr = j % 16;
+            size_t pos = q * 16 + rr / 2 + 8 * (rr % 2);
+            if (pos >= wc.size()) pos = j;   // incomplete last group keeps its order
+            r[pos] = wc[j];
+        }
+        if (wc.size() % 16) {
+            const size_t q0 = wc.size() / 16 * 16;
+            for (size_t j = q0; j < wc.size(); ++j) r[j] = wc[j];
+        }
+        wc.swap(r);
+    }
     c.w_cols.upload(wc, st);
     c.w_deg.upload(wd, st);
     c.w_grid.upload(wg.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wg, st);

@@ -19,7 +19,7 @@ struct NuSliceDev {
     int tile_first;     // first gridding tile of this slice in the tile pool
     int frpw;           // rows per work-group of the forward row FFT
     int lfl, lcl;       // log2 transforms (coils) per work-group: forward rows, columns
-    int pad1;
+    int fip;            // bit 0: row FFTs, bit 1: column FFTs in a single LDS buffer
     int64_t samp_off;   // first sample of this slice in the sample pools
     int64_t tlist_off;  // first entry of this slice in the tile sample list
     int64_t img_off;    // image-side offset (elements) for the current apply
@@ -128,6 +128,11 @@ class NufftOp : public mm_op {
             degrid_t_ = value != 0;
             return true;
         }
+        if (key == "nu_pair_cols") {
+            pair_cols_ = value != 0;
+            cfgs_.clear();
+            return true;
+        }
         if (key == "nu_degrid_stage") {
             degrid_stage_ = value != 0;
             return true;
@@ -229,6 +234,7 @@ class NufftOp : public mm_op {
     // of each distinct 2-D trajectory; slices that share a trajectory share
     // its rectangles
     bool grid_stage_ = true;
+    bool pair_cols_ = true;   // XCD-paired order of the column-pass items (option "nu_pair_cols")
     // staged degridding (option "nu_degrid_stage", default on): sample
     // blocks of each distinct 2-D trajectory, (tile | tile2 << 16, first
     // entry of dsamp_, samples, 0)
