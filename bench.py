@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--hio-variant", type=int, default=None, help="high-order MFMA kernel variant 0..3")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="skip the untimed autotuner of the NUFFT SENSE operator (mm_op_autotune)")
     p.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                    help="tuning option of the sampling operator (mm_op_set_option), e.g. nu_degrid_t=0")
     return p.parse_args()
@@ -418,6 +420,7 @@ def run_warmstart(a, mm, torch, dev):
     host = fam.host_batch(seeds)
     prob = fam.make(seeds, host)
     E, R, y = prob["E"], prob["R"], prob["y"]
+    tuned = None if a.no_autotune or a.set else E.autotune(10)
     NitMax, rel = 200, 1e-4
     zeros = torch.zeros((P, N, N), dtype=torch.complex64, device=dev)
 
@@ -464,7 +467,8 @@ def run_warmstart(a, mm, torch, dev):
                        "x0 from the learned 6x(3x3 conv, 32 ch) warm start with zero-init fallback",
            "problems_per_step": P, "image": [N, N], "coils": Cc, "samples": M, "rel_resid": rel,
            "max_iterations": NitMax, "network": "6 layers 3x3 conv, 2-32-32-32-32-32-2, residual skip, "
-                                                "bf16 weights, fp32 accumulation"}
+                                                "bf16 weights, fp32 accumulation",
+           "autotuned": tuned}
     out = {"metric": "CG-SENSE solves/s at rel. resid 1e-4, learned warm start vs zero-init",
            "value": P * a.steps / zw["el"], "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": 1e3 * zw["el"] / a.steps, "higher_is_better": True,
@@ -543,6 +547,8 @@ def main():
         for kv in a.set:
             k, v = kv.split("=")
             W["S"].set_option(k, int(v))
+        if W["kind"] in ("nufft", "batch") and not (a.no_autotune or a.set):
+            W["cfg"]["autotuned"] = E.autotune(10)
         opt = {"resThresh": None, "resThreshRel": 1e-8, "xChangeThresh": None}
         b = E.H * y
         bb = float(torch.sum(torch.abs(b.reshape(-1).to(torch.complex128)) ** 2))

@@ -165,8 +165,23 @@ int mm_op_profile_stage(mm_op* op, int idx, double* ms_per_launch, int* launches
                         int name_len);
 /* Tuning knob (no counterpart in the reference): "hio_variant" (0..3: bit 0
  * phase on FP64 MFMA, bit 1 hardware sine/cosine) and "hio_mfma" (0/1) of a
- * high-order operator or a SENSE operator built on one. */
+ * high-order operator or a SENSE operator built on one; for a NUFFT operator
+ * (or a SENSE operator built on one) "nu_stage_samples" (16..256),
+ * "nu_stage_entries" (64..2048), "nu_degrid_block" (16..256), and the 0/1
+ * switches "nu_pair_cols", "nu_xcd_order", "nu_grid_stage",
+ * "nu_degrid_stage", "nu_degrid_t". */
 int mm_op_set_option(mm_op* op, const char* key, int value);
+/* Current value of a tuning option of mm_op_set_option (no counterpart in
+ * the reference). */
+int mm_op_get_option(mm_op* op, const char* key, int* value);
+/* The project's autotuner (no counterpart in the reference; the batched
+ * configuration asks for tile and bin sizes chosen by one): for a SENSE
+ * operator on a 2-D NUFFT, times candidate staged-gridding rectangle caps,
+ * degridding block sizes and work orders on this device with the normal
+ * operator's stage timers (`iters` applications each) and keeps the
+ * fastest; the choice can be read back with mm_op_get_option.
+ * MM_ERR_UNSUPPORTED for other operators. */
+int mm_op_autotune(mm_op* op, int iters);
 /* The hipStream_t the context runs on. */
 void* mm_ctx_stream(mm_ctx* ctx);
 

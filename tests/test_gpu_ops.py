@@ -239,6 +239,41 @@ def test_degrid_staged_vs_global(mm, kind, nrep):
     assert np.array_equal(f_stage, f_glob)
 
 
+def test_nufft_tuning_options_and_autotune(mm):
+    """Every staged-partition option value keeps the SENSE operator within the
+    per-application bound; the autotuner picks from its candidate sets and
+    the tuned operator still matches the oracle."""
+    print("[case] tuning options + autotune, 3 spiral slices N=96 C=8", flush=True)
+    rng = np.random.default_rng(8)
+    N, C = 96, 8
+    kl = mm.workloads.spiral(9000, 40)
+    maps = [crand(rng, (N, N, C)) for _ in range(3)]
+    E = mm.senseOp(mm.nufftOp.batch([([N, N], kl, None)] * 3, 1.5, 4), mm.rcvrOp.batch(maps))
+    So = NufftPlan([N, N], kl, None, 1.5, 4)
+    xs = [crand(rng, (N, N)) for _ in range(3)]
+    xcat = np.concatenate([x.reshape(-1, order="F") for x in xs])
+    want = np.concatenate([o_mrSampFunc(o_mrSampFunc(x, "notransp", So, RcvrPlan(mp)), "transp", So, RcvrPlan(mp))
+                           .reshape(-1, order="F") for x, mp in zip(xs, maps)])
+    for key, vals in [("nu_stage_samples", [16, 96, 256]), ("nu_stage_entries", [64, 768, 2048]),
+                      ("nu_degrid_block", [16, 128, 256]), ("nu_pair_cols", [0, 1]), ("nu_xcd_order", [1, 0])]:
+        for v in vals:
+            E.set_option(key, v)
+            assert E.get_option(key) == v
+            e = rel(E.normal(xcat), want)
+            print(f"  {key}={v}: E'E vs oracle {e:.2e}", flush=True)
+            assert e < TOL
+        E.set_option(key, vals[1])
+    with pytest.raises(Exception):
+        E.set_option("nu_stage_samples", 4096)
+    chosen = E.autotune(3)
+    print(f"  autotune chose {chosen}", flush=True)
+    assert (chosen["nu_stage_samples"], chosen["nu_stage_entries"]) in [(96, 768), (128, 1024), (192, 1536),
+                                                                          (256, 2048)]
+    assert chosen["nu_degrid_block"] in (32, 64, 128)
+    assert chosen["nu_pair_cols"] in (0, 1) and chosen["nu_xcd_order"] in (0, 1)
+    assert rel(E.normal(xcat), want) < TOL
+
+
 def _hio_plans(mm, rng, N=12, K=300, conc=True, mask=False, Lh=16):
     gx = (np.arange(N) - N / 2) * 0.01
     X, Y = np.meshgrid(gx, gx, indexing="ij")

@@ -21,7 +21,7 @@ EXPORTS = [
     "mm_ctx_sync", "mm_nufft_plan_create", "mm_rcvr_plan_create", "mm_rcvr_plan_create_multi",
     "mm_hio_plan_create", "mm_hio_set_phi_div", "mm_sense_create", "mm_op_shape", "mm_op_nslice",
     "mm_op_apply", "mm_op_apply_normal", "mm_cg_opts_default", "mm_cgne_solve", "mm_cg_set_graph",
-    "mm_op_profile", "mm_op_set_option", "mm_ctx_stream", "mm_op_destroy", "mm_cg_last_fallback",
+    "mm_op_profile", "mm_op_set_option", "mm_op_get_option", "mm_op_autotune", "mm_ctx_stream", "mm_op_destroy", "mm_cg_last_fallback",
     "mm_op_profile_stage",
     "mm_wsnet_create", "mm_wsnet_num_params", "mm_wsnet_set_params", "mm_wsnet_get_params",
     "mm_wsnet_forward", "mm_wsnet_loss_grad", "mm_wsnet_train_step", "mm_wsnet_destroy",
@@ -86,6 +86,8 @@ def lib():
     L.mm_op_destroy.argtypes = [vp]
     L.mm_op_profile.argtypes = [vp, i32, vp, vp, i64, i32, P(dbl), C.c_char_p, i32]
     L.mm_op_set_option.argtypes = [vp, C.c_char_p, i32]
+    L.mm_op_get_option.argtypes = [vp, C.c_char_p, C.POINTER(i32)]
+    L.mm_op_autotune.argtypes = [vp, i32]
     L.mm_op_profile_stage.argtypes = [vp, i32, P(dbl), P(i32), C.c_char_p, i32]
     L.mm_ctx_stream.argtypes = [vp]
     L.mm_ctx_stream.restype = vp

@@ -317,6 +317,22 @@ int mm_op_set_option(mm_op* op, const char* key, int value) {
     });
 }
 
+int mm_op_get_option(mm_op* op, const char* key, int* value) {
+    return guarded([&] {
+        if (!op || !key || !value) fail(MM_ERR_ARG, "get_option: NULL argument");
+        if (!op->get_option(key, value)) fail(MM_ERR_ARG, std::string("get_option: unknown key ") + key);
+    });
+}
+
+int mm_op_autotune(mm_op* op, int iters) {
+    return guarded([&] {
+        if (!op) fail(MM_ERR_ARG, "autotune: NULL operator");
+        (void)hipSetDevice(op->ctx->device);
+        if (!op->autotune(iters)) fail(MM_ERR_UNSUPPORTED, "autotune: nothing to tune for this operator");
+        solvers().erase(op);
+    });
+}
+
 void* mm_ctx_stream(mm_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int mm_op_destroy(mm_op* op) {

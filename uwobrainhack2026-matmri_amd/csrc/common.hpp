@@ -110,6 +110,11 @@ struct mm_op {
     virtual void set_skip(const int* flags) { (void)flags; }
     // Tuning knobs (kernel variants); false if the key is unknown.
     virtual bool set_option(const std::string& key, int value) { (void)key; (void)value; return false; }
+    virtual bool get_option(const std::string& key, int* value) const { (void)key; (void)value; return false; }
+    // Time candidate tuning options of the normal operator on the device
+    // (`iters` applications each) and keep the fastest; false if the
+    // operator has nothing to tune.
+    virtual bool autotune(int iters) { (void)iters; return false; }
     // Time the operator's dominant kernel: `iters` back-to-back launches on
     // the context stream between two HIP events.  Returns ms per launch and
     // the kernel's name; false if the operator has no single dominant kernel.

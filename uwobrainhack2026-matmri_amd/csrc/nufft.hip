@@ -66,14 +66,14 @@ int nufft_kb_nkern(int kwidth, double os) {
 
 // Rectangles of the staged 2-D gridding for one trajectory, given its CSR
 // rows (rp: G1*G2+1 pointers from 0, en: (sample, weight) entries): 16x16
-// tiles, cut into quadrants while the union of their samples exceeds
-// kStageSamples or their entries kStageEntries.  A rectangle of at most 4x4
+// tiles, cut into quadrants while the union of their samples exceeds cap_s
+// or their entries cap_e.  A rectangle of at most 4x4
 // cells that still exceeds them (dense k-space centre) is cut into chunks of
 // consecutive samples of its sorted union, each within both limits (kind 2).
 // Staged rows keep the sample order of the global rows (local sample ids are
 // ranks in the sorted union, or in the chunk).  npcells / ndense count the
 // cells of the partial blocks and the dense rectangles of the trajectory.
-static void build_grid_tiles(int G1, int G2, int M, const int* rp, const int2* en,
+static void build_grid_tiles(int G1, int G2, int M, const int* rp, const int2* en, int cap_s, int cap_e,
                              std::vector<GridTile>& tiles, std::vector<int>& stage,
                              std::vector<int>& ptr, std::vector<int2>& lent, int& npcells, int& ndense) {
     std::vector<int> stamp((size_t)M, -1), loc((size_t)M, 0), uni;
@@ -132,7 +132,7 @@ static void build_grid_tiles(int G1, int G2, int M, const int* rp, const int2* e
             }
         ++id;
         std::sort(uni.begin(), uni.end());
-        if ((int)uni.size() <= kStageSamples && ne <= kStageEntries) {
+        if ((int)uni.size() <= cap_s && ne <= cap_e) {
             if (!uni.empty()) emit(r, 0, 0, (int)uni.size());
             else {
                 GridTile t{};   // no samples: the cells are written as zeros
@@ -157,7 +157,7 @@ static void build_grid_tiles(int G1, int G2, int M, const int* rp, const int2* e
             const size_t first = tiles.size();
             int u0 = 0, cs = 0, ce = 0;
             for (int i = 0; i < (int)uni.size(); ++i) {
-                if (cs + 1 > kStageSamples || ce + cnt[(size_t)i] > kStageEntries) {
+                if (cs + 1 > cap_s || ce + cnt[(size_t)i] > cap_e) {
                     emit(r, 2, u0, i);
                     u0 = i;
                     cs = ce = 0;
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
 }
 
 // Degridding from an LDS-staged grid region (2-D): work item = (slice,
-// chunk, sample block); a block holds up to kDegridBlock samples whose first
+// chunk, sample block); a block holds up to bcap samples whose first
 // tap lies in one kDegridTile x kDegridTile tile, in sample order, with
 // their first taps, weights, dcf and ids stored contiguously in block order
 // (plan time).  The work-group copies the block's sample data and the tile's
@@ -571,7 +571,7 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
                                                       const float* __restrict__ bwt,
                                                       const float* __restrict__ bdcf,
                                                       const float2* __restrict__ ws,
-                                                      float2* __restrict__ y, NuRun run) {
+                                                      float2* __restrict__ y, int bcap, NuRun run) {
     constexpr int SPP = 256 / BCT;
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
@@ -591,9 +591,9 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
     // LDS: grid cells [RW*RW][BCT], then per sample: id, first taps, dcf, weights
     const int nst = RW * RW * BCT;
     int* s_id = (int*)(lds + nst);
-    uint32_t* s_base = (uint32_t*)(s_id + kDegridBlock);
-    float* s_dcf = (float*)(s_base + kDegridBlock);
-    float* s_w = s_dcf + kDegridBlock;
+    uint32_t* s_base = (uint32_t*)(s_id + bcap);
+    float* s_dcf = (float*)(s_base + bcap);
+    float* s_w = s_dcf + bcap;
     for (int i = threadIdx.x; i < n; i += kFftThreads) {
         s_id[i] = bid[blk.y + i];
         s_base[i] = bbase[blk.y + i];
@@ -1419,10 +1419,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     std::vector<uint32_t> sbase;
     std::vector<float> swt, sdcf;
     std::vector<int> tlist, glist_h, gbin_h;
-    std::vector<int2> csr_ent_h;
-    std::vector<int> gstage_h, gptr_h, dsamp_h;
-    std::vector<uint32_t> dbase_h;
-    std::vector<float> ddcf_h, dwt_h;
+    std::vector<int2>& csr_ent_h = csr_ent_host_;
     std::vector<int2> gent_h;
     std::map<uint64_t, int> traj_seen;   // trajectory hash -> first slice with it
     int64_t voff = 0, moff = 0;
@@ -1483,11 +1480,6 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                 d.samp_off = e.samp_off;
                 d.csr_off = e.csr_off;
                 d.ent_off = e.ent_off;
-                gtile_first_.push_back(gtile_first_[twin]);
-                gtile_count_.push_back(gtile_count_[twin]);
-                dblk_first_.push_back(dblk_first_[twin]);
-                dblk_count_.push_back(dblk_count_[twin]);
-                gpcells_.push_back(gpcells_[twin]);
                 d.tlist_off = e.tlist_off;
                 tlist_off_.push_back(e.tlist_off);
                 tw_first_.push_back((int)tw_host_.size());
@@ -1583,55 +1575,6 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                     }
                 }
             }
-            {
-                // degridding blocks: samples bucketed by the kDegridTile^2
-                // tile of their first tap, in sample order, cut into blocks of
-                // at most kDegridBlock; ids, first taps, weights and dcf are
-                // copied in that order
-                const int ntx = (d.G1 + kDegridTile - 1) / kDegridTile;
-                const int nty = (d.G2 + kDegridTile - 1) / kDegridTile;
-                std::vector<int> tcnt((size_t)ntx * nty + 1, 0);
-                auto tile_of = [&](int m) { return (g0[1][m] / kDegridTile) * ntx + g0[0][m] / kDegridTile; };
-                for (int m = 0; m < d.M; ++m) tcnt[(size_t)tile_of(m) + 1]++;
-                for (size_t i = 0; i + 1 < tcnt.size(); ++i) tcnt[i + 1] += tcnt[i];
-                const int64_t base = (int64_t)dsamp_h.size();
-                dsamp_h.resize(dsamp_h.size() + (size_t)d.M);
-                std::vector<int> pos(tcnt.begin(), tcnt.end() - 1);
-                for (int m = 0; m < d.M; ++m) dsamp_h[(size_t)(base + pos[(size_t)tile_of(m)]++)] = m;
-                for (int i = 0; i < d.M; ++i) {
-                    const int m = dsamp_h[(size_t)(base + i)];
-                    dbase_h.push_back(sbase[(size_t)(d.samp_off + m)]);
-                    ddcf_h.push_back(sdcf[(size_t)(d.samp_off + m)]);
-                    for (int q = 0; q < Ws; ++q) dwt_h.push_back(swt[(size_t)(d.samp_off + m) * Ws + q]);
-                }
-                dblk_first_.push_back((int)dblk_host_.size());
-                for (int t = 0; t < ntx * nty; ++t)
-                    for (int k = tcnt[(size_t)t]; k < tcnt[(size_t)t + 1]; k += kDegridBlock) {
-                        const int nb = std::min(kDegridBlock, tcnt[(size_t)t + 1] - k);
-                        dblk_host_.push_back(make_int4(((t % ntx) * kDegridTile) | (((t / ntx) * kDegridTile) << 16),
-                                                       (int)(base + k), nb, 0));
-                    }
-                dblk_count_.push_back((int)dblk_host_.size() - dblk_first_.back());
-                if (dwt_h.size() > (size_t)INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples");
-            }
-            gtile_first_.push_back((int)gtile_host_.size());
-            int npc = 0, nde = 0;
-            build_grid_tiles(d.G1, d.G2, d.M, cnt.data(), csr_ent_h.data() + e0, gtile_host_, gstage_h, gptr_h,
-                             gent_h, npc, nde);
-            gtile_count_.push_back((int)gtile_host_.size() - gtile_first_.back());
-            gpcells_.push_back(npc);
-            {
-                // the rectangles (chunks counted once) tile the grid and hold
-                // every entry once
-                int64_t cells = 0, ents = 0;
-                for (int t = gtile_first_.back(); t < (int)gtile_host_.size(); ++t) {
-                    const GridTile& q = gtile_host_[(size_t)t];
-                    if (q.kind == 0 || q.pofs == q.pofs0) cells += (int64_t)q.w * q.h;
-                    ents += q.n_ent;
-                }
-                if (cells != ncell || ents != nent)
-                    fail(MM_ERR_HIP, "nufftOp: internal: gridding rectangles do not tile the grid");
-            }
             tlist_off_.push_back((int64_t)tlist.size());
             d.tlist_off = tlist_off_.back();
             tw_first_.push_back((int)tw_host_.size());
@@ -1650,11 +1593,6 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
             moff += d.M;
             continue;
         }
-        gtile_first_.push_back((int)gtile_host_.size());
-        gtile_count_.push_back(0);
-        gpcells_.push_back(0);
-        dblk_first_.push_back((int)dblk_host_.size());
-        dblk_count_.push_back(0);
         // gridding tiles: each sample joins every tile its footprint touches
         const int nt1 = (d.G1 + kTile - 1) / kTile, nt2 = (d.G2 + kTile - 1) / kTile;
         std::vector<std::vector<int>> lists((size_t)nt1 * nt2);
@@ -1796,6 +1734,9 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     sbase_.upload(sbase, st);
     swt_.upload(swt, st);
     sdcf_.upload(sdcf, st);
+    sbase_host_ = sbase;
+    swt_host_ = swt;
+    sdcf_host_ = sdcf;
     tlist_.upload(tlist.empty() ? std::vector<int>{0} : tlist, st);
     tiles_.upload(tw_host_.empty() ? std::vector<TileWork>{TileWork{}} : tw_host_, st);
     // the transposed data layout is used when any tile is gathered
@@ -1804,15 +1745,8 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     if (csr_) {
         csr_ptr_.upload(csr_ptr_host_, st);
         csr_ent_.upload(csr_ent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : csr_ent_h, st);
-        gstage_.upload(gstage_h.empty() ? std::vector<int>{0} : gstage_h, st);
-        gptr_.upload(gptr_h.empty() ? std::vector<int>{0} : gptr_h, st);
-        gent_.upload(gent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : gent_h, st);
-        dblk_.upload(dblk_host_.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : dblk_host_, st);
-        dsamp_.upload(dsamp_h.empty() ? std::vector<int>{0} : dsamp_h, st);
-        dbase_.upload(dbase_h.empty() ? std::vector<uint32_t>{0u} : dbase_h, st);
-        ddcf_.upload(ddcf_h.empty() ? std::vector<float>{0.f} : ddcf_h, st);
-        dwt_.upload(dwt_h.empty() ? std::vector<float>{0.f} : dwt_h, st);
     }
+    build_partitions();
     gwork_.upload(gw_host_.empty() ? std::vector<GatherWork>{GatherWork{}} : gw_host_, st);
     glist_.upload(glist_h.empty() ? std::vector<int>{0} : glist_h, st);
     gbin_.upload(gbin_h.empty() ? std::vector<int>{0} : gbin_h, st);
@@ -1823,6 +1757,191 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
 // G*T values and the G-1 pass twiddles
 static size_t fft_lds_bytes(int G, int T, bool inplace) {
     return (size_t)((inplace ? 1 : 2) * G * T + G) * sizeof(float2);
+}
+
+// Plan-time partitions of every distinct 2-D trajectory (slices that share
+// one share its partitions): the staged gridding rectangles (caps
+// stage_samples_ / stage_entries_) and the staged degridding blocks
+// (degrid_block_ samples per kDegridTile^2 tile), with their pools uploaded.
+// Rebuilt when a tuning option changes them; every work table is rebuilt
+// with the next configuration.
+void NufftOp::build_partitions() {
+    const int ns = (int)sl_.size();
+    gtile_host_.clear();
+    dblk_host_.clear();
+    gtile_first_.assign((size_t)ns, 0);
+    gtile_count_.assign((size_t)ns, 0);
+    gpcells_.assign((size_t)ns, 0);
+    dblk_first_.assign((size_t)ns, 0);
+    dblk_count_.assign((size_t)ns, 0);
+    cfgs_.clear();
+    if (ndim_ != 2) return;
+    std::vector<int> gstage_h, gptr_h, dsamp_h;
+    std::vector<int2> gent_h;
+    std::vector<uint32_t> dbase_h;
+    std::vector<float> ddcf_h, dwt_h;
+    std::map<int64_t, int> first_of;   // CSR offset -> first slice with that trajectory
+    for (int s = 0; s < ns; ++s) {
+        const NuSliceDev& d = sl_[(size_t)s];
+        auto it = first_of.find(d.csr_off);
+        if (it != first_of.end()) {
+            const size_t t = (size_t)it->second;
+            gtile_first_[(size_t)s] = gtile_first_[t];
+            gtile_count_[(size_t)s] = gtile_count_[t];
+            gpcells_[(size_t)s] = gpcells_[t];
+            dblk_first_[(size_t)s] = dblk_first_[t];
+            dblk_count_[(size_t)s] = dblk_count_[t];
+            continue;
+        }
+        first_of.emplace(d.csr_off, s);
+        const int64_t ncell = (int64_t)d.G1 * d.G2;
+        const int* rp = csr_ptr_host_.data() + d.csr_off;
+        const int2* en = csr_ent_host_.data() + d.ent_off;
+        const int64_t nent = rp[ncell];
+        gtile_first_[(size_t)s] = (int)gtile_host_.size();
+        int npc = 0, nde = 0;
+        build_grid_tiles(d.G1, d.G2, d.M, rp, en, stage_samples_, stage_entries_, gtile_host_, gstage_h, gptr_h,
+                         gent_h, npc, nde);
+        gtile_count_[(size_t)s] = (int)gtile_host_.size() - gtile_first_[(size_t)s];
+        gpcells_[(size_t)s] = npc;
+        {
+            // the rectangles (chunks counted once) tile the grid and hold
+            // every entry once
+            int64_t cells = 0, ents = 0;
+            for (int t = gtile_first_[(size_t)s]; t < (int)gtile_host_.size(); ++t) {
+                const GridTile& q = gtile_host_[(size_t)t];
+                if (q.kind == 0 || q.pofs == q.pofs0) cells += (int64_t)q.w * q.h;
+                ents += q.n_ent;
+            }
+            if (cells != ncell || ents != nent)
+                fail(MM_ERR_HIP, "nufftOp: internal: gridding rectangles do not tile the grid");
+        }
+        // degridding blocks: samples bucketed by the kDegridTile^2 tile of
+        // their first tap, in sample order, cut into blocks of at most
+        // degrid_block_; ids, first taps, weights and dcf copied in that order
+        const int Ws = d.W1 + d.W2;
+        const uint32_t* sb = sbase_host_.data() + d.samp_off;
+        const int ntx = (d.G1 + kDegridTile - 1) / kDegridTile;
+        const int nty = (d.G2 + kDegridTile - 1) / kDegridTile;
+        std::vector<int> tcnt((size_t)ntx * nty + 1, 0);
+        auto tile_of = [&](int m) {
+            return (int)((sb[m] >> 16) / kDegridTile) * ntx + (int)((sb[m] & 0xffff) / kDegridTile);
+        };
+        for (int m = 0; m < d.M; ++m) tcnt[(size_t)tile_of(m) + 1]++;
+        for (size_t i = 0; i + 1 < tcnt.size(); ++i) tcnt[i + 1] += tcnt[i];
+        const int64_t base = (int64_t)dsamp_h.size();
+        dsamp_h.resize(dsamp_h.size() + (size_t)d.M);
+        std::vector<int> pos(tcnt.begin(), tcnt.end() - 1);
+        for (int m = 0; m < d.M; ++m) dsamp_h[(size_t)(base + pos[(size_t)tile_of(m)]++)] = m;
+        for (int i = 0; i < d.M; ++i) {
+            const int m = dsamp_h[(size_t)(base + i)];
+            dbase_h.push_back(sb[m]);
+            ddcf_h.push_back(sdcf_host_[(size_t)(d.samp_off + m)]);
+            for (int q = 0; q < Ws; ++q) dwt_h.push_back(swt_host_[(size_t)(d.samp_off + m) * Ws + q]);
+        }
+        dblk_first_[(size_t)s] = (int)dblk_host_.size();
+        for (int t = 0; t < ntx * nty; ++t)
+            for (int k = tcnt[(size_t)t]; k < tcnt[(size_t)t + 1]; k += degrid_block_) {
+                const int nb = std::min(degrid_block_, tcnt[(size_t)t + 1] - k);
+                dblk_host_.push_back(make_int4(((t % ntx) * kDegridTile) | (((t / ntx) * kDegridTile) << 16),
+                                               (int)(base + k), nb, 0));
+            }
+        dblk_count_[(size_t)s] = (int)dblk_host_.size() - dblk_first_[(size_t)s];
+        if (dwt_h.size() > (size_t)INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples");
+    }
+    hipStream_t st = ctx->stream;
+    gstage_.upload(gstage_h.empty() ? std::vector<int>{0} : gstage_h, st);
+    gptr_.upload(gptr_h.empty() ? std::vector<int>{0} : gptr_h, st);
+    gent_.upload(gent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : gent_h, st);
+    dblk_.upload(dblk_host_.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : dblk_host_, st);
+    dsamp_.upload(dsamp_h.empty() ? std::vector<int>{0} : dsamp_h, st);
+    dbase_.upload(dbase_h.empty() ? std::vector<uint32_t>{0u} : dbase_h, st);
+    ddcf_.upload(ddcf_h.empty() ? std::vector<float>{0.f} : ddcf_h, st);
+    dwt_.upload(dwt_h.empty() ? std::vector<float>{0.f} : dwt_h, st);
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+bool NufftOp::set_option(const std::string& key, int value) {
+    auto flag = [&](bool& f, bool clear) {
+        f = value != 0;
+        if (clear) cfgs_.clear();
+        return true;
+    };
+    if (key == "nu_degrid_t") return flag(degrid_t_, false);
+    if (key == "nu_pair_cols") return flag(pair_cols_, true);
+    if (key == "nu_degrid_stage") return flag(degrid_stage_, false);
+    if (key == "nu_grid_stage") return flag(grid_stage_, false);
+    if (key == "nu_xcd_order") return flag(xcd_order_, true);
+    // caps keep every staged work-group within 64 KB of LDS at 16 transforms
+    if (key == "nu_stage_samples" || key == "nu_stage_entries" || key == "nu_degrid_block") {
+        int* dst = key == "nu_stage_samples" ? &stage_samples_
+                   : key == "nu_stage_entries" ? &stage_entries_ : &degrid_block_;
+        const int lo = key == "nu_stage_entries" ? 64 : 16, hi = key == "nu_stage_entries" ? 2048 : 256;
+        if (value < lo || value > hi) return false;
+        if (*dst != value) {
+            *dst = value;
+            build_partitions();
+        }
+        return true;
+    }
+    return false;
+}
+
+bool NufftOp::get_option(const std::string& key, int* value) const {
+    if (!value) return false;
+    if (key == "nu_degrid_t") *value = degrid_t_;
+    else if (key == "nu_pair_cols") *value = pair_cols_;
+    else if (key == "nu_degrid_stage") *value = degrid_stage_;
+    else if (key == "nu_grid_stage") *value = grid_stage_;
+    else if (key == "nu_xcd_order") *value = xcd_order_;
+    else if (key == "nu_stage_samples") *value = stage_samples_;
+    else if (key == "nu_stage_entries") *value = stage_entries_;
+    else if (key == "nu_degrid_block") *value = degrid_block_;
+    else return false;
+    return true;
+}
+
+bool NufftOp::autotune_sense(const float2* maps, int C, const float* W, int iters) {
+    if (ndim_ != 2) return false;
+    iters = std::max(iters, 2);
+    DevBuf<float2> in((size_t)std::max<int64_t>(V_total_, 1)), out((size_t)std::max<int64_t>(V_total_, 1));
+    HIPCHECK(hipMemsetAsync(in.p, 0, sizeof(float2) * in.n, ctx->stream));
+    // time of the named stages per application (ms); one untimed warm-up
+    auto measure = [&](std::initializer_list<const char*> stages) {
+        std::vector<std::pair<std::string, std::pair<double, int>>> k;
+        profile_normal(in.p, out.p, 1, maps, C, W, 1, &k);
+        profile_normal(in.p, out.p, 1, maps, C, W, iters, &k);
+        double t = 0;
+        for (const auto& e : k)
+            for (const char* nm : stages)
+                if (e.first == nm) t += e.second.first * e.second.second;
+        return t;
+    };
+    // one option at a time: candidate values, the stages they move
+    auto tune = [&](std::vector<std::vector<std::pair<const char*, int>>> cands,
+                    std::initializer_list<const char*> stages) {
+        double best = 1e30;
+        size_t bi = 0;
+        for (size_t i = 0; i < cands.size(); ++i) {
+            for (const auto& kv : cands[i]) set_option(kv.first, kv.second);
+            const double t = measure(stages);
+            if (t < best) {
+                best = t;
+                bi = i;
+            }
+        }
+        for (const auto& kv : cands[bi]) set_option(kv.first, kv.second);
+    };
+    tune({{{"nu_stage_samples", 96}, {"nu_stage_entries", 768}},
+          {{"nu_stage_samples", 128}, {"nu_stage_entries", 1024}},
+          {{"nu_stage_samples", 192}, {"nu_stage_entries", 1536}},
+          {{"nu_stage_samples", 256}, {"nu_stage_entries", 2048}}},
+         {"nu_grid"});
+    tune({{{"nu_degrid_block", 32}}, {{"nu_degrid_block", 64}}, {{"nu_degrid_block", 128}}}, {"nu_degrid"});
+    tune({{{"nu_pair_cols", 1}}, {{"nu_pair_cols", 0}}}, {"nu_cols"});
+    tune({{{"nu_xcd_order", 0}}, {{"nu_xcd_order", 1}}}, {"nu_grid", "nu_degrid"});
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return true;
 }
 
 NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
@@ -2107,7 +2226,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.w_db.upload(wdb.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wdb, st);
     c.n_db = (int)wdb.size();
     c.lds_db = (size_t)(kDegridTile + W_ - 1) * (kDegridTile + W_ - 1) * BCT * sizeof(float2) +
-               (size_t)kDegridBlock * (3 + 2 * W_) * sizeof(float);
+               (size_t)degrid_block_ * (3 + 2 * W_) * sizeof(float);
     c.n_grid = (int)wg.size();
     if (gather_) c.yT.alloc(std::max<int64_t>(ytoff, 1));
     c.ws.alloc(std::max<int64_t>(wsoff, 1));
@@ -2156,10 +2275,12 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         if (ndim_ == 2 && degrid_stage_ && c.n_db > 0) {
             if (wt4)
                 hipLaunchKernelGGL((nu_degrid_tile<BCT, 4>), dim3(c.n_db), dim3(256), c.lds_db, st, sl, c.w_db.p,
-                                   dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y, r);
+                                   dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y,
+                                   degrid_block_, r);
             else
                 hipLaunchKernelGGL((nu_degrid_tile<BCT, 0>), dim3(c.n_db), dim3(256), c.lds_db, st, sl, c.w_db.p,
-                                   dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y, r);
+                                   dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y,
+                                   degrid_block_, r);
         } else if (BCT >= 2 && degrid_t_) {
             if (wt4)
                 hipLaunchKernelGGL((nu_degrid_t<BCT, 4>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p,

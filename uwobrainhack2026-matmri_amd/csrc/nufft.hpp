@@ -59,8 +59,8 @@ constexpr int kTile = 16;
 constexpr int kTileChunk = 512;
 
 // One rectangle of grid cells for 2-D CSR gridding, with the union of the
-// samples that reach it (sorted, at most kStageSamples) and the cells'
-// (local sample, weight) rows (at most kStageEntries entries), staged in
+// samples that reach it (sorted, at most stage_samples_) and the cells'
+// (local sample, weight) rows (at most stage_entries_ entries), staged in
 // LDS by its work-group.  kind 0: the whole rectangle, written to the grid;
 // kind 2: one chunk of the samples of a dense rectangle (k-space centre),
 // written to a partial block; a second launch sums the blocks in chunk order
@@ -90,9 +90,6 @@ struct GridItem {
     int n_ent, pofs, pofs0, nparts;
 };
 constexpr int kDegridTile = 8;      // tile side of the staged degridding
-constexpr int kDegridBlock = 64;    // samples per staged degridding block
-constexpr int kStageSamples = 128;
-constexpr int kStageEntries = 1024;
 
 class NufftOp : public mm_op {
    public:
@@ -119,35 +116,22 @@ class NufftOp : public mm_op {
     void profile_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                         const float* W, int iters,
                         std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
-    // tuning: degridding variant ("nu_degrid_t": 1 = one lane per (sample, coil)),
-    // 2-D gridding from LDS-staged rectangles ("nu_grid_stage": 1) or from
-    // the global CSR rows only (0), 2-D degridding from LDS-staged tiles
-    // ("nu_degrid_stage": 1) or from the grid in global memory (0)
-    bool set_option(const std::string& key, int value) override {
-        if (key == "nu_degrid_t") {
-            degrid_t_ = value != 0;
-            return true;
-        }
-        if (key == "nu_pair_cols") {
-            pair_cols_ = value != 0;
-            cfgs_.clear();
-            return true;
-        }
-        if (key == "nu_degrid_stage") {
-            degrid_stage_ = value != 0;
-            return true;
-        }
-        if (key == "nu_grid_stage") {
-            grid_stage_ = value != 0;
-            return true;
-        }
-        if (key == "nu_xcd_order") {
-            xcd_order_ = value != 0;
-            cfgs_.clear();
-            return true;
-        }
-        return false;
-    }
+    // Tuning options (mm_op_set_option); the plan-time partitions are rebuilt
+    // when their caps change:
+    //   nu_stage_samples / nu_stage_entries  staged gridding rectangle caps
+    //   nu_degrid_block                      samples per staged degridding block
+    //   nu_pair_cols        XCD-paired order of the column-pass items (0/1)
+    //   nu_xcd_order        XCD-aware order of the staged items (0/1)
+    //   nu_grid_stage       2-D gridding from staged rectangles (1) or the global CSR rows (0)
+    //   nu_degrid_stage     2-D degridding from staged tiles (1) or the grid in global memory (0)
+    //   nu_degrid_t         unstaged degridding with one lane per (sample, coil) (1)
+    bool set_option(const std::string& key, int value) override;
+    bool get_option(const std::string& key, int* value) const override;
+    // The autotuner of the SENSE normal operator (nrep 1, C coils): times the
+    // candidate gridding caps, degridding block sizes, column-pass order and
+    // staged-item order with profile_normal (`iters` applications each),
+    // one option at a time, and keeps the fastest.
+    bool autotune_sense(const float2* maps, int C, const float* W, int iters);
     void set_skip(const int* flags) override { skip_ = flags; }
     bool gather() const { return gather_; }
     int ndim() const { return ndim_; }
@@ -234,6 +218,13 @@ class NufftOp : public mm_op {
     // of each distinct 2-D trajectory; slices that share a trajectory share
     // its rectangles
     bool grid_stage_ = true;
+    int stage_samples_ = 128, stage_entries_ = 1024;   // staged gridding rectangle caps
+    int degrid_block_ = 64;                             // samples per staged degridding block
+    void build_partitions();
+    // host copies the partitions are rebuilt from
+    std::vector<int2> csr_ent_host_;
+    std::vector<uint32_t> sbase_host_;
+    std::vector<float> swt_host_, sdcf_host_;
     bool pair_cols_ = true;   // XCD-paired order of the column-pass items (option "nu_pair_cols")
     // staged degridding (option "nu_degrid_stage", default on): sample
     // blocks of each distinct 2-D trajectory, (tile | tile2 << 16, first

@@ -165,6 +165,11 @@ bool SenseOp::profile_dominant(int adjoint, const float2* in, float2* out, int64
     return false;
 }
 
+bool SenseOp::autotune(int iters) {
+    if (!nu_) return false;
+    return nu_->autotune_sense(static_cast<RcvrOp*>(R_)->maps(), C_, W, iters);
+}
+
 void SenseOp::apply_normal(const float2* in, float2* out, int64_t nrep, float2* tmp) {
     if (nu_) {
         nu_->apply_sense_normal(in, out, nrep, static_cast<RcvrOp*>(R_)->maps(), C_, W);

@@ -39,6 +39,8 @@ class SenseOp : public mm_op {
     bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
                           double* ms, std::string* name) override;
     bool set_option(const std::string& key, int value) override { return S_->set_option(key, value); }
+    bool get_option(const std::string& key, int* value) const override { return S_->get_option(key, value); }
+    bool autotune(int iters) override;
     void set_skip(const int* flags) override { S_->set_skip(flags); }
     mm_op* sampling() const { return S_; }
     int coils() const { return C_; }

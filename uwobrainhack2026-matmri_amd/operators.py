@@ -97,6 +97,20 @@ class LinearOp:
         check(_lib.lib().mm_op_set_option(self._plan.h, key.encode(), int(value)))
         return self
 
+    def get_option(self, key):
+        """Current value of a tuning knob (see mm_op_get_option)."""
+        v = C.c_int()
+        check(_lib.lib().mm_op_get_option(self._plan.h, key.encode(), C.byref(v)))
+        return v.value
+
+    TUNED_OPTIONS = ("nu_stage_samples", "nu_stage_entries", "nu_degrid_block", "nu_pair_cols", "nu_xcd_order")
+
+    def autotune(self, iters=10):
+        """Run the device autotuner of the normal operator (mm_op_autotune)
+        and return the options it chose."""
+        check(_lib.lib().mm_op_autotune(self._plan.h, int(iters)))
+        return {k: self.get_option(k) for k in self.TUNED_OPTIONS}
+
     def normal(self, x):
         """``A'*(A*x)`` in one call (the operator cgne iterates)."""
         return _apply(self._plan.h, 2, x, self.image_shape, self.image_shape, self._shapes())
