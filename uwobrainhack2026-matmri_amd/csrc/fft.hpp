@@ -224,14 +224,16 @@ __device__ __forceinline__ void fft_pass(const float2* __restrict__ a, float2* _
 }
 
 // Butterflies a thread holds in registers in a single-buffer pass of radix
-// R (8 complex values at most).
-__host__ __device__ constexpr int fft_inplace_maxb(int R) { return R > 8 ? 1 : 8 / R; }
+// R: 8 complex values at most, 16 in kernels with 16 interleaved transforms
+// (which run at fewer waves per SIMD).
+__host__ __device__ constexpr int fft_inplace_maxb(int R, int T) { return (T >= 16 ? 16 : 8) / R; }
 
 // Whether every pass of the transform (length n, T interleaved transforms)
-// can run in a single buffer: each thread's butterflies fit its registers.
-inline bool fft_inplace_ok(const FftFactors& f, int T) {
+// can run in a single buffer: each thread's butterflies (at most cap / R)
+// fit its registers; cap = 16 for T >= 16, else 8 (fft_inplace_maxb).
+inline bool fft_inplace_ok(const FftFactors& f, int T, int cap) {
     for (int s = 0; s < f.nf; ++s)
-        if ((f.n / f.r[s]) * T > fft_inplace_maxb(f.r[s]) * kFftThreads) return false;
+        if ((f.n / f.r[s]) * T > (cap / f.r[s]) * kFftThreads) return false;
     return true;
 }
 
@@ -241,7 +243,7 @@ inline bool fft_inplace_ok(const FftFactors& f, int T) {
 template <int R, int T, int SGN, bool TW, bool BIG>
 __device__ __forceinline__ void fft_pass_inplace(float2* __restrict__ a, int n, int p,
                                                  const float2* __restrict__ twp) {
-    constexpr int MAXB = fft_inplace_maxb(R);
+    constexpr int MAXB = fft_inplace_maxb(R, T);
     constexpr int LT = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6;
     const int nb = n / R;
     const int total = nb * T;

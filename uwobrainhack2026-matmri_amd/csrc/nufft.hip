@@ -309,7 +309,7 @@ __host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_
 // index = (position, row, transform) with the transform fastest.  Offsets
 // inside a slice are 32-bit (the plan checks V*BCT and G1*G2*BCT < 2^31).
 template <int BCT, bool BIG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ? 4 : 8, 8))) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
                                                    const float2* __restrict__ twp,
@@ -1284,7 +1284,7 @@ __global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl
 // pixel, each summing half of the coils; the halves are added through a
 // lane shuffle.
 template <int BCT, bool BIG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ? 4 : 8, 8))) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
                                                    const float2* __restrict__ twp,
@@ -1991,7 +1991,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
             FftFactors f1, f2;
             fft_factorize(d.G1, f1);
             fft_factorize(d.G2, f2);
-            const bool ip1 = fft_inplace_ok(f1, TR), ip2 = fft_inplace_ok(f2, TC);
+            const bool ip1 = fft_inplace_ok(f1, TR, TR >= 16 ? 16 : 8), ip2 = fft_inplace_ok(f2, TC, 8);
             d.fip = (ip1 ? 1 : 0) | (ip2 ? 2 : 0);
             c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR, ip1));
             c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, TC, ip2));
