@@ -296,13 +296,14 @@ __device__ __forceinline__ int64_t chunk_stride(const NuRun& r, int BCT) { retur
 
 // Transforms per work-group of the FFT passes (compile-time): the row
 // passes take all BCT transforms of a chunk for max(1, 8/BCT) rows, the
-// column passes 8 transforms (8/CL columns of CL = min(BCT, 8) transforms).
+// column passes 8 transforms (8/CL columns of CL = min(BCT, 8) transforms),
+// or all 16 of a 16-transform chunk (one column: whole 128-byte lines).
 // Grids with a side above kFftSmallMax (BIG) take one row / one column per
 // work-group so that the two LDS buffers fit.
 __host__ __device__ constexpr int clog2(int v) { return v <= 1 ? 0 : 1 + clog2(v / 2); }
 __host__ __device__ constexpr int rows_T(int BCT, bool BIG) { return BIG || BCT >= 8 ? BCT : 8; }
-__host__ __device__ constexpr int cols_CL(int BCT) { return BCT < 8 ? BCT : 8; }
-__host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_CL(BCT) : 8; }
+__host__ __device__ constexpr int cols_CL(int BCT) { return BCT < 16 ? (BCT < 8 ? BCT : 8) : 16; }
+__host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_CL(BCT) : (BCT >= 16 ? 16 : 8); }
 
 // Forward rows: coil multiply + deapodisation, image written wrapped into
 // the zero-padded row (positions (i1 - N1/2) mod G1), FFT along g1.  Lane
@@ -1991,7 +1992,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
             FftFactors f1, f2;
             fft_factorize(d.G1, f1);
             fft_factorize(d.G2, f2);
-            const bool ip1 = fft_inplace_ok(f1, TR, TR >= 16 ? 16 : 8), ip2 = fft_inplace_ok(f2, TC, 8);
+            const bool ip1 = fft_inplace_ok(f1, TR, TR >= 16 ? 16 : 8), ip2 = fft_inplace_ok(f2, TC, TC >= 16 ? 16 : 8);
             d.fip = (ip1 ? 1 : 0) | (ip2 ? 2 : 0);
             c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR, ip1));
             c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, TC, ip2));
