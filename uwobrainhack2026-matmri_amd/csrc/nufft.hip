@@ -1179,41 +1179,74 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
     const int* tp = tptr + T.ptr_off;
     for (int i = threadIdx.x; i <= cells; i += 256) sp[i] = tp[i];
     __syncthreads();
-    float2* zg = ws + T.zg + b;
-    float2* pb = parts + T.part + (int64_t)T.pofs * BCT + b;   // kind 2: this chunk's block
-    for (int c = grp; c < cells; c += NG) {
-        float2 acc = make_float2(0.f, 0.f);
-        if (b < lanes) {
+    // gather: CPL consecutive transforms per lane (one 8*CPL-byte LDS read of
+    // the sample's values per entry), BCT/CPL lanes per cell
+    constexpr int CPL = BCT >= 2 ? 2 : 1;
+    constexpr int LPC = BCT / CPL;
+    constexpr int NGC = 256 / LPC;
+    const int bc = (int)(threadIdx.x & (LPC - 1)) * CPL;
+    const int gc = (int)threadIdx.x / LPC;
+    float2* zg = ws + T.zg + bc;
+    float2* pb = parts + T.part + (int64_t)T.pofs * BCT + bc;   // kind 2: this chunk's block
+    const float2* svb = sv + bc;
+    for (int c = gc; c < cells; c += NGC) {
+        float2 acc[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) acc[k] = make_float2(0.f, 0.f);
+        if (bc < lanes) {
             int e = sp[c];
             const int e1 = sp[c + 1];
             for (; e + 4 <= e1; e += 4) {
                 int2 q[4];
-                float2 v[4];
+                float2 v[4][CPL];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) q[i] = se[e + i];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = sv[q[i].x * BCT + b];
+                for (int i = 0; i < 4; ++i) {
+                    if constexpr (CPL == 2) {
+                        const float4 u = *reinterpret_cast<const float4*>(svb + q[i].x * BCT);
+                        v[i][0] = make_float2(u.x, u.y);
+                        v[i][CPL - 1] = make_float2(u.z, u.w);
+                    } else {
+                        v[i][0] = svb[q[i].x * BCT];
+                    }
+                }
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const float w = __int_as_float(q[i].y);
-                    acc.x = fmaf(w, v[i].x, acc.x);
-                    acc.y = fmaf(w, v[i].y, acc.y);
+#pragma unroll
+                    for (int k = 0; k < CPL; ++k) {
+                        acc[k].x = fmaf(w, v[i][k].x, acc[k].x);
+                        acc[k].y = fmaf(w, v[i][k].y, acc[k].y);
+                    }
                 }
             }
             for (; e < e1; ++e) {
                 const int2 q = se[e];
-                const float2 v = sv[q.x * BCT + b];
                 const float w = __int_as_float(q.y);
-                acc.x = fmaf(w, v.x, acc.x);
-                acc.y = fmaf(w, v.y, acc.y);
+#pragma unroll
+                for (int k = 0; k < CPL; ++k) {
+                    const float2 v = svb[q.x * BCT + k];
+                    acc[k].x = fmaf(w, v.x, acc[k].x);
+                    acc[k].y = fmaf(w, v.y, acc[k].y);
+                }
             }
+            // a transform past the chunk's last one stays zero
+#pragma unroll
+            for (int k = 1; k < CPL; ++k)
+                if (bc + k >= lanes) acc[k] = make_float2(0.f, 0.f);
         }
+        float2* dst;
         if (T.kind == 0) {
             const int r = c / T.w;
-            zg[((int64_t)(T.g2 + r) * T.G1 + T.g1 + (c - r * T.w)) * BCT] = acc;
+            dst = zg + ((int64_t)(T.g2 + r) * T.G1 + T.g1 + (c - r * T.w)) * BCT;
         } else {
-            pb[c * BCT] = acc;
+            dst = pb + c * BCT;
         }
+        if constexpr (CPL == 2)
+            *reinterpret_cast<float4*>(dst) = make_float4(acc[0].x, acc[0].y, acc[CPL - 1].x, acc[CPL - 1].y);
+        else
+            dst[0] = acc[0];
     }
 }
 
