@@ -410,6 +410,7 @@ CgSolver::CgSolver(mm_op* E) : E_(E) {
     q_.alloc(n_);
     tmp_.alloc(E->n_data());
     HIPCHECK(hipHostMalloc((void**)&done_h_, sizeof(int) * ns, hipHostMallocDefault));
+    HIPCHECK(hipEventCreateWithFlags(&poll_ev_, hipEventDisableTiming));
     HIPCHECK(hipStreamSynchronize(st));
 }
 
@@ -417,6 +418,7 @@ CgSolver::~CgSolver() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (done_h_) (void)hipHostFree(done_h_);
+    if (poll_ev_) (void)hipEventDestroy(poll_ev_);
 }
 
 void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x) {
@@ -497,6 +499,11 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
         ~SkipGuard() { E->set_skip(nullptr); }
     } guard{E_};
     E_->set_skip(done_d_.p);
+    // The done flags are copied to the host every `poll` iterations; the host
+    // reads them only after the next iteration has been queued, so the device
+    // never waits for the host (a solve that has finished runs one more
+    // iteration whose kernels all return at once).
+    bool pending = false;
     for (int it = 0; it < NitMax; ++it) {
         const int explicit_res = (it % expResN) == 0;
         if (explicit_res || !use_graph_) {
@@ -511,12 +518,17 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
             }
             HIPCHECK(hipGraphLaunch(graph_exec_, st));
         }
-        if ((it + 1) % poll == 0 && it + 1 < NitMax) {
-            HIPCHECK(hipMemcpyAsync(done_h_, done_d_.p, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
+        if (pending) {
+            pending = false;
+            HIPCHECK(hipEventSynchronize(poll_ev_));
             bool all = true;
             for (int s = 0; s < ns; ++s) all &= done_h_[s] != 0;
             if (all) break;
+        }
+        if ((it + 1) % poll == 0 && it + 2 < NitMax) {
+            HIPCHECK(hipMemcpyAsync(done_h_, done_d_.p, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipEventRecord(poll_ev_, st));
+            pending = true;
         }
     }
     // results

@@ -32,6 +32,7 @@ class CgSolver {
     DevBuf<int> done_d_;
     DevBuf<float2> b_, r_, p_, q_, tmp_;
     int* done_h_ = nullptr;
+    hipEvent_t poll_ev_ = nullptr;   // recorded after the copy of the done flags
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     float2* graph_x_ = nullptr;

@@ -11,32 +11,36 @@ namespace mm {
 
 void mm_op_default_normal(mm_op* op, const float2* in, float2* out, int64_t nrep, float2* tmp);
 
+// Slices of one image size V run in one launch: image j = slice*nrep + rep,
+// image j's coil data at j*C*V, slice s's maps at s*C*V.
 __global__ __launch_bounds__(256) void rcvr_fwd_k(const float2* __restrict__ x, const float2* __restrict__ maps,
-                           float2* __restrict__ y, int64_t V, int C, int64_t nrep) {
-    const int64_t n = V * C * nrep;
+                           float2* __restrict__ y, int64_t V, int C, int64_t nrep, int64_t nimg) {
+    const int64_t n = V * C * nimg;
     for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * 256) {
         const int64_t v = i % V;
         const int64_t rest = i / V;
         const int c = (int)(rest % C);
-        const int64_t r = rest / C;
-        y[i] = cmulf(x[r * V + v], maps[(int64_t)c * V + v]);
+        const int64_t j = rest / C;
+        y[i] = cmulf(x[j * V + v], maps[((j / nrep) * C + c) * V + v]);
     }
 }
 
 __global__ __launch_bounds__(256) void rcvr_adj_k(const float2* __restrict__ y, const float2* __restrict__ maps,
-                           float2* __restrict__ x, int64_t V, int C, int64_t nrep) {
-    const int64_t n = V * nrep;
+                           float2* __restrict__ x, int64_t V, int C, int64_t nrep, int64_t nimg) {
+    const int64_t n = V * nimg;
     for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * 256) {
         const int64_t v = i % V;
-        const int64_t r = i / V;
+        const int64_t j = i / V;
+        const float2* mp = maps + (j / nrep) * C * V + v;
+        const float2* q = y + j * C * V + v;
         float2 s = make_float2(0.f, 0.f);
         for (int c = 0; c < C; ++c) {
-            const float2 m = maps[(int64_t)c * V + v];
-            const float2 q = y[(r * C + c) * V + v];
-            s.x += m.x * q.x + m.y * q.y;
-            s.y += m.x * q.y - m.y * q.x;
+            const float2 m = mp[(int64_t)c * V];
+            const float2 d = q[(int64_t)c * V];
+            s.x += m.x * d.x + m.y * d.y;
+            s.y += m.x * d.y - m.y * d.x;
         }
         x[i] = s;
     }
@@ -76,18 +80,22 @@ RcvrOp::RcvrOp(mm_ctx* c, int nslice, const int64_t* V, int64_t C, const float* 
 
 void RcvrOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     hipStream_t st = ctx->stream;
-    for (size_t s = 0; s < Vs_.size(); ++s) {
-        const int64_t V = Vs_[s];
-        const float2* mp = maps_.p + start_[s] * C_;
-        const int64_t io = start_[s] * nrep, dof = start_[s] * C_ * nrep;
+    // Runs of consecutive slices with one image size go in one launch.
+    for (size_t s0 = 0; s0 < Vs_.size();) {
+        size_t s1 = s0 + 1;
+        while (s1 < Vs_.size() && Vs_[s1] == Vs_[s0]) ++s1;
+        const int64_t V = Vs_[s0], nimg = (int64_t)(s1 - s0) * nrep;
+        const float2* mp = maps_.p + start_[s0] * C_;
+        const int64_t io = start_[s0] * nrep, dof = start_[s0] * C_ * nrep;
         if (!adjoint) {
-            hipLaunchKernelGGL(rcvr_fwd_k, dim3(grid_for(V * C_ * nrep)), dim3(256), 0, st, in + io, mp,
-                               out + dof, V, C_, nrep);
+            hipLaunchKernelGGL(rcvr_fwd_k, dim3(grid_for(V * C_ * nimg)), dim3(256), 0, st, in + io, mp,
+                               out + dof, V, C_, nrep, nimg);
         } else {
-            hipLaunchKernelGGL(rcvr_adj_k, dim3(grid_for(V * nrep)), dim3(256), 0, st, in + dof, mp,
-                               out + io, V, C_, nrep);
+            hipLaunchKernelGGL(rcvr_adj_k, dim3(grid_for(V * nimg)), dim3(256), 0, st, in + dof, mp,
+                               out + io, V, C_, nrep, nimg);
         }
         LAUNCHCHECK();
+        s0 = s1;
     }
 }
 
