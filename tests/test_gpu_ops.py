@@ -239,6 +239,35 @@ def test_degrid_staged_vs_global(mm, kind, nrep):
     assert np.array_equal(f_stage, f_glob)
 
 
+@pytest.mark.parametrize("nslice,nrep", [(3, 8), (2, 20), (1, 40), (4, 1)])
+def test_grid_pairs_vs_single(mm, nslice, nrep):
+    """Staged gridding items that serve two (slice, chunk) units of one
+    trajectory (option nu_grid_pairs) against one unit per item: each unit's
+    cells sum the same products in the same order, so the adjoints are
+    bitwise equal (odd unit counts leave one unit unpaired; nrep > 16 pairs
+    the chunks of one slice, the last chunk partly filled)."""
+    print(f"[case] paired gridding nslice={nslice} nrep={nrep}", flush=True)
+    rng = np.random.default_rng(9)
+    imN, kl = [96, 96], mm.workloads.spiral(9000, 40)
+    S = mm.nufftOp.batch([(imN, kl, None)] * nslice, 1.5, 4)
+    M = kl.shape[0]
+    y = crand(rng, (nslice * M * nrep,)).astype(np.complex64)
+    S.set_option("nu_grid_pairs", 1)
+    assert S.get_option("nu_grid_pairs") == 1
+    a_pair = np.asarray(S.H * y).reshape(-1, order="F")
+    S.set_option("nu_grid_pairs", 0)
+    a_one = np.asarray(S.H * y).reshape(-1, order="F")
+    V = imN[0] * imN[1]
+    # slice 0 against the oracle: its data are the first M * nrep values
+    # (repetition-major), its images the first V * nrep
+    y0 = y[: M * nrep].reshape(nrep, M).T
+    want = NufftPlan(imN, kl, None, 1.5, 4).adjoint(y0 if nrep > 1 else y0[:, 0])
+    e_o = rel(a_pair[: V * nrep], np.asarray(want).reshape(-1, order="F"))
+    print(f"  slice 0 vs oracle {e_o:.2e}; equal: {np.array_equal(a_pair, a_one)}", flush=True)
+    assert e_o < TOL
+    assert np.array_equal(a_pair, a_one)
+
+
 def test_nufft_tuning_options_and_autotune(mm):
     """Every staged-partition option value keeps the SENSE operator within the
     per-application bound; the autotuner picks from its candidate sets and
@@ -255,7 +284,8 @@ def test_nufft_tuning_options_and_autotune(mm):
     want = np.concatenate([o_mrSampFunc(o_mrSampFunc(x, "notransp", So, RcvrPlan(mp)), "transp", So, RcvrPlan(mp))
                            .reshape(-1, order="F") for x, mp in zip(xs, maps)])
     for key, vals in [("nu_stage_samples", [16, 96, 256]), ("nu_stage_entries", [64, 768, 2048]),
-                      ("nu_degrid_block", [16, 128, 256]), ("nu_pair_cols", [0, 1]), ("nu_xcd_order", [1, 0])]:
+                      ("nu_degrid_block", [16, 128, 256]), ("nu_pair_cols", [0, 1]), ("nu_xcd_order", [1, 0]),
+                      ("nu_grid_pairs", [0, 1])]:
         for v in vals:
             E.set_option(key, v)
             assert E.get_option(key) == v
@@ -271,6 +301,7 @@ def test_nufft_tuning_options_and_autotune(mm):
                                                                           (256, 2048)]
     assert chosen["nu_degrid_block"] in (32, 64, 128)
     assert chosen["nu_pair_cols"] in (0, 1) and chosen["nu_xcd_order"] in (0, 1)
+    assert chosen["nu_grid_pairs"] in (0, 1)
     assert rel(E.normal(xcat), want) < TOL
 
 

@@ -1123,49 +1123,57 @@ __global__ __launch_bounds__(256) void nu_grid_csr(const NuSliceDev* __restrict_
 }
 
 // Staged CSR gridding: one work item per rectangle (or chunk of a dense
-// rectangle) of one (slice, chunk).  The work-group first copies, with every
-// lane, the transposed data of its samples ([n_stage][BCT]), its (local
-// sample, weight) rows and row pointers into LDS; each (cell, transform) lane
-// then sums its row from LDS in sample order -- the products and the order
-// of a sum over the global CSR row.  Kind 0 writes the grid cells; kind 2
-// (a chunk of a dense rectangle) writes a partial block, which nu_grid_merge
-// adds in chunk order into the grid (deterministic for a given plan).
-template <int BCT>
+// rectangle) of one or (SG = 2) two (slice, chunk) units of one trajectory.
+// The work-group first copies, with every lane, the transposed data of its
+// samples ([n_stage][BCT * SG]: the second unit's transforms after the
+// first's), its (local sample, weight) rows and row pointers into LDS; each
+// (cell, transform) lane then sums its row from LDS in sample order -- the
+// products and the order of a sum over the global CSR row.  Kind 0 writes the
+// grid cells; kind 2 (a chunk of a dense rectangle) writes a partial block,
+// which nu_grid_merge adds in chunk order into the grid (deterministic for a
+// given plan).  A unit whose slice has finished is neither read nor written.
+template <int BCT, int SG>
 __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__ work,
                                                     const int* __restrict__ stage,
                                                     const int* __restrict__ tptr,
                                                     const int2* __restrict__ tent,
                                                     const float2* __restrict__ yt, float2* __restrict__ ws,
                                                     float2* __restrict__ parts, const int* __restrict__ skip) {
-    constexpr int NG = 256 / BCT;  // lane groups
+    constexpr int TT = BCT * SG;   // transforms per staged sample
+    static_assert(256 % TT == 0, "transforms per sample must divide the work-group");
     extern __shared__ float2 lds[];
     // the item carries every offset: one scalar load before the data loads
     const GridItem T = work[blockIdx.x];
-    if (skip && skip[T.slice]) return;
-    const int b = threadIdx.x & (BCT - 1);
-    const int grp = threadIdx.x / BCT;
-    const int lanes = T.lanes;
-    const float2* ych = yt + T.yt + b;
+    const bool act0 = !(skip && skip[T.slice]);
+    const bool act1 = SG > 1 && T.slice2 >= 0 && !(skip && skip[T.slice2]);
+    if (!act0 && !act1) return;
     const int cells = T.w * T.h;
-    const int nsv = T.n_stage * BCT;
-    float2* sv = lds;                              // [n_stage][BCT]
+    const int nsv = T.n_stage * TT;
+    float2* sv = lds;                              // [n_stage][TT]
     int2* se = (int2*)(lds + nsv);                 // [n_ent]
     int* sp = (int*)(se + T.n_ent);                // [cells + 1]
     const int* sid = stage + T.stage_off;
-    // element i of sv is transform (i mod BCT) = b of sample i / BCT
-    for (int i0 = threadIdx.x; i0 < nsv; i0 += 4 * 256) {
-        int m[4];
+    {
+        // element i of sv is transform (i mod TT) of sample i / TT; 256 is a
+        // multiple of TT, so the transform (and its unit) is fixed per lane
+        const int jb = (int)threadIdx.x & (TT - 1);
+        const int u1 = SG > 1 ? jb / BCT : 0;
+        const bool ld = u1 ? act1 : act0;
+        const float2* ych = yt + (u1 ? T.yt2 : T.yt) + (jb & (BCT - 1));
+        for (int i0 = threadIdx.x; i0 < nsv; i0 += 4 * 256) {
+            int m[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * 256;
-            m[u] = i < nsv ? sid[i / BCT] : 0;
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * 256;
+                m[u] = i < nsv ? sid[i / TT] : 0;
+            }
+            float2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = ld ? ych[(int64_t)m[u] * BCT] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * 256 < nsv) sv[i0 + u * 256] = v[u];
         }
-        float2 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = ych[(int64_t)m[u] * BCT];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (i0 + u * 256 < nsv) sv[i0 + u * 256] = v[u];
     }
     const int2* te = tent + T.ent_off;
     for (int i0 = threadIdx.x; i0 < T.n_ent; i0 += 4 * 256) {
@@ -1179,21 +1187,26 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
     const int* tp = tptr + T.ptr_off;
     for (int i = threadIdx.x; i <= cells; i += 256) sp[i] = tp[i];
     __syncthreads();
-    // gather: CPL consecutive transforms per lane (one 8*CPL-byte LDS read of
-    // the sample's values per entry), BCT/CPL lanes per cell
+    // gather: CPL consecutive transforms of one unit per lane (one
+    // 8*CPL-byte LDS read of the sample's values per entry), TT/CPL lanes per
+    // cell
     constexpr int CPL = BCT >= 2 ? 2 : 1;
-    constexpr int LPC = BCT / CPL;
+    constexpr int LPC = TT / CPL;
     constexpr int NGC = 256 / LPC;
     const int bc = (int)(threadIdx.x & (LPC - 1)) * CPL;
     const int gc = (int)threadIdx.x / LPC;
-    float2* zg = ws + T.zg + bc;
-    float2* pb = parts + T.part + (int64_t)T.pofs * BCT + bc;   // kind 2: this chunk's block
+    const int u1 = SG > 1 ? bc / BCT : 0;
+    const int bl = bc & (BCT - 1);
+    const int lanes = u1 ? T.lanes2 : T.lanes;
+    const bool act = u1 ? act1 : act0;
+    float2* zg = ws + (u1 ? T.zg2 : T.zg) + bl;
+    float2* pb = parts + (u1 ? T.part2 : T.part) + (int64_t)T.pofs * BCT + bl;   // kind 2: this chunk's block
     const float2* svb = sv + bc;
     for (int c = gc; c < cells; c += NGC) {
         float2 acc[CPL];
 #pragma unroll
         for (int k = 0; k < CPL; ++k) acc[k] = make_float2(0.f, 0.f);
-        if (bc < lanes) {
+        if (act && bl < lanes) {
             int e = sp[c];
             const int e1 = sp[c + 1];
             for (; e + 4 <= e1; e += 4) {
@@ -1204,11 +1217,11 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     if constexpr (CPL == 2) {
-                        const float4 u = *reinterpret_cast<const float4*>(svb + q[i].x * BCT);
+                        const float4 u = *reinterpret_cast<const float4*>(svb + q[i].x * TT);
                         v[i][0] = make_float2(u.x, u.y);
                         v[i][CPL - 1] = make_float2(u.z, u.w);
                     } else {
-                        v[i][0] = svb[q[i].x * BCT];
+                        v[i][0] = svb[q[i].x * TT];
                     }
                 }
 #pragma unroll
@@ -1226,7 +1239,7 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
                 const float w = __int_as_float(q.y);
 #pragma unroll
                 for (int k = 0; k < CPL; ++k) {
-                    const float2 v = svb[q.x * BCT + k];
+                    const float2 v = svb[q.x * TT + k];
                     acc[k].x = fmaf(w, v.x, acc[k].x);
                     acc[k].y = fmaf(w, v.y, acc[k].y);
                 }
@@ -1234,8 +1247,9 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
             // a transform past the chunk's last one stays zero
 #pragma unroll
             for (int k = 1; k < CPL; ++k)
-                if (bc + k >= lanes) acc[k] = make_float2(0.f, 0.f);
+                if (bl + k >= lanes) acc[k] = make_float2(0.f, 0.f);
         }
+        if (!act) continue;
         float2* dst;
         if (T.kind == 0) {
             const int r = c / T.w;
@@ -1906,6 +1920,7 @@ bool NufftOp::set_option(const std::string& key, int value) {
     if (key == "nu_degrid_stage") return flag(degrid_stage_, false);
     if (key == "nu_grid_stage") return flag(grid_stage_, false);
     if (key == "nu_xcd_order") return flag(xcd_order_, true);
+    if (key == "nu_grid_pairs") return flag(grid_pairs_, true);
     // caps keep every staged work-group within 64 KB of LDS at 16 transforms
     if (key == "nu_stage_samples" || key == "nu_stage_entries" || key == "nu_degrid_block") {
         int* dst = key == "nu_stage_samples" ? &stage_samples_
@@ -1928,6 +1943,7 @@ bool NufftOp::get_option(const std::string& key, int* value) const {
     else if (key == "nu_degrid_stage") *value = degrid_stage_;
     else if (key == "nu_grid_stage") *value = grid_stage_;
     else if (key == "nu_xcd_order") *value = xcd_order_;
+    else if (key == "nu_grid_pairs") *value = grid_pairs_;
     else if (key == "nu_stage_samples") *value = stage_samples_;
     else if (key == "nu_stage_entries") *value = stage_entries_;
     else if (key == "nu_degrid_block") *value = degrid_block_;
@@ -1966,6 +1982,7 @@ bool NufftOp::autotune_sense(const float2* maps, int C, const float* W, int iter
         }
         for (const auto& kv : cands[bi]) set_option(kv.first, kv.second);
     };
+    tune({{{"nu_grid_pairs", 1}}, {{"nu_grid_pairs", 0}}}, {"nu_grid"});
     tune({{{"nu_stage_samples", 96}, {"nu_stage_entries", 768}},
           {{"nu_stage_samples", 128}, {"nu_stage_entries", 1024}},
           {{"nu_stage_samples", 192}, {"nu_stage_entries", 1536}},
@@ -2005,6 +2022,11 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     std::vector<int4> wdb;
     std::vector<int64_t> wdb_cost;
     std::vector<int64_t> wga_cost, wcsr_cost, wgt_cost;
+    struct GridUnit {
+        int s = 0, lanes = 0;
+        int64_t yt = 0, zg = 0, part = 0;
+    };
+    std::vector<GridUnit> gunits;
     c.lds_gt = 16;
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
@@ -2114,36 +2136,94 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
                     wcsr_cost.push_back(rp[c1] - rp[c0]);
                     c0 = c1;
                 }
-                // staged rectangles and chunks of dense rectangles: one item each
-                for (int t = gtile_first_[s]; t < gtile_first_[s] + gtile_count_[s]; ++t) {
-                    const GridTile& T = gtile_host_[(size_t)t];
-                    GridItem it{};
-                    it.yt = d.yt_off + (int64_t)ch * d.M * BCT;
-                    it.zg = d.ws_off + (int64_t)ch * d.ws_chunk;
-                    it.part = d.part_off + (int64_t)ch * d.part_chunk;
-                    it.slice = s;
-                    it.G1 = d.G1;
-                    it.kind = T.kind;
-                    it.lanes = sense ? C : (int)std::min<int64_t>(BCT, nrep - (int64_t)ch * BCT);
-                    it.g1 = T.g1;
-                    it.g2 = T.g2;
-                    it.w = T.w;
-                    it.h = T.h;
-                    it.stage_off = T.stage_off;
-                    it.n_stage = T.n_stage;
-                    it.ptr_off = T.ptr_off;
-                    it.ent_off = T.ent_off;
-                    it.n_ent = T.n_ent;
-                    it.pofs = T.pofs;
-                    it.pofs0 = T.pofs0;
-                    it.nparts = T.nparts;
-                    wgt.push_back(it);
-                    wgt_cost.push_back((int64_t)T.n_ent + 2 * (int64_t)T.n_stage);
-                    if (T.kind == 2 && T.pofs == T.pofs0) wgm.push_back(it);
-                    c.lds_gt = std::max(c.lds_gt, (size_t)T.n_stage * BCT * sizeof(float2) +
-                                                      (size_t)T.n_ent * sizeof(int2) +
-                                                      (size_t)(T.w * T.h + 1) * sizeof(int));
+                // staged rectangles: the (slice, chunk) unit; items are made
+                // below, after the units of one trajectory have been paired
+                GridUnit u;
+                u.s = s;
+                u.yt = d.yt_off + (int64_t)ch * d.M * BCT;
+                u.zg = d.ws_off + (int64_t)ch * d.ws_chunk;
+                u.part = d.part_off + (int64_t)ch * d.part_chunk;
+                u.lanes = sense ? C : (int)std::min<int64_t>(BCT, nrep - (int64_t)ch * BCT);
+                gunits.push_back(u);
+            }
+        }
+    }
+    if (csr_) {
+        // Units of one trajectory are paired in slice order (option
+        // "nu_grid_pairs"): an item stages its rectangle's entries once for
+        // both.  Not when the staged samples of two units would need more
+        // than 96 KB of LDS.
+        size_t lds1 = 16, lds2 = 16;
+        for (const GridTile& T : gtile_host_) {
+            const size_t rest = (size_t)T.n_ent * sizeof(int2) + (size_t)(T.w * T.h + 1) * sizeof(int);
+            lds1 = std::max(lds1, (size_t)T.n_stage * BCT * sizeof(float2) + rest);
+            lds2 = std::max(lds2, (size_t)T.n_stage * 2 * BCT * sizeof(float2) + rest);
+        }
+        std::vector<std::pair<int, int>> groups;
+        if (grid_pairs_ && lds2 <= 96 * 1024) {
+            std::map<int, int> open;   // first rectangle of a trajectory -> unpaired unit
+            for (int k = 0; k < (int)gunits.size(); ++k) {
+                const int key = gtile_first_[(size_t)gunits[(size_t)k].s];
+                auto o = open.find(key);
+                if (o == open.end()) {
+                    open.emplace(key, k);
+                } else {
+                    groups.emplace_back(o->second, k);
+                    open.erase(o);
                 }
+            }
+            for (const auto& o : open) groups.emplace_back(o.second, -1);
+            std::sort(groups.begin(), groups.end());
+        } else {
+            for (int k = 0; k < (int)gunits.size(); ++k) groups.emplace_back(k, -1);
+        }
+        c.gt_pairs = 0;
+        for (const auto& g : groups) c.gt_pairs |= g.second >= 0;
+        c.lds_gt = c.gt_pairs ? lds2 : lds1;
+        for (const auto& g : groups) {
+            const GridUnit& u = gunits[(size_t)g.first];
+            const int s = u.s;
+            for (int t = gtile_first_[s]; t < gtile_first_[s] + gtile_count_[s]; ++t) {
+                const GridTile& T = gtile_host_[(size_t)t];
+                GridItem it{};
+                it.yt = u.yt;
+                it.zg = u.zg;
+                it.part = u.part;
+                it.slice = s;
+                it.G1 = c.hs[s].G1;
+                it.kind = T.kind;
+                it.lanes = u.lanes;
+                it.slice2 = -1;
+                it.g1 = T.g1;
+                it.g2 = T.g2;
+                it.w = T.w;
+                it.h = T.h;
+                it.stage_off = T.stage_off;
+                it.n_stage = T.n_stage;
+                it.ptr_off = T.ptr_off;
+                it.ent_off = T.ent_off;
+                it.n_ent = T.n_ent;
+                it.pofs = T.pofs;
+                it.pofs0 = T.pofs0;
+                it.nparts = T.nparts;
+                if (T.kind == 2 && T.pofs == T.pofs0) wgm.push_back(it);
+                if (g.second >= 0) {
+                    const GridUnit& v = gunits[(size_t)g.second];
+                    GridItem m2 = it;   // the merge of the second unit's dense rectangle
+                    m2.yt = v.yt;
+                    m2.zg = v.zg;
+                    m2.part = v.part;
+                    m2.slice = v.s;
+                    m2.lanes = v.lanes;
+                    if (T.kind == 2 && T.pofs == T.pofs0) wgm.push_back(m2);
+                    it.yt2 = v.yt;
+                    it.zg2 = v.zg;
+                    it.part2 = v.part;
+                    it.slice2 = v.s;
+                    it.lanes2 = v.lanes;
+                }
+                wgt.push_back(it);
+                wgt_cost.push_back(((int64_t)T.n_ent + 2 * (int64_t)T.n_stage) * (g.second >= 0 ? 2 : 1));
             }
         }
     }
@@ -2333,9 +2413,12 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     };
     auto adjoint_from = [&](const float2* y, float2* x, NuRun r) {
         if (csr_ && grid_stage_) {
-            if (c.n_gt > 0)
-                hipLaunchKernelGGL(nu_grid_tile<BCT>, dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p, gstage_.p,
-                                   gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
+            if (c.n_gt > 0 && c.gt_pairs)
+                hipLaunchKernelGGL((nu_grid_tile<BCT, 2>), dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p,
+                                   gstage_.p, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
+            else if (c.n_gt > 0)
+                hipLaunchKernelGGL((nu_grid_tile<BCT, 1>), dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p,
+                                   gstage_.p, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
             if (c.n_gm > 0) {
                 LAUNCHCHECK();
                 hipLaunchKernelGGL(nu_grid_merge<BCT>, dim3(c.n_gm), dim3(256), 0, st, c.w_gm.p, c.parts.p,

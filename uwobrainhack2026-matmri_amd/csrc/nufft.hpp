@@ -84,10 +84,14 @@ struct GridItem {
     int64_t yt;          // transposed data of the (slice, chunk) in yT
     int64_t zg;          // grid of the (slice, chunk) in the workspace
     int64_t part;        // partial blocks of the (slice, chunk) (kind 2)
+    // second (slice, chunk) of the same trajectory handled by the item
+    // (slice2 < 0: none); its transforms follow the first's in the lanes
+    int64_t yt2, zg2, part2;
     int slice, G1, kind, lanes;
     int g1, g2, w, h;
     int stage_off, n_stage, ptr_off, ent_off;
     int n_ent, pofs, pofs0, nparts;
+    int slice2, lanes2;
 };
 constexpr int kDegridTile = 8;      // tile side of the staged degridding
 
@@ -164,6 +168,7 @@ class NufftOp : public mm_op {
         DevBuf<GridItem> w_gt, w_gm;   // staged items; merges of dense rectangles
         int n_gt = 0, n_gm = 0;
         size_t lds_gt = 0;
+        int gt_pairs = 0;              // items carry two (slice, chunk) units
     };
     Cfg& config(int64_t nrep, int sense, int C);
     // mode 0: the requested direction with the standard data layout;
@@ -219,6 +224,10 @@ class NufftOp : public mm_op {
     // its rectangles
     bool grid_stage_ = true;
     int stage_samples_ = 128, stage_entries_ = 1024;   // staged gridding rectangle caps
+    // one staged gridding item serves two (slice, chunk) units of one
+    // trajectory, so its rectangle's CSR entries are read once for both
+    // (option "nu_grid_pairs")
+    bool grid_pairs_ = true;
     int degrid_block_ = 64;                             // samples per staged degridding block
     void build_partitions();
     // host copies the partitions are rebuilt from

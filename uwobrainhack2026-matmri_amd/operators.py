@@ -103,7 +103,8 @@ class LinearOp:
         check(_lib.lib().mm_op_get_option(self._plan.h, key.encode(), C.byref(v)))
         return v.value
 
-    TUNED_OPTIONS = ("nu_stage_samples", "nu_stage_entries", "nu_degrid_block", "nu_pair_cols", "nu_xcd_order")
+    TUNED_OPTIONS = ("nu_grid_pairs", "nu_stage_samples", "nu_stage_entries", "nu_degrid_block", "nu_pair_cols",
+                     "nu_xcd_order")
 
     def autotune(self, iters=10):
         """Run the device autotuner of the normal operator (mm_op_autotune)
