@@ -1921,6 +1921,7 @@ bool NufftOp::set_option(const std::string& key, int value) {
     if (key == "nu_grid_stage") return flag(grid_stage_, false);
     if (key == "nu_xcd_order") return flag(xcd_order_, true);
     if (key == "nu_grid_pairs") return flag(grid_pairs_, true);
+    if (key == "nu_snake") return flag(snake_, true);
     // caps keep every staged work-group within 64 KB of LDS at 16 transforms
     if (key == "nu_stage_samples" || key == "nu_stage_entries" || key == "nu_degrid_block") {
         int* dst = key == "nu_stage_samples" ? &stage_samples_
@@ -1944,6 +1945,7 @@ bool NufftOp::get_option(const std::string& key, int* value) const {
     else if (key == "nu_grid_stage") *value = grid_stage_;
     else if (key == "nu_xcd_order") *value = xcd_order_;
     else if (key == "nu_grid_pairs") *value = grid_pairs_;
+    else if (key == "nu_snake") *value = snake_;
     else if (key == "nu_stage_samples") *value = stage_samples_;
     else if (key == "nu_stage_entries") *value = stage_entries_;
     else if (key == "nu_degrid_block") *value = degrid_block_;
@@ -1991,6 +1993,8 @@ bool NufftOp::autotune_sense(const float2* maps, int C, const float* W, int iter
     tune({{{"nu_degrid_block", 32}}, {{"nu_degrid_block", 64}}, {{"nu_degrid_block", 128}}}, {"nu_degrid"});
     tune({{{"nu_pair_cols", 1}}, {{"nu_pair_cols", 0}}}, {"nu_cols"});
     tune({{{"nu_xcd_order", 0}}, {{"nu_xcd_order", 1}}}, {"nu_grid", "nu_degrid"});
+    tune({{{"nu_snake", 1}}, {{"nu_snake", 0}}},
+         {"nu_fwd_rows", "nu_cols", "nu_degrid", "nu_grid", "nu_adj_rows"});
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     return true;
 }
@@ -2266,10 +2270,16 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     // (option "nu_xcd_order") the items of (slice, chunk) q go to the
     // positions = q (mod kXcds), heaviest first within each XCD, so a slice's
     // data stays in one XCD's L2.
-    auto order = [&](auto& items, const std::vector<int64_t>& cost, auto key) {
+    // With the alternating slice order (option "nu_snake") the items are
+    // sorted by slice first, ascending (dir > 0) or descending (dir < 0).
+    auto order = [&](auto& items, const std::vector<int64_t>& cost, auto key, auto slice_of, int dir) {
         std::vector<size_t> ord(items.size());
         std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+            if (dir != 0 && slice_of(items[a]) != slice_of(items[b]))
+                return dir > 0 ? slice_of(items[a]) < slice_of(items[b]) : slice_of(items[a]) > slice_of(items[b]);
+            return cost[a] > cost[b];
+        });
         std::vector<size_t> pick;
         if (xcd_order_) {
             std::vector<std::vector<size_t>> q(kXcds);
@@ -2290,8 +2300,27 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         for (size_t i = 0; i < pick.size(); ++i) sorted[i] = items[pick[i]];
         items.swap(sorted);
     };
-    order(wdb, wdb_cost, [&](const int4& w) { return (int64_t)w.x * 131 + w.y; });
-    order(wgt, wgt_cost, [&](const GridItem& g) { return (int64_t)g.slice * 131 + (g.zg - c.hs[g.slice].ws_off) / std::max<int64_t>(1, c.hs[g.slice].ws_chunk); });
+    // Alternating slice order: each stage of the normal operator walks the
+    // slices in the opposite direction to the stage before it (forward rows
+    // ascending, forward columns descending, degridding ascending, gridding
+    // descending, adjoint columns ascending, adjoint rows descending), so it
+    // starts on the slices whose data the previous stage touched last and
+    // which are still in the memory-side cache.
+    const int sd = snake_ ? 1 : 0;
+    auto slice4 = [](const int4& w) { return w.x; };
+    order(wdb, wdb_cost, [&](const int4& w) { return (int64_t)w.x * 131 + w.y; }, slice4, sd);
+    order(wgt, wgt_cost, [&](const GridItem& g) { return (int64_t)g.slice * 131 + (g.zg - c.hs[g.slice].ws_off) / std::max<int64_t>(1, c.hs[g.slice].ws_chunk); },
+          [](const GridItem& g) { return g.slice; }, -sd);
+    auto by_slice_desc = [&](auto& items, auto slice_of) {
+        std::stable_sort(items.begin(), items.end(),
+                         [&](const auto& a, const auto& b) { return slice_of(a) > slice_of(b); });
+    };
+    std::vector<int4> wcf = wc;   // forward column items
+    if (snake_) {
+        by_slice_desc(wr, slice4);
+        by_slice_desc(wcf, slice4);
+        by_slice_desc(wgm, [](const GridItem& g) { return g.slice; });
+    }
     // heaviest gridding items first
     {
         std::vector<size_t> ord(wg.size());
@@ -2311,20 +2340,25 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         // 128 bytes): place items 2k and 2k + 1 of every 16 at positions k and
         // k + 8, which the dispatcher sends to the same XCD at about the same
         // time, so each line is fetched into one L2
-        std::vector<int4> r(wc.size());
-        for (size_t j = 0; j < wc.size(); ++j) {
-            const size_t q = j / 16, rr = j % 16;
-            size_t pos = q * 16 + rr / 2 + 8 * (rr % 2);
-            if (pos >= wc.size()) pos = j;   // incomplete last group keeps its order
-            r[pos] = wc[j];
-        }
-        if (wc.size() % 16) {
-            const size_t q0 = wc.size() / 16 * 16;
-            for (size_t j = q0; j < wc.size(); ++j) r[j] = wc[j];
-        }
-        wc.swap(r);
+        auto pair = [](std::vector<int4>& v) {
+            std::vector<int4> r(v.size());
+            for (size_t j = 0; j < v.size(); ++j) {
+                const size_t q = j / 16, rr = j % 16;
+                size_t pos = q * 16 + rr / 2 + 8 * (rr % 2);
+                if (pos >= v.size()) pos = j;   // incomplete last group keeps its order
+                r[pos] = v[j];
+            }
+            if (v.size() % 16) {
+                const size_t q0 = v.size() / 16 * 16;
+                for (size_t j = q0; j < v.size(); ++j) r[j] = v[j];
+            }
+            v.swap(r);
+        };
+        pair(wc);
+        pair(wcf);
     }
     c.w_cols.upload(wc, st);
+    c.w_fcols.upload(wcf, st);
     c.w_deg.upload(wd, st);
     c.w_grid.upload(wg.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wg, st);
     c.w_csr.upload(wcsr.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wcsr, st);
@@ -2381,7 +2415,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_fwd_rows");
         if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
+            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_fcols.p,
                                facs_.p, twid_.p, c.ws.p, r.skip);
             LAUNCHCHECK();
             mark("nu_cols");

@@ -153,6 +153,7 @@ class NufftOp : public mm_op {
         std::vector<NuSliceDev> hs;
         DevBuf<NuSliceDev> sl;
         DevBuf<int4> w_rows, w_frows, w_cols, w_deg, w_grid;
+        DevBuf<int4> w_fcols;    // forward column items (w_cols: adjoint)
         int n_rows = 0, n_frows = 0, n_cols = 0, n_deg = 0, n_grid = 0;
         size_t lds_rows = 0, lds_cols = 0;
         DevBuf<float2> ws, parts, yT, ydata;
@@ -228,6 +229,10 @@ class NufftOp : public mm_op {
     // trajectory, so its rectangle's CSR entries are read once for both
     // (option "nu_grid_pairs")
     bool grid_pairs_ = true;
+    // alternating slice order of the normal operator's stages (option
+    // "nu_snake"): each stage starts on the slices the previous one touched
+    // last
+    bool snake_ = true;
     int degrid_block_ = 64;                             // samples per staged degridding block
     void build_partitions();
     // host copies the partitions are rebuilt from
