@@ -1134,15 +1134,27 @@ __global__ __launch_bounds__(256) void nu_grid_csr(const NuSliceDev* __restrict_
 // given plan).  A unit whose slice has finished is neither read nor written.
 template <int BCT, int SG>
 __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__ work,
-                                                    const int* __restrict__ stage,
+                                                    const int* __restrict__ isid, int sid_stride,
                                                     const int* __restrict__ tptr,
                                                     const int2* __restrict__ tent,
                                                     const float2* __restrict__ yt, float2* __restrict__ ws,
                                                     float2* __restrict__ parts, const int* __restrict__ skip) {
     constexpr int TT = BCT * SG;   // transforms per staged sample
     static_assert(256 % TT == 0, "transforms per sample must divide the work-group");
+    constexpr int SPR = 256 / TT;  // staged samples per pass of the work-group
+    constexpr int NPRE = 8;        // sample ids each lane loads up front
     extern __shared__ float2 lds[];
-    // the item carries every offset: one scalar load before the data loads
+    // element i of the staged samples is transform (i mod TT) of sample
+    // i / TT; 256 is a multiple of TT, so a lane's transform (and unit) is
+    // fixed and its samples are q0, q0 + SPR, ...
+    const int jb = (int)threadIdx.x & (TT - 1);
+    const int q0 = (int)threadIdx.x / TT;
+    // the item's sample ids are addressed by the block index alone: they are
+    // loaded together with the item record, not after it
+    const int* sid = isid + (int64_t)blockIdx.x * sid_stride;
+    int m[NPRE];
+#pragma unroll
+    for (int k = 0; k < NPRE; ++k) m[k] = sid[min(q0 + k * SPR, sid_stride - 1)];   // unconditional loads
     const GridItem T = work[blockIdx.x];
     const bool act0 = !(skip && skip[T.slice]);
     const bool act1 = SG > 1 && T.slice2 >= 0 && !(skip && skip[T.slice2]);
@@ -1152,37 +1164,31 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
     float2* sv = lds;                              // [n_stage][TT]
     int2* se = (int2*)(lds + nsv);                 // [n_ent]
     int* sp = (int*)(se + T.n_ent);                // [cells + 1]
-    const int* sid = stage + T.stage_off;
     {
-        // element i of sv is transform (i mod TT) of sample i / TT; 256 is a
-        // multiple of TT, so the transform (and its unit) is fixed per lane
-        const int jb = (int)threadIdx.x & (TT - 1);
         const int u1 = SG > 1 ? jb / BCT : 0;
-        const bool ld = u1 ? act1 : act0;
         const float2* ych = yt + (u1 ? T.yt2 : T.yt) + (jb & (BCT - 1));
-        for (int i0 = threadIdx.x; i0 < nsv; i0 += 4 * 256) {
-            int m[4];
+        // every lane loads (a valid sample: the row's padding ids are 0), so
+        // no load waits on a branch; values of a finished unit are staged but
+        // never read (its lanes do not gather)
+        float2 v[NPRE];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int i = i0 + u * 256;
-                m[u] = i < nsv ? sid[i / TT] : 0;
-            }
-            float2 v[4];
+        for (int k = 0; k < NPRE; ++k) v[k] = ych[(int64_t)m[k] * BCT];
+        // the first 1024 entries are loaded before any LDS store
+        const int2* te = tent + T.ent_off;
+        int2 e[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = ld ? ych[(int64_t)m[u] * BCT] : make_float2(0.f, 0.f);
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (i0 + u * 256 < nsv) sv[i0 + u * 256] = v[u];
+        for (int u = 0; u < 4; ++u) {
+            const int i = threadIdx.x + u * 256;
+            e[u] = i < T.n_ent ? te[i] : make_int2(0, 0);
         }
-    }
-    const int2* te = tent + T.ent_off;
-    for (int i0 = threadIdx.x; i0 < T.n_ent; i0 += 4 * 256) {
-        int2 q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = i0 + u * 256 < T.n_ent ? te[i0 + u * 256] : make_int2(0, 0);
+        for (int k = 0; k < NPRE; ++k)
+            if (q0 + k * SPR < T.n_stage) sv[(q0 + k * SPR) * TT + jb] = v[k];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (i0 + u * 256 < T.n_ent) se[i0 + u * 256] = q[u];
+            if (threadIdx.x + u * 256 < T.n_ent) se[threadIdx.x + u * 256] = e[u];
+        for (int q = q0 + NPRE * SPR; q < T.n_stage; q += SPR) sv[q * TT + jb] = ych[(int64_t)sid[q] * BCT];
+        for (int i = threadIdx.x + 4 * 256; i < T.n_ent; i += 256) se[i] = te[i];
     }
     const int* tp = tptr + T.ptr_off;
     for (int i = threadIdx.x; i <= cells; i += 256) sp[i] = tp[i];
@@ -1898,7 +1904,7 @@ void NufftOp::build_partitions() {
         if (dwt_h.size() > (size_t)INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples");
     }
     hipStream_t st = ctx->stream;
-    gstage_.upload(gstage_h.empty() ? std::vector<int>{0} : gstage_h, st);
+    gstage_host_.swap(gstage_h);
     gptr_.upload(gptr_h.empty() ? std::vector<int>{0} : gptr_h, st);
     gent_.upload(gent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : gent_h, st);
     dblk_.upload(dblk_host_.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : dblk_host_, st);
@@ -2363,6 +2369,17 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.w_grid.upload(wg.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wg, st);
     c.w_csr.upload(wcsr.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wcsr, st);
     c.n_csr = (int)wcsr.size();
+    {
+        int mx = 1;
+        for (const GridItem& g : wgt) mx = std::max(mx, g.n_stage);
+        c.gt_sid_stride = (mx + 15) & ~15;
+        std::vector<int> rows(std::max<size_t>(1, wgt.size() * (size_t)c.gt_sid_stride), 0);
+        for (size_t i = 0; i < wgt.size(); ++i)
+            std::copy(gstage_host_.begin() + wgt[i].stage_off,
+                      gstage_host_.begin() + wgt[i].stage_off + wgt[i].n_stage,
+                      rows.begin() + (int64_t)i * c.gt_sid_stride);
+        c.gt_sid.upload(rows, st);
+    }
     c.w_gt.upload(wgt.empty() ? std::vector<GridItem>(1) : wgt, st);
     c.n_gt = (int)wgt.size();
     c.w_gm.upload(wgm.empty() ? std::vector<GridItem>(1) : wgm, st);
@@ -2449,10 +2466,10 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         if (csr_ && grid_stage_) {
             if (c.n_gt > 0 && c.gt_pairs)
                 hipLaunchKernelGGL((nu_grid_tile<BCT, 2>), dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p,
-                                   gstage_.p, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
+                                   c.gt_sid.p, c.gt_sid_stride, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
             else if (c.n_gt > 0)
                 hipLaunchKernelGGL((nu_grid_tile<BCT, 1>), dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p,
-                                   gstage_.p, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
+                                   c.gt_sid.p, c.gt_sid_stride, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
             if (c.n_gm > 0) {
                 LAUNCHCHECK();
                 hipLaunchKernelGGL(nu_grid_merge<BCT>, dim3(c.n_gm), dim3(256), 0, st, c.w_gm.p, c.parts.p,

@@ -170,6 +170,10 @@ class NufftOp : public mm_op {
         int n_gt = 0, n_gm = 0;
         size_t lds_gt = 0;
         int gt_pairs = 0;              // items carry two (slice, chunk) units
+        // staged sample ids of item i at gt_sid[i * gt_sid_stride]: read
+        // from the block index alone, before the item record arrives
+        DevBuf<int> gt_sid;
+        int gt_sid_stride = 16;
     };
     Cfg& config(int64_t nrep, int sense, int C);
     // mode 0: the requested direction with the standard data layout;
@@ -253,7 +257,8 @@ class NufftOp : public mm_op {
     std::vector<GridTile> gtile_host_;
     std::vector<int> gtile_first_, gtile_count_;   // per slice
     std::vector<int> gpcells_;                     // per slice: cells of the partial blocks
-    DevBuf<int> gstage_, gptr_;
+    DevBuf<int> gptr_;
+    std::vector<int> gstage_host_;   // staged sample ids of the rectangles (host)
     DevBuf<int2> gent_;
     // gather gridding (2-D, every grid side >= 24)
     bool gather_ = false;
