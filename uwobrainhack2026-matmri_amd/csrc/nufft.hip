@@ -595,27 +595,57 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
     uint32_t* s_base = (uint32_t*)(s_id + bcap);
     float* s_dcf = (float*)(s_base + bcap);
     float* s_w = s_dcf + bcap;
-    for (int i = threadIdx.x; i < n; i += kFftThreads) {
-        s_id[i] = bid[blk.y + i];
-        s_base[i] = bbase[blk.y + i];
-        s_dcf[i] = bdcf[blk.y + i];
-    }
-    for (int i = threadIdx.x; i < n * WS; i += kFftThreads) s_w[i] = bwt[(int64_t)blk.y * WS + i];
-    for (int i0 = threadIdx.x; i0 < nst; i0 += 4 * 256) {
+    // every staging load of a pass is issued before its LDS stores, with no
+    // branch between them: the block's first taps, dcf and ids (n <= 256),
+    // its weights (four per lane per pass) and the grid region (four per
+    // lane per pass; wrapped cells by a select unless the region is wider
+    // than the grid, a uniform case)
+    const bool wide = RW > G1 || RW > G2;
+    auto cell = [&](int i) {
+        const int cl = i / BCT;
+        const int l2 = cl / RW, l1 = cl - l2 * RW;
+        int g1 = t1 + l1, g2 = t2 + l2;
+        if (wide) {
+            g1 %= G1;
+            g2 %= G2;
+        } else {
+            g1 = g1 >= G1 ? g1 - G1 : g1;
+            g2 = g2 >= G2 ? g2 - G2 : g2;
+        }
+        return (g2 * G1 + g1) * BCT + (i & (BCT - 1));
+    };
+    {
+        const int i = min((int)threadIdx.x, n - 1);
+        const int pid = bid[blk.y + i];
+        const uint32_t pb = bbase[blk.y + i];
+        const float pd = bdcf[blk.y + i];
+        const int nw = n * WS;
+        const float* wsrc = bwt + (int64_t)blk.y * WS;
+        float w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[u] = wsrc[min((int)threadIdx.x + u * 256, nw - 1)];
         float2 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = min(i0 + u * 256, nst - 1);
-            const int cl = i / BCT;
-            const int l2 = cl / RW, l1 = cl - l2 * RW;
-            int g1 = t1 + l1, g2 = t2 + l2;
-            while (g1 >= G1) g1 -= G1;
-            while (g2 >= G2) g2 -= G2;
-            v[u] = zg[(g2 * G1 + g1) * BCT + (i & (BCT - 1))];
+        for (int u = 0; u < 4; ++u) v[u] = zg[cell(min((int)threadIdx.x + u * 256, nst - 1))];
+        if ((int)threadIdx.x < n) {
+            s_id[threadIdx.x] = pid;
+            s_base[threadIdx.x] = pb;
+            s_dcf[threadIdx.x] = pd;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (i0 + u * 256 < nst) lds[i0 + u * 256] = v[u];
+            if ((int)threadIdx.x + u * 256 < nw) s_w[threadIdx.x + u * 256] = w[u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if ((int)threadIdx.x + u * 256 < nst) lds[threadIdx.x + u * 256] = v[u];
+        for (int i0 = threadIdx.x + 4 * 256; i0 < nw; i0 += 256) s_w[i0] = wsrc[i0];
+        for (int i0 = threadIdx.x + 4 * 256; i0 < nst; i0 += 4 * 256) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = zg[cell(min(i0 + u * 256, nst - 1))];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (i0 + u * 256 < nst) lds[i0 + u * 256] = v[u];
+        }
     }
     __syncthreads();
     for (int k = (int)(threadIdx.x / BCT); k < n; k += SPP) {
