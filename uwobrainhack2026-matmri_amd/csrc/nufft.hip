@@ -329,7 +329,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     float2* A = lds;
     float2* Bb = ip ? nullptr : lds + G1 * T;
     float2* tw = lds + (ip ? 1 : 2) * G1 * T;
-    for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     for (int i = (N1 - N1 / 2) * T + threadIdx.x; i < (G1 - N1 / 2) * T; i += kFftThreads)
         A[i] = make_float2(0.f, 0.f);
     const int lanes = lanes_of(run, BCT, ch);
@@ -339,34 +338,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     const float2* mp = run.maps + S.maps_off;
     const int total = N1 * T;
     // four elements per thread in flight: every load of a round is issued
-    // before the first LDS store
-    for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
-        float2 val[4];
+    // before the first LDS store.  The loads use clamped (valid) addresses
+    // for every element and elements outside the image are scaled by 0, so
+    // no load sits behind a branch (a branch would make it wait).
+    auto stage = [&](auto value) {
+        // full rounds (every element in range) store without a predicate, so
+        // no load can be sunk behind one; then at most one partial round
+        auto round = [&](int i0, bool full) {
+            float2 val[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int idx = i0 + u * 256;
-            const int t = idx & (T - 1);
-            const int i1 = idx >> LT;
-            const int bl = t & (BCT - 1);
-            const int i2 = r0 + (t >> LB);
-            val[u] = make_float2(0.f, 0.f);
-            if (idx < total && i2 < N2 && bl < lanes) {
-                const int v = i2 * N1 + i1;
-                const float2 q = run.sense ? cmulf(xs[v], mp[bl * V + v]) : xs[bl * V + v];
-                const float f = d1[i1] * d2[i2];
+            for (int u = 0; u < 4; ++u) {
+                const int idx = i0 + u * 256;
+                const int t = idx & (T - 1);
+                const int i1 = min(idx >> LT, N1 - 1);
+                const int bl = t & (BCT - 1);
+                const int i2 = r0 + (t >> LB);
+                const float ok = idx < total && i2 < N2 && bl < lanes ? 1.f : 0.f;
+                const int i2c = min(i2, N2 - 1);
+                const float2 q = value(i2c * N1 + i1, min(bl, lanes - 1));
+                const float f = d1[i1] * d2[i2c] * ok;
                 val[u] = make_float2(q.x * f, q.y * f);
             }
-        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int idx = i0 + u * 256;
-            if (idx >= total) break;
-            const int i1 = idx >> LT;
-            int pos = i1 - N1 / 2;
-            if (pos < 0) pos += G1;
-            A[pos * T + (idx & (T - 1))] = val[u];
-        }
-    }
+            for (int u = 0; u < 4; ++u) {
+                const int idx = i0 + u * 256;
+                const int i1 = idx >> LT;
+                int pos = i1 - N1 / 2;
+                if (pos < 0) pos += G1;
+                if (full || idx < total) A[pos * T + (idx & (T - 1))] = val[u];
+            }
+        };
+        int i0 = threadIdx.x;
+        for (; i0 + 3 * 256 < total; i0 += 4 * 256) round(i0, true);
+        if (i0 < total) round(i0, false);
+    };
+    if (run.sense)
+        stage([&](int v, int b) { return cmulf(xs[v], mp[b * V + v]); });
+    else
+        stage([&](int v, int b) { return xs[b * V + v]; });
+    // the pass twiddles after the data loads, which they then do not delay
+    for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     __syncthreads();
     const float2* R = fft_lds<T, 1, BIG>(A, Bb, facs[S.fac1], tw);
     float2* out = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
@@ -401,48 +412,47 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     float2* Bb = ip ? nullptr : lds + G2 * T;
     float2* tw = lds + (ip ? 1 : 2) * G2 * T;
     float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b0;
-    for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
     const int ncol = min(CPW, G1 - c0);
+    // four loads per thread in flight, from clamped (valid) addresses; a
+    // column past the grid's last is scaled by 0, and full rounds store
+    // without a predicate, so no load sits behind a branch
+    auto stage = [&](int rows, auto dst_of) {
+        const int total = rows * T;
+        auto round = [&](int i0, bool full) {
+            float2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = i0 + u * 256;
+                const int t = idx & (T - 1);
+                const int col = t >> LCL;
+                const float ok = col < ncol ? 1.f : 0.f;
+                const int r = min(idx >> LT, rows - 1);
+                const float2 q = g[((int64_t)r * G1 + c0 + min(col, ncol - 1)) * BCT + (t & (CL - 1))];
+                v[u] = make_float2(q.x * ok, q.y * ok);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int idx = i0 + u * 256;
+                if (full || idx < total) A[dst_of(idx)] = v[u];
+            }
+        };
+        int i0 = threadIdx.x;
+        for (; i0 + 3 * 256 < total; i0 += 4 * 256) round(i0, true);
+        if (i0 < total) round(i0, false);
+    };
     if (!ADJ) {
         for (int i = (N2 - N2 / 2) * T + threadIdx.x; i < (G2 - N2 / 2) * T; i += kFftThreads)
             A[i] = make_float2(0.f, 0.f);
-        const int total = N2 * T;
-        for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
-            float2 v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = i0 + u * 256;
-                const int t = idx & (T - 1);
-                const int col = t >> LCL;
-                v[u] = idx < total && col < ncol ? g[((idx >> LT) * G1 + c0 + col) * BCT + (t & (CL - 1))]
-                                                 : make_float2(0.f, 0.f);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = i0 + u * 256;
-                if (idx >= total) break;
-                int pos = (idx >> LT) - N2 / 2;
-                if (pos < 0) pos += G2;
-                A[pos * T + (idx & (T - 1))] = v[u];
-            }
-        }
+        stage(N2, [&](int idx) {
+            int pos = (idx >> LT) - N2 / 2;
+            if (pos < 0) pos += G2;
+            return pos * T + (idx & (T - 1));
+        });
     } else {
-        const int total = G2 * T;
-        for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
-            float2 v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = i0 + u * 256;
-                const int t = idx & (T - 1);
-                const int col = t >> LCL;
-                v[u] = idx < total && col < ncol ? g[((idx >> LT) * G1 + c0 + col) * BCT + (t & (CL - 1))]
-                                                 : make_float2(0.f, 0.f);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (i0 + u * 256 < total) A[i0 + u * 256] = v[u];
-        }
+        stage(G2, [](int idx) { return idx; });
     }
+    // the pass twiddles after the data loads, which they then do not delay
+    for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
     __syncthreads();
     const float2* R = fft_lds<T, ADJ ? -1 : 1, BIG>(A, Bb, facs[S.fac2], tw);
     if (!ADJ) {
@@ -1387,23 +1397,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     float2* A = lds;
     float2* Bb = ip ? nullptr : lds + G1 * T;
     float2* tw = lds + (ip ? 1 : 2) * G1 * T;
-    for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     const float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
     const int total = G1 * T;
-    for (int i0 = threadIdx.x; i0 < total; i0 += 4 * 256) {
+    // four loads per thread in flight, from clamped (valid) addresses; rows
+    // past the image are scaled by 0, and full rounds store without a
+    // predicate, so no load sits behind a branch
+    auto round = [&](int i0, bool full) {
         float2 v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int idx = i0 + u * 256;
             const int t = idx & (T - 1);
             const int i2 = r0 + (t >> LB);
-            v[u] = idx < total && i2 < N2 ? g[(i2 * G1 + (idx >> LT)) * BCT + (t & (BCT - 1))]
-                                          : make_float2(0.f, 0.f);
+            const float ok = i2 < N2 ? 1.f : 0.f;
+            const float2 q = g[(min(i2, N2 - 1) * G1 + min(idx >> LT, G1 - 1)) * BCT + (t & (BCT - 1))];
+            v[u] = make_float2(q.x * ok, q.y * ok);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (i0 + u * 256 < total) A[i0 + u * 256] = v[u];
+            if (full || i0 + u * 256 < total) A[i0 + u * 256] = v[u];
+    };
+    {
+        int i0 = threadIdx.x;
+        for (; i0 + 3 * 256 < total; i0 += 4 * 256) round(i0, true);
+        if (i0 < total) round(i0, false);
     }
+    // the pass twiddles after the data loads, which they then do not delay
+    for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     __syncthreads();
     const float2* R = fft_lds<T, -1, BIG>(A, Bb, facs[S.fac1], tw);
     const int lanes = lanes_of(run, BCT, ch);
