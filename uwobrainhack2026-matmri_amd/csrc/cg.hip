@@ -32,10 +32,11 @@ struct CgSeg {
 };
 
 constexpr int kCgBlock = 256;
-// threads of the per-slice scalar kernels (cg_alpha, cg_beta, cg_init,
-// zero_scal_k); a compile-time constant, as blockDim.x would be a memory load
+// threads of the per-slice scalar kernels: one wave per slice in cg_alpha,
+// cg_beta and cg_init (a lane per partial-sum block), one slice per lane in
+// zero_scal_k; a compile-time constant, as blockDim.x would be a memory load
 constexpr int kCgScalBlock = 64;
-constexpr int kCgMaxBlocksPerSlice = 64;
+constexpr int kCgMaxBlocksPerSlice = kCgScalBlock;
 
 __device__ __forceinline__ double2 warp_sum2(double2 v) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -75,6 +76,15 @@ __device__ __forceinline__ void acc_cdot(double2& s, float2 a, float2 b) {
     s.y += (double)a.x * b.y - (double)a.y * b.x;
 }
 
+// Sum over the slice's partial-sum blocks: one wave per slice, one block
+// per lane (at most kCgMaxBlocksPerSlice = 64), an xor butterfly in a fixed
+// order, so every lane holds the same deterministic total.  The loads are
+// issued together (a loop over the partials would wait on each).
+__device__ __forceinline__ double wave_total(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // block -> (slice, index within slice) via the segment table
 __device__ __forceinline__ int find_slice(const CgSeg* seg, const int* blk2slice) {
     return blk2slice[blockIdx.x];
@@ -104,24 +114,25 @@ __global__ __launch_bounds__(kCgBlock) void cg_dot_pq(const CgSeg* __restrict__ 
     }
 }
 
-// alpha = rho / (p'Ap)  (cgne.m:228); keeps |p|^2 for |x - xold|^2
+// alpha = rho / (p'Ap)  (cgne.m:228); keeps |p|^2 for |x - xold|^2.  One
+// wave per slice.
 __global__ __launch_bounds__(kCgScalBlock) void cg_alpha(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                          const double2* __restrict__ part, double* __restrict__ pp_out, int nslice) {
-    const int s = blockIdx.x * kCgScalBlock + threadIdx.x;
+    const int s = blockIdx.x, lane = threadIdx.x;
     if (s >= nslice || sc[s].done) return;
     const CgSeg g = seg[s];
-    double2 pq = make_double2(0, 0);
-    double pp = 0;
-    for (int b = 0; b < g.nblk; ++b) {
-        pq.x += part[2 * (g.blk0 + b)].x;
-        pq.y += part[2 * (g.blk0 + b)].y;
-        pp += part[2 * (g.blk0 + b) + 1].x;
-    }
+    const bool in = lane < g.nblk;
+    const int pb = g.blk0 + (in ? lane : 0);
+    const double2 a = part[2 * pb];
+    const double b = part[2 * pb + 1].x;
+    const double pqx = wave_total(in ? a.x : 0.0), pqy = wave_total(in ? a.y : 0.0);
+    const double pp = wave_total(in ? b : 0.0);
+    if (lane != 0) return;
     // complex division rho / pq
-    const double den = pq.x * pq.x + pq.y * pq.y;
+    const double den = pqx * pqx + pqy * pqy;
     const double rr = sc[s].rho_re, ri = sc[s].rho_im;
-    sc[s].al_re = (rr * pq.x + ri * pq.y) / den;
-    sc[s].al_im = (ri * pq.x - rr * pq.y) / den;
+    sc[s].al_re = (rr * pqx + ri * pqy) / den;
+    sc[s].al_im = (ri * pqx - rr * pqy) / den;
     pp_out[s] = pp;
 }
 
@@ -221,15 +232,14 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_init(const CgSeg* __restrict_
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
                         int nslice, int nitmax, double resThresh, double resThreshRel,
                         double stopThresh, int x0fallback) {
-    const int s = blockIdx.x * kCgScalBlock + threadIdx.x;
+    const int s = blockIdx.x, lane = threadIdx.x;   // one wave per slice
     if (s >= nslice) return;
     const CgSeg g = seg[s];
-    double bb = 0, xx = 0, rr = 0;
-    for (int b = 0; b < g.nblk; ++b) {
-        bb += part_b[2 * (g.blk0 + b)].x;
-        xx += part_b[2 * (g.blk0 + b) + 1].x;
-        rr += part_r[2 * (g.blk0 + b)].x;
-    }
+    const bool in = lane < g.nblk;
+    const int pb = g.blk0 + (in ? lane : 0);
+    const double b0 = part_b[2 * pb].x, x0 = part_b[2 * pb + 1].x, r0 = part_r[2 * pb].x;
+    double bb = wave_total(in ? b0 : 0.0), xx = wave_total(in ? x0 : 0.0), rr = wave_total(in ? r0 : 0.0);
+    if (lane != 0) return;
     CgScal c{};
     // warm-start fallback: a start worse than x0 = 0 (whose residual is b)
     // is replaced by x0 = 0 (cg_fallback_k resets x, r and p)
@@ -280,19 +290,19 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict_
                         int nslice, int nitmax_alloc, int explicit_res, double xChangeThresh,
                         double resChangeThresh, int stopOnResInc, double resIncThresh,
                         int stopOnXdifInc, int nseExtraIt, int* __restrict__ done_host) {
-    const int s = blockIdx.x * kCgScalBlock + threadIdx.x;
+    const int s = blockIdx.x, lane = threadIdx.x;   // one wave per slice
     if (s >= nslice) return;
-    CgScal c = sc[s];
-    if (c.done) {
-        done_host[s] = 1;
+    if (sc[s].done) {
+        if (lane == 0) done_host[s] = 1;
         return;
     }
     const CgSeg g = seg[s];
-    double rr = 0, xx = 0;
-    for (int b = 0; b < g.nblk; ++b) {
-        rr += part[2 * (g.blk0 + b)].x;
-        xx += part[2 * (g.blk0 + b) + 1].x;
-    }
+    const bool in = lane < g.nblk;
+    const int pb = g.blk0 + (in ? lane : 0);
+    const double r0 = part[2 * pb].x, x0 = part[2 * pb + 1].x;
+    const double rr = wave_total(in ? r0 : 0.0), xx = wave_total(in ? x0 : 0.0);
+    if (lane != 0) return;
+    CgScal c = sc[s];
     const double a2 = c.al_re * c.al_re + c.al_im * c.al_im;
     const double xd = a2 * pp[s];
     const int64_t h = (int64_t)s * (nitmax_alloc + 1);
@@ -389,7 +399,10 @@ CgSolver::CgSolver(mm_op* E) : E_(E) {
         const int64_t end = s + 1 < ns ? E->slice_image_off(s + 1) : E->n_image();
         seg[s].off = off;
         seg[s].len = end - off;
-        const int64_t want = (seg[s].len + kCgBlock * 4 - 1) / (kCgBlock * 4);
+        // one element per lane up to kCgMaxBlocksPerSlice blocks: the vector
+        // kernels are latency-bound, and a lane's loop iterations would each
+        // wait for their loads
+        const int64_t want = (seg[s].len + kCgBlock - 1) / kCgBlock;
         seg[s].nblk = (int)std::max<int64_t>(1, std::min<int64_t>(want, kCgMaxBlocksPerSlice));
         seg[s].blk0 = (int)b2s.size();
         for (int b = 0; b < seg[s].nblk; ++b) b2s.push_back(s);
@@ -424,11 +437,10 @@ CgSolver::~CgSolver() {
 void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x) {
     hipStream_t st = E_->ctx->stream;
     const int ns = E_->nslice();
-    const int sb = (ns + kCgScalBlock - 1) / kCgScalBlock;
     E_->apply_normal(p_.p, q_.p, 1, tmp_.p);
     hipLaunchKernelGGL(cg_dot_pq, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, q_.p,
                        part_.p);
-    hipLaunchKernelGGL(cg_alpha, dim3(sb), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, ns);
+    hipLaunchKernelGGL(cg_alpha, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, ns);
     hipLaunchKernelGGL(cg_update, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, x, p_.p,
                        r_.p, q_.p, explicit_res ? 0 : 1, part_.p);
     if (explicit_res) {
@@ -439,7 +451,7 @@ void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x
         hipLaunchKernelGGL(copy_rr_k, dim3((nblk_ + 255) / 256), dim3(256), 0, st, part2_.p, part_.p,
                            nblk_);
     }
-    hipLaunchKernelGGL(cg_beta, dim3(sb), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, hres_.p, hxn_.p,
+    hipLaunchKernelGGL(cg_beta, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, hres_.p, hxn_.p,
                        hxd_.p, ns, nitmax_, explicit_res, o.xChangeThresh, o.resChangeThresh,
                        o.stopOnResInc, o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt, done_d_.p);
     hipLaunchKernelGGL(cg_update_p, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, r_.p);
@@ -481,7 +493,7 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     hipLaunchKernelGGL(zero_scal_k, dim3(sb), dim3(kCgScalBlock), 0, st, sc_.p, ns);
     hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p, q_.p,
                        r_.p, p_.p, part_.p);
-    hipLaunchKernelGGL(cg_init, dim3(sb), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
+    hipLaunchKernelGGL(cg_init, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
                        hxn_.p, hxd_.p, ns, NitMax, o.resThresh, o.resThreshRel, o.stopThresh,
                        (x0 && o.x0Fallback) ? 1 : 0);
     LAUNCHCHECK();

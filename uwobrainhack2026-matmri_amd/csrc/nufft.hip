@@ -1442,31 +1442,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
             const int i1 = (idx / H) >> LR;
             const int i2 = r0 + r;
             const bool live = idx < tot && i2 < N2;
-            float2 s = make_float2(0.f, 0.f);
-            int pos = i1 - N1 / 2;
-            if (pos < 0) pos += G1;
-            if (live) {
-                const int v = i2 * N1 + i1;
-                const float2* src = R + pos * T + r * BCT;
+            // every lane issues its map and deapodization loads at once, from
+            // clamped (valid) addresses, and uses them unconditionally (a
+            // factor 0 drops coils past the last and dead lanes), so no load
+            // sits behind a branch
+            const int i1c = min(i1, N1 - 1), i2c = min(i2, N2 - 1);
+            const int v = i2c * N1 + i1c;
+            const float f1 = d1[i1c], f2 = d2[i2c];
+            float2 m[HB];
 #pragma unroll
-                for (int q = 0; q < HB; ++q) {
-                    const int b = h * HB + q;
-                    if (b < lanes) {
-                        const float2 m = mp[b * V + v];
-                        const float2 u = src[b];
-                        s.x += m.x * u.x + m.y * u.y;   // conj(maps) * u
-                        s.y += m.x * u.y - m.y * u.x;
-                    }
-                }
+            for (int q = 0; q < HB; ++q) m[q] = mp[min(h * HB + q, lanes - 1) * V + v];
+            const float f = f1 * f2;
+            int pos = i1c - N1 / 2;
+            if (pos < 0) pos += G1;
+            const float2* src = R + pos * T + r * BCT;
+            float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int q = 0; q < HB; ++q) {
+                const int b = h * HB + q;
+                const float k = live && b < lanes ? 1.f : 0.f;
+                const float2 u = src[b];
+                const float2 mk = make_float2(m[q].x * k, m[q].y * k);
+                s.x += mk.x * u.x + mk.y * u.y;   // conj(maps) * u
+                s.y += mk.x * u.y - mk.y * u.x;
             }
+            // each half is scaled before the lane sum (the factor is the
+            // pixel's on both lanes), so the factor's load is used on every
+            // lane and is issued with the map loads
+            s.x *= f;
+            s.y *= f;
             if constexpr (H == 2) {
                 s.x += __shfl_xor(s.x, 1);
                 s.y += __shfl_xor(s.y, 1);
             }
-            if (live && h == 0) {
-                const float f = d1[i1] * d2[i2];
-                xo[i2 * N1 + i1] = make_float2(s.x * f, s.y * f);
-            }
+            if (live && h == 0) xo[i2 * N1 + i1] = s;
         }
     } else {
         const int tot = N1 * T;
