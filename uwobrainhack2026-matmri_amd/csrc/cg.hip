@@ -37,6 +37,9 @@ constexpr int kCgBlock = 256;
 // zero_scal_k; a compile-time constant, as blockDim.x would be a memory load
 constexpr int kCgScalBlock = 64;
 constexpr int kCgMaxBlocksPerSlice = kCgScalBlock;
+// elements per lane whose loads the per-iteration vector kernels issue
+// together (a lane's loop iterations would each wait for their loads)
+constexpr int kCgUnroll = 4;
 
 __device__ __forceinline__ double2 warp_sum2(double2 v) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -102,10 +105,24 @@ __global__ __launch_bounds__(kCgBlock) void cg_dot_pq(const CgSeg* __restrict__ 
     const CgSeg g = seg[s];
     const int bl = blockIdx.x - g.blk0;
     double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
-    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
-        const float2 a = p[g.off + i], b = q[g.off + i];
-        acc_cdot(v[0], a, b);
-        acc_cdot(v[1], a, a);
+    const int64_t S = (int64_t)g.nblk * kCgBlock;
+    for (int64_t i0 = (int64_t)bl * kCgBlock + threadIdx.x; i0 < g.len; i0 += kCgUnroll * S) {
+        float2 a[kCgUnroll], c[kCgUnroll];
+        float w[kCgUnroll];
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            const int64_t i = i0 + u * S;
+            w[u] = i < g.len ? 1.f : 0.f;
+            const int64_t k = g.off + min(i, g.len - 1);
+            a[u] = p[k];
+            c[u] = q[k];
+        }
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            const float2 aw = make_float2(a[u].x * w[u], a[u].y * w[u]);
+            acc_cdot(v[0], aw, c[u]);
+            acc_cdot(v[1], aw, a[u]);
+        }
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
@@ -151,21 +168,34 @@ __global__ __launch_bounds__(kCgBlock) void cg_update(const CgSeg* __restrict__ 
     const int bl = blockIdx.x - g.blk0;
     const double are = sc[s].al_re, aim = sc[s].al_im;
     double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
-    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
-        const int64_t k = g.off + i;
-        const float2 pv = p[k];
-        float2 xv = x[k];
-        xv.x = (float)(xv.x + (are * pv.x - aim * pv.y));
-        xv.y = (float)(xv.y + (are * pv.y + aim * pv.x));
-        x[k] = xv;
-        acc_cdot(v[1], xv, xv);
-        if (r_update) {
-            const float2 qv = q[k];
-            float2 rv = r[k];
-            rv.x = (float)(rv.x - (are * qv.x - aim * qv.y));
-            rv.y = (float)(rv.y - (are * qv.y + aim * qv.x));
-            r[k] = rv;
-            acc_cdot(v[0], rv, rv);
+    const int64_t S = (int64_t)g.nblk * kCgBlock;
+    for (int64_t i0 = (int64_t)bl * kCgBlock + threadIdx.x; i0 < g.len; i0 += kCgUnroll * S) {
+        float2 pv[kCgUnroll], xv[kCgUnroll], qv[kCgUnroll], rv[kCgUnroll];
+        int64_t k[kCgUnroll];
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            k[u] = g.off + min(i0 + u * S, g.len - 1);
+            pv[u] = p[k[u]];
+            xv[u] = x[k[u]];
+            qv[u] = q[k[u]];   // read also when r is not updated (a valid vector)
+            rv[u] = r[k[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            const bool in = i0 + u * S < g.len;
+            const float w = in ? 1.f : 0.f;
+            float2 xn;
+            xn.x = (float)(xv[u].x + (are * pv[u].x - aim * pv[u].y));
+            xn.y = (float)(xv[u].y + (are * pv[u].y + aim * pv[u].x));
+            if (in) x[k[u]] = xn;
+            acc_cdot(v[1], make_float2(xn.x * w, xn.y * w), xn);
+            if (r_update) {
+                float2 rn;
+                rn.x = (float)(rv[u].x - (are * qv[u].x - aim * qv[u].y));
+                rn.y = (float)(rv[u].y - (are * qv[u].y + aim * qv[u].x));
+                if (in) r[k[u]] = rn;
+                acc_cdot(v[0], make_float2(rn.x * w, rn.y * w), rn);
+            }
         }
     }
     block_sum<2>(v);
@@ -365,10 +395,20 @@ __global__ __launch_bounds__(kCgBlock) void cg_update_p(const CgSeg* __restrict_
         return;
     }
     const double be = sc[s].beta;
-    for (int64_t i = (int64_t)bl * kCgBlock + threadIdx.x; i < g.len; i += (int64_t)g.nblk * kCgBlock) {
-        const int64_t k = g.off + i;
-        const float2 rv = r[k], pv = p[k];
-        p[k] = make_float2((float)(rv.x + be * pv.x), (float)(rv.y + be * pv.y));
+    const int64_t S = (int64_t)g.nblk * kCgBlock;
+    for (int64_t i0 = (int64_t)bl * kCgBlock + threadIdx.x; i0 < g.len; i0 += kCgUnroll * S) {
+        float2 rv[kCgUnroll], pv[kCgUnroll];
+        int64_t k[kCgUnroll];
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u) {
+            k[u] = g.off + min(i0 + u * S, g.len - 1);
+            rv[u] = r[k[u]];
+            pv[u] = p[k[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kCgUnroll; ++u)
+            if (i0 + u * S < g.len)
+                p[k[u]] = make_float2((float)(rv[u].x + be * pv[u].x), (float)(rv[u].y + be * pv[u].y));
     }
 }
 
@@ -399,10 +439,8 @@ CgSolver::CgSolver(mm_op* E) : E_(E) {
         const int64_t end = s + 1 < ns ? E->slice_image_off(s + 1) : E->n_image();
         seg[s].off = off;
         seg[s].len = end - off;
-        // one element per lane up to kCgMaxBlocksPerSlice blocks: the vector
-        // kernels are latency-bound, and a lane's loop iterations would each
-        // wait for their loads
-        const int64_t want = (seg[s].len + kCgBlock - 1) / kCgBlock;
+        // kCgUnroll elements per lane, up to kCgMaxBlocksPerSlice blocks
+        const int64_t want = (seg[s].len + kCgBlock * kCgUnroll - 1) / (kCgBlock * kCgUnroll);
         seg[s].nblk = (int)std::max<int64_t>(1, std::min<int64_t>(want, kCgMaxBlocksPerSlice));
         seg[s].blk0 = (int)b2s.size();
         for (int b = 0; b < seg[s].nblk; ++b) b2s.push_back(s);
