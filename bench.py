@@ -340,6 +340,11 @@ def nufft_roofline(mm, E, dev, slices, workload, iters=20):
         byts += nufft_stage_bytes(n, n, g, g, m, c).get(kname, 0.0)
     if byts <= 0:
         return None, stages
+    # the formula covers one pass over all slices (the column FFT makes two
+    # passes per application); a launch covers one slice group's share
+    passes = 2 if kname == "nu_cols" else 1
+    launches = stages.get(kname, {}).get("launches_per_application", passes) or passes
+    byts = byts * passes / launches
     ach = byts / (ms.value * 1e-3) / 1e9
     kern_file = {"nu_grid": "nu_grid"}.get(kname, kname)
     return ({"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,

@@ -167,7 +167,8 @@ int mm_op_profile_stage(mm_op* op, int idx, double* ms_per_launch, int* launches
  * phase on FP64 MFMA, bit 1 hardware sine/cosine) and "hio_mfma" (0/1) of a
  * high-order operator or a SENSE operator built on one; for a NUFFT operator
  * (or a SENSE operator built on one) "nu_stage_samples" (16..256),
- * "nu_stage_entries" (64..2048), "nu_degrid_block" (16..256), and the 0/1
+ * "nu_stage_entries" (64..2048), "nu_degrid_block" (16..256),
+ * "nu_slice_group" (slices per group of the normal operator, 0: one), and the 0/1
  * switches "nu_pair_cols", "nu_xcd_order", "nu_grid_pairs", "nu_snake", "nu_grid_stage",
  * "nu_degrid_stage", "nu_degrid_t". */
 int mm_op_set_option(mm_op* op, const char* key, int value);

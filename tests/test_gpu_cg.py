@@ -114,3 +114,36 @@ def test_cgne_multislice_equals_single(mm):
         print(f"  N={N} it {len(r)-1}/{len(rs)-1} status {s}/{ss} x diff {e:.2e}", flush=True)
         assert s == 1 and ss == 1 and abs(len(r) - len(rs)) <= 5 and e < 3e-2
         off += N * N
+
+
+def test_cgne_slice_groups_bitwise(mm):
+    """The normal operator run group by group (option nu_slice_group, with the
+    alternating slice order and finished slices dropping out) gives the same
+    batched solve as one group: the same kernels compute every slice in the
+    same way, so iterates, histories and iteration counts are bitwise equal."""
+    print("[case] cgne 7 shared-trajectory slices, slice groups 0 / 3 / 2", flush=True)
+    rng = np.random.default_rng(12)
+    N, M, C, P = 48, 3000, 4, 7
+    kl = mm.workloads.spiral(M, N // 2)
+    mp, ys = [], []
+    for _ in range(P):
+        maps = mm.workloads.gaussian_coils(N, N, C, rng)
+        ph = mm.workloads.ellipse_phantom(N, N, rng)
+        ys.append(mm.workloads.add_noise(o_mrSampFunc(ph, "notransp", NufftPlan([N, N], kl, None, 1.5, 4),
+                                                      RcvrPlan(maps)), rng))
+        mp.append(maps)
+    E = mm.senseOp(mm.nufftOp.batch([([N, N], kl, None)] * P, 1.5, 4), mm.rcvrOp.batch(mp))
+    ycat = np.concatenate([y.reshape(-1, order="F") for y in ys])
+    opt = {"resThresh": None, "resThreshRel": 1e-8, "xChangeThresh": None}
+    runs = []
+    for g in (0, 3, 2):
+        E.set_option("nu_slice_group", g)
+        assert E.get_option("nu_slice_group") == g
+        x, r, *_, st = mm.cgne(E, ycat, None, 60, opt)
+        runs.append((np.asarray(x), [np.asarray(h) for h in r], list(st)))
+        print(f"  group {g}: iterations {[len(h) - 1 for h in r]} status {st}", flush=True)
+    x0, r0, s0 = runs[0]
+    assert len(set(len(h) for h in r0)) > 1   # slices stop at different iterations
+    for x, r, st in runs[1:]:
+        assert st == s0 and np.array_equal(x, x0)
+        assert all(np.array_equal(a, b) for a, b in zip(r, r0))

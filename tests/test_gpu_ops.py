@@ -285,7 +285,7 @@ def test_nufft_tuning_options_and_autotune(mm):
                            .reshape(-1, order="F") for x, mp in zip(xs, maps)])
     for key, vals in [("nu_stage_samples", [16, 96, 256]), ("nu_stage_entries", [64, 768, 2048]),
                       ("nu_degrid_block", [16, 128, 256]), ("nu_pair_cols", [0, 1]), ("nu_xcd_order", [1, 0]),
-                      ("nu_grid_pairs", [0, 1]), ("nu_snake", [0, 1])]:
+                      ("nu_grid_pairs", [0, 1]), ("nu_snake", [0, 1]), ("nu_slice_group", [1, 0, 2])]:
         for v in vals:
             E.set_option(key, v)
             assert E.get_option(key) == v
@@ -302,6 +302,7 @@ def test_nufft_tuning_options_and_autotune(mm):
     assert chosen["nu_degrid_block"] in (32, 64, 128)
     assert chosen["nu_pair_cols"] in (0, 1) and chosen["nu_xcd_order"] in (0, 1)
     assert chosen["nu_grid_pairs"] in (0, 1) and chosen["nu_snake"] in (0, 1)
+    assert chosen["nu_slice_group"] == 0   # three slices: one group
     assert rel(E.normal(xcat), want) < TOL
 
 

@@ -1987,6 +1987,14 @@ bool NufftOp::set_option(const std::string& key, int value) {
     if (key == "nu_xcd_order") return flag(xcd_order_, true);
     if (key == "nu_grid_pairs") return flag(grid_pairs_, true);
     if (key == "nu_snake") return flag(snake_, true);
+    if (key == "nu_slice_group") {
+        if (value < 0) return false;
+        if (slice_group_ != value) {
+            slice_group_ = value;
+            cfgs_.clear();
+        }
+        return true;
+    }
     // caps keep every staged work-group within 64 KB of LDS at 16 transforms
     if (key == "nu_stage_samples" || key == "nu_stage_entries" || key == "nu_degrid_block") {
         int* dst = key == "nu_stage_samples" ? &stage_samples_
@@ -2011,6 +2019,7 @@ bool NufftOp::get_option(const std::string& key, int* value) const {
     else if (key == "nu_xcd_order") *value = xcd_order_;
     else if (key == "nu_grid_pairs") *value = grid_pairs_;
     else if (key == "nu_snake") *value = snake_;
+    else if (key == "nu_slice_group") *value = slice_group_;
     else if (key == "nu_stage_samples") *value = stage_samples_;
     else if (key == "nu_stage_entries") *value = stage_entries_;
     else if (key == "nu_degrid_block") *value = degrid_block_;
@@ -2060,6 +2069,16 @@ bool NufftOp::autotune_sense(const float2* maps, int C, const float* W, int iter
     tune({{{"nu_xcd_order", 0}}, {{"nu_xcd_order", 1}}}, {"nu_grid", "nu_degrid"});
     tune({{{"nu_snake", 1}}, {{"nu_snake", 0}}},
          {"nu_fwd_rows", "nu_cols", "nu_degrid", "nu_grid", "nu_adj_rows"});
+    {
+        // one group, or two or three groups of slices
+        const int ns = (int)sl_.size();
+        std::vector<std::vector<std::pair<const char*, int>>> cand{{{"nu_slice_group", 0}}};
+        if (ns >= 16) {
+            cand.push_back({{"nu_slice_group", (ns + 1) / 2}});
+            cand.push_back({{"nu_slice_group", (ns + 2) / 3}});
+        }
+        tune(cand, {"nu_fwd_rows", "nu_cols", "nu_degrid", "nu_grid", "nu_adj_rows"});
+    }
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     return true;
 }
@@ -2085,6 +2104,11 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     const int BCT = c.BCT;
     const int ns = (int)sl_.size();
     c.hs = sl_;
+    // slice groups (2-D staged paths only)
+    const int gsz = ndim_ == 2 && csr_ && grid_stage_ && degrid_stage_ && slice_group_ > 0 &&
+                            slice_group_ < ns ? slice_group_ : ns;
+    const int ngroups = (ns + gsz - 1) / gsz;
+    auto group_of = [&](int s) { return s / gsz; };
     int64_t wsoff = 0, poff = 0, coff = 0, ytoff = 0;
     std::vector<int4> wr, wfr, wc, wd, wg, wga, wcsr;
     std::vector<GridItem> wgt, wgm;
@@ -2230,9 +2254,11 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         }
         std::vector<std::pair<int, int>> groups;
         if (grid_pairs_ && lds2 <= 96 * 1024) {
-            std::map<int, int> open;   // first rectangle of a trajectory -> unpaired unit
+            // (first rectangle of a trajectory, slice group) -> unpaired unit
+            std::map<int64_t, int> open;
             for (int k = 0; k < (int)gunits.size(); ++k) {
-                const int key = gtile_first_[(size_t)gunits[(size_t)k].s];
+                const int su = gunits[(size_t)k].s;
+                const int64_t key = (int64_t)gtile_first_[(size_t)su] * 65536 + group_of(su);
                 auto o = open.find(key);
                 if (o == open.end()) {
                     open.emplace(key, k);
@@ -2395,11 +2421,6 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wg[ord[i]];
         wg.swap(sorted);
     }
-    hipStream_t st = ctx->stream;
-    c.sl.upload(c.hs, st);
-    c.w_rows.upload(wr, st);
-    c.w_frows.upload(wfr, st);
-    c.n_frows = (int)wfr.size();
     if (pair_cols_) {
         // neighbouring columns share cache lines (a column touches 64 of each
         // 128 bytes): place items 2k and 2k + 1 of every 16 at positions k and
@@ -2422,6 +2443,40 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         pair(wc);
         pair(wcf);
     }
+    // slice groups: each list ordered group by group (the order within a
+    // group kept, so this is the last reordering), with the span of every
+    // group
+    c.goff.assign((size_t)ngroups, {});
+    c.gcnt.assign((size_t)ngroups, {});
+    auto split = [&](auto& items, auto slice_of, int L) {
+        std::stable_sort(items.begin(), items.end(), [&](const auto& a, const auto& b) {
+            return group_of(slice_of(a)) < group_of(slice_of(b));
+        });
+        int off = 0;
+        for (int g = 0; g < ngroups; ++g) {
+            int n = 0;
+            while (off + n < (int)items.size() && group_of(slice_of(items[(size_t)(off + n)])) == g) ++n;
+            c.goff[(size_t)g][(size_t)L] = off;
+            c.gcnt[(size_t)g][(size_t)L] = n;
+            off += n;
+        }
+    };
+    {
+        auto s4 = [](const int4& w) { return w.x; };
+        auto sg = [](const GridItem& g) { return g.slice; };
+        split(wr, s4, kLRows);
+        split(wfr, s4, kLFrows);
+        split(wc, s4, kLCols);
+        split(wcf, s4, kLFcols);
+        split(wdb, s4, kLDb);
+        split(wgt, sg, kLGt);
+        split(wgm, sg, kLGm);
+    }
+    hipStream_t st = ctx->stream;
+    c.sl.upload(c.hs, st);
+    c.w_rows.upload(wr, st);
+    c.w_frows.upload(wfr, st);
+    c.n_frows = (int)wfr.size();
     c.w_cols.upload(wc, st);
     c.w_fcols.upload(wcf, st);
     c.w_deg.upload(wd, st);
@@ -2480,31 +2535,44 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)nu_degrid_tile<BCT, 0>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)nu_grid_tile<BCT, 1>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)nu_grid_tile<BCT, 2>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
         attr_set = true;
     }
     const bool wt4 = ndim_ == 2 && W_ == 4;
     const NuSliceDev* sl = c.sl.p;
-    auto forward = [&](const float2* x, float2* y, NuRun r) {
-        hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(c.n_frows), dim3(256), c.lds_rows, st, sl, c.w_frows.p, facs_.p,
-                           twid_.p, deap_.p, x, c.ws.p, r);
+    // (first item, items) of list L in slice group g; g < 0: the whole list
+    auto span = [&](int L, int g, int n_all) -> std::pair<int, int> {
+        if (g < 0) return {0, n_all};
+        return {c.goff[(size_t)g][(size_t)L], c.gcnt[(size_t)g][(size_t)L]};
+    };
+    auto forward = [&](const float2* x, float2* y, NuRun r, int g) {
+        const auto fr = span(kLFrows, g, c.n_frows), fc = span(kLFcols, g, c.n_cols), db = span(kLDb, g, c.n_db);
+        if (fr.second > 0)
+            hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(fr.second), dim3(256), c.lds_rows, st, sl,
+                               c.w_frows.p + fr.first, facs_.p, twid_.p, deap_.p, x, c.ws.p, r);
         LAUNCHCHECK();
         mark("nu_fwd_rows");
-        if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_fcols.p,
-                               facs_.p, twid_.p, c.ws.p, r.skip);
+        if (ndim_ == 2 && fc.second > 0) {
+            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(fc.second), dim3(256), c.lds_cols, st, sl,
+                               c.w_fcols.p + fc.first, facs_.p, twid_.p, c.ws.p, r.skip);
             LAUNCHCHECK();
             mark("nu_cols");
         }
-        if (ndim_ == 2 && degrid_stage_ && c.n_db > 0) {
+        if (ndim_ == 2 && degrid_stage_ && db.second > 0) {
             if (wt4)
-                hipLaunchKernelGGL((nu_degrid_tile<BCT, 4>), dim3(c.n_db), dim3(256), c.lds_db, st, sl, c.w_db.p,
+                hipLaunchKernelGGL((nu_degrid_tile<BCT, 4>), dim3(db.second), dim3(256), c.lds_db, st, sl, c.w_db.p + db.first,
                                    dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y,
                                    degrid_block_, r);
             else
-                hipLaunchKernelGGL((nu_degrid_tile<BCT, 0>), dim3(c.n_db), dim3(256), c.lds_db, st, sl, c.w_db.p,
+                hipLaunchKernelGGL((nu_degrid_tile<BCT, 0>), dim3(db.second), dim3(256), c.lds_db, st, sl, c.w_db.p + db.first,
                                    dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y,
                                    degrid_block_, r);
+        } else if (g >= 0) {
+            // a slice group without degridding blocks: nothing to degrid
         } else if (BCT >= 2 && degrid_t_) {
             if (wt4)
                 hipLaunchKernelGGL((nu_degrid_t<BCT, 4>), dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p,
@@ -2521,18 +2589,25 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_degrid");
     };
-    auto adjoint_from = [&](const float2* y, float2* x, NuRun r) {
+    auto adjoint_from = [&](const float2* y, float2* x, NuRun r, int g) {
+        const auto gt = span(kLGt, g, c.n_gt), gm = span(kLGm, g, c.n_gm), cc = span(kLCols, g, c.n_cols),
+                   rw = span(kLRows, g, c.n_rows);
         if (csr_ && grid_stage_) {
-            if (c.n_gt > 0 && c.gt_pairs)
-                hipLaunchKernelGGL((nu_grid_tile<BCT, 2>), dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p,
-                                   c.gt_sid.p, c.gt_sid_stride, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
-            else if (c.n_gt > 0)
-                hipLaunchKernelGGL((nu_grid_tile<BCT, 1>), dim3(c.n_gt), dim3(256), c.lds_gt, st, c.w_gt.p,
-                                   c.gt_sid.p, c.gt_sid_stride, gptr_.p, gent_.p, y, c.ws.p, c.parts.p, r.skip);
-            if (c.n_gm > 0) {
+            // the item's sample-id row is addressed by its block index:
+            // a group's rows start at its first item's
+            const int* sid = c.gt_sid.p + (int64_t)gt.first * c.gt_sid_stride;
+            if (gt.second > 0 && c.gt_pairs)
+                hipLaunchKernelGGL((nu_grid_tile<BCT, 2>), dim3(gt.second), dim3(256), c.lds_gt, st,
+                                   c.w_gt.p + gt.first, sid, c.gt_sid_stride, gptr_.p, gent_.p, y, c.ws.p, c.parts.p,
+                                   r.skip);
+            else if (gt.second > 0)
+                hipLaunchKernelGGL((nu_grid_tile<BCT, 1>), dim3(gt.second), dim3(256), c.lds_gt, st,
+                                   c.w_gt.p + gt.first, sid, c.gt_sid_stride, gptr_.p, gent_.p, y, c.ws.p, c.parts.p,
+                                   r.skip);
+            if (gm.second > 0) {
                 LAUNCHCHECK();
-                hipLaunchKernelGGL(nu_grid_merge<BCT>, dim3(c.n_gm), dim3(256), 0, st, c.w_gm.p, c.parts.p,
-                                   c.ws.p, r.skip);
+                hipLaunchKernelGGL(nu_grid_merge<BCT>, dim3(gm.second), dim3(256), 0, st, c.w_gm.p + gm.first,
+                                   c.parts.p, c.ws.p, r.skip);
             }
         } else if (csr_)
             hipLaunchKernelGGL(nu_grid_csr<BCT>, dim3(c.n_csr), dim3(256), 0, st, sl, c.w_csr.p, csr_ptr_.p,
@@ -2547,14 +2622,15 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                c.parts.p, c.counters.p, r);
         LAUNCHCHECK();
         mark("nu_grid");
-        if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
-                               facs_.p, twid_.p, c.ws.p, r.skip);
+        if (ndim_ == 2 && cc.second > 0) {
+            hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(cc.second), dim3(256), c.lds_cols, st, sl,
+                               c.w_cols.p + cc.first, facs_.p, twid_.p, c.ws.p, r.skip);
             LAUNCHCHECK();
             mark("nu_cols");
         }
-        hipLaunchKernelGGL((nu_adj_rows<BCT, BIG>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p, facs_.p,
-                           twid_.p, deap_.p, (const float2*)c.ws.p, x, r);
+        if (rw.second > 0)
+            hipLaunchKernelGGL((nu_adj_rows<BCT, BIG>), dim3(rw.second), dim3(256), c.lds_rows, st, sl,
+                               c.w_rows.p + rw.first, facs_.p, twid_.p, deap_.p, (const float2*)c.ws.p, x, r);
         LAUNCHCHECK();
         mark("nu_adj_rows");
     };
@@ -2562,17 +2638,21 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         NuRun rf = run;
         rf.ylayout = gather_ ? 1 : 0;
         float2* ybuf = gather_ ? c.yT.p : c.ydata.p;
-        forward(in, ybuf, rf);
         NuRun ra = run;
         ra.W = nullptr;
         ra.ylayout = rf.ylayout;
-        adjoint_from(ybuf, out, ra);
+        // slice groups one after another (one group: the whole lists)
+        const int ng = (int)c.goff.size();
+        for (int g = 0; g < ng; ++g) {
+            forward(in, ybuf, rf, ng > 1 ? g : -1);
+            adjoint_from(ybuf, out, ra, ng > 1 ? g : -1);
+        }
         return;
     }
     if (!adjoint) {
         NuRun rf = run;
         rf.ylayout = 0;
-        forward(in, out, rf);
+        forward(in, out, rf, -1);
         return;
     }
     NuRun ra = run;
@@ -2583,10 +2663,10 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_transpose_y");
         ra.ylayout = 1;
-        adjoint_from(c.yT.p, out, ra);
+        adjoint_from(c.yT.p, out, ra, -1);
     } else {
         ra.ylayout = 0;
-        adjoint_from(in, out, ra);
+        adjoint_from(in, out, ra, -1);
     }
 }
 
@@ -2654,8 +2734,10 @@ void NufftOp::profile_normal(const float2* in, float2* out, int64_t nrep, const 
     // one untimed application (plan set-up, first-launch costs)
     run(c, 0, in, out, maps, Wt, 1);
     Prof p;
-    p.ev.resize(16);
-    p.names.resize(16);
+    // marks per application: up to 7 stages per slice group
+    const size_t nev = 8 * std::max<size_t>(1, c.goff.size()) + 8;
+    p.ev.resize(nev);
+    p.names.resize(nev);
     for (auto& e : p.ev) HIPCHECK(hipEventCreate(&e));
     std::map<std::string, std::pair<double, int>> acc;
     int launches_per_app = 0;

@@ -2,6 +2,7 @@
 // composer).  The plan follows nufftOp.doPrep (operators/@nufftOp/nufftOp.m:
 // 138-274); forward and adjoint follow nufftOp.mtimes (:354-394).
 #pragma once
+#include <array>
 #include <map>
 
 #include "common.hpp"
@@ -94,6 +95,8 @@ struct GridItem {
     int slice2, lanes2;
 };
 constexpr int kDegridTile = 8;      // tile side of the staged degridding
+// work lists of the normal operator that the slice groups split
+enum NuList { kLRows = 0, kLFrows, kLCols, kLFcols, kLDb, kLGt, kLGm, kNuLists };
 
 class NufftOp : public mm_op {
    public:
@@ -170,6 +173,10 @@ class NufftOp : public mm_op {
         int n_gt = 0, n_gm = 0;
         size_t lds_gt = 0;
         int gt_pairs = 0;              // items carry two (slice, chunk) units
+        // slice groups of the normal operator (option "nu_slice_group"): the
+        // work lists are ordered group by group; group g's items of list L
+        // are [goff[g][L], goff[g][L] + gcnt[g][L])
+        std::vector<std::array<int, kNuLists>> goff, gcnt;
         // staged sample ids of item i at gt_sid[i * gt_sid_stride]: read
         // from the block index alone, before the item record arrives
         DevBuf<int> gt_sid;
@@ -237,6 +244,10 @@ class NufftOp : public mm_op {
     // "nu_snake"): each stage starts on the slices the previous one touched
     // last
     bool snake_ = true;
+    // slices per group of the normal operator (option "nu_slice_group"; 0:
+    // one group): every stage runs over one group before the next group
+    // starts, so a group's intermediate data stay in the memory-side cache
+    int slice_group_ = 0;
     int degrid_block_ = 64;                             // samples per staged degridding block
     void build_partitions();
     // host copies the partitions are rebuilt from
