@@ -283,16 +283,18 @@ __device__ __forceinline__ void fft_pass_inplace(float2* __restrict__ a, int n, 
     }
 }
 
-// Full transform in LDS; tw = the pass twiddles of fft_pass_twiddles.  With
-// b == nullptr every pass runs in the single buffer a (the caller checked
-// fft_inplace_ok), else the passes ping-pong between a and b.  All threads
-// of the block take part; returns the buffer holding the result.  The
-// caller must __syncthreads() after filling `a` (and tw).
+// Passes s0 .. s1 - 1 of a transform in LDS; tw = the pass twiddles of
+// fft_pass_twiddles.  With b == nullptr every pass runs in the single buffer
+// a (the caller checked fft_inplace_ok), else the passes ping-pong between a
+// and b.  All threads of the block take part; returns the buffer holding the
+// result.  The caller must __syncthreads() after filling `a` (and tw).
 template <int T, int SGN, bool BIG = false>
-__device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactors& f, const float2* tw) {
+__device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const FftFactors& f, const float2* tw, int s0,
+                                                 int s1) {
     const int n = f.n;
     int p = 1;
-    for (int s = 0; s < f.nf; ++s) {
+    for (int s = 0; s < s0; ++s) p *= f.r[s];
+    for (int s = s0; s < s1; ++s) {
         const int R = f.r[s];
         const float2* twp = tw + (p - 1);
         if (b == nullptr) {
@@ -345,6 +347,110 @@ __device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactor
         b = t;
     }
     return a;
+}
+
+// Full transform in LDS (fft_lds_range over every pass).
+template <int T, int SGN, bool BIG = false>
+__device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactors& f, const float2* tw) {
+    return fft_lds_range<T, SGN, BIG>(a, b, f, tw, 0, f.nf);
+}
+
+// First pass (radix R, P = 1, no twiddles) with its inputs taken from
+// src(q, t) -- the value at position q of interleaved transform t -- instead
+// of LDS, so the data are loaded straight into the butterflies' registers
+// (every load of a round issued before any use); the outputs go to `a`.
+// The caller synchronises before the next pass reads `a`.
+template <int R, int T, int SGN, class Src>
+__device__ __forceinline__ void fft_first_pass_src(float2* __restrict__ a, int n, Src src) {
+    constexpr int MAXB = fft_inplace_maxb(R, T);
+    constexpr int LT = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6;
+    const int nb = n / R;
+    const int total = nb * T;
+    for (int w0 = 0; w0 < total; w0 += MAXB * kFftThreads) {
+        float2 v[MAXB][R];
+#pragma unroll
+        for (int q = 0; q < MAXB; ++q) {
+            const int w = min(w0 + (int)threadIdx.x + q * kFftThreads, total - 1);
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[q][j] = src((w >> LT) + j * nb, w & (T - 1));
+        }
+#pragma unroll
+        for (int q = 0; q < MAXB; ++q) {
+            const int w = w0 + (int)threadIdx.x + q * kFftThreads;
+            if (w < total) {
+                dft_small<R>(v[q], (float)SGN);
+                float2* d = a + (w >> LT) * R * T + (w & (T - 1));
+#pragma unroll
+                for (int j = 0; j < R; ++j) d[j * T] = v[q][j];
+            }
+        }
+    }
+}
+
+template <int T, int SGN, class Src>
+__device__ __forceinline__ void fft_first_pass_src(int R, float2* a, int n, Src src) {
+    switch (R) {
+        case 8: fft_first_pass_src<8, T, SGN>(a, n, src); break;
+        case 4: fft_first_pass_src<4, T, SGN>(a, n, src); break;
+        case 2: fft_first_pass_src<2, T, SGN>(a, n, src); break;
+        case 3: fft_first_pass_src<3, T, SGN>(a, n, src); break;
+        case 5: fft_first_pass_src<5, T, SGN>(a, n, src); break;
+        default: fft_first_pass_src<7, T, SGN>(a, n, src); break;
+    }
+}
+
+// Last pass (radix R after the radices of product p > 1) reading its inputs
+// from LDS `a` and handing each output to dst(q, t, value) -- position q of
+// transform t -- instead of writing LDS back.  No synchronisation needed
+// after it.
+template <int R, int T, int SGN, bool BIG, class Dst>
+__device__ __forceinline__ void fft_last_pass_dst(const float2* __restrict__ a, int n, int p,
+                                                  const float2* __restrict__ twp, Dst dst) {
+    constexpr int MAXB = fft_inplace_maxb(R, T);
+    constexpr int LT = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6;
+    const int nb = n / R;
+    const int total = nb * T;
+    const unsigned pinv = mod_small_inv<BIG>(p);
+    for (int w0 = 0; w0 < total; w0 += MAXB * kFftThreads) {
+        float2 v[MAXB][R];
+#pragma unroll
+        for (int q = 0; q < MAXB; ++q) {
+            const int w = min(w0 + (int)threadIdx.x + q * kFftThreads, total - 1);
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[q][j] = a[w + j * nb * T];
+        }
+#pragma unroll
+        for (int q = 0; q < MAXB; ++q) {
+            const int w = w0 + (int)threadIdx.x + q * kFftThreads;
+            if (w < total) {
+                const int i = w >> LT;
+                const int t = w & (T - 1);
+                const int k = mod_small<BIG>(i, p, pinv);
+                const float2* c = twp + k * (R - 1);
+#pragma unroll
+                for (int j = 1; j < R; ++j) {
+                    const float2 cj = c[j - 1];
+                    const float cy = SGN > 0 ? cj.y : -cj.y;
+                    v[q][j] = make_float2(v[q][j].x * cj.x - v[q][j].y * cy, v[q][j].x * cy + v[q][j].y * cj.x);
+                }
+                dft_small<R>(v[q], (float)SGN);
+#pragma unroll
+                for (int j = 0; j < R; ++j) dst((i - k) * R + k + j * p, t, v[q][j]);
+            }
+        }
+    }
+}
+
+template <int T, int SGN, bool BIG, class Dst>
+__device__ __forceinline__ void fft_last_pass_dst(int R, const float2* a, int n, int p, const float2* twp, Dst dst) {
+    switch (R) {
+        case 8: fft_last_pass_dst<8, T, SGN, BIG>(a, n, p, twp, dst); break;
+        case 4: fft_last_pass_dst<4, T, SGN, BIG>(a, n, p, twp, dst); break;
+        case 2: fft_last_pass_dst<2, T, SGN, BIG>(a, n, p, twp, dst); break;
+        case 3: fft_last_pass_dst<3, T, SGN, BIG>(a, n, p, twp, dst); break;
+        case 5: fft_last_pass_dst<5, T, SGN, BIG>(a, n, p, twp, dst); break;
+        default: fft_last_pass_dst<7, T, SGN, BIG>(a, n, p, twp, dst); break;
+    }
 }
 
 }  // namespace mm

@@ -413,6 +413,50 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     float2* tw = lds + (ip ? 1 : 2) * G2 * T;
     float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b0;
     const int ncol = min(CPW, G1 - c0);
+    const FftFactors& F = facs[S.fac2];
+    if (F.nf >= 2) {
+        // The first pass takes its inputs straight from the grid into the
+        // butterflies' registers and the last pass writes its outputs
+        // straight to the grid: two LDS round trips fewer than staging the
+        // columns and storing them from LDS.  Forward: image row n2 sits at
+        // position (n2 - N2/2) mod G2, the others are zero; adjoint: rows
+        // n2 < N2 are read back from those positions.  Loads use clamped
+        // (valid) addresses and zero by a factor, so none waits on a branch.
+        constexpr int SG = ADJ ? -1 : 1;
+        auto src = [&](int q, int t) {
+            const int col = t >> LCL;
+            int r = q;
+            float ok = col < ncol ? 1.f : 0.f;
+            if (!ADJ) {
+                r = q + N2 / 2;
+                if (r >= G2) r -= G2;
+                ok = r < N2 ? ok : 0.f;
+                r = min(r, N2 - 1);
+            }
+            const float2 v = g[((int64_t)r * G1 + c0 + min(col, ncol - 1)) * BCT + (t & (CL - 1))];
+            return make_float2(v.x * ok, v.y * ok);
+        };
+        fft_first_pass_src<T, SG>(F.r[0], A, G2, src);
+        // the pass twiddles after the data loads, which they then do not delay
+        for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
+        __syncthreads();
+        const float2* Rb = fft_lds_range<T, SG, BIG>(A, Bb, F, tw, 1, F.nf - 1);
+        int p = 1;
+        for (int k = 0; k + 1 < F.nf; ++k) p *= F.r[k];
+        auto dst = [&](int q, int t, float2 v) {
+            const int col = t >> LCL;
+            int r = q;
+            bool keep = col < ncol;
+            if (ADJ) {
+                r = q + N2 / 2;
+                if (r >= G2) r -= G2;
+                keep = keep && r < N2;
+            }
+            if (keep) g[((int64_t)r * G1 + c0 + col) * BCT + (t & (CL - 1))] = v;
+        };
+        fft_last_pass_dst<T, SG, BIG>(F.r[F.nf - 1], Rb, G2, p, tw + (p - 1), dst);
+        return;
+    }
     // four loads per thread in flight, from clamped (valid) addresses; a
     // column past the grid's last is scaled by 0, and full rounds store
     // without a predicate, so no load sits behind a branch
