@@ -329,13 +329,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     float2* A = lds;
     float2* Bb = ip ? nullptr : lds + G1 * T;
     float2* tw = lds + (ip ? 1 : 2) * G1 * T;
-    for (int i = (N1 - N1 / 2) * T + threadIdx.x; i < (G1 - N1 / 2) * T; i += kFftThreads)
-        A[i] = make_float2(0.f, 0.f);
     const int lanes = lanes_of(run, BCT, ch);
     const float* d1 = deap + S.deap1;
     const float* d2 = deap + S.deap2;
     const float2* xs = x + S.img_off + (run.sense ? (int64_t)ch * V : (int64_t)ch * BCT * V);
     const float2* mp = run.maps + S.maps_off;
+    const FftFactors& F = facs[S.fac1];
+    float2* out = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
+    if (F.nf >= 2) {
+        // The first pass takes its inputs (coil image times deapodization,
+        // image column i1 at position (i1 - N1/2) mod G1, zeros elsewhere)
+        // straight into the butterflies' registers and the last pass writes
+        // the rows straight to the workspace: two LDS round trips fewer.
+        // Loads use clamped (valid) addresses and zero by a factor.
+        auto direct = [&](auto value) {
+            auto src = [&](int q, int t) {
+                int i1 = q + N1 / 2;
+                if (i1 >= G1) i1 -= G1;
+                const int bl = t & (BCT - 1);
+                const int i2 = r0 + (t >> LB);
+                const float ok = i1 < N1 && i2 < N2 && bl < lanes ? 1.f : 0.f;
+                const int i1c = min(i1, N1 - 1), i2c = min(i2, N2 - 1);
+                const float2 v = value(i2c * N1 + i1c, min(bl, lanes - 1));
+                const float f = d1[i1c] * d2[i2c] * ok;
+                return make_float2(v.x * f, v.y * f);
+            };
+            fft_first_pass_src<T, 1>(F.r[0], A, G1, src);
+        };
+        if (run.sense)
+            direct([&](int v, int b) { return cmulf(xs[v], mp[b * V + v]); });
+        else
+            direct([&](int v, int b) { return xs[b * V + v]; });
+        // the pass twiddles after the data loads, which they then do not delay
+        for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
+        __syncthreads();
+        const float2* Rb = fft_lds_range<T, 1, BIG>(A, Bb, F, tw, 1, F.nf - 1);
+        int p = 1;
+        for (int k = 0; k + 1 < F.nf; ++k) p *= F.r[k];
+        fft_last_pass_dst<T, 1, BIG>(F.r[F.nf - 1], Rb, G1, p, tw + (p - 1), [&](int q, int t, float2 v) {
+            const int i2 = r0 + (t >> LB);
+            if (i2 < N2) out[(i2 * G1 + q) * BCT + (t & (BCT - 1))] = v;
+        });
+        return;
+    }
+    for (int i = (N1 - N1 / 2) * T + threadIdx.x; i < (G1 - N1 / 2) * T; i += kFftThreads)
+        A[i] = make_float2(0.f, 0.f);
     const int total = N1 * T;
     // four elements per thread in flight: every load of a round is issued
     // before the first LDS store.  The loads use clamped (valid) addresses
@@ -379,8 +417,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     // the pass twiddles after the data loads, which they then do not delay
     for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     __syncthreads();
-    const float2* R = fft_lds<T, 1, BIG>(A, Bb, facs[S.fac1], tw);
-    float2* out = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
+    const float2* R = fft_lds<T, 1, BIG>(A, Bb, F, tw);
     const int tot2 = G1 * T;
     for (int idx = threadIdx.x; idx < tot2; idx += kFftThreads) {
         const int t = idx & (T - 1);
@@ -1442,6 +1479,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     float2* Bb = ip ? nullptr : lds + G1 * T;
     float2* tw = lds + (ip ? 1 : 2) * G1 * T;
     const float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
+    const FftFactors& F = facs[S.fac1];
+    const float2* R;
+    if (F.nf >= 2) {
+        // the first pass takes its inputs straight from the workspace rows
+        // into the butterflies' registers (one LDS round trip fewer); rows
+        // past the image are scaled by 0
+        fft_first_pass_src<T, -1>(F.r[0], A, G1, [&](int q, int t) {
+            const int i2 = r0 + (t >> LB);
+            const float ok = i2 < N2 ? 1.f : 0.f;
+            const float2 v = g[(min(i2, N2 - 1) * G1 + q) * BCT + (t & (BCT - 1))];
+            return make_float2(v.x * ok, v.y * ok);
+        });
+        // the pass twiddles after the data loads, which they then do not delay
+        for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
+        __syncthreads();
+        R = fft_lds_range<T, -1, BIG>(A, Bb, F, tw, 1, F.nf);
+    } else {
     const int total = G1 * T;
     // four loads per thread in flight, from clamped (valid) addresses; rows
     // past the image are scaled by 0, and full rounds store without a
@@ -1469,7 +1523,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     // the pass twiddles after the data loads, which they then do not delay
     for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     __syncthreads();
-    const float2* R = fft_lds<T, -1, BIG>(A, Bb, facs[S.fac1], tw);
+    R = fft_lds<T, -1, BIG>(A, Bb, F, tw);
+    }
     const int lanes = lanes_of(run, BCT, ch);
     const float* d1 = deap + S.deap1;
     const float* d2 = deap + S.deap2;
