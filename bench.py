@@ -352,6 +352,13 @@ def nufft_roofline(mm, E, dev, slices, workload, iters=20):
              "algorithmic_bytes_per_launch": byts}, stages)
 
 
+def applications_per_s(stages):
+    """Normal-operator applications (E'E of every slice of the batch) per
+    second from the per-stage launch times of one application."""
+    t = sum(v["ms_per_launch"] * v["launches_per_application"] for v in stages.values())
+    return 1e3 / t if t > 0 else None
+
+
 def _cref():
     """The C/OpenMP FP64 restatement (oracle/c/mmref.c), or None if not built."""
     try:
@@ -493,6 +500,7 @@ def run_warmstart(a, mm, torch, dev):
     if roof:
         out["roofline"] = roof
     out["normal_operator_stages"] = stages
+    out["normal_operator_applications_per_s"] = applications_per_s(stages)
     if not a.no_cpu:
         out["nrmse_vs_fp64_oracle"] = oracle_nrmse_warmstart(fam, host, prob, zw["x"], zz["x"])
         cb = cpu_baseline_warmstart(fam, host, prob, a.cpu_seconds)
@@ -601,10 +609,14 @@ def main():
                                "traffic": read_traffic(name.value.decode(), "highorder"),
                                "kernel": name.value.decode(), "ms_per_launch": kms,
                                "mfma_products_per_flop": 3}
+            # one application of the high-order SENSE operator is one launch
+            # of that kernel
+            out["operator_applications_per_s"] = 1e3 / kms
     if W["kind"] in ("nufft", "batch"):
         if roof_nu:
             out["roofline"] = roof_nu
         out["normal_operator_stages"] = stages_nu
+        out["normal_operator_applications_per_s"] = applications_per_s(stages_nu)
     if not a.no_cpu:
         cb = {"highorder": cpu_baseline_highorder, "nufft": cpu_baseline_nufft,
               "batch": cpu_baseline_batch}[W["kind"]](W["P"], a.cpu_seconds)
