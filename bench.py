@@ -609,9 +609,6 @@ def main():
                                "traffic": read_traffic(name.value.decode(), "highorder"),
                                "kernel": name.value.decode(), "ms_per_launch": kms,
                                "mfma_products_per_flop": 3}
-            # one application of the high-order SENSE operator is one launch
-            # of that kernel
-            out["operator_applications_per_s"] = 1e3 / kms
     if W["kind"] in ("nufft", "batch"):
         if roof_nu:
             out["roofline"] = roof_nu
