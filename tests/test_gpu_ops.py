@@ -215,9 +215,10 @@ def test_grid_staged_vs_global_rows(mm, kind, nrep):
 @pytest.mark.parametrize("kind,nrep", [("uniform", 8), ("uniform", 3), ("spiral", 8), ("spiral", 16),
                                        ("w8", 5), ("small", 2)])
 def test_degrid_staged_vs_global(mm, kind, nrep):
-    """2-D degridding from LDS-staged tiles against the gather from the grid
-    in global memory (option nu_degrid_stage 0): the same taps, products and
-    order, so the forward outputs are bitwise equal; both match the oracle."""
+    """2-D degridding from LDS-staged tiles (with pairs of sparse tiles merged
+    into one block, and without) against the gather from the grid in global
+    memory (option nu_degrid_stage 0): the same taps, products and order, so
+    the forward outputs are bitwise equal; all match the oracle."""
     print(f"[case] staged degridding {kind} nrep={nrep}", flush=True)
     rng = np.random.default_rng(6)
     if kind == "spiral":
@@ -231,8 +232,11 @@ def test_degrid_staged_vs_global(mm, kind, nrep):
     x = crand(rng, tuple(imN) + (nrep,))
     Sg = mm.nufftOp(imN, kloc, None, 1, 1.5, W)
     f_stage = np.asarray(Sg * x)
+    Sg.set_option("nu_degrid_merge", 0)   # every block on one tile
+    f_one = np.asarray(Sg * x)
     Sg.set_option("nu_degrid_stage", 0)
     f_glob = np.asarray(Sg * x)
+    assert np.array_equal(f_one, f_glob)
     e_o = rel(f_stage, NufftPlan(imN, kloc, None, 1.5, W).forward(x))
     print(f"  vs oracle {e_o:.2e}; equal to the global gather: {np.array_equal(f_stage, f_glob)}", flush=True)
     assert e_o < TOL
@@ -285,7 +289,8 @@ def test_nufft_tuning_options_and_autotune(mm):
                            .reshape(-1, order="F") for x, mp in zip(xs, maps)])
     for key, vals in [("nu_stage_samples", [16, 96, 256]), ("nu_stage_entries", [64, 768, 2048]),
                       ("nu_degrid_block", [16, 128, 256]), ("nu_pair_cols", [0, 1]), ("nu_xcd_order", [1, 0]),
-                      ("nu_grid_pairs", [0, 1]), ("nu_snake", [0, 1]), ("nu_slice_group", [1, 0, 2])]:
+                      ("nu_grid_pairs", [0, 1]), ("nu_snake", [0, 1]), ("nu_slice_group", [1, 0, 2]),
+                      ("nu_degrid_merge", [0, 1])]:
         for v in vals:
             E.set_option(key, v)
             assert E.get_option(key) == v

@@ -650,7 +650,8 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
 
 // Degridding from an LDS-staged grid region (2-D): work item = (slice,
 // chunk, sample block); a block holds up to bcap samples whose first
-// tap lies in one kDegridTile x kDegridTile tile, in sample order, with
+// tap lies in one kDegridTile x kDegridTile tile (or two such tiles side by
+// side along g1 whose samples together fit one block), in sample order, with
 // their first taps, weights, dcf and ids stored contiguously in block order
 // (plan time).  The work-group copies the block's sample data and the tile's
 // (kDegridTile + W - 1)^2 cells (wrapped) into LDS; each (sample, transform)
@@ -671,19 +672,22 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
     const int4 wk = work[blockIdx.x];
     if (skipped(run, wk.x)) return;
     const NuSliceDev& S = sl[wk.x];
-    const int4 blk = blocks[wk.z];   // (t1 | t2 << 16, first block-order entry, samples, 0)
+    const int4 blk = blocks[wk.z];   // (t1 | t2 << 16, first block-order entry, samples, tile width)
     const int ch = wk.y;
     const int b = threadIdx.x & (BCT - 1);
     const int G1 = S.G1, G2 = S.G2;
     const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2, WS = W1 + W2;
-    const int RW = kDegridTile + (WT > 0 ? WT : (W1 > W2 ? W1 : W2)) - 1;   // staged side
+    // staged region: the tile (kDegridTile, or twice that along g1 for two
+    // merged sparse tiles: blk.w) plus the kernel's reach
+    const int RW1 = blk.w + (WT > 0 ? WT : (W1 > W2 ? W1 : W2)) - 1;
+    const int RW2 = kDegridTile + (WT > 0 ? WT : (W1 > W2 ? W1 : W2)) - 1;
     const int t1 = blk.x & 0xffff, t2 = blk.x >> 16;
     const int n = blk.z;
     const int lanes = lanes_of(run, BCT, ch);
     const int64_t stride = chunk_stride(run, BCT);
     const float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    // LDS: grid cells [RW*RW][BCT], then per sample: id, first taps, dcf, weights
-    const int nst = RW * RW * BCT;
+    // LDS: grid cells [RW2][RW1][BCT], then per sample: id, first taps, dcf, weights
+    const int nst = RW1 * RW2 * BCT;
     int* s_id = (int*)(lds + nst);
     uint32_t* s_base = (uint32_t*)(s_id + bcap);
     float* s_dcf = (float*)(s_base + bcap);
@@ -693,10 +697,10 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
     // its weights (four per lane per pass) and the grid region (four per
     // lane per pass; wrapped cells by a select unless the region is wider
     // than the grid, a uniform case)
-    const bool wide = RW > G1 || RW > G2;
+    const bool wide = RW1 > G1 || RW2 > G2;
     auto cell = [&](int i) {
         const int cl = i / BCT;
-        const int l2 = cl / RW, l1 = cl - l2 * RW;
+        const int l2 = cl / RW1, l1 = cl - l2 * RW1;
         int g1 = t1 + l1, g2 = t2 + l2;
         if (wide) {
             g1 %= G1;
@@ -756,7 +760,7 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
 #pragma unroll
         for (int s2 = 0; s2 < (WT > 0 ? WT : 8); ++s2) {
             if (s2 >= W2) break;
-            const float2* row = lds + ((l2_0 + s2) * RW + l1_0) * BCT + b;
+            const float2* row = lds + ((l2_0 + s2) * RW1 + l1_0) * BCT + b;
 #pragma unroll
             for (int s1 = 0; s1 < (WT > 0 ? WT : 8); ++s1) {
                 if (s1 >= W1) break;
@@ -2054,12 +2058,24 @@ void NufftOp::build_partitions() {
             for (int q = 0; q < Ws; ++q) dwt_h.push_back(swt_host_[(size_t)(d.samp_off + m) * Ws + q]);
         }
         dblk_first_[(size_t)s] = (int)dblk_host_.size();
-        for (int t = 0; t < ntx * nty; ++t)
+        for (int t = 0; t < ntx * nty; ++t) {
+            const int corner = ((t % ntx) * kDegridTile) | (((t / ntx) * kDegridTile) << 16);
+            // two sparse tiles side by side (their samples are consecutive in
+            // bucket order) share one block and one staged region (option
+            // "nu_degrid_merge")
+            if (degrid_merge_ && (t % ntx) % 2 == 0 && (t % ntx) + 1 < ntx) {
+                const int n2 = tcnt[(size_t)t + 2] - tcnt[(size_t)t];
+                if (n2 > 0 && n2 <= degrid_block_) {
+                    dblk_host_.push_back(make_int4(corner, (int)(base + tcnt[(size_t)t]), n2, 2 * kDegridTile));
+                    ++t;
+                    continue;
+                }
+            }
             for (int k = tcnt[(size_t)t]; k < tcnt[(size_t)t + 1]; k += degrid_block_) {
                 const int nb = std::min(degrid_block_, tcnt[(size_t)t + 1] - k);
-                dblk_host_.push_back(make_int4(((t % ntx) * kDegridTile) | (((t / ntx) * kDegridTile) << 16),
-                                               (int)(base + k), nb, 0));
+                dblk_host_.push_back(make_int4(corner, (int)(base + k), nb, kDegridTile));
             }
+        }
         dblk_count_[(size_t)s] = (int)dblk_host_.size() - dblk_first_[(size_t)s];
         if (dwt_h.size() > (size_t)INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples");
     }
@@ -2088,6 +2104,14 @@ bool NufftOp::set_option(const std::string& key, int value) {
     if (key == "nu_xcd_order") return flag(xcd_order_, true);
     if (key == "nu_grid_pairs") return flag(grid_pairs_, true);
     if (key == "nu_snake") return flag(snake_, true);
+    if (key == "nu_degrid_merge") {
+        const bool v = value != 0;
+        if (v != degrid_merge_) {
+            degrid_merge_ = v;
+            build_partitions();
+        }
+        return true;
+    }
     if (key == "nu_slice_group") {
         if (value < 0) return false;
         if (slice_group_ != value) {
@@ -2120,6 +2144,7 @@ bool NufftOp::get_option(const std::string& key, int* value) const {
     else if (key == "nu_xcd_order") *value = xcd_order_;
     else if (key == "nu_grid_pairs") *value = grid_pairs_;
     else if (key == "nu_snake") *value = snake_;
+    else if (key == "nu_degrid_merge") *value = degrid_merge_;
     else if (key == "nu_slice_group") *value = slice_group_;
     else if (key == "nu_stage_samples") *value = stage_samples_;
     else if (key == "nu_stage_entries") *value = stage_entries_;
@@ -2165,6 +2190,7 @@ bool NufftOp::autotune_sense(const float2* maps, int C, const float* W, int iter
           {{"nu_stage_samples", 192}, {"nu_stage_entries", 1536}},
           {{"nu_stage_samples", 256}, {"nu_stage_entries", 2048}}},
          {"nu_grid"});
+    tune({{{"nu_degrid_merge", 1}}, {{"nu_degrid_merge", 0}}}, {"nu_degrid"});
     tune({{{"nu_degrid_block", 32}}, {{"nu_degrid_block", 64}}, {{"nu_degrid_block", 128}}}, {"nu_degrid"});
     tune({{{"nu_pair_cols", 1}}, {{"nu_pair_cols", 0}}}, {"nu_cols"});
     tune({{{"nu_xcd_order", 0}}, {{"nu_xcd_order", 1}}}, {"nu_grid", "nu_degrid"});
@@ -2605,7 +2631,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.n_deg = (int)wd.size();
     c.w_db.upload(wdb.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wdb, st);
     c.n_db = (int)wdb.size();
-    c.lds_db = (size_t)(kDegridTile + W_ - 1) * (kDegridTile + W_ - 1) * BCT * sizeof(float2) +
+    c.lds_db = (size_t)((degrid_merge_ ? 2 : 1) * kDegridTile + W_ - 1) * (kDegridTile + W_ - 1) * BCT * sizeof(float2) +
                (size_t)degrid_block_ * (3 + 2 * W_) * sizeof(float);
     c.n_grid = (int)wg.size();
     if (gather_) c.yT.alloc(std::max<int64_t>(ytoff, 1));

@@ -249,6 +249,9 @@ class NufftOp : public mm_op {
     // starts, so a group's intermediate data stay in the memory-side cache
     int slice_group_ = 0;
     int degrid_block_ = 64;                             // samples per staged degridding block
+    // two sparse tiles side by side share a degridding block (option
+    // "nu_degrid_merge")
+    bool degrid_merge_ = true;
     void build_partitions();
     // host copies the partitions are rebuilt from
     std::vector<int2> csr_ent_host_;
@@ -256,8 +259,8 @@ class NufftOp : public mm_op {
     std::vector<float> swt_host_, sdcf_host_;
     bool pair_cols_ = true;   // XCD-paired order of the column-pass items (option "nu_pair_cols")
     // staged degridding (option "nu_degrid_stage", default on): sample
-    // blocks of each distinct 2-D trajectory, (tile | tile2 << 16, first
-    // entry of dsamp_, samples, 0)
+    // blocks of each distinct 2-D trajectory, (g1 | g2 << 16 of the tile's
+    // corner, first entry of dsamp_, samples, tile width along g1)
     bool degrid_stage_ = true;
     std::vector<int4> dblk_host_;
     std::vector<int> dblk_first_, dblk_count_;   // per slice
