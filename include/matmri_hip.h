@@ -205,6 +205,11 @@ int mm_op_destroy(mm_op* op);
  * layer W[cout][cin][3][3] then b[cout] (38178 values).  A new network has
  * all parameters zero, i.e. x0 = x. */
 int mm_wsnet_create(mm_ctx* ctx, int64_t N1, int64_t N2, mm_wsnet** out);
+/* Network input normalisation (no counterpart in the reference; used with
+ * the warm start): out = x / (ssq + eps * max(ssq)) per pixel, ssq = sum over
+ * coils of |maps|^2 of the coil operator R (mm_rcvr_plan_create*), the
+ * maximum over every slice; x and out are image-side vectors of R. */
+int mm_rcvr_normalize_ssq(mm_op* R, const void* x, void* out, float eps, int on_device);
 int mm_wsnet_num_params(const mm_wsnet* net, int64_t* n);
 /* Sets the FP32 master parameters and resets the optimiser state. */
 int mm_wsnet_set_params(mm_wsnet* net, const float* theta, int on_device);

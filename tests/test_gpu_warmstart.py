@@ -246,3 +246,33 @@ def test_trained_warm_start_gain(mm):
     # the zero-init solve there, not degrade it
     assert ez < 0.3
     assert ew <= 1.1 * ez + 1e-3
+
+
+@pytest.mark.parametrize("dims", [[(24, 20)] * 3, [(24, 20), (16, 16), (16, 16), (30, 12)]])
+def test_normalize_by_coil_ssq(mm, dims):
+    """mm_rcvr_normalize_ssq: x / (ssq + eps * max ssq), ssq = sum_c |S_c|^2
+    per pixel and the maximum over every slice, against numpy in FP64
+    (slices of equal and of uneven sizes)."""
+    import torch
+    print(f"[case] coil ssq normalisation, slices {dims}", flush=True)
+    rng = np.random.default_rng(21)
+    C, eps = 5, 1e-3
+    maps = [(rng.standard_normal((n1, n2, C)) + 1j * rng.standard_normal((n1, n2, C))).astype(np.complex64)
+            for n1, n2 in dims]
+    R = mm.rcvrOp.batch(maps)
+    x = [(rng.standard_normal((n1, n2)) + 1j * rng.standard_normal((n1, n2))).astype(np.complex64)
+         for n1, n2 in dims]
+    xf = np.concatenate([v.reshape(-1, order="F") for v in x])
+    ssq = np.concatenate([(np.abs(m.astype(np.complex128)) ** 2).sum(axis=2).reshape(-1, order="F") for m in maps])
+    want = xf.astype(np.complex128) / (ssq + eps * ssq.max())
+    xt = torch.from_numpy(xf).cuda()
+    out = torch.empty_like(xt)
+    torch.cuda.synchronize()
+    mm._lib.check(mm._lib.lib().mm_rcvr_normalize_ssq(R._plan.h, xt.data_ptr(), out.data_ptr(), eps, 1))
+    mm._lib.check(mm._lib.lib().mm_ctx_sync(mm._lib.context()))
+    got = out.cpu().numpy()
+    host = np.empty_like(xf)
+    mm._lib.check(mm._lib.lib().mm_rcvr_normalize_ssq(R._plan.h, xf.ctypes.data, host.ctypes.data, eps, 0))
+    e = _rel(got, want)
+    print(f"  rel. error {e:.2e}", flush=True)
+    assert e < 1e-6 and np.array_equal(got, host)

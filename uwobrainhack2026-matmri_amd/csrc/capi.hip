@@ -129,6 +129,28 @@ int mm_rcvr_plan_create_multi(mm_ctx* ctx, int nslice, const int64_t* V, int64_t
     });
 }
 
+int mm_rcvr_normalize_ssq(mm_op* R, const void* x, void* out, float eps, int on_device) {
+    return guarded([&] {
+        if (!R || !x || !out) fail(MM_ERR_ARG, "rcvrOp: NULL argument");
+        auto* rc = dynamic_cast<RcvrOp*>(R);
+        if (!rc) fail(MM_ERR_ARG, "rcvrOp: not a coil operator");
+        if (!(eps >= 0.f)) fail(MM_ERR_ARG, "rcvrOp: eps must be >= 0");
+        mm_ctx* ctx = rc->ctx;
+        set_device(ctx);
+        const int64_t n = rc->n_image();
+        if (on_device) {
+            rc->normalize_by_coil_ssq((const float2*)x, (float2*)out, eps);
+            return;
+        }
+        DevBuf<float2> dx, dy;
+        dx.upload((const float2*)x, (size_t)n, ctx->stream);
+        dy.alloc((size_t)n);
+        rc->normalize_by_coil_ssq(dx.p, dy.p, eps);
+        HIPCHECK(hipMemcpyAsync(out, dy.p, sizeof(float2) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int mm_hio_plan_create(mm_ctx* ctx, int64_t V, int64_t K, int Lh, int Lc, const double* x,
                        const double* y, const double* z, const double* b0, const double* t,
                        const double* h, const double* g, const double* mask, mm_op** out) {

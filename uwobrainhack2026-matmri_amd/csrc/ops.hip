@@ -99,6 +99,83 @@ void RcvrOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     }
 }
 
+// Sum of squares of the coil maps per pixel (slices of one image size V in
+// one launch, as rcvr_fwd_k) and, per block, its maximum.
+constexpr int kSsqBlocks = 1024;
+__global__ __launch_bounds__(256) void rcvr_ssq_k(const float2* __restrict__ maps, float* __restrict__ ssq,
+                                                  float* __restrict__ bmax, int64_t V, int C, int64_t nimg) {
+    __shared__ float red[256];
+    const int64_t n = V * nimg;
+    float mx = 0.f;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t v = i % V;
+        const float2* mp = maps + (i / V) * C * V + v;
+        float s = 0.f;
+        for (int c = 0; c < C; ++c) {
+            const float2 m = mp[(int64_t)c * V];
+            s += m.x * m.x + m.y * m.y;
+        }
+        ssq[i] = s;
+        mx = fmaxf(mx, s);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmax[blockIdx.x] = red[0];
+}
+
+// out = x / (ssq + eps * max), the maximum taken over the kSsqBlocks block
+// maxima (every block reduces them itself)
+__global__ __launch_bounds__(256) void rcvr_norm_k(const float2* __restrict__ x, const float* __restrict__ ssq,
+                                                   const float* __restrict__ bmax, int nb, float eps,
+                                                   float2* __restrict__ out, int64_t n) {
+    __shared__ float red[256];
+    float mx = 0.f;
+    for (int i = threadIdx.x; i < nb; i += 256) mx = fmaxf(mx, bmax[i]);
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    const float d0 = eps * red[0];
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float d = ssq[i] + d0;
+        out[i] = make_float2(x[i].x / d, x[i].y / d);
+    }
+}
+
+void RcvrOp::normalize_by_coil_ssq(const float2* x, float2* out, float eps) {
+    hipStream_t st = ctx->stream;
+    ssq_.ensure((size_t)V_total_);
+    // one set of block maxima per run of equal-size slices
+    int nruns = 0;
+    for (size_t s0 = 0; s0 < Vs_.size(); ++nruns) {
+        size_t s1 = s0 + 1;
+        while (s1 < Vs_.size() && Vs_[s1] == Vs_[s0]) ++s1;
+        s0 = s1;
+    }
+    ssq_max_.ensure((size_t)nruns * kSsqBlocks);
+    HIPCHECK(hipMemsetAsync(ssq_max_.p, 0, sizeof(float) * nruns * kSsqBlocks, st));
+    int r = 0;
+    for (size_t s0 = 0; s0 < Vs_.size(); ++r) {
+        size_t s1 = s0 + 1;
+        while (s1 < Vs_.size() && Vs_[s1] == Vs_[s0]) ++s1;
+        const int64_t V = Vs_[s0], nimg = (int64_t)(s1 - s0);
+        const int nb = (int)std::min<int64_t>(kSsqBlocks, (V * nimg + 255) / 256);
+        hipLaunchKernelGGL(rcvr_ssq_k, dim3(nb), dim3(256), 0, st, maps_.p + start_[s0] * C_,
+                           ssq_.p + start_[s0], ssq_max_.p + (size_t)r * kSsqBlocks, V, C_, nimg);
+        LAUNCHCHECK();
+        s0 = s1;
+    }
+    hipLaunchKernelGGL(rcvr_norm_k, dim3(grid_for(V_total_)), dim3(256), 0, st, x, ssq_.p, ssq_max_.p,
+                       nruns * kSsqBlocks, eps, out, V_total_);
+    LAUNCHCHECK();
+}
+
 SenseOp::SenseOp(mm_ctx* c, mm_op* S, mm_op* R, const float* Wh) : S_(S), R_(R) {
     ctx = c;
     if (!S || !R) fail(MM_ERR_ARG, "mrSampFunc: samplingOp and R are required");

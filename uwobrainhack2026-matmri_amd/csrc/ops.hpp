@@ -18,12 +18,16 @@ class RcvrOp : public mm_op {
     void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
     int coils() const { return C_; }
     const float2* maps() const { return maps_.p; }
+    // out = x / (ssq + eps * max(ssq)), ssq = sum_c |maps_c|^2 per pixel,
+    // max over every slice (the warm-start network's input)
+    void normalize_by_coil_ssq(const float2* x, float2* out, float eps);
 
    private:
     int C_ = 1;
     std::vector<int64_t> Vs_, start_;
     int64_t V_total_ = 0;
     DevBuf<float2> maps_;
+    DevBuf<float> ssq_, ssq_max_;
 };
 
 class SenseOp : public mm_op {

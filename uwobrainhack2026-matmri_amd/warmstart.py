@@ -247,21 +247,19 @@ class ProblemFamily:
         return dict(E=E, R=R, x_true=x_true, y=y.reshape(-1), xin=xin, seeds=list(seeds))
 
 
-def coil_ssq(R):
-    """sum_c |S_c|^2 per pixel: R'(R 1), on the device."""
-    ones = torch.ones(R.image_shape[::-1] if R.nslice == 1 else R.image_shape, dtype=torch.complex64,
-                      device="cuda")
-    return (R.H * (R * ones)).real
-
-
 def input_image(E, R, y, dcf, eps=1e-3):
-    """Network input of a SENSE problem: R'F'(dcf .* y) / (sum_c |S_c|^2),
-    with dcf per sample (repeated over coils and slices); device tensors,
-    flat image vector out."""
+    """Network input of a SENSE problem: R'F'(dcf .* y) / (ssq + eps max ssq),
+    ssq = sum_c |S_c|^2 per pixel (the maximum over every slice), with dcf
+    per sample (repeated over coils and slices); device tensors, flat image
+    vector out.  The normalisation is one library call
+    (mm_rcvr_normalize_ssq)."""
     M = dcf.numel()
-    xg = E.H * (y.reshape(-1, M) * dcf.reshape(1, M)).reshape(-1)
-    ssq = coil_ssq(R).reshape(-1)
-    return (xg.reshape(-1) / (ssq + eps * ssq.max())).to(torch.complex64)
+    xg = (E.H * (y.reshape(-1, M) * dcf.reshape(1, M)).reshape(-1)).reshape(-1).to(torch.complex64).contiguous()
+    out = torch.empty_like(xg)
+    torch.cuda.current_stream(xg.device).synchronize()
+    check(_lib.lib().mm_rcvr_normalize_ssq(R._plan.h, xg.data_ptr(), out.data_ptr(), float(eps), 1))
+    check(_lib.lib().mm_ctx_sync(_lib.context()))
+    return out
 
 
 def cg_opts(rel_resid, NitMax):
