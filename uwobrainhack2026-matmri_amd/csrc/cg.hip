@@ -32,9 +32,9 @@ struct CgSeg {
 };
 
 constexpr int kCgBlock = 256;
-// threads of the per-slice scalar kernels: one wave per slice in cg_alpha,
-// cg_beta and cg_init (a lane per partial-sum block), one slice per lane in
-// zero_scal_k; a compile-time constant, as blockDim.x would be a memory load
+// threads of the per-slice scalar kernels: one wave per slice in cg_beta and
+// cg_init (a lane per partial-sum block), one slice per lane in zero_scal_k;
+// a compile-time constant, as blockDim.x would be a memory load
 constexpr int kCgScalBlock = 64;
 constexpr int kCgMaxBlocksPerSlice = kCgScalBlock;
 // elements per lane whose loads the per-iteration vector kernels issue
@@ -131,32 +131,16 @@ __global__ __launch_bounds__(kCgBlock) void cg_dot_pq(const CgSeg* __restrict__ 
     }
 }
 
-// alpha = rho / (p'Ap)  (cgne.m:228); keeps |p|^2 for |x - xold|^2.  One
-// wave per slice.
-__global__ __launch_bounds__(kCgScalBlock) void cg_alpha(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
-                         const double2* __restrict__ part, double* __restrict__ pp_out, int nslice) {
-    const int s = blockIdx.x, lane = threadIdx.x;
-    if (s >= nslice || sc[s].done) return;
-    const CgSeg g = seg[s];
-    const bool in = lane < g.nblk;
-    const int pb = g.blk0 + (in ? lane : 0);
-    const double2 a = part[2 * pb];
-    const double b = part[2 * pb + 1].x;
-    const double pqx = wave_total(in ? a.x : 0.0), pqy = wave_total(in ? a.y : 0.0);
-    const double pp = wave_total(in ? b : 0.0);
-    if (lane != 0) return;
-    // complex division rho / pq
-    const double den = pqx * pqx + pqy * pqy;
-    const double rr = sc[s].rho_re, ri = sc[s].rho_im;
-    sc[s].al_re = (rr * pqx + ri * pqy) / den;
-    sc[s].al_im = (ri * pqx - rr * pqy) / den;
-    pp_out[s] = pp;
-}
-
-// x += alpha p; if `r_update`: r -= alpha q.  Partials: [0] = r'r, [1] = x'x.
+// alpha = rho / (p'Ap) (cgne.m:228), then x += alpha p and, if `r_update`,
+// r -= alpha q.  Every block reduces the slice's p'Ap partial sums (dpart,
+// from cg_dot_pq) itself -- one wave, a fixed order, so all blocks hold the
+// same alpha -- and the slice's first block records alpha and |p|^2 (for
+// |x - xold|^2).  Partials out: [0] = r'r, [1] = x'x.
 __global__ __launch_bounds__(kCgBlock) void cg_update(const CgSeg* __restrict__ seg,
                                                       const int* __restrict__ b2s,
-                                                      const CgScal* __restrict__ sc,
+                                                      CgScal* __restrict__ sc,
+                                                      const double2* __restrict__ dpart,
+                                                      double* __restrict__ pp_out,
                                                       float2* __restrict__ x,
                                                       const float2* __restrict__ p,
                                                       float2* __restrict__ r,
@@ -166,7 +150,31 @@ __global__ __launch_bounds__(kCgBlock) void cg_update(const CgSeg* __restrict__ 
     if (sc[s].done) return;
     const CgSeg g = seg[s];
     const int bl = blockIdx.x - g.blk0;
-    const double are = sc[s].al_re, aim = sc[s].al_im;
+    __shared__ double al[2];
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const bool in = lane < g.nblk;
+        const int pb = g.blk0 + (in ? lane : 0);
+        const double2 a = dpart[2 * pb];
+        const double b = dpart[2 * pb + 1].x;
+        const double pqx = wave_total(in ? a.x : 0.0), pqy = wave_total(in ? a.y : 0.0);
+        const double pp = wave_total(in ? b : 0.0);
+        if (lane == 0) {
+            // complex division rho / pq
+            const double den = pqx * pqx + pqy * pqy;
+            const double rr = sc[s].rho_re, ri = sc[s].rho_im;
+            const double are = (rr * pqx + ri * pqy) / den, aim = (ri * pqx - rr * pqy) / den;
+            al[0] = are;
+            al[1] = aim;
+            if (bl == 0) {
+                sc[s].al_re = are;
+                sc[s].al_im = aim;
+                pp_out[s] = pp;
+            }
+        }
+    }
+    __syncthreads();
+    const double are = al[0], aim = al[1];
     double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
     const int64_t S = (int64_t)g.nblk * kCgBlock;
     for (int64_t i0 = (int64_t)bl * kCgBlock + threadIdx.x; i0 < g.len; i0 += kCgUnroll * S) {
@@ -454,6 +462,7 @@ CgSolver::CgSolver(mm_op* E) : E_(E) {
     sc_.alloc(ns);
     part_.alloc((size_t)2 * nblk_);
     part2_.alloc((size_t)2 * nblk_);
+    partu_.alloc((size_t)2 * nblk_);
     pp_.alloc(ns);
     b_.alloc(n_);
     r_.alloc(n_);
@@ -478,18 +487,17 @@ void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x
     E_->apply_normal(p_.p, q_.p, 1, tmp_.p);
     hipLaunchKernelGGL(cg_dot_pq, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, q_.p,
                        part_.p);
-    hipLaunchKernelGGL(cg_alpha, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, ns);
-    hipLaunchKernelGGL(cg_update, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, x, p_.p,
-                       r_.p, q_.p, explicit_res ? 0 : 1, part_.p);
+    hipLaunchKernelGGL(cg_update, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, part_.p, pp_.p, x,
+                       p_.p, r_.p, q_.p, explicit_res ? 0 : 1, partu_.p);
     if (explicit_res) {
         // r = b - N x; r'r into part2, x'x stays in part_[..][1]
         E_->apply_normal(x, q_.p, 1, tmp_.p);
         hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p,
                            q_.p, r_.p, (float2*)nullptr, part2_.p);
-        hipLaunchKernelGGL(copy_rr_k, dim3((nblk_ + 255) / 256), dim3(256), 0, st, part2_.p, part_.p,
+        hipLaunchKernelGGL(copy_rr_k, dim3((nblk_ + 255) / 256), dim3(256), 0, st, part2_.p, partu_.p,
                            nblk_);
     }
-    hipLaunchKernelGGL(cg_beta, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part_.p, pp_.p, hres_.p, hxn_.p,
+    hipLaunchKernelGGL(cg_beta, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, partu_.p, pp_.p, hres_.p, hxn_.p,
                        hxd_.p, ns, nitmax_, explicit_res, o.xChangeThresh, o.resChangeThresh,
                        o.stopOnResInc, o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt, done_d_.p);
     hipLaunchKernelGGL(cg_update_p, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, r_.p);

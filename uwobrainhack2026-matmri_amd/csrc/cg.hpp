@@ -28,6 +28,7 @@ class CgSolver {
     DevBuf<int> b2s_;
     DevBuf<CgScal> sc_;
     DevBuf<double2> part_, part2_;
+    DevBuf<double2> partu_;   // partial sums of cg_update (r'r, x'x)
     DevBuf<double> pp_, hres_, hxn_, hxd_;
     DevBuf<int> done_d_;
     DevBuf<float2> b_, r_, p_, q_, tmp_;
