@@ -327,11 +327,15 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict_
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
                         int nslice, int nitmax_alloc, int explicit_res, double xChangeThresh,
                         double resChangeThresh, int stopOnResInc, double resIncThresh,
-                        int stopOnXdifInc, int nseExtraIt, int* __restrict__ done_host) {
+                        int stopOnXdifInc, int nseExtraIt, int* __restrict__ done_host,
+                        int* __restrict__ done_map) {
     const int s = blockIdx.x, lane = threadIdx.x;   // one wave per slice
     if (s >= nslice) return;
     if (sc[s].done) {
-        if (lane == 0) done_host[s] = 1;
+        if (lane == 0) {
+            done_host[s] = 1;
+            done_map[s] = 1;
+        }
         return;
     }
     const CgSeg g = seg[s];
@@ -384,6 +388,7 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict_
     c.done = finished ? 2 : 0;
     sc[s] = c;
     done_host[s] = finished;
+    done_map[s] = finished;
 }
 
 // p = r + beta p (cgne.m:328); a slice that finished this iteration (done == 2)
@@ -469,7 +474,10 @@ CgSolver::CgSolver(mm_op* E) : E_(E) {
     p_.alloc(n_);
     q_.alloc(n_);
     tmp_.alloc(E->n_data());
-    HIPCHECK(hipHostMalloc((void**)&done_h_, sizeof(int) * ns, hipHostMallocDefault));
+    // the done flags also go to coherent, device-mapped host memory, which
+    // cg_beta writes directly: the host polls them without a copy
+    HIPCHECK(hipHostMalloc((void**)&done_h_, sizeof(int) * ns, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHECK(hipHostGetDevicePointer((void**)&done_map_, done_h_, 0));
     HIPCHECK(hipEventCreateWithFlags(&poll_ev_, hipEventDisableTiming));
     HIPCHECK(hipStreamSynchronize(st));
 }
@@ -499,7 +507,7 @@ void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x
     }
     hipLaunchKernelGGL(cg_beta, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, partu_.p, pp_.p, hres_.p, hxn_.p,
                        hxd_.p, ns, nitmax_, explicit_res, o.xChangeThresh, o.resChangeThresh,
-                       o.stopOnResInc, o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt, done_d_.p);
+                       o.stopOnResInc, o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt, done_d_.p, done_map_);
     hipLaunchKernelGGL(cg_update_p, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, r_.p);
     LAUNCHCHECK();
 }
@@ -549,18 +557,21 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
         LAUNCHCHECK();
     }
     const int expResN = o.expResN > 0 ? o.expResN : 100;
-    const int poll = 4;
+    // every iteration for large batches (an iteration outlasts the host's
+    // wait and relaunch by far), every 4th for small ones
+    const int poll = n_ >= (int64_t)1 << 20 ? 1 : 4;
     // finished slices are left out of the operator from the next iteration on
     HIPCHECK(hipMemsetAsync(done_d_.p, 0, sizeof(int) * ns, st));
+    std::memset(done_h_, 0, sizeof(int) * ns);
     struct SkipGuard {
         mm_op* E;
         ~SkipGuard() { E->set_skip(nullptr); }
     } guard{E_};
     E_->set_skip(done_d_.p);
-    // The done flags are copied to the host every `poll` iterations; the host
-    // reads them only after the next iteration has been queued, so the device
-    // never waits for the host (a solve that has finished runs one more
-    // iteration whose kernels all return at once).
+    // cg_beta writes the done flags to mapped host memory as well; the host
+    // reads them after every `poll` iterations, once the next iteration has
+    // been queued, so the device never waits for the host (a solve that has
+    // finished runs one more iteration whose kernels all return at once).
     bool pending = false;
     for (int it = 0; it < NitMax; ++it) {
         const int explicit_res = (it % expResN) == 0;
@@ -584,7 +595,6 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
             if (all) break;
         }
         if ((it + 1) % poll == 0 && it + 2 < NitMax) {
-            HIPCHECK(hipMemcpyAsync(done_h_, done_d_.p, sizeof(int) * ns, hipMemcpyDeviceToHost, st));
             HIPCHECK(hipEventRecord(poll_ev_, st));
             pending = true;
         }

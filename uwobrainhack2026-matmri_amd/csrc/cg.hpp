@@ -32,7 +32,8 @@ class CgSolver {
     DevBuf<double> pp_, hres_, hxn_, hxd_;
     DevBuf<int> done_d_;
     DevBuf<float2> b_, r_, p_, q_, tmp_;
-    int* done_h_ = nullptr;
+    int* done_h_ = nullptr;     // host view of the done flags (coherent, mapped)
+    int* done_map_ = nullptr;   // device view of the same memory
     hipEvent_t poll_ev_ = nullptr;   // recorded after the copy of the done flags
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
