@@ -1,4 +1,4 @@
-// Mixed-radix (8,4,2,3,5,7) Stockham FFT on data resident in LDS.
+// Mixed-radix (16,12,8,4,2,3,5,7) Stockham FFT on data resident in LDS.
 //
 // Layout: element (position q, transform t) of `T` interleaved transforms of
 // length n lives at buf[q*T + t], so consecutive lanes work on consecutive
@@ -29,13 +29,19 @@ struct FftFactors {
     int r[kMaxFactors] = {};
 };
 
-// Radix order: 8s first, then 4, 2, 3, 5, 7.  Returns false when n has a
-// prime factor above 7 (the grid-size rule of nufftOp.m:146-152 never
-// produces one).
-inline bool fft_factorize(int n, FftFactors& f) {
+// Radix order: 8s first, then 4, 2, 3, 5, 7; `wide` (passes of 16
+// interleaved transforms, whose registers hold 16 values per thread):
+// 16s and 12s before those (192 = 16 x 12: two passes instead of three).
+// Returns false when n has a prime factor above 7 (the grid-size rule of
+// nufftOp.m:146-152 never produces one).
+inline bool fft_factorize(int n, FftFactors& f, bool wide = false) {
     f = FftFactors();
     f.n = n;
     int m = n;
+    if (wide) {
+        while (m % 16 == 0 && f.nf < kMaxFactors) { f.r[f.nf++] = 16; m /= 16; }
+        while (m % 12 == 0 && f.nf < kMaxFactors) { f.r[f.nf++] = 12; m /= 12; }
+    }
     while (m % 8 == 0 && f.nf < kMaxFactors) { f.r[f.nf++] = 8; m /= 8; }
     while (m % 4 == 0 && f.nf < kMaxFactors) { f.r[f.nf++] = 4; m /= 4; }
     const int primes[4] = {2, 3, 5, 7};
@@ -71,10 +77,58 @@ __host__ __device__ inline float2 cmulf(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+template <int R>
+__host__ __device__ inline void dft_small(float2 (&v)[R], float sgn);
+
+// Length A*B DFT as B length-A DFTs of the stride-B subsequences, the
+// factors exp(sgn*2*pi*i*j2*q1/(A*B)) and A length-B DFTs:
+// X_{q1 + A q2} = sum_j2 w^(j2 q1) [sum_j1 v_{B j1 + j2} w_A^(j1 q1)] w_B^(j2 q2).
+// cs/sn: cos/sin(2*pi*m/(A*B)) for m <= (A-1)(B-1).
+template <int A, int B>
+__host__ __device__ __forceinline__ void dft_split(float2 (&v)[A * B], float sgn, const float* cs, const float* sn) {
+    float2 y[B][A];
+#pragma unroll
+    for (int j2 = 0; j2 < B; ++j2) {
+#pragma unroll
+        for (int j1 = 0; j1 < A; ++j1) y[j2][j1] = v[B * j1 + j2];
+        dft_small<A>(y[j2], sgn);
+    }
+#pragma unroll
+    for (int j2 = 1; j2 < B; ++j2)
+#pragma unroll
+        for (int q1 = 1; q1 < A; ++q1) {
+            const int m = j2 * q1;
+            const float c = cs[m], sw = sgn * sn[m];
+            const float2 t = y[j2][q1];
+            y[j2][q1] = make_float2(t.x * c - t.y * sw, t.x * sw + t.y * c);
+        }
+#pragma unroll
+    for (int q1 = 0; q1 < A; ++q1) {
+        float2 z[B];
+#pragma unroll
+        for (int j2 = 0; j2 < B; ++j2) z[j2] = y[j2][q1];
+        dft_small<B>(z, sgn);
+#pragma unroll
+        for (int q2 = 0; q2 < B; ++q2) v[q1 + A * q2] = z[q2];
+    }
+}
+
 // In-register length-R DFT: X_q = sum_j v_j exp(sgn*2*pi*i*j*q/R).
 template <int R>
 __host__ __device__ inline void dft_small(float2 (&v)[R], float sgn) {
-    if constexpr (R == 8) {
+    if constexpr (R == 16) {
+        // 4 x 4; cos/sin(2*pi*m/16), m = 0..9
+        const float c1 = 0.92387953251128676f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
+        const float cs[10] = {1.f, c1, h, s1, 0.f, -s1, -h, -c1, -1.f, -c1};
+        const float sn[10] = {0.f, s1, h, c1, 1.f, c1, h, s1, 0.f, -s1};
+        dft_split<4, 4>(v, sgn, cs, sn);
+    } else if constexpr (R == 12) {
+        // 4 x 3; cos/sin(2*pi*m/12), m = 0..6
+        const float r = 0.86602540378443865f;
+        const float cs[7] = {1.f, r, 0.5f, 0.f, -0.5f, -r, -1.f};
+        const float sn[7] = {0.f, 0.5f, r, 1.f, r, 0.5f, 0.f};
+        dft_split<4, 3>(v, sgn, cs, sn);
+    } else if constexpr (R == 8) {
         // two length-4 DFTs (even / odd inputs) and the w8^q factors,
         // w8 = exp(sgn*2*pi*i/8) = (1 + sgn*i)/sqrt(2)
         float2 e[4] = {v[0], v[2], v[4], v[6]};
@@ -225,15 +279,19 @@ __device__ __forceinline__ void fft_pass(const float2* __restrict__ a, float2* _
 
 // Butterflies a thread holds in registers in a single-buffer pass of radix
 // R: 8 complex values at most, 16 in kernels with 16 interleaved transforms
-// (which run at fewer waves per SIMD).
-__host__ __device__ constexpr int fft_inplace_maxb(int R, int T) { return (T >= 16 ? 16 : 8) / R; }
+// (which run at fewer waves per SIMD); at least one butterfly (radix 12 and
+// 16 take one).
+__host__ __device__ constexpr int fft_inplace_maxb(int R, int T) {
+    return (T >= 16 ? 16 : 8) / R > 0 ? (T >= 16 ? 16 : 8) / R : 1;
+}
 
 // Whether every pass of the transform (length n, T interleaved transforms)
-// can run in a single buffer: each thread's butterflies (at most cap / R)
+// can run in a single buffer: each thread's butterflies (at most cap / R, at
+// least one)
 // fit its registers; cap = 16 for T >= 16, else 8 (fft_inplace_maxb).
 inline bool fft_inplace_ok(const FftFactors& f, int T, int cap) {
     for (int s = 0; s < f.nf; ++s)
-        if ((f.n / f.r[s]) * T > (cap / f.r[s]) * kFftThreads) return false;
+        if ((f.n / f.r[s]) * T > (cap / f.r[s] > 0 ? cap / f.r[s] : 1) * kFftThreads) return false;
     return true;
 }
 
@@ -288,7 +346,7 @@ __device__ __forceinline__ void fft_pass_inplace(float2* __restrict__ a, int n, 
 // a (the caller checked fft_inplace_ok), else the passes ping-pong between a
 // and b.  All threads of the block take part; returns the buffer holding the
 // result.  The caller must __syncthreads() after filling `a` (and tw).
-template <int T, int SGN, bool BIG = false>
+template <int T, int SGN, bool BIG = false, bool WIDE = false>
 __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const FftFactors& f, const float2* tw, int s0,
                                                  int s1) {
     const int n = f.n;
@@ -300,6 +358,8 @@ __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const Fft
         if (b == nullptr) {
             if (s == 0) {
                 switch (R) {
+                    case 16: if constexpr (WIDE) fft_pass_inplace<16, T, SGN, false, BIG>(a, n, p, twp); break;
+                    case 12: if constexpr (WIDE) fft_pass_inplace<12, T, SGN, false, BIG>(a, n, p, twp); break;
                     case 8: fft_pass_inplace<8, T, SGN, false, BIG>(a, n, p, twp); break;
                     case 4: fft_pass_inplace<4, T, SGN, false, BIG>(a, n, p, twp); break;
                     case 2: fft_pass_inplace<2, T, SGN, false, BIG>(a, n, p, twp); break;
@@ -309,6 +369,8 @@ __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const Fft
                 }
             } else {
                 switch (R) {
+                    case 16: if constexpr (WIDE) fft_pass_inplace<16, T, SGN, true, BIG>(a, n, p, twp); break;
+                    case 12: if constexpr (WIDE) fft_pass_inplace<12, T, SGN, true, BIG>(a, n, p, twp); break;
                     case 8: fft_pass_inplace<8, T, SGN, true, BIG>(a, n, p, twp); break;
                     case 4: fft_pass_inplace<4, T, SGN, true, BIG>(a, n, p, twp); break;
                     case 2: fft_pass_inplace<2, T, SGN, true, BIG>(a, n, p, twp); break;
@@ -323,6 +385,8 @@ __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const Fft
         }
         if (s == 0) {
             switch (R) {
+                case 16: if constexpr (WIDE) fft_pass<16, T, SGN, false, BIG>(a, b, n, p, twp); break;
+                case 12: if constexpr (WIDE) fft_pass<12, T, SGN, false, BIG>(a, b, n, p, twp); break;
                 case 8: fft_pass<8, T, SGN, false, BIG>(a, b, n, p, twp); break;
                 case 4: fft_pass<4, T, SGN, false, BIG>(a, b, n, p, twp); break;
                 case 2: fft_pass<2, T, SGN, false, BIG>(a, b, n, p, twp); break;
@@ -332,6 +396,8 @@ __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const Fft
             }
         } else {
             switch (R) {
+                case 16: if constexpr (WIDE) fft_pass<16, T, SGN, true, BIG>(a, b, n, p, twp); break;
+                case 12: if constexpr (WIDE) fft_pass<12, T, SGN, true, BIG>(a, b, n, p, twp); break;
                 case 8: fft_pass<8, T, SGN, true, BIG>(a, b, n, p, twp); break;
                 case 4: fft_pass<4, T, SGN, true, BIG>(a, b, n, p, twp); break;
                 case 2: fft_pass<2, T, SGN, true, BIG>(a, b, n, p, twp); break;
@@ -350,9 +416,9 @@ __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const Fft
 }
 
 // Full transform in LDS (fft_lds_range over every pass).
-template <int T, int SGN, bool BIG = false>
+template <int T, int SGN, bool BIG = false, bool WIDE = false>
 __device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactors& f, const float2* tw) {
-    return fft_lds_range<T, SGN, BIG>(a, b, f, tw, 0, f.nf);
+    return fft_lds_range<T, SGN, BIG, WIDE>(a, b, f, tw, 0, f.nf);
 }
 
 // First pass (radix R, P = 1, no twiddles) with its inputs taken from
@@ -387,9 +453,11 @@ __device__ __forceinline__ void fft_first_pass_src(float2* __restrict__ a, int n
     }
 }
 
-template <int T, int SGN, class Src>
+template <int T, int SGN, bool WIDE = false, class Src>
 __device__ __forceinline__ void fft_first_pass_src(int R, float2* a, int n, Src src) {
     switch (R) {
+        case 16: if constexpr (WIDE) fft_first_pass_src<16, T, SGN>(a, n, src); break;
+        case 12: if constexpr (WIDE) fft_first_pass_src<12, T, SGN>(a, n, src); break;
         case 8: fft_first_pass_src<8, T, SGN>(a, n, src); break;
         case 4: fft_first_pass_src<4, T, SGN>(a, n, src); break;
         case 2: fft_first_pass_src<2, T, SGN>(a, n, src); break;
@@ -441,9 +509,11 @@ __device__ __forceinline__ void fft_last_pass_dst(const float2* __restrict__ a, 
     }
 }
 
-template <int T, int SGN, bool BIG, class Dst>
+template <int T, int SGN, bool BIG, bool WIDE = false, class Dst>
 __device__ __forceinline__ void fft_last_pass_dst(int R, const float2* a, int n, int p, const float2* twp, Dst dst) {
     switch (R) {
+        case 16: if constexpr (WIDE) fft_last_pass_dst<16, T, SGN, BIG>(a, n, p, twp, dst); break;
+        case 12: if constexpr (WIDE) fft_last_pass_dst<12, T, SGN, BIG>(a, n, p, twp, dst); break;
         case 8: fft_last_pass_dst<8, T, SGN, BIG>(a, n, p, twp, dst); break;
         case 4: fft_last_pass_dst<4, T, SGN, BIG>(a, n, p, twp, dst); break;
         case 2: fft_last_pass_dst<2, T, SGN, BIG>(a, n, p, twp, dst); break;

@@ -434,12 +434,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
 // input G2 rows, output rows i2 < N2 read from (i2 - N2/2) mod G2.  In place
 // in `ws`.
 template <int BCT, bool BIG, int ADJ>
-__global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void nu_cols(const NuSliceDev* __restrict__ sl,
                                                const int4* __restrict__ work,
                                                const FftFactors* __restrict__ facs,
                                                const float2* __restrict__ twp,
                                                float2* __restrict__ ws, const int* __restrict__ skip) {
     constexpr int T = cols_T(BCT, BIG), LT = clog2(T), CL = cols_CL(BCT), LCL = clog2(CL), CPW = T / CL;
+    // passes of 16 transforms run the radix-16/12 factorization (fac2)
+    constexpr bool kWide = T >= 16 && !BIG;
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
     if (skip && skip[wk.x]) return;
@@ -475,11 +477,11 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
             const float2 v = g[((int64_t)r * G1 + c0 + min(col, ncol - 1)) * BCT + (t & (CL - 1))];
             return make_float2(v.x * ok, v.y * ok);
         };
-        fft_first_pass_src<T, SG>(F.r[0], A, G2, src);
+        fft_first_pass_src<T, SG, kWide>(F.r[0], A, G2, src);
         // the pass twiddles after the data loads, which they then do not delay
         for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
         __syncthreads();
-        const float2* Rb = fft_lds_range<T, SG, BIG>(A, Bb, F, tw, 1, F.nf - 1);
+        const float2* Rb = fft_lds_range<T, SG, BIG, kWide>(A, Bb, F, tw, 1, F.nf - 1);
         int p = 1;
         for (int k = 0; k + 1 < F.nf; ++k) p *= F.r[k];
         auto dst = [&](int q, int t, float2 v) {
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
             }
             if (keep) g[((int64_t)r * G1 + c0 + col) * BCT + (t & (CL - 1))] = v;
         };
-        fft_last_pass_dst<T, SG, BIG>(F.r[F.nf - 1], Rb, G2, p, tw + (p - 1), dst);
+        fft_last_pass_dst<T, SG, BIG, kWide>(F.r[F.nf - 1], Rb, G2, p, tw + (p - 1), dst);
         return;
     }
     // four loads per thread in flight, from clamped (valid) addresses; a
@@ -537,7 +539,7 @@ __global__ __launch_bounds__(256) void nu_cols(const NuSliceDev* __restrict__ sl
     // the pass twiddles after the data loads, which they then do not delay
     for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
     __syncthreads();
-    const float2* R = fft_lds<T, ADJ ? -1 : 1, BIG>(A, Bb, facs[S.fac2], tw);
+    const float2* R = fft_lds<T, ADJ ? -1 : 1, BIG, kWide>(A, Bb, facs[S.fac2], tw);
     if (!ADJ) {
         const int total = G2 * T;
         for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
@@ -1621,16 +1623,17 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
 
     std::vector<FftFactors> facs;
     std::vector<float2> twid;
-    std::map<int, std::pair<int, int>> fft_of;  // G -> (factor idx, twiddle offset)
-    auto fft_for = [&](int G) {
-        auto it = fft_of.find(G);
+    // (G, wide) -> (factor idx, twiddle offset)
+    std::map<std::pair<int, bool>, std::pair<int, int>> fft_of;
+    auto fft_for = [&](int G, bool wide) {
+        auto it = fft_of.find({G, wide});
         if (it != fft_of.end()) return it->second;
         FftFactors f;
-        if (!fft_factorize(G, f)) fail(MM_ERR_ARG, "nufftOp: grid size with a prime factor > 7");
+        if (!fft_factorize(G, f, wide)) fail(MM_ERR_ARG, "nufftOp: grid size with a prime factor > 7");
         std::pair<int, int> r((int)facs.size(), (int)twid.size());
         facs.push_back(f);
         fft_pass_twiddles(f, [&](float c, float sn) { twid.push_back(make_float2(c, sn)); });
-        fft_of[G] = r;
+        fft_of[{G, wide}] = r;
         return r;
     };
     std::vector<float> deap;
@@ -1657,12 +1660,15 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         d.M = (int)M[s];
         d.W1 = kwidth;
         d.W2 = ndim == 2 ? kwidth : 1;
-        auto f1 = fft_for(d.G1);
-        auto f2 = fft_for(d.G2);
+        auto f1 = fft_for(d.G1, false);
+        auto f2 = fft_for(d.G2, false);
+        auto f2w = fft_for(d.G2, true);
         d.fac1 = f1.first;
         d.tw1 = f1.second;
         d.fac2 = f2.first;
         d.tw2 = f2.second;
+        d.fac2w = f2w.first;
+        d.tw2w = f2w.second;
         // deapodisation with 1/sqrt(G1*G2) and the ramp constant folded in
         const double sgn = (((n1 / 2) + (ndim == 2 ? n2 / 2 : 0)) & 1) ? -1.0 : 1.0;
         const double scl = sgn / std::sqrt((double)d.G1 * d.G2);
@@ -2264,9 +2270,25 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         d.cpw = cpw;
         d.lcl = ilog2(cl);
         {
+            // column passes of 16 transforms use the radix-16/12
+            // factorization where it is made of those radices only and
+            // takes fewer passes (192 = 16 x 12: two instead of three;
+            // 0.93 -> 0.77 us per 8-coil slice)
+            bool wide = !big_ && TC >= 16;
+            if (wide) {
+                FftFactors fw, fn;
+                fft_factorize(d.G2, fw, true);
+                fft_factorize(d.G2, fn, false);
+                for (int k = 0; k < fw.nf; ++k) wide = wide && (fw.r[k] == 16 || fw.r[k] == 12);
+                wide = wide && fw.nf < fn.nf;
+            }
+            if (wide) {
+                d.fac2 = d.fac2w;
+                d.tw2 = d.tw2w;
+            }
             FftFactors f1, f2;
             fft_factorize(d.G1, f1);
-            fft_factorize(d.G2, f2);
+            fft_factorize(d.G2, f2, wide);
             const bool ip1 = fft_inplace_ok(f1, TR, TR >= 16 ? 16 : 8), ip2 = fft_inplace_ok(f2, TC, TC >= 16 ? 16 : 8);
             d.fip = (ip1 ? 1 : 0) | (ip2 ? 2 : 0);
             c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR, ip1));

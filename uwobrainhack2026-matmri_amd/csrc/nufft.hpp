@@ -21,6 +21,8 @@ struct NuSliceDev {
     int frpw;           // rows per work-group of the forward row FFT
     int lfl, lcl;       // log2 transforms (coils) per work-group: forward rows, columns
     int fip;            // bit 0: row FFTs, bit 1: column FFTs in a single LDS buffer
+    int fac2w, tw2w;    // the column length's radix-16/12 factorization (passes of
+                        // 16 transforms): used for fac2/tw2 when the column pass has them
     int64_t samp_off;   // first sample of this slice in the sample pools
     int64_t tlist_off;  // first entry of this slice in the tile sample list
     int64_t img_off;    // image-side offset (elements) for the current apply
