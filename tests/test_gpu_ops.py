@@ -41,6 +41,10 @@ NUFFT_CASES = [
     ((48, 40), 2000, 4, 1.375, False, 20),
     ((64,), 300, 6, 1.5, True, 2),
     ((96, 110), 5000, 8, 2.0, False, 1),
+    # 192-point columns of 16 transforms: the radix 16 x 12 column passes
+    # (12 reps: a partly filled 16-transform chunk; 8 reps: two columns of 8)
+    ((64, 128), 4000, 4, 1.5, True, 12),
+    ((48, 128), 3000, 6, 1.5, False, 8),
     # grid sides above 1024 (1080 x 144; 1-D 2100): FFT index arithmetic at large n
     ((704, 96), 6000, 4, 1.5, False, 2),
     ((1400,), 800, 6, 1.5, True, 3),
