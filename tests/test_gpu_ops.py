@@ -306,8 +306,8 @@ def test_nufft_tuning_options_and_autotune(mm):
         E.set_option("nu_stage_samples", 4096)
     chosen = E.autotune(3)
     print(f"  autotune chose {chosen}", flush=True)
-    assert (chosen["nu_stage_samples"], chosen["nu_stage_entries"]) in [(96, 768), (128, 1024), (192, 1536),
-                                                                          (256, 2048)]
+    assert (chosen["nu_stage_samples"], chosen["nu_stage_entries"]) in [(64, 512), (96, 768), (128, 1024),
+                                                                          (192, 1536), (256, 2048)]
     assert chosen["nu_degrid_block"] in (32, 64, 128)
     assert chosen["nu_pair_cols"] in (0, 1) and chosen["nu_xcd_order"] in (0, 1)
     assert chosen["nu_grid_pairs"] in (0, 1) and chosen["nu_snake"] in (0, 1)

@@ -2191,7 +2191,8 @@ bool NufftOp::autotune_sense(const float2* maps, int C, const float* W, int iter
         for (const auto& kv : cands[bi]) set_option(kv.first, kv.second);
     };
     tune({{{"nu_grid_pairs", 1}}, {{"nu_grid_pairs", 0}}}, {"nu_grid"});
-    tune({{{"nu_stage_samples", 96}, {"nu_stage_entries", 768}},
+    tune({{{"nu_stage_samples", 64}, {"nu_stage_entries", 512}},
+          {{"nu_stage_samples", 96}, {"nu_stage_entries", 768}},
           {{"nu_stage_samples", 128}, {"nu_stage_entries", 1024}},
           {{"nu_stage_samples", 192}, {"nu_stage_entries", 1536}},
           {{"nu_stage_samples", 256}, {"nu_stage_entries", 2048}}},
