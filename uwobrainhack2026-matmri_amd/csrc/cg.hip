@@ -88,8 +88,8 @@ __device__ __forceinline__ double wave_total(double v) {
     return v;
 }
 
-// block -> (slice, index within slice) via the segment table
-__device__ __forceinline__ int find_slice(const CgSeg* seg, const int* blk2slice) {
+// block -> slice via the block-to-slice table
+__device__ __forceinline__ int find_slice(const CgSeg*, const int* blk2slice) {
     return blk2slice[blockIdx.x];
 }
 
@@ -446,7 +446,6 @@ CgSolver::CgSolver(mm_op* E) : E_(E) {
     const int ns = E->nslice();
     std::vector<CgSeg> seg(ns);
     std::vector<int> b2s;
-    int64_t tot = 0;
     for (int s = 0; s < ns; ++s) {
         const int64_t off = E->slice_image_off(s);
         const int64_t end = s + 1 < ns ? E->slice_image_off(s + 1) : E->n_image();
@@ -457,7 +456,6 @@ CgSolver::CgSolver(mm_op* E) : E_(E) {
         seg[s].nblk = (int)std::max<int64_t>(1, std::min<int64_t>(want, kCgMaxBlocksPerSlice));
         seg[s].blk0 = (int)b2s.size();
         for (int b = 0; b < seg[s].nblk; ++b) b2s.push_back(s);
-        tot += seg[s].len;
     }
     nblk_ = (int)b2s.size();
     n_ = E->n_image();

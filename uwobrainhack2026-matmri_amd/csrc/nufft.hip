@@ -1315,7 +1315,7 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
             if (q0 + k * SPR < T.n_stage) sv[(q0 + k * SPR) * TT + jb] = v[k];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (threadIdx.x + u * 256 < T.n_ent) se[threadIdx.x + u * 256] = e[u];
+            if ((int)threadIdx.x + u * 256 < T.n_ent) se[threadIdx.x + u * 256] = e[u];
         for (int q = q0 + NPRE * SPR; q < T.n_stage; q += SPR) sv[q * TT + jb] = ych[(int64_t)sid[q] * BCT];
         for (int i = threadIdx.x + 4 * 256; i < T.n_ent; i += 256) se[i] = te[i];
     }
