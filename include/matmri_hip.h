@@ -18,6 +18,7 @@
  *   mm_hio_plan_create      sampHighOrder(b0,sampTimes,phs_spha,phs_conc,phs_grid,b0mask,...)
  *                           operators/@sampHighOrder/sampHighOrder.m:129-291, :496-588
  *   mm_hio_set_phi_div      sampHighOrder.setPhiDiv  sampHighOrder.m:590-622
+ *   mm_nufft_set_toeplitz   nufftOp.prepToep          nufftOp.m:410-432 (applied :348-352)
  *   mm_sense_create         mrSampFunc(x,transp,samplingOp,R,opt)  operators/mrSampFunc.m:82-159
  *   mm_op_apply             A*x / A'*y  (mtimes/ctranspose overloads:
  *                           nufftOp.m:276-408, rcvrOp.m:96-159,
@@ -97,6 +98,15 @@ int mm_hio_plan_create(mm_ctx* ctx, int64_t V, int64_t K, int Lh, int Lc,
 /* setPhiDiv: replaces the forward by (1i*exp(1i*Phi).*dPhi).' * x / sqrt(V)
  * with dPhi from hd [Lh x K] and gd [Lc x K] (no B0 term); NULL,NULL clears. */
 int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd);
+/* prepToep (operators/@nufftOp/nufftOp.m:410-432, applied at :348-352): turns
+ * a NUFFT plan whose grid is at least 2N - 1 per side (os = 2) into the
+ * Toeplitz form of its normal operator.  psf: host complex64 (interleaved),
+ * per slice the point-spread image S2'*(dcf or ones) of a NUFFT on the doubled
+ * image size (2 N1 x 2 N2, column-major), slices concatenated.  Afterwards
+ * both directions of mm_op_apply map an image to an image (n_data = n_image):
+ * crop(fftnc(convMat .* ifftnc(pad(x)))), convMat = 2^ndim ifftnc(psf); a
+ * SENSE operator built on it has its normal operator R'(T(R x)) only. */
+int mm_nufft_set_toeplitz(mm_op* op, const float* psf);
 
 /* SENSE encoding E = S o R with optional real data weights W ([M x C], used
  * only by mm_cgne_solve and by mm_op_apply_normal). */

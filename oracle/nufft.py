@@ -24,6 +24,12 @@ Pinned by (``tests/test_oracle_nufft.py``):
 * a brute-force non-uniform DFT with the "+" sign and centred image index,
   which bounds the gridding approximation (and fails for a wrong sign, a
   wrong centring or a wrong scale).
+
+:class:`ToepPlan` (``prepToep``, ``nufftOp.m:410-432``, applied at
+``:348-352``) is pinned by the reference's own Toeplitz test (against
+``S0'*(S0*x)`` with os = 2, W = 8, rms < 1e-3, ``@nufftOp/tests.m:104-124``)
+and by the exact normal operator ``E' diag(dcf^2) E`` of the brute-force
+NUDFT (fails for dcf applied once, a conjugated kernel or a wrong scale).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -230,3 +236,54 @@ def nudft_centred(imN, kloc, x):
     E = np.exp(1j * ph) / np.sqrt(np.prod(imN))
     xa = np.asarray(x, dtype=np.complex128).reshape(int(np.prod(imN)), -1, order="F")
     return E @ xa
+
+
+def padcrop(x, szIn):
+    """``general/padcrop.m:1-93`` over the leading ``len(szIn)`` dimensions:
+    a longer size zero-pads with ``floor(npc/2)`` before the data, a shorter
+    one keeps the entries from ``floor(npc/2)`` on (``npc`` = size change)."""
+    y = np.asarray(x)
+    for d, n in enumerate(szIn):
+        n = int(n)
+        npc = n - y.shape[d]
+        if npc > 0:
+            pad = [(0, 0)] * y.ndim
+            pad[d] = (npc // 2, npc - npc // 2)
+            y = np.pad(y, pad)
+        elif npc < 0:
+            idx = [slice(None)] * y.ndim
+            idx[d] = slice((-npc) // 2, (-npc) // 2 + n)
+            y = y[tuple(idx)]
+    return y
+
+
+class ToepPlan:
+    """``S.prepToep(tos, kwidth)`` (``nufftOp.m:410-432``) and the Toeplitz
+    application of ``nufftOp.m:348-352``: ``S * x`` (and ``S' * x``) becomes
+    ``padcrop(fftnc(convMat .* ifftnc(padcrop(x, 2N))), N)`` with
+    ``convMat = 2^nd * ifftnc(S2' * dcf)`` (``S2' * ones`` without dcf) and
+    ``S2 = nufftOp(2N, kloc, dcf, ., tos, kwidth)``; ``tos`` defaults to
+    ``0.5 (os - 1) + 1`` and ``kwidth`` to the operator's (``:412-419``)."""
+
+    def __init__(self, imN, kloc, dcf=None, os=1.5, kwidth=6, tos=None, tkwidth=None):
+        self.imN = [int(n) for n in np.atleast_1d(imN)]
+        nd = len(self.imN)
+        if tos is None:
+            tos = 0.5 * (os - 1) + 1                                  # :414
+        if tkwidth is None:
+            tkwidth = kwidth                                          # :416-418
+        S = NufftPlan([2 * n for n in self.imN], kloc, dcf, tos, tkwidth)   # :419
+        w = np.ones(S.M) if dcf is None else np.asarray(dcf, np.float64).reshape(-1)
+        psf = S.adjoint(w)                                            # :420-424 (dcf applied again inside)
+        self.convMat = (2 ** nd) * ifftnc(psf, nd)                    # :425
+
+    def apply(self, x):
+        """``S * x`` of the prepared operator (``nufftOp.m:348-352``)."""
+        nd = len(self.imN)
+        x = np.asarray(x, dtype=np.complex128)
+        reps = x.shape[nd:]
+        nrep = int(np.prod(reps)) if reps else 1
+        xa = x.reshape(tuple(self.imN) + (nrep,), order="F")
+        z = ifftnc(padcrop(xa, [2 * n for n in self.imN]), nd)
+        z = fftnc(self.convMat[..., None] * z, nd)
+        return padcrop(z, self.imN).reshape(tuple(self.imN) + reps, order="F")

@@ -9,7 +9,7 @@ import pytest
 from conftest import load_pkg
 from oracle.highorder import HighOrderPlan
 from oracle.mrsamp import mrSampFunc as o_mrSampFunc
-from oracle.nufft import NufftPlan
+from oracle.nufft import NufftPlan, ToepPlan
 from oracle.rcvr import RcvrPlan
 
 pytestmark = pytest.mark.gpu
@@ -274,6 +274,52 @@ def test_grid_pairs_vs_single(mm, nslice, nrep):
     print(f"  slice 0 vs oracle {e_o:.2e}; equal: {np.array_equal(a_pair, a_one)}", flush=True)
     assert e_o < TOL
     assert np.array_equal(a_pair, a_one)
+
+
+@pytest.mark.parametrize("nD,with_dcf,nrep", [(2, True, 2), (2, False, 1), (1, True, 1)])
+def test_toeplitz_vs_oracle(mm, nD, with_dcf, nrep):
+    """prepToep (nufftOp.m:410-432): both directions of the prepared operator
+    against the oracle's Toeplitz form, and within the reference's own bound
+    of the accurate normal operator S0'*(S0*x) (os 2, W 8: rms < 1e-3,
+    @nufftOp/tests.m:104-124)."""
+    print(f"[case] Toeplitz nD={nD} dcf={with_dcf} nrep={nrep}", flush=True)
+    rng = np.random.default_rng(50 + nD + 2 * nrep)
+    imN = [64] * nD
+    kloc = rng.random((int(0.25 * np.prod(imN)), nD)) - 0.5
+    dcf = rng.random(kloc.shape[0]) if with_dcf else None
+    T = mm.nufftOp(imN, kloc, dcf, 1, 1.5, 6).prepToep()
+    x = crand(rng, tuple(imN) + ((nrep,) if nrep > 1 else ()))
+    got, got_h = np.asarray(T * x), np.asarray(T.H * x)
+    want = ToepPlan(imN, kloc, dcf, 1.5, 6).apply(x)
+    S0 = NufftPlan(imN, kloc, dcf, 2, 8)
+    rms = np.sqrt(np.mean(np.abs(got - S0.adjoint(S0.forward(x))) ** 2))
+    e, e_h = rel(got, want), rel(got_h, want)
+    print(f"  vs oracle {e:.2e} / adjoint {e_h:.2e}; rms vs S0'S0 {rms:.2e}", flush=True)
+    assert e < TOL and e_h < TOL
+    assert rms < 1e-3
+
+
+def test_toeplitz_sense_normal_multislice(mm):
+    """A SENSE operator on a two-slice Toeplitz NUFFT (different trajectories
+    and dcf): E'E x = R'(T(R x)) per slice against the oracle."""
+    print("[case] Toeplitz SENSE normal, 2 slices 48x48 C=4", flush=True)
+    rng = np.random.default_rng(57)
+    N, C = 48, 4
+    kls = [rng.random((900, 2)) - 0.5 for _ in range(2)]
+    dcfs = [0.5 + rng.random(900) for _ in range(2)]
+    maps = [crand(rng, (N, N, C)) for _ in range(2)]
+    T = mm.nufftOp.batch([([N, N], kl, d) for kl, d in zip(kls, dcfs)], 1.5, 4).prepToep()
+    E = mm.senseOp(T, mm.rcvrOp.batch(maps))
+    xs = [crand(rng, (N, N)) for _ in range(2)]
+    xcat = np.concatenate([x.reshape(-1, order="F") for x in xs])
+    got = np.asarray(E.normal(xcat)).reshape(-1, order="F")
+    want = []
+    for x, kl, d, mp in zip(xs, kls, dcfs, maps):
+        Tp = ToepPlan([N, N], kl, d, 1.5, 4)
+        want.append(sum(np.conj(mp[..., c]) * Tp.apply(mp[..., c] * x) for c in range(C)).reshape(-1, order="F"))
+    e = rel(got, np.concatenate(want))
+    print(f"  vs oracle {e:.2e}", flush=True)
+    assert e < TOL
 
 
 def test_nufft_tuning_options_and_autotune(mm):

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle.fftnc import fftnc, ifftnc
-from oracle.nufft import NufftPlan, grid_size, kb_beta, kb_nkern, nudft_centred
+from oracle.nufft import NufftPlan, ToepPlan, grid_size, kb_beta, kb_nkern, nudft_centred, padcrop
 
 
 # -- closed-form plan constants (nufftOp.m:146-156) -----------------------------
@@ -176,3 +176,47 @@ def test_ifftnc_fftnc_roundtrip_unitary():
     z = np.zeros((16, 24), complex)
     z[:12, :18] = a[:, :, 0]
     assert np.allclose(p, np.fft.ifft2(z) * np.sqrt(16 * 24), atol=1e-13)
+
+
+@pytest.mark.parametrize("nD", [1, 2])
+def test_toeplitz_vs_accurate_normal(nD):
+    """The reference's Toeplitz test (@nufftOp/tests.m:104-124): prepToep of an
+    os = 1.5, W = 6 operator against S0'*(S0*x) with os = 2, W = 8, rms < 1e-3."""
+    rng = np.random.default_rng(40 + nD)
+    imN = [64] * nD
+    kloc = rng.random((int(0.25 * np.prod(imN)), nD)) - 0.5
+    dcf = rng.random(kloc.shape[0])
+    T = ToepPlan(imN, kloc, dcf, 1.5, 6)
+    x = rng.standard_normal(tuple(imN) + (2,)) + 1j * rng.standard_normal(tuple(imN) + (2,))
+    S0 = NufftPlan(imN, kloc, dcf, 2, 8)
+    d = S0.adjoint(S0.forward(x)) - T.apply(x)
+    assert np.sqrt(np.mean(np.abs(d) ** 2)) < 1e-3
+
+
+def test_toeplitz_vs_exact_nudft_normal():
+    """Brute force: with an accurate point-spread NUFFT (tos = 2, W = 8) the
+    Toeplitz form equals the exact normal operator E' diag(dcf^2) E of the
+    centred NUDFT (dcf applied in both directions, nufftOp.m:356-358 and
+    :391-393); dcf applied once, a conjugated kernel or a wrong scale fail."""
+    rng = np.random.default_rng(44)
+    imN = [12, 10]
+    kloc = rng.random((90, 2)) - 0.5
+    dcf = 0.5 + rng.random(90)
+    x = rng.standard_normal(tuple(imN)) + 1j * rng.standard_normal(tuple(imN))
+    E = nudft_centred(imN, kloc, np.eye(int(np.prod(imN))))
+    want = (E.conj().T @ (dcf ** 2 * (E @ x.reshape(-1, order="F")))).reshape(imN, order="F")
+    got = ToepPlan(imN, kloc, dcf, 1.5, 6, tos=2, tkwidth=8).apply(x)
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert rel < 1e-4
+    once = (E.conj().T @ (dcf * (E @ x.reshape(-1, order="F")))).reshape(imN, order="F")
+    assert np.linalg.norm(got - once) / np.linalg.norm(once) > 1e-2
+
+
+def test_padcrop_round_trip():
+    """padcrop (general/padcrop.m): padding by floor(npc/2) in front and
+    cropping from floor(npc/2) are inverse; odd changes put the extra entry
+    after the data."""
+    a = np.arange(12.0).reshape(3, 4)
+    p = padcrop(a, [6, 7])
+    assert p.shape == (6, 7) and np.array_equal(p[1:4, 1:5], a)
+    assert np.array_equal(padcrop(p, [3, 4]), a)

@@ -19,7 +19,7 @@ MM_OK, MM_ERR_ARG, MM_ERR_OOM, MM_ERR_DEVICE, MM_ERR_HIP, MM_ERR_UNSUPPORTED = r
 EXPORTS = [
     "mm_last_error", "mm_version", "mm_device_count", "mm_ctx_create", "mm_ctx_destroy",
     "mm_ctx_sync", "mm_nufft_plan_create", "mm_rcvr_plan_create", "mm_rcvr_plan_create_multi",
-    "mm_hio_plan_create", "mm_hio_set_phi_div", "mm_sense_create", "mm_op_shape", "mm_op_nslice",
+    "mm_hio_plan_create", "mm_hio_set_phi_div", "mm_nufft_set_toeplitz", "mm_sense_create", "mm_op_shape", "mm_op_nslice",
     "mm_op_apply", "mm_op_apply_normal", "mm_cg_opts_default", "mm_cgne_solve", "mm_cg_set_graph",
     "mm_op_profile", "mm_op_set_option", "mm_op_get_option", "mm_op_autotune", "mm_ctx_stream", "mm_op_destroy", "mm_cg_last_fallback",
     "mm_op_profile_stage",
@@ -73,6 +73,7 @@ def lib():
     L.mm_rcvr_plan_create_multi.argtypes = [vp, i32, P(i64), i64, vp, P(vp)]
     L.mm_hio_plan_create.argtypes = [vp, i64, i64, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, P(vp)]
     L.mm_hio_set_phi_div.argtypes = [vp, vp, vp]
+    L.mm_nufft_set_toeplitz.argtypes = [vp, vp]
     L.mm_sense_create.argtypes = [vp, vp, vp, vp, P(vp)]
     L.mm_op_shape.argtypes = [vp, P(i64), P(i64)]
     L.mm_op_nslice.argtypes = [vp]

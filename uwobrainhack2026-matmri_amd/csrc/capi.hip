@@ -170,6 +170,15 @@ int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd) {
     });
 }
 
+int mm_nufft_set_toeplitz(mm_op* op, const float* psf) {
+    return guarded([&] {
+        auto* n = dynamic_cast<NufftOp*>(op);
+        if (!n) fail(MM_ERR_ARG, "prepToep: not a NUFFT operator");
+        set_device(op->ctx);
+        n->set_toeplitz(reinterpret_cast<const float2*>(psf));
+    });
+}
+
 int mm_sense_create(mm_ctx* ctx, mm_op* S, mm_op* R, const float* W, mm_op** out) {
     return guarded([&] {
         if (!ctx || !out) fail(MM_ERR_ARG, "mrSampFunc: NULL argument");

@@ -650,6 +650,37 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
     }
 }
 
+// Toeplitz normal operator, middle stage: each slice's grid Z[g2][g1][b]
+// (after the forward row and column FFTs) times the spectrum of its
+// point-spread kernel, K[g2][g1], in place.  Grid: (cells * BCT / 1024,
+// slice, chunk); four elements per lane, every load issued before the
+// stores.
+template <int BCT>
+__global__ __launch_bounds__(256) void nu_toep_mul(const NuSliceDev* __restrict__ sl,
+                                                   const float2* __restrict__ km,
+                                                   const int64_t* __restrict__ koff,
+                                                   float2* __restrict__ ws, const int* __restrict__ skip) {
+    const int s = blockIdx.y;
+    if (skip && skip[s]) return;
+    const NuSliceDev& S = sl[s];
+    const int n = (int)S.ws_chunk;
+    const int e0 = (int)blockIdx.x * 1024 + (int)threadIdx.x;
+    if (e0 >= n) return;
+    const float2* K = km + koff[s];
+    float2* z = ws + S.ws_off + (int64_t)blockIdx.z * S.ws_chunk;
+    float2 v[4], k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = min(e0 + u * 256, n - 1);
+        v[u] = z[e];
+        k[u] = K[e / BCT];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (e0 + u * 256 < n)
+            z[e0 + u * 256] = make_float2(v[u].x * k[u].x - v[u].y * k[u].y, v[u].x * k[u].y + v[u].y * k[u].x);
+}
+
 // Degridding from an LDS-staged grid region (2-D): work item = (slice,
 // chunk, sample block); a block holds up to bcap samples whose first
 // tap lies in one kDegridTile x kDegridTile tile (or two such tiles side by
@@ -2784,6 +2815,41 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_adj_rows");
     };
+    if (mode == 2) {
+        // Toeplitz normal operator: forward row and column FFTs, the kernel
+        // spectrum multiply, adjoint column and row FFTs
+        if (c.n_frows > 0)
+            hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(c.n_frows), dim3(256), c.lds_rows, st, sl, c.w_frows.p,
+                               facs_.p, twid_.p, deap_.p, in, c.ws.p, run);
+        LAUNCHCHECK();
+        mark("nu_fwd_rows");
+        if (ndim_ == 2 && c.n_cols > 0) {
+            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_fcols.p,
+                               facs_.p, twid_.p, c.ws.p, run.skip);
+            LAUNCHCHECK();
+            mark("nu_cols");
+        }
+        int64_t mx = 1;
+        for (const NuSliceDev& d : c.hs) mx = std::max(mx, d.ws_chunk);
+        const int nchunk = (int)(c.sense ? c.nrep : (c.nrep + BCT - 1) / BCT);
+        hipLaunchKernelGGL(nu_toep_mul<BCT>, dim3((unsigned)((mx + 1023) / 1024), (unsigned)c.hs.size(), (unsigned)nchunk),
+                           dim3(256), 0, st, sl, (const float2*)toep_m_.p, (const int64_t*)toep_off_.p, c.ws.p,
+                           run.skip);
+        LAUNCHCHECK();
+        mark("nu_toep_mul");
+        if (ndim_ == 2 && c.n_cols > 0) {
+            hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
+                               facs_.p, twid_.p, c.ws.p, run.skip);
+            LAUNCHCHECK();
+            mark("nu_cols");
+        }
+        if (c.n_rows > 0)
+            hipLaunchKernelGGL((nu_adj_rows<BCT, BIG>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p,
+                               facs_.p, twid_.p, deap_.p, (const float2*)c.ws.p, out, run);
+        LAUNCHCHECK();
+        mark("nu_adj_rows");
+        return;
+    }
     if (mode == 1) {
         NuRun rf = run;
         rf.ylayout = gather_ ? 1 : 0;
@@ -2829,7 +2895,13 @@ void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const floa
     r.B = c.nrep;
     r.maps = maps;
     r.W = W;
-    r.skip = mode == 1 ? skip_ : nullptr;
+    if (toep_) {
+        // both directions, and the normal operator, are the Toeplitz form
+        if (W) fail(MM_ERR_UNSUPPORTED, "nufftOp: data weights with the Toeplitz form");
+        mode = mode == 1 ? 3 : 2;
+    }
+    r.skip = mode == 1 || mode == 3 ? skip_ : nullptr;
+    if (mode == 3) mode = 2;
     if (mode == 1 && !gather_) c.ydata.ensure((size_t)(c.sense ? M_total_ * c.C * c.nrep : M_total_ * c.nrep));
     if (big_) {
         switch (c.BCT) {
@@ -2850,6 +2922,78 @@ void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const floa
     }
 }
 
+void NufftOp::set_toeplitz(const float2* psf) {
+    if (!psf) fail(MM_ERR_ARG, "nufftOp: NULL point-spread image");
+    const int ns = (int)sl_.size();
+    hipStream_t st = ctx->stream;
+    std::vector<float> deap(deap_.n);
+    if (deap_.n) HIPCHECK(hipMemcpy(deap.data(), deap_.p, deap_.n * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<float2> km;
+    std::vector<int64_t> koff;
+    int64_t pofs = 0;
+    for (int s = 0; s < ns; ++s) {
+        NuSliceDev& d = sl_[(size_t)s];
+        const int N1 = d.N1, N2 = ndim_ == 2 ? d.N2 : 1, G1 = d.G1, G2 = ndim_ == 2 ? d.G2 : 1;
+        if (G1 < 2 * N1 - 1 || G2 < 2 * N2 - 1) fail(MM_ERR_ARG, "nufftOp: Toeplitz form needs a grid of at least 2N - 1");
+        const int P1 = 2 * N1, P2 = ndim_ == 2 ? 2 * N2 : 1;   // point-spread image size
+        const double alpha = std::pow(2.0, ndim_) / std::sqrt((double)P1 * P2);
+        // h(D1, D2) = alpha psf[N1 + D1, N2 + D2] for |D| <= N - 1 (the only
+        // lags between a pixel and a pixel of the cropped output);
+        // K[g2][g1] = sum_D h(D) exp(+2 pi i (g1 D1 / G1 + g2 D2 / G2)), the
+        // forward passes' sign, separably in FP64
+        const int L1 = 2 * N1 - 1, L2 = 2 * N2 - 1;
+        auto ex = [](int G) {
+            std::vector<std::pair<double, double>> e((size_t)G);
+            for (int k = 0; k < G; ++k) e[(size_t)k] = {std::cos(2 * M_PI * k / G), std::sin(2 * M_PI * k / G)};
+            return e;
+        };
+        const auto e1 = ex(G1), e2 = ex(G2);
+        std::vector<double> ar((size_t)G1 * L2), ai((size_t)G1 * L2);   // [g1][D2]
+        for (int j2 = 0; j2 < L2; ++j2) {
+            const int col = P2 > 1 ? N2 + j2 - (N2 - 1) : 0;   // psf column of D2 = j2 - (N2 - 1)
+            for (int g1 = 0; g1 < G1; ++g1) {
+                double sr = 0, si = 0;
+                for (int j1 = 0; j1 < L1; ++j1) {
+                    const int D1 = j1 - (N1 - 1);
+                    const float2 h = psf[pofs + (int64_t)(N1 + D1) + (int64_t)P1 * col];
+                    const auto& w = e1[(size_t)((((int64_t)g1 * D1) % G1 + G1) % G1)];
+                    sr += h.x * w.first - h.y * w.second;
+                    si += h.x * w.second + h.y * w.first;
+                }
+                ar[(size_t)g1 * L2 + j2] = sr;
+                ai[(size_t)g1 * L2 + j2] = si;
+            }
+        }
+        koff.push_back((int64_t)km.size());
+        const size_t base = km.size();
+        km.resize(base + (size_t)G1 * G2);
+        for (int g2 = 0; g2 < G2; ++g2)
+            for (int g1 = 0; g1 < G1; ++g1) {
+                double sr = 0, si = 0;
+                for (int j2 = 0; j2 < L2; ++j2) {
+                    const int D2 = j2 - (N2 - 1);
+                    const auto& w = e2[(size_t)((((int64_t)g2 * D2) % G2 + G2) % G2)];
+                    const double xr = ar[(size_t)g1 * L2 + j2], xi = ai[(size_t)g1 * L2 + j2];
+                    sr += xr * w.first - xi * w.second;
+                    si += xr * w.second + xi * w.first;
+                }
+                km[base + (size_t)g2 * G1 + g1] = make_float2((float)(alpha * sr), (float)(alpha * si));
+            }
+        pofs += (int64_t)P1 * P2;
+        // flat scaling 1/sqrt(G1 G2) on both sides (unitary FFT pair)
+        d.deap1 = (int)deap.size();
+        for (int i = 0; i < d.N1; ++i) deap.push_back((float)(1.0 / std::sqrt((double)G1 * G2)));
+        d.deap2 = (int)deap.size();
+        for (int i = 0; i < std::max(1, ndim_ == 2 ? d.N2 : 1); ++i) deap.push_back(1.f);
+    }
+    deap_.upload(deap, st);
+    toep_m_.upload(km, st);
+    toep_off_.upload(koff, st);
+    HIPCHECK(hipStreamSynchronize(st));
+    toep_ = true;
+    cfgs_.clear();
+}
+
 void NufftOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
     run(config(nrep, 0, 0), adjoint, in, out, nullptr, nullptr, 0);
@@ -2858,6 +3002,7 @@ void NufftOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
 void NufftOp::apply_sense(int adjoint, const float2* in, float2* out, int64_t nrep,
                           const float2* maps, int C, const float* Wt) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
+    if (toep_) fail(MM_ERR_UNSUPPORTED, "nufftOp: a SENSE operator on the Toeplitz form has its normal operator only");
     run(config(nrep, 1, C), adjoint, in, out, maps, adjoint ? nullptr : Wt);
 }
 

@@ -105,8 +105,18 @@ class NufftOp : public mm_op {
     NufftOp(mm_ctx* ctx, int ndim, int nslice, const int64_t* imN, const int64_t* M,
             const double* const* kloc, const double* const* dcf, double os, int kwidth);
     int64_t n_image() const override { return V_total_; }
-    int64_t n_data() const override { return M_total_; }
+    int64_t n_data() const override { return toep_ ? V_total_ : M_total_; }
     int nslice() const override { return (int)sl_.size(); }
+    // Toeplitz form of the normal operator (nufftOp.prepToep,
+    // operators/@nufftOp/nufftOp.m:410-432, applied at :348-352): `psf`
+    // holds each slice's point-spread image A2'(dcf.^2) on the doubled
+    // image grid (2 N1 x 2 N2, column-major, slices concatenated).  After the
+    // call both directions apply crop(h * pad(x)), h the psf scaled by
+    // 2^ndim / sqrt(prod 2 N_d): row and column FFTs on this plan's grid
+    // (G_d >= 2 N_d), a multiply by the spectrum of h, inverse column and
+    // row FFTs.  The spectrum is formed on the host in FP64.
+    void set_toeplitz(const float2* psf);
+    bool toeplitz() const { return toep_; }
     int64_t slice_image_off(int s) const override { return img_start_[s]; }
     int64_t slice_data_off(int s) const override { return dat_start_[s]; }
     void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
@@ -193,6 +203,9 @@ class NufftOp : public mm_op {
     template <int BCT, bool BIG>
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
     bool big_ = false;
+    bool toep_ = false;
+    DevBuf<float2> toep_m_;      // per slice the spectrum of h on the grid, Z[g2][g1] order
+    DevBuf<int64_t> toep_off_;   // per slice: first entry in toep_m_
     const int* skip_ = nullptr;   // per-slice skip flags of the normal operator   // a grid side above kFftSmallMax: FFT kernels with the wide modulo
     // event marks for profile_normal (inactive unless prof_ is set)
     struct Prof {

@@ -223,6 +223,33 @@ class nufftOp(LinearOp):
         else:
             self.image_shape = (int(sum(np.prod(n) for n in self.imN)),)
             self.data_shape = (int(sum(self.M)),)
+        self.isToep = False
+
+    def prepToep(self, tos=None, kwidth=None):
+        """``prepToep(tos, kwidth)`` (nufftOp.m:410-432): a copy whose ``*``
+        and ``'*`` both apply the normal operator in Toeplitz form
+        (nufftOp.m:348-352), ``crop(fftnc(convMat .* ifftnc(pad(x, 2N))))``
+        with ``convMat = 2^nd ifftnc(S2' * dcf)`` and ``S2 = nufftOp(2N, kloc,
+        dcf, useGPU, tos, kwidth)`` (``tos`` default ``(os - 1) / 2 + 1``,
+        ``kwidth`` default this operator's).  The point-spread image
+        ``S2' * dcf`` (or ``S2' * ones``) is computed by the device NUFFT; the
+        returned operator applies it on a grid of at least 2N per side."""
+        if self.isToep:
+            return self
+        if tos is None:
+            tos = 0.5 * (self.os - 1) + 1
+        if kwidth is None:
+            kwidth = self.kwidth
+        slices = list(zip(self.imN, self._kl, self._dc))
+        S2 = nufftOp.batch([([2 * n for n in imN], kl, dc) for imN, kl, dc in slices], tos, kwidth)
+        w = np.concatenate([np.ones(kl.shape[0]) if dc is None else dc for _, kl, dc in slices])
+        psf = np.asarray(S2.H * w.astype(np.complex64).reshape(S2.data_shape, order="F"))
+        psf = np.ascontiguousarray(psf.reshape(-1, order="F").astype(np.complex64))
+        T = nufftOp.batch(slices, 2.0, self.kwidth)
+        check(_lib.lib().mm_nufft_set_toeplitz(T._plan.h, psf.ctypes.data))
+        T.data_shape = T.image_shape
+        T.isToep = True
+        return T
 
 
 class rcvrOp(LinearOp):
