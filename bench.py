@@ -352,6 +352,59 @@ def nufft_roofline(mm, E, dev, slices, workload, iters=20):
              "algorithmic_bytes_per_launch": byts}, stages)
 
 
+def toeplitz_normal(mm, fam, prob, dev, slices):
+    """The Toeplitz form of the same SENSE normal operator (nufftOp.prepToep,
+    nufftOp.m:410-432: row and column FFTs on a grid of at least 2N per side
+    around a kernel-spectrum multiply, no gridding): per-stage times and
+    applications per second on the workload's slices, and the HBM roofline
+    of its dominant stage (algorithmic bytes: the same per-stage formulas on
+    the Toeplitz grid, the multiply reading and writing the grid once).  The
+    solves use the gridding form: the two differ by the interpolation
+    kernel's approximation (the reference's own bound is rms 1e-3)."""
+    import ctypes as C
+    import torch
+    t0 = time.perf_counter()
+    T = fam.S.prepToep()
+    t_prep = time.perf_counter() - t0
+    E = mm.senseOp(T, prob["R"])
+    ms = C.c_double()
+    name = C.create_string_buffer(64)
+    ni = int(sum(n * n for n, _, _ in slices))
+    xi = torch.zeros(ni, dtype=torch.complex64, device=dev)
+    xo = torch.empty_like(xi)
+    L = mm.lib()
+    if L.mm_op_profile(E._plan.h, 2, xi.data_ptr(), xo.data_ptr(), 1, 20, C.byref(ms), name, 64) != 0:
+        return None
+    stages = {}
+    k = 0
+    while True:
+        sm, sl, sn = C.c_double(), C.c_int(), C.create_string_buffer(64)
+        if L.mm_op_profile_stage(E._plan.h, k, C.byref(sm), C.byref(sl), sn, 64) != 0:
+            break
+        stages[sn.value.decode()] = {"ms_per_launch": sm.value, "launches_per_application": sl.value}
+        k += 1
+    kname = name.value.decode()
+    byts = 0.0
+    for n, m, c in slices:
+        g = mm.workloads.grid_size(n, 2.0)
+        f = nufft_stage_bytes(n, n, g, g, m, c)
+        f["nu_toep_mul"] = 2.0 * 8 * g * g * c          # grid in and out
+        f["nu_cols_toep"] = 2.0 * 8 * g * n * c         # N2 rows in, N2 rows out
+        byts += f.get(kname, 0.0)
+    passes = 2 if kname == "nu_cols" else 1
+    launches = stages.get(kname, {}).get("launches_per_application", passes) or passes
+    byts = byts * passes / launches
+    ach = byts / (ms.value * 1e-3) / 1e9 if byts > 0 else None
+    dom_bytes = byts
+    del E, T
+    torch.cuda.empty_cache()
+    return {"applications_per_s": applications_per_s(stages), "stages": stages, "prep_s": t_prep,
+            "grid": [mm.workloads.grid_size(slices[0][0], 2.0)] * 2,
+            "dominant": {"kernel": kname, "ms_per_launch": ms.value, "algorithmic_bytes_per_launch": dom_bytes,
+                         "achieved": ach, "unit": "GB/s",
+                         "frac": ach / PEAK_HBM_GBS if ach else None}}
+
+
 def applications_per_s(stages):
     """Normal-operator applications (E'E of every slice of the batch) per
     second from the per-stage launch times of one application."""
@@ -501,6 +554,7 @@ def run_warmstart(a, mm, torch, dev):
         out["roofline"] = roof
     out["normal_operator_stages"] = stages
     out["normal_operator_applications_per_s"] = applications_per_s(stages)
+    out["toeplitz_normal"] = toeplitz_normal(mm, fam, prob, dev, [(N, M, Cc)] * P)
     if not a.no_cpu:
         out["nrmse_vs_fp64_oracle"] = oracle_nrmse_warmstart(fam, host, prob, zw["x"], zz["x"])
         cb = cpu_baseline_warmstart(fam, host, prob, a.cpu_seconds)

@@ -297,6 +297,13 @@ def test_toeplitz_vs_oracle(mm, nD, with_dcf, nrep):
     print(f"  vs oracle {e:.2e} / adjoint {e_h:.2e}; rms vs S0'S0 {rms:.2e}", flush=True)
     assert e < TOL and e_h < TOL
     assert rms < 1e-3
+    # the three-kernel middle stage (forward columns, multiply, adjoint
+    # columns) against the fused column kernel
+    assert T.get_option("nu_toep_fused") == 1
+    T.set_option("nu_toep_fused", 0)
+    e_u = rel(np.asarray(T * x), want)
+    print(f"  unfused vs oracle {e_u:.2e}", flush=True)
+    assert e_u < TOL
 
 
 def test_toeplitz_sense_normal_multislice(mm):

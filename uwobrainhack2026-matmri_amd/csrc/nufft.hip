@@ -650,6 +650,65 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
     }
 }
 
+// Toeplitz normal operator, 2-D middle stage in one pass: per column, the
+// forward FFT along g2 of the N2 image rows (zero elsewhere, wrapped as in
+// nu_cols), the multiply by the kernel spectrum K[g2][g1] in LDS, the
+// inverse FFT and the store of the N2 rows the adjoint rows read back; the
+// grid rows beyond N2 never reach HBM.  Items and LDS as nu_cols.
+template <int BCT, bool BIG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void nu_cols_toep(
+    const NuSliceDev* __restrict__ sl, const int4* __restrict__ work, const FftFactors* __restrict__ facs,
+    const float2* __restrict__ twp, const float2* __restrict__ km, const int64_t* __restrict__ koff,
+    float2* __restrict__ ws, const int* __restrict__ skip) {
+    constexpr int T = cols_T(BCT, BIG), LT = clog2(T), CL = cols_CL(BCT), LCL = clog2(CL), CPW = T / CL;
+    constexpr bool kWide = T >= 16 && !BIG;
+    extern __shared__ float2 lds[];
+    const int4 wk = work[blockIdx.x];
+    if (skip && skip[wk.x]) return;
+    const NuSliceDev& S = sl[wk.x];
+    const int ch = wk.y, c0 = wk.z, b0 = wk.w;
+    const int N2 = S.N2, G1 = S.G1, G2 = S.G2;
+    const bool ip = (S.fip >> 1) & 1;
+    float2* A = lds;
+    float2* Bb = ip ? nullptr : lds + G2 * T;
+    float2* tw = lds + (ip ? 1 : 2) * G2 * T;
+    float2* g = ws + S.ws_off + (int64_t)ch * S.ws_chunk + b0;
+    const float2* K = km + koff[wk.x];
+    const int ncol = min(CPW, G1 - c0);
+    const FftFactors& F = facs[S.fac2];
+    auto src = [&](int q, int t) {
+        const int col = t >> LCL;
+        int r = q + N2 / 2;
+        if (r >= G2) r -= G2;
+        const float ok = col < ncol && r < N2 ? 1.f : 0.f;
+        r = min(r, N2 - 1);
+        const float2 v = g[((int64_t)r * G1 + c0 + min(col, ncol - 1)) * BCT + (t & (CL - 1))];
+        return make_float2(v.x * ok, v.y * ok);
+    };
+    fft_first_pass_src<T, 1, kWide>(F.r[0], A, G2, src);
+    for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
+    __syncthreads();
+    float2* R = const_cast<float2*>(fft_lds_range<T, 1, BIG, kWide>(A, Bb, F, tw, 1, F.nf));
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < G2 * T; idx += kFftThreads) {
+        const int t = idx & (T - 1);
+        const float2 k = K[(idx >> LT) * G1 + c0 + min(t >> LCL, ncol - 1)];
+        const float2 v = R[idx];
+        R[idx] = make_float2(v.x * k.x - v.y * k.y, v.x * k.y + v.y * k.x);
+    }
+    __syncthreads();
+    const float2* Rb = fft_lds_range<T, -1, BIG, kWide>(R, R == A ? Bb : A, F, tw, 0, F.nf - 1);
+    int p = 1;
+    for (int k = 0; k + 1 < F.nf; ++k) p *= F.r[k];
+    auto dst = [&](int q, int t, float2 v) {
+        const int col = t >> LCL;
+        int r = q + N2 / 2;
+        if (r >= G2) r -= G2;
+        if (col < ncol && r < N2) g[((int64_t)r * G1 + c0 + col) * BCT + (t & (CL - 1))] = v;
+    };
+    fft_last_pass_dst<T, -1, BIG, kWide>(F.r[F.nf - 1], Rb, G2, p, tw + (p - 1), dst);
+}
+
 // Toeplitz normal operator, middle stage: each slice's grid Z[g2][g1][b]
 // (after the forward row and column FFTs) times the spectrum of its
 // point-spread kernel, K[g2][g1], in place.  Grid: (cells * BCT / 1024,
@@ -2140,6 +2199,7 @@ bool NufftOp::set_option(const std::string& key, int value) {
     if (key == "nu_grid_stage") return flag(grid_stage_, false);
     if (key == "nu_xcd_order") return flag(xcd_order_, true);
     if (key == "nu_grid_pairs") return flag(grid_pairs_, true);
+    if (key == "nu_toep_fused") return flag(toep_fused_ok_, false);
     if (key == "nu_snake") return flag(snake_, true);
     if (key == "nu_degrid_merge") {
         const bool v = value != 0;
@@ -2180,6 +2240,7 @@ bool NufftOp::get_option(const std::string& key, int* value) const {
     else if (key == "nu_grid_stage") *value = grid_stage_;
     else if (key == "nu_xcd_order") *value = xcd_order_;
     else if (key == "nu_grid_pairs") *value = grid_pairs_;
+    else if (key == "nu_toep_fused") *value = toep_fused_ok_;
     else if (key == "nu_snake") *value = snake_;
     else if (key == "nu_degrid_merge") *value = degrid_merge_;
     else if (key == "nu_slice_group") *value = slice_group_;
@@ -2823,6 +2884,14 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                facs_.p, twid_.p, deap_.p, in, c.ws.p, run);
         LAUNCHCHECK();
         mark("nu_fwd_rows");
+        if (ndim_ == 2 && toep_fused_ok_) {
+            if (c.n_cols > 0)
+                hipLaunchKernelGGL((nu_cols_toep<BCT, BIG>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
+                                   facs_.p, twid_.p, (const float2*)toep_m_.p, (const int64_t*)toep_off_.p, c.ws.p,
+                                   run.skip);
+            LAUNCHCHECK();
+            mark("nu_cols_toep");
+        } else {
         if (ndim_ == 2 && c.n_cols > 0) {
             hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_fcols.p,
                                facs_.p, twid_.p, c.ws.p, run.skip);
@@ -2842,6 +2911,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                facs_.p, twid_.p, c.ws.p, run.skip);
             LAUNCHCHECK();
             mark("nu_cols");
+        }
         }
         if (c.n_rows > 0)
             hipLaunchKernelGGL((nu_adj_rows<BCT, BIG>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p,
@@ -2936,6 +3006,27 @@ void NufftOp::set_toeplitz(const float2* psf) {
         const int N1 = d.N1, N2 = ndim_ == 2 ? d.N2 : 1, G1 = d.G1, G2 = ndim_ == 2 ? d.G2 : 1;
         if (G1 < 2 * N1 - 1 || G2 < 2 * N2 - 1) fail(MM_ERR_ARG, "nufftOp: Toeplitz form needs a grid of at least 2N - 1");
         const int P1 = 2 * N1, P2 = ndim_ == 2 ? 2 * N2 : 1;   // point-spread image size
+        {
+            // slices with the same grid and a byte-identical point-spread
+            // image (one trajectory and dcf) share one spectrum
+            int same = -1;
+            int64_t q = 0;
+            for (int t = 0; t < s && same < 0; ++t) {
+                const NuSliceDev& e = sl_[(size_t)t];
+                const int64_t pt = (int64_t)2 * e.N1 * (ndim_ == 2 ? 2 * e.N2 : 1);
+                if (e.N1 == d.N1 && e.N2 == d.N2 && e.G1 == d.G1 && e.G2 == d.G2 &&
+                    std::memcmp(psf + q, psf + pofs, (size_t)pt * sizeof(float2)) == 0)
+                    same = t;
+                q += pt;
+            }
+            if (same >= 0) {
+                koff.push_back(koff[(size_t)same]);
+                pofs += (int64_t)P1 * P2;
+                d.deap1 = sl_[(size_t)same].deap1;
+                d.deap2 = sl_[(size_t)same].deap2;
+                continue;
+            }
+        }
         const double alpha = std::pow(2.0, ndim_) / std::sqrt((double)P1 * P2);
         // h(D1, D2) = alpha psf[N1 + D1, N2 + D2] for |D| <= N - 1 (the only
         // lags between a pixel and a pixel of the cropped output);

@@ -204,6 +204,9 @@ class NufftOp : public mm_op {
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
     bool big_ = false;
     bool toep_ = false;
+    // 2-D Toeplitz middle stage as one column kernel (option "nu_toep_fused",
+    // default on); off: forward columns, nu_toep_mul, adjoint columns
+    bool toep_fused_ok_ = true;
     DevBuf<float2> toep_m_;      // per slice the spectrum of h on the grid, Z[g2][g1] order
     DevBuf<int64_t> toep_off_;   // per slice: first entry in toep_m_
     const int* skip_ = nullptr;   // per-slice skip flags of the normal operator   // a grid side above kFftSmallMax: FFT kernels with the wide modulo
