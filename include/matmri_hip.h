@@ -180,7 +180,7 @@ int mm_op_profile_stage(mm_op* op, int idx, double* ms_per_launch, int* launches
  * "nu_stage_entries" (64..2048), "nu_degrid_block" (16..256),
  * "nu_slice_group" (slices per group of the normal operator, 0: one), and the 0/1
  * switches "nu_pair_cols", "nu_xcd_order", "nu_grid_pairs", "nu_snake", "nu_degrid_merge", "nu_grid_stage",
- * "nu_degrid_stage", "nu_degrid_t". */
+ * "nu_degrid_stage", "nu_degrid_t", "nu_toep_fused" (the Toeplitz form's 2-D middle stage in one pass). */
 int mm_op_set_option(mm_op* op, const char* key, int value);
 /* Current value of a tuning option of mm_op_set_option (no counterpart in
  * the reference). */
