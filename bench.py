@@ -405,6 +405,29 @@ def toeplitz_normal(mm, fam, prob, dev, slices):
                          "frac": ach / PEAK_HBM_GBS if ach else None}}
 
 
+def cpu_baseline_toeplitz(fam, P, seconds=10.0):
+    """The oracle's Toeplitz form (FP64 NumPy, ``oracle.nufft.ToepPlan``,
+    one thread) of the same SENSE normal operator, R'(T(R x)) for 8-coil
+    128 x 128 slices, timed on as many slices as fit `seconds` (plan built
+    once, untimed) and scaled to applications of the whole P-slice batch."""
+    import numpy as np
+    from oracle.nufft import ToepPlan
+    N, C = fam.p["N"], fam.p["C"]
+    T = ToepPlan([N, N], fam.S._kl[0], fam.S._dc[0], 1.5, fam.S.kwidth)
+    rng = np.random.default_rng(0)
+    maps = rng.standard_normal((N, N, C)) + 1j * rng.standard_normal((N, N, C))
+    x = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    n, t_tot = 0, 0.0
+    while t_tot < seconds and n < P:
+        t0 = time.perf_counter()
+        _ = np.sum(np.conj(maps) * T.apply(maps * x[..., None]), axis=2)
+        t_tot += time.perf_counter() - t0
+        n += 1
+    return {"value": n / t_tot / P, "unit": "normal-operator applications/s (whole batch)", "cores": 1,
+            "kind": "port", "sample": f"NumPy FP64 Toeplitz form (oracle ToepPlan) on {n} of the {P} slices, "
+                                      f"{t_tot:.1f} s, scaled to the batch"}
+
+
 def applications_per_s(stages):
     """Normal-operator applications (E'E of every slice of the batch) per
     second from the per-stage launch times of one application."""
@@ -555,6 +578,8 @@ def run_warmstart(a, mm, torch, dev):
     out["normal_operator_stages"] = stages
     out["normal_operator_applications_per_s"] = applications_per_s(stages)
     out["toeplitz_normal"] = toeplitz_normal(mm, fam, prob, dev, [(N, M, Cc)] * P)
+    if out["toeplitz_normal"] and not a.no_cpu:
+        out["toeplitz_normal"]["cpu_baseline"] = cpu_baseline_toeplitz(fam, P)
     if not a.no_cpu:
         out["nrmse_vs_fp64_oracle"] = oracle_nrmse_warmstart(fam, host, prob, zw["x"], zz["x"])
         cb = cpu_baseline_warmstart(fam, host, prob, a.cpu_seconds)
