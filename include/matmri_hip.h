@@ -104,8 +104,11 @@ int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd);
  * per slice the point-spread image S2'*(dcf or ones) of a NUFFT on the doubled
  * image size (2 N1 x 2 N2, column-major), slices concatenated.  Afterwards
  * both directions of mm_op_apply map an image to an image (n_data = n_image):
- * crop(fftnc(convMat .* ifftnc(pad(x)))), convMat = 2^ndim ifftnc(psf); a
- * SENSE operator built on it has its normal operator R'(T(R x)) only. */
+ * crop(fftnc(convMat .* ifftnc(pad(x)))), convMat = 2^ndim ifftnc(psf).  A
+ * SENSE operator built on it maps, like mrSampFunc on such a nufftOp,
+ * forward x -> T(R x) (coil images) and adjoint y -> R'(T y); its normal
+ * operator is R'(T(R x)).  mm_cgne_solve rejects it (MM_ERR_UNSUPPORTED):
+ * b = E'(W.*y) needs the gridding form. */
 int mm_nufft_set_toeplitz(mm_op* op, const float* psf);
 
 /* SENSE encoding E = S o R with optional real data weights W ([M x C], used

@@ -64,6 +64,10 @@ class HighOrderPlan:
         self.kbase = None
         self.ksphaDiv = None
         self.kconcDiv = None
+        # exp(1i*kbase) kept between calls when set (repeated applications in
+        # an iterative solve); the values are those of the direct model
+        self.cache_exp = False
+        self._ekb = None
 
     # -- sampHighOrder.m:496-588 --------------------------------------------
     def phase(self, phs_spha=None, phs_conc=None, sampTimes=None, vsl=slice(None), tsl=slice(None)):
@@ -106,6 +110,14 @@ class HighOrderPlan:
             self.kbase = self.phase()
         return self.kbase
 
+    def _exp_kb(self):
+        """``exp(1i*kbase)``, [V x K]; cached when ``cache_exp`` is set."""
+        if self.cache_exp:
+            if self._ekb is None:
+                self._ekb = np.exp(1j * self._kb())
+            return self._ekb
+        return np.exp(1j * self._kb())
+
     # -- sampHighOrder.m:361-391 ---------------------------------------------
     def forward(self, x):
         """``S*x``: ``imSize + reps`` -> ``kSize + reps`` (direct)."""
@@ -115,7 +127,7 @@ class HighOrderPlan:
         xa = x.reshape(self.V, -1, order="F")
         kb = self._kb()
         if self.ksphaDiv is None:
-            y = np.exp(1j * kb.T) @ xa
+            y = self._exp_kb().T @ xa
         else:
             d = self.phase(self.ksphaDiv, self.kconcDiv, np.array([1.0]))
             y = (1j * np.exp(1j * kb) * d).T @ xa
@@ -128,7 +140,9 @@ class HighOrderPlan:
         ndk = len(self.kSize) if not (len(self.kSize) == 2 and self.kSize[1] == 1) else 1
         reps = y.shape[ndk:]
         ya = y.reshape(self.K, -1, order="F")
-        x = np.exp(-1j * self._kb()) @ ya / np.sqrt(self.V)
+        # conj(E) @ y computed as conj(E @ conj(y)): the same products, with
+        # no conjugated copy of the [V x K] matrix
+        x = np.conj(self._exp_kb() @ np.conj(ya)) / np.sqrt(self.V)
         return x.reshape(self.imSize + reps, order="F") if reps else x.reshape(self.imSize, order="F")
 
     # -- sampHighOrder.m:393-457 (approach 3: phase recomputed per segment) ---

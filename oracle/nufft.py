@@ -123,6 +123,11 @@ class NufftPlan:
         self.comp = [deapod(self.imN[d], self.osN[d], self.kwidth, self.os) for d in range(nd)]
         self._prep_matrix()
         self._prep_shift()
+        # single = True evaluates the same steps in complex64 arithmetic (a
+        # model of a single-precision implementation of this operator, used
+        # only to size the tolerances of iterative-solver tests)
+        self.single = False
+        self._c32 = None
 
     # -- nufftOp.m:187-257 ---------------------------------------------------
     def _prep_matrix(self):
@@ -188,9 +193,49 @@ class NufftPlan:
             c = c * self.comp[d].reshape(bshape)
         return c
 
+    def _single_consts(self):
+        if self._c32 is None:
+            cm = self.convMat.astype(np.complex64)
+            self._c32 = (cm, cm.T.tocsr(), self._comp_nd().astype(np.float32),
+                         self.fftShiftArray.astype(np.complex64),
+                         None if self.dcf is None else self.dcf.astype(np.float32))
+        return self._c32
+
+    def _apply_single(self, x, adjoint):
+        """The forward (``:378-394``) or adjoint (``:354-377``) steps in
+        complex64: unitary FFTs with the same padding and crop, float32
+        deapodisation, interpolation matrix and dcf."""
+        nd = len(self.imN)
+        cm, cmT, comp, sh, dcf = self._single_consts()
+        axes = tuple(range(nd))
+        imN = tuple(int(n) for n in self.imN)
+        osN = tuple(int(g) for g in self.osN)
+        if not adjoint:
+            x = np.asarray(x).astype(np.complex64)
+            reps = x.shape[nd:]
+            nrep = int(np.prod(reps)) if reps else 1
+            z = x.reshape(imN + (nrep,), order="F") * comp[..., None]
+            z = np.fft.ifftn(z, s=osN, axes=axes, norm="ortho").astype(np.complex64) * sh[..., None]
+            y = cm @ z.reshape(-1, nrep, order="F")
+            if dcf is not None:
+                y = y * dcf[:, None]
+            return y.astype(np.complex128).reshape((self.M,) + reps, order="F")
+        y = np.asarray(x).astype(np.complex64)
+        reps = y.shape[1:]
+        nrep = int(np.prod(reps)) if reps else 1
+        ya = y.reshape(self.M, nrep, order="F")
+        if dcf is not None:
+            ya = ya * dcf[:, None]
+        z = (cmT @ ya).reshape(osN + (nrep,), order="F") * np.conj(sh)[..., None]
+        z = np.fft.fftn(z, axes=axes, norm="ortho").astype(np.complex64)
+        xa = z[tuple(slice(0, n) for n in imN)] * comp[..., None]
+        return xa.astype(np.complex128).reshape(imN + reps, order="F")
+
     def forward(self, x):
         """``N*x``: image ``imN + reps`` -> samples ``[M] + reps``
         (``nufftOp.m:378-394``)."""
+        if self.single:
+            return self._apply_single(x, False)
         nd = len(self.imN)
         x = np.asarray(x, dtype=np.complex128)
         reps = x.shape[nd:]
@@ -208,6 +253,8 @@ class NufftPlan:
     def adjoint(self, y):
         """``N'*y``: samples ``[M] + reps`` -> image ``imN + reps``
         (``nufftOp.m:354-377``)."""
+        if self.single:
+            return self._apply_single(y, True)
         nd = len(self.imN)
         y = np.asarray(y, dtype=np.complex128)
         reps = y.shape[1:]

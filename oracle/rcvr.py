@@ -19,10 +19,14 @@ class RcvrPlan:
         self.imN = self.maps.shape[:-1]
         self.nDim = len(self.imN)
         self.C = self.maps.shape[-1]
+        # complex64 evaluation (see NufftPlan.single)
+        self.single = False
 
     def forward(self, x):
         """``R*x`` (``rcvrOp.m:152``): ``imN`` -> ``imN + [C]``."""
         x = np.asarray(x, dtype=np.complex128).reshape(self.imN, order="F")
+        if self.single:
+            return (x.astype(np.complex64)[..., None] * self.maps.astype(np.complex64)).astype(np.complex128)
         return x[..., None] * self.maps
 
     def adjoint(self, y, doLoop=False):
@@ -32,6 +36,9 @@ class RcvrPlan:
             y = y.reshape(self.imN + (self.C,), order="F")
         else:  # MATLAB implicit expansion of singleton dims (mrSampFunc sumDims)
             y = np.broadcast_to(y, self.imN + (self.C,))
+        if self.single:
+            return np.sum(y.astype(np.complex64) * np.conj(self.maps.astype(np.complex64)),
+                          axis=self.nDim).astype(np.complex128)
         if doLoop:
             acc = 0
             for c in range(self.C):

@@ -15,9 +15,14 @@ exact arithmetic within n iterations), the reference's restart rule (the
 first iteration is a steepest-descent step because ``mod(0, expResN) == 0``),
 each stopping status of ``cgne.m:286-335``, ``powermethod`` against
 ``numpy.linalg.eigvalsh`` (``powermethod.m:77-95``), and ``bfista`` with
-``A = I``, ``R = I`` against the closed form ``soft(b, lam/2)``.
+``A = I``, ``R = I`` against the closed form ``soft(b, lam/2)``; its k-space
+projection ``opt.kMask`` (``bfista.m:161-169``) by the closed form of
+``A = I``, ``lam = 0``: the iterates converge to the projection of ``b``,
+``ifft2(kMask .* fft2(b))``, and every iterate lies in its range.
 """
 import numpy as np
+
+from .fftnc import fftnc, ifftnc
 
 
 def _as_handle(A):
@@ -181,6 +186,9 @@ def bfista(Ain, bin, Rin, lam, x0=None, NitMax=200, opt=None):
     maxEig = opt.get("maxEig", None)
     resThresh = opt.get("resThresh", 1e-5)
     xMask = opt.get("xMask", None)
+    kMask = opt.get("kMask", None)
+    if kMask is not None and not (np.ndim(kMask) == 2 and np.shape(kMask)[1] > 1):
+        raise ValueError("TODO: code for other dims, or require an fftFunc supplied to bfista")   # :167
     nRestart = opt.get("nRestart", NitMax + 1)
     A = _as_handle(Ain)
     R = _as_handle(Rin)
@@ -210,6 +218,8 @@ def bfista(Ain, bin, Rin, lam, x0=None, NitMax=200, opt=None):
         x = R(x, "transp")
         if xMask is not None:
             x = x * xMask
+        if kMask is not None:                                       # :161-169
+            x = ifftnc(fftnc(x, 2, 0, 0) * kMask, 2, 0, 0)
         xdiff = x - x_prev                                          # :195-201
         t = 0.5 * (1 + np.sqrt(1 + 4 * t_prev ** 2))
         y = x + (t_prev - 1) / t * xdiff

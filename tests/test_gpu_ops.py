@@ -150,6 +150,39 @@ def test_multislice_sense_vs_oracle(mm):
         ox += N * N
 
 
+def test_uneven_batch_64_slices_16_coils(mm):
+    """The uneven batch at full size: 64 slices of N in 96..160 (13 grid sizes
+    144..240 at os 1.5, among them 150 = 2*3*5^2, 196 = 2^2*7^2 and
+    210 = 2*3*5*7), 6000..20000 spiral samples each, 16 coils, one SENSE
+    operator tuned by the autotuner.  E and E' of every slice against the
+    oracle (nufftOp.m:146-152 grid sizes, rel-L2 <= 1e-5)."""
+    print("[case] uneven batch 64 slices C=16 (autotuned)", flush=True)
+    B = mm.workloads.batch_problems(seed=4, nslice=64, C=16)
+    grids = sorted({mm.workloads.grid_size(b["N"]) for b in B})
+    print(f"  grid sizes {grids}", flush=True)
+    assert len(grids) == 13 and {150, 196, 210}.issubset(grids)
+    E = mm.senseOp(mm.nufftOp.batch([([b["N"]] * 2, b["kloc"], None) for b in B], 1.5, 4),
+                   mm.rcvrOp.batch([b["maps"] for b in B]))
+    print(f"  autotune: {E.autotune(2)}", flush=True)
+    rng = np.random.default_rng(40)
+    xs = [crand(rng, (b["N"], b["N"])) for b in B]
+    ys = [crand(rng, (b["M"], 16)) for b in B]
+    yg = E * np.concatenate([x.reshape(-1, order="F") for x in xs])
+    xg = E.H * np.concatenate([y.reshape(-1, order="F") for y in ys])
+    oy = ox = 0
+    worst = 0.0
+    for b, x, y in zip(B, xs, ys):
+        N, M = b["N"], b["M"]
+        So, Ro = NufftPlan([N, N], b["kloc"], None, 1.5, 4), RcvrPlan(b["maps"])
+        e_f = rel(yg[oy:oy + M * 16], o_mrSampFunc(x, "notransp", So, Ro))
+        e_a = rel(xg[ox:ox + N * N], o_mrSampFunc(y, "transp", So, Ro))
+        worst = max(worst, e_f, e_a)
+        assert e_f < TOL and e_a < TOL, f"slice N={N} M={M}: fwd {e_f:.2e} adj {e_a:.2e}"
+        oy += M * 16
+        ox += N * N
+    print(f"  worst rel-L2 over 64 slices, both directions: {worst:.2e}", flush=True)
+
+
 def test_multislice_shared_trajectory(mm):
     """Slices with the same trajectory share their plan data (sample weights,
     CSR gridding rows): slices 0 and 2 share one spiral, slice 1 has its own,
@@ -304,6 +337,52 @@ def test_toeplitz_vs_oracle(mm, nD, with_dcf, nrep):
     e_u = rel(np.asarray(T * x), want)
     print(f"  unfused vs oracle {e_u:.2e}", flush=True)
     assert e_u < TOL
+
+
+@pytest.mark.parametrize("N,C", [(128, 8), (96, 16)])
+def test_toeplitz_sense_radix16_columns(mm, N, C):
+    """The Toeplitz middle stage on grids whose column FFTs factor into radix
+    16 and 12 only (N = 128 -> G = 256 = 16 x 16, N = 96 -> G = 192 = 16 x 12)
+    with 8 or 16 transforms per column item (a SENSE operator with 8 or 16
+    coils): the fused column kernel and the three-kernel form against the
+    oracle's Toeplitz form, E'E x = R'(T(R x))."""
+    print(f"[case] Toeplitz SENSE normal N={N} C={C} (radix-16/12 columns)", flush=True)
+    rng = np.random.default_rng(58)
+    kl = mm.workloads.spiral(int(N * N * 0.6), N // 2)
+    dcf = mm.workloads.spiral_dcf(kl.shape[0], N // 2)
+    maps = crand(rng, (N, N, C))
+    T = mm.nufftOp([N, N], kl, dcf, 1, 1.5, 4).prepToep()
+    E = mm.senseOp(T, mm.rcvrOp(maps))
+    x = crand(rng, (N, N))
+    Tp = ToepPlan([N, N], kl, dcf, 1.5, 4)
+    want = sum(np.conj(maps[..., c]) * Tp.apply(maps[..., c] * x) for c in range(C))
+    e = rel(E.normal(x), want)
+    T.set_option("nu_toep_fused", 0)
+    e_u = rel(E.normal(x), want)
+    print(f"  fused vs oracle {e:.2e}; unfused {e_u:.2e}", flush=True)
+    assert e < TOL and e_u < TOL
+
+
+def test_toeplitz_sense_forward_adjoint(mm):
+    """mrSampFunc on a prepToep'd nufftOp (nufftOp.m:348-352): the forward is
+    T(R x) (coil images), the adjoint R'(T y); and cgne refuses the operator."""
+    print("[case] Toeplitz SENSE forward/adjoint N=48 C=4", flush=True)
+    rng = np.random.default_rng(59)
+    N, C = 48, 4
+    kl = rng.random((1200, 2)) - 0.5
+    maps = crand(rng, (N, N, C))
+    T = mm.nufftOp([N, N], kl, None, 1, 1.5, 4).prepToep()
+    E = mm.senseOp(T, mm.rcvrOp(maps))
+    Tp = ToepPlan([N, N], kl, None, 1.5, 4)
+    x = crand(rng, (N, N))
+    y = crand(rng, (N, N, C))
+    want_f = np.stack([Tp.apply(maps[..., c] * x) for c in range(C)], axis=-1)
+    want_a = sum(np.conj(maps[..., c]) * Tp.apply(y[..., c]) for c in range(C))
+    e_f, e_a = rel(E * x, want_f), rel(E.H * y, want_a)
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+    with pytest.raises(NotImplementedError):
+        mm.cgne(E, y, None, 3, {})
 
 
 def test_toeplitz_sense_normal_multislice(mm):

@@ -220,3 +220,31 @@ def test_padcrop_round_trip():
     p = padcrop(a, [6, 7])
     assert p.shape == (6, 7) and np.array_equal(p[1:4, 1:5], a)
     assert np.array_equal(padcrop(p, [3, 4]), a)
+
+
+def test_single_precision_evaluation():
+    """NufftPlan.single / RcvrPlan.single evaluate the same steps in complex64:
+    within single-precision rounding (1e-6) of the FP64 evaluation in both
+    directions, yet not equal to it (it does round), and the rounding is of
+    the order of a complex64 ulp, not of the NUFFT approximation error."""
+    from oracle.rcvr import RcvrPlan
+    rng = np.random.default_rng(17)
+    imN, M = [40, 36], 2500
+    kloc = rng.random((M, 2)) - 0.5
+    dcf = rng.random(M) + 0.5
+    S = NufftPlan(imN, kloc, dcf, 1.5, 4)
+    x = rng.standard_normal(imN + [2]) + 1j * rng.standard_normal(imN + [2])
+    y = rng.standard_normal((M, 2)) + 1j * rng.standard_normal((M, 2))
+    f64, a64 = S.forward(x), S.adjoint(y)
+    S.single = True
+    f32, a32 = S.forward(x), S.adjoint(y)
+    for u, v in ((f32, f64), (a32, a64)):
+        e = np.linalg.norm(u - v) / np.linalg.norm(v)
+        assert 1e-9 < e < 1e-6
+    maps = rng.standard_normal(imN + [3]) + 1j * rng.standard_normal(imN + [3])
+    R = RcvrPlan(maps)
+    xr = x[..., 0]
+    r64, b64 = R.forward(xr), R.adjoint(R.forward(xr))
+    R.single = True
+    assert 1e-9 < np.linalg.norm(R.forward(xr) - r64) / np.linalg.norm(r64) < 1e-6
+    assert 1e-9 < np.linalg.norm(R.adjoint(r64) - b64) / np.linalg.norm(b64) < 1e-6

@@ -192,6 +192,35 @@ def test_bfista_identity_closed_form():
     assert np.allclose(x, softthresh(b, lam / 2), atol=1e-8)
 
 
+def test_bfista_kmask_projection_closed_form():
+    """opt.kMask (bfista.m:161-169): with A = I, R = I and lam = 0 the
+    objective is ||x - b||^2 over the range of the k-space projection
+    P x = ifft2(kMask .* fft2(x)) (uncentred, unscaled: fftnc/ifftnc with
+    doShift = doScale = 0), whose minimiser is P b; every iterate is in that
+    range.  An all-ones mask changes nothing; a mask of any other shape
+    raises like the reference."""
+    rng = np.random.default_rng(19)
+    N1, N2 = 12, 10
+    b = _rand_c(rng, (N1, N2))
+    km = (rng.random((N1, N2)) > 0.4).astype(float)
+    eye = lambda v, tr: v
+    x, *_ = bfista(eye, b, eye, 0.0, np.zeros((N1, N2), complex), 400,
+                   {"maxEig": 1.0, "resThresh": 1e-30, "kMask": km})
+    Pb = np.fft.ifft2(km * np.fft.fft2(b))
+    assert np.linalg.norm(x - Pb) / np.linalg.norm(Pb) < 1e-10
+    # a single step already lies in the range of the projection
+    x1, *_ = bfista(eye, b, eye, 0.3, np.zeros((N1, N2), complex), 1, {"maxEig": 1.0, "kMask": km})
+    assert np.max(np.abs(np.fft.fft2(x1)[km == 0])) < 1e-12
+    # a wrong projection (conjugated mask or a scaled transform pair) misses
+    wrong = np.fft.ifft2(km * np.fft.fft2(b)) * np.sqrt(N1 * N2)
+    assert np.linalg.norm(x - wrong) / np.linalg.norm(wrong) > 0.5
+    xo, *_ = bfista(eye, b, eye, 0.3, np.zeros((N1, N2), complex), 30, {"maxEig": 1.0})
+    x1s, *_ = bfista(eye, b, eye, 0.3, np.zeros((N1, N2), complex), 30, {"maxEig": 1.0, "kMask": np.ones((N1, N2))})
+    assert np.allclose(xo, x1s, atol=1e-12)
+    with pytest.raises(ValueError):
+        bfista(eye, b, eye, 0.3, None, 3, {"maxEig": 1.0, "kMask": np.ones(N1)})
+
+
 def test_cgne_fp32_io_sensitivity():
     """Pins the tolerance of the device cgne tests: on a 64^2 / 4-coil spiral
     SENSE problem solved to relative residual 1e-4, rounding only the input and

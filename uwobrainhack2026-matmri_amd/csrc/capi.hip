@@ -257,6 +257,10 @@ int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax, 
         mm_cg_opts o;
         mm_cg_opts_default(&o);
         if (opt) o = *opt;
+        if (auto* se = dynamic_cast<SenseOp*>(E); se && se->toeplitz())
+            fail(MM_ERR_UNSUPPORTED,
+                 "cgne: the SENSE operator is on a Toeplitz-form NUFFT (prepToep), whose forward maps images "
+                 "to coil images; solve with the gridding-form NUFFT");
         mm_ctx* ctx = E->ctx;
         set_device(ctx);
         auto& m = solvers();

@@ -22,6 +22,10 @@ struct CgScal {
     double beta;                 // betaFact * res_sq_new / res_sq_old (real; cgne.m:328)
     double stopThresh;           // < 0: off
     double resThresh;            // effective absolute threshold (< 0: off)
+    // stopping options of this solve (cgne.m:21-89), set by cg_init: kernel
+    // arguments would be frozen into the captured iteration graph
+    double xChangeThresh, resChangeThresh, resIncThresh;
+    int stopOnResInc, stopOnXdifInc, nseExtraIt;
     int nit, nitmax, done, status;
     int resIncs, xdifIncs, lastXdifInc, fallback;  // fallback: x0 replaced by 0
 };
@@ -269,7 +273,9 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_init(const CgSeg* __restrict_
                         const double2* __restrict__ part_b, const double2* __restrict__ part_r,
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
                         int nslice, int nitmax, double resThresh, double resThreshRel,
-                        double stopThresh, int x0fallback) {
+                        double stopThresh, int x0fallback, double xChangeThresh,
+                        double resChangeThresh, int stopOnResInc, double resIncThresh,
+                        int stopOnXdifInc, int nseExtraIt) {
     const int s = blockIdx.x, lane = threadIdx.x;   // one wave per slice
     if (s >= nslice) return;
     const CgSeg g = seg[s];
@@ -291,6 +297,12 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_init(const CgSeg* __restrict_
     c.nit = 0;
     c.nitmax = nitmax;
     c.stopThresh = stopThresh;
+    c.xChangeThresh = xChangeThresh;
+    c.resChangeThresh = resChangeThresh;
+    c.resIncThresh = resIncThresh;
+    c.stopOnResInc = stopOnResInc;
+    c.stopOnXdifInc = stopOnXdifInc;
+    c.nseExtraIt = nseExtraIt;
     double th = resThresh;
     if (resThreshRel >= 0) th = th >= 0 ? fmax(th, resThreshRel * bb) : resThreshRel * bb;
     c.resThresh = th;
@@ -325,9 +337,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_fallback_k(const CgSeg* __restric
 __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                         const double2* __restrict__ part, const double* __restrict__ pp,
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
-                        int nslice, int nitmax_alloc, int explicit_res, double xChangeThresh,
-                        double resChangeThresh, int stopOnResInc, double resIncThresh,
-                        int stopOnXdifInc, int nseExtraIt, int* __restrict__ done_host,
+                        int nslice, int nitmax_alloc, int explicit_res, int* __restrict__ done_host,
                         int* __restrict__ done_map) {
     const int s = blockIdx.x, lane = threadIdx.x;   // one wave per slice
     if (s >= nslice) return;
@@ -353,6 +363,9 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict_
     hxn[h + n + 1] = xx;
     hxd[h + n + 1] = xd;
     int finished = 0;
+    const double xChangeThresh = c.xChangeThresh, resChangeThresh = c.resChangeThresh;
+    const double resIncThresh = c.resIncThresh;
+    const int stopOnResInc = c.stopOnResInc, stopOnXdifInc = c.stopOnXdifInc, nseExtraIt = c.nseExtraIt;
     if (xChangeThresh >= 0 && xd < xChangeThresh) { finished = 1; c.status = 10; }
     if (c.resThresh >= 0 && rr < c.resThresh) { finished = 1; c.status = 1; }
     if (resChangeThresh >= 0 && fabs((hres[h + n + 1] - hres[h + n]) / hres[h + n]) < resChangeThresh) {
@@ -487,6 +500,11 @@ CgSolver::~CgSolver() {
     if (poll_ev_) (void)hipEventDestroy(poll_ev_);
 }
 
+void CgSolver::drop_graph() {
+    if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+}
+
 void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x) {
     hipStream_t st = E_->ctx->stream;
     const int ns = E_->nslice();
@@ -504,8 +522,7 @@ void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x
                            nblk_);
     }
     hipLaunchKernelGGL(cg_beta, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, partu_.p, pp_.p, hres_.p, hxn_.p,
-                       hxd_.p, ns, nitmax_, explicit_res, o.xChangeThresh, o.resChangeThresh,
-                       o.stopOnResInc, o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt, done_d_.p, done_map_);
+                       hxd_.p, ns, nitmax_, explicit_res, done_d_.p, done_map_);
     hipLaunchKernelGGL(cg_update_p, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, r_.p);
     LAUNCHCHECK();
 }
@@ -522,13 +539,9 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
         hxn_.alloc((size_t)ns * (NitMax + 1));
         hxd_.alloc((size_t)ns * (NitMax + 1));
         done_d_.alloc(ns);
-        if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
-        if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+        drop_graph();
     }
-    if (graph_x_ != x) {
-        if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
-        if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
-    }
+    if (graph_x_ != x) drop_graph();
     // b = E'(W.*y)  (cgne.m:136-140)
     if (E_->W) {
         hipLaunchKernelGGL(scale_copy_k, dim3(256), dim3(256), 0, st, y, E_->W, tmp_.p, E_->n_data());
@@ -547,7 +560,8 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
                        r_.p, p_.p, part_.p);
     hipLaunchKernelGGL(cg_init, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
                        hxn_.p, hxd_.p, ns, NitMax, o.resThresh, o.resThreshRel, o.stopThresh,
-                       (x0 && o.x0Fallback) ? 1 : 0);
+                       (x0 && o.x0Fallback) ? 1 : 0, o.xChangeThresh, o.resChangeThresh, o.stopOnResInc,
+                       o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt);
     LAUNCHCHECK();
     if (x0 && o.x0Fallback) {
         hipLaunchKernelGGL(cg_fallback_k, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p, x,
@@ -570,6 +584,10 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     // reads them after every `poll` iterations, once the next iteration has
     // been queued, so the device never waits for the host (a solve that has
     // finished runs one more iteration whose kernels all return at once).
+    // the captured iteration holds the operator's kernels, work tables and
+    // buffers: recapture after a configuration change of E or any device
+    // free since the capture (the preamble above may itself have reallocated)
+    if (graph_exec_ && (E_->config_gen() != graph_gen_ || free_epoch().load() != graph_epoch_)) drop_graph();
     bool pending = false;
     for (int it = 0; it < NitMax; ++it) {
         const int explicit_res = (it % expResN) == 0;
@@ -582,6 +600,8 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
                 HIPCHECK(hipStreamEndCapture(st, &graph_));
                 HIPCHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
                 graph_x_ = x;
+                graph_gen_ = E_->config_gen();
+                graph_epoch_ = free_epoch().load();
             }
             HIPCHECK(hipGraphLaunch(graph_exec_, st));
         }

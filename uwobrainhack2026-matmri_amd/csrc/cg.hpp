@@ -20,6 +20,7 @@ class CgSolver {
 
    private:
     void launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x);
+    void drop_graph();
     mm_op* E_;
     int64_t n_ = 0;
     int nblk_ = 0;
@@ -38,6 +39,8 @@ class CgSolver {
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     float2* graph_x_ = nullptr;
+    uint64_t graph_gen_ = 0;     // E's configuration generation at capture
+    uint64_t graph_epoch_ = 0;   // device free count at capture
 };
 
 }  // namespace mm

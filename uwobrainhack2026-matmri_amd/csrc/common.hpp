@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <memory>
@@ -36,6 +37,14 @@ inline void hip_check(hipError_t e, const char* what, const char* file, int line
 #define HIPCHECK(x) ::mm::hip_check((x), #x, __FILE__, __LINE__)
 #define LAUNCHCHECK() ::mm::hip_check(hipGetLastError(), "kernel launch", __FILE__, __LINE__)
 
+// Counts device frees made through DevBuf.  A captured hipGraph holds raw
+// device addresses; its owner records this count at capture and recaptures
+// when it has moved (some buffer the graph may point at was freed).
+inline std::atomic<uint64_t>& free_epoch() {
+    static std::atomic<uint64_t> e{0};
+    return e;
+}
+
 // Owning device allocation (hipMalloc / hipFree).
 template <class T>
 struct DevBuf {
@@ -60,7 +69,10 @@ struct DevBuf {
     // grow-only reallocation (contents are not kept)
     void ensure(size_t count) { if (count > n) alloc(count); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            free_epoch().fetch_add(1, std::memory_order_relaxed);
+        }
         p = nullptr;
         n = 0;
     }
@@ -108,6 +120,11 @@ struct mm_op {
     // whose flag is nonzero may be left out (their output is not defined).
     // The solver installs its "slice finished" flags here; NULL = none.
     virtual void set_skip(const int* flags) { (void)flags; }
+    // Configuration generation: bumped whenever a tuning option, the
+    // Toeplitz form or the autotuner changes which kernels run or rebuilds
+    // work tables.  Composite operators report the sum over their parts.
+    uint64_t gen_ = 0;
+    virtual uint64_t config_gen() const { return gen_; }
     // Tuning knobs (kernel variants); false if the key is unknown.
     virtual bool set_option(const std::string& key, int value) { (void)key; (void)value; return false; }
     virtual bool get_option(const std::string& key, int* value) const { (void)key; (void)value; return false; }

@@ -188,6 +188,7 @@ HioOp::HioOp(mm_ctx* c, int64_t V, int64_t K, int Lh, int Lc, const double* x, c
 }
 
 void HioOp::set_phi_div(const double* hd, const double* gd) {
+    ++gen_;
     if (!hd && !gd) {
         phidiv_ = false;
         return;

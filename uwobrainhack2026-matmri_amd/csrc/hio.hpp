@@ -20,6 +20,7 @@ class HioOp : public mm_op {
     bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters,
                           double* ms, std::string* name) override;
     bool set_option(const std::string& key, int value) override {
+        ++gen_;
         if (key == "hio_variant") { variant_ = value & 3; return true; }
         if (key == "hio_mfma") { use_mfma_ = value != 0; return true; }
         return false;

@@ -2080,6 +2080,7 @@ static size_t fft_lds_bytes(int G, int T, bool inplace) {
 // Rebuilt when a tuning option changes them; every work table is rebuilt
 // with the next configuration.
 void NufftOp::build_partitions() {
+    ++gen_;
     const int ns = (int)sl_.size();
     gtile_host_.clear();
     dblk_host_.clear();
@@ -2188,6 +2189,7 @@ void NufftOp::build_partitions() {
 }
 
 bool NufftOp::set_option(const std::string& key, int value) {
+    ++gen_;   // captured solver graphs may hold the old kernels or tables
     auto flag = [&](bool& f, bool clear) {
         f = value != 0;
         if (clear) cfgs_.clear();
@@ -3083,6 +3085,7 @@ void NufftOp::set_toeplitz(const float2* psf) {
     HIPCHECK(hipStreamSynchronize(st));
     toep_ = true;
     cfgs_.clear();
+    ++gen_;
 }
 
 void NufftOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
@@ -3093,7 +3096,7 @@ void NufftOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
 void NufftOp::apply_sense(int adjoint, const float2* in, float2* out, int64_t nrep,
                           const float2* maps, int C, const float* Wt) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
-    if (toep_) fail(MM_ERR_UNSUPPORTED, "nufftOp: a SENSE operator on the Toeplitz form has its normal operator only");
+    if (toep_) fail(MM_ERR_HIP, "nufftOp: internal: fused SENSE apply on the Toeplitz form");
     run(config(nrep, 1, C), adjoint, in, out, maps, adjoint ? nullptr : Wt);
 }
 

@@ -204,6 +204,11 @@ class NufftOp : public mm_op {
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
     bool big_ = false;
     bool toep_ = false;
+
+   public:
+    bool is_toeplitz() const { return toep_; }
+
+   private:
     // 2-D Toeplitz middle stage as one column kernel (option "nu_toep_fused",
     // default on); off: forward columns, nu_toep_mul, adjoint columns
     bool toep_fused_ok_ = true;

@@ -199,7 +199,10 @@ SenseOp::SenseOp(mm_ctx* c, mm_op* S, mm_op* R, const float* Wh) : S_(S), R_(R) 
 
 void SenseOp::apply_impl(int adjoint, const float2* in, float2* out, int64_t nrep, const float* Wf) {
     RcvrOp* rc = static_cast<RcvrOp*>(R_);
-    if (nu_) {
+    // On the Toeplitz form (nufftOp.m:348-352) S maps images to images in
+    // both directions: forward T(R x) gives coil images, adjoint R'(T y)
+    // takes them.  That goes through the composition below.
+    if (nu_ && !nu_->is_toeplitz()) {
         nu_->apply_sense(adjoint, in, out, nrep, rc->maps(), C_, Wf);
         return;
     }

@@ -46,6 +46,9 @@ class SenseOp : public mm_op {
     bool get_option(const std::string& key, int* value) const override { return S_->get_option(key, value); }
     bool autotune(int iters) override;
     void set_skip(const int* flags) override { S_->set_skip(flags); }
+    uint64_t config_gen() const override { return gen_ + S_->config_gen() + R_->config_gen(); }
+    // the sampling operator is a NUFFT in the Toeplitz form (prepToep)
+    bool toeplitz() const { return nu_ && nu_->is_toeplitz(); }
     mm_op* sampling() const { return S_; }
     int coils() const { return C_; }
     // per-kernel times of the last normal-operator profile

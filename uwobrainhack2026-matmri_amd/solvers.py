@@ -133,14 +133,37 @@ def _as_handle(A):
     return A
 
 
-def _vdot(a, b):
+# -- device-resident vectors for bfista / powermethod ------------------------
+# NumPy arrays (MATLAB column-major meaning) become complex64 CUDA tensors
+# whose row-major shape is the column-major shape reversed -- the layout the
+# device operators read and write in place -- and come back the same way.
+
+def _to_dev(a):
     if _is_torch(a):
-        return torch.vdot(a.reshape(-1).to(torch.complex128), b.reshape(-1).to(torch.complex128)).item()
-    return np.vdot(np.asarray(a).reshape(-1, order="F"), np.asarray(b).reshape(-1, order="F"))
+        return a.to(torch.complex64)
+    a = np.asarray(a)
+    return torch.as_tensor(np.ascontiguousarray(np.transpose(a)).astype(np.complex64), device="cuda")
 
 
-def _norm(a):
-    return float(np.sqrt(np.real(_vdot(a, a))))
+def _to_host(t, like):
+    return t if _is_torch(like) else np.transpose(t.cpu().numpy())
+
+
+def _mask_dev(m):
+    if m is None:
+        return None
+    if _is_torch(m):
+        return m.to("cuda")
+    return torch.as_tensor(np.ascontiguousarray(np.transpose(np.asarray(m))), device="cuda")
+
+
+def _dot64(a, b):
+    """conj(a) . b summed in FP64 on the device (a host complex)."""
+    return torch.vdot(a.reshape(-1).to(torch.complex128), b.reshape(-1).to(torch.complex128)).item()
+
+
+def _sq64(a):
+    return float(torch.sum(torch.abs(a.reshape(-1).to(torch.complex128)) ** 2).item())
 
 
 def softthresh(w, lam):
@@ -156,79 +179,77 @@ def softthresh(w, lam):
 
 
 def powermethod(A, x0, NitMax=50, doAtA=1, xdiffThresh=1e-3):
-    """``powermethod.m:14-66``: largest eigenvalue of ``A'A`` (or ``A``)."""
+    """``powermethod.m:14-66``: largest eigenvalue of ``A'A`` (or ``A``), the
+    iterate kept on the GPU (norms and the Rayleigh quotient in FP64)."""
     h = _as_handle(A)
-    x = x0
-    x_prev = x0
+    x = _to_dev(x0)
+    x_prev = x
     nit = 0
     while True:
         x = h(h(x, "notransp"), "transp") if doAtA else h(x, "notransp")
-        x = x / _norm(x)
-        xdiff = _norm(x - x_prev)
+        x = x / float(np.sqrt(_sq64(x)))
+        xdiff = float(np.sqrt(_sq64(x - x_prev)))
         x_prev = x
         nit += 1
         if xdiff < xdiffThresh or nit >= NitMax:
             break
     AtAx = h(h(x, "notransp"), "transp") if doAtA else h(x, "notransp")
-    return _vdot(x, AtAx) / _vdot(x, x)
+    return _dot64(x, AtAx) / _sq64(x)
 
 
 def bfista(A, b, R, lam, x0=None, NitMax=200, opt=None):
-    """``bfista.m:1-246``.  Returns ``(x, resSqAll, RxAll, testAll, maxEig)``."""
+    """``bfista.m:1-246`` with every vector on the GPU: ``A`` runs through the
+    device operators (mm_op_apply on device pointers), ``R`` is called on the
+    device tensors (an orthonormal transform, ``R'R = I``, as balanced FISTA
+    requires), soft-thresholding, the projections and the momentum step are
+    device array operations, and the cost terms are summed in FP64.  NumPy
+    inputs come back as NumPy arrays.  Options: ``maxEig``, ``resThresh``,
+    ``xMask``, ``kMask`` (k-space projection, ``:161-169``), ``nRestart``.
+    Returns ``(x, resSqAll, RxAll, testAll, maxEig)``."""
     opt = dict(opt or {})
-    maxEig = opt.get("maxEig")
     resThresh = opt.get("resThresh", 1e-5)
-    xMask = opt.get("xMask")
     nRestart = opt.get("nRestart", NitMax + 1)
-    h = _as_handle(A)
-    Rh = _as_handle(R)
-    if x0 is None:
-        x0 = h(b, "transp")
-    if maxEig is None:
-        maxEig = powermethod(h, x0)
-    stepSz = 0.9 / (2 * abs(maxEig))
-    y = x0
-    resSqAll = np.zeros(NitMax + 1)
-    RxAll = np.zeros(NitMax + 1)
-    testAll = np.zeros((NitMax + 1, 2))
-    nit, nitToRestart, t_prev = 1, 1, 1.0
-    x_prev = x0
-    x = x0
-    finished = False
-    while not finished:
-        residual_y = h(y, "notransp") - b
-        grad = 2 * h(residual_y, "transp")
-        resSqAll[nit - 1] = np.real(_vdot(residual_y, residual_y))
-        g = y - stepSz * grad
-        x = Rh(g, "notransp")
-        RxAll[nit - 1] = float((lam * x).abs().sum()) if _is_torch(x) else np.sum(np.abs(lam * x))
-        x = softthresh(x, lam * stepSz)
-        x = Rh(x, "transp")
+    kMask = opt.get("kMask")
+    if kMask is not None and not (np.ndim(kMask) == 2 and np.shape(kMask)[1] > 1):
+        raise ValueError("bfista: opt.kMask must be a 2-D mask (other dimensions are not supported)")
+    xMask, kMask = _mask_dev(opt.get("xMask")), _mask_dev(kMask)
+    lam = float(lam) if np.ndim(lam) == 0 and not _is_torch(lam) else _mask_dev(lam)   # scalar or per coefficient
+    h, Rh = _as_handle(A), _as_handle(R)
+    bd = _to_dev(b)
+    x0d = h(bd, "transp") if x0 is None else _to_dev(x0)          # :73-75
+    maxEig = opt.get("maxEig")
+    if maxEig is None:                                             # :81-89
+        maxEig = powermethod(h, x0d)
+    step = 0.9 / (2 * abs(maxEig))                                 # :90
+    resSq, Rx, test = [], [], [[np.nan, np.nan]]
+    y = x_prev = x = x0d
+    t_prev = 1.0
+    since_restart = 1
+    for nit in range(1, NitMax + 1):                               # :127-231
+        r = h(y, "notransp") - bd
+        g = y - (2 * step) * h(r, "transp")
+        resSq.append(_sq64(r))
+        w = Rh(g, "notransp")
+        Rx.append(float(torch.sum(torch.abs(lam * w).to(torch.float64)).item()))
+        x = Rh(softthresh(w, lam * step), "transp")
         if xMask is not None:
             x = x * xMask
-        xdiff = x - x_prev
+        if kMask is not None:                                      # uncentred, unscaled pair
+            x = torch.fft.ifft2(torch.fft.fft2(x) * kMask)
+        dx = x - x_prev
         t = 0.5 * (1 + np.sqrt(1 + 4 * t_prev ** 2))
-        y = x + (t_prev - 1) / t * xdiff
-        t_prev = t
-        x_prev = x
-        testVal = None
-        if nit >= NitMax:
-            finished = True
-        if nit > 1:
-            numAve = 3
-            tv = np.abs(np.diff(resSqAll[:nit] + RxAll[:nit])) / (resSqAll[nit - 1] + RxAll[nit - 1])
-            i1 = max(1, nit - numAve)
-            testVal = np.mean(tv[i1 - 1:])
-            testAll[nit - 1, 0] = testVal
-            testAll[nit - 1, 1] = np.real(_vdot(xdiff, xdiff)) / np.real(_vdot(x, x))
-        if nit > 1 and testVal < resThresh:
-            finished = True
-        nit += 1
-        nitToRestart += 1
-        if nitToRestart >= nRestart:
-            nitToRestart = 1
-            t_prev = 1.0
-    n = nit - 1
-    testAll = testAll[:n].copy()
-    testAll[0, :] = np.nan
-    return x, resSqAll[:n], RxAll[:n], testAll, maxEig
+        y = x + ((t_prev - 1) / t) * dx
+        t_prev, x_prev = t, x
+        stop = nit >= NitMax
+        if nit > 1:                                                # :208-222
+            cost = np.asarray(resSq) + np.asarray(Rx)
+            tv = np.abs(np.diff(cost)) / cost[-1]
+            tval = float(np.mean(tv[max(1, nit - 3) - 1:]))
+            test.append([tval, _sq64(dx) / _sq64(x)])
+            stop = stop or tval < resThresh
+        since_restart += 1
+        if since_restart >= nRestart:
+            since_restart, t_prev = 1, 1.0
+        if stop:
+            break
+    return (_to_host(x, b), np.asarray(resSq), np.asarray(Rx), np.asarray(test, dtype=float), maxEig)
