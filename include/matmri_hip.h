@@ -223,6 +223,15 @@ int mm_wsnet_create(mm_ctx* ctx, int64_t N1, int64_t N2, mm_wsnet** out);
  * coils of |maps|^2 of the coil operator R (mm_rcvr_plan_create*), the
  * maximum over every slice; x and out are image-side vectors of R. */
 int mm_rcvr_normalize_ssq(mm_op* R, const void* x, void* out, float eps, int on_device);
+/* The same normalisation with no allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * hipGraph stream capture, serving loops, harnesses that forbid hipMalloc.
+ * x and out are device pointers.  Its scratch (per-pixel sums of squares and
+ * per-block maxima: n_image + 1024 floats per run of equal-size slices) is a
+ * static 16 MiB array per device (4194304 floats); larger operators return
+ * MM_ERR_ARG.  Every call on one device shares that array, so calls that
+ * overlap on one device (different streams, or host threads) race. */
+int mm_rcvr_normalize_ssq_noalloc(mm_op* R, const void* x, void* out, float eps);
 int mm_wsnet_num_params(const mm_wsnet* net, int64_t* n);
 /* Sets the FP32 master parameters and resets the optimiser state. */
 int mm_wsnet_set_params(mm_wsnet* net, const float* theta, int on_device);

@@ -26,6 +26,7 @@ EXPORTS = [
     "mm_wsnet_create", "mm_wsnet_num_params", "mm_wsnet_set_params", "mm_wsnet_get_params",
     "mm_wsnet_forward", "mm_wsnet_loss_grad", "mm_wsnet_train_step", "mm_wsnet_destroy",
     "mm_rcvr_normalize_ssq",
+    "mm_rcvr_normalize_ssq_noalloc",
 ]
 
 
@@ -101,6 +102,7 @@ def lib():
     L.mm_wsnet_loss_grad.argtypes = [vp, vp, vp, i64, P(dbl), vp, i32]
     L.mm_wsnet_train_step.argtypes = [vp, vp, vp, i64, C.c_float, P(dbl), i32]
     L.mm_rcvr_normalize_ssq.argtypes = [vp, vp, vp, C.c_float, i32]
+    L.mm_rcvr_normalize_ssq_noalloc.argtypes = [vp, vp, vp, C.c_float]
     L.mm_wsnet_destroy.argtypes = [vp]
     _lib = L
     return L

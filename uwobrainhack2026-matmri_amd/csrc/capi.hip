@@ -2,6 +2,7 @@
 // Every entry point converts exceptions into MM_ERR_* codes and keeps the
 // message for mm_last_error().
 #include <cstring>
+#include <mutex>
 
 #include "cg.hpp"
 #include "hio.hpp"
@@ -148,6 +149,52 @@ int mm_rcvr_normalize_ssq(mm_op* R, const void* x, void* out, float eps, int on_
         rc->normalize_by_coil_ssq(dx.p, dy.p, eps);
         HIPCHECK(hipMemcpyAsync(out, dy.p, sizeof(float2) * n, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"
+
+namespace {
+// Static scratch of mm_rcvr_normalize_ssq_noalloc: per-pixel coil sums of
+// squares and per-block maxima, kNoAllocScratchFloats floats (16 MiB) on
+// every device the library runs on.  Host code uses only the per-device
+// addresses from hipGetSymbolAddress, never the symbol itself.
+constexpr int64_t kNoAllocScratchFloats = int64_t(1) << 22;
+__device__ __align__(256) float scratch[kNoAllocScratchFloats];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on `device` (the current device),
+// looked up once per device
+float* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) fail(MM_ERR_DEVICE, "no-alloc scratch: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        float* p = nullptr;
+        HIPCHECK(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+extern "C" {
+
+int mm_rcvr_normalize_ssq_noalloc(mm_op* R, const void* x, void* out, float eps) {
+    return guarded([&] {
+        if (!R || !x || !out) fail(MM_ERR_ARG, "rcvrOp: NULL argument");
+        auto* rc = dynamic_cast<RcvrOp*>(R);
+        if (!rc) fail(MM_ERR_ARG, "rcvrOp: not a coil operator");
+        if (!(eps >= 0.f)) fail(MM_ERR_ARG, "rcvrOp: eps must be >= 0");
+        const int64_t need = rc->ssq_scratch_floats();
+        if (need > kNoAllocScratchFloats)
+            fail(MM_ERR_ARG, "rcvrOp: no-alloc normalisation needs " + std::to_string(need) +
+                                 " scratch floats, above its static maximum of " +
+                                 std::to_string(kNoAllocScratchFloats) + " (use mm_rcvr_normalize_ssq)");
+        set_device(rc->ctx);
+        float* sp = scratch_on(rc->ctx->device);
+        rc->normalize_by_coil_ssq_with((const float2*)x, (float2*)out, eps, sp, sp + rc->n_image());
     });
 }
 

@@ -148,18 +148,36 @@ __global__ __launch_bounds__(256) void rcvr_norm_k(const float2* __restrict__ x,
     }
 }
 
-void RcvrOp::normalize_by_coil_ssq(const float2* x, float2* out, float eps) {
-    hipStream_t st = ctx->stream;
-    ssq_.ensure((size_t)V_total_);
-    // one set of block maxima per run of equal-size slices
+int RcvrOp::ssq_runs() const {
     int nruns = 0;
     for (size_t s0 = 0; s0 < Vs_.size(); ++nruns) {
         size_t s1 = s0 + 1;
         while (s1 < Vs_.size() && Vs_[s1] == Vs_[s0]) ++s1;
         s0 = s1;
     }
-    ssq_max_.ensure((size_t)nruns * kSsqBlocks);
-    HIPCHECK(hipMemsetAsync(ssq_max_.p, 0, sizeof(float) * nruns * kSsqBlocks, st));
+    return nruns;
+}
+
+int64_t RcvrOp::ssq_scratch_floats() const { return V_total_ + (int64_t)ssq_runs() * kSsqBlocks; }
+
+__global__ __launch_bounds__(256) void fill_zero_f_k(float* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0.f;
+}
+
+void RcvrOp::normalize_by_coil_ssq(const float2* x, float2* out, float eps) {
+    ssq_.ensure((size_t)V_total_);
+    ssq_max_.ensure((size_t)ssq_runs() * kSsqBlocks);
+    normalize_by_coil_ssq_with(x, out, eps, ssq_.p, ssq_max_.p);
+}
+
+// ssq: V_total_ floats; bmax: ssq_runs() * kSsqBlocks floats (cleared here
+// by a fill kernel, so the sequence stays capturable into a graph)
+void RcvrOp::normalize_by_coil_ssq_with(const float2* x, float2* out, float eps, float* ssq, float* bmax) {
+    hipStream_t st = ctx->stream;
+    const int nruns = ssq_runs();
+    hipLaunchKernelGGL(fill_zero_f_k, dim3(grid_for((int64_t)nruns * kSsqBlocks)), dim3(256), 0, st, bmax,
+                       (int64_t)nruns * kSsqBlocks);
+    LAUNCHCHECK();
     int r = 0;
     for (size_t s0 = 0; s0 < Vs_.size(); ++r) {
         size_t s1 = s0 + 1;
@@ -167,11 +185,11 @@ void RcvrOp::normalize_by_coil_ssq(const float2* x, float2* out, float eps) {
         const int64_t V = Vs_[s0], nimg = (int64_t)(s1 - s0);
         const int nb = (int)std::min<int64_t>(kSsqBlocks, (V * nimg + 255) / 256);
         hipLaunchKernelGGL(rcvr_ssq_k, dim3(nb), dim3(256), 0, st, maps_.p + start_[s0] * C_,
-                           ssq_.p + start_[s0], ssq_max_.p + (size_t)r * kSsqBlocks, V, C_, nimg);
+                           ssq + start_[s0], bmax + (size_t)r * kSsqBlocks, V, C_, nimg);
         LAUNCHCHECK();
         s0 = s1;
     }
-    hipLaunchKernelGGL(rcvr_norm_k, dim3(grid_for(V_total_)), dim3(256), 0, st, x, ssq_.p, ssq_max_.p,
+    hipLaunchKernelGGL(rcvr_norm_k, dim3(grid_for(V_total_)), dim3(256), 0, st, x, ssq, bmax,
                        nruns * kSsqBlocks, eps, out, V_total_);
     LAUNCHCHECK();
 }

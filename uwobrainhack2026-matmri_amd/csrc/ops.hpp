@@ -21,8 +21,14 @@ class RcvrOp : public mm_op {
     // out = x / (ssq + eps * max(ssq)), ssq = sum_c |maps_c|^2 per pixel,
     // max over every slice (the warm-start network's input)
     void normalize_by_coil_ssq(const float2* x, float2* out, float eps);
+    // the same with caller-provided scratch: `ssq` holds n_image() floats,
+    // `bmax` ssq_scratch_floats() - n_image() floats; allocates nothing
+    void normalize_by_coil_ssq_with(const float2* x, float2* out, float eps, float* ssq, float* bmax);
+    // floats of scratch normalize_by_coil_ssq_with needs in all
+    int64_t ssq_scratch_floats() const;
 
    private:
+    int ssq_runs() const;   // runs of consecutive slices of one image size
     int C_ = 1;
     std::vector<int64_t> Vs_, start_;
     int64_t V_total_ = 0;
