@@ -461,7 +461,7 @@ def _hio_plans(mm, rng, N=12, K=300, conc=True, mask=False, Lh=16):
     return So, Sg
 
 
-@pytest.mark.parametrize("conc,mask,Lh,variant", [(False, False, 16, 0), (True, True, 16, 0), (True, True, 16, 3),
+@pytest.mark.parametrize("conc,mask,Lh,variant", [(False, False, 16, 0), (True, True, 16, 0), (True, True, 16, 3), (True, True, 16, 7),
                                                   (True, False, 36, 0)])
 def test_highorder_vs_oracle(mm, conc, mask, Lh, variant):
     print(f"[case] highorder conc={conc} mask={mask} Lh={Lh} variant={variant}", flush=True)
@@ -502,7 +502,7 @@ def test_highorder_sense_vs_oracle(mm):
     assert e_f < TOL and e_a < TOL
 
 
-@pytest.mark.parametrize("f0,variant", [(0.0, 0), (70e3, 0), (70e3, 1), (70e3, 2), (70e3, 3)])
+@pytest.mark.parametrize("f0,variant", [(0.0, 0), (70e3, 0), (70e3, 1), (70e3, 2), (70e3, 3), (70e3, 7)])
 def test_highorder_full_grid_precision(mm, f0, variant):
     """220x220 grid, 32 coils, 16 harmonics + B0 of the high-order workload on
     1200 of its 24000 time points (spread over the whole readout, so the late

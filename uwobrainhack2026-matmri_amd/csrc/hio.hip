@@ -92,6 +92,13 @@ HioOp::HioOp(mm_ctx* c, int64_t V, int64_t K, int Lh, int Lc, const double* x, c
     : V_(V), K_(K) {
     ctx = c;
     if (V < 1 || K < 1) fail(MM_ERR_ARG, "sampHighOrder: empty image or k-space");
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && ncu > 0)
+            num_cu_ = ncu;
+        else
+            (void)hipGetLastError();
+    }
     if (Lh < 0 || Lh > 36 || Lc < 0 || Lc > 4) fail(MM_ERR_ARG, "sampHighOrder: basis count out of range");
     if (!x || !y || !z || !t || (Lh > 0 && !h) || (Lc > 0 && !g))
         fail(MM_ERR_ARG, "sampHighOrder: NULL argument");
@@ -511,7 +518,7 @@ void HioOp::run_mfma(int adjoint, const float2* in, float2* out, int ncol, const
     c.amax = amax_.p;
     c.variant = variant_;
     const int gwmax = maps ? C * (32 / C) : 32;
-    hio_mfma_split(c.nrows, c.nblk, &c.nsplit, &c.bps);
+    hio_mfma_split(c.nrows, c.nblk, 2 * num_cu_, &c.nsplit, &c.bps);
     Bs_.ensure(hio_mfma_bs_bytes(c.nblk, std::min(gwmax, ncol)));
     mpart_.ensure((size_t)c.nsplit * c.nrows * 64);
     c.Bs = Bs_.p;

@@ -21,7 +21,7 @@ class HioOp : public mm_op {
                           double* ms, std::string* name) override;
     bool set_option(const std::string& key, int value) override {
         ++gen_;
-        if (key == "hio_variant") { variant_ = value & 3; return true; }
+        if (key == "hio_variant") { variant_ = value & 7; return true; }
         if (key == "hio_mfma") { use_mfma_ = value != 0; return true; }
         return false;
     }
@@ -52,6 +52,7 @@ class HioOp : public mm_op {
     int64_t Vp_ = 0, Kp_ = 0;   // row counts padded to a multiple of 16
     int Lnb_ = 0;    // terms without the B0 column (phiDiv basis)
     int Lh_ = 0, Lc_ = 0;
+    int num_cu_ = 256;   // compute units of the device (work-group slots of the MFMA GEMM)
     double maxphi_ = 0;
     DevBuf<double> Q_;    // [V][r] compressed voxel basis
     DevBuf<double> Hc_;   // [K][r] compressed coefficients, revolutions

@@ -80,6 +80,24 @@ __device__ __forceinline__ void sincos_rev_hw(double ph, float& c, float& s) {
     c = __builtin_amdgcn_cosf(r);
 }
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// cos/sin of 2*pi*ph (revolutions, FP64) with the hardware sine/cosine on
+// the fractional revolution, split into fp16 hi and scaled lo parts for the
+// A operand: (ah.x, ah.y) = hi(cos, sin), (al.x, al.y) = lo(cos, sin) * 2048.
+// The hi pair is one packed round-toward-zero conversion, and each lo is
+// 2048*v - 2048*hi in one mixed-precision FMA (hi read back as fp16), both
+// exact: v - hi is representable and the scaling is a power of two.
+__device__ __forceinline__ void sincos_split_hw(double ph, half2_t& ah, half2_t& al) {
+    const float r = (float)__builtin_amdgcn_fract(ph);   // [0, 1) revolution
+    const float s = __builtin_amdgcn_sinf(r);
+    const float c = __builtin_amdgcn_cosf(r);
+    ah = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(c, s));
+    const float lc = fmaf((float)ah.x, -kLoScale, c * kLoScale);
+    const float ls = fmaf((float)ah.y, -kLoScale, s * kLoScale);
+    al = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(lc, ls));
+}
+
 // Variants: MPHASE computes the phase tile on the FP64 MFMA (16x16x4: lane l
 // holds rows l/16 + 4i of column l%16, so the A rows are permuted to deliver
 // the 4 consecutive reduction indices the f16 16x16x32 A operand needs),
@@ -173,9 +191,17 @@ __global__ __launch_bounds__(512) void hio_mfma_gemm(const double* __restrict__ 
         for (int t = 0; t < RT; ++t) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
+                if constexpr (HWSIN) {
+                    half2_t h2, l2;
+                    sincos_split_hw(ph[t][i], h2, l2);
+                    Ah[t][2 * i] = h2.x;
+                    Ah[t][2 * i + 1] = h2.y;
+                    Al[t][2 * i] = l2.x;
+                    Al[t][2 * i + 1] = l2.y;
+                    continue;
+                }
                 float cv, sv;
-                if constexpr (HWSIN) sincos_rev_hw(ph[t][i], cv, sv);
-                else sincos_rev(ph[t][i], cv, sv);
+                sincos_rev(ph[t][i], cv, sv);
                 _Float16 h, l;
                 split_f16(cv, h, l);
                 Ah[t][2 * i] = h;
@@ -212,6 +238,123 @@ __global__ __launch_bounds__(512) void hio_mfma_gemm(const double* __restrict__ 
                 const int row = rowbase + 16 * t + 4 * g + i;
                 if (row < nrows)
                     part[((size_t)split * nrows + row) * (16 * NCT) + 16 * c + l16] =
+                        accm[t][c][i] + accl[t][c][i] * kLoInv;
+            }
+}
+
+// Software-pipelined form of the FP64-MFMA-phase / hardware-sine variant.
+// Iteration b issues the fp16 MFMAs of reduction block b and, independent of
+// them in the same basic block, the FP64 phase MFMAs, sine/cosine and fp16
+// split of block b+1, so one wave keeps the matrix pipe busy while its VALU
+// prepares the next A operands.  The reduction factors come straight from
+// global memory (L2) two blocks ahead; only the pre-split B' operands go
+// through double-buffered LDS.  Loads use clamped, always-valid indices (a
+// load under a branch would be waited for at the join); the extra phase
+// work for the clamped block b+1 of the last iteration is discarded.  Same
+// products, same accumulation order as hio_mfma_gemm<.., true, true>.
+template <int NCT, int RT, int RP>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4))) void hio_mfma_gemm_pipe(const double* __restrict__ rowfac, int nrows,
+                                                          const double* __restrict__ redfac, int nblk_total,
+                                                          const half8_t* __restrict__ Bs,
+                                                          float* __restrict__ part, int blk_per_split) {
+    constexpr int NB = NCT * 2 * 64;   // half8 operands per reduction block
+    constexpr int NS = RP / 4;         // FP64 MFMA K-steps
+    __shared__ half8_t sB[2][NB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int rowbase = blockIdx.x * (8 * RT * 16) + wave * RT * 16;
+    const int b0 = blockIdx.y * blk_per_split;
+    const int b1 = min(nblk_total, b0 + blk_per_split);
+    const int tid = threadIdx.x;
+    // FP64 A row m holds reduction index 4*(m%4) + m/4, so the MFMA result
+    // lane item i is index 4g + i, the order the f16 A operand needs
+    const int arow = 4 * (l16 & 3) + (l16 >> 2);
+    double rf[RT][NS];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int row = rowbase + 16 * t + l16;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) rf[t][q] = row < nrows ? rowfac[(size_t)row * RP + 4 * q + g] : 0.0;
+    }
+    auto load_a = [&](int b, double (&a)[NS]) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) a[q] = redfac[((size_t)b * 16 + arow) * RP + 4 * q + g];
+    };
+    auto make_a = [&](const double (&a)[NS], half8_t (&Ah)[RT], half8_t (&Al)[RT]) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            double4_t d = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < NS; ++q) d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], rf[t][q], d, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                half2_t h2, l2;
+                sincos_split_hw(d[i], h2, l2);
+                Ah[t][2 * i] = h2.x;
+                Ah[t][2 * i + 1] = h2.y;
+                Al[t][2 * i] = l2.x;
+                Al[t][2 * i + 1] = l2.y;
+            }
+        }
+    };
+    float4_t accm[RT][NCT], accl[RT][NCT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            accm[t][c] = float4_t{0.f, 0.f, 0.f, 0.f};
+            accl[t][c] = float4_t{0.f, 0.f, 0.f, 0.f};
+        }
+    if (b0 < b1) {
+        const int blast = b1 - 1;
+        half8_t Ah[RT], Al[RT];
+        double an[NS];
+        {
+            double a0[NS];
+            load_a(b0, a0);
+            load_a(min(b0 + 1, blast), an);
+            if (tid < NB) sB[0][tid] = Bs[(size_t)b0 * NB + tid];
+            make_a(a0, Ah, Al);
+        }
+        __syncthreads();
+        for (int b = b0; b < b1; ++b) {
+            const int buf = (b - b0) & 1;
+            const half8_t pfB = Bs[(size_t)min(b + 1, blast) * NB + (tid % NB)];
+            double an2[NS];
+            load_a(min(b + 2, blast), an2);
+            half8_t Nh[RT], Nl[RT];
+            make_a(an, Nh, Nl);
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) {
+                const half8_t Bh = sB[buf][(c * 2 + 0) * 64 + lane];
+                const half8_t Bl = sB[buf][(c * 2 + 1) * 64 + lane];
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    accm[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[t], Bh, accm[t][c], 0, 0, 0);
+                    accl[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[t], Bl, accl[t][c], 0, 0, 0);
+                    accl[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[t], Bh, accl[t][c], 0, 0, 0);
+                }
+            }
+            if (b + 1 < b1 && tid < NB) sB[buf ^ 1][tid] = pfB;
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                Ah[t] = Nh[t];
+                Al[t] = Nl[t];
+            }
+#pragma unroll
+            for (int q = 0; q < NS; ++q) an[q] = an2[q];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int c = 0; c < NCT; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = rowbase + 16 * t + 4 * g + i;
+                if (row < nrows)
+                    part[((size_t)blockIdx.y * nrows + row) * (16 * NCT) + 16 * c + l16] =
                         accm[t][c][i] + accl[t][c][i] * kLoInv;
             }
 }
@@ -371,6 +514,11 @@ static int grid_1d(int64_t n, int bs = 256) {
 
 template <int NCT, int RP>
 static void launch_gemm(hipStream_t st, const HioMfmaCall& c, int gx, int nsplit, int bps) {
+    if (c.variant & 4) {
+        hipLaunchKernelGGL((hio_mfma_gemm_pipe<NCT, 2, RP>), dim3(gx, nsplit), dim3(512), 0, st, c.rowfac, c.nrows,
+                           c.redfac, c.nblk, (const half8_t*)c.Bs, c.part, bps);
+        return;
+    }
     switch (c.variant & 3) {
         case 0:
             hipLaunchKernelGGL((hio_mfma_gemm<NCT, 2, RP, false, false>), dim3(gx, nsplit), dim3(512), 0, st,
@@ -454,13 +602,32 @@ void hio_mfma_apply(hipStream_t st, const HioMfmaCall& c, int prof_iters, double
 
 size_t hio_mfma_bs_bytes(int nblk, int gw) { return (size_t)nblk * (gw <= 16 ? 2 : 4) * 128 * 16; }
 
-void hio_mfma_split(int nrows, int nblk, int* nsplit, int* bps) {
+// Reduction splits: a launch is gx x nsplit work-groups of equal work
+// (bps blocks each), resident `slots` at a time (two 512-thread work-groups
+// per CU at 4 waves per SIMD), so it runs ceil(gx*nsplit/slots) rounds of
+// bps blocks.  Pick the split count minimising rounds * bps plus the cost of
+// writing and re-reading one more set of partial sums (~1.3 blocks of work
+// per split).  A count just above a multiple of the slots leaves a nearly
+// empty last round: 94 x 11 = 1034 work-groups on 512 slots take three
+// rounds of 276 blocks where 94 x 16 takes three of 190.
+void hio_mfma_split(int nrows, int nblk, int slots, int* nsplit, int* bps) {
     const int gx = (nrows + 255) / 256;
-    int ns = std::max(1, (1024 + gx - 1) / gx);
-    ns = std::min(ns, std::max(1, nblk / 8));
-    int b = (nblk + ns - 1) / ns;
-    ns = (nblk + b - 1) / b;
-    *nsplit = ns;
+    slots = std::max(slots, 1);
+    const int nsmax = std::max(1, std::min(64, nblk / 8));
+    int best = 1;
+    double best_cost = 1e300;
+    for (int ns = 1; ns <= nsmax; ++ns) {
+        const int b = (nblk + ns - 1) / ns;
+        const int nsu = (nblk + b - 1) / b;   // splits actually holding blocks
+        const int64_t wgs = (int64_t)gx * nsu;
+        const double cost = (double)((wgs + slots - 1) / slots) * b + 1.3 * nsu;
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = ns;
+        }
+    }
+    const int b = (nblk + best - 1) / best;
+    *nsplit = (nblk + b - 1) / b;
     *bps = b;
 }
 

@@ -29,6 +29,6 @@ struct HioMfmaCall {
 
 void hio_mfma_apply(hipStream_t st, const HioMfmaCall& c, int prof_iters, double* prof_ms);
 size_t hio_mfma_bs_bytes(int nblk, int gw);
-void hio_mfma_split(int nrows, int nblk, int* nsplit, int* bps);
+void hio_mfma_split(int nrows, int nblk, int slots, int* nsplit, int* bps);
 
 }  // namespace mm
