@@ -20,10 +20,18 @@
  *   mm_hio_set_phi_div      sampHighOrder.setPhiDiv  sampHighOrder.m:590-622
  *   mm_nufft_set_toeplitz   nufftOp.prepToep          nufftOp.m:410-432 (applied :348-352)
  *   mm_sense_create         mrSampFunc(x,transp,samplingOp,R,opt)  operators/mrSampFunc.m:82-159
+ *   mm_cart_plan_create     mrSampFunc's Cartesian-mask samplingOp  mrSampFunc.m:108-114, :137-143
+ *   mm_sumdims_plan_create  mrSampFunc's opt.sumDims                mrSampFunc.m:100-105
+ *   mm_interp_traj_time     interpTrajTime(phs_spha,phs_conc,tdwell,delTime,datatime)
+ *                           findDel/interpTrajTime.m:1-27
+ *   mm_hio_dense_normal     mrSampFuncMat(y,Crcvr,b0,sampTimes,phs_spha,phs_conc,phs_grid,imMask)
+ *                           operators/mrSampFuncMat.m:1-47
  *   mm_op_apply             A*x / A'*y  (mtimes/ctranspose overloads:
  *                           nufftOp.m:276-408, rcvrOp.m:96-159,
  *                           sampHighOrder.m:293-391, mrSampFunc.m:82-159)
  *   mm_cgne_solve           cgne(A,b,x0,NitMax,opt,gam,R,W)  iterativeSolvers/cgne.m:1-341
+ *   mm_cgne_solve_ex        the same with the gam/R Tikhonov terms and opt.gtruth
+ *                           (cgne.m:171-179, :219-227, :201-204, :271-274)
  *
  * `bfista`/`powermethod` (iterativeSolvers/bfista.m, powermethod.m) stay on
  * the host side and call mm_op_apply; their regulariser R is host code.
@@ -111,6 +119,45 @@ int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd);
  * b = E'(W.*y) needs the gridding form. */
 int mm_nufft_set_toeplitz(mm_op* op, const float* psf);
 
+/* Cartesian sampling: mrSampFunc with a k-space mask as samplingOp
+ * (operators/mrSampFunc.m:108-114, :137-143): forward y = mask .* ifftnc(x),
+ * adjoint x = fftnc(mask .* y), the centred unitary transforms of
+ * general/ifftnc.m and general/fftnc.m over the ndim (1 or 2) image
+ * dimensions; imN[0] (x imN[1]) is the image and k-space size, mask a real
+ * column-major [V] array.  Dimensions up to 2730 with prime factors <= 7 run
+ * as FFTs, other lengths up to 65536 as direct DFTs (MM_ERR_UNSUPPORTED
+ * above).  Usable on its own or as the S of mm_sense_create. */
+int mm_cart_plan_create(mm_ctx* ctx, int ndim, const int64_t* imN, const float* mask, mm_op** out);
+
+/* opt.sumDims of mrSampFunc (operators/mrSampFunc.m:100-105): forward sums a
+ * column-major array of nd <= 7 dimensions `dims` over the dimensions whose
+ * sum_flags[d] is nonzero (kept as singletons); adjoint broadcasts back along
+ * them, as MATLAB's implicit expansion does in the 'transp' branch.  Trailing
+ * repetitions (nrep of mm_op_apply) are kept. */
+int mm_sumdims_plan_create(mm_ctx* ctx, int nd, const int64_t* dims, const int* sum_flags, mm_op** out);
+
+/* [AtA,Aty] = mrSampFuncMat(y,Crcvr,b0,sampTimes,phs_spha,phs_conc,phs_grid,
+ * imMask) (operators/mrSampFuncMat.m:1-47), the AtA part, for the high-order
+ * operator `op` (mm_hio_plan_create without b0mask): over the Vm voxels with
+ * mask[v] != 0 (mask [V] or NULL = all, column-major order),
+ * AtA[v,w] = G[v,w] * sum_c conj(maps_c(v)) maps_c(w) with G = S'*S and
+ * S = exp(1i*single(kbase))/sqrt(Vm) the dense [K x Vm] encoding matrix in
+ * single precision, as the reference forms it.  maps: host [V x C] c64;
+ * AtA: [Vm x Vm] c64, host or device per on_device; *Vm_out receives Vm
+ * (AtA == NULL: size query only).  The Gram product runs on rocBLAS (cgemm);
+ * workspace K*Vm + (host output) Vm*Vm complex values.  Aty is the adjoint
+ * of the SENSE operator (mm_op_apply) restricted to the mask. */
+int mm_hio_dense_normal(mm_op* op, int64_t C, const float* maps, const unsigned char* mask, void* AtA,
+                        int64_t* Vm_out, int on_device);
+
+/* [phs_spha,phs_conc] = interpTrajTime(...) (findDel/interpTrajTime.m:1-27)
+ * for one array: out [nq x ncol] = interp1(trajTim, phs, datatime, 'makima',
+ * NaN) along the first dimension of phs [L x ncol] (column-major), with
+ * trajTim = -delTime + tdwell*(0:L-1); FP64.  Pointers host or device per
+ * on_device. */
+int mm_interp_traj_time(mm_ctx* ctx, const double* phs, int64_t L, int ncol, double tdwell, double delTime,
+                        const double* datatime, int64_t nq, double* out, int on_device);
+
 /* SENSE encoding E = S o R with optional real data weights W ([M x C], used
  * only by mm_cgne_solve and by mm_op_apply_normal). */
 int mm_sense_create(mm_ctx* ctx, mm_op* S, mm_op* R, const float* W, mm_op** out);
@@ -160,6 +207,17 @@ void mm_cg_opts_default(mm_cg_opts* o);
 int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax,
                   const mm_cg_opts* opt, double* resSqAll, double* xnormAll,
                   double* xdiffAll, int* nit, int* status, int on_device);
+/* cgne with the Tikhonov terms and the ground-truth history of cgne.m:
+ * the normal operator becomes E'(W.*E x) + sum_n gam[n] R[n]'(R[n] x)
+ * (cgne.m:171-179, :219-227), R[n] a device operator on E's image (e.g. a
+ * coil, Cartesian or NUFFT operator) or NULL for the identity, gam[n] = 0
+ * dropping the term (cgne.m:91-93); gtruth (image, host/device per
+ * on_device, or NULL) fills mseAll with |gtruth - x_n|^2 per iteration
+ * (cgne.m:201-204, :271-274), laid out like resSqAll (zeros without it). */
+int mm_cgne_solve_ex(mm_op* E, const void* y, const void* x0, void* x, int NitMax,
+                     const mm_cg_opts* opt, int nreg, const double* gam, mm_op* const* R,
+                     const void* gtruth, double* resSqAll, double* mseAll, double* xnormAll,
+                     double* xdiffAll, int* nit, int* status, int on_device);
 
 /* Measurement hook (no counterpart in the reference): times the operator's
  * dominant kernel with HIP events on the context stream (device pointers

@@ -49,7 +49,7 @@ def mrSampFunc(x, transp, samplingOp, R=None, opt=None):
         if R is not None:
             x = R.forward(x)
         if sumDims is not None:
-            for d in sumDims:
+            for d in np.atleast_1d(sumDims):
                 x = np.sum(x, axis=d - 1, keepdims=True)
         if is_op:
             x = samplingOp.forward(x)

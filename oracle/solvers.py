@@ -54,12 +54,18 @@ def cgne(Ain, bin, x0=None, NitMax=None, opt=None, gam=None, Rin=None, W=None, n
     stopOnXdifInc = opt.get("stopOnXdifInc", 0)
     noiseVar = opt.get("noiseVar", None)
     gtruth = opt.get("gtruth", None)
-    if gam is not None and np.all(np.asarray(gam) == 0):           # :91-93
+    cell = isinstance(gam, (list, tuple))                           # gam/R as MATLAB cells
+    if gam is not None and not cell and np.all(np.asarray(gam) == 0):   # :91-93
         gam = None
     if gam is None:
         Rin = None
     A = _as_handle(Ain)
-    R = None if Rin is None else _as_handle(Rin)
+    if Rin is None:
+        R = None
+    elif cell:                                                      # :117-126
+        R = [_as_handle(r) for r in Rin]
+    else:
+        R = _as_handle(Rin)
 
     b = A(bin if W is None else W * bin, "transp")                  # :136-140
     if x0 is None:                                                  # :143-145
@@ -77,8 +83,12 @@ def cgne(Ain, bin, x0=None, NitMax=None, opt=None, gam=None, Rin=None, W=None, n
 
     def normal(v):
         q = A(A(v, "notransp") if W is None else W * A(v, "notransp"), "transp")
-        if gam is not None:
-            q = q + gam * R(R(v, "notransp"), "transp")
+        if gam is not None:                                         # :171-179, :219-227
+            if cell:
+                for g, r in zip(gam, R):
+                    q = q + g * r(r(v, "notransp"), "transp")
+            else:
+                q = q + gam * R(R(v, "notransp"), "transp")
         return q
 
     xold = x0

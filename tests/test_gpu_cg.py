@@ -363,3 +363,31 @@ def test_cgne_highorder_220_32coils_30_iterations(mm):
     assert maxphi > 1e4 and st == 0 and len(rg) == 31
     assert tight.sum() >= 10 and np.all(dg[tight] < 1e-3)
     assert e < 3 * d
+
+
+def test_cgne_tikhonov_terms_and_mse_history(mm):
+    """Tikhonov terms in the device solve (cgne.m:171-179, :219-227): gam and
+    R as cells with R_1 = identity and R_2 a Cartesian k-space weighting
+    (cartOp, a device operator), and the mse history against a ground truth
+    (cgne.m:201-204, :271-274); 8 iterations against the FP64 oracle:
+    iterate NRMSE <= 1e-4, resSq and mse histories within 1e-3."""
+    print("[case] cgne with gam = {eye, Cartesian weighting} and opt.gtruth, N=64 C=4", flush=True)
+    So, Ro, Sg, Rg, y, ph = _problem(mm, seed=15)
+    kx = np.fft.fftfreq(64)
+    mask = (np.hypot(kx[:, None], kx[None, :]) > 0.3).astype(float)   # a high-pass penalty
+    A = lambda x, tr: o_mrSampFunc(x, tr, So, Ro)
+    gam = [0.02, 0.5]
+    eye_o = lambda v, tr: v
+    cart_o = lambda v, tr: o_mrSampFunc(v, tr, mask)
+    opt = {"resThresh": None, "xChangeThresh": None, "gtruth": ph}
+    x0 = np.zeros((64, 64), complex)
+    xo, ro, mo, *_ = o_cgne(A, y, x0, 8, opt, gam, [eye_o, cart_o])
+    xg, rg, mg, *_, st = mm.cgne(mm.mrSampHandle(Sg, Rg), y, x0, 8, opt, gam, [mm.eye, mm.cartOp(mask)])
+    ro, mo = np.real(ro), np.real(mo)
+    e = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    dr, dm = np.max(np.abs(rg - ro) / ro), np.max(np.abs(mg - mo) / mo)
+    print(f"  x nrmse {e:.2e}; resSq max rel {dr:.2e}; mse max rel {dm:.2e}", flush=True)
+    assert st == 0 and e < 1e-4 and dr < 1e-3 and dm < 1e-3
+    # without them the solve differs (the terms are applied)
+    xn, *_ = mm.cgne(mm.mrSampHandle(Sg, Rg), y, x0, 8, {"resThresh": None, "xChangeThresh": None})
+    assert np.linalg.norm(xn - xg) / np.linalg.norm(xg) > 1e-3

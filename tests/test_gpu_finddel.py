@@ -1,0 +1,56 @@
+"""interpTrajTime and findDelAuto (findDel/) on the device against the
+oracle, on the reference's spiral trajectory (tests/golden/traj_spiral_R4.npz)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_pkg
+from oracle.finddel import findDelAuto as o_findDelAuto
+from oracle.finddel import interpTrajTime as o_interpTrajTime
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "traj_spiral_R4.npz")
+
+
+@pytest.fixture(scope="module")
+def mm():
+    m = load_pkg()
+    if m.device_count() < 1:
+        pytest.fail("no GPU visible to libmatmri_hip")
+    return m
+
+
+@pytest.mark.parametrize("delay,start", [(0.0, 50.0), (2.0, 50.0), (-3.7, 0.0), (1.25, 14995.0)])
+def test_interp_traj_time_vs_oracle(mm, delay, start):
+    """The device makima resampling (FP64) of phs_spha [15000 x 16] and
+    phs_conc [15000 x 4] at the acquisition times, delayed: equal to the
+    oracle to FP64 rounding, NaN at the same places (times past the end)."""
+    print(f"[case] interpTrajTime delay={delay} start={start}", flush=True)
+    d = np.load(GOLD)
+    dt = d["datatime"].reshape(-1) + start
+    a, b = mm.finddel.interpTrajTime(d["phs_spha"], d["phs_conc"], 1.0, delay, dt)
+    ao, bo = o_interpTrajTime(d["phs_spha"], d["phs_conc"], 1.0, delay, dt)
+    for u, v in ((a, ao), (b, bo)):
+        assert u.shape == v.shape
+        assert np.array_equal(np.isnan(u), np.isnan(v))
+        ok = ~np.isnan(v)
+        err = np.max(np.abs(u[ok] - v[ok])) / np.max(np.abs(v[ok])) if ok.any() else 0.0
+        print(f"  max rel err {err:.2e}, NaN {int((~ok).sum())}", flush=True)
+        assert err < 1e-12
+
+
+def test_find_del_auto_vs_oracle(mm):
+    """findDelAuto.m on a 24x24 / 3-coil high-order problem simulated with a
+    2 us delay on 2000 samples of the spiral: the device search (device
+    sampHighOrder, cgne, setPhiDiv) lands where the FP64 oracle's does
+    (within 0.01 us) and within 0.02 us of the applied delay."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle_finddel import _delay_problem
+    print("[case] findDelAuto 24x24 C=3 K=2000, delay 2 us", flush=True)
+    d, dt, grid, b0, maps, y = _delay_problem()
+    est, per = mm.finddel.findDelAuto(0.0, y, 1.0, d["phs_spha"], d["phs_conc"], dt, grid, b0, maps, 12)
+    est_o, per_o = o_findDelAuto(0.0, y, 1.0, d["phs_spha"], d["phs_conc"], dt, grid, b0, maps, 12)
+    print(f"  device {est:.4f} ({len(per)} steps), oracle {est_o:.4f} ({len(per_o)} steps)", flush=True)
+    assert abs(est - est_o) < 0.01 and abs(est - 2.0) < 0.02

@@ -221,6 +221,30 @@ def test_bfista_kmask_projection_closed_form():
         bfista(eye, b, eye, 0.3, None, 3, {"maxEig": 1.0, "kMask": np.ones(N1)})
 
 
+def test_cgne_tikhonov_cell_terms():
+    """gam and R as MATLAB cells (cgne.m:117-126, :171-179): the solution of
+    (A'A + g1 R1'R1 + g2 R2'R2) x = A'b, and the mse history |gtruth - x_n|^2
+    (cgne.m:201-204, :271-274) equals the distance of each iterate."""
+    rng = np.random.default_rng(44)
+    A = _rand_c(rng, (14, 10))
+    R1, R2 = _rand_c(rng, (6, 10)), np.eye(10)
+    b = _rand_c(rng, 14)
+    g = [0.4, 0.15]
+    gt = _rand_c(rng, 10)
+    x, res, mse, *_ = cgne(Dense(A), b, np.zeros(10, complex), 60, {"resThresh": 1e-26, "xChangeThresh": None,
+                                                                     "gtruth": gt}, g, [Dense(R1), Dense(R2)])
+    lhs = A.conj().T @ A + g[0] * R1.conj().T @ R1 + g[1] * np.eye(10)
+    xs = np.linalg.solve(lhs, A.conj().T @ b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+    assert abs(np.real(mse[0]) - np.linalg.norm(gt) ** 2) < 1e-9
+    assert abs(np.real(mse[-1]) - np.linalg.norm(gt - x) ** 2) < 1e-8 * np.linalg.norm(gt) ** 2
+    # a single R equals the same term given as a one-element cell
+    x1, *_ = cgne(Dense(A), b, np.zeros(10, complex), 5, {"resThresh": None, "xChangeThresh": None}, 0.4, Dense(R1))
+    x2, *_ = cgne(Dense(A), b, np.zeros(10, complex), 5, {"resThresh": None, "xChangeThresh": None}, [0.4],
+                  [Dense(R1)])
+    assert np.allclose(x1, x2, rtol=0, atol=1e-14)
+
+
 def test_cgne_fp32_io_sensitivity():
     """Pins the tolerance of the device cgne tests: on a 64^2 / 4-coil spiral
     SENSE problem solved to relative residual 1e-4, rounding only the input and

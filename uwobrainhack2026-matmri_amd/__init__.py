@@ -6,10 +6,12 @@ The operators and solvers keep the reference's MATLAB interfaces
 of ``include/matmri_hip.h`` (``libmatmri_hip.so`` in this directory).
 """
 from ._lib import MMError, OutOfMemory, device_count, lib, LIB_PATH, EXPORTS  # noqa: F401
-from .operators import (LinearOp, mrSampFunc, mrSampHandle, nufftOp, rcvrOp,  # noqa: F401
-                        sampHighOrder, senseOp)
+from .operators import (LinearOp, cartOp, mrSampFunc, mrSampHandle, nufftOp, rcvrOp,  # noqa: F401
+                        sampHighOrder, senseOp, sumDimsOp)
 from . import workloads  # noqa: F401
-from .solvers import bfista, cgne, powermethod, set_cg_graph, softthresh  # noqa: F401
+from . import finddel  # noqa: F401
+from .finddel import findDelAuto, interpTrajTime, load_traj_mat  # noqa: F401
+from .solvers import bfista, cgne, eye, powermethod, set_cg_graph, softthresh  # noqa: F401
 
 
 def __getattr__(name):
@@ -21,5 +23,6 @@ def __getattr__(name):
         return mod if name == "warmstart" else mod.WarmStartNet
     raise AttributeError(name)
 
-__all__ = ["nufftOp", "rcvrOp", "sampHighOrder", "senseOp", "mrSampFunc", "mrSampHandle", "cgne",
-           "bfista", "powermethod", "softthresh", "set_cg_graph", "device_count", "lib", "WarmStartNet"]
+__all__ = ["nufftOp", "rcvrOp", "sampHighOrder", "senseOp", "cartOp", "sumDimsOp", "mrSampFunc", "mrSampHandle", "cgne",
+           "bfista", "powermethod", "softthresh", "set_cg_graph", "device_count", "lib", "WarmStartNet",
+           "findDelAuto", "interpTrajTime", "load_traj_mat", "eye"]

@@ -26,6 +26,11 @@ EXPORTS = [
     "mm_wsnet_create", "mm_wsnet_num_params", "mm_wsnet_set_params", "mm_wsnet_get_params",
     "mm_wsnet_forward", "mm_wsnet_loss_grad", "mm_wsnet_train_step", "mm_wsnet_destroy",
     "mm_rcvr_normalize_ssq",
+    "mm_cart_plan_create",
+    "mm_sumdims_plan_create",
+    "mm_cgne_solve_ex",
+    "mm_interp_traj_time",
+    "mm_hio_dense_normal",
     "mm_rcvr_normalize_ssq_noalloc",
 ]
 
@@ -74,6 +79,10 @@ def lib():
     L.mm_rcvr_plan_create_multi.argtypes = [vp, i32, P(i64), i64, vp, P(vp)]
     L.mm_hio_plan_create.argtypes = [vp, i64, i64, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, P(vp)]
     L.mm_hio_set_phi_div.argtypes = [vp, vp, vp]
+    L.mm_cart_plan_create.argtypes = [vp, i32, P(i64), vp, P(vp)]
+    L.mm_sumdims_plan_create.argtypes = [vp, i32, P(i64), P(i32), P(vp)]
+    L.mm_interp_traj_time.argtypes = [vp, vp, i64, i32, dbl, dbl, vp, i64, vp, i32]
+    L.mm_hio_dense_normal.argtypes = [vp, i64, vp, vp, vp, P(i64), i32]
     L.mm_nufft_set_toeplitz.argtypes = [vp, vp]
     L.mm_sense_create.argtypes = [vp, vp, vp, vp, P(vp)]
     L.mm_op_shape.argtypes = [vp, P(i64), P(i64)]
@@ -84,6 +93,7 @@ def lib():
     L.mm_cg_opts_default.argtypes = [P(CgOpts)]
     L.mm_cg_opts_default.restype = None
     L.mm_cgne_solve.argtypes = [vp, vp, vp, vp, i32, P(CgOpts), vp, vp, vp, vp, vp, i32]
+    L.mm_cgne_solve_ex.argtypes = [vp, vp, vp, vp, i32, P(CgOpts), i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32]
     L.mm_cg_set_graph.argtypes = [vp, i32]
     L.mm_cg_last_fallback.argtypes = [vp, vp]
     L.mm_op_destroy.argtypes = [vp]

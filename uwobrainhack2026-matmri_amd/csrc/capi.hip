@@ -4,9 +4,11 @@
 #include <cstring>
 #include <mutex>
 
+#include "cart.hpp"
 #include "cg.hpp"
 #include "hio.hpp"
 #include "ops.hpp"
+#include "traj.hpp"
 #include "wsnet.hpp"
 
 using namespace mm;
@@ -91,6 +93,7 @@ int mm_ctx_destroy(mm_ctx* ctx) {
         if (!ctx) return;
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        blas_handle_destroy(ctx->blas);
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -208,6 +211,84 @@ int mm_hio_plan_create(mm_ctx* ctx, int64_t V, int64_t K, int Lh, int Lc, const 
     });
 }
 
+int mm_cart_plan_create(mm_ctx* ctx, int ndim, const int64_t* imN, const float* mask, mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out || !imN || !mask) fail(MM_ERR_ARG, "mrSampFunc: NULL argument");
+        *out = nullptr;
+        set_device(ctx);
+        *out = new CartOp(ctx, ndim, imN, mask);
+    });
+}
+
+int mm_sumdims_plan_create(mm_ctx* ctx, int nd, const int64_t* dims, const int* sum_flags, mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(MM_ERR_ARG, "mrSampFunc: NULL argument");
+        *out = nullptr;
+        set_device(ctx);
+        *out = new SumDimsOp(ctx, nd, dims, sum_flags);
+    });
+}
+
+int mm_interp_traj_time(mm_ctx* ctx, const double* phs, int64_t L, int ncol, double tdwell, double delTime,
+                        const double* datatime, int64_t nq, double* out, int on_device) {
+    return guarded([&] {
+        if (!ctx || !phs || !datatime || !out) fail(MM_ERR_ARG, "interpTrajTime: NULL argument");
+        set_device(ctx);
+        if (on_device) {
+            interp_traj_time(ctx->stream, phs, L, ncol, tdwell, delTime, datatime, nq, out);
+            return;
+        }
+        if (L < 1 || ncol < 1 || nq < 0) fail(MM_ERR_ARG, "interpTrajTime: bad sizes");
+        DevBuf<double> dp, dt, dout;
+        dp.upload(phs, (size_t)L * ncol, ctx->stream);
+        dt.upload(datatime, (size_t)nq, ctx->stream);
+        dout.alloc((size_t)nq * ncol);
+        interp_traj_time(ctx->stream, dp.p, L, ncol, tdwell, delTime, dt.p, nq, dout.p);
+        if (nq) HIPCHECK(hipMemcpyAsync(out, dout.p, sizeof(double) * nq * ncol, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mm_hio_dense_normal(mm_op* op, int64_t C, const float* maps, const unsigned char* mask, void* AtA,
+                        int64_t* Vm_out, int on_device) {
+    return guarded([&] {
+        auto* h = dynamic_cast<HioOp*>(op);
+        if (!h) fail(MM_ERR_ARG, "mrSampFuncMat: not a high-order operator");
+        if (!maps || C < 1) fail(MM_ERR_ARG, "mrSampFuncMat: NULL maps");
+        mm_ctx* ctx = op->ctx;
+        set_device(ctx);
+        const int64_t V = h->n_image(), K = h->samples();
+        std::vector<int> vidx;
+        for (int64_t v = 0; v < V; ++v)
+            if (!mask || mask[v]) vidx.push_back((int)v);
+        const int64_t Vm = (int64_t)vidx.size();
+        if (Vm_out) *Vm_out = Vm;
+        if (!AtA) return;   // size query
+        if (Vm == 0) fail(MM_ERR_ARG, "mrSampFuncMat: empty image mask");
+        // masked maps [Vm x C]
+        std::vector<float2> mm((size_t)Vm * C);
+        const float2* mp = reinterpret_cast<const float2*>(maps);
+        for (int64_t c = 0; c < C; ++c)
+            for (int64_t j = 0; j < Vm; ++j) mm[(size_t)(c * Vm + j)] = mp[c * V + vidx[(size_t)j]];
+        if (!ctx->blas) ctx->blas = blas_handle_create();
+        hipStream_t st = ctx->stream;
+        DevBuf<int> dv;
+        DevBuf<float2> dm, S, G;
+        dv.upload(vidx, st);
+        dm.upload(mm, st);
+        S.alloc((size_t)(K * Vm));
+        float2* out = (float2*)AtA;
+        if (!on_device) {
+            G.alloc((size_t)(Vm * Vm));
+            out = G.p;
+        }
+        h->dense_normal(dv.p, Vm, dm.p, (int)C, S.p, out, ctx->blas);
+        if (!on_device)
+            HIPCHECK(hipMemcpyAsync(AtA, out, sizeof(float2) * Vm * Vm, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    });
+}
+
 int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd) {
     return guarded([&] {
         auto* h = dynamic_cast<HioOp*>(op);
@@ -296,18 +377,25 @@ void mm_cg_opts_default(mm_cg_opts* o) {
     o->x0Fallback = 0;
 }
 
-int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax, const mm_cg_opts* opt,
-                  double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status,
-                  int on_device) {
+int mm_cgne_solve_ex(mm_op* E, const void* y, const void* x0, void* x, int NitMax, const mm_cg_opts* opt,
+                     int nreg, const double* gam, mm_op* const* R, const void* gtruth, double* resSqAll,
+                     double* mseAll, double* xnormAll, double* xdiffAll, int* nit, int* status, int on_device) {
     return guarded([&] {
         if (!E || !y || !x) fail(MM_ERR_ARG, "cgne: NULL argument");
-        mm_cg_opts o;
-        mm_cg_opts_default(&o);
-        if (opt) o = *opt;
+        if (nreg < 0 || (nreg > 0 && !gam)) fail(MM_ERR_ARG, "cgne: nreg > 0 needs gam");
         if (auto* se = dynamic_cast<SenseOp*>(E); se && se->toeplitz())
             fail(MM_ERR_UNSUPPORTED,
                  "cgne: the SENSE operator is on a Toeplitz-form NUFFT (prepToep), whose forward maps images "
                  "to coil images; solve with the gridding-form NUFFT");
+        mm_cg_opts o;
+        mm_cg_opts_default(&o);
+        if (opt) o = *opt;
+        std::vector<std::pair<double, mm_op*>> regs;
+        for (int i = 0; i < nreg; ++i) {
+            mm_op* r = R ? R[i] : nullptr;
+            if (r && r->ctx->device != E->ctx->device) fail(MM_ERR_ARG, "cgne: R on another device");
+            if (gam[i] != 0.0) regs.emplace_back(gam[i], r);   // gam == 0 drops the term (cgne.m:91-93)
+        }
         mm_ctx* ctx = E->ctx;
         set_device(ctx);
         auto& m = solvers();
@@ -315,9 +403,9 @@ int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax, 
         if (it == m.end()) it = m.emplace(E, std::make_unique<CgSolver>(E)).first;
         CgSolver& s = *it->second;
         const int64_t ni = E->n_image(), nd = E->n_data();
-        const float2 *dy = (const float2*)y, *dx0 = (const float2*)x0;
+        const float2 *dy = (const float2*)y, *dx0 = (const float2*)x0, *dgt = (const float2*)gtruth;
         float2* dx = (float2*)x;
-        DevBuf<float2> hy, hx0, hx;
+        DevBuf<float2> hy, hx0, hx, hgt;
         if (!on_device) {
             hy.upload((const float2*)y, (size_t)nd, ctx->stream);
             dy = hy.p;
@@ -325,15 +413,26 @@ int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax, 
                 hx0.upload((const float2*)x0, (size_t)ni, ctx->stream);
                 dx0 = hx0.p;
             }
+            if (gtruth) {
+                hgt.upload((const float2*)gtruth, (size_t)ni, ctx->stream);
+                dgt = hgt.p;
+            }
             hx.alloc((size_t)ni);
             dx = hx.p;
         }
-        s.solve(dy, dx0, dx, NitMax, o, resSqAll, xnormAll, xdiffAll, nit, status);
+        s.solve(dy, dx0, dx, NitMax, o, resSqAll, xnormAll, xdiffAll, nit, status, regs, dgt, mseAll);
         if (!on_device) {
             HIPCHECK(hipMemcpyAsync(x, dx, sizeof(float2) * ni, hipMemcpyDeviceToHost, ctx->stream));
             HIPCHECK(hipStreamSynchronize(ctx->stream));
         }
     });
+}
+
+int mm_cgne_solve(mm_op* E, const void* y, const void* x0, void* x, int NitMax, const mm_cg_opts* opt,
+                  double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status,
+                  int on_device) {
+    return mm_cgne_solve_ex(E, y, x0, x, NitMax, opt, 0, nullptr, nullptr, nullptr, resSqAll, nullptr, xnormAll,
+                            xdiffAll, nit, status, on_device);
 }
 
 int mm_cg_set_graph(mm_op* E, int use_graph) {

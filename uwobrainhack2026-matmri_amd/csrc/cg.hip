@@ -149,6 +149,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_update(const CgSeg* __restrict__ 
                                                       const float2* __restrict__ p,
                                                       float2* __restrict__ r,
                                                       const float2* __restrict__ q, int r_update,
+                                                      const float2* __restrict__ gt,
                                                       double2* __restrict__ part) {
     const int s = find_slice(seg, b2s);
     if (sc[s].done) return;
@@ -200,7 +201,14 @@ __global__ __launch_bounds__(kCgBlock) void cg_update(const CgSeg* __restrict__ 
             xn.x = (float)(xv[u].x + (are * pv[u].x - aim * pv[u].y));
             xn.y = (float)(xv[u].y + (are * pv[u].y + aim * pv[u].x));
             if (in) x[k[u]] = xn;
-            acc_cdot(v[1], make_float2(xn.x * w, xn.y * w), xn);
+            // |x|^2 in .x; |gtruth - x|^2 (cgne.m:271-274) in .y, the slot
+            // the imaginary part of x'x (identically zero) would take
+            v[1].x += ((double)xn.x * xn.x + (double)xn.y * xn.y) * w;
+            if (gt) {
+                const float2 g = gt[k[u]];
+                const double dr = (double)g.x - xn.x, di = (double)g.y - xn.y;
+                v[1].y += (dr * dr + di * di) * w;
+            }
             if (r_update) {
                 float2 rn;
                 rn.x = (float)(rv[u].x - (are * qv[u].x - aim * qv[u].y));
@@ -248,6 +256,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_norm_b(const CgSeg* __restrict__ 
                                                       const int* __restrict__ b2s,
                                                       const float2* __restrict__ b,
                                                       float2* __restrict__ x, int copy,
+                                                      const float2* __restrict__ gt,
                                                       double2* __restrict__ part) {
     const int s = find_slice(seg, b2s);
     const CgSeg g = seg[s];
@@ -259,7 +268,11 @@ __global__ __launch_bounds__(kCgBlock) void cg_norm_b(const CgSeg* __restrict__ 
         acc_cdot(v[0], bv, bv);
         const float2 xv = copy ? bv : x[k];
         if (copy) x[k] = bv;
-        acc_cdot(v[1], xv, xv);
+        v[1].x += (double)xv.x * xv.x + (double)xv.y * xv.y;
+        if (gt) {
+            const double dr = (double)gt[k].x - xv.x, di = (double)gt[k].y - xv.y;
+            v[1].y += dr * dr + di * di;
+        }
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
@@ -272,6 +285,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_norm_b(const CgSeg* __restrict__ 
 __global__ __launch_bounds__(kCgScalBlock) void cg_init(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                         const double2* __restrict__ part_b, const double2* __restrict__ part_r,
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
+                        double* __restrict__ hmse,
                         int nslice, int nitmax, double resThresh, double resThreshRel,
                         double stopThresh, int x0fallback, double xChangeThresh,
                         double resChangeThresh, int stopOnResInc, double resIncThresh,
@@ -282,7 +296,9 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_init(const CgSeg* __restrict_
     const bool in = lane < g.nblk;
     const int pb = g.blk0 + (in ? lane : 0);
     const double b0 = part_b[2 * pb].x, x0 = part_b[2 * pb + 1].x, r0 = part_r[2 * pb].x;
+    const double m0 = part_b[2 * pb + 1].y;
     double bb = wave_total(in ? b0 : 0.0), xx = wave_total(in ? x0 : 0.0), rr = wave_total(in ? r0 : 0.0);
+    const double me = wave_total(in ? m0 : 0.0);
     if (lane != 0) return;
     CgScal c{};
     // warm-start fallback: a start worse than x0 = 0 (whose residual is b)
@@ -312,6 +328,7 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_init(const CgSeg* __restrict_
     hres[h] = rr;
     hxn[h] = xx;
     hxd[h] = xx;
+    if (hmse) hmse[h] = me;
 }
 
 // x = 0, r = p = b for the slices cg_init marked as falling back
@@ -337,6 +354,7 @@ __global__ __launch_bounds__(kCgBlock) void cg_fallback_k(const CgSeg* __restric
 __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict__ seg, CgScal* __restrict__ sc,
                         const double2* __restrict__ part, const double* __restrict__ pp,
                         double* __restrict__ hres, double* __restrict__ hxn, double* __restrict__ hxd,
+                        double* __restrict__ hmse,
                         int nslice, int nitmax_alloc, int explicit_res, int* __restrict__ done_host,
                         int* __restrict__ done_map) {
     const int s = blockIdx.x, lane = threadIdx.x;   // one wave per slice
@@ -351,8 +369,9 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict_
     const CgSeg g = seg[s];
     const bool in = lane < g.nblk;
     const int pb = g.blk0 + (in ? lane : 0);
-    const double r0 = part[2 * pb].x, x0 = part[2 * pb + 1].x;
+    const double r0 = part[2 * pb].x, x0 = part[2 * pb + 1].x, m0 = part[2 * pb + 1].y;
     const double rr = wave_total(in ? r0 : 0.0), xx = wave_total(in ? x0 : 0.0);
+    const double me = wave_total(in ? m0 : 0.0);
     if (lane != 0) return;
     CgScal c = sc[s];
     const double a2 = c.al_re * c.al_re + c.al_im * c.al_im;
@@ -362,6 +381,7 @@ __global__ __launch_bounds__(kCgScalBlock) void cg_beta(const CgSeg* __restrict_
     hres[h + n + 1] = rr;
     hxn[h + n + 1] = xx;
     hxd[h + n + 1] = xd;
+    if (hmse) hmse[h + n + 1] = me;
     int finished = 0;
     const double xChangeThresh = c.xChangeThresh, resChangeThresh = c.resChangeThresh;
     const double resIncThresh = c.resIncThresh;
@@ -438,6 +458,18 @@ __global__ __launch_bounds__(kCgBlock) void cg_update_p(const CgSeg* __restrict_
     }
 }
 
+// out += g * in (the Tikhonov terms gam * R'R v of cgne.m:171-179, :219-227)
+__global__ __launch_bounds__(kCgBlock) void cg_axpy_k(float2* __restrict__ out, const float2* __restrict__ in,
+                                                      float g, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)kCgBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kCgBlock) {
+        const float2 a = in[i];
+        float2 o = out[i];
+        o.x += g * a.x;
+        o.y += g * a.y;
+        out[i] = o;
+    }
+}
+
 __global__ __launch_bounds__(kCgBlock) void copy_rr_k(const double2* __restrict__ src, double2* __restrict__ dst, int nblk) {
     const int b = blockIdx.x * kCgBlock + threadIdx.x;
     if (b < nblk) dst[2 * b] = src[2 * b];
@@ -505,32 +537,69 @@ void CgSolver::drop_graph() {
     if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
 }
 
+// out = E'(W.*E v) + sum_n gam_n R_n'(R_n v)  (R_n NULL: the identity)
+uint64_t CgSolver::config_gen() const {
+    uint64_t g = E_->config_gen();
+    for (const auto& rg : regs_)
+        if (rg.second) g += rg.second->config_gen();
+    return g;
+}
+
+void CgSolver::normal(const float2* v, float2* out) {
+    hipStream_t st = E_->ctx->stream;
+    E_->apply_normal(v, out, 1, tmp_.p);
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n_ + kCgBlock - 1) / kCgBlock, 4096));
+    for (const auto& rg : regs_) {
+        const float2* t = v;
+        if (rg.second) {
+            rg.second->apply_normal(v, rbuf_.p, 1, rtmp_.p);
+            t = rbuf_.p;
+        }
+        hipLaunchKernelGGL(cg_axpy_k, dim3(grid), dim3(kCgBlock), 0, st, out, t, (float)rg.first, n_);
+        LAUNCHCHECK();
+    }
+}
+
 void CgSolver::launch_iteration(int explicit_res, const mm_cg_opts& o, float2* x) {
     hipStream_t st = E_->ctx->stream;
     const int ns = E_->nslice();
-    E_->apply_normal(p_.p, q_.p, 1, tmp_.p);
+    normal(p_.p, q_.p);
     hipLaunchKernelGGL(cg_dot_pq, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, q_.p,
                        part_.p);
     hipLaunchKernelGGL(cg_update, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, part_.p, pp_.p, x,
-                       p_.p, r_.p, q_.p, explicit_res ? 0 : 1, partu_.p);
+                       p_.p, r_.p, q_.p, explicit_res ? 0 : 1, gt_, partu_.p);
     if (explicit_res) {
         // r = b - N x; r'r into part2, x'x stays in part_[..][1]
-        E_->apply_normal(x, q_.p, 1, tmp_.p);
+        normal(x, q_.p);
         hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p,
                            q_.p, r_.p, (float2*)nullptr, part2_.p);
         hipLaunchKernelGGL(copy_rr_k, dim3((nblk_ + 255) / 256), dim3(256), 0, st, part2_.p, partu_.p,
                            nblk_);
     }
     hipLaunchKernelGGL(cg_beta, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, partu_.p, pp_.p, hres_.p, hxn_.p,
-                       hxd_.p, ns, nitmax_, explicit_res, done_d_.p, done_map_);
+                       hxd_.p, gt_ ? hmse_.p : (double*)nullptr, ns, nitmax_, explicit_res, done_d_.p, done_map_);
     hipLaunchKernelGGL(cg_update_p, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, p_.p, r_.p);
     LAUNCHCHECK();
 }
 
 void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, const mm_cg_opts& o,
-                     double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status) {
+                     double* resSqAll, double* xnormAll, double* xdiffAll, int* nit, int* status,
+                     const std::vector<std::pair<double, mm_op*>>& regs, const float2* gtruth, double* mseAll) {
     hipStream_t st = E_->ctx->stream;
     const int ns = E_->nslice();
+    for (const auto& rg : regs)
+        if (rg.second && (rg.second->n_image() != n_ || rg.second->W))
+            fail(MM_ERR_ARG, "cgne: a regulariser R must map the image of E (and carry no data weights)");
+    if (regs != regs_ || gtruth != gt_) drop_graph();
+    regs_ = regs;
+    gt_ = gtruth;
+    int64_t rmax = 0;
+    for (const auto& rg : regs_)
+        if (rg.second) rmax = std::max(rmax, rg.second->n_data());
+    if (rmax) {
+        rbuf_.ensure((size_t)n_);
+        rtmp_.ensure((size_t)rmax);
+    }
     const int sb = (ns + kCgScalBlock - 1) / kCgScalBlock;
     if (NitMax < 0) fail(MM_ERR_ARG, "cgne: NitMax must be >= 0");
     if (NitMax != nitmax_) {
@@ -538,6 +607,7 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
         hres_.alloc((size_t)ns * (NitMax + 1));
         hxn_.alloc((size_t)ns * (NitMax + 1));
         hxd_.alloc((size_t)ns * (NitMax + 1));
+        hmse_.alloc((size_t)ns * (NitMax + 1));
         done_d_.alloc(ns);
         drop_graph();
     }
@@ -552,14 +622,15 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     // x0 (cgne.m:143-145)
     if (x0) HIPCHECK(hipMemcpyAsync(x, x0, sizeof(float2) * n_, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(cg_norm_b, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, b_.p, x,
-                       x0 ? 0 : 1, part2_.p);
+                       x0 ? 0 : 1, gt_, part2_.p);
     // r = b - N x0, p = r  (cgne.m:165-180)
-    E_->apply_normal(x, q_.p, 1, tmp_.p);
+    normal(x, q_.p);
     hipLaunchKernelGGL(zero_scal_k, dim3(sb), dim3(kCgScalBlock), 0, st, sc_.p, ns);
     hipLaunchKernelGGL(cg_resid, dim3(nblk_), dim3(kCgBlock), 0, st, seg_.p, b2s_.p, sc_.p, b_.p, q_.p,
                        r_.p, p_.p, part_.p);
     hipLaunchKernelGGL(cg_init, dim3(ns), dim3(kCgScalBlock), 0, st, seg_.p, sc_.p, part2_.p, part_.p, hres_.p,
-                       hxn_.p, hxd_.p, ns, NitMax, o.resThresh, o.resThreshRel, o.stopThresh,
+                       hxn_.p, hxd_.p, gt_ ? hmse_.p : (double*)nullptr, ns, NitMax, o.resThresh, o.resThreshRel,
+                       o.stopThresh,
                        (x0 && o.x0Fallback) ? 1 : 0, o.xChangeThresh, o.resChangeThresh, o.stopOnResInc,
                        o.resIncThresh, o.stopOnXdifInc, o.nseExtraIt);
     LAUNCHCHECK();
@@ -587,7 +658,7 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     // the captured iteration holds the operator's kernels, work tables and
     // buffers: recapture after a configuration change of E or any device
     // free since the capture (the preamble above may itself have reallocated)
-    if (graph_exec_ && (E_->config_gen() != graph_gen_ || free_epoch().load() != graph_epoch_)) drop_graph();
+    if (graph_exec_ && (config_gen() != graph_gen_ || free_epoch().load() != graph_epoch_)) drop_graph();
     bool pending = false;
     for (int it = 0; it < NitMax; ++it) {
         const int explicit_res = (it % expResN) == 0;
@@ -600,7 +671,7 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
                 HIPCHECK(hipStreamEndCapture(st, &graph_));
                 HIPCHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
                 graph_x_ = x;
-                graph_gen_ = E_->config_gen();
+                graph_gen_ = config_gen();
                 graph_epoch_ = free_epoch().load();
             }
             HIPCHECK(hipGraphLaunch(graph_exec_, st));
@@ -621,7 +692,8 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     std::vector<CgScal> hs(ns);
     HIPCHECK(hipMemcpyAsync(hs.data(), sc_.p, sizeof(CgScal) * ns, hipMemcpyDeviceToHost, st));
     const size_t hn = (size_t)ns * (NitMax + 1);
-    std::vector<double> a(hn), b(hn), c(hn);
+    std::vector<double> a(hn), b(hn), c(hn), d(gt_ ? hn : 0);
+    if (gt_) HIPCHECK(hipMemcpyAsync(d.data(), hmse_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(a.data(), hres_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(b.data(), hxn_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(c.data(), hxd_.p, sizeof(double) * hn, hipMemcpyDeviceToHost, st));
@@ -635,6 +707,10 @@ void CgSolver::solve(const float2* y, const float2* x0, float2* x, int NitMax, c
     if (resSqAll) std::memcpy(resSqAll, a.data(), sizeof(double) * hn);
     if (xnormAll) std::memcpy(xnormAll, b.data(), sizeof(double) * hn);
     if (xdiffAll) std::memcpy(xdiffAll, c.data(), sizeof(double) * hn);
+    if (mseAll) {
+        if (gt_) std::memcpy(mseAll, d.data(), sizeof(double) * hn);
+        else std::memset(mseAll, 0, sizeof(double) * hn);
+    }
 }
 
 }  // namespace mm

@@ -92,6 +92,8 @@ struct mm_ctx {
     bool own_stream = false;
     // staging for host-pointer calls
     mm::DevBuf<float2> stage_in, stage_out, stage_aux;
+    // rocBLAS handle (created on first use by the dense-matrix path)
+    void* blas = nullptr;
 };
 
 // Abstract linear operator.  Device entry points are asynchronous on the

@@ -556,3 +556,61 @@ void HioOp::apply_sense(int adjoint, const float2* in, float2* out, int64_t nrep
 }
 
 }  // namespace mm
+
+namespace mm {
+
+// Dense encoding matrix of mrSampFuncMat (operators/mrSampFuncMat.m:33-37):
+// S[k, j] = exp(1i * single(kbase(v_j, k))) / sqrt(Vm) for the masked voxels
+// v_j -- the phase rounded to single precision in radians, as the reference
+// does before its single-precision exp.  Column-major [K x Vm].
+__global__ __launch_bounds__(256) void hio_dense_s_k(const double* __restrict__ Q, const double* __restrict__ Hc,
+                                                     int rp, int64_t K, const int* __restrict__ vidx, int64_t Vm,
+                                                     float scale, float2* __restrict__ S) {
+    const int64_t n = K * Vm;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t k = i % K, j = i / K;
+        const double* q = Q + (int64_t)vidx[j] * rp;
+        const double* h = Hc + k * rp;
+        double ph = 0.0;
+        for (int r = 0; r < rp; ++r) ph = fma(q[r], h[r], ph);
+        const float a = (float)(ph * 6.283185307179586476925287);
+        float sn, cs;
+        sincosf(a, &sn, &cs);
+        S[i] = make_float2(cs * scale, sn * scale);
+    }
+}
+
+// AtA[v, w] = G[v, w] * sum_c conj(m_c(v)) m_c(w)  (mrSampFuncMat.m:38-44),
+// m: masked coil maps [Vm x C]; in place on G (column-major [Vm x Vm])
+__global__ __launch_bounds__(256) void hio_dense_maps_k(float2* __restrict__ G, const float2* __restrict__ m, int64_t Vm,
+                                                        int C) {
+    const int64_t n = Vm * Vm;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t v = i % Vm, w = i / Vm;
+        float2 s = make_float2(0.f, 0.f);
+        for (int c = 0; c < C; ++c) {
+            const float2 a = m[(int64_t)c * Vm + v], b = m[(int64_t)c * Vm + w];
+            s.x += a.x * b.x + a.y * b.y;   // conj(a) * b
+            s.y += a.x * b.y - a.y * b.x;
+        }
+        const float2 g = G[i];
+        G[i] = make_float2(g.x * s.x - g.y * s.y, g.x * s.y + g.y * s.x);
+    }
+}
+
+void HioOp::dense_normal(const int* vidx, int64_t Vm, const float2* mmaps, int C, float2* S, float2* AtA,
+                         void* blas) {
+    if (phidiv_) fail(MM_ERR_ARG, "mrSampFuncMat: the operator has setPhiDiv applied");
+    hipStream_t st = ctx->stream;
+    const int64_t n = K_ * Vm;
+    const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(hio_dense_s_k, dim3(g1), dim3(256), 0, st, Q_.p, Hc_.p, rp_, K_, vidx, Vm,
+                       (float)(1.0 / std::sqrt((double)Vm)), S);
+    LAUNCHCHECK();
+    dense_gram(blas, st, S, K_, Vm, AtA);
+    const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((Vm * Vm + 255) / 256, 65536));
+    hipLaunchKernelGGL(hio_dense_maps_k, dim3(g2), dim3(256), 0, st, AtA, mmaps, Vm, C);
+    LAUNCHCHECK();
+}
+
+}  // namespace mm

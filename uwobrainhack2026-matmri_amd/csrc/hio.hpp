@@ -29,6 +29,11 @@ class HioOp : public mm_op {
                        int C, int iters, double* ms, std::string* name);
     int phase_terms() const { return L_; }
     int rank() const { return r_; }
+    // mrSampFuncMat's A'A over the masked voxels vidx [Vm] (device): S gets
+    // the dense [K x Vm] encoding matrix (workspace), AtA the [Vm x Vm] result;
+    // mmaps = masked coil maps [Vm x C]; blas = a rocBLAS handle
+    void dense_normal(const int* vidx, int64_t Vm, const float2* mmaps, int C, float2* S, float2* AtA, void* blas);
+    int64_t samples() const { return K_; }
     // max |Phi| in radians over the plan (host FP64, computed at creation)
     double max_abs_phase() const { return maxphi_; }
 
@@ -65,5 +70,9 @@ class HioOp : public mm_op {
     DevBuf<float> mpart_;
     DevBuf<unsigned> amax_;
 };
+
+void* blas_handle_create();
+void blas_handle_destroy(void* h);
+void dense_gram(void* blas, hipStream_t st, const float2* S, int64_t m, int64_t n, float2* G);
 
 }  // namespace mm

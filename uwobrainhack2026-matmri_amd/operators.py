@@ -376,7 +376,59 @@ class senseOp(LinearOp):
             self.data_shape = (nd,)
 
 
+class cartOp(LinearOp):
+    """A Cartesian k-space mask used as mrSampFunc's ``samplingOp``
+    (mrSampFunc.m:108-114, :137-143) as a device operator: ``cartOp(mask) * x``
+    is ``mask .* ifftnc(x, nDim)``, its adjoint ``fftnc(mask .* y, nDim)``
+    (nDim = ndims(mask), 1 for a column).  Real masks of 1 or 2 dimensions."""
+
+    def __init__(self, mask):
+        m = np.asarray(mask)
+        nd = m.ndim
+        if nd == 2 and m.shape[1] == 1:
+            nd = 1
+        if nd not in (1, 2):
+            raise NotImplementedError("Cartesian masks of 1 or 2 dimensions only")
+        if np.iscomplexobj(m) and np.any(np.imag(m) != 0):
+            raise NotImplementedError("complex Cartesian masks are not supported")
+        imN = tuple(int(n) for n in m.shape[:nd])
+        self._mask = np.asfortranarray(np.real(m).astype(np.float32)).reshape(-1, order="F")
+        n = np.array(imN + ((1,) if nd == 1 else ()), dtype=np.int64)
+        h = C.c_void_p()
+        check(_lib.lib().mm_cart_plan_create(_lib.context(), nd, n.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             self._mask.ctypes.data, C.byref(h)))
+        self._plan = _Plan(h)
+        self.image_shape = imN
+        self.data_shape = imN
+        self.nslice = 1
+
+
+class sumDimsOp(LinearOp):
+    """``opt.sumDims`` of mrSampFunc (mrSampFunc.m:100-105): ``*`` sums an
+    array of shape ``dims`` over the (1-based) dimensions ``sumDims``,
+    keeping them as singletons; the adjoint broadcasts back (MATLAB's
+    implicit expansion in the 'transp' branch)."""
+
+    def __init__(self, dims, sumDims):
+        dims = tuple(int(d) for d in dims)
+        flags = np.array([1 if (d + 1) in set(int(k) for k in sumDims) else 0 for d in range(len(dims))],
+                         dtype=np.int32)
+        if any(int(k) > len(dims) for k in sumDims):
+            dims = dims + (1,) * (max(int(k) for k in sumDims) - len(dims))
+            flags = np.array([1 if (d + 1) in set(int(k) for k in sumDims) else 0 for d in range(len(dims))],
+                             dtype=np.int32)
+        dd = np.array(dims, dtype=np.int64)
+        h = C.c_void_p()
+        check(_lib.lib().mm_sumdims_plan_create(_lib.context(), len(dims), dd.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                flags.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(h)))
+        self._plan = _Plan(h)
+        self.image_shape = dims
+        self.data_shape = tuple(1 if f else n for n, f in zip(dims, flags))
+        self.nslice = 1
+
+
 _sense_cache = {}
+_aux_cache = {}
 
 
 def _sense_for(S, R, W=None):
@@ -389,56 +441,107 @@ def _sense_for(S, R, W=None):
     return e[0]
 
 
+def _cart_for(mask):
+    m = np.asarray(mask)
+    key = ("cart", m.shape, m.dtype.str, m.tobytes())
+    op = _aux_cache.get(key)
+    if op is None:
+        op = _aux_cache[key] = cartOp(m)
+    return op
+
+
+def _sumdims_for(dims, sumDims):
+    key = ("sum", tuple(dims), tuple(int(k) for k in np.atleast_1d(sumDims)))
+    op = _aux_cache.get(key)
+    if op is None:
+        op = _aux_cache[key] = sumDimsOp(dims, np.atleast_1d(sumDims))
+    return op
+
+
+def _np_to_dev(a):
+    """A NumPy array (column-major meaning) as a complex64 CUDA tensor with
+    the reversed shape the device operators use."""
+    a = np.asarray(a)
+    return torch.as_tensor(np.ascontiguousarray(np.transpose(a)).astype(np.complex64), device="cuda")
+
+
+def _dev_to_np(t):
+    return np.transpose(t.cpu().numpy())
+
+
 def mrSampFunc(x, transp, samplingOp, R=None, opt=None):
-    """``mrSampFunc(x, transp, samplingOp, R, opt)`` (mrSampFunc.m:1-161) for
-    device sampling operators.  Supported options: ``imNFull``/``daNFull``
-    (vectorisation), ``W``, ``tikReg``, ``hardReal``.  ``sumDims`` and
-    Cartesian masks are not part of the accelerated path."""
+    """``mrSampFunc(x, transp, samplingOp, R, opt)`` (mrSampFunc.m:1-161) on
+    the device.  ``samplingOp`` is a device operator (nufftOp, sampHighOrder)
+    or a Cartesian mask (run as :class:`cartOp`); options ``imNFull`` /
+    ``daNFull`` (vectorisation), ``W``, ``tikReg``, ``hardReal`` and
+    ``sumDims`` (a :class:`sumDimsOp` between R and the sampling, with the
+    vectors kept on the GPU between the three operators)."""
     opt = dict(opt or {})
-    if not isinstance(samplingOp, LinearOp):
-        raise NotImplementedError("Cartesian-mask sampling is not implemented on the device path")
-    if opt.get("sumDims") is not None:
-        raise NotImplementedError("sumDims is not implemented on the device path")
+    S = samplingOp if isinstance(samplingOp, LinearOp) else _cart_for(samplingOp)
     imNFull, daNFull = opt.get("imNFull"), opt.get("daNFull")
     tikReg, W = opt.get("tikReg"), opt.get("W")
+    sumDims = opt.get("sumDims")
+    if sumDims is not None and np.size(sumDims) == 0:
+        sumDims = None
     hardReal = opt.get("hardReal", 0) or 0
     if tikReg is not None and (imNFull is None or daNFull is None):
         raise ValueError("Tikhonov regularization requires opt.imNFull and opt.daNFull")
-    E = _sense_for(samplingOp, R) if R is not None else samplingOp
-    xp = torch if _is_torch(x) else np
+    host = not _is_torch(x)
+    if sumDims is not None:
+        # R, the sum and S as three device operators on device tensors
+        full = tuple(R.data_shape) if R is not None else tuple(imNFull if imNFull is not None else np.shape(x))
+        Sm = _sumdims_for(full, sumDims)
+
+        def fwd(v):
+            v = R.mtimes(v) if R is not None else v
+            return S.mtimes(Sm.mtimes(v))
+
+        def adj(v):
+            v = Sm.H.mtimes(S.H.mtimes(v))
+            return R.H.mtimes(v) if R is not None else v
+    else:
+        E = _sense_for(S, R) if R is not None else S
+        fwd, adj = E.mtimes, E.H.mtimes
     if transp == "notransp":
+        xv = np.asarray(x) if host else x
         if hardReal:
-            x = xp.real(x) + 0j if xp is np else torch.real(x).to(torch.complex64)
+            xv = np.real(xv) + 0j if host else torch.real(xv).to(torch.complex64)
         if tikReg is not None:
             t = np.atleast_1d(tikReg)
-            x_tik = t[0] * x.real + 1j * t[1] * x.imag if t.size == 2 else t[0] * x
-        if imNFull is not None and xp is np:
-            x = np.reshape(x, tuple(imNFull), order="F")
-        y = E.mtimes(x)
+            x_tik = t[0] * xv.real + 1j * t[1] * xv.imag if t.size == 2 else t[0] * xv
+        if imNFull is not None and host:
+            xv = np.reshape(xv, tuple(imNFull), order="F")
+        if sumDims is not None and host:
+            y = _dev_to_np(fwd(_np_to_dev(xv)))
+        else:
+            y = fwd(xv)
         if daNFull is not None:
-            y = y.reshape(-1, order="F") if xp is np else y.reshape(-1)
+            y = y.reshape(-1, order="F") if host else y.reshape(-1)
         if tikReg is not None:
-            y = xp.concatenate([y.reshape(-1, order="F") if xp is np else y.reshape(-1),
-                                x_tik.reshape(-1, order="F") if xp is np else x_tik.reshape(-1)]) \
-                if xp is np else torch.cat([y.reshape(-1), x_tik.reshape(-1)])
+            y = np.concatenate([y.reshape(-1, order="F"), x_tik.reshape(-1, order="F")]) if host else \
+                torch.cat([y.reshape(-1), x_tik.reshape(-1)])
         return y
     if transp == "transp":
+        xv = np.asarray(x) if host else x
         if tikReg is not None:
             nda = int(np.prod(daNFull))
-            x_tik = x[nda:]
-            x = x[:nda]
+            x_tik = xv[nda:]
+            xv = xv[:nda]
         if W is not None:
-            x = x * (np.asarray(W) if xp is np else torch.as_tensor(np.asarray(W), device=x.device))
-        if daNFull is not None and xp is np:
-            x = np.reshape(x, tuple(daNFull), order="F")
-        out = E.H.mtimes(x)
+            xv = xv * (np.asarray(W) if host else torch.as_tensor(np.asarray(W), device=xv.device))
+        if daNFull is not None and host:
+            xv = np.reshape(xv, tuple(daNFull), order="F")
+        if sumDims is not None and host:
+            out = _dev_to_np(adj(_np_to_dev(xv)))
+        else:
+            out = adj(xv)
         if imNFull is not None:
-            out = out.reshape(-1, order="F") if xp is np else out.reshape(-1)
+            out = out.reshape(-1, order="F") if host else out.reshape(-1)
         if tikReg is not None:
             t = np.atleast_1d(tikReg)
             out = out + (t[0] * x_tik.real + 1j * t[1] * x_tik.imag if t.size == 2 else t[0] * x_tik)
         if hardReal:
-            out = out.real + 0j if xp is np else torch.real(out).to(torch.complex64)
+            out = out.real + 0j if host else torch.real(out).to(torch.complex64)
         return out
     raise ValueError("transp must be 'notransp' or 'transp'")
 
@@ -455,4 +558,5 @@ class mrSampHandle:
         return mrSampFunc(x, transp, self.S, self.R, self.opt)
 
     def sense(self, W=None):
-        return _sense_for(self.S, self.R, W)
+        S = self.S if isinstance(self.S, LinearOp) else _cart_for(self.S)
+        return _sense_for(S, self.R, W)

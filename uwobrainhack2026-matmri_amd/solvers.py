@@ -63,15 +63,51 @@ def _device_sense(A, W):
     return None
 
 
+class _Eye:
+    """The identity as a Tikhonov operator R of :func:`cgne` (``R'R = I``)."""
+
+    def __repr__(self):
+        return "eye"
+
+
+eye = _Eye()
+
+
+def _regs(gam, R):
+    """(gam_n, R_n) pairs of cgne's Tikhonov terms (cgne.m:91-133): scalar gam
+    with one R, or lists (MATLAB cells) of both; R_n a device operator or
+    ``eye``.  All-zero gam means none (cgne.m:91-93)."""
+    if gam is None:
+        return []
+    gl = list(gam) if isinstance(gam, (list, tuple)) else [gam]
+    if not isinstance(gam, (list, tuple)) and np.ndim(gam) > 0:
+        raise NotImplementedError("cgne: gam must be a scalar (or a list of scalars with a list of R)")
+    if all(float(np.real(g)) == 0.0 for g in gl):
+        return []
+    if R is None:
+        raise ValueError("cgne: Tikhonov terms need R (cgne.m:94-96)")
+    rl = list(R) if isinstance(R, (list, tuple)) else [R]
+    if len(rl) != len(gl):
+        raise ValueError("cgne: gam and R must have the same number of terms")
+    out = []
+    for g, r in zip(gl, rl):
+        if r is eye:
+            out.append((float(g), None))
+        elif isinstance(r, LinearOp):
+            out.append((float(g), r))
+        else:
+            raise NotImplementedError("cgne: R must be a device operator (LinearOp) or solvers.eye on the device path")
+    return out
+
+
 def cgne(A, b, x0=None, NitMax=None, opt=None, gam=None, R=None, W=None):
     """Returns ``(x, resSqAll, mseAll, xnormAll, xdiffAll, stopThresh, status)``
-    like ``cgne.m:1``.  ``x0=None`` is the reference default ``x0 = A'b``."""
+    like ``cgne.m:1``.  ``x0=None`` is the reference default ``x0 = A'b``.
+    Tikhonov terms ``gam``/``R`` (R a device operator or :data:`eye`) and
+    ``opt.gtruth`` (the mse history) run inside the device solve."""
     NitMax = 1000 if NitMax is None else int(NitMax)
     opt = dict(opt or {})
-    if gam is not None and np.any(np.asarray(gam) != 0):
-        raise NotImplementedError("cgne: Tikhonov terms are not implemented on the device path")
-    if opt.get("gtruth") is not None:
-        raise NotImplementedError("cgne: opt.gtruth (mse history) is not implemented on the device path")
+    regs = _regs(gam, R)
     E = _device_sense(A, W)
     if E is None:
         raise NotImplementedError("cgne: A must be a device SENSE operator (senseOp / mrSampHandle)")
@@ -94,31 +130,39 @@ def cgne(A, b, x0=None, NitMax=None, opt=None, gam=None, R=None, W=None):
     hres = np.zeros(ns * (NitMax + 1))
     hxn = np.zeros_like(hres)
     hxd = np.zeros_like(hres)
+    hmse = np.zeros_like(hres)
     nit = np.zeros(ns, dtype=np.int32)
     status = np.zeros(ns, dtype=np.int32)
+    gam_arr = (C.c_double * max(1, len(regs)))(*[g for g, _ in regs])
+    R_arr = (C.c_void_p * max(1, len(regs)))(*[None if r is None else r._plan.h.value for _, r in regs])
+    gamv = C.cast(gam_arr, C.c_void_p) if regs else None
+    Rv = C.cast(R_arr, C.c_void_p) if regs else None
+    gt = opt.get("gtruth")
     if _is_torch(b):
         yb = b.to(torch.complex64).contiguous().reshape(-1)
         x0b = None if x0 is None else x0.to(torch.complex64).contiguous().reshape(-1)
+        gtb = None if gt is None else (gt if _is_torch(gt) else torch.as_tensor(
+            np.ascontiguousarray(np.transpose(np.asarray(gt))), device=b.device)).to(torch.complex64).reshape(-1)
         x = torch.empty(ni, dtype=torch.complex64, device=b.device)
         torch.cuda.current_stream(b.device).synchronize()
-        check(L.mm_cgne_solve(E._plan.h, ptr(yb), ptr(x0b), ptr(x), NitMax, C.byref(o), hres.ctypes.data,
-                              hxn.ctypes.data, hxd.ctypes.data, nit.ctypes.data, status.ctypes.data, 1))
+        check(L.mm_cgne_solve_ex(E._plan.h, ptr(yb), ptr(x0b), ptr(x), NitMax, C.byref(o), len(regs), gamv, Rv,
+                                 ptr(gtb), hres.ctypes.data, hmse.ctypes.data, hxn.ctypes.data, hxd.ctypes.data,
+                                 nit.ctypes.data, status.ctypes.data, 1))
         x = x.reshape(tuple(reversed(E.image_shape)))
     else:
         yb = np.asfortranarray(np.asarray(b)).reshape(-1, order="F").astype(np.complex64)
         x0b = None if x0 is None else np.asfortranarray(np.asarray(x0)).reshape(-1, order="F").astype(np.complex64)
+        gtb = None if gt is None else np.asfortranarray(np.asarray(gt)).reshape(-1, order="F").astype(np.complex64)
         x = np.empty(ni, dtype=np.complex64)
-        check(L.mm_cgne_solve(E._plan.h, yb.ctypes.data, ptr(x0b), x.ctypes.data, NitMax, C.byref(o),
-                              hres.ctypes.data, hxn.ctypes.data, hxd.ctypes.data, nit.ctypes.data,
-                              status.ctypes.data, 0))
+        check(L.mm_cgne_solve_ex(E._plan.h, yb.ctypes.data, ptr(x0b), x.ctypes.data, NitMax, C.byref(o),
+                                 len(regs), gamv, Rv, ptr(gtb), hres.ctypes.data, hmse.ctypes.data,
+                                 hxn.ctypes.data, hxd.ctypes.data, nit.ctypes.data, status.ctypes.data, 0))
         x = x.reshape(E.image_shape, order="F")
     if ns == 1:
         n = int(nit[0]) + 1
-        return (x, hres[:n], np.zeros(n), hxn[:n], hxd[:n], stopThresh, int(status[0]))
-    res = [hres[s * (NitMax + 1): s * (NitMax + 1) + nit[s] + 1] for s in range(ns)]
-    xn = [hxn[s * (NitMax + 1): s * (NitMax + 1) + nit[s] + 1] for s in range(ns)]
-    xd = [hxd[s * (NitMax + 1): s * (NitMax + 1) + nit[s] + 1] for s in range(ns)]
-    return (x, res, None, xn, xd, stopThresh, status.tolist())
+        return (x, hres[:n], hmse[:n], hxn[:n], hxd[:n], stopThresh, int(status[0]))
+    cut = lambda h: [h[s * (NitMax + 1): s * (NitMax + 1) + nit[s] + 1] for s in range(ns)]
+    return (x, cut(hres), cut(hmse) if gt is not None else None, cut(hxn), cut(hxd), stopThresh, status.tolist())
 
 
 def set_cg_graph(A, use_graph):
