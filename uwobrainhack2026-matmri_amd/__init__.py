@@ -6,8 +6,8 @@ The operators and solvers keep the reference's MATLAB interfaces
 of ``include/matmri_hip.h`` (``libmatmri_hip.so`` in this directory).
 """
 from ._lib import MMError, OutOfMemory, device_count, lib, LIB_PATH, EXPORTS  # noqa: F401
-from .operators import (LinearOp, cartOp, mrSampFunc, mrSampHandle, nufftOp, rcvrOp,  # noqa: F401
-                        sampHighOrder, senseOp, sumDimsOp)
+from .operators import (LinearOp, cartOp, mrSampFunc, mrSampFuncMat, mrSampHandle, nufftOp,  # noqa: F401
+                        rcvrOp, sampHighOrder, senseOp, sumDimsOp)
 from . import workloads  # noqa: F401
 from . import finddel  # noqa: F401
 from .finddel import findDelAuto, interpTrajTime, load_traj_mat  # noqa: F401
@@ -25,4 +25,4 @@ def __getattr__(name):
 
 __all__ = ["nufftOp", "rcvrOp", "sampHighOrder", "senseOp", "cartOp", "sumDimsOp", "mrSampFunc", "mrSampHandle", "cgne",
            "bfista", "powermethod", "softthresh", "set_cg_graph", "device_count", "lib", "WarmStartNet",
-           "findDelAuto", "interpTrajTime", "load_traj_mat", "eye"]
+           "findDelAuto", "interpTrajTime", "load_traj_mat", "eye", "mrSampFuncMat"]

@@ -546,6 +546,37 @@ def mrSampFunc(x, transp, samplingOp, R=None, opt=None):
     raise ValueError("transp must be 'notransp' or 'transp'")
 
 
+def mrSampFuncMat(y, Crcvr, b0, sampTimes, phs_spha, phs_conc, phs_grid, imMask=None, on_device=False):
+    """``[AtA, Aty] = mrSampFuncMat(y, Crcvr, b0, sampTimes, phs_spha, phs_conc,
+    phs_grid, imMask)`` (operators/mrSampFuncMat.m:1-47) on the device: the
+    dense ``[Vm x Vm]`` normal matrix over the masked voxels (the
+    single-precision dense encoding matrix built by a HIP kernel, its Gram
+    matrix on rocBLAS, the coil maps folded in by a second kernel) and
+    ``Aty = R'*(S'*y)`` over the mask.  With ``on_device`` AtA stays in HBM
+    as a CUDA tensor (row-major ``(Vm, Vm)`` = the column-major matrix
+    transposed); otherwise NumPy arrays."""
+    S = sampHighOrder(b0, sampTimes, phs_spha, phs_conc, phs_grid)
+    R = rcvrOp(Crcvr)
+    V = int(np.prod(S.image_shape))
+    mask = np.ones(V, np.uint8) if imMask is None else \
+        np.ascontiguousarray(np.asarray(imMask).reshape(-1, order="F").astype(np.uint8))
+    Aty = np.asarray(senseOp(S, R).H * np.asarray(y)).reshape(-1, order="F")[mask.astype(bool)]
+    L = _lib.lib()
+    vm = C.c_int64()
+    check(L.mm_hio_dense_normal(S._plan.h, R.C, R._maps.ctypes.data, mask.ctypes.data, None, C.byref(vm), 0))
+    n = vm.value
+    if on_device:
+        AtA = torch.empty((n, n), dtype=torch.complex64, device="cuda")
+        torch.cuda.synchronize()
+        check(L.mm_hio_dense_normal(S._plan.h, R.C, R._maps.ctypes.data, mask.ctypes.data, AtA.data_ptr(),
+                                    C.byref(vm), 1))
+        return AtA, Aty
+    AtA = np.empty((n, n), dtype=np.complex64, order="F")
+    check(L.mm_hio_dense_normal(S._plan.h, R.C, R._maps.ctypes.data, mask.ctypes.data, AtA.ctypes.data,
+                                C.byref(vm), 0))
+    return AtA, Aty
+
+
 class mrSampHandle:
     """The handle ``@(x, transp) mrSampFunc(x, transp, S, R, opt)`` the demos
     pass to cgne (demo_regSENSE.m:89-99).  cgne recognises it and runs the
