@@ -18,6 +18,7 @@
  *   mm_hio_plan_create      sampHighOrder(b0,sampTimes,phs_spha,phs_conc,phs_grid,b0mask,...)
  *                           operators/@sampHighOrder/sampHighOrder.m:129-291, :496-588
  *   mm_hio_set_phi_div      sampHighOrder.setPhiDiv  sampHighOrder.m:590-622
+ *   mm_hio_interp_create    sampHighOrder(...,approach=1,opt)  sampHighOrder.m:314-359, :624-902
  *   mm_nufft_set_toeplitz   nufftOp.prepToep          nufftOp.m:410-432 (applied :348-352)
  *   mm_sense_create         mrSampFunc(x,transp,samplingOp,R,opt)  operators/mrSampFunc.m:82-159
  *   mm_cart_plan_create     mrSampFunc's Cartesian-mask samplingOp  mrSampFunc.m:108-114, :137-143
@@ -103,6 +104,15 @@ int mm_hio_plan_create(mm_ctx* ctx, int64_t V, int64_t K, int Lh, int Lc,
                        const double* x, const double* y, const double* z,
                        const double* b0, const double* t, const double* h,
                        const double* g, const double* mask, mm_op** out);
+/* sampHighOrder approach 1 (interpolated, sampHighOrder.m:314-359,
+ * :624-902): y = phsShft .* sum_n svdTime(:,n) .* NUFFT(x .* svdSpace(:,n)) and
+ * its adjoint, over a single-slice NUFFT `nufft` of the V-voxel image onto
+ * the K samples (kept by the caller for the operator's lifetime);
+ * svdSpace [V x Ns], svdTime [K x Ns], phsShft [K], complex64 host arrays
+ * (the truncated SVD of the non-linear phase, from the plan-time host
+ * code).  The Ns modulated images of each input run as one NUFFT batch. */
+int mm_hio_interp_create(mm_ctx* ctx, mm_op* nufft, int64_t V, int64_t K, int Ns, const float* svdSpace,
+                         const float* svdTime, const float* phsShft, mm_op** out);
 /* setPhiDiv: replaces the forward by (1i*exp(1i*Phi).*dPhi).' * x / sqrt(V)
  * with dPhi from hd [Lh x K] and gd [Lc x K] (no B0 term); NULL,NULL clears. */
 int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd);

@@ -31,6 +31,7 @@ EXPORTS = [
     "mm_cgne_solve_ex",
     "mm_interp_traj_time",
     "mm_hio_dense_normal",
+    "mm_hio_interp_create",
     "mm_rcvr_normalize_ssq_noalloc",
 ]
 
@@ -83,6 +84,7 @@ def lib():
     L.mm_sumdims_plan_create.argtypes = [vp, i32, P(i64), P(i32), P(vp)]
     L.mm_interp_traj_time.argtypes = [vp, vp, i64, i32, dbl, dbl, vp, i64, vp, i32]
     L.mm_hio_dense_normal.argtypes = [vp, i64, vp, vp, vp, P(i64), i32]
+    L.mm_hio_interp_create.argtypes = [vp, vp, i64, i64, i32, vp, vp, vp, P(vp)]
     L.mm_nufft_set_toeplitz.argtypes = [vp, vp]
     L.mm_sense_create.argtypes = [vp, vp, vp, vp, P(vp)]
     L.mm_op_shape.argtypes = [vp, P(i64), P(i64)]

@@ -7,6 +7,7 @@
 #include "cart.hpp"
 #include "cg.hpp"
 #include "hio.hpp"
+#include "hio_interp.hpp"
 #include "ops.hpp"
 #include "traj.hpp"
 #include "wsnet.hpp"
@@ -286,6 +287,16 @@ int mm_hio_dense_normal(mm_op* op, int64_t C, const float* maps, const unsigned 
         if (!on_device)
             HIPCHECK(hipMemcpyAsync(AtA, out, sizeof(float2) * Vm * Vm, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
+    });
+}
+
+int mm_hio_interp_create(mm_ctx* ctx, mm_op* nufft, int64_t V, int64_t K, int Ns, const float* svdSpace,
+                         const float* svdTime, const float* phsShft, mm_op** out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(MM_ERR_ARG, "sampHighOrder: NULL argument");
+        *out = nullptr;
+        set_device(ctx);
+        *out = new HioInterpOp(ctx, nufft, V, K, Ns, svdSpace, svdTime, phsShft);
     });
 }
 

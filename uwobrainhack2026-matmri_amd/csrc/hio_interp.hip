@@ -1,0 +1,112 @@
+// Interpolated expanded-encoding operator (sampHighOrder approach 1,
+// operators/@sampHighOrder/sampHighOrder.m:314-359): the linear phase terms
+// are a NUFFT, the rest a truncated SVD exp(1i*Phi_res) ~ U S V' of Ns terms,
+//   forward  y = phsShft .* sum_n svdTime_n .* NUFFT(x .* svdSpace_n)
+//   adjoint  x = sum_n conj(svdSpace_n) .* NUFFT'(conj(phsShft .* svdTime_n) .* y)
+// The Ns modulated images of every input image go through the NUFFT as one
+// batch of R*Ns transforms (image r, term n at transform r + R*n); the
+// spatial modulation and the time-domain combination are one kernel each.
+#include "hio_interp.hpp"
+
+namespace mm {
+
+// imgs[v, r + R n] = x[v, r] * sp[v, n]
+__global__ __launch_bounds__(256) void hin_modulate_k(const float2* __restrict__ x, const float2* __restrict__ sp,
+                                                      float2* __restrict__ imgs, int64_t V, int64_t R, int Ns) {
+    const int64_t n = V * R * Ns;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t v = i % V, b = i / V;
+        const int64_t r = b % R, t = b / R;
+        imgs[i] = cmulf(x[r * V + v], sp[t * V + v]);
+    }
+}
+
+// y[k, r] = ph[k] * sum_n st[k, n] * Y[k, r + R n]
+__global__ __launch_bounds__(256) void hin_combine_k(const float2* __restrict__ Y, const float2* __restrict__ st,
+                                                     const float2* __restrict__ ph, float2* __restrict__ y, int64_t K,
+                                                     int64_t R, int Ns) {
+    const int64_t n = K * R;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t k = i % K, r = i / K;
+        float2 s = make_float2(0.f, 0.f);
+        for (int t = 0; t < Ns; ++t) {
+            const float2 p = cmulf(st[(int64_t)t * K + k], Y[(r + R * t) * K + k]);
+            s.x += p.x;
+            s.y += p.y;
+        }
+        y[i] = cmulf(ph[k], s);
+    }
+}
+
+// Y[k, r + R n] = conj(ph[k] * st[k, n]) * y[k, r]
+__global__ __launch_bounds__(256) void hin_spread_k(const float2* __restrict__ y, const float2* __restrict__ st,
+                                                    const float2* __restrict__ ph, float2* __restrict__ Y, int64_t K,
+                                                    int64_t R, int Ns) {
+    const int64_t n = K * R * Ns;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t k = i % K, b = i / K;
+        const int64_t r = b % R, t = b / R;
+        const float2 w = cmulf(ph[k], st[t * K + k]);
+        const float2 v = y[r * K + k];
+        Y[i] = make_float2(w.x * v.x + w.y * v.y, w.x * v.y - w.y * v.x);   // conj(w) * v
+    }
+}
+
+// x[v, r] = sum_n conj(sp[v, n]) * imgs[v, r + R n]
+__global__ __launch_bounds__(256) void hin_demodulate_k(const float2* __restrict__ imgs, const float2* __restrict__ sp,
+                                                        float2* __restrict__ x, int64_t V, int64_t R, int Ns) {
+    const int64_t n = V * R;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t v = i % V, r = i / V;
+        float2 s = make_float2(0.f, 0.f);
+        for (int t = 0; t < Ns; ++t) {
+            const float2 a = sp[(int64_t)t * V + v], b = imgs[(r + R * t) * V + v];
+            s.x += a.x * b.x + a.y * b.y;
+            s.y += a.x * b.y - a.y * b.x;
+        }
+        x[i] = s;
+    }
+}
+
+static int hin_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+HioInterpOp::HioInterpOp(mm_ctx* c, mm_op* nufft, int64_t V, int64_t K, int Ns, const float* svdSpace,
+                         const float* svdTime, const float* phsShft)
+    : nu_(nufft), V_(V), K_(K), Ns_(Ns) {
+    ctx = c;
+    if (!nufft || !svdSpace || !svdTime || !phsShft) fail(MM_ERR_ARG, "sampHighOrder: NULL argument");
+    if (Ns < 1 || V < 1 || K < 1) fail(MM_ERR_ARG, "sampHighOrder: empty interpolation terms");
+    if (nufft->nslice() != 1 || nufft->n_image() != V || nufft->n_data() != K)
+        fail(MM_ERR_ARG, "sampHighOrder: the NUFFT does not map the image to the samples");
+    sp_.upload(reinterpret_cast<const float2*>(svdSpace), (size_t)(V * Ns), ctx->stream);
+    st_.upload(reinterpret_cast<const float2*>(svdTime), (size_t)(K * Ns), ctx->stream);
+    ph_.upload(reinterpret_cast<const float2*>(phsShft), (size_t)K, ctx->stream);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void HioInterpOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
+    if (nrep < 1) fail(MM_ERR_ARG, "sampHighOrder: nrep must be >= 1");
+    hipStream_t st = ctx->stream;
+    const int64_t R = nrep;
+    imgs_.ensure((size_t)(V_ * R * Ns_));
+    dat_.ensure((size_t)(K_ * R * Ns_));
+    if (!adjoint) {
+        hipLaunchKernelGGL(hin_modulate_k, dim3(hin_grid(V_ * R * Ns_)), dim3(256), 0, st, in, sp_.p, imgs_.p, V_, R,
+                           Ns_);
+        LAUNCHCHECK();
+        nu_->apply(0, imgs_.p, dat_.p, R * Ns_);
+        hipLaunchKernelGGL(hin_combine_k, dim3(hin_grid(K_ * R)), dim3(256), 0, st, dat_.p, st_.p, ph_.p, out, K_, R,
+                           Ns_);
+        LAUNCHCHECK();
+    } else {
+        hipLaunchKernelGGL(hin_spread_k, dim3(hin_grid(K_ * R * Ns_)), dim3(256), 0, st, in, st_.p, ph_.p, dat_.p, K_,
+                           R, Ns_);
+        LAUNCHCHECK();
+        nu_->apply(1, dat_.p, imgs_.p, R * Ns_);
+        hipLaunchKernelGGL(hin_demodulate_k, dim3(hin_grid(V_ * R)), dim3(256), 0, st, imgs_.p, sp_.p, out, V_, R,
+                           Ns_);
+        LAUNCHCHECK();
+    }
+}
+
+}  // namespace mm

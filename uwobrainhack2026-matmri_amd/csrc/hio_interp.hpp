@@ -1,0 +1,29 @@
+// Interpolated expanded-encoding operator (sampHighOrder approach 1) --
+// declarations.
+#pragma once
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace mm {
+
+class HioInterpOp : public mm_op {
+   public:
+    // nufft: a single-slice NUFFT of the image onto the K samples (not
+    // owned); svdSpace [V x Ns], svdTime [K x Ns], phsShft [K] complex64
+    HioInterpOp(mm_ctx* ctx, mm_op* nufft, int64_t V, int64_t K, int Ns, const float* svdSpace,
+                const float* svdTime, const float* phsShft);
+    int64_t n_image() const override { return V_; }
+    int64_t n_data() const override { return K_; }
+    void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
+    uint64_t config_gen() const override { return gen_ + nu_->config_gen(); }
+    bool set_option(const std::string& key, int value) override { ++gen_; return nu_->set_option(key, value); }
+    bool get_option(const std::string& key, int* value) const override { return nu_->get_option(key, value); }
+
+   private:
+    mm_op* nu_;
+    int64_t V_, K_;
+    int Ns_;
+    DevBuf<float2> sp_, st_, ph_, imgs_, dat_;
+};
+
+}  // namespace mm

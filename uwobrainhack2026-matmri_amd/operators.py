@@ -306,8 +306,10 @@ class sampHighOrder(LinearOp):
 
     def __init__(self, b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask=None, useGPU=1,
                  useSingle=0, approach=0, opt=None):
-        if approach == 1:
-            raise NotImplementedError("sampHighOrder approach 1 (interpolated) is not implemented")
+        self.approach = 0 if approach in (None, 2, 3) else int(approach)
+        if self.approach == 1:
+            self._init_interp(b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask, dict(opt or {}))
+            return
         gx = np.asarray(phs_grid["x"], np.float64)
         self.imSize = gx.shape
         V = gx.size
@@ -344,16 +346,167 @@ class sampHighOrder(LinearOp):
         self.data_shape = tuple(self.kSize)
         self.nslice = 1
 
+    # -- approach 1: interpolated (sampHighOrder.m:314-359, :624-902) -------
+    def _init_interp(self, b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask, opt):
+        """Plan of the interpolated operator.  On the host: the NUFFT
+        locations and phase ramp of the linear terms (createNufftForInterp,
+        :624-701), the smoothing and time subsampling of the coefficients
+        (subSampTime, :959-982), the small QR and SVD factorisations.  On the
+        device: every product with exp(1i*Phi_res) of the truncated SVD (a
+        randomized range finder with two power iterations over the direct
+        operator of the remaining terms), the NUFFT and the operator."""
+        svdThresh = float(opt.get("svdThresh", 0.05))
+        subFact = max(1, int(opt.get("subFact", 5)))
+        if int(opt.get("subFactSpc", 1)) > 1:
+            raise NotImplementedError("sampHighOrder approach 1: subFactSpc > 1 (not recommended by the reference)")
+        ntry = int(opt.get("svdNtry", 30))
+        gx, gy, gz = (np.asarray(phs_grid[c], np.float64) for c in ("x", "y", "z"))
+        if gx.ndim != 2 or min(gx.shape) < 2:
+            raise NotImplementedError("sampHighOrder approach 1: 2-D axial grids only")
+        t = np.asarray(sampTimes, np.float64).reshape(-1)
+        K = t.size
+        hs = np.asarray(phs_spha, np.float64).reshape(np.shape(phs_spha)[0], -1)
+        if hs.shape[1] != K:
+            raise ValueError("Dimension mismatch between phs_spha and sampTimes")
+        hc = None if phs_conc is None or np.size(phs_conc) == 0 else \
+            np.asarray(phs_conc, np.float64).reshape(np.shape(phs_conc)[0], -1)
+        N1, N2 = gx.shape
+        V = N1 * N2
+        if gz[1, 1] - gz[0, 0] != 0:
+            raise ValueError("non-axial slices not supported for interpolated approach")
+        # createNufftForInterp: which grid axis each linear term runs along
+        if abs(gx[0, 1] - gx[0, 0]) > np.finfo(np.float64).eps:
+            if gy[0, 1] - gy[0, 0] != 0:
+                raise ValueError("non-axial slices not supported for interpolated approach")
+            kl = np.stack([np.sign(gy[1, 0] - gy[0, 0] or -1) * hs[2] * abs(gy[1, 0] - gy[0, 0]),
+                           np.sign(gx[0, 1] - gx[0, 0] or -1) * hs[1] * abs(gx[0, 1] - gx[0, 0])], axis=1)
+            shift = gy[N1 // 2, 0] * hs[2] + gx[0, N2 // 2] * hs[1]
+        else:
+            kl = np.stack([np.sign(gx[1, 0] - gx[0, 0] or -1) * hs[1] * abs(gx[1, 0] - gx[0, 0]),
+                           np.sign(gy[0, 1] - gy[0, 0] or -1) * hs[2] * abs(gy[0, 1] - gy[0, 0])], axis=1)
+            shift = gx[N1 // 2, 0] * hs[1] + gy[0, N2 // 2] * hs[2]
+        kloc = kl / (2 * np.pi)
+        phsShft = np.exp(1j * shift)
+        nu = nufftOp([N1, N2], kloc, None, 1, 1.5, 6)
+        # subSampTime: smoothing window filtNd(2*(h+1), 0.5, 1, 'gauss') without
+        # its first tap, every subFact-th sample plus the last
+        inds = np.arange(0, K, subFact)
+        if inds[-1] != K - 1:
+            inds = np.append(inds, K - 1)
+        hs_s, hc_s = hs.copy(), None if hc is None else hc.copy()
+        if subFact > 1:
+            half = int(np.ceil(subFact / 2))
+            sz = 2 * (half + 1)
+            g = np.exp(-0.5 * (np.arange(-sz // 2, sz // 2) / (sz * 0.5 / 2.355)) ** 2)
+            win = g[1:] / g[1:].sum()
+            for a, a0 in ((hs_s, hs), (hc_s, hc)):
+                if a is None:
+                    continue
+                for r in range(a.shape[0]):
+                    a[r] = np.convolve(a0[r], win, mode="same")
+                a[:, :half] = a0[:, :half]
+                a[:, -half:] = a0[:, -half:]
+        hs_s = hs_s[:, inds]
+        hs_s[1:3] = 0.0   # the linear terms are the NUFFT's
+        hc_s = None if hc_s is None else hc_s[:, inds]
+        B = sampHighOrder(b0, t[inds], hs_s, hc_s, phs_grid, b0mask)
+        mask = None if b0mask is None or np.size(b0mask) == 0 else \
+            (np.asarray(b0mask).reshape(-1, order="F") > 0.5)
+        U, sv, Vc, Ns = _interp_svd(B, V, len(inds), mask, svdThresh, ntry)
+        st = (Vc[:, :Ns] * sv[:Ns])            # conj(V*S): Vc holds conj(V)
+        if subFact > 1:
+            from scipy.interpolate import PchipInterpolator
+            tk = np.arange(K)
+            st = PchipInterpolator(inds, st.real, axis=0)(tk) + 1j * PchipInterpolator(inds, st.imag, axis=0)(tk)
+        sp = U[:, :Ns]
+        if mask is not None:
+            full = np.zeros((V, Ns), complex)
+            full[mask] = sp
+            sp = full
+        self._sp = np.asfortranarray(sp.astype(np.complex64))
+        self._st = np.asfortranarray(st.astype(np.complex64))
+        self._ph = np.ascontiguousarray(phsShft.astype(np.complex64))
+        h = C.c_void_p()
+        check(_lib.lib().mm_hio_interp_create(_lib.context(), nu._plan.h, V, K, Ns, self._sp.ctypes.data,
+                                              self._st.ctypes.data, self._ph.ctypes.data, C.byref(h)))
+        self._plan = _Plan(h, keep=(nu._plan,))
+        self._nufft = nu
+        self.Ns = Ns
+        self.svdValues = sv
+        self.imSize = gx.shape
+        self.kSize = (K,)
+        self.Lh, self.Lc = hs.shape[0], 0 if hc is None else hc.shape[0]
+        self.image_shape = tuple(self.imSize)
+        self.data_shape = (K,)
+        self.nslice = 1
+
     def setPhiDiv(self, ksphaDiv, kconcDiv=None):
         """``setPhiDiv`` (sampHighOrder.m:590-622): the forward becomes
         ``(1i*exp(1i*Phi).*dPhi).' * x / sqrt(V)``.  Returns the operator,
         like the reference's value-class mutator."""
+        if self.approach == 1:
+            raise NotImplementedError("phiDiv not yet implemented for interpolated approach")   # :335-337
         hd = np.ascontiguousarray(np.asarray(ksphaDiv, np.float64).reshape(self.Lh, -1, order="F").T)
         gd = None
         if self.Lc:
             gd = np.ascontiguousarray(np.asarray(kconcDiv, np.float64).reshape(self.Lc, -1, order="F").T)
         check(_lib.lib().mm_hio_set_phi_div(self._plan.h, ptr(hd), ptr(gd)))
         return self
+
+
+def _interp_svd(B, V, Ks, mask, svdThresh, ntry):
+    """Top singular triplets of b = exp(1i*Phi_res) [V x Ks] (rows of the
+    mask only) by a randomized range finder with two power iterations: the
+    products b*X and b'*Q run on the device through B, the direct operator
+    of the remaining phase terms (B*x = b.'x/sqrt(V), B'*y = conj(b)y/sqrt(V),
+    so b*X = conj(B'*conj(X))*sqrt(V) and b'*Q = conj(B*conj(Q))*sqrt(V));
+    the QR and the small SVD on the host.  Grows the rank like the
+    reference's svds loop (x1.2 until the smallest value found is below
+    svdThresh times the largest) and keeps Ns = the first index below it
+    (inclusive, sampHighOrder.m:818-836).  Returns (U [Vm x l], s, conj(V)
+    [Ks x l], Ns)."""
+    sq = np.sqrt(V)
+    dev = "cuda"
+
+    def bx(X):      # b X, X host [Ks x l] -> host [Vm x l]
+        Xt = torch.as_tensor(np.ascontiguousarray(np.conj(X).T).astype(np.complex64), device=dev)   # (l, Ks)
+        Y = (B.H * Xt).cpu().numpy().astype(np.complex128)       # (l, N2, N1): conj(b X) / sqrt(V)
+        Y = np.conj(Y.reshape(Y.shape[0], -1).T) * sq              # column-major images as columns
+        return Y if mask is None else Y[mask]
+
+    def bhq(Q):     # b' Q, Q host [Vm x l] -> host [Ks x l]
+        Qf = Q
+        if mask is not None:
+            Qf = np.zeros((V, Q.shape[1]), complex)
+            Qf[mask] = Q
+        Qt = torch.as_tensor(np.ascontiguousarray(_to_rowmajor_images(np.conj(Qf), B.imSize)).astype(np.complex64),
+                             device=dev)
+        Y = (B * Qt).cpu().numpy().astype(np.complex128)          # (l, Ks)
+        return np.conj(Y.T) * sq
+
+    rng = np.random.default_rng(12345)
+    while True:
+        if ntry > 200:
+            raise ValueError("many large singular values. Try providing mask or adjusting svdThresh.")
+        l = min(ntry + 10, Ks, V if mask is None else int(mask.sum()))
+        Om = rng.standard_normal((Ks, l)) + 1j * rng.standard_normal((Ks, l))
+        Q, _ = np.linalg.qr(bx(Om))
+        for _ in range(2):
+            Z, _ = np.linalg.qr(bhq(Q))
+            Q, _ = np.linalg.qr(bx(Z))
+        Bs = np.conj(bhq(Q)).T                                       # Q' b  [l x Ks]
+        Ub, s, Vh = np.linalg.svd(Bs, full_matrices=False)
+        U = Q @ Ub
+        below = np.nonzero(s / s.max() < svdThresh)[0]
+        if len(below) and (below[0] < ntry or l >= min(Ks, Q.shape[0])):
+            return U, s, Vh.T, int(below[0]) + 1
+        ntry = int(np.ceil(1.2 * ntry))
+
+
+def _to_rowmajor_images(Q, imSize):
+    """[V x l] column-major images -> (l, N2, N1) row-major tensor layout."""
+    l = Q.shape[1]
+    return Q.T.reshape((l,) + tuple(reversed(imSize)))
 
 
 class senseOp(LinearOp):
