@@ -11,12 +11,15 @@
  *   rcvrOp maps path           operators/@rcvrOp/rcvrOp.m:145-157
  *   mrSampFunc (E = S o R)     operators/mrSampFunc.m:82-159
  *   cgne                       iterativeSolvers/cgne.m:136-335
+ *   sampHighOrder direct       operators/@sampHighOrder/sampHighOrder.m:361-391,
+ *                              phase of prepForDirect :496-588 (oracle/highorder.py)
  * and is pinned against it by tests/test_oracle_c.py (1e-12 per
  * application, identical CG histories to 1e-10).
  *
  * Arrays are column-major and complex values interleaved (re, im), like
  * MATLAB and like NumPy complex128 arrays in Fortran order.
  */
+#define _GNU_SOURCE
 #include <complex.h>
 #include <math.h>
 #include <stdlib.h>
@@ -393,13 +396,25 @@ static cplx dot_c(const cplx* a, const cplx* b, size_t n) {
     return sr + I * si;
 }
 
+/* an encoding operator E: image [V] -> data [D] and back */
+typedef struct {
+    void (*fwd)(void* s, const double* x, double* y);
+    void (*adj)(void* s, const double* y, double* x);
+    void* s;
+    size_t V, D;
+} refop;
+
+static void sense_fwd(void* s, const double* x, double* y) { mmref_forward((mmref_sense*)s, x, y); }
+static void sense_adj(void* s, const double* y, double* x) { mmref_adjoint((mmref_sense*)s, y, x); }
+
 /* cgne on E'E x = E'y (cgne.m:136-335): x0 = NULL means x0 = b; thresholds
  * < 0 are off; resThresh_rel (if >= 0) adds |r|^2 < resThresh_rel*|b|^2.
  * resSqAll (NitMax+1 doubles) may be NULL.  Returns the iteration count. */
-int mmref_cgne(mmref_sense* s, const double* yin, const double* x0in, double* xout, int NitMax,
-               double resThresh, double resThresh_rel, double xChangeThresh, int expResN, double* resSqAll) {
-    const size_t V = (size_t)s->N1 * s->N2;
-    const size_t D = (size_t)s->M * s->C;
+static int cgne_core(const refop* op, const double* yin, const double* x0in, double* xout, int NitMax,
+                     double resThresh, double resThresh_rel, double xChangeThresh, int expResN, double* resSqAll) {
+    void* s = op->s;
+    const size_t V = op->V;
+    const size_t D = op->D;
     cplx* b = (cplx*)malloc(sizeof(cplx) * V);
     cplx* r = (cplx*)malloc(sizeof(cplx) * V);
     cplx* p = (cplx*)malloc(sizeof(cplx) * V);
@@ -407,29 +422,29 @@ int mmref_cgne(mmref_sense* s, const double* yin, const double* x0in, double* xo
     cplx* xold = (cplx*)malloc(sizeof(cplx) * V);
     cplx* tmp = (cplx*)malloc(sizeof(cplx) * D);
     cplx* x = (cplx*)xout;
-    mmref_adjoint(s, yin, (double*)b);
+    op->adj(s, yin, (double*)b);
     if (x0in) memcpy(xold, x0in, sizeof(cplx) * V);
     else memcpy(xold, b, sizeof(cplx) * V);
     const double bb = dot_re(b, b, V);
     double thr = resThresh;
     if (resThresh_rel >= 0) thr = thr >= 0 ? fmax(thr, resThresh_rel * bb) : resThresh_rel * bb;
     /* r = b - E'E x0, p = r */
-    mmref_forward(s, (const double*)xold, (double*)tmp);
-    mmref_adjoint(s, (const double*)tmp, (double*)q);
+    op->fwd(s, (const double*)xold, (double*)tmp);
+    op->adj(s, (const double*)tmp, (double*)q);
     for (size_t i = 0; i < V; ++i) { r[i] = b[i] - q[i]; p[i] = r[i]; }
     double rr_old = dot_re(r, r, V);
     if (resSqAll) resSqAll[0] = rr_old;
     if (expResN <= 0) expResN = 100;
     int nit = 0, finished = NitMax <= 0;
     while (!finished) {
-        mmref_forward(s, (const double*)p, (double*)tmp);
-        mmref_adjoint(s, (const double*)tmp, (double*)q);
+        op->fwd(s, (const double*)p, (double*)tmp);
+        op->adj(s, (const double*)tmp, (double*)q);
         const cplx alph = rr_old / dot_c(p, q, V);
         for (size_t i = 0; i < V; ++i) x[i] = xold[i] + alph * p[i];
         double betaFact;
         if (nit % expResN == 0) {
-            mmref_forward(s, (const double*)x, (double*)tmp);
-            mmref_adjoint(s, (const double*)tmp, (double*)r);
+            op->fwd(s, (const double*)x, (double*)tmp);
+            op->adj(s, (const double*)tmp, (double*)r);
             for (size_t i = 0; i < V; ++i) r[i] = b[i] - r[i];
             betaFact = 0.0;
         } else {
@@ -455,4 +470,247 @@ int mmref_cgne(mmref_sense* s, const double* yin, const double* x0in, double* xo
     if (NitMax <= 0) memcpy(x, xold, sizeof(cplx) * V);
     free(b); free(r); free(p); free(q); free(xold); free(tmp);
     return nit;
+}
+
+int mmref_cgne(mmref_sense* s, const double* yin, const double* x0in, double* xout, int NitMax,
+               double resThresh, double resThresh_rel, double xChangeThresh, int expResN, double* resSqAll) {
+    const refop op = {sense_fwd, sense_adj, s, (size_t)s->N1 * s->N2, (size_t)s->M * s->C};
+    return cgne_core(&op, yin, x0in, xout, NitMax, resThresh, resThresh_rel, xChangeThresh, expResN, resSqAll);
+}
+
+/* ---- expanded-encoding (high-order) SENSE operator ------------------------
+ * sampHighOrder.m:361-391 (direct worker) composed with rcvrOp.m:145-157:
+ *   Phi[v, k] = sum_l B[v, l] H[k, l]     (prepForDirect :496-588; the caller
+ *                                          passes the basis values with the
+ *                                          mask applied and the B0 map as
+ *                                          columns of B, the coefficients and
+ *                                          the sample times as columns of H)
+ *   y[k, c]  = sum_v exp(+i Phi[v, k]) maps[v, c] x[v] / sqrt(V)
+ *   x[v]     = sum_c conj(maps[v, c]) sum_k exp(-i Phi[v, k]) y[k, c] / sqrt(V)
+ * Optionally exp(i Phi) is kept ([K][V], 16 bytes a value) between
+ * applications, as repeated solves with one operator allow. */
+enum { HIO_CB = 8, HIO_T1 = 32, HIO_T2 = 256 };
+
+typedef struct {
+    int V, K, Lt, C, Cp;  /* Cp: C rounded up to a multiple of 8 */
+    double* B;        /* [Lt][V] (column-major V x Lt) */
+    double* H;        /* [Lt][K] (column-major K x Lt) */
+    cplx* maps;       /* [V][C] */
+    double *er, *ei;  /* exp(i Phi) [K][V], or NULL */
+    double *zr, *zi;  /* [V][Cp] maps .* x */
+    double *wr, *wi;  /* [K][Cp] data transposed */
+} mmref_hio;
+
+void mmref_hio_destroy(mmref_hio* s) {
+    if (!s) return;
+    free(s->B); free(s->H); free(s->maps); free(s->er); free(s->ei);
+    free(s->zr); free(s->zi); free(s->wr); free(s->wi);
+    free(s);
+}
+
+/* exp(i*sgn*Phi) along a run of n voxels at one time point (dir 0: v0..,
+ * k fixed) or of n time points at one voxel (dir 1: k0.., v fixed); Phi
+ * summed over l in ascending order, as prepForDirect does */
+static void hio_exp_run(const mmref_hio* s, int dir, int k, int v, int n, double sgn, double* re, double* im) {
+    double phi[4096];
+    for (int j0 = 0; j0 < n; j0 += 4096) {
+        const int m = n - j0 < 4096 ? n - j0 : 4096;
+        if (s->er) {
+            for (int j = 0; j < m; ++j) {
+                const size_t i = dir == 0 ? (size_t)k * s->V + v + j0 + j : (size_t)(k + j0 + j) * s->V + v;
+                re[j0 + j] = s->er[i];
+                im[j0 + j] = sgn * s->ei[i];
+            }
+            continue;
+        }
+        for (int j = 0; j < m; ++j) phi[j] = 0.0;
+        for (int l = 0; l < s->Lt; ++l) {
+            if (dir == 0) {
+                const double h = s->H[(size_t)l * s->K + k];
+                const double* b = s->B + (size_t)l * s->V + v + j0;
+                for (int j = 0; j < m; ++j) phi[j] += b[j] * h;
+            } else {
+                const double b = s->B[(size_t)l * s->V + v];
+                const double* h = s->H + (size_t)l * s->K + k + j0;
+                for (int j = 0; j < m; ++j) phi[j] += b * h[j];
+            }
+        }
+        for (int j = 0; j < m; ++j) {
+            double sn, cs;
+            sincos(phi[j], &sn, &cs);
+            re[j0 + j] = cs;
+            im[j0 + j] = sgn * sn;
+        }
+    }
+}
+
+/* B: column-major V x Lt, H: column-major K x Lt, maps: column-major V x C
+ * complex (interleaved) */
+mmref_hio* mmref_hio_create(int V, int K, int Lt, const double* B, const double* H, int C, const double* maps,
+                            int cache_exp) {
+    mmref_hio* s = (mmref_hio*)calloc(1, sizeof(mmref_hio));
+    if (!s) return NULL;
+    s->V = V; s->K = K; s->Lt = Lt; s->C = C;
+    s->B = (double*)malloc(sizeof(double) * (size_t)V * Lt);
+    s->H = (double*)malloc(sizeof(double) * (size_t)K * Lt);
+    s->maps = (cplx*)malloc(sizeof(cplx) * (size_t)V * C);
+    s->Cp = (C + HIO_CB - 1) / HIO_CB * HIO_CB;
+    s->zr = (double*)calloc((size_t)V * s->Cp, sizeof(double));
+    s->zi = (double*)calloc((size_t)V * s->Cp, sizeof(double));
+    s->wr = (double*)calloc((size_t)K * s->Cp, sizeof(double));
+    s->wi = (double*)calloc((size_t)K * s->Cp, sizeof(double));
+    if (!s->B || !s->H || !s->maps || !s->zr || !s->zi || !s->wr || !s->wi) { mmref_hio_destroy(s); return NULL; }
+    memcpy(s->B, B, sizeof(double) * (size_t)V * Lt);
+    memcpy(s->H, H, sizeof(double) * (size_t)K * Lt);
+    const cplx* mp = (const cplx*)maps;
+    for (int v = 0; v < V; ++v)
+        for (int c = 0; c < C; ++c) s->maps[(size_t)v * C + c] = mp[v + (size_t)V * c];
+    if (cache_exp) {
+        s->er = (double*)malloc(sizeof(double) * (size_t)K * V);
+        s->ei = (double*)malloc(sizeof(double) * (size_t)K * V);
+        if (!s->er || !s->ei) { mmref_hio_destroy(s); return NULL; }
+        double *er = s->er, *ei = s->ei;
+        s->er = s->ei = NULL;   /* computed while the cache fills */
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < K; ++k) hio_exp_run(s, 0, k, 0, V, 1.0, er + (size_t)k * V, ei + (size_t)k * V);
+        s->er = er;
+        s->ei = ei;
+    }
+    return s;
+}
+
+/* Both directions are complex products of a tile of exp(+-i Phi) values
+ * [rows x n] with a [n x C] operand, coils in blocks of 8 (padded with
+ * zeros), two rows at a time so the 32 accumulators stay in registers. */
+
+typedef double v4d __attribute__((vector_size(32)));
+
+static inline v4d ld4(const double* p) { v4d v; memcpy(&v, p, sizeof(v)); return v; }
+static inline void st4(double* p, v4d v) { memcpy(p, &v, sizeof(v)); }
+
+/* acc0 (row 0) and acc1 (row 1), each [re 8][im 8], += sum_j e_row[j] z[j][0..7] */
+static inline void mac2x8(int n, const double* er0, const double* ei0, const double* er1, const double* ei1,
+                          const double* zr, const double* zi, size_t ldz, double* acc0, double* acc1) {
+    v4d a0r0 = ld4(acc0), a0r1 = ld4(acc0 + 4), a0i0 = ld4(acc0 + 8), a0i1 = ld4(acc0 + 12);
+    v4d a1r0 = ld4(acc1), a1r1 = ld4(acc1 + 4), a1i0 = ld4(acc1 + 8), a1i1 = ld4(acc1 + 12);
+    for (int j = 0; j < n; ++j) {
+        const double* r = zr + (size_t)j * ldz;
+        const double* i = zi + (size_t)j * ldz;
+        const v4d r0 = ld4(r), r1 = ld4(r + 4), i0 = ld4(i), i1 = ld4(i + 4);
+        const v4d e0r = {er0[j], er0[j], er0[j], er0[j]}, e0i = {ei0[j], ei0[j], ei0[j], ei0[j]};
+        const v4d e1r = {er1[j], er1[j], er1[j], er1[j]}, e1i = {ei1[j], ei1[j], ei1[j], ei1[j]};
+        a0r0 += e0r * r0 - e0i * i0;
+        a0r1 += e0r * r1 - e0i * i1;
+        a0i0 += e0r * i0 + e0i * r0;
+        a0i1 += e0r * i1 + e0i * r1;
+        a1r0 += e1r * r0 - e1i * i0;
+        a1r1 += e1r * r1 - e1i * i1;
+        a1i0 += e1r * i0 + e1i * r0;
+        a1i1 += e1r * i1 + e1i * r1;
+    }
+    st4(acc0, a0r0); st4(acc0 + 4, a0r1); st4(acc0 + 8, a0i0); st4(acc0 + 12, a0i1);
+    st4(acc1, a1r0); st4(acc1 + 4, a1r1); st4(acc1 + 8, a1i0); st4(acc1 + 12, a1i1);
+}
+
+/* acc[rows][Cp/8][2][8] += E[rows x n] * Z[n x Cp]; E rows at stride lde */
+static void tile_mac(int rows, int n, const double* er, const double* ei, int lde, const double* zr,
+                     const double* zi, int Cp, double* acc) {
+    static const double zero[HIO_T2] = {0};
+    const int nb = Cp / HIO_CB;
+    for (int cb = 0; cb < nb; ++cb)
+        for (int r = 0; r < rows; r += 2) {
+            const int r1 = r + 1 < rows ? r + 1 : r;   /* odd count: the last row twice, second result dropped */
+            double dummy[2 * HIO_CB];
+            double* a0 = acc + ((size_t)r * nb + cb) * 2 * HIO_CB;
+            double* a1 = r + 1 < rows ? acc + ((size_t)r1 * nb + cb) * 2 * HIO_CB : dummy;
+            if (a1 == dummy) memset(dummy, 0, sizeof(dummy));
+            mac2x8(n, er + (size_t)r * lde, ei + (size_t)r * lde, r + 1 < rows ? er + (size_t)r1 * lde : zero,
+                   r + 1 < rows ? ei + (size_t)r1 * lde : zero, zr + cb * HIO_CB, zi + cb * HIO_CB, Cp, a0, a1);
+        }
+}
+
+void mmref_hio_forward(mmref_hio* s, const double* xin, double* yout) {
+    const int V = s->V, K = s->K, C = s->C, Cp = s->Cp;
+    const cplx* x = (const cplx*)xin;
+    cplx* y = (cplx*)yout;
+    const double scale = 1.0 / sqrt((double)V);
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < V; ++v)
+        for (int c = 0; c < C; ++c) {
+            const cplx z = s->maps[(size_t)v * C + c] * x[v];
+            s->zr[(size_t)v * Cp + c] = creal(z);
+            s->zi[(size_t)v * Cp + c] = cimag(z);
+        }
+#pragma omp parallel
+    {
+        double* acc = (double*)malloc(sizeof(double) * HIO_T1 * 2 * Cp);
+        double* er = (double*)malloc(sizeof(double) * HIO_T1 * HIO_T2);
+        double* ei = (double*)malloc(sizeof(double) * HIO_T1 * HIO_T2);
+#pragma omp for schedule(dynamic)
+        for (int k0 = 0; k0 < K; k0 += HIO_T1) {
+            const int nk = k0 + HIO_T1 < K ? HIO_T1 : K - k0;
+            memset(acc, 0, sizeof(double) * HIO_T1 * 2 * Cp);
+            for (int v0 = 0; v0 < V; v0 += HIO_T2) {
+                const int nv = v0 + HIO_T2 < V ? HIO_T2 : V - v0;
+                for (int k = 0; k < nk; ++k) hio_exp_run(s, 0, k0 + k, v0, nv, 1.0, er + k * HIO_T2, ei + k * HIO_T2);
+                tile_mac(nk, nv, er, ei, HIO_T2, s->zr + (size_t)v0 * Cp, s->zi + (size_t)v0 * Cp, Cp, acc);
+            }
+            const int nb = Cp / HIO_CB;
+            for (int k = 0; k < nk; ++k)
+                for (int c = 0; c < C; ++c) {
+                    const double* a = acc + ((size_t)k * nb + c / HIO_CB) * 2 * HIO_CB;
+                    y[k0 + k + (size_t)K * c] = scale * (a[c % HIO_CB] + I * a[HIO_CB + c % HIO_CB]);
+                }
+        }
+        free(acc); free(er); free(ei);
+    }
+}
+
+void mmref_hio_adjoint(mmref_hio* s, const double* yin, double* xout) {
+    const int V = s->V, K = s->K, C = s->C, Cp = s->Cp;
+    const cplx* y = (const cplx*)yin;
+    cplx* x = (cplx*)xout;
+    const double scale = 1.0 / sqrt((double)V);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < K; ++k)
+        for (int c = 0; c < C; ++c) {
+            s->wr[(size_t)k * Cp + c] = creal(y[k + (size_t)K * c]);
+            s->wi[(size_t)k * Cp + c] = cimag(y[k + (size_t)K * c]);
+        }
+#pragma omp parallel
+    {
+        double* acc = (double*)malloc(sizeof(double) * HIO_T1 * 2 * Cp);
+        double* er = (double*)malloc(sizeof(double) * HIO_T1 * HIO_T2);
+        double* ei = (double*)malloc(sizeof(double) * HIO_T1 * HIO_T2);
+#pragma omp for schedule(dynamic)
+        for (int v0 = 0; v0 < V; v0 += HIO_T1) {
+            const int nv = v0 + HIO_T1 < V ? HIO_T1 : V - v0;
+            memset(acc, 0, sizeof(double) * HIO_T1 * 2 * Cp);
+            for (int k0 = 0; k0 < K; k0 += HIO_T2) {
+                const int nk = k0 + HIO_T2 < K ? HIO_T2 : K - k0;
+                for (int v = 0; v < nv; ++v)
+                    hio_exp_run(s, 1, k0, v0 + v, nk, -1.0, er + v * HIO_T2, ei + v * HIO_T2);   /* exp(-i Phi) */
+                tile_mac(nv, nk, er, ei, HIO_T2, s->wr + (size_t)k0 * Cp, s->wi + (size_t)k0 * Cp, Cp, acc);
+            }
+            const int nb = Cp / HIO_CB;
+            for (int v = 0; v < nv; ++v) {
+                cplx t = 0;
+                for (int c = 0; c < C; ++c) {
+                    const double* a = acc + ((size_t)v * nb + c / HIO_CB) * 2 * HIO_CB;
+                    t += conj(s->maps[(size_t)(v0 + v) * C + c]) * (a[c % HIO_CB] + I * a[HIO_CB + c % HIO_CB]);
+                }
+                x[v0 + v] = scale * t;
+            }
+        }
+        free(acc); free(er); free(ei);
+    }
+}
+
+static void hio_fwd(void* s, const double* x, double* y) { mmref_hio_forward((mmref_hio*)s, x, y); }
+static void hio_adj(void* s, const double* y, double* x) { mmref_hio_adjoint((mmref_hio*)s, y, x); }
+
+int mmref_hio_cgne(mmref_hio* s, const double* yin, const double* x0in, double* xout, int NitMax,
+                   double resThresh, double resThresh_rel, double xChangeThresh, int expResN, double* resSqAll) {
+    const refop op = {hio_fwd, hio_adj, s, (size_t)s->V, (size_t)s->K * s->C};
+    return cgne_core(&op, yin, x0in, xout, NitMax, resThresh, resThresh_rel, xChangeThresh, expResN, resSqAll);
 }

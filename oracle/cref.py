@@ -51,6 +51,13 @@ def lib():
     L.mmref_cgne.restype = i32
     L.mmref_cgne.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, dbl, i32, vp]
     L.mmref_num_threads.restype = i32
+    L.mmref_hio_create.restype = vp
+    L.mmref_hio_create.argtypes = [i32, i32, i32, vp, vp, i32, vp, i32]
+    L.mmref_hio_destroy.argtypes = [vp]
+    L.mmref_hio_forward.argtypes = [vp, vp, vp]
+    L.mmref_hio_adjoint.argtypes = [vp, vp, vp]
+    L.mmref_hio_cgne.restype = i32
+    L.mmref_hio_cgne.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, dbl, i32, vp]
     _lib = L
     return L
 
@@ -111,3 +118,77 @@ class SenseRef:
                                int(NitMax), g(resThresh), g(resThreshRel), g(xChangeThresh), int(expResN),
                                hist.ctypes.data)
         return x.reshape((self.N, self.N), order="F"), hist[: nit + 1], int(nit)
+
+
+def _cgne_call(fn, h, y, x0, shape, NitMax, resThresh, resThreshRel, xChangeThresh, expResN):
+    yf = np.asfortranarray(np.asarray(y, dtype=np.complex128)).reshape(-1, order="F")
+    x0f = None if x0 is None else np.asfortranarray(np.asarray(x0, dtype=np.complex128)).reshape(-1, order="F")
+    x = np.empty(int(np.prod(shape)), dtype=np.complex128)
+    hist = np.zeros(NitMax + 1)
+    g = lambda v: -1.0 if v is None else float(v)
+    nit = fn(h, yf.ctypes.data, None if x0f is None else x0f.ctypes.data, x.ctypes.data, int(NitMax), g(resThresh),
+             g(resThreshRel), g(xChangeThresh), int(expResN), hist.ctypes.data)
+    return x.reshape(shape, order="F"), hist[: nit + 1], int(nit)
+
+
+class HighOrderSenseRef:
+    """FP64 expanded-encoding SENSE operator E = S o R (direct approach) of
+    one 2-D slice with ``cgne``; the phase model is the NumPy oracle's
+    (``oracle.highorder.HighOrderPlan``: harmonics, concomitant terms and
+    B0, with the mask), handed over as basis columns B [V x Lt] and
+    coefficient columns H [K x Lt].  ``cache_exp`` keeps exp(i Phi) between
+    applications (16 V K bytes)."""
+
+    def __init__(self, b0, sampTimes, phs_spha, phs_conc, phs_grid, maps, b0mask=None, cache_exp=False):
+        from .basis import basis_conc, basis_harm
+        from .highorder import HighOrderPlan
+        P = HighOrderPlan(b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask)
+        mask = P.b0mask
+        cols_b, cols_h = [], []
+        for n in range(1, P.phs_spha.shape[0] + 1):
+            bf = basis_harm(P.x, P.y, P.z, n)
+            cols_b.append(bf * mask if (n > 4 and mask is not None) else bf)
+            cols_h.append(P.phs_spha[n - 1])
+        for n in range(1, P.phs_conc.shape[0] + 1):
+            bf = basis_conc(P.x, P.y, P.z, n)
+            cols_b.append(bf * mask if mask is not None else bf)
+            cols_h.append(P.phs_conc[n - 1])
+        if P.K > 1:
+            cols_b.append(P.b0 * mask if mask is not None else P.b0)
+            cols_h.append(P.sampTimes.reshape(-1, order="F"))
+        self._B = np.asfortranarray(np.stack(cols_b, axis=1))
+        self._H = np.asfortranarray(np.stack(cols_h, axis=1))
+        mp = np.asfortranarray(np.asarray(maps, dtype=np.complex128))
+        self.imSize, self.V, self.K = P.imSize, P.V, P.K
+        self.C = mp.shape[-1]
+        self._maps = mp.reshape(-1, order="F")
+        h = lib().mmref_hio_create(self.V, self.K, self._B.shape[1], self._B.ctypes.data, self._H.ctypes.data,
+                                   self.C, self._maps.ctypes.data, int(bool(cache_exp)))
+        if not h:
+            raise MemoryError("mmref_hio_create failed")
+        self._h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().mmref_hio_destroy(self._h)
+        except Exception:
+            pass
+
+    def forward(self, x):
+        xf = np.asfortranarray(np.asarray(x, dtype=np.complex128)).reshape(-1, order="F")
+        y = np.empty(self.K * self.C, dtype=np.complex128)
+        lib().mmref_hio_forward(self._h, xf.ctypes.data, y.ctypes.data)
+        return y.reshape((self.K, self.C), order="F")
+
+    def adjoint(self, y):
+        yf = np.asfortranarray(np.asarray(y, dtype=np.complex128)).reshape(-1, order="F")
+        x = np.empty(self.V, dtype=np.complex128)
+        lib().mmref_hio_adjoint(self._h, yf.ctypes.data, x.ctypes.data)
+        return x.reshape(self.imSize, order="F")
+
+    def cgne(self, y, x0=None, NitMax=100, resThresh=None, resThreshRel=None, xChangeThresh=None,
+             expResN=100):
+        """Returns (x, resSqAll, nit)."""
+        return _cgne_call(lib().mmref_hio_cgne, self._h, y, x0, self.imSize, NitMax, resThresh, resThreshRel,
+                          xChangeThresh, expResN)

@@ -204,8 +204,8 @@ def test_cg_x0_fallback(mm):
 def test_trained_warm_start_gain(mm):
     """The committed network (trained on the device on CG solutions of 20000
     spiral SENSE problems) against zero-init cgne on 100 held-out problems:
-    fewer iterations to relative residual 1e-4 (1.6x measured on 1000), no
-    fallback, and -- on one problem -- a solution as close to the converged
+    at least 1.5x fewer iterations to relative residual 1e-4 (1.61x measured
+    on these 100 and on all 1000), no fallback, and -- on one problem -- a solution as close to the converged
     FP64 oracle solution as the zero-init one (NRMSE 0.127 vs 0.133 measured)."""
     import torch
     from oracle.nufft import NufftPlan
@@ -226,7 +226,7 @@ def test_trained_warm_start_gain(mm):
           flush=True)
     assert np.all(stw == 1) and np.all(stz == 1)     # stopped on the residual threshold
     assert fb.sum() == 0
-    assert ratio >= 1.4
+    assert ratio >= 1.5
     # FP64 oracle, problem 0, solved far past 1e-4
     N, M, Cc = 128, 16384, 8
     So = NufftPlan([N, N], fam.kloc, None, 1.5, 4)

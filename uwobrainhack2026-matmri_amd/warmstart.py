@@ -304,10 +304,11 @@ def solve_zero(prob, rel_resid=1e-4, NitMax=200):
 
 def solve_refined(prob, rel_resid, NitMax, rounds=1):
     """Zero-init cgne followed by restarts from the current iterate (each
-    restart recomputes the residual b - E'E x explicitly, which is iterative
-    refinement): FP32 CG alone stalls near a relative residual of 1e-5, the
-    restarts carry it below 1e-6.  Returns (x, final relative residual per
-    slice, total iterations per slice)."""
+    restart recomputes the residual b - E'E x explicitly).  With complex64
+    operators the relative residual stalls near 1e-5 and the restarts do
+    not lower it below that floor (the explicit residual carries the same
+    FP32 operator error), so a 1e-6 target stops at the floor.  Returns (x,
+    final relative residual per slice, total iterations per slice)."""
     x0 = torch.zeros_like(prob["xin"])
     tot = None
     rel = None
