@@ -36,6 +36,11 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def note(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 sys.path.insert(0, ROOT)
 
 PEAK_FP16_MFMA_TFLOPS = 2500.0   # MI355X dense fp16 matrix peak (the high-order GEMM runs on it)
@@ -51,6 +56,12 @@ def parse():
                    help="copy: a plain 1 GiB device copy (HBM counter calibration, not a solve)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--heldout", type=int, default=1000, help="held-out problems of the warm-start workload")
+    p.add_argument("--precision", action="store_true",
+                   help="highorder: add the full-size precision study against the C/OpenMP FP64 oracle "
+                        "(per-application errors and 30-iteration NRMSE; a few CPU minutes)")
+    p.add_argument("--no-batch-roofline", action="store_true",
+                   help="skip the 64-slice batch roofline of the warm-start line")
     p.add_argument("--hio-variant", type=int, default=None, help="high-order MFMA kernel variant 0..3")
     p.add_argument("--no-autotune", action="store_true",
                    help="skip the untimed autotuner of the NUFFT SENSE operator (mm_op_autotune)")
@@ -181,36 +192,94 @@ def cpu_baseline_batch(Ps, seconds):
 
 
 def cpu_baseline_highorder(P, seconds):
-    """FP64 oracle (NumPy) forward application of the same SENSE operator on a
-    bounded sample of time points, extrapolated to a full solve."""
+    """The C/OpenMP FP64 restatement (oracle/c/mmref.c, direct approach: the
+    phase and exp(i Phi) recomputed every application, as the reference's
+    direct worker evaluates exp(1i*kbase) per call) of the same SENSE
+    operator, forward + adjoint on a bounded sample of time points, scaled
+    to a 30-iteration solve (65 applications)."""
     import numpy as np
-    from oracle.highorder import HighOrderPlan
-    from oracle.rcvr import RcvrPlan
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
-    except Exception:
-        threads = os.cpu_count() or 1
+    cref = _cref()
+    if cref is None:
+        return None
     K = P["K"]
-    Ro = RcvrPlan(P["maps"])
-    x = Ro.forward(P["phantom"])
-    ks = 64
-    elapsed, done = 0.0, 0
-    # grow the sample until it takes a fraction of the budget, then measure
+    ks, x = 256, P["phantom"]
     while True:
-        Ssub = HighOrderPlan(P["b0"], P["t"][:ks], P["h"][:, :ks], None, P["grid"])
+        E = cref.HighOrderSenseRef(P["b0"], P["t"][:ks], P["h"][:, :ks], None, P["grid"], P["maps"])
         t0 = time.perf_counter()
-        Ssub.forward(x)
+        y = E.forward(x)
+        E.adjoint(y)
         dt = time.perf_counter() - t0
         if dt > seconds / 3 or ks >= K:
-            elapsed, done = dt, ks
             break
         ks = min(K, int(ks * max(2.0, (seconds / 3) / max(dt, 1e-3))))
-    per_app = elapsed * K / done
+    per_app = dt / 2 * K / ks
     apps = 2 * 30 + 5  # cgne with 30 iterations: 32 forward + 33 adjoint applications
-    return {"value": 1.0 / (per_app * apps), "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 NumPy oracle forward (exp + zgemm) over {done} of {K} samples, "
-                      f"{elapsed:.2f} s, scaled to one 30-iteration solve ({apps} applications)"}
+    return {"value": 1.0 / (per_app * apps), "unit": "solves/s", "cores": cref.num_threads(), "kind": "port",
+            "sample": f"C/OpenMP FP64 restatement (oracle/c/mmref.c), one forward + one adjoint over {ks} of "
+                      f"{K} samples ({dt:.2f} s), scaled to one 30-iteration solve ({apps} applications)"}
+
+
+def highorder_precision(mm, torch, dev, NitMax=30):
+    """The precision study on the high-order operator with a 70 kHz
+    zeroth-order term (max |Phi| > 1e4 rad at late samples): per-application
+    relative L2 error of the device forward and adjoint, and the NRMSE of a
+    30-iteration device cgne against the same 30 iterations of the FP64
+    C/OpenMP oracle (exp(i Phi) cached between its applications), from zero
+    and with no early stop, at the full 220 x 220 / 24000 / 32-coil size."""
+    import numpy as np
+    cref = _cref()
+    if cref is None:
+        return None
+    P = mm.workloads.highorder_problem(seed=2, f0_hz=70e3)
+    t0 = time.perf_counter()
+    Eo = cref.HighOrderSenseRef(P["b0"], P["t"], P["h"], None, P["grid"], P["maps"], cache_exp=True)
+    S = mm.sampHighOrder(P["b0"], P["t"], P["h"], None, P["grid"])
+    E = mm.senseOp(S, mm.rcvrOp(P["maps"]))
+    rng = np.random.default_rng(3)
+    N, K, C = P["N"], P["K"], P["C"]
+    x = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    yr = rng.standard_normal((K, C)) + 1j * rng.standard_normal((K, C))
+    dev_fwd = (E * torch.as_tensor(np.ascontiguousarray(x.T), dtype=torch.complex64, device=dev)).cpu().numpy()
+    dev_adj = (E.H * torch.as_tensor(np.ascontiguousarray(yr.T), dtype=torch.complex64, device=dev)).cpu().numpy()
+    yo, xo = Eo.forward(x), Eo.adjoint(yr)
+    ef = float(np.linalg.norm(dev_fwd.reshape(C, K).T - yo) / np.linalg.norm(yo))
+    ea = float(np.linalg.norm(dev_adj.reshape(N, N).T - xo) / np.linalg.norm(xo))
+    y = Eo.forward(P["phantom"])
+    y = y + np.sqrt(np.mean(np.abs(y) ** 2)) / 40.0 / np.sqrt(2) * (rng.standard_normal(y.shape)
+                                                                    + 1j * rng.standard_normal(y.shape))
+    note("precision: per-application errors done; oracle cgne")
+    cref.set_verbose(True)
+    xo30, ro, nit = Eo.cgne(y, np.zeros((N, N), complex), NitMax)
+    # the same FP64 solve behind complex64 operator I/O: the drift that
+    # single-precision operator inputs and outputs alone cause
+    note("precision: oracle cgne with complex64 operator I/O")
+    Eo.set_round_io(True)
+    x3, r3, _ = Eo.cgne(y, np.zeros((N, N), complex), NitMax)
+    Eo.set_round_io(False)
+    cref.set_verbose(False)
+    yt = torch.as_tensor(np.ascontiguousarray(y.T), dtype=torch.complex64, device=dev)
+    xg, rg, *_ = mm.cgne(E, yt, torch.zeros(N * N, dtype=torch.complex64, device=dev), NitMax,
+                         {"resThresh": None, "xChangeThresh": None})
+    xg = xg.cpu().numpy().reshape(N, N).T
+    rg = np.asarray(rg, dtype=np.float64).reshape(-1)
+    nrmse = float(np.linalg.norm(xg - xo30) / np.linalg.norm(xo30))
+    drift = float(np.linalg.norm(x3 - xo30) / np.linalg.norm(xo30))
+    dg = np.abs(rg[: len(ro)] - ro) / ro
+    d3 = np.abs(r3[: len(ro)] - ro) / ro
+    from oracle.highorder import HighOrderPlan
+    late = slice(K - 64, K)
+    maxphi = float(np.max(np.abs(HighOrderPlan(P["b0"], P["t"][late], P["h"][:, late], None, P["grid"]).phase())))
+    return {"workload": "precision: 220x220, 32 coils, K=24000 @1us, 16 SH + B0 + 70 kHz f0 term",
+            "max_abs_phase_rad_last_64_samples": maxphi,
+            "forward_rel_l2": ef, "adjoint_rel_l2": ea,
+            "cgne_iterations": int(nit), "nrmse_30_iterations": nrmse,
+            "nrmse_30_iterations_fp64_with_c64_io": drift,
+            "nrmse_ratio_device_over_c64_io": nrmse / drift if drift > 0 else None,
+            "resSq_rel_diff_per_iteration": [float(v) for v in dg],
+            "resSq_rel_diff_per_iteration_c64_io": [float(v) for v in d3],
+            "rel_resid_final": float(np.sqrt(ro[-1] / ro[0])),
+            "oracle": f"C/OpenMP FP64 (oracle/c/mmref.c), {cref.num_threads()} threads",
+            "seconds": time.perf_counter() - t0}
 
 
 def cpu_baseline_nufft(P, seconds):
@@ -352,6 +421,58 @@ def nufft_roofline(mm, E, dev, slices, workload, iters=20):
              "algorithmic_bytes_per_launch": byts}, stages)
 
 
+def copy_gbps(torch, dev, reps=5):
+    """Measured device copy rate (1 GiB read + 1 GiB written): the practical
+    HBM ceiling the roofline fractions are also quoted against."""
+    n = 1 << 28
+    x = torch.empty(n, dtype=torch.float32, device=dev).uniform_()
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * 4 * n * reps / el / 1e9
+
+
+def annotate_roofline(roof, copy, llc_mb=256.0):
+    """Adds the fraction of the measured copy rate and whether one launch's
+    compulsory bytes fit in the 256 MB memory-side cache (then part of the
+    input can come from it, and the algorithmic rate is cache-assisted)."""
+    if not roof:
+        return roof
+    roof["copy_GBps"] = copy
+    roof["frac_of_copy"] = roof["achieved"] / copy
+    mb = roof["algorithmic_bytes_per_launch"] / 1e6
+    roof["cache_assisted"] = bool(mb < 2 * llc_mb)
+    roof["note"] = (f"{mb:.0f} MB of compulsory traffic per launch; the previous stage's output (about half of "
+                    f"it) can still sit in the {llc_mb:.0f} MB memory-side cache, so the algorithmic rate is "
+                    "cache-assisted" if roof["cache_assisted"] else
+                    f"{mb:.0f} MB per launch, well above the {llc_mb:.0f} MB memory-side cache")
+    return roof
+
+
+def batch_roofline(mm, torch, dev, copy):
+    """Dominant-kernel roofline of the uneven 64-slice, 16-coil batch (about
+    1.4 GB per normal-operator application): a shape the memory-side cache
+    cannot hold."""
+    Ps = mm.workloads.batch_problems(seed=4, nslice=64, C=16)
+    S = mm.nufftOp.batch([([p["N"], p["N"]], p["kloc"], None) for p in Ps], 1.5, 4)
+    E = mm.senseOp(S, mm.rcvrOp.batch([p["maps"] for p in Ps]))
+    roof, stages = nufft_roofline(mm, E, dev, [(p["N"], p["M"], 16) for p in Ps], "batch")
+    if roof:
+        roof = annotate_roofline(roof, copy)
+        roof["workload"] = "batch: 64 uneven slices (N 96..160), 16 coils, default tuning options"
+        roof["normal_operator_applications_per_s"] = applications_per_s(stages)
+    del E, S
+    torch.cuda.empty_cache()
+    return roof
+
+
 def toeplitz_normal(mm, fam, prob, dev, slices):
     """The Toeplitz form of the same SENSE normal operator (nufftOp.prepToep,
     nufftOp.m:410-432: row and column FFTs on a grid of at least 2N per side
@@ -406,26 +527,30 @@ def toeplitz_normal(mm, fam, prob, dev, slices):
 
 
 def cpu_baseline_toeplitz(fam, P, seconds=10.0):
-    """The oracle's Toeplitz form (FP64 NumPy, ``oracle.nufft.ToepPlan``,
-    one thread) of the same SENSE normal operator, R'(T(R x)) for 8-coil
-    128 x 128 slices, timed on as many slices as fit `seconds` (plan built
-    once, untimed) and scaled to applications of the whole P-slice batch."""
+    """The CPU normal operator of the same SENSE slices: forward + adjoint of
+    the C/OpenMP FP64 restatement (oracle/c/mmref.c; the gridding form the
+    reference's CPU solves apply), timed on as many of the 8-coil 128 x 128
+    slices as fit `seconds` and scaled to applications of the P-slice
+    batch."""
     import numpy as np
-    from oracle.nufft import ToepPlan
+    cref = _cref()
+    if cref is None:
+        return None
     N, C = fam.p["N"], fam.p["C"]
-    T = ToepPlan([N, N], fam.S._kl[0], fam.S._dc[0], 1.5, fam.S.kwidth)
     rng = np.random.default_rng(0)
     maps = rng.standard_normal((N, N, C)) + 1j * rng.standard_normal((N, N, C))
     x = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    E = cref.SenseRef(N, fam.kloc, None, maps, fam.p["os"], fam.p["kwidth"])
     n, t_tot = 0, 0.0
-    while t_tot < seconds and n < P:
+    while t_tot < seconds:
         t0 = time.perf_counter()
-        _ = np.sum(np.conj(maps) * T.apply(maps * x[..., None]), axis=2)
+        E.adjoint(E.forward(x))
         t_tot += time.perf_counter() - t0
         n += 1
-    return {"value": n / t_tot / P, "unit": "normal-operator applications/s (whole batch)", "cores": 1,
-            "kind": "port", "sample": f"NumPy FP64 Toeplitz form (oracle ToepPlan) on {n} of the {P} slices, "
-                                      f"{t_tot:.1f} s, scaled to the batch"}
+    return {"value": n / t_tot / P, "unit": "normal-operator applications/s (whole batch)",
+            "cores": cref.num_threads(), "kind": "port",
+            "sample": f"C/OpenMP FP64 restatement (oracle/c/mmref.c) E'E of one slice, {n} applications in "
+                      f"{t_tot:.1f} s, scaled to the {P}-slice batch"}
 
 
 def applications_per_s(stages):
@@ -499,47 +624,68 @@ def run_warmstart(a, mm, torch, dev):
     ws = mm.warmstart
     net = mm.WarmStartNet.load()
     P = 100
+    NCH = max(1, a.heldout // P)   # 100-problem chunks of the held-out set
     fam = ws.ProblemFamily(P)
     for kv in a.set:
         k, v = kv.split("=")
         fam.S.set_option(k, int(v))
     N, M, Cc = fam.p["N"], fam.p["M"], fam.p["C"]
-    seeds = list(range(ws.TRAIN_SEED0 + ws.HELDOUT_OFFSET, ws.TRAIN_SEED0 + ws.HELDOUT_OFFSET + P))
-    host = fam.host_batch(seeds)
-    prob = fam.make(seeds, host)
+    seed0 = ws.TRAIN_SEED0 + ws.HELDOUT_OFFSET
+    chunks = []
+    for c in range(NCH):
+        seeds = list(range(seed0 + c * P, seed0 + (c + 1) * P))
+        host = fam.host_batch(seeds)
+        chunks.append((fam.make(seeds, host), host))
+        note(f"held-out chunk {c + 1}/{NCH} on the device")
+    prob, host = chunks[0]
     E, R, y = prob["E"], prob["R"], prob["y"]
-    tuned = None if a.no_autotune or a.set else E.autotune(10)
+    tuned = None if a.no_autotune or a.set else E.autotune(10)   # the NUFFT plan all chunks share
     NitMax, rel = 200, 1e-4
     zeros = torch.zeros((P, N, N), dtype=torch.complex64, device=dev)
 
-    def warm_step():
-        xin = ws.input_image(E, R, y, fam.dcf_t).reshape(P, N, N)
+    def warm_step(j):
+        pr = chunks[j % NCH][0]
+        xin = ws.input_image(pr["E"], pr["R"], pr["y"], fam.dcf_t).reshape(P, N, N)
         x0 = ws.warm_x0(net, xin)
-        return ws.solve(E, y, x0, rel, NitMax, fallback=True)
+        return ws.solve(pr["E"], pr["y"], x0, rel, NitMax, fallback=True)
 
-    def zero_step():
-        return ws.solve(E, y, zeros, rel, NitMax)
+    def zero_step(j):
+        pr = chunks[j % NCH][0]
+        return ws.solve(pr["E"], pr["y"], zeros, rel, NitMax)
 
+    # one untimed pass over every held-out problem: iteration counts of the
+    # whole held-out set (and every chunk's solver graph captured)
+    every = {"zero": [], "warm": [], "fallbacks": 0, "b2": [], "rel_w": []}
+    for j in range(NCH):
+        _, nz, _, _, hz = zero_step(j)
+        _, nw, _, fb, hw = warm_step(j)
+        every["zero"].append(nz)
+        note(f"priming pass: chunk {j + 1}/{NCH} solved")
+        every["warm"].append(nw)
+        every["fallbacks"] += int(fb.sum())
+        b2 = np.array([h[0] for h in hz])
+        every["rel_w"].append(np.sqrt(np.array([h[-1] for h in hw]) / b2))
+    it_zero, it_warm = np.concatenate(every["zero"]), np.concatenate(every["warm"])
     res = {}
+    note("timed steps")
     for name, fn in (("zero", zero_step), ("warm", warm_step)):
-        for _ in range(a.warmup):
-            fn()
+        for j in range(a.warmup):
+            fn(j)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         nits, fbs = [], 0
-        for _ in range(a.steps):
-            x, nit, st, fb, hist = fn()
+        for j in range(a.steps):
+            x, nit, st, fb, hist = fn(j)
             nits.append(nit)
             fbs += int(fb.sum())
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         nits = np.concatenate(nits)
         res[name] = dict(el=el, nit=nits, fallbacks=fbs, status=np.asarray(st), x=x)
-    # relative residuals against |b| (the zero-init history starts at |b|^2)
-    _, _, _, _, hz = zero_step()
-    b2 = np.array([h[0] for h in hz])
-    _, _, _, _, hw = warm_step()
-    rel_w = np.sqrt(np.array([h[-1] for h in hw]) / b2)
+    rel_w = np.concatenate(every["rel_w"])
+    # the chunk the last timed step solved, for the oracle comparison
+    prob, host = chunks[(a.steps - 1) % NCH]
+    E, R, y = prob["E"], prob["R"], prob["y"]
     # network inference alone
     xin = ws.input_image(E, R, y, fam.dcf_t).reshape(P, N, N)
     torch.cuda.synchronize()
@@ -548,12 +694,15 @@ def run_warmstart(a, mm, torch, dev):
         ws.warm_x0(net, xin)
     torch.cuda.synchronize()
     t_inf = (time.perf_counter() - t0) / 10
+    note("roofline")
     roof, stages = nufft_roofline(mm, E, dev, [(N, M, Cc)] * P, "warmstart")
     zw, zz = res["warm"], res["zero"]
-    cfg = {"workload": "warmstart: 100 held-out 2-D spiral SENSE problems per step (128x128 c64, 8 coils, "
-                       "M=16384 VD spiral, KB W=4 os=1.5, SNR 40), one multi-slice solve to rel. resid 1e-4; "
-                       "x0 from the learned 6x(3x3 conv, 32 ch) warm start with zero-init fallback",
-           "problems_per_step": P, "image": [N, N], "coils": Cc, "samples": M, "rel_resid": rel,
+    cfg = {"workload": f"warmstart: {NCH * P} held-out 2-D spiral SENSE problems (128x128 c64, 8 coils, "
+                       "M=16384 VD spiral, KB W=4 os=1.5, SNR 40), 100 per step (step j solves chunk j mod "
+                       f"{NCH}) as one multi-slice solve to rel. resid 1e-4; x0 from the learned "
+                       "6x(3x3 conv, 32 ch) warm start with zero-init fallback",
+           "problems_per_step": P, "heldout_problems": NCH * P, "image": [N, N], "coils": Cc, "samples": M,
+           "rel_resid": rel,
            "max_iterations": NitMax, "network": "6 layers 3x3 conv, 2-32-32-32-32-32-2, residual skip, "
                                                 "bf16 weights, fp32 accumulation",
            "autotuned": tuned}
@@ -570,18 +719,29 @@ def run_warmstart(a, mm, torch, dev):
            "warm_start": {"iterations_mean": float(zw["nit"].mean()), "iterations_max": int(zw["nit"].max()),
                           "fallbacks": zw["fallbacks"], "inference_ms_per_step": 1e3 * t_inf},
            "iteration_ratio_zero_over_warm": float(zz["nit"].mean() / zw["nit"].mean()),
+           "heldout_all": {"problems": int(it_zero.size), "iterations_mean_zero": float(it_zero.mean()),
+                           "iterations_mean_warm": float(it_warm.mean()),
+                           "iteration_ratio_zero_over_warm": float(it_zero.mean() / it_warm.mean()),
+                           "fallbacks": every["fallbacks"]},
            "speedup_warm_over_zero": float(zz["el"] / zw["el"]),
            "rel_resid_final_max": float(rel_w.max()),
            "status_last": {str(int(k)): int(v) for k, v in zip(*np.unique(zw["status"], return_counts=True))}}
+    copy = copy_gbps(torch, dev)
     if roof:
-        out["roofline"] = roof
+        out["roofline"] = annotate_roofline(roof, copy)
     out["normal_operator_stages"] = stages
     out["normal_operator_applications_per_s"] = applications_per_s(stages)
+    if not a.no_batch_roofline:
+        note("64-slice batch roofline")
+        out["roofline_above_cache"] = batch_roofline(mm, torch, dev, copy)
+    note("Toeplitz normal operator")
     out["toeplitz_normal"] = toeplitz_normal(mm, fam, prob, dev, [(N, M, Cc)] * P)
     if out["toeplitz_normal"] and not a.no_cpu:
         out["toeplitz_normal"]["cpu_baseline"] = cpu_baseline_toeplitz(fam, P)
     if not a.no_cpu:
+        note("FP64 oracle NRMSE")
         out["nrmse_vs_fp64_oracle"] = oracle_nrmse_warmstart(fam, host, prob, zw["x"], zz["x"])
+        note("CPU baseline")
         cb = cpu_baseline_warmstart(fam, host, prob, a.cpu_seconds)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
@@ -693,6 +853,10 @@ def main():
             out["roofline"] = roof_nu
         out["normal_operator_stages"] = stages_nu
         out["normal_operator_applications_per_s"] = applications_per_s(stages_nu)
+    if W["kind"] == "highorder" and a.precision:
+        with torch.cuda.stream(stream):
+            note("precision study (C/OpenMP FP64 oracle at full size)")
+            out["precision"] = highorder_precision(mm, torch, dev)
     if not a.no_cpu:
         cb = {"highorder": cpu_baseline_highorder, "nufft": cpu_baseline_nufft,
               "batch": cpu_baseline_batch}[W["kind"]](W["P"], a.cpu_seconds)

@@ -244,14 +244,17 @@ int mm_op_profile(mm_op* op, int adjoint, const void* in, void* out, int64_t nre
  * MM_ERR_ARG past the last stage. */
 int mm_op_profile_stage(mm_op* op, int idx, double* ms_per_launch, int* launches, char* kernel_name,
                         int name_len);
-/* Tuning knob (no counterpart in the reference): "hio_variant" (0..3: bit 0
- * phase on FP64 MFMA, bit 1 hardware sine/cosine) and "hio_mfma" (0/1) of a
+/* Tuning knob (no counterpart in the reference): "hio_variant" (0..7: bit 0
+ * phase on FP64 MFMA, bit 1 hardware sine/cosine, bit 2 the software-pipelined
+ * form of variant 3) and "hio_mfma" (0/1) of a
  * high-order operator or a SENSE operator built on one; for a NUFFT operator
  * (or a SENSE operator built on one) "nu_stage_samples" (16..256),
  * "nu_stage_entries" (64..2048), "nu_degrid_block" (16..256),
  * "nu_slice_group" (slices per group of the normal operator, 0: one), and the 0/1
- * switches "nu_pair_cols", "nu_xcd_order", "nu_grid_pairs", "nu_snake", "nu_degrid_merge", "nu_grid_stage",
- * "nu_degrid_stage", "nu_degrid_t", "nu_toep_fused" (the Toeplitz form's 2-D middle stage in one pass). */
+ * switches "nu_pair_cols", "nu_xcd_order", "nu_grid_pairs", "nu_grid_region" (with "nu_xcd_order": the
+ * staged gridding items of one spatial region of a trajectory's rectangles on one XCD, for every slice),
+ * "nu_snake", "nu_degrid_merge", "nu_grid_stage", "nu_degrid_stage", "nu_degrid_t", "nu_toep_fused" (the
+ * Toeplitz form's 2-D middle stage in one pass). */
 int mm_op_set_option(mm_op* op, const char* key, int value);
 /* Current value of a tuning option of mm_op_set_option (no counterpart in
  * the reference). */

@@ -22,6 +22,7 @@
 #define _GNU_SOURCE
 #include <complex.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -396,6 +397,11 @@ static cplx dot_c(const cplx* a, const cplx* b, size_t n) {
     return sr + I * si;
 }
 
+static int g_verbose = 0;
+
+/* v != 0: cgne prints one line per iteration on stderr */
+void mmref_set_verbose(int v) { g_verbose = v; }
+
 /* an encoding operator E: image [V] -> data [D] and back */
 typedef struct {
     void (*fwd)(void* s, const double* x, double* y);
@@ -458,6 +464,10 @@ static int cgne_core(const refop* op, const double* yin, const double* x0in, dou
             xd += creal(conj(d) * d);
         }
         if (resSqAll) resSqAll[nit + 1] = rr;
+        if (g_verbose) {
+            fprintf(stderr, "[mmref] cgne iteration %d: resSq %.6e\n", nit + 1, rr);
+            fflush(stderr);
+        }
         if (xChangeThresh >= 0 && xd < xChangeThresh) finished = 1;
         if (thr >= 0 && rr < thr) finished = 1;
         const double be = betaFact * rr / rr_old;
@@ -499,7 +509,14 @@ typedef struct {
     double *er, *ei;  /* exp(i Phi) [K][V], or NULL */
     double *zr, *zi;  /* [V][Cp] maps .* x */
     double *wr, *wi;  /* [K][Cp] data transposed */
+    int round_io;     /* inputs and outputs of every application rounded to complex64 */
 } mmref_hio;
+
+/* v != 0: every application rounds its input and its output to single
+ * precision (an FP64 operator behind complex64 I/O) */
+void mmref_hio_set_round_io(mmref_hio* s, int v) { s->round_io = v; }
+
+static inline cplx round_c64(cplx z) { return (double)(float)creal(z) + I * (double)(float)cimag(z); }
 
 void mmref_hio_destroy(mmref_hio* s) {
     if (!s) return;
@@ -637,7 +654,7 @@ void mmref_hio_forward(mmref_hio* s, const double* xin, double* yout) {
 #pragma omp parallel for schedule(static)
     for (int v = 0; v < V; ++v)
         for (int c = 0; c < C; ++c) {
-            const cplx z = s->maps[(size_t)v * C + c] * x[v];
+            const cplx z = s->maps[(size_t)v * C + c] * (s->round_io ? round_c64(x[v]) : x[v]);
             s->zr[(size_t)v * Cp + c] = creal(z);
             s->zi[(size_t)v * Cp + c] = cimag(z);
         }
@@ -659,7 +676,8 @@ void mmref_hio_forward(mmref_hio* s, const double* xin, double* yout) {
             for (int k = 0; k < nk; ++k)
                 for (int c = 0; c < C; ++c) {
                     const double* a = acc + ((size_t)k * nb + c / HIO_CB) * 2 * HIO_CB;
-                    y[k0 + k + (size_t)K * c] = scale * (a[c % HIO_CB] + I * a[HIO_CB + c % HIO_CB]);
+                    const cplx r = scale * (a[c % HIO_CB] + I * a[HIO_CB + c % HIO_CB]);
+                    y[k0 + k + (size_t)K * c] = s->round_io ? round_c64(r) : r;
                 }
         }
         free(acc); free(er); free(ei);
@@ -674,8 +692,9 @@ void mmref_hio_adjoint(mmref_hio* s, const double* yin, double* xout) {
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < K; ++k)
         for (int c = 0; c < C; ++c) {
-            s->wr[(size_t)k * Cp + c] = creal(y[k + (size_t)K * c]);
-            s->wi[(size_t)k * Cp + c] = cimag(y[k + (size_t)K * c]);
+            const cplx u = s->round_io ? round_c64(y[k + (size_t)K * c]) : y[k + (size_t)K * c];
+            s->wr[(size_t)k * Cp + c] = creal(u);
+            s->wi[(size_t)k * Cp + c] = cimag(u);
         }
 #pragma omp parallel
     {
@@ -699,7 +718,7 @@ void mmref_hio_adjoint(mmref_hio* s, const double* yin, double* xout) {
                     const double* a = acc + ((size_t)v * nb + c / HIO_CB) * 2 * HIO_CB;
                     t += conj(s->maps[(size_t)(v0 + v) * C + c]) * (a[c % HIO_CB] + I * a[HIO_CB + c % HIO_CB]);
                 }
-                x[v0 + v] = scale * t;
+                x[v0 + v] = s->round_io ? round_c64(scale * t) : scale * t;
             }
         }
         free(acc); free(er); free(ei);

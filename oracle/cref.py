@@ -51,15 +51,22 @@ def lib():
     L.mmref_cgne.restype = i32
     L.mmref_cgne.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, dbl, i32, vp]
     L.mmref_num_threads.restype = i32
+    L.mmref_set_verbose.argtypes = [i32]
     L.mmref_hio_create.restype = vp
     L.mmref_hio_create.argtypes = [i32, i32, i32, vp, vp, i32, vp, i32]
     L.mmref_hio_destroy.argtypes = [vp]
+    L.mmref_hio_set_round_io.argtypes = [vp, i32]
     L.mmref_hio_forward.argtypes = [vp, vp, vp]
     L.mmref_hio_adjoint.argtypes = [vp, vp, vp]
     L.mmref_hio_cgne.restype = i32
     L.mmref_hio_cgne.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, dbl, i32, vp]
     _lib = L
     return L
+
+
+def set_verbose(v):
+    """Per-iteration cgne lines on stderr (long solves report progress)."""
+    lib().mmref_set_verbose(int(bool(v)))
 
 
 def num_threads():
@@ -174,6 +181,11 @@ class HighOrderSenseRef:
                 lib().mmref_hio_destroy(self._h)
         except Exception:
             pass
+
+    def set_round_io(self, v):
+        """Round every application's input and output to complex64 (an FP64
+        operator behind single-precision I/O)."""
+        lib().mmref_hio_set_round_io(self._h, int(bool(v)))
 
     def forward(self, x):
         xf = np.asfortranarray(np.asarray(x, dtype=np.complex128)).reshape(-1, order="F")

@@ -427,6 +427,15 @@ def test_nufft_tuning_options_and_autotune(mm):
                       ("nu_degrid_block", [16, 128, 256]), ("nu_pair_cols", [0, 1]), ("nu_xcd_order", [1, 0]),
                       ("nu_grid_pairs", [0, 1]), ("nu_snake", [0, 1]), ("nu_slice_group", [1, 0, 2]),
                       ("nu_degrid_merge", [0, 1])]:
+        if key == "nu_xcd_order":
+            # the region order of the staged gridding (under the XCD order)
+            # only moves work between XCDs: bitwise the same result
+            E.set_option("nu_xcd_order", 1)
+            E.set_option("nu_grid_region", 0)
+            ref = E.normal(xcat)
+            E.set_option("nu_grid_region", 1)
+            assert E.get_option("nu_grid_region") == 1
+            assert np.array_equal(E.normal(xcat), ref)
         for v in vals:
             E.set_option(key, v)
             assert E.get_option(key) == v
@@ -443,6 +452,7 @@ def test_nufft_tuning_options_and_autotune(mm):
     assert chosen["nu_degrid_block"] in (32, 64, 128)
     assert chosen["nu_pair_cols"] in (0, 1) and chosen["nu_xcd_order"] in (0, 1)
     assert chosen["nu_grid_pairs"] in (0, 1) and chosen["nu_snake"] in (0, 1)
+    assert chosen["nu_grid_region"] in (0, 1)
     assert chosen["nu_slice_group"] == 0   # three slices: one group
     assert rel(E.normal(xcat), want) < TOL
 

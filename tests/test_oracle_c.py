@@ -110,3 +110,26 @@ def test_highorder_cgne_matches_numpy_oracle(built):
     assert nit == 12
     assert np.allclose(rc, np.real(ro), rtol=1e-8, atol=0)
     assert np.linalg.norm(xc - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def test_highorder_round_io_matches_numpy_rounding(built):
+    """set_round_io: each application's input and output rounded to
+    complex64, as the NumPy oracle wrapped in complex64 casts."""
+    from oracle.highorder import HighOrderPlan
+    P, hc, m, rng = _ho_problem(seed=9)
+    N, K, C = P["N"], P["K"], P["C"]
+    So, Ro = HighOrderPlan(P["b0"], P["t"], P["h"], hc, P["grid"]), RcvrPlan(P["maps"])
+    E = cref.HighOrderSenseRef(P["b0"], P["t"], P["h"], hc, P["grid"], P["maps"])
+    E.set_round_io(True)
+    c64 = lambda a: np.asarray(a).astype(np.complex64).astype(np.complex128)
+    x = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    y = rng.standard_normal((K, C)) + 1j * rng.standard_normal((K, C))
+    yo = c64(mrSampFunc(c64(x), "notransp", So, Ro)).reshape(K, C, order="F")
+    xo = c64(mrSampFunc(c64(y), "transp", So, Ro)).reshape(N, N, order="F")
+    yc, xc = E.forward(x), E.adjoint(y)
+    # the same rounded values up to an ulp of complex64 where the FP64 sums differ in their last bits
+    assert np.linalg.norm(yc - yo) / np.linalg.norm(yo) < 1e-7
+    assert np.linalg.norm(xc - xo) / np.linalg.norm(xo) < 1e-7
+    assert np.array_equal(yc, c64(yc)) and np.array_equal(xc, c64(xc))
+    E.set_round_io(False)
+    assert not np.array_equal(E.forward(x), c64(E.forward(x)))

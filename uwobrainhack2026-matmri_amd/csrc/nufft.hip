@@ -2120,6 +2120,19 @@ void NufftOp::build_partitions() {
         gtile_count_[(size_t)s] = (int)gtile_host_.size() - gtile_first_[(size_t)s];
         gpcells_[(size_t)s] = npc;
         {
+            // regions: the rectangles in partition order (spatially coherent)
+            // cut into kXcds runs of equal staging cost
+            int64_t total = 0, acc = 0;
+            for (int t = gtile_first_[(size_t)s]; t < (int)gtile_host_.size(); ++t)
+                total += (int64_t)gtile_host_[(size_t)t].n_ent + 2 * (int64_t)gtile_host_[(size_t)t].n_stage + 1;
+            for (int t = gtile_first_[(size_t)s]; t < (int)gtile_host_.size(); ++t) {
+                GridTile& q = gtile_host_[(size_t)t];
+                const int64_t cost = (int64_t)q.n_ent + 2 * (int64_t)q.n_stage + 1;
+                q.region = (int)std::min<int64_t>(kXcds - 1, (int64_t)kXcds * (2 * acc + cost) / (2 * total));
+                acc += cost;
+            }
+        }
+        {
             // the rectangles (chunks counted once) tile the grid and hold
             // every entry once
             int64_t cells = 0, ents = 0;
@@ -2201,6 +2214,7 @@ bool NufftOp::set_option(const std::string& key, int value) {
     if (key == "nu_grid_stage") return flag(grid_stage_, false);
     if (key == "nu_xcd_order") return flag(xcd_order_, true);
     if (key == "nu_grid_pairs") return flag(grid_pairs_, true);
+    if (key == "nu_grid_region") return flag(grid_region_, true);
     if (key == "nu_toep_fused") return flag(toep_fused_ok_, false);
     if (key == "nu_snake") return flag(snake_, true);
     if (key == "nu_degrid_merge") {
@@ -2242,6 +2256,7 @@ bool NufftOp::get_option(const std::string& key, int* value) const {
     else if (key == "nu_grid_stage") *value = grid_stage_;
     else if (key == "nu_xcd_order") *value = xcd_order_;
     else if (key == "nu_grid_pairs") *value = grid_pairs_;
+    else if (key == "nu_grid_region") *value = grid_region_;
     else if (key == "nu_toep_fused") *value = toep_fused_ok_;
     else if (key == "nu_snake") *value = snake_;
     else if (key == "nu_degrid_merge") *value = degrid_merge_;
@@ -2294,7 +2309,10 @@ bool NufftOp::autotune_sense(const float2* maps, int C, const float* W, int iter
     tune({{{"nu_degrid_merge", 1}}, {{"nu_degrid_merge", 0}}}, {"nu_degrid"});
     tune({{{"nu_degrid_block", 32}}, {{"nu_degrid_block", 64}}, {{"nu_degrid_block", 128}}}, {"nu_degrid"});
     tune({{{"nu_pair_cols", 1}}, {{"nu_pair_cols", 0}}}, {"nu_cols"});
-    tune({{{"nu_xcd_order", 0}}, {{"nu_xcd_order", 1}}}, {"nu_grid", "nu_degrid"});
+    tune({{{"nu_xcd_order", 0}},
+          {{"nu_xcd_order", 1}, {"nu_grid_region", 0}},
+          {{"nu_xcd_order", 1}, {"nu_grid_region", 1}}},
+         {"nu_grid", "nu_degrid"});
     tune({{{"nu_snake", 1}}, {{"nu_snake", 0}}},
          {"nu_fwd_rows", "nu_cols", "nu_degrid", "nu_grid", "nu_adj_rows"});
     {
@@ -2343,6 +2361,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     std::vector<int4> wdb;
     std::vector<int64_t> wdb_cost;
     std::vector<int64_t> wga_cost, wcsr_cost, wgt_cost;
+    std::vector<int> wgt_region;
     struct GridUnit {
         int s = 0, lanes = 0;
         int64_t yt = 0, zg = 0, part = 0;
@@ -2563,6 +2582,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
                 }
                 wgt.push_back(it);
                 wgt_cost.push_back(((int64_t)T.n_ent + 2 * (int64_t)T.n_stage) * (g.second >= 0 ? 2 : 1));
+                wgt_region.push_back(T.region);
             }
         }
     }
@@ -2618,7 +2638,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         std::vector<size_t> pick;
         if (xcd_order_) {
             std::vector<std::vector<size_t>> q(kXcds);
-            for (size_t i : ord) q[(size_t)(key(items[i]) % kXcds)].push_back(i);
+            for (size_t i : ord) q[(size_t)(key(i) % kXcds)].push_back(i);
             std::vector<size_t> pos(kXcds, 0);
             for (bool any = true; any;) {
                 any = false;
@@ -2643,8 +2663,18 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     // which are still in the memory-side cache.
     const int sd = snake_ ? 1 : 0;
     auto slice4 = [](const int4& w) { return w.x; };
-    order(wdb, wdb_cost, [&](const int4& w) { return (int64_t)w.x * 131 + w.y; }, slice4, sd);
-    order(wgt, wgt_cost, [&](const GridItem& g) { return (int64_t)g.slice * 131 + (g.zg - c.hs[g.slice].ws_off) / std::max<int64_t>(1, c.hs[g.slice].ws_chunk); },
+    order(wdb, wdb_cost, [&](size_t i) { return (int64_t)wdb[i].x * 131 + wdb[i].y; }, slice4, sd);
+    // Staged gridding: with option "nu_grid_region" the items of one region
+    // of a trajectory's rectangles go to one XCD for every slice, so the
+    // region's shared tables (sample ids, row pointers, entries) and the
+    // samples its neighbouring rectangles both stage stay in that XCD's L2;
+    // otherwise one (slice, chunk)'s items go to one XCD.
+    order(wgt, wgt_cost,
+          [&](size_t i) {
+              const GridItem& g = wgt[i];
+              if (grid_region_) return (int64_t)wgt_region[i];
+              return (int64_t)g.slice * 131 + (g.zg - c.hs[g.slice].ws_off) / std::max<int64_t>(1, c.hs[g.slice].ws_chunk);
+          },
           [](const GridItem& g) { return g.slice; }, -sd);
     auto by_slice_desc = [&](auto& items, auto slice_of) {
         std::stable_sort(items.begin(), items.end(),

@@ -78,7 +78,7 @@ struct GridTile {
     int ent_off;         // first (local sample, weight) entry
     int pofs, pofs0;     // kind 2: this chunk's partial block, the rectangle's first
     int nparts;          // kind 2: chunks of the rectangle
-    int pad0;
+    int region;          // one of kXcds spatially contiguous groups of equal staging cost
 };
 // One work item of the staged gridding: a rectangle of one (slice, chunk)
 // with every offset its work-group needs, so that one scalar load precedes
@@ -263,6 +263,7 @@ class NufftOp : public mm_op {
     // trajectory, so its rectangle's CSR entries are read once for both
     // (option "nu_grid_pairs")
     bool grid_pairs_ = true;
+    bool grid_region_ = true;   // with xcd_order_: staged gridding items of a region on one XCD
     // alternating slice order of the normal operator's stages (option
     // "nu_snake"): each stage starts on the slices the previous one touched
     // last

@@ -104,7 +104,7 @@ class LinearOp:
         return v.value
 
     TUNED_OPTIONS = ("nu_grid_pairs", "nu_stage_samples", "nu_stage_entries", "nu_degrid_block", "nu_pair_cols",
-                     "nu_xcd_order", "nu_snake", "nu_slice_group", "nu_degrid_merge")
+                     "nu_xcd_order", "nu_grid_region", "nu_snake", "nu_slice_group", "nu_degrid_merge")
 
     def autotune(self, iters=10):
         """Run the device autotuner of the normal operator (mm_op_autotune)
