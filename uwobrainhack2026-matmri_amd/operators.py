@@ -301,7 +301,8 @@ class sampHighOrder(LinearOp):
     ``phs_conc`` ``[Nc, kshape]`` or ``None``; ``phs_grid`` a dict of
     ``x, y, z`` position arrays (m) with the image shape.  Approaches 0, 2
     and 3 give the same values and share one kernel; approach 1
-    (interpolation) is not implemented.
+    (interpolation: ``opt.svdThresh``, ``opt.subFact``) runs the truncated
+    SVD form over a device NUFFT (``mm_hio_interp_create``).
     """
 
     def __init__(self, b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask=None, useGPU=1,
