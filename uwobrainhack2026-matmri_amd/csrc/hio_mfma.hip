@@ -111,8 +111,11 @@ __global__ __launch_bounds__(512) void hio_mfma_gemm(const double* __restrict__ 
     constexpr int NB = NCT * 2 * 64;   // half8 operands per reduction block
     constexpr int NR = 16 * RP;        // reduction factors per block
     constexpr int NS = RP / 4;         // FP64 MFMA K-steps
-    __shared__ half8_t sB[2][NB];
-    __shared__ double sR[2][NR];
+    // padded to one entry per thread: every thread loads and stores each
+    // block (no load or store under a branch)
+    static_assert(NB <= 512 && NR <= 512, "operands per block exceed the work-group");
+    __shared__ half8_t sB[2][512];
+    __shared__ double sR[2][512];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, l16 = lane & 15;
     const int rowbase = blockIdx.x * (8 * RT * 16) + wave * RT * 16;
@@ -144,16 +147,20 @@ __global__ __launch_bounds__(512) void hio_mfma_gemm(const double* __restrict__ 
     double pfR = 0.0;
     const int tid = threadIdx.x;
     if (b0 < b1) {
-        if (tid < NB) sB[0][tid] = Bs[(size_t)b0 * NB + tid];
-        if (tid < NR) sR[0][tid] = redfac[(size_t)b0 * NR + tid];
+        sB[0][tid] = Bs[(size_t)b0 * NB + (tid % NB)];
+        sR[0][tid] = redfac[(size_t)b0 * NR + min(tid, NR - 1)];
     }
     __syncthreads();
     for (int b = b0; b < b1; ++b) {
         const int buf = (b - b0) & 1;
         const bool more = b + 1 < b1;
-        if (more) {
-            if (tid < NB) pfB = Bs[(size_t)(b + 1) * NB + tid];
-            if (tid < NR) pfR = redfac[(size_t)(b + 1) * NR + tid];
+        {
+            // unconditional loads from clamped, always-valid indices: a load
+            // under a branch is waited for at the join, which would stall every
+            // block on its own prefetch
+            const size_t bn = (size_t)(more ? b + 1 : b);
+            pfB = Bs[bn * NB + (tid % NB)];
+            pfR = redfac[bn * NR + min(tid, NR - 1)];
         }
         double ph[RT][4];
         if constexpr (MPHASE) {
@@ -222,10 +229,10 @@ __global__ __launch_bounds__(512) void hio_mfma_gemm(const double* __restrict__ 
                 accl[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[t], Bh, accl[t][c], 0, 0, 0);
             }
         }
-        if (more) {
-            if (tid < NB) sB[buf ^ 1][tid] = pfB;
-            if (tid < NR) sR[buf ^ 1][tid] = pfR;
-        }
+        // the other buffer was last read before the previous barrier; in the
+        // last iteration this stores a copy of the current block, never read
+        sB[buf ^ 1][tid] = pfB;
+        sR[buf ^ 1][tid] = pfR;
         __syncthreads();
     }
     // D layout of the 16x16 MFMA: lane holds rows 4*(lane/16)+i, column lane%16

@@ -24,7 +24,7 @@ struct HioMfmaCall {
     float* part;           // nsplit * nrows * 16*NCT floats
     unsigned* amax;        // one word
     int nsplit, bps;       // reduction splits and blocks per split
-    int variant;           // bit 0: phase on FP64 MFMA, bit 1: hardware sin/cos
+    int variant;           // bit 0: phase on FP64 MFMA, bit 1: hardware sin/cos, bit 2: software-pipelined form
 };
 
 void hio_mfma_apply(hipStream_t st, const HioMfmaCall& c, int prof_iters, double* prof_ms);
