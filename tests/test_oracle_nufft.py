@@ -248,3 +248,88 @@ def test_single_precision_evaluation():
     R.single = True
     assert 1e-9 < np.linalg.norm(R.forward(xr) - r64) / np.linalg.norm(r64) < 1e-6
     assert 1e-9 < np.linalg.norm(R.adjoint(r64) - b64) / np.linalg.norm(b64) < 1e-6
+
+
+# -- odd image sides (nufftOp.m:171-175, :258-273) -------------------------------
+def _odd_closed_form(S, kloc, x):
+    """What fftShiftArray makes of an odd side N = 2s + 1: the centred NUDFT
+    (positions i - N/2, half-integers) times exp(i pi sum_d N_d / 2), and,
+    because the ramp exp(-i pi g N / G) over g = 0..G-1 is anti-periodic
+    (g -> g + G flips its sign), negated once per odd side along which the
+    sample's taps all lie at wrapped (negative-frequency) grid indices.
+    Samples whose taps straddle g = 0 mix both signs and are left out.
+    Returns (closed form, samples kept)."""
+    imN = [int(n) for n in S.imN]
+    W = S.kwidth
+    yr = nudft_centred(imN, kloc, x).reshape(-1) * np.exp(1j * np.pi * sum(imN) / 2)
+    sgn = np.ones(len(kloc))
+    keep = np.ones(len(kloc), bool)
+    for d, n in enumerate(imN):
+        if n % 2:
+            f = np.floor(kloc[:, d] * S.osN[d])
+            pos, neg = f - W / 2 + 1 >= 0, f + W / 2 <= -1
+            keep &= pos | neg
+            sgn *= np.where(neg, -1.0, 1.0)
+    return sgn * yr, keep
+
+
+@pytest.mark.parametrize("imN,W", [([15, 20], 4), ([16, 21], 6), ([15, 17], 4), ([13], 6), ([21, 14], 4)])
+def test_odd_size_closed_form(imN, W):
+    """For odd sides the reference's forward is not a NUDFT (its grid carries
+    an anti-periodic ramp), but it is this closed form up to the gridding
+    error: a constant i per odd side, a sign per wrapped side."""
+    rng = np.random.default_rng(60 + sum(imN))
+    nd = len(imN)
+    kloc = rng.random((1500, nd)) - 0.5
+    x = _rand_c(rng, imN)
+    S = NufftPlan(imN, kloc, None, 1.5, W)
+    want, keep = _odd_closed_form(S, kloc, x)
+    y = S.forward(x).reshape(-1)
+    assert keep.mean() > 0.6
+    assert np.linalg.norm((y - want)[keep]) / np.linalg.norm(want[keep]) < 2e-2
+    # the adjoint is the exact transpose (nufftOp.m:354-377)
+    yy = _rand_c(rng, (len(kloc),))
+    d1 = np.vdot(x.reshape(-1, order="F"), S.adjoint(yy).reshape(-1, order="F"))
+    d2 = np.vdot(y, yy)
+    assert abs(d1 - d2) < 1e-10 * abs(d1)
+
+
+def test_odd_size_closed_form_rejects_mistakes():
+    """Plausible slips in the odd-side ramp (the even-size formula with
+    floor(N/2), a conjugated ramp, no constant i) miss the closed form by
+    O(1), so the test above pins the ramp."""
+    rng = np.random.default_rng(66)
+    imN, W = [15, 20], 4
+    kloc = rng.random((1500, 2)) - 0.5
+    x = _rand_c(rng, imN)
+    S = NufftPlan(imN, kloc, None, 1.5, W)
+    want, keep = _odd_closed_form(S, kloc, x)
+    good = S.fftShiftArray.copy()
+    g = [np.arange(int(n)) for n in S.osN]
+    ramp_floor = np.exp(1j * np.pi * (-(g[0][:, None] - S.osN[0] / 2) * (imN[0] // 2) / S.osN[0]
+                                      - (g[1][None, :] - S.osN[1] / 2) * imN[1] / S.osN[1]))
+    for bad in (ramp_floor, np.conj(good), good * -1j):
+        S.fftShiftArray = bad
+        y = S.forward(x).reshape(-1)
+        assert np.linalg.norm((y - want)[keep]) / np.linalg.norm(want[keep]) > 0.3
+    S.fftShiftArray = good
+    y = S.forward(x).reshape(-1)
+    assert np.linalg.norm((y - want)[keep]) / np.linalg.norm(want[keep]) < 2e-2
+
+
+@pytest.mark.parametrize("imN", [[11, 13], [9], [15, 10], [12, 10]])
+def test_toeplitz_odd_sizes_vs_exact_nudft_normal(imN):
+    """prepToep on odd sides (nufftOp.m:410-432).  The point-spread image comes
+    from the adjoint NUFFT on the doubled grid 2N, whose ramp constant
+    conj(exp(i pi sum_d 2 N_d / 2)) = (-1)^(sum_d N_d) survives into the
+    kernel, so the Toeplitz form is (-1)^(sum N_d) E' diag(dcf^2) E of the
+    exact NUDFT (an odd total side length negates it)."""
+    rng = np.random.default_rng(45 + sum(imN))
+    kloc = rng.random((90, len(imN))) - 0.5
+    dcf = 0.5 + rng.random(90)
+    x = rng.standard_normal(tuple(imN)) + 1j * rng.standard_normal(tuple(imN))
+    E = nudft_centred(imN, kloc, np.eye(int(np.prod(imN))))
+    want = (E.conj().T @ (dcf ** 2 * (E @ x.reshape(-1, order="F")))).reshape(imN, order="F")
+    want = want * (-1.0) ** sum(imN)
+    got = ToepPlan(imN, kloc, dcf, 1.5, 6, tos=2, tkwidth=8).apply(x)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-4

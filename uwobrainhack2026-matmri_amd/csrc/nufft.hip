@@ -8,9 +8,12 @@
 // grid index per dimension and its W weights per dimension.
 //
 // Forward (nufftOp.m:378-394): y = dcf .* C * (P .* ifftnc(comp .* x, pad G)).
-// The k-space phase ramp P = fftShiftArray (:258-273) equals an image-domain
-// circular shift by N/2 times the constant (-1)^(N1/2+N2/2), so the image is
-// written wrapped into the zero-padded FFT input and no ramp is multiplied.
+// For even sides the k-space phase ramp P = fftShiftArray (:258-273) equals
+// an image-domain circular shift by N/2 times the constant
+// (-1)^(N1/2+N2/2), so the image is written wrapped into the zero-padded FFT
+// input and no ramp is multiplied.  An odd side N = 2s + 1 is written
+// wrapped by s and the rest of its ramp, i exp(-i pi g / G), is multiplied
+// into the grid (nu_odd_ramp).
 // Adjoint (:354-377) is the exact Hermitian transpose: gridding, FFT, crop,
 // deapodisation.
 //
@@ -30,6 +33,8 @@
 //                 heavy tiles split and merged by the last-arriving
 //                 work-group
 //   nu_adj_rows   FFT along g1 + crop + deapod + conj-coil combine
+//   nu_odd_ramp   odd image sides only: the part of the ramp P that is not a
+//                 circular shift, multiplied into the grid
 #include <algorithm>
 #include <cstring>
 #include <cmath>
@@ -738,6 +743,38 @@ __global__ __launch_bounds__(256) void nu_toep_mul(const NuSliceDev* __restrict_
     for (int u = 0; u < 4; ++u)
         if (e0 + u * 256 < n)
             z[e0 + u * 256] = make_float2(v[u].x * k[u].x - v[u].y * k[u].y, v[u].x * k[u].y + v[u].y * k[u].x);
+}
+
+// Odd image sides (fftShiftArray, nufftOp.m:258-273).  For N = 2s + 1 the
+// ramp exp(-i pi (g - G/2) N / G), g = 0..G-1, is the circular shift by s
+// (the wrapped placement of the row and column passes), the constant (-1)^s
+// (folded into the deapodisation) and the factor i exp(-i pi g / G), which
+// this kernel multiplies into the grid of every chunk (CONJ: the conjugate,
+// for the adjoint).  That factor is not G-periodic: taps that wrap past
+// g = 0 see it with the other sign, exactly as the reference's grid does.
+// Grid [g2][g1][b] of the slice's chunk blockIdx.z; even slices return.
+template <int BCT, int CONJ>
+__global__ __launch_bounds__(256) void nu_odd_ramp(const NuSliceDev* __restrict__ sl, float2* __restrict__ ws,
+                                                   const int* __restrict__ skip) {
+    const int s = blockIdx.y;
+    if (skip && skip[s]) return;
+    const NuSliceDev& S = sl[s];
+    const int odd = S.odd;
+    if (!odd) return;
+    const int n = (int)S.ws_chunk;
+    const int e = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (e >= n) return;
+    float2* z = ws + S.ws_off + (int64_t)blockIdx.z * S.ws_chunk;
+    const int cell = e / BCT;
+    const int g1 = cell % S.G1, g2 = cell / S.G1;
+    // angle in half-turns: i per odd side, minus g / G per odd side
+    float a = 0.f;
+    if (odd & 1) a += 0.5f - (float)g1 / (float)S.G1;
+    if (odd & 2) a += 0.5f - (float)g2 / (float)S.G2;
+    float sn, cs;
+    sincospif(CONJ ? -a : a, &sn, &cs);
+    const float2 v = z[e];
+    z[e] = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
 }
 
 // Degridding from an LDS-staged grid region (2-D): work item = (slice,
@@ -1738,8 +1775,8 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         NuSliceDev d{};
         const int64_t n1 = imN[2 * s], n2 = ndim == 2 ? imN[2 * s + 1] : 1;
         if (n1 < 2 || n2 < 1 || (ndim == 2 && n2 < 2)) fail(MM_ERR_ARG, "nufftOp: image size too small");
-        if ((n1 & 1) || (ndim == 2 && (n2 & 1)))
-            fail(MM_ERR_UNSUPPORTED, "nufftOp: odd image sizes are not implemented on gfx950");
+        d.odd = (int)(n1 & 1) | (ndim == 2 && (n2 & 1) ? 2 : 0);
+        if (d.odd) any_odd_ = true;
         if (M[s] < 1) fail(MM_ERR_ARG, "nufftOp: no samples");
         if (!kloc || !kloc[s]) fail(MM_ERR_ARG, "nufftOp: kloc is NULL");
         d.N1 = (int)n1;
@@ -1759,7 +1796,9 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         d.tw2 = f2.second;
         d.fac2w = f2w.first;
         d.tw2w = f2w.second;
-        // deapodisation with 1/sqrt(G1*G2) and the ramp constant folded in
+        // deapodisation with 1/sqrt(G1*G2) and the real part (-1)^(s1+s2),
+        // s = floor(N/2), of the ramp constant exp(i pi (N1+N2)/2) folded in
+        // (an odd side's factor i goes with its ramp, nu_odd_ramp)
         const double sgn = (((n1 / 2) + (ndim == 2 ? n2 / 2 : 0)) & 1) ? -1.0 : 1.0;
         const double scl = sgn / std::sqrt((double)d.G1 * d.G2);
         auto c1 = deapod((int)n1, d.G1, kwidth, os);
@@ -2824,6 +2863,21 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         if (g < 0) return {0, n_all};
         return {c.goff[(size_t)g][(size_t)L], c.gcnt[(size_t)g][(size_t)L]};
     };
+    // the odd-size ramp over the grids of every slice and chunk (slices
+    // with even sides return at once)
+    auto odd_ramp = [&](bool conj, const int* skip) {
+        if (!any_odd_) return;
+        int64_t mx = 1;
+        for (const NuSliceDev& d : c.hs) mx = std::max(mx, d.ws_chunk);
+        const int nchunk = (int)(c.sense ? c.nrep : (c.nrep + BCT - 1) / BCT);
+        const dim3 grid((unsigned)((mx + 255) / 256), (unsigned)c.hs.size(), (unsigned)nchunk);
+        if (conj)
+            hipLaunchKernelGGL((nu_odd_ramp<BCT, 1>), grid, dim3(256), 0, st, sl, c.ws.p, skip);
+        else
+            hipLaunchKernelGGL((nu_odd_ramp<BCT, 0>), grid, dim3(256), 0, st, sl, c.ws.p, skip);
+        LAUNCHCHECK();
+        mark("nu_odd_ramp");
+    };
     auto forward = [&](const float2* x, float2* y, NuRun r, int g) {
         const auto fr = span(kLFrows, g, c.n_frows), fc = span(kLFcols, g, c.n_cols), db = span(kLDb, g, c.n_db);
         if (fr.second > 0)
@@ -2837,6 +2891,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
             LAUNCHCHECK();
             mark("nu_cols");
         }
+        odd_ramp(false, r.skip);
         if (ndim_ == 2 && degrid_stage_ && db.second > 0) {
             if (wt4)
                 hipLaunchKernelGGL((nu_degrid_tile<BCT, 4>), dim3(db.second), dim3(256), c.lds_db, st, sl, c.w_db.p + db.first,
@@ -2897,6 +2952,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                c.parts.p, c.counters.p, r);
         LAUNCHCHECK();
         mark("nu_grid");
+        odd_ramp(true, r.skip);
         if (ndim_ == 2 && cc.second > 0) {
             hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(cc.second), dim3(256), c.lds_cols, st, sl,
                                c.w_cols.p + cc.first, facs_.p, twid_.p, c.ws.p, r.skip);

@@ -23,6 +23,7 @@ struct NuSliceDev {
     int fip;            // bit 0: row FFTs, bit 1: column FFTs in a single LDS buffer
     int fac2w, tw2w;    // the column length's radix-16/12 factorization (passes of
                         // 16 transforms): used for fac2/tw2 when the column pass has them
+    int odd;            // bit 0: N1 odd, bit 1: N2 odd (the grid takes the odd-size ramp)
     int64_t samp_off;   // first sample of this slice in the sample pools
     int64_t tlist_off;  // first entry of this slice in the tile sample list
     int64_t img_off;    // image-side offset (elements) for the current apply
@@ -204,6 +205,7 @@ class NufftOp : public mm_op {
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
     bool big_ = false;
     bool toep_ = false;
+    bool any_odd_ = false;   // some slice has an odd image side: the ramp stage runs
 
    public:
     bool is_toeplitz() const { return toep_; }
