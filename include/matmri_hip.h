@@ -80,7 +80,10 @@ int mm_ctx_sync(mm_ctx* ctx);
  * cycles/pixel in [-0.5, 0.5) (wrapped), dcf[s] = [M] or NULL.  `os` and
  * `kwidth` follow nufftOp (kwidth even, <= 8).  Data of all slices is
  * concatenated: images [V_s x B] then samples [M_s x B] per slice, where B is
- * the number of transforms per slice passed to mm_op_apply as `nrep`. */
+ * the number of transforms per slice passed to mm_op_apply as `nrep`.
+ * ndim = 3 (one volume, nslice = 1): imN[0..2] = N1, N2, N3, kloc[0] =
+ * column-major [M x 3], grid sides up to 512 (nufftOp.m:195-197, :244-245,
+ * :268-271).  Odd sides follow the reference's ramp exactly. */
 int mm_nufft_plan_create(mm_ctx* ctx, int ndim, int nslice, const int64_t* imN,
                          const int64_t* M, const double* const* kloc,
                          const double* const* dcf, double os, int kwidth,

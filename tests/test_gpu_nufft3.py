@@ -1,0 +1,124 @@
+"""GPU parity of the 3-D NUFFT (nufftOp with a 3-D imN) against the FP64
+oracle: forward and adjoint at the per-application bound 1e-5 (even and odd
+sides, W 4/6/8, os 1.5/2, dcf, 1 to 20 transforms: partly filled chunks),
+the reference's on-grid known answer in 3-D (@nufftOp/tests.m:90-101), and a
+3-D SENSE cgne against the FP64 loop.  The oracle's 3-D operator is pinned by
+the brute-force NUDFT (tests/test_oracle_nufft.py::test_vs_nudft) and the
+odd-side closed form.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_pkg
+from oracle.fftnc import fftnc
+from oracle.mrsamp import mrSampFunc as o_mrSampFunc
+from oracle.nufft import NufftPlan
+from oracle.rcvr import RcvrPlan
+from oracle.solvers import cgne as o_cgne
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a).reshape(-1, order="F")
+    b = np.asarray(b).reshape(-1, order="F")
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def crand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.fixture(scope="module")
+def mm():
+    m = load_pkg()
+    if m.device_count() < 1:
+        pytest.fail("no GPU visible to libmatmri_hip")
+    return m
+
+
+CASES_3D = [
+    # imN, M, kwidth, os, dcf, nrep
+    ((12, 16, 20), 2000, 6, 1.5, False, 1),
+    ((16, 16, 16), 3000, 4, 1.5, True, 3),
+    ((15, 17, 14), 2500, 4, 1.5, False, 2),
+    ((24, 20, 18), 4000, 8, 2.0, True, 1),
+    ((20, 20, 20), 3000, 4, 1.5, False, 16),
+    ((20, 18, 9), 2000, 6, 1.375, True, 20),
+]
+
+
+@pytest.mark.parametrize("imN,M,W,os_,use_dcf,nrep", CASES_3D)
+def test_nufft3_vs_oracle(mm, imN, M, W, os_, use_dcf, nrep):
+    print(f"[case] 3-D nufft imN={imN} M={M} W={W} os={os_} dcf={use_dcf} nrep={nrep}", flush=True)
+    rng = np.random.default_rng(90 + sum(imN) + nrep)
+    kloc = rng.random((M, 3)) - 0.5
+    dcf = rng.random(M) + 0.5 if use_dcf else None
+    So = NufftPlan(list(imN), kloc, dcf, os_, W)
+    Sg = mm.nufftOp(list(imN), kloc, dcf, 1, os_, W)
+    x = crand(rng, imN + (nrep,))
+    y = crand(rng, (M, nrep))
+    e_f = rel(Sg * x, So.forward(x))
+    e_a = rel(Sg.H * y, So.adjoint(y))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+
+
+def test_nufft3_on_grid_known_answer(mm):
+    """@nufftOp/tests.m:90-101 in 3-D: on-grid kloc makes the forward a
+    unitary centred inverse DFT, fftnc(S*im) == im (1e-4, L2 / numel)."""
+    print("[case] 3-D nufft on-grid KAT 12x16x20", flush=True)
+    imN = [12, 16, 20]
+    axes = [(np.arange(n) - n / 2) / n for n in imN]
+    g = np.meshgrid(*axes, indexing="ij")
+    kloc = np.stack([a.reshape(-1, order="F") for a in g], axis=1)
+    Sg = mm.nufftOp(imN, kloc, None, 1, 1.5, 6)
+    im = np.random.default_rng(4).random(imN)
+    k = (Sg * im).reshape(imN, order="F")
+    err = np.sqrt(np.sum(np.abs(fftnc(k, 3) - im) ** 2)) / im.size
+    print(f"  KAT err {err:.2e}", flush=True)
+    assert err < 1e-4
+
+
+def test_nufft3_sense_cgne_matches_oracle(mm):
+    """A 3-D SENSE operator (rcvrOp over the 3-D NUFFT, mrSampFunc.m:97-107,
+    :126-144) and cgne (cgne.m:213-336) for 8 iterations against the FP64
+    loop: residual history within 1e-3, iterate NRMSE < 1e-4."""
+    print("[case] 3-D sense cgne 16x16x12 C=4", flush=True)
+    rng = np.random.default_rng(95)
+    imN, M, C = (16, 16, 12), 2500, 4
+    kl = rng.random((M, 3)) - 0.5
+    maps = crand(rng, imN + (C,))
+    So, Ro = NufftPlan(list(imN), kl, None, 1.5, 4), RcvrPlan(maps)
+    Sg, Rg = mm.nufftOp(list(imN), kl, None, 1, 1.5, 4), mm.rcvrOp(maps)
+    E = mm.senseOp(Sg, Rg)
+    x = crand(rng, imN)
+    e_f = rel(E * x, o_mrSampFunc(x, "notransp", So, Ro))
+    A = lambda v, tr: o_mrSampFunc(v, tr, So, Ro)
+    y = A(crand(rng, imN), "notransp")
+    e_a = rel(E.H * y, A(y, "transp"))
+    print(f"  sense fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+    opt = {"resThresh": None, "xChangeThresh": None}
+    x0 = np.zeros(imN, complex)
+    nit = 8
+    xo, ro, *_ = o_cgne(A, y, x0, nit, opt)
+    xg, rg, *_ = mm.cgne(E, y, x0, nit, opt)
+    ro = np.real(ro)
+    d = np.abs(rg - ro) / ro
+    nrmse = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    print(f"  resSq rel diff max {d.max():.2e}; x nrmse {nrmse:.2e}", flush=True)
+    assert np.all(d < 1e-3)
+    assert nrmse < 1e-4
+
+
+def test_nufft3_rejects_unsupported(mm):
+    """Grid sides above 512 and multi-volume 3-D plans raise, not fault."""
+    kl = np.zeros((4, 3))
+    with pytest.raises(NotImplementedError):
+        mm.nufftOp([400, 8, 8], kl, None, 1, 1.5, 4)
+    with pytest.raises(NotImplementedError):
+        mm.nufftOp.batch([([8, 8, 8], kl, None), ([8, 8, 8], kl, None)], 1.5, 4)
+    with pytest.raises(NotImplementedError):
+        mm.nufftOp([8, 8, 8], kl, None, 1, 1.5, 4).prepToep()

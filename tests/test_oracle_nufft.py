@@ -135,11 +135,11 @@ def test_kat_odd_half_size_sign():
 
 
 # -- brute-force NUDFT ("+" sign, centred, unitary) bounds the approximation -----
-@pytest.mark.parametrize("W,bound", [(4, 5e-3), (6, 2e-4)])
-def test_vs_nudft(W, bound):
+@pytest.mark.parametrize("imN,W,bound", [([16, 20], 4, 5e-3), ([16, 20], 6, 2e-4), ([8, 10, 6], 4, 5e-3),
+                                         ([8, 10, 6], 6, 2e-4)])
+def test_vs_nudft(imN, W, bound):
     rng = np.random.default_rng(11)
-    imN = [16, 20]
-    kloc = rng.random((300, 2)) - 0.5
+    kloc = rng.random((300, len(imN))) - 0.5
     x = _rand_c(rng, imN)
     S = NufftPlan(imN, kloc, None, 1.5, W)
     y = S.forward(x).reshape(-1)
@@ -273,7 +273,8 @@ def _odd_closed_form(S, kloc, x):
     return sgn * yr, keep
 
 
-@pytest.mark.parametrize("imN,W", [([15, 20], 4), ([16, 21], 6), ([15, 17], 4), ([13], 6), ([21, 14], 4)])
+@pytest.mark.parametrize("imN,W", [([15, 20], 4), ([16, 21], 6), ([15, 17], 4), ([13], 6), ([21, 14], 4),
+                                   ([15, 16, 13], 4)])
 def test_odd_size_closed_form(imN, W):
     """For odd sides the reference's forward is not a NUDFT (its grid carries
     an anti-periodic ramp), but it is this closed form up to the gridding
@@ -285,7 +286,7 @@ def test_odd_size_closed_form(imN, W):
     S = NufftPlan(imN, kloc, None, 1.5, W)
     want, keep = _odd_closed_form(S, kloc, x)
     y = S.forward(x).reshape(-1)
-    assert keep.mean() > 0.6
+    assert keep.mean() > 0.4
     assert np.linalg.norm((y - want)[keep]) / np.linalg.norm(want[keep]) < 2e-2
     # the adjoint is the exact transpose (nufftOp.m:354-377)
     yy = _rand_c(rng, (len(kloc),))

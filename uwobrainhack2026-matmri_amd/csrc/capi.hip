@@ -8,6 +8,7 @@
 #include "cg.hpp"
 #include "hio.hpp"
 #include "hio_interp.hpp"
+#include "nufft3.hpp"
 #include "ops.hpp"
 #include "traj.hpp"
 #include "wsnet.hpp"
@@ -113,6 +114,11 @@ int mm_nufft_plan_create(mm_ctx* ctx, int ndim, int nslice, const int64_t* imN, 
     return guarded([&] {
         if (!ctx || !out || !imN || !M || !kloc) fail(MM_ERR_ARG, "nufftOp: NULL argument");
         set_device(ctx);
+        if (ndim == 3) {
+            if (nslice != 1) fail(MM_ERR_UNSUPPORTED, "nufftOp: 3-D plans hold one volume");
+            *out = new Nufft3Op(ctx, imN, M[0], kloc[0], dcf ? dcf[0] : nullptr, os, kwidth);
+            return;
+        }
         *out = new NufftOp(ctx, ndim, nslice, imN, M, kloc, dcf, os, kwidth);
     });
 }

@@ -220,7 +220,7 @@ static double bessel_i0(double x) {
 }
 
 // KB LUT with the zero sentinel, normalised as nufftOp.m:163-164.
-static std::vector<double> kb_lut(int kwidth, double os) {
+std::vector<double> nufft_kb_lut(int kwidth, double os) {
     const double bet = nufft_kb_beta(kwidth, os);
     const int nk = nufft_kb_nkern(kwidth, os);
     std::vector<double> k(nk + 1);
@@ -239,7 +239,7 @@ static std::vector<double> kb_lut(int kwidth, double os) {
 
 // Deapodisation comp{d} of nufftOp.m:171-182 (real; the reference's
 // x./sin(x) with imaginary x equals a/sinh(a), a = sqrt(beta^2 - x^2)).
-static std::vector<double> deapod(int N, int G, int kwidth, double os) {
+std::vector<double> nufft_deapod(int N, int G, int kwidth, double os) {
     const double bet = nufft_kb_beta(kwidth, os);
     std::vector<double> c(N);
     double cmin = 0;
@@ -1745,7 +1745,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     os_ = os;
     beta = nufft_kb_beta(kwidth, os);
     nkern = nufft_kb_nkern(kwidth, os);
-    const std::vector<double> lut = kb_lut(kwidth, os);
+    const std::vector<double> lut = nufft_kb_lut(kwidth, os);
     const int Nk = nkern;
 
     std::vector<FftFactors> facs;
@@ -1801,12 +1801,12 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         // (an odd side's factor i goes with its ramp, nu_odd_ramp)
         const double sgn = (((n1 / 2) + (ndim == 2 ? n2 / 2 : 0)) & 1) ? -1.0 : 1.0;
         const double scl = sgn / std::sqrt((double)d.G1 * d.G2);
-        auto c1 = deapod((int)n1, d.G1, kwidth, os);
+        auto c1 = nufft_deapod((int)n1, d.G1, kwidth, os);
         d.deap1 = (int)deap.size();
         for (double v : c1) deap.push_back((float)(v * scl));
         d.deap2 = (int)deap.size();
         if (ndim == 2) {
-            auto c2 = deapod((int)n2, d.G2, kwidth, os);
+            auto c2 = nufft_deapod((int)n2, d.G2, kwidth, os);
             for (double v : c2) deap.push_back((float)v);
         } else {
             deap.push_back(1.f);

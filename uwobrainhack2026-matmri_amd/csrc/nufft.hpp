@@ -313,5 +313,9 @@ class NufftOp : public mm_op {
 int nufft_grid_size(int64_t n, double os);
 double nufft_kb_beta(int kwidth, double os);
 int nufft_kb_nkern(int kwidth, double os);
+// KB lookup table with its zero sentinel (nufftOp.m:157-165), nkern + 1 values
+std::vector<double> nufft_kb_lut(int kwidth, double os);
+// deapodisation comp{d} of one side (nufftOp.m:171-182), N values
+std::vector<double> nufft_deapod(int N, int G, int kwidth, double os);
 
 }  // namespace mm

@@ -1,0 +1,527 @@
+// 3-D Kaiser-Bessel gridding NUFFT for gfx950 (nufftOp with a 3-D imN).
+//
+// Plan (host, FP64, once): the closed forms of nufftOp.doPrep
+// (operators/@nufftOp/nufftOp.m:138-274) for three sides -- grid sizes,
+// KB LUT, deapodisation per side, W weights per side and sample by linear
+// LUT interpolation (:216-227), wrapped first grid index per side
+// (:230-246) -- and the transposed interpolation matrix as CSR rows per grid
+// cell, (sample, w1 w2 w3 dcf) in sample order.
+//
+// Forward (:378-394), per chunk of BCT transforms:
+//   n3_rows_fwd   coil image times deapodisation, wrapped into the
+//                 zero-padded row, FFT along g1 (rows i2 < N2 of planes i3 < N3)
+//   n3_axis<0>    FFT along g2 (input rows wrapped by floor(N2/2)), then
+//                 along g3 (input planes wrapped by floor(N3/2)), in place
+//   n3_odd_ramp   odd sides only: the ramp factor that is not a shift
+//   n3_degrid     W^3 taps per (sample, transform), dcf; transposed through
+//                 LDS so the data side is written sample-fastest
+// Adjoint (:354-377): a transpose to [M][BCT] (n3_transpose_y), a
+// deterministic gather per grid cell over its CSR row with the dcf in the
+// row weights (n3_grid_csr), the conjugate ramp, inverse FFTs along g3 and
+// g2 that keep only the N3 / N2 outputs the crop reads, and the row FFT with
+// crop and deapodisation (n3_rows_adj).
+//
+// Grid layout in HBM: Z[g3][g2][g1][b], b = transform within the chunk, so a
+// tap and every FFT position of a pencil group touch BCT (or T) consecutive
+// complex values.  The ramp fftShiftArray (:258-273) of an even side is the
+// circular shift by N/2 times (-1)^(N/2) (the wrapped placement and a sign
+// in the deapodisation); an odd side N = 2s + 1 is placed wrapped by s and
+// the grid takes i exp(-i pi g / G) per odd side (n3_odd_ramp).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "nufft.hpp"
+#include "nufft3.hpp"
+
+namespace mm {
+
+// transforms per work-group of the FFT passes: a chunk's BCT transforms of
+// T / BCT pencils (16 for 16-transform chunks, else 8)
+template <int BCT>
+__host__ __device__ constexpr int n3_T() {
+    return BCT >= 16 ? 16 : 8;
+}
+
+__device__ __forceinline__ int wrap_pos(int i, int s, int n) {
+    const int p = i - s;
+    return p < 0 ? p + n : p;
+}
+
+// LDS of a pass over sides of length n: two buffers of n*T values and the
+// n - 1 pass twiddles
+template <int BCT>
+static size_t n3_lds(int n) {
+    return (size_t)(2 * n * n3_T<BCT>() + n) * sizeof(float2);
+}
+
+// Rows, forward: lines L = i2 + N2 i3 (i2 < N2, i3 < N3), T / BCT of them per
+// work-group.  Image reads run along i1 (coalesced), grid writes along
+// (g1, b).
+template <int BCT>
+__global__ __launch_bounds__(256) void n3_rows_fwd(Nu3Geom g, const FftFactors* __restrict__ fac,
+                                                   const float2* __restrict__ twp,
+                                                   const float* __restrict__ deap,
+                                                   const float2* __restrict__ x, float2* __restrict__ ws,
+                                                   int ch, int lanes) {
+    constexpr int T = n3_T<BCT>(), LPG = T / BCT;
+    extern __shared__ float2 lds[];
+    const FftFactors& F = *fac;
+    const int n = g.G1;
+    float2* A = lds;
+    float2* B = lds + n * T;
+    float2* tw = lds + 2 * n * T;
+    const int L0 = (int)blockIdx.x * LPG;
+    const int nl = g.N2 * g.N3;
+    for (int i = threadIdx.x; i < n * T; i += kFftThreads) A[i] = make_float2(0.f, 0.f);
+    for (int i = threadIdx.x; i < n - 1; i += kFftThreads) tw[i] = twp[i];
+    __syncthreads();
+    const float* d1 = deap;
+    const float* d2 = deap + g.N1;
+    const float* d3 = d2 + g.N2;
+    for (int idx = threadIdx.x; idx < g.N1 * T; idx += kFftThreads) {
+        const int i1 = idx % g.N1, t = idx / g.N1;
+        const int b = t % BCT, L = L0 + t / BCT;
+        if (L >= nl || b >= lanes) continue;
+        const int i2 = L % g.N2, i3 = L / g.N2;
+        const float2 v = x[(int64_t)(ch * BCT + b) * g.V + (int64_t)L * g.N1 + i1];
+        const float f = d1[i1] * d2[i2] * d3[i3];
+        A[wrap_pos(i1, g.s1, n) * T + t] = make_float2(v.x * f, v.y * f);
+    }
+    __syncthreads();
+    const float2* R = fft_lds<T, 1>(A, B, F, tw);
+    for (int idx = threadIdx.x; idx < n * T; idx += kFftThreads) {
+        const int b = idx % BCT, q = (idx / BCT) % n, j = idx / (BCT * n);
+        const int L = L0 + j;
+        if (L >= nl) continue;
+        const int i2 = L % g.N2, i3 = L / g.N2;
+        ws[(((int64_t)i3 * g.G2 + i2) * g.G1 + q) * BCT + b] = R[q * T + j * BCT + b];
+    }
+}
+
+// FFT along g2 (ax = 2: pencils (g1, i3), i3 < N3) or g3 (ax = 3: pencils
+// (g1, g2)), in place, T / BCT consecutive g1 per work-group so every
+// position is T consecutive values.  Forward: inputs q < N_ax placed at
+// (q - s) mod G, all G outputs.  Adjoint: all G inputs, outputs q < N_ax
+// read from (q - s) mod G.  A work-group reads its pencils whole (into LDS)
+// before writing any of them.
+template <int BCT, int ADJ>
+__global__ __launch_bounds__(256) void n3_axis(Nu3Geom g, int ax, const FftFactors* __restrict__ fac,
+                                               const float2* __restrict__ twp, float2* __restrict__ ws) {
+    constexpr int T = n3_T<BCT>(), LPG = T / BCT, SGN = ADJ ? -1 : 1;
+    extern __shared__ float2 lds[];
+    const FftFactors& F = *fac;
+    const int n = ax == 2 ? g.G2 : g.G3;
+    const int na = ax == 2 ? g.N2 : g.N3;
+    const int s = ax == 2 ? g.s2 : g.s3;
+    const int gx = (g.G1 + LPG - 1) / LPG;
+    const int gb = (int)blockIdx.x % gx, outer = (int)blockIdx.x / gx;
+    const int g1_0 = gb * LPG;
+    const int nt = min(T, (g.G1 - g1_0) * BCT);
+    int64_t base, qs;
+    if (ax == 2) {
+        base = ((int64_t)outer * g.G2 * g.G1 + g1_0) * BCT;
+        qs = (int64_t)g.G1 * BCT;
+    } else {
+        base = ((int64_t)outer * g.G1 + g1_0) * BCT;
+        qs = (int64_t)g.G2 * g.G1 * BCT;
+    }
+    float2* A = lds;
+    float2* B = lds + n * T;
+    float2* tw = lds + 2 * n * T;
+    for (int i = threadIdx.x; i < n - 1; i += kFftThreads) tw[i] = twp[i];
+    if (!ADJ) {
+        for (int i = threadIdx.x; i < n * T; i += kFftThreads) A[i] = make_float2(0.f, 0.f);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < na * T; idx += kFftThreads) {
+            const int t = idx % T, q = idx / T;
+            if (t < nt) A[wrap_pos(q, s, n) * T + t] = ws[base + q * qs + t];
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < n * T; idx += kFftThreads) {
+            const int t = idx % T, q = idx / T;
+            A[idx] = t < nt ? ws[base + q * qs + t] : make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    const float2* R = fft_lds<T, SGN>(A, B, F, tw);
+    if (!ADJ) {
+        for (int idx = threadIdx.x; idx < n * T; idx += kFftThreads) {
+            const int t = idx % T, q = idx / T;
+            if (t < nt) ws[base + q * qs + t] = R[idx];
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < na * T; idx += kFftThreads) {
+            const int t = idx % T, q = idx / T;
+            if (t < nt) ws[base + q * qs + t] = R[wrap_pos(q, s, n) * T + t];
+        }
+    }
+}
+
+// Rows, adjoint: inverse FFT along g1 of rows i2 < N2 of planes i3 < N3,
+// crop (pixel i1 from (i1 - s1) mod G1) and deapodisation.
+template <int BCT>
+__global__ __launch_bounds__(256) void n3_rows_adj(Nu3Geom g, const FftFactors* __restrict__ fac,
+                                                   const float2* __restrict__ twp,
+                                                   const float* __restrict__ deap,
+                                                   const float2* __restrict__ ws, float2* __restrict__ x,
+                                                   int ch, int lanes) {
+    constexpr int T = n3_T<BCT>(), LPG = T / BCT;
+    extern __shared__ float2 lds[];
+    const FftFactors& F = *fac;
+    const int n = g.G1;
+    float2* A = lds;
+    float2* B = lds + n * T;
+    float2* tw = lds + 2 * n * T;
+    const int L0 = (int)blockIdx.x * LPG;
+    const int nl = g.N2 * g.N3;
+    for (int i = threadIdx.x; i < n - 1; i += kFftThreads) tw[i] = twp[i];
+    for (int idx = threadIdx.x; idx < n * T; idx += kFftThreads) {
+        const int b = idx % BCT, q = (idx / BCT) % n, j = idx / (BCT * n);
+        const int L = L0 + j;
+        float2 v = make_float2(0.f, 0.f);
+        if (L < nl) {
+            const int i2 = L % g.N2, i3 = L / g.N2;
+            v = ws[(((int64_t)i3 * g.G2 + i2) * g.G1 + q) * BCT + b];
+        }
+        A[q * T + j * BCT + b] = v;
+    }
+    __syncthreads();
+    const float2* R = fft_lds<T, -1>(A, B, F, tw);
+    const float* d1 = deap;
+    const float* d2 = deap + g.N1;
+    const float* d3 = d2 + g.N2;
+    for (int idx = threadIdx.x; idx < g.N1 * T; idx += kFftThreads) {
+        const int i1 = idx % g.N1, t = idx / g.N1;
+        const int b = t % BCT, L = L0 + t / BCT;
+        if (L >= nl || b >= lanes) continue;
+        const int i2 = L % g.N2, i3 = L / g.N2;
+        const float2 v = R[wrap_pos(i1, g.s1, n) * T + t];
+        const float f = d1[i1] * d2[i2] * d3[i3];
+        x[(int64_t)(ch * BCT + b) * g.V + (int64_t)L * g.N1 + i1] = make_float2(v.x * f, v.y * f);
+    }
+}
+
+// Odd sides: i exp(-i pi g_d / G_d) per odd side d (CONJ: the conjugate).
+template <int BCT, int CONJ>
+__global__ __launch_bounds__(256) void n3_odd_ramp(Nu3Geom g, float2* __restrict__ ws) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= g.chunk) return;
+    const int64_t cell = e / BCT;
+    const int g1 = (int)(cell % g.G1);
+    const int64_t r = cell / g.G1;
+    const int g2 = (int)(r % g.G2), g3 = (int)(r / g.G2);
+    float a = 0.f;   // half-turns
+    if (g.odd & 1) a += 0.5f - (float)g1 / (float)g.G1;
+    if (g.odd & 2) a += 0.5f - (float)g2 / (float)g.G2;
+    if (g.odd & 4) a += 0.5f - (float)g3 / (float)g.G3;
+    float sn, cs;
+    sincospif(CONJ ? -a : a, &sn, &cs);
+    const float2 v = ws[e];
+    ws[e] = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+}
+
+// Degridding: lane = (sample, transform), transform fastest, so the BCT
+// lanes of a sample read each tap's BCT consecutive values; W^3 taps in
+// (s3, s2, s1) order with weight w3 w2 w1, then dcf.  The block's results
+// go through LDS and are written sample-fastest.
+template <int BCT>
+__global__ __launch_bounds__(256) void n3_degrid(Nu3Geom g, const uint32_t* __restrict__ base,
+                                                 const float* __restrict__ wt, const float* __restrict__ dcf,
+                                                 const float2* __restrict__ ws, float2* __restrict__ y, int ch,
+                                                 int lanes) {
+    constexpr int SPB = 256 / BCT;
+    __shared__ float2 st[256];
+    const int b = threadIdx.x % BCT, ml = threadIdx.x / BCT;
+    const int m = (int)blockIdx.x * SPB + ml;
+    float2 acc = make_float2(0.f, 0.f);
+    if (m < g.M) {
+        const uint32_t bs = base[m];
+        const int g10 = (int)(bs & 1023u), g20 = (int)((bs >> 10) & 1023u), g30 = (int)(bs >> 20);
+        const int W = g.W;
+        const float* w = wt + (int64_t)m * 3 * W;
+        int c3 = g30;
+        for (int s3 = 0; s3 < W; ++s3) {
+            const float w3 = w[2 * W + s3];
+            int c2 = g20;
+            for (int s2 = 0; s2 < W; ++s2) {
+                const float w23 = w3 * w[W + s2];
+                const float2* row = ws + (((int64_t)c3 * g.G2 + c2) * g.G1) * BCT + b;
+                int c1 = g10;
+                for (int s1 = 0; s1 < W; ++s1) {
+                    const float wk = w23 * w[s1];
+                    const float2 z = row[(int64_t)c1 * BCT];
+                    acc.x += wk * z.x;
+                    acc.y += wk * z.y;
+                    if (++c1 == g.G1) c1 = 0;
+                }
+                if (++c2 == g.G2) c2 = 0;
+            }
+            if (++c3 == g.G3) c3 = 0;
+        }
+        const float d = dcf[m];
+        acc.x *= d;
+        acc.y *= d;
+    }
+    st[b * SPB + ml] = acc;
+    __syncthreads();
+    const int b2 = threadIdx.x / SPB, m2 = (int)blockIdx.x * SPB + (int)threadIdx.x % SPB;
+    if (m2 < g.M && b2 < lanes) y[(int64_t)(ch * BCT + b2) * g.M + m2] = st[threadIdx.x];
+}
+
+// y of one chunk into the transposed layout yT[m][b] (read sample-fastest,
+// written transform-fastest through LDS); the CSR weights carry the dcf.
+template <int BCT>
+__global__ __launch_bounds__(256) void n3_transpose_y(Nu3Geom g, const float2* __restrict__ y,
+                                                      float2* __restrict__ yT, int ch, int lanes) {
+    constexpr int SPB = 256 / BCT;
+    __shared__ float2 st[256];
+    const int b = threadIdx.x / SPB, ml = threadIdx.x % SPB;
+    const int m = (int)blockIdx.x * SPB + ml;
+    float2 v = make_float2(0.f, 0.f);
+    if (m < g.M && b < lanes) v = y[(int64_t)(ch * BCT + b) * g.M + m];
+    st[threadIdx.x] = v;
+    __syncthreads();
+    const int b2 = threadIdx.x % BCT, ml2 = threadIdx.x / BCT, m2 = (int)blockIdx.x * SPB + ml2;
+    if (m2 < g.M) yT[(int64_t)m2 * BCT + b2] = st[b2 * SPB + ml2];
+}
+
+// Gridding as an output-stationary gather: lane = (cell, transform) sums
+// its cell's CSR row in sample order; every cell written once.
+template <int BCT>
+__global__ __launch_bounds__(256) void n3_grid_csr(const int* __restrict__ rp, const int2* __restrict__ ent,
+                                                   const float2* __restrict__ yT, float2* __restrict__ ws,
+                                                   int64_t ncell) {
+    const int64_t cell = (int64_t)blockIdx.x * (256 / BCT) + threadIdx.x / BCT;
+    const int b = threadIdx.x % BCT;
+    if (cell >= ncell) return;
+    const int e0 = rp[cell], e1 = rp[cell + 1];
+    float2 acc = make_float2(0.f, 0.f);
+    for (int e = e0; e < e1; ++e) {
+        const int2 en = ent[e];
+        const float w = __int_as_float(en.y);
+        const float2 v = yT[(int64_t)en.x * BCT + b];
+        acc.x += w * v.x;
+        acc.y += w * v.y;
+    }
+    ws[cell * BCT + b] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// host
+
+Nufft3Op::Nufft3Op(mm_ctx* c, const int64_t* imN, int64_t M, const double* kloc, const double* dcf, double os,
+                   int kwidth) {
+    ctx = c;
+    if (kwidth < 2 || kwidth > 8 || (kwidth & 1))
+        fail(MM_ERR_ARG, "nufftOp: kwidth must be even and in [2, 8]");
+    if (!(os > 1.0)) fail(MM_ERR_ARG, "nufftOp: os must be > 1");
+    if (M < 1) fail(MM_ERR_ARG, "nufftOp: no samples");
+    if (M > INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples");
+    if (!kloc) fail(MM_ERR_ARG, "nufftOp: kloc is NULL");
+    int N[3], G[3];
+    for (int d = 0; d < 3; ++d) {
+        if (imN[d] < 2) fail(MM_ERR_ARG, "nufftOp: image size too small");
+        if (imN[d] > 100000) fail(MM_ERR_ARG, "nufftOp: image size too large");
+        N[d] = (int)imN[d];
+        G[d] = nufft_grid_size(N[d], os);
+        if (G[d] > 512)
+            fail(MM_ERR_UNSUPPORTED, "nufftOp: 3-D grid sides above 512 are not implemented (LDS FFT passes)");
+        if (G[d] < kwidth) fail(MM_ERR_ARG, "nufftOp: grid side smaller than the kernel width");
+    }
+    const int W = kwidth;
+    const int64_t ncell = (int64_t)G[0] * G[1] * G[2];
+    const int64_t nent = M * W * W * W;
+    if (nent > (int64_t)INT32_MAX) fail(MM_ERR_ARG, "nufftOp: too many samples for the 3-D gridding rows");
+    g_.N1 = N[0];
+    g_.N2 = N[1];
+    g_.N3 = N[2];
+    g_.G1 = G[0];
+    g_.G2 = G[1];
+    g_.G3 = G[2];
+    g_.s1 = N[0] / 2;
+    g_.s2 = N[1] / 2;
+    g_.s3 = N[2] / 2;
+    g_.M = (int)M;
+    g_.W = W;
+    g_.odd = (N[0] & 1) | ((N[1] & 1) << 1) | ((N[2] & 1) << 2);
+    g_.V = (int64_t)N[0] * N[1] * N[2];
+    hipStream_t st = ctx->stream;
+
+    std::vector<FftFactors> facs(3);
+    std::vector<float2> twid;
+    for (int d = 0; d < 3; ++d) {
+        if (!fft_factorize(G[d], facs[d], false)) fail(MM_ERR_ARG, "nufftOp: grid size with a prime factor > 7");
+        tw_off_[d] = (int)twid.size();
+        fft_pass_twiddles(facs[d], [&](float cs, float sn) { twid.push_back(make_float2(cs, sn)); });
+    }
+    // deapodisation per side; side 1 carries 1/sqrt(G1 G2 G3) (unitary
+    // ifftnc) and (-1)^(s1+s2+s3), the real part of the ramp constant
+    // exp(i pi (N1+N2+N3)/2) (an odd side's i goes with its ramp)
+    std::vector<float> deap;
+    const double sgn = ((g_.s1 + g_.s2 + g_.s3) & 1) ? -1.0 : 1.0;
+    const double scl = sgn / std::sqrt((double)ncell);
+    for (int d = 0; d < 3; ++d) {
+        const auto cd = nufft_deapod(N[d], G[d], kwidth, os);
+        for (double v : cd) deap.push_back((float)(d == 0 ? v * scl : v));
+    }
+    // per-sample weights and first taps (nufftOp.m:187-246)
+    const std::vector<double> lut = nufft_kb_lut(kwidth, os);
+    const int Nk = nufft_kb_nkern(kwidth, os);
+    std::vector<float> wt((size_t)M * 3 * W), dcfv((size_t)M);
+    std::vector<double> wd((size_t)M * 3 * W);
+    std::vector<uint32_t> base((size_t)M);
+    std::vector<int> g0((size_t)M * 3);
+    for (int64_t m = 0; m < M; ++m) {
+        uint32_t packed = 0;
+        for (int d = 0; d < 3; ++d) {
+            const double ka = (kloc[(int64_t)d * M + m] + 0.5) * G[d];
+            if (!std::isfinite(ka)) fail(MM_ERR_ARG, "nufftOp: non-finite kloc");
+            const double fl = std::floor(ka);
+            const double fr = fl - ka;
+            for (int si = 0; si < W; ++si) {
+                const int soff = -W / 2 + 1 + si;
+                const double ki = (fr + soff) * (Nk - 1) / W + Nk / 2.0 + 0.5;
+                const double fk = std::floor(ki);
+                const double frac = ki - fk;
+                int ii = (int)fk;   // 1-based
+                ii = std::min(std::max(ii, 1), Nk);
+                const double w = (1 - frac) * lut[(size_t)ii - 1] + frac * lut[(size_t)ii];
+                wd[(size_t)m * 3 * W + (size_t)d * W + si] = w;
+                wt[(size_t)m * 3 * W + (size_t)d * W + si] = (float)w;
+            }
+            int64_t gfirst = ((int64_t)fl - W / 2 + 1 + G[d] / 2) % G[d];
+            if (gfirst < 0) gfirst += G[d];
+            g0[(size_t)m * 3 + d] = (int)gfirst;
+            packed |= (uint32_t)gfirst << (10 * d);
+        }
+        base[(size_t)m] = packed;
+        dcfv[(size_t)m] = dcf ? (float)dcf[m] : 1.f;
+    }
+    // CSR rows of the transposed interpolation matrix, each in sample order
+    std::vector<int> rp((size_t)ncell + 1, 0);
+    auto for_taps = [&](int64_t m, auto fn) {
+        int c3 = g0[(size_t)m * 3 + 2];
+        for (int s3 = 0; s3 < W; ++s3) {
+            int c2 = g0[(size_t)m * 3 + 1];
+            for (int s2 = 0; s2 < W; ++s2) {
+                int c1 = g0[(size_t)m * 3];
+                for (int s1 = 0; s1 < W; ++s1) {
+                    fn(((int64_t)c3 * G[1] + c2) * G[0] + c1, s1, s2, s3);
+                    if (++c1 == G[0]) c1 = 0;
+                }
+                if (++c2 == G[1]) c2 = 0;
+            }
+            if (++c3 == G[2]) c3 = 0;
+        }
+    };
+    for (int64_t m = 0; m < M; ++m) for_taps(m, [&](int64_t cell, int, int, int) { ++rp[(size_t)cell + 1]; });
+    for (int64_t i = 0; i < ncell; ++i) rp[(size_t)i + 1] += rp[(size_t)i];
+    std::vector<int> fillp(rp.begin(), rp.end() - 1);
+    std::vector<int2> ent((size_t)nent);
+    for (int64_t m = 0; m < M; ++m) {
+        const double* w = wd.data() + (size_t)m * 3 * W;
+        const double dm = dcf ? dcf[m] : 1.0;
+        for_taps(m, [&](int64_t cell, int s1, int s2, int s3) {
+            const float v = (float)(w[s1] * w[W + s2] * w[2 * W + s3] * dm);
+            int bits;
+            std::memcpy(&bits, &v, sizeof bits);
+            ent[(size_t)fillp[(size_t)cell]++] = make_int2((int)m, bits);
+        });
+    }
+    facs_.upload(facs, st);
+    twid_.upload(twid, st);
+    deap_.upload(deap, st);
+    base_.upload(base, st);
+    wt_.upload(wt, st);
+    dcf_.upload(dcfv, st);
+    rp_.upload(rp, st);
+    ent_.upload(ent, st);
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+template <int BCT>
+void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) {
+    hipStream_t st = ctx->stream;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)n3_rows_fwd<BCT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)n3_rows_adj<BCT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipGetLastError();
+        attr_set = true;
+    }
+    constexpr int T = n3_T<BCT>(), LPG = T / BCT, SPB = 256 / BCT;
+    Nu3Geom g = g_;
+    const int64_t ncell = (int64_t)g.G1 * g.G2 * g.G3;
+    g.chunk = ncell * BCT;
+    ws_.ensure((size_t)g.chunk);
+    if (adjoint) yT_.ensure((size_t)g.M * BCT);
+    const unsigned rows_blocks = (unsigned)((g.N2 * g.N3 + LPG - 1) / LPG);
+    const unsigned gx = (unsigned)((g.G1 + LPG - 1) / LPG);
+    const unsigned samp_blocks = (unsigned)((g.M + SPB - 1) / SPB);
+    const unsigned elem_blocks = (unsigned)((g.chunk + 255) / 256);
+    const unsigned cell_blocks = (unsigned)((ncell * BCT + 255) / 256);
+    const int64_t nchunk = (nrep + BCT - 1) / BCT;
+    for (int64_t ch = 0; ch < nchunk; ++ch) {
+        const int lanes = (int)std::min<int64_t>(BCT, nrep - ch * BCT);
+        if (!adjoint) {
+            hipLaunchKernelGGL(n3_rows_fwd<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+                               twid_.p + tw_off_[0], deap_.p, in, ws_.p, (int)ch, lanes);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+                               facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            LAUNCHCHECK();
+            if (g.odd) {
+                hipLaunchKernelGGL((n3_odd_ramp<BCT, 0>), dim3(elem_blocks), dim3(256), 0, st, g, ws_.p);
+                LAUNCHCHECK();
+            }
+            hipLaunchKernelGGL(n3_degrid<BCT>, dim3(samp_blocks), dim3(256), 0, st, g, base_.p, wt_.p, dcf_.p,
+                               (const float2*)ws_.p, out, (int)ch, lanes);
+            LAUNCHCHECK();
+        } else {
+            hipLaunchKernelGGL(n3_transpose_y<BCT>, dim3(samp_blocks), dim3(256), 0, st, g, in, yT_.p, (int)ch,
+                               lanes);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL(n3_grid_csr<BCT>, dim3(cell_blocks), dim3(256), 0, st, rp_.p, ent_.p,
+                               (const float2*)yT_.p, ws_.p, ncell);
+            LAUNCHCHECK();
+            if (g.odd) {
+                hipLaunchKernelGGL((n3_odd_ramp<BCT, 1>), dim3(elem_blocks), dim3(256), 0, st, g, ws_.p);
+                LAUNCHCHECK();
+            }
+            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+                               facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL(n3_rows_adj<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+                               twid_.p + tw_off_[0], deap_.p, (const float2*)ws_.p, out, (int)ch, lanes);
+            LAUNCHCHECK();
+        }
+    }
+}
+
+void Nufft3Op::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
+    if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
+    int bct = 1;
+    while (bct < nrep && bct < 16) bct <<= 1;
+    switch (bct) {
+        case 1: launch<1>(adjoint, in, out, nrep); break;
+        case 2: launch<2>(adjoint, in, out, nrep); break;
+        case 4: launch<4>(adjoint, in, out, nrep); break;
+        case 8: launch<8>(adjoint, in, out, nrep); break;
+        default: launch<16>(adjoint, in, out, nrep); break;
+    }
+}
+
+}  // namespace mm

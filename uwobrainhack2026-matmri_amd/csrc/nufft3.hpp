@@ -1,0 +1,46 @@
+// 3-D Kaiser-Bessel gridding NUFFT on gfx950: nufftOp with numel(imN) == 3
+// (operators/@nufftOp/nufftOp.m:138-274 for the plan, :354-394 for the two
+// directions; the 3-D cases of :195-197, :244-245, :268-271).
+#pragma once
+#include <array>
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace mm {
+
+// Geometry every 3-D kernel takes by value.
+struct Nu3Geom {
+    int N1, N2, N3, G1, G2, G3;
+    int s1, s2, s3;     // floor(N_d / 2): the wrapped placement of each side
+    int M, W, odd;      // odd: bit d set when side d is odd
+    int64_t V;          // N1 N2 N3
+    int64_t chunk;      // grid elements of one chunk: G1 G2 G3 BCT
+};
+
+class Nufft3Op : public mm_op {
+   public:
+    // imN = {N1, N2, N3}; kloc = column-major [M x 3] cycles/pixel in
+    // [-0.5, 0.5) (wrapped); dcf = [M] or NULL.
+    Nufft3Op(mm_ctx* ctx, const int64_t* imN, int64_t M, const double* kloc, const double* dcf, double os,
+             int kwidth);
+    int64_t n_image() const override { return g_.V; }
+    int64_t n_data() const override { return g_.M; }
+    void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
+
+   private:
+    template <int BCT>
+    void launch(int adjoint, const float2* in, float2* out, int64_t nrep);
+    Nu3Geom g_{};
+    std::array<int, 3> tw_off_{};        // per side: first twiddle in twid_
+    DevBuf<FftFactors> facs_;            // per side
+    DevBuf<float2> twid_;
+    DevBuf<float> deap_;                 // N1 + N2 + N3 values, scale and sign in side 1
+    DevBuf<uint32_t> base_;              // first taps g1 | g2 << 10 | g3 << 20 (wrapped)
+    DevBuf<float> wt_, dcf_;             // per sample: W weights per side; dcf (1 without)
+    DevBuf<int> rp_;                     // CSR gridding: row pointers per grid cell
+    DevBuf<int2> ent_;                   // (sample, w1 w2 w3 dcf as float bits), sample order
+    DevBuf<float2> ws_, yT_;             // one chunk's grid; transposed data
+};
+
+}  // namespace mm

@@ -166,7 +166,8 @@ class nufftOp(LinearOp):
     ``kloc`` is ``[M x nD]`` in cycles/pixel, ``[-0.5, 0.5)`` (wrapped).
     ``kwidth`` defaults to 6 and ``os`` to 1.5 like the reference properties
     (nufftOp.m:65-66).  Only the GPU path exists (``useGPU`` is accepted and
-    ignored); 1-D and 2-D images are implemented.
+    ignored); 1-D, 2-D and 3-D images are implemented (3-D: one volume per
+    plan, grid sides up to 512, no Toeplitz form).
     """
 
     def __init__(self, imN, kloc, dcf=None, useGPU=1, os=1.5, kwidth=6):
@@ -193,13 +194,20 @@ class nufftOp(LinearOp):
     def _build(self, slices, os, kwidth):
         nd = len(slices[0][0])
         ns = len(slices)
-        imN = np.zeros(2 * ns, dtype=np.int64)
+        if nd == 3 and ns != 1:
+            raise NotImplementedError("3-D nufftOp: one volume per plan")
+        if nd not in (1, 2, 3) or any(len(s[0]) != nd for s in slices):
+            raise ValueError("nufftOp: every slice needs the same 1, 2 or 3 image dimensions")
+        imN = np.zeros(3 if nd == 3 else 2 * ns, dtype=np.int64)
         Ms = np.zeros(ns, dtype=np.int64)
         self._kl = []
         self._dc = []
         for s, (n, kl, dc) in enumerate(slices):
-            imN[2 * s] = n[0]
-            imN[2 * s + 1] = n[1] if nd == 2 else 1
+            if nd == 3:
+                imN[:] = n
+            else:
+                imN[2 * s] = n[0]
+                imN[2 * s + 1] = n[1] if nd == 2 else 1
             kl = np.asfortranarray(np.asarray(kl, np.float64).reshape(-1, nd))
             Ms[s] = kl.shape[0]
             self._kl.append(kl)
@@ -236,6 +244,8 @@ class nufftOp(LinearOp):
         returned operator applies it on a grid of at least 2N per side."""
         if self.isToep:
             return self
+        if len(self.imN[0]) == 3:
+            raise NotImplementedError("prepToep of a 3-D nufftOp is not implemented")
         if tos is None:
             tos = 0.5 * (self.os - 1) + 1
         if kwidth is None:
