@@ -25,13 +25,13 @@
 //   nu_fwd_rows   coil multiply + deapod + wrap/zero-pad + FFT along g1
 //   nu_cols       FFT along g2 (fwd: zero-pad from N2 rows; adj: crop to N2)
 //   nu_degrid     gather of W^2 taps per sample (dcf, optional weights fused)
-//   nu_grid_csr   2-D gridding as an output-stationary gather: one lane per
-//                 (cell, transform) sums, in sample order, the CSR row of
+//   nu_grid_tile  2-D gridding from LDS-staged rectangles of the CSR rows
+//                 (+ nu_grid_merge for the chunks of dense rectangles)
+//   nu_grid_csr   gridding as an output-stationary gather from the global
+//                 CSR rows (1-D always, 2-D as an option): one lane per
+//                 (cell, transform) sums, in sample order, the row of
 //                 (sample, w1*w2*dcf) pairs that reach the cell -- every cell
 //                 written once, no atomics, deterministic
-//   nu_grid       1-D gridding: LDS-tiled scatter (float atomics in LDS),
-//                 heavy tiles split and merged by the last-arriving
-//                 work-group
 //   nu_adj_rows   FFT along g1 + crop + deapod + conj-coil combine
 //   nu_odd_ramp   odd image sides only: the part of the ramp P that is not a
 //                 circular shift, multiplied into the grid
@@ -1016,286 +1016,11 @@ __global__ __launch_bounds__(256) void nu_transpose_y(const NuSliceDev* __restri
     for (int i = threadIdx.x; i < n * BCT; i += kFftThreads) dst[i] = buf[i];
 }
 
-// Gather gridding: each thread owns (cell, transform) pairs of rows
-// [r0, r1) of one tile and sums, in a fixed order, the samples of the W x W
-// base-cell bins that reach the cell -- no atomics, deterministic.  The data
-// side is read in the transposed layout, so the BCT lanes of a cell read one
-// contiguous BCT*8-byte row per sample.
-constexpr int kMaxBins = (kTile + 7) * (kTile + 7) + 1;
 // CSR gridding: entries per work item (lanes x entries stays balanced)
 constexpr int kCsrItemEntries = 2048;
 // accelerator complex dies (each with its own L2); work-groups are dealt to
 // them round-robin by workgroup index
 constexpr int kXcds = 8;
-
-// entries of a gathered tile that fit the LDS staging area (32 KB)
-template <int BCT>
-constexpr int gather_max_entries(int W) {
-    return (32 * 1024) / (8 * W + 8 * BCT);
-}
-
-template <int BCT, int WT>
-__device__ __forceinline__ void grid_gather_rows(const NuSliceDev& S, int ch, const GatherWork g,
-                                                 const int* __restrict__ glist,
-                                                 const int* __restrict__ gbin,
-                                                 const float* __restrict__ swt,
-                                                 const float* __restrict__ sdcf,
-                                                 const float2* __restrict__ yt,
-                                                 float2* __restrict__ ws, const NuRun& run, int* sb,
-                                                 float* stage) {
-    const int W = WT > 0 ? WT : S.W1;
-    const int NB1 = kTile + W - 1;
-    const int nbins = NB1 * NB1 + 1;
-    for (int i = threadIdx.x; i < nbins; i += kFftThreads) sb[i] = gbin[g.bin_off + i];
-    __syncthreads();
-    const int n = sb[nbins - 1];   // entries of the tile (fits the staging area)
-    const int* list = glist + g.ent_off;
-    const float2* ych = yt + S.yt_off + (int64_t)ch * S.M * BCT;
-    // stage: per entry 2W weights (dim-1 weights times dcf) and its BCT data values
-    float* sw = stage;                                          // [n][2W]
-    float2* sy = reinterpret_cast<float2*>(stage + 2 * W * n);  // [n][BCT]
-    for (int i = threadIdx.x; i < n * 2 * W; i += kFftThreads) {
-        const int e = i / (2 * W), q = i % (2 * W);
-        const int64_t sm = S.samp_off + list[e];
-        float w = swt[sm * (2 * W) + q];
-        if (q < W) w *= sdcf[sm];
-        sw[i] = w;
-    }
-    for (int i = threadIdx.x; i < n * BCT; i += kFftThreads) {
-        const int e = i / BCT, b = i & (BCT - 1);
-        sy[i] = ych[(int64_t)list[e] * BCT + b];
-    }
-    __syncthreads();
-    const int G1 = S.G1, G2 = S.G2;
-    const int lanes = lanes_of(run, BCT, ch);
-    float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    const int pairs = (g.r1 - g.r0) * kTile * BCT;
-    for (int p = threadIdx.x; p < pairs; p += kFftThreads) {
-        const int b = p & (BCT - 1);
-        const int cell = p / BCT;
-        const int l1 = cell % kTile, l2 = g.r0 + cell / kTile;
-        const int g1 = g.t1 * kTile + l1, g2 = g.t2 * kTile + l2;
-        if (g1 >= G1 || g2 >= G2) continue;
-        float2 acc = make_float2(0.f, 0.f);
-        if (b < lanes) {
-            for (int dy = 0; dy < W; ++dy) {
-                const int s2 = W - 1 - dy;
-                const int rowb = (l2 + dy) * NB1 + l1;
-                for (int dx = 0; dx < W; ++dx) {
-                    const int s1 = W - 1 - dx;
-                    const int e1 = sb[rowb + dx + 1];
-                    for (int e = sb[rowb + dx]; e < e1; ++e) {
-                        const float w = sw[e * 2 * W + s1] * sw[e * 2 * W + W + s2];
-                        const float2 v = sy[e * BCT + b];
-                        acc.x = fmaf(w, v.x, acc.x);
-                        acc.y = fmaf(w, v.y, acc.y);
-                    }
-                }
-            }
-        }
-        zg[((int64_t)g2 * G1 + g1) * BCT + b] = acc;
-    }
-}
-
-template <int BCT, int WT>
-__device__ __forceinline__ void grid_scatter_tile(const NuSliceDev& S, int ch, const TileWork tw,
-                                                  const int* __restrict__ tlist,
-                                                  const uint32_t* __restrict__ sbase,
-                                                  const float* __restrict__ swt,
-                                                  const float* __restrict__ sdcf,
-                                                  const float2* __restrict__ y, float2* __restrict__ ws,
-                                                  float2* __restrict__ parts, int* __restrict__ counters,
-                                                  const NuRun& run, float* accr, float* acci, int& s_last) {
-    constexpr int CELLS = kTile * kTile * BCT;
-    const int G1 = S.G1, G2 = S.G2, M = S.M;
-    const int W1 = WT > 0 ? WT : S.W1, W2 = WT > 0 ? WT : S.W2;
-    for (int i = threadIdx.x; i < CELLS; i += kFftThreads) {
-        accr[i] = 0.f;
-        acci[i] = 0.f;
-    }
-    __syncthreads();
-    const int t1_0 = tw.t1 * kTile, t2_0 = tw.t2 * kTile;
-    const int lanes = lanes_of(run, BCT, ch);
-    const int64_t stride = chunk_stride(run, BCT);
-    const int* list = tlist + S.tlist_off + tw.start;
-    if constexpr (WT == 4) {
-        // Each thread owns one transform b and a contiguous run of the list,
-        // which is sorted by base cell: the 4x4 footprint of consecutive
-        // samples with the same base is summed in registers and added to the
-        // LDS tile once per distinct base.  In the dense centre of a
-        // variable-density trajectory hundreds of samples share a base, so
-        // this removes almost all same-address LDS atomics.
-        const int b = threadIdx.x & (BCT - 1);
-        const int SPB = 256 / BCT;
-        const int per = (tw.count + SPB - 1) / SPB;
-        const int j0 = (threadIdx.x / BCT) * per;
-        const int j1 = min(tw.count, j0 + per);
-        if (b < lanes && j0 < j1) {
-            float2 acc[4][4];
-            uint32_t cur = sbase[S.samp_off + list[j0]];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[a][c] = make_float2(0.f, 0.f);
-            auto flush = [&](uint32_t base) {
-                int l1[4], l2[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    int g = (int)(base & 0xffff) + q;
-                    g -= g >= G1 ? G1 : 0;
-                    l1[q] = g - t1_0;
-                    int h = (int)(base >> 16) + q;
-                    h -= h >= G2 ? G2 : 0;
-                    l2[q] = h - t2_0;
-                }
-#pragma unroll
-                for (int s2 = 0; s2 < 4; ++s2) {
-                    if ((unsigned)l2[s2] >= (unsigned)kTile) continue;
-#pragma unroll
-                    for (int s1 = 0; s1 < 4; ++s1) {
-                        if ((unsigned)l1[s1] >= (unsigned)kTile) continue;
-                        const int c = (l2[s2] * kTile + l1[s1]) * BCT + b;
-                        atomicAdd(&accr[c], acc[s2][s1].x);
-                        atomicAdd(&acci[c], acc[s2][s1].y);
-                    }
-                }
-            };
-            for (int j = j0; j < j1; j += 4) {
-                // loads of four samples are issued before any of them is used
-                int mq[4];
-                uint32_t bq[4];
-                float dq[4];
-                float2 vq[4];
-                float4 wa[4], wb[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) mq[q] = j + q < j1 ? list[j + q] : -1;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (mq[q] < 0) continue;
-                    const int64_t sm = S.samp_off + mq[q];
-                    bq[q] = sbase[sm];
-                    dq[q] = sdcf[sm];
-                    vq[q] = run.ylayout ? y[S.yt_off + ((int64_t)ch * M + mq[q]) * BCT + b]
-                                        : y[S.dat_off + ((int64_t)ch * stride + b) * M + mq[q]];
-                    wa[q] = *reinterpret_cast<const float4*>(swt + sm * 8);
-                    wb[q] = *reinterpret_cast<const float4*>(swt + sm * 8 + 4);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (mq[q] < 0) continue;
-                    if (bq[q] != cur) {
-                        flush(cur);
-#pragma unroll
-                        for (int a = 0; a < 4; ++a)
-#pragma unroll
-                            for (int c = 0; c < 4; ++c) acc[a][c] = make_float2(0.f, 0.f);
-                        cur = bq[q];
-                    }
-                    const float w1[4] = {wa[q].x, wa[q].y, wa[q].z, wa[q].w};
-                    const float w2[4] = {wb[q].x, wb[q].y, wb[q].z, wb[q].w};
-                    const float vx0 = vq[q].x * dq[q], vy0 = vq[q].y * dq[q];
-#pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2) {
-                        const float vx = w2[s2] * vx0, vy = w2[s2] * vy0;
-#pragma unroll
-                        for (int s1 = 0; s1 < 4; ++s1) {
-                            acc[s2][s1].x = fmaf(w1[s1], vx, acc[s2][s1].x);
-                            acc[s2][s1].y = fmaf(w1[s1], vy, acc[s2][s1].y);
-                        }
-                    }
-                }
-            }
-            flush(cur);
-        }
-    } else {
-    // Samples that are adjacent in the list are adjacent on the trajectory and
-    // hit the same cells; the lanes of one pass take samples Q apart instead
-    // (a transposed walk of the SPB x Q list), so that the LDS atomics of a
-    // wave rarely target the same address.
-    const int SPB = 256 / BCT;
-    const int Q = (tw.count + SPB - 1) / SPB;
-    const int total = SPB * Q * BCT;
-    for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
-        const int b = idx & (BCT - 1);
-        const int k = idx / BCT;
-        const int kk = (k % SPB) * Q + k / SPB;
-        if (b >= lanes || kk >= tw.count) continue;
-        const int m = list[kk];
-        const int64_t sm = S.samp_off + m;
-        const uint32_t base = sbase[sm];
-        const float d = sdcf[sm];
-        float2 v = run.ylayout ? y[S.yt_off + ((int64_t)ch * M + m) * BCT + b]
-                               : y[S.dat_off + ((int64_t)ch * stride + b) * M + m];
-        v.x *= d;
-        v.y *= d;
-        float w1[8], w2[8];
-        load_weights<WT>(swt + sm * (W1 + W2), W1, W2, w1, w2);
-        // tile-local tap coordinates (the footprint may wrap around the grid)
-        int l1[8], l2[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            int g = (int)(base & 0xffff) + s;
-            g -= g >= G1 ? G1 : 0;
-            l1[s] = g - t1_0;
-            int h = (int)(base >> 16) + s;
-            h -= h >= G2 ? G2 : 0;
-            l2[s] = h - t2_0;
-        }
-#pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2) {
-            if (s2 >= W2 || (unsigned)l2[s2] >= (unsigned)kTile) continue;
-            const float wy = w2[s2];
-#pragma unroll
-            for (int s1 = 0; s1 < 8; ++s1) {
-                if (s1 >= W1 || (unsigned)l1[s1] >= (unsigned)kTile) continue;
-                const float w = w1[s1] * wy;
-                const int c = (l2[s2] * kTile + l1[s1]) * BCT + b;
-                atomicAdd(&accr[c], w * v.x);
-                atomicAdd(&acci[c], w * v.y);
-            }
-        }
-    }
-    }
-    __syncthreads();
-    float2* zg = ws + S.ws_off + (int64_t)ch * S.ws_chunk;
-    if (tw.slot < 0) {
-        for (int i = threadIdx.x; i < CELLS; i += kFftThreads) {
-            const int b = i & (BCT - 1);
-            const int l1 = (i / BCT) % kTile, l2 = i / (BCT * kTile);
-            const int g1 = t1_0 + l1, g2 = t2_0 + l2;
-            if (g1 < G1 && g2 < G2) zg[((int64_t)g2 * G1 + g1) * BCT + b] = make_float2(accr[i], acci[i]);
-        }
-        return;
-    }
-    // split tile: publish this part, the last arriving work-group sums all
-    // parts in slot order and writes the tile
-    float2* pbase = parts + S.part_off + (int64_t)ch * S.part_chunk;
-    float2* mine = pbase + (int64_t)tw.slot * CELLS;
-    for (int i = threadIdx.x; i < CELLS; i += kFftThreads) mine[i] = make_float2(accr[i], acci[i]);
-    __threadfence();   // every wave's partial is visible device-wide before the count
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int* cnt = counters + S.cnt_off + (int64_t)ch * S.cnt_chunk + tw.counter;
-        const int prev = __hip_atomic_fetch_add(cnt, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == tw.nparts - 1);
-        if (s_last) __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    for (int i = threadIdx.x; i < CELLS; i += kFftThreads) {
-        float2 sacc = make_float2(0.f, 0.f);
-        for (int p = 0; p < tw.nparts; ++p) {
-            const float2 q = pbase[(int64_t)(tw.first_slot + p) * CELLS + i];
-            sacc.x += q.x;
-            sacc.y += q.y;
-        }
-        const int b = i & (BCT - 1);
-        const int l1 = (i / BCT) % kTile, l2 = i / (BCT * kTile);
-        const int g1 = t1_0 + l1, g2 = t2_0 + l2;
-        if (g1 < G1 && g2 < G2) zg[((int64_t)g2 * G1 + g1) * BCT + b] = sacc;
-    }
-}
 
 // CSR gridding: work item = (slice, chunk, first cell, cell count | log2(P)
 // << 24).  Lanes are (cell, part, transform) triples, BCT consecutive lanes
@@ -1557,38 +1282,6 @@ __global__ __launch_bounds__(256) void nu_grid_merge(const GridItem* __restrict_
     }
 }
 
-// Gridding: one launch over two kinds of work items (int4.w): 1 = a chunk of
-// a dense tile (register-run scatter into LDS), 0 = cell rows of a sparse
-// tile (atomic-free gather).  Each tile is handled by one kind only.
-template <int BCT, int WT>
-__global__ __launch_bounds__(256) void nu_grid(const NuSliceDev* __restrict__ sl,
-                                               const int4* __restrict__ work,
-                                               const TileWork* __restrict__ tiles,
-                                               const int* __restrict__ tlist,
-                                               const GatherWork* __restrict__ gwk,
-                                               const int* __restrict__ glist,
-                                               const int* __restrict__ gbin,
-                                               const uint32_t* __restrict__ sbase,
-                                               const float* __restrict__ swt,
-                                               const float* __restrict__ sdcf,
-                                               const float2* __restrict__ y, float2* __restrict__ ws,
-                                               float2* __restrict__ parts, int* __restrict__ counters,
-                                               NuRun run) {
-    constexpr int CELLS = kTile * kTile * BCT;
-    // scatter: two CELLS-float planes; gather: 32 KB entry staging
-    constexpr int AREA = 2 * CELLS > 8192 ? 2 * CELLS : 8192;
-    __shared__ float area[AREA];
-    __shared__ int sb[kMaxBins];
-    __shared__ int s_last;
-    const int4 wk = work[blockIdx.x];
-    const NuSliceDev& S = sl[wk.x];
-    if (wk.w == 0)
-        grid_gather_rows<BCT, WT>(S, wk.y, gwk[wk.z], glist, gbin, swt, sdcf, y, ws, run, sb, area);
-    else
-        grid_scatter_tile<BCT, WT>(S, wk.y, tiles[wk.z], tlist, sbase, swt, sdcf, y, ws, parts, counters, run,
-                                   area, area + CELLS, s_last);
-}
-
 // Adjoint rows: FFT along g1, crop to N1 positions, deapodisation and (SENSE)
 // the conjugate-coil combine.  In the combine two lanes share an output
 // pixel, each summing half of the coils; the halves are added through a
@@ -1766,7 +1459,6 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     std::vector<float> deap;
     std::vector<uint32_t> sbase;
     std::vector<float> swt, sdcf;
-    std::vector<int> tlist, glist_h, gbin_h;
     std::vector<int2>& csr_ent_h = csr_ent_host_;
     std::vector<int2> gent_h;
     std::map<uint64_t, int> traj_seen;   // trajectory hash -> first slice with it
@@ -1833,15 +1525,6 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                 d.samp_off = e.samp_off;
                 d.csr_off = e.csr_off;
                 d.ent_off = e.ent_off;
-                d.tlist_off = e.tlist_off;
-                tlist_off_.push_back(e.tlist_off);
-                tw_first_.push_back((int)tw_host_.size());
-                tw_count_.push_back(0);
-                gw_first_.push_back((int)gw_host_.size());
-                gw_count_.push_back(0);
-                nslots_.push_back(0);
-                ncounters_.push_back(0);
-                d.tile_first = tw_first_.back();
                 sl_.push_back(d);
                 Vs_.push_back(n1 * n2);
                 Ms_.push_back(d.M);
@@ -1887,7 +1570,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
             sbase.push_back((uint32_t)g0[0][m] | ((uint32_t)g0[1][m] << 16));
             sdcf.push_back(dc ? (float)dc[m] : 1.f);
         }
-        if (ndim == 2) {
+        {
             // CSR gridding: row pointers (per cell) and (sample, weight) entries
             const int64_t ncell = (int64_t)d.G1 * d.G2;
             const int64_t nent = (int64_t)d.M * d.W1 * d.W2;
@@ -1928,15 +1611,6 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
                     }
                 }
             }
-            tlist_off_.push_back((int64_t)tlist.size());
-            d.tlist_off = tlist_off_.back();
-            tw_first_.push_back((int)tw_host_.size());
-            tw_count_.push_back(0);
-            gw_first_.push_back((int)gw_host_.size());
-            gw_count_.push_back(0);
-            nslots_.push_back(0);
-            ncounters_.push_back(0);
-            d.tile_first = tw_first_.back();
             sl_.push_back(d);
             Vs_.push_back(n1 * n2);
             Ms_.push_back(d.M);
@@ -1944,138 +1618,7 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
             dat_start_.push_back(moff);
             voff += n1 * n2;
             moff += d.M;
-            continue;
         }
-        // gridding tiles: each sample joins every tile its footprint touches
-        const int nt1 = (d.G1 + kTile - 1) / kTile, nt2 = (d.G2 + kTile - 1) / kTile;
-        std::vector<std::vector<int>> lists((size_t)nt1 * nt2);
-        for (int m = 0; m < d.M; ++m) {
-            int tl1[8], tl2[8], n1t = 0, n2t = 0;
-            for (int si = 0; si < d.W1; ++si) {
-                int g = g0[0][m] + si;
-                if (g >= d.G1) g -= d.G1;
-                const int t = g / kTile;
-                bool seen = false;
-                for (int q = 0; q < n1t; ++q) seen |= tl1[q] == t;
-                if (!seen) tl1[n1t++] = t;
-            }
-            for (int si = 0; si < d.W2; ++si) {
-                int g = g0[1][m] + si;
-                if (g >= d.G2) g -= d.G2;
-                const int t = g / kTile;
-                bool seen = false;
-                for (int q = 0; q < n2t; ++q) seen |= tl2[q] == t;
-                if (!seen) tl2[n2t++] = t;
-            }
-            for (int a = 0; a < n2t; ++a)
-                for (int b = 0; b < n1t; ++b) lists[(size_t)tl2[a] * nt1 + tl1[b]].push_back(m);
-        }
-        tlist_off_.push_back((int64_t)tlist.size());
-        d.tlist_off = tlist_off_.back();
-        tw_first_.push_back((int)tw_host_.size());
-        int nslots = 0, ncnt = 0;
-        int64_t lstart = 0;
-        // A tile is gridded by the register-run scatter when its samples pile
-        // up on few base cells (dense centre of a variable-density
-        // trajectory), otherwise by the atomic-free gather.
-        const bool can_gather = ndim == 2 && d.G1 >= 24 && d.G2 >= 24;
-        std::vector<char> dense((size_t)nt1 * nt2, 1);
-        for (int t2 = 0; t2 < nt2; ++t2)
-            for (int t1 = 0; t1 < nt1; ++t1) {
-                auto& L = lists[(size_t)t2 * nt1 + t1];
-                // by base cell: runs of samples with the same footprint are
-                // accumulated in registers by the gridding kernel
-                std::stable_sort(L.begin(), L.end(), [&](int a, int b) {
-                    return sbase[(size_t)d.samp_off + a] < sbase[(size_t)d.samp_off + b];
-                });
-                int distinct = 0;
-                for (size_t q = 0; q < L.size(); ++q)
-                    distinct += (q == 0 || sbase[(size_t)d.samp_off + L[q]] != sbase[(size_t)d.samp_off + L[q - 1]]);
-                const int gmax = std::min(gather_max_entries<16>(kwidth), 4096);
-                const bool dn = !can_gather || (L.size() >= 64 && (int)L.size() >= 3 * distinct) ||
-                                (int)L.size() > gmax;
-                dense[(size_t)t2 * nt1 + t1] = dn;
-                if (!dn) continue;
-                const int cnt = (int)L.size();
-                const int np = std::max(1, (cnt + kTileChunk - 1) / kTileChunk);
-                const int first = np > 1 ? nslots : -1;
-                const int ctr = np > 1 ? ncnt++ : -1;
-                for (int p = 0; p < np; ++p) {
-                    TileWork w{};
-                    w.t1 = t1;
-                    w.t2 = t2;
-                    w.start = (int)(lstart + (int64_t)p * kTileChunk);
-                    w.count = std::min(kTileChunk, cnt - p * kTileChunk);
-                    if (w.count < 0) w.count = 0;
-                    w.nparts = np;
-                    w.slot = np > 1 ? first + p : -1;
-                    w.first_slot = first;
-                    w.counter = ctr;
-                    tw_host_.push_back(w);
-                }
-                if (np > 1) nslots += np;
-                tlist.insert(tlist.end(), L.begin(), L.end());
-                lstart += cnt;
-            }
-        tw_count_.push_back((int)tw_host_.size() - tw_first_.back());
-        // gather bins: every (sample, touched tile) pair goes to the bin of its
-        // base cell relative to the tile origin
-        if (can_gather) {
-            const int NB1 = kTile + kwidth - 1;
-            gw_first_.push_back((int)gw_host_.size());
-            for (int t2 = 0; t2 < nt2; ++t2)
-                for (int t1 = 0; t1 < nt1; ++t1) {
-                    if (dense[(size_t)t2 * nt1 + t1]) continue;
-                    const auto& L = lists[(size_t)t2 * nt1 + t1];
-                    std::vector<std::pair<int, int>> eb;  // (bin, sample)
-                    eb.reserve(L.size());
-                    for (int m : L) {
-                        // bin = base cell - tile origin + (W-1), taken modulo G
-                        const int r1p = ((g0[0][m] - t1 * kTile + (kwidth - 1)) % d.G1 + d.G1) % d.G1;
-                        const int r2p = ((g0[1][m] - t2 * kTile + (kwidth - 1)) % d.G2 + d.G2) % d.G2;
-                        if (r1p >= NB1 || r2p >= NB1) continue;  // footprint misses the tile
-                        eb.emplace_back(r2p * NB1 + r1p, m);
-                    }
-                    std::stable_sort(eb.begin(), eb.end(),
-                                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
-                                         return a.first < b.first;
-                                     });
-                    GatherWork gwk{};
-                    gwk.t1 = t1;
-                    gwk.t2 = t2;
-                    gwk.ent_off = (int64_t)glist_h.size();
-                    gwk.bin_off = (int64_t)gbin_h.size();
-                    std::vector<int> cnt(NB1 * NB1 + 1, 0);
-                    for (auto& q : eb) cnt[q.first + 1]++;
-                    for (int i = 0; i < NB1 * NB1; ++i) cnt[i + 1] += cnt[i];
-                    gbin_h.insert(gbin_h.end(), cnt.begin(), cnt.end());
-                    for (auto& q : eb) glist_h.push_back(q.second);
-                    // split heavy tiles by cell rows (no merge needed)
-                    const int n = (int)eb.size();
-                    const int nsplit = std::min(kTile, std::max(1, (n + 47) / 48));
-                    const int rows = (kTile + nsplit - 1) / nsplit;
-                    for (int r = 0; r < kTile; r += rows) {
-                        gwk.r0 = r;
-                        gwk.r1 = std::min(kTile, r + rows);
-                        gw_host_.push_back(gwk);
-                        gw_cost_.push_back((int64_t)n * (gwk.r1 - gwk.r0) / kTile + 16);
-                    }
-                }
-            gw_count_.push_back((int)gw_host_.size() - gw_first_.back());
-        } else {
-            gw_first_.push_back((int)gw_host_.size());
-            gw_count_.push_back(0);
-        }
-        nslots_.push_back(nslots);
-        ncounters_.push_back(ncnt);
-        d.tile_first = tw_first_.back();
-        sl_.push_back(d);
-        Vs_.push_back(n1 * n2);
-        Ms_.push_back(d.M);
-        img_start_.push_back(voff);
-        dat_start_.push_back(moff);
-        voff += n1 * n2;
-        moff += d.M;
     }
     V_total_ = voff;
     M_total_ = moff;
@@ -2090,19 +1633,9 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
     sbase_host_ = sbase;
     swt_host_ = swt;
     sdcf_host_ = sdcf;
-    tlist_.upload(tlist.empty() ? std::vector<int>{0} : tlist, st);
-    tiles_.upload(tw_host_.empty() ? std::vector<TileWork>{TileWork{}} : tw_host_, st);
-    // the transposed data layout is used when any tile is gathered
-    csr_ = ndim == 2;
-    gather_ = csr_ || !gw_host_.empty();
-    if (csr_) {
-        csr_ptr_.upload(csr_ptr_host_, st);
-        csr_ent_.upload(csr_ent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : csr_ent_h, st);
-    }
+    csr_ptr_.upload(csr_ptr_host_, st);
+    csr_ent_.upload(csr_ent_h.empty() ? std::vector<int2>{make_int2(0, 0)} : csr_ent_h, st);
     build_partitions();
-    gwork_.upload(gw_host_.empty() ? std::vector<GatherWork>{GatherWork{}} : gw_host_, st);
-    glist_.upload(glist_h.empty() ? std::vector<int>{0} : glist_h, st);
-    gbin_.upload(gbin_h.empty() ? std::vector<int>{0} : gbin_h, st);
     HIPCHECK(hipStreamSynchronize(st));
 }
 
@@ -2391,16 +1924,16 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     const int ns = (int)sl_.size();
     c.hs = sl_;
     // slice groups (2-D staged paths only)
-    const int gsz = ndim_ == 2 && csr_ && grid_stage_ && degrid_stage_ && slice_group_ > 0 &&
+    const int gsz = ndim_ == 2 && grid_stage_ && degrid_stage_ && slice_group_ > 0 &&
                             slice_group_ < ns ? slice_group_ : ns;
     const int ngroups = (ns + gsz - 1) / gsz;
     auto group_of = [&](int s) { return s / gsz; };
-    int64_t wsoff = 0, poff = 0, coff = 0, ytoff = 0;
-    std::vector<int4> wr, wfr, wc, wd, wg, wga, wcsr;
+    int64_t wsoff = 0, poff = 0, ytoff = 0;
+    std::vector<int4> wr, wfr, wc, wd, wcsr;
     std::vector<GridItem> wgt, wgm;
     std::vector<int4> wdb;
     std::vector<int64_t> wdb_cost;
-    std::vector<int64_t> wga_cost, wcsr_cost, wgt_cost;
+    std::vector<int64_t> wcsr_cost, wgt_cost;
     std::vector<int> wgt_region;
     struct GridUnit {
         int s = 0, lanes = 0;
@@ -2464,12 +1997,9 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         d.ws_chunk = (int64_t)d.G1 * d.G2 * BCT;
         d.ws_off = wsoff;
         wsoff += d.ws_chunk * nchunk;
-        d.part_chunk = csr_ ? (int64_t)gpcells_[s] * BCT : (int64_t)nslots_[s] * kTile * kTile * BCT;
+        d.part_chunk = (int64_t)gpcells_[s] * BCT;
         d.part_off = poff;
         poff += d.part_chunk * nchunk;
-        d.cnt_chunk = csr_ ? 0 : ncounters_[s];
-        d.cnt_off = coff;
-        coff += d.cnt_chunk * nchunk;
         d.yt_off = ytoff;
         ytoff += (int64_t)d.M * BCT * nchunk;
         for (int ch = 0; ch < nchunk; ++ch) {
@@ -2484,16 +2014,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
                     wdb.push_back(make_int4(s, ch, k, 0));
                     wdb_cost.push_back(dblk_host_[(size_t)k].z);
                 }
-            // gridding items: .w = 1 scatter chunk, 0 gather rows
-            for (int t = 0; t < tw_count_[s]; ++t) {
-                wg.push_back(make_int4(s, ch, tw_first_[s] + t, 1));
-                wga_cost.push_back(tw_host_[tw_first_[s] + t].count + 16);
-            }
-            for (int t = 0; t < gw_count_[s]; ++t) {
-                wg.push_back(make_int4(s, ch, gw_first_[s] + t, 0));
-                wga_cost.push_back(gw_cost_[gw_first_[s] + t]);
-            }
-            if (csr_) {
+            {
                 // blocks of NG = 256/BCT cells; a block whose longest row
                 // exceeds 64 entries gets P lane groups per cell (P parts of
                 // each row) and is cut into items of NG/P cells; light blocks
@@ -2544,7 +2065,7 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
             }
         }
     }
-    if (csr_) {
+    if (ndim_ == 2) {
         // Units of one trajectory are paired in slice order (option
         // "nu_grid_pairs"): an item stages its rectangle's entries once for
         // both.  Not when the staged samples of two units would need more
@@ -2726,15 +2247,6 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         by_slice_desc(wcf, slice4);
         by_slice_desc(wgm, [](const GridItem& g) { return g.slice; });
     }
-    // heaviest gridding items first
-    {
-        std::vector<size_t> ord(wg.size());
-        std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return wga_cost[a] > wga_cost[b]; });
-        std::vector<int4> sorted(wg.size());
-        for (size_t i = 0; i < ord.size(); ++i) sorted[i] = wg[ord[i]];
-        wg.swap(sorted);
-    }
     if (pair_cols_) {
         // neighbouring columns share cache lines (a column touches 64 of each
         // 128 bytes): place items 2k and 2k + 1 of every 16 at positions k and
@@ -2794,7 +2306,6 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.w_cols.upload(wc, st);
     c.w_fcols.upload(wcf, st);
     c.w_deg.upload(wd, st);
-    c.w_grid.upload(wg.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wg, st);
     c.w_csr.upload(wcsr.empty() ? std::vector<int4>{make_int4(0, 0, 0, 0)} : wcsr, st);
     c.n_csr = (int)wcsr.size();
     {
@@ -2820,12 +2331,9 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     c.n_db = (int)wdb.size();
     c.lds_db = (size_t)((degrid_merge_ ? 2 : 1) * kDegridTile + W_ - 1) * (kDegridTile + W_ - 1) * BCT * sizeof(float2) +
                (size_t)degrid_block_ * (3 + 2 * W_) * sizeof(float);
-    c.n_grid = (int)wg.size();
-    if (gather_) c.yT.alloc(std::max<int64_t>(ytoff, 1));
+    c.yT.alloc(std::max<int64_t>(ytoff, 1));
     c.ws.alloc(std::max<int64_t>(wsoff, 1));
     c.parts.alloc(std::max<int64_t>(poff, 1));
-    c.counters.alloc(std::max<int64_t>(coff, 1));
-    HIPCHECK(hipMemsetAsync(c.counters.p, 0, sizeof(int) * std::max<int64_t>(coff, 1), st));
     HIPCHECK(hipStreamSynchronize(st));
     cfgs_[key] = std::move(cp);
     return c;
@@ -2922,7 +2430,7 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     auto adjoint_from = [&](const float2* y, float2* x, NuRun r, int g) {
         const auto gt = span(kLGt, g, c.n_gt), gm = span(kLGm, g, c.n_gm), cc = span(kLCols, g, c.n_cols),
                    rw = span(kLRows, g, c.n_rows);
-        if (csr_ && grid_stage_) {
+        if (grid_stage_ && ndim_ == 2) {
             // the item's sample-id row is addressed by its block index:
             // a group's rows start at its first item's
             const int* sid = c.gt_sid.p + (int64_t)gt.first * c.gt_sid_stride;
@@ -2939,17 +2447,10 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                 hipLaunchKernelGGL(nu_grid_merge<BCT>, dim3(gm.second), dim3(256), 0, st, c.w_gm.p + gm.first,
                                    c.parts.p, c.ws.p, r.skip);
             }
-        } else if (csr_)
+        } else {
             hipLaunchKernelGGL(nu_grid_csr<BCT>, dim3(c.n_csr), dim3(256), 0, st, sl, c.w_csr.p, csr_ptr_.p,
                                csr_ent_.p, y, c.ws.p, r);
-        else if (wt4)
-            hipLaunchKernelGGL((nu_grid<BCT, 4>), dim3(c.n_grid), dim3(256), 0, st, sl, c.w_grid.p, tiles_.p,
-                               tlist_.p, gwork_.p, glist_.p, gbin_.p, sbase_.p, swt_.p, sdcf_.p, y, c.ws.p,
-                               c.parts.p, c.counters.p, r);
-        else
-            hipLaunchKernelGGL((nu_grid<BCT, 0>), dim3(c.n_grid), dim3(256), 0, st, sl, c.w_grid.p, tiles_.p,
-                               tlist_.p, gwork_.p, glist_.p, gbin_.p, sbase_.p, swt_.p, sdcf_.p, y, c.ws.p,
-                               c.parts.p, c.counters.p, r);
+        }
         LAUNCHCHECK();
         mark("nu_grid");
         odd_ramp(true, r.skip);
@@ -3011,8 +2512,8 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     }
     if (mode == 1) {
         NuRun rf = run;
-        rf.ylayout = gather_ ? 1 : 0;
-        float2* ybuf = gather_ ? c.yT.p : c.ydata.p;
+        rf.ylayout = 1;
+        float2* ybuf = c.yT.p;
         NuRun ra = run;
         ra.W = nullptr;
         ra.ylayout = rf.ylayout;
@@ -3032,17 +2533,12 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     }
     NuRun ra = run;
     ra.W = nullptr;
-    if (gather_) {
-        ra.ylayout = 0;
-        hipLaunchKernelGGL(nu_transpose_y<BCT>, dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, in, c.yT.p, ra);
-        LAUNCHCHECK();
-        mark("nu_transpose_y");
-        ra.ylayout = 1;
-        adjoint_from(c.yT.p, out, ra, -1);
-    } else {
-        ra.ylayout = 0;
-        adjoint_from(in, out, ra, -1);
-    }
+    ra.ylayout = 0;
+    hipLaunchKernelGGL(nu_transpose_y<BCT>, dim3(c.n_deg), dim3(256), 0, st, sl, c.w_deg.p, in, c.yT.p, ra);
+    LAUNCHCHECK();
+    mark("nu_transpose_y");
+    ra.ylayout = 1;
+    adjoint_from(c.yT.p, out, ra, -1);
 }
 
 void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps, const float* W,
@@ -3061,7 +2557,6 @@ void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const floa
     }
     r.skip = mode == 1 || mode == 3 ? skip_ : nullptr;
     if (mode == 3) mode = 2;
-    if (mode == 1 && !gather_) c.ydata.ensure((size_t)(c.sense ? M_total_ * c.C * c.nrep : M_total_ * c.nrep));
     if (big_) {
         switch (c.BCT) {
             case 1: launch<1, true>(c, adjoint, r, in, out, mode); break;

@@ -17,7 +17,6 @@ struct NuSliceDev {
     int tw1, tw2;       // offset into the twiddle pool (float2)
     int deap1, deap2;   // offset into the deapodisation pool (float)
     int rpw, cpw;       // rows / columns per work-group in the FFT passes
-    int tile_first;     // first gridding tile of this slice in the tile pool
     int frpw;           // rows per work-group of the forward row FFT
     int lfl, lcl;       // log2 transforms (coils) per work-group: forward rows, columns
     int fip;            // bit 0: row FFTs, bit 1: column FFTs in a single LDS buffer
@@ -25,42 +24,17 @@ struct NuSliceDev {
                         // 16 transforms): used for fac2/tw2 when the column pass has them
     int odd;            // bit 0: N1 odd, bit 1: N2 odd (the grid takes the odd-size ramp)
     int64_t samp_off;   // first sample of this slice in the sample pools
-    int64_t tlist_off;  // first entry of this slice in the tile sample list
     int64_t img_off;    // image-side offset (elements) for the current apply
     int64_t maps_off;   // coil-map offset (elements)
     int64_t dat_off;    // data-side offset (elements)
     int64_t ws_off;     // workspace offset of chunk 0 (elements)
     int64_t ws_chunk;   // workspace elements per chunk = G2*G1*BCT
-    int64_t part_off;   // partial-tile slot offset of chunk 0
-    int64_t part_chunk; // partial-tile elements per chunk
-    int64_t cnt_off;    // counter offset of chunk 0
-    int64_t cnt_chunk;  // counters per chunk
+    int64_t part_off;   // partial blocks of the dense rectangles: offset of chunk 0
+    int64_t part_chunk; // partial-block elements per chunk
     int64_t yt_off;     // transposed data ([M][BCT] per chunk) offset of chunk 0
     int64_t csr_off;    // first row pointer of this slice (CSR gridding), G1*G2 + 1 of them
     int64_t ent_off;    // first CSR entry of this slice
 };
-
-// One gather-gridding work item: rows [r0, r1) of the 16x16 cells of tile
-// (t1, t2).  The tile's samples are bucketed by base cell relative to the
-// tile origin into (16+W-1)^2 bins (CSR offsets at bin_off).
-struct GatherWork {
-    int t1, t2, r0, r1;
-    int64_t ent_off;    // first entry of the tile in the gather list
-    int64_t bin_off;    // first bin offset of the tile
-};
-
-// One gridding work item: a chunk of the sample list of one tile.
-struct TileWork {
-    int t1, t2;         // tile coordinates (16x16 grid cells)
-    int start, count;   // range in the slice's tile sample list
-    int slot;           // partial slot (relative to the slice) or -1 if the tile is not split
-    int nparts;         // number of chunks of this tile
-    int first_slot;     // first partial slot of this tile
-    int counter;        // split-tile counter index (relative to the slice)
-};
-
-constexpr int kTile = 16;
-constexpr int kTileChunk = 512;
 
 // One rectangle of grid cells for 2-D CSR gridding, with the union of the
 // samples that reach it (sorted, at most stage_samples_) and the cells'
@@ -153,7 +127,6 @@ class NufftOp : public mm_op {
     // one option at a time, and keeps the fastest.
     bool autotune_sense(const float2* maps, int C, const float* W, int iters);
     void set_skip(const int* flags) override { skip_ = flags; }
-    bool gather() const { return gather_; }
     int ndim() const { return ndim_; }
     const std::vector<int64_t>& slice_V() const { return Vs_; }
     const std::vector<int64_t>& slice_M() const { return Ms_; }
@@ -168,14 +141,11 @@ class NufftOp : public mm_op {
         int64_t nrep = 0;
         std::vector<NuSliceDev> hs;
         DevBuf<NuSliceDev> sl;
-        DevBuf<int4> w_rows, w_frows, w_cols, w_deg, w_grid;
+        DevBuf<int4> w_rows, w_frows, w_cols, w_deg;
         DevBuf<int4> w_fcols;    // forward column items (w_cols: adjoint)
-        int n_rows = 0, n_frows = 0, n_cols = 0, n_deg = 0, n_grid = 0;
+        int n_rows = 0, n_frows = 0, n_cols = 0, n_deg = 0;
         size_t lds_rows = 0, lds_cols = 0;
-        DevBuf<float2> ws, parts, yT, ydata;
-        DevBuf<int> counters;
-        DevBuf<int4> w_gath;
-        int n_gath = 0;
+        DevBuf<float2> ws, parts, yT;
         DevBuf<int4> w_csr;      // CSR gridding items: (slice, chunk, first cell, cells)
         int n_csr = 0;
         // staged CSR gridding items: (slice, chunk, tile, lp | c0 << 8 | cells << 16)
@@ -232,22 +202,16 @@ class NufftOp : public mm_op {
     std::vector<NuSliceDev> sl_;          // static part of each slice
     std::vector<int64_t> Vs_, Ms_, img_start_, dat_start_;
     int64_t V_total_ = 0, M_total_ = 0;
-    // tile work per slice: first index / count in tw_all_
-    std::vector<int> tw_first_, tw_count_, nslots_, ncounters_;
-    std::vector<TileWork> tw_host_;
     // static device pools
     DevBuf<FftFactors> facs_;
     DevBuf<float2> twid_;
     DevBuf<float> deap_;
     DevBuf<uint32_t> sbase_;
     DevBuf<float> swt_, sdcf_;
-    DevBuf<int> tlist_;
-    DevBuf<TileWork> tiles_;
-    std::vector<int64_t> tlist_off_;       // start of each slice's list in tlist_
     std::map<std::pair<int64_t, int>, std::unique_ptr<Cfg>> cfgs_;
-    // CSR gridding (2-D): per cell the (sample, weight) pairs that reach it,
-    // in sample order; row pointers are relative to the slice's ent_off
-    bool csr_ = false;
+    // CSR gridding (1-D and 2-D): per cell the (sample, weight) pairs that
+    // reach it, in sample order; row pointers are relative to the slice's
+    // ent_off
     bool degrid_t_ = true;   // BCT lanes per sample in degridding (option "nu_degrid_t")
     // XCD-aware order of the gridding items (option "nu_xcd_order"); off: on
     // the MI355X it measured slower than the heaviest-first order (318 vs 297 us
@@ -300,13 +264,6 @@ class NufftOp : public mm_op {
     DevBuf<int> gptr_;
     std::vector<int> gstage_host_;   // staged sample ids of the rectangles (host)
     DevBuf<int2> gent_;
-    // gather gridding (2-D, every grid side >= 24)
-    bool gather_ = false;
-    std::vector<GatherWork> gw_host_;
-    std::vector<int> gw_first_, gw_count_;
-    std::vector<int64_t> gw_cost_;
-    DevBuf<GatherWork> gwork_;
-    DevBuf<int> glist_, gbin_;
 };
 
 // Plan constants from the closed forms of nufftOp.m:146-182 (host, FP64).
