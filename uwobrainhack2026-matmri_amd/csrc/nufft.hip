@@ -850,9 +850,12 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
         float w[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) w[u] = wsrc[min((int)threadIdx.x + u * 256, nw - 1)];
-        float2 v[4];
+        // NV values per lane in the first round: the whole region of two
+        // merged 8x8 tiles (19 x 11 cells x 8 transforms) in one round trip
+        constexpr int NV = 8;
+        float2 v[NV];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = zg[cell(min((int)threadIdx.x + u * 256, nst - 1))];
+        for (int u = 0; u < NV; ++u) v[u] = zg[cell(min((int)threadIdx.x + u * 256, nst - 1))];
         if ((int)threadIdx.x < n) {
             s_id[threadIdx.x] = pid;
             s_base[threadIdx.x] = pb;
@@ -862,10 +865,10 @@ __global__ __launch_bounds__(256) void nu_degrid_tile(const NuSliceDev* __restri
         for (int u = 0; u < 4; ++u)
             if ((int)threadIdx.x + u * 256 < nw) s_w[threadIdx.x + u * 256] = w[u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < NV; ++u)
             if ((int)threadIdx.x + u * 256 < nst) lds[threadIdx.x + u * 256] = v[u];
         for (int i0 = threadIdx.x + 4 * 256; i0 < nw; i0 += 256) s_w[i0] = wsrc[i0];
-        for (int i0 = threadIdx.x + 4 * 256; i0 < nst; i0 += 4 * 256) {
+        for (int i0 = threadIdx.x + NV * 256; i0 < nst; i0 += 4 * 256) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) v[u] = zg[cell(min(i0 + u * 256, nst - 1))];
 #pragma unroll
@@ -1137,10 +1140,23 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
 #pragma unroll
     for (int k = 0; k < NPRE; ++k) m[k] = sid[min(q0 + k * SPR, sid_stride - 1)];   // unconditional loads
     const GridItem T = work[blockIdx.x];
+    const int cells = T.w * T.h;
+    // the rectangle's first 1024 (local sample, weight) entries and first
+    // 512 row pointers: plan tables addressed by the item record alone, so
+    // their loads (unconditional, from clamped indices) are in flight with
+    // the sample ids and the samples, not after them
+    const int2* te = tent + T.ent_off;
+    const int* tp = tptr + T.ptr_off;
+    const int ne_last = max(T.n_ent - 1, 0);
+    int2 e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) e[u] = te[min((int)threadIdx.x + u * 256, ne_last)];
+    int pr[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) pr[u] = tp[min((int)threadIdx.x + u * 256, cells)];
     const bool act0 = !(skip && skip[T.slice]);
     const bool act1 = SG > 1 && T.slice2 >= 0 && !(skip && skip[T.slice2]);
     if (!act0 && !act1) return;
-    const int cells = T.w * T.h;
     const int nsv = T.n_stage * TT;
     float2* sv = lds;                              // [n_stage][TT]
     int2* se = (int2*)(lds + nsv);                 // [n_ent]
@@ -1154,25 +1170,19 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
         float2 v[NPRE];
 #pragma unroll
         for (int k = 0; k < NPRE; ++k) v[k] = ych[(int64_t)m[k] * BCT];
-        // the first 1024 entries are loaded before any LDS store
-        const int2* te = tent + T.ent_off;
-        int2 e[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = threadIdx.x + u * 256;
-            e[u] = i < T.n_ent ? te[i] : make_int2(0, 0);
-        }
 #pragma unroll
         for (int k = 0; k < NPRE; ++k)
             if (q0 + k * SPR < T.n_stage) sv[(q0 + k * SPR) * TT + jb] = v[k];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if ((int)threadIdx.x + u * 256 < T.n_ent) se[threadIdx.x + u * 256] = e[u];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if ((int)threadIdx.x + u * 256 <= cells) sp[threadIdx.x + u * 256] = pr[u];
         for (int q = q0 + NPRE * SPR; q < T.n_stage; q += SPR) sv[q * TT + jb] = ych[(int64_t)sid[q] * BCT];
         for (int i = threadIdx.x + 4 * 256; i < T.n_ent; i += 256) se[i] = te[i];
+        for (int i = threadIdx.x + 2 * 256; i <= cells; i += 256) sp[i] = tp[i];
     }
-    const int* tp = tptr + T.ptr_off;
-    for (int i = threadIdx.x; i <= cells; i += 256) sp[i] = tp[i];
     __syncthreads();
     // gather: CPL consecutive transforms of one unit per lane (one
     // 8*CPL-byte LDS read of the sample's values per entry), TT/CPL lanes per
