@@ -141,30 +141,46 @@ def test_cgne_config1_full_size_20_iterations(mm):
 
 def test_cgne_relative_threshold(mm):
     """Stop at |r|^2 < 1e-8 |b|^2 (relative residual 1e-4) with cgne's default
-    x0 = b.  At this residual the solution is sensitive to FP32 rounding of the
-    operator's input/output alone (the FP64 oracle with only that rounding
-    moves by ~1e-3 and needs a few more iterations; see
-    test_oracle_solvers.py::test_cgne_fp32_io_sensitivity), so the device
-    solve is held to that band and to the same error against the phantom."""
+    x0 = b.  Near this residual single-precision rounding of the operator
+    changes how many iterations reach it: the FP64 oracle loop needs n_o, the
+    same loop with its operator evaluated in complex64 (_single_oracle) n_s.
+    The device (FP32 operator, FP64 scalars) must stop with status 1 at the
+    first iteration below the threshold, after n_o .. n_s + 3 iterations.
+    Its iterate is then compared with the FP64 oracle run for the same number
+    of iterations n_g: within 3 max(d, ds), where d and ds are the distances
+    of the FP32-I/O-rounded and the single-precision oracle runs of n_g
+    iterations from that FP64 run (computed here on the same problem).  The
+    error against the phantom is printed; the NRMSE bound limits its change
+    (triangle inequality)."""
     print("[case] cgne default x0, relative residual 1e-4", flush=True)
     So, Ro, Sg, Rg, y, ph = _problem(mm, seed=6)
     A = lambda x, tr: o_mrSampFunc(x, tr, So, Ro)
+    A32 = lambda x, tr: _c64(o_mrSampFunc(_c64(x), tr, So, Ro))
+    As = _single_oracle(So, Ro)
     b = A(y, "transp")
     bb = np.real(np.vdot(b, b))
-    xo, ro, *_, so = o_cgne(A, y, None, 80, {"resThresh": 1e-8 * bb, "xChangeThresh": None})
+    thr = {"resThresh": 1e-8 * bb, "xChangeThresh": None}
+    _, ro, *_, so = o_cgne(A, y, None, 80, thr)
+    _, rs, *_, ss = o_cgne(As, y, None, 80, thr)
     xg, rg, _, _, _, _, st = mm.cgne(mm.mrSampHandle(Sg, Rg), y, None, 80,
                                      {"resThresh": None, "resThreshRel": 1e-8, "xChangeThresh": None})
-    nrmse = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    rg = np.asarray(rg)
+    n_o, n_s, n_g = len(ro) - 1, len(rs) - 1, len(rg) - 1
+    fixed = {"resThresh": None, "xChangeThresh": None}
+    xo, *_ = o_cgne(A, y, None, n_g, fixed)
+    x3, *_ = o_cgne(A32, y, None, n_g, fixed)
+    xs, *_ = o_cgne(As, y, None, n_g, fixed)
+    dist = lambda u: np.linalg.norm(u - xo) / np.linalg.norm(xo)
+    d, ds, nrmse = dist(x3), dist(xs), dist(xg)
     eo = np.linalg.norm(xo - ph) / np.linalg.norm(ph)
     eg = np.linalg.norm(xg - ph) / np.linalg.norm(ph)
-    print(f"  oracle it {len(ro) - 1} status {so}; device it {len(rg) - 1} status {st}; nrmse {nrmse:.2e}; "
+    print(f"  iterations: FP64 oracle {n_o}, single-precision oracle {n_s}, device {n_g} (status {st}); at "
+          f"{n_g} iterations: drift d {d:.2e} ds {ds:.2e}, device {nrmse:.2e} ({nrmse / max(d, ds):.2f} x); "
           f"error vs phantom {eo:.4e} / {eg:.4e}", flush=True)
-    assert so == 1 and st == 1
-    assert len(ro) <= len(rg) <= len(ro) + 10
-    assert nrmse < 2e-2
-    # the extra FP32 iterations move the noisy solution by a few percent of
-    # its (~11 %) error against the phantom
-    assert abs(eg - eo) / eo < 5e-2
+    assert so == 1 and ss == 1 and st == 1
+    assert rg[-1] < 1e-8 * bb * (1 + 1e-5) and rg[-2] >= 1e-8 * bb * (1 - 1e-5)
+    assert n_o <= n_g <= n_s + 3
+    assert nrmse < 3 * max(d, ds)
 
 
 def test_cgne_multislice_equals_single(mm):
