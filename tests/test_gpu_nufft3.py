@@ -12,7 +12,7 @@ import pytest
 from conftest import load_pkg
 from oracle.fftnc import fftnc
 from oracle.mrsamp import mrSampFunc as o_mrSampFunc
-from oracle.nufft import NufftPlan
+from oracle.nufft import NufftPlan, ToepPlan
 from oracle.rcvr import RcvrPlan
 from oracle.solvers import cgne as o_cgne
 
@@ -113,6 +113,51 @@ def test_nufft3_sense_cgne_matches_oracle(mm):
     assert nrmse < 1e-4
 
 
+@pytest.mark.parametrize("imN,nrep", [((12, 10, 8), 1), ((9, 10, 7), 2)])
+def test_nufft3_toeplitz_vs_oracle(mm, imN, nrep):
+    """prepToep of a 3-D nufftOp (nufftOp.m:410-432, applied :348-352): both
+    directions against the oracle's Toeplitz form (1e-5); for even sides also
+    within the reference's own bound (rms < 1e-3) of the accurate normal
+    operator S0'*(S0*x) (os 2, W 8, @nufftOp/tests.m:104-124)."""
+    print(f"[case] 3-D Toeplitz imN={imN} nrep={nrep}", flush=True)
+    rng = np.random.default_rng(97 + sum(imN))
+    kloc = rng.random((int(0.5 * np.prod(imN)), 3)) - 0.5
+    dcf = rng.random(kloc.shape[0]) + 0.5
+    T = mm.nufftOp(list(imN), kloc, dcf, 1, 1.5, 6).prepToep()
+    x = crand(rng, imN + ((nrep,) if nrep > 1 else ()))
+    want = ToepPlan(list(imN), kloc, dcf, 1.5, 6).apply(x)
+    e, e_h = rel(np.asarray(T * x), want), rel(np.asarray(T.H * x), want)
+    print(f"  vs oracle {e:.2e} / adjoint {e_h:.2e}", flush=True)
+    assert e < TOL and e_h < TOL
+    if all(n % 2 == 0 for n in imN):
+        S0 = NufftPlan(list(imN), kloc, dcf, 2, 8)
+        rms = np.sqrt(np.mean(np.abs(np.asarray(T * x) - S0.adjoint(S0.forward(x))) ** 2))
+        print(f"  rms vs S0'S0 {rms:.2e}", flush=True)
+        assert rms < 1e-3
+
+
+def test_nufft3_toeplitz_sense(mm):
+    """mrSampFunc on a 3-D prepToep'd nufftOp: forward T(R x) (coil volumes),
+    adjoint R'(T y), against the oracle; cgne refuses the operator."""
+    print("[case] 3-D Toeplitz SENSE 10x8x8 C=3", flush=True)
+    rng = np.random.default_rng(99)
+    imN, C = (10, 8, 8), 3
+    kl = rng.random((300, 3)) - 0.5
+    maps = crand(rng, imN + (C,))
+    T = mm.nufftOp(list(imN), kl, None, 1, 1.5, 4).prepToep()
+    E = mm.senseOp(T, mm.rcvrOp(maps))
+    Tp = ToepPlan(list(imN), kl, None, 1.5, 4)
+    x = crand(rng, imN)
+    y = crand(rng, imN + (C,))
+    want_f = np.stack([Tp.apply(maps[..., c] * x) for c in range(C)], axis=-1)
+    want_a = sum(np.conj(maps[..., c]) * Tp.apply(y[..., c]) for c in range(C))
+    e_f, e_a = rel(E * x, want_f), rel(E.H * y, want_a)
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+    with pytest.raises(NotImplementedError):
+        mm.cgne(E, y, None, 3, {})
+
+
 def test_nufft3_rejects_unsupported(mm):
     """Grid sides above 512 and multi-volume 3-D plans raise, not fault."""
     kl = np.zeros((4, 3))
@@ -120,5 +165,3 @@ def test_nufft3_rejects_unsupported(mm):
         mm.nufftOp([400, 8, 8], kl, None, 1, 1.5, 4)
     with pytest.raises(NotImplementedError):
         mm.nufftOp.batch([([8, 8, 8], kl, None), ([8, 8, 8], kl, None)], 1.5, 4)
-    with pytest.raises(NotImplementedError):
-        mm.nufftOp([8, 8, 8], kl, None, 1, 1.5, 4).prepToep()

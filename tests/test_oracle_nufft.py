@@ -318,7 +318,7 @@ def test_odd_size_closed_form_rejects_mistakes():
     assert np.linalg.norm((y - want)[keep]) / np.linalg.norm(want[keep]) < 2e-2
 
 
-@pytest.mark.parametrize("imN", [[11, 13], [9], [15, 10], [12, 10]])
+@pytest.mark.parametrize("imN", [[11, 13], [9], [15, 10], [12, 10], [6, 5, 4], [6, 4, 4]])
 def test_toeplitz_odd_sizes_vs_exact_nudft_normal(imN):
     """prepToep on odd sides (nufftOp.m:410-432).  The point-spread image comes
     from the adjoint NUFFT on the doubled grid 2N, whose ramp constant

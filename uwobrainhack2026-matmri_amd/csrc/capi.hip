@@ -317,9 +317,13 @@ int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd) {
 
 int mm_nufft_set_toeplitz(mm_op* op, const float* psf) {
     return guarded([&] {
+        set_device(op->ctx);
+        if (auto* n3 = dynamic_cast<Nufft3Op*>(op)) {
+            n3->set_toeplitz(reinterpret_cast<const float2*>(psf));
+            return;
+        }
         auto* n = dynamic_cast<NufftOp*>(op);
         if (!n) fail(MM_ERR_ARG, "prepToep: not a NUFFT operator");
-        set_device(op->ctx);
         n->set_toeplitz(reinterpret_cast<const float2*>(psf));
     });
 }

@@ -221,6 +221,18 @@ __global__ __launch_bounds__(256) void n3_odd_ramp(Nu3Geom g, float2* __restrict
     ws[e] = make_float2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
 }
 
+// Toeplitz middle stage: the grid of one chunk times the kernel spectrum
+// (one value per cell, shared by the chunk's BCT transforms).
+template <int BCT>
+__global__ __launch_bounds__(256) void n3_toep_mul(Nu3Geom g, const float2* __restrict__ km,
+                                                   float2* __restrict__ ws) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= g.chunk) return;
+    const float2 k = km[e / BCT];
+    const float2 v = ws[e];
+    ws[e] = make_float2(v.x * k.x - v.y * k.y, v.x * k.y + v.y * k.x);
+}
+
 // Degridding: lane = (sample, transform), transform fastest, so the BCT
 // lanes of a sample read each tap's BCT consecutive values; W^3 taps in
 // (s3, s2, s1) order with weight w3 w2 w1, then dcf.  The block's results
@@ -461,7 +473,7 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
     const int64_t ncell = (int64_t)g.G1 * g.G2 * g.G3;
     g.chunk = ncell * BCT;
     ws_.ensure((size_t)g.chunk);
-    if (adjoint) yT_.ensure((size_t)g.M * BCT);
+    if (adjoint && !toep_) yT_.ensure((size_t)g.M * BCT);
     const unsigned rows_blocks = (unsigned)((g.N2 * g.N3 + LPG - 1) / LPG);
     const unsigned gx = (unsigned)((g.G1 + LPG - 1) / LPG);
     const unsigned samp_blocks = (unsigned)((g.M + SPB - 1) / SPB);
@@ -470,6 +482,31 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
     const int64_t nchunk = (nrep + BCT - 1) / BCT;
     for (int64_t ch = 0; ch < nchunk; ++ch) {
         const int lanes = (int)std::min<int64_t>(BCT, nrep - ch * BCT);
+        if (toep_) {
+            // both directions: forward FFTs, spectrum multiply, inverse FFTs
+            hipLaunchKernelGGL(n3_rows_fwd<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+                               twid_.p + tw_off_[0], toep_deap_.p, in, ws_.p, (int)ch, lanes);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+                               facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL(n3_toep_mul<BCT>, dim3(elem_blocks), dim3(256), 0, st, g, (const float2*)toep_k_.p,
+                               ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+                               facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
+            LAUNCHCHECK();
+            hipLaunchKernelGGL(n3_rows_adj<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+                               twid_.p + tw_off_[0], toep_deap_.p, (const float2*)ws_.p, out, (int)ch, lanes);
+            LAUNCHCHECK();
+            continue;
+        }
         if (!adjoint) {
             hipLaunchKernelGGL(n3_rows_fwd<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
                                twid_.p + tw_off_[0], deap_.p, in, ws_.p, (int)ch, lanes);
@@ -509,6 +546,87 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
             LAUNCHCHECK();
         }
     }
+}
+
+void Nufft3Op::set_toeplitz(const float2* psf) {
+    if (!psf) fail(MM_ERR_ARG, "nufftOp: NULL point-spread image");
+    const int N[3] = {g_.N1, g_.N2, g_.N3}, G[3] = {g_.G1, g_.G2, g_.G3};
+    for (int d = 0; d < 3; ++d)
+        if (G[d] < 2 * N[d] - 1) fail(MM_ERR_ARG, "nufftOp: Toeplitz form needs a grid of at least 2N - 1");
+    const int P[3] = {2 * N[0], 2 * N[1], 2 * N[2]};
+    const int L[3] = {2 * N[0] - 1, 2 * N[1] - 1, 2 * N[2] - 1};
+    const double alpha = 8.0 / std::sqrt((double)P[0] * P[1] * P[2]);
+    // h(D) = alpha psf[N + D] for |D_d| <= N_d - 1 (the lags between two
+    // voxels of the cropped volume); K = sum_D h(D) exp(+2 pi i g.D / G), the
+    // forward passes' sign, one side at a time in FP64
+    auto ex = [](int Gd) {
+        std::vector<std::pair<double, double>> e((size_t)Gd);
+        for (int k = 0; k < Gd; ++k) e[(size_t)k] = {std::cos(2 * M_PI * k / Gd), std::sin(2 * M_PI * k / Gd)};
+        return e;
+    };
+    const auto e1 = ex(G[0]), e2 = ex(G[1]), e3 = ex(G[2]);
+    auto tw = [](const std::vector<std::pair<double, double>>& e, int gd, int D, int Gd) {
+        return e[(size_t)((((int64_t)gd * D) % Gd + Gd) % Gd)];
+    };
+    // side 1: A1[j3][j2][g1]
+    std::vector<double> ar((size_t)L[2] * L[1] * G[0]), ai(ar.size());
+    for (int j3 = 0; j3 < L[2]; ++j3)
+        for (int j2 = 0; j2 < L[1]; ++j2) {
+            const int64_t col = (int64_t)(N[1] + j2 - (N[1] - 1)) * P[0] + (int64_t)(N[2] + j3 - (N[2] - 1)) * P[0] * P[1];
+            for (int g1 = 0; g1 < G[0]; ++g1) {
+                double sr = 0, si = 0;
+                for (int j1 = 0; j1 < L[0]; ++j1) {
+                    const int D1 = j1 - (N[0] - 1);
+                    const float2 h = psf[col + N[0] + D1];
+                    const auto w = tw(e1, g1, D1, G[0]);
+                    sr += h.x * w.first - h.y * w.second;
+                    si += h.x * w.second + h.y * w.first;
+                }
+                const size_t o = ((size_t)j3 * L[1] + j2) * G[0] + g1;
+                ar[o] = sr;
+                ai[o] = si;
+            }
+        }
+    // side 2: A2[j3][g2][g1]
+    std::vector<double> br((size_t)L[2] * G[1] * G[0]), bi(br.size());
+    for (int j3 = 0; j3 < L[2]; ++j3)
+        for (int g2 = 0; g2 < G[1]; ++g2)
+            for (int g1 = 0; g1 < G[0]; ++g1) {
+                double sr = 0, si = 0;
+                for (int j2 = 0; j2 < L[1]; ++j2) {
+                    const auto w = tw(e2, g2, j2 - (N[1] - 1), G[1]);
+                    const size_t o = ((size_t)j3 * L[1] + j2) * G[0] + g1;
+                    sr += ar[o] * w.first - ai[o] * w.second;
+                    si += ar[o] * w.second + ai[o] * w.first;
+                }
+                const size_t q = ((size_t)j3 * G[1] + g2) * G[0] + g1;
+                br[q] = sr;
+                bi[q] = si;
+            }
+    // side 3: K[g3][g2][g1]
+    std::vector<float2> km((size_t)G[2] * G[1] * G[0]);
+    for (int g3 = 0; g3 < G[2]; ++g3)
+        for (int g2 = 0; g2 < G[1]; ++g2)
+            for (int g1 = 0; g1 < G[0]; ++g1) {
+                double sr = 0, si = 0;
+                for (int j3 = 0; j3 < L[2]; ++j3) {
+                    const auto w = tw(e3, g3, j3 - (N[2] - 1), G[2]);
+                    const size_t o = ((size_t)j3 * G[1] + g2) * G[0] + g1;
+                    sr += br[o] * w.first - bi[o] * w.second;
+                    si += br[o] * w.second + bi[o] * w.first;
+                }
+                km[((size_t)g3 * G[1] + g2) * G[0] + g1] = make_float2((float)(alpha * sr), (float)(alpha * si));
+            }
+    // flat scaling 1/sqrt(G1 G2 G3) on both sides (unitary FFT pair)
+    std::vector<float> dflat((size_t)(N[0] + N[1] + N[2]), 1.f);
+    const float f = (float)(1.0 / std::sqrt((double)G[0] * G[1] * G[2]));
+    for (int i = 0; i < N[0]; ++i) dflat[(size_t)i] = f;
+    hipStream_t st = ctx->stream;
+    toep_k_.upload(km, st);
+    toep_deap_.upload(dflat, st);
+    HIPCHECK(hipStreamSynchronize(st));
+    toep_ = true;
+    ++gen_;
 }
 
 void Nufft3Op::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {

@@ -25,8 +25,17 @@ class Nufft3Op : public mm_op {
     Nufft3Op(mm_ctx* ctx, const int64_t* imN, int64_t M, const double* kloc, const double* dcf, double os,
              int kwidth);
     int64_t n_image() const override { return g_.V; }
-    int64_t n_data() const override { return g_.M; }
+    int64_t n_data() const override { return toep_ ? g_.V : g_.M; }
     void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
+    // Toeplitz form (nufftOp.prepToep, operators/@nufftOp/nufftOp.m:410-432,
+    // applied at :348-352): `psf` is the point-spread volume A2'(dcf) on the
+    // doubled grid 2N1 x 2N2 x 2N3 (column-major).  Afterwards both
+    // directions apply crop(h * pad(x)), h the psf scaled by
+    // 2^3 / sqrt(8 N1 N2 N3): FFTs along the three sides on this plan's grid
+    // (G_d >= 2 N_d - 1), a multiply by the spectrum of h (formed on the host
+    // in FP64), inverse FFTs.
+    void set_toeplitz(const float2* psf);
+    bool is_toeplitz() const { return toep_; }
 
    private:
     template <int BCT>
@@ -41,6 +50,9 @@ class Nufft3Op : public mm_op {
     DevBuf<int> rp_;                     // CSR gridding: row pointers per grid cell
     DevBuf<int2> ent_;                   // (sample, w1 w2 w3 dcf as float bits), sample order
     DevBuf<float2> ws_, yT_;             // one chunk's grid; transposed data
+    bool toep_ = false;
+    DevBuf<float2> toep_k_;              // spectrum of h, [g3][g2][g1]
+    DevBuf<float> toep_deap_;            // flat 1 / sqrt(G1 G2 G3) on side 1, ones elsewhere
 };
 
 }  // namespace mm

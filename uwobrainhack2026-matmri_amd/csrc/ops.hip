@@ -208,6 +208,7 @@ SenseOp::SenseOp(mm_ctx* c, mm_op* S, mm_op* R, const float* Wh) : S_(S), R_(R) 
     nu_ = dynamic_cast<NufftOp*>(S);
     if (nu_ && C_ > 16) nu_ = nullptr;  // unfused path
     hio_ = dynamic_cast<HioOp*>(S);
+    nu3_ = dynamic_cast<Nufft3Op*>(S);
     if (Wh) {
         Wd_.upload(Wh, (size_t)n_data(), ctx->stream);
         HIPCHECK(hipStreamSynchronize(ctx->stream));

@@ -4,6 +4,7 @@
 #include "fft.hpp"
 #include "hio.hpp"
 #include "nufft.hpp"
+#include "nufft3.hpp"
 
 namespace mm {
 
@@ -54,7 +55,7 @@ class SenseOp : public mm_op {
     void set_skip(const int* flags) override { S_->set_skip(flags); }
     uint64_t config_gen() const override { return gen_ + S_->config_gen() + R_->config_gen(); }
     // the sampling operator is a NUFFT in the Toeplitz form (prepToep)
-    bool toeplitz() const { return nu_ && nu_->is_toeplitz(); }
+    bool toeplitz() const { return (nu_ && nu_->is_toeplitz()) || (nu3_ && nu3_->is_toeplitz()); }
     mm_op* sampling() const { return S_; }
     int coils() const { return C_; }
     // per-kernel times of the last normal-operator profile
@@ -66,6 +67,7 @@ class SenseOp : public mm_op {
     mm_op* R_;
     int C_ = 1;
     NufftOp* nu_ = nullptr;
+    Nufft3Op* nu3_ = nullptr;   // 3-D NUFFT: composed through the generic path
     HioOp* hio_ = nullptr;
     DevBuf<float2> tmp_;
     DevBuf<float> Wd_;

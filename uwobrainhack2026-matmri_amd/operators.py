@@ -167,7 +167,7 @@ class nufftOp(LinearOp):
     ``kwidth`` defaults to 6 and ``os`` to 1.5 like the reference properties
     (nufftOp.m:65-66).  Only the GPU path exists (``useGPU`` is accepted and
     ignored); 1-D, 2-D and 3-D images are implemented (3-D: one volume per
-    plan, grid sides up to 512, no Toeplitz form).
+    plan, grid sides up to 512).
     """
 
     def __init__(self, imN, kloc, dcf=None, useGPU=1, os=1.5, kwidth=6):
@@ -244,8 +244,6 @@ class nufftOp(LinearOp):
         returned operator applies it on a grid of at least 2N per side."""
         if self.isToep:
             return self
-        if len(self.imN[0]) == 3:
-            raise NotImplementedError("prepToep of a 3-D nufftOp is not implemented")
         if tos is None:
             tos = 0.5 * (self.os - 1) + 1
         if kwidth is None:
