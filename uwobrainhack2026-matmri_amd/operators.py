@@ -317,7 +317,9 @@ class sampHighOrder(LinearOp):
                  useSingle=0, approach=0, opt=None):
         self.approach = 0 if approach in (None, 2, 3) else int(approach)
         if self.approach == 1:
-            self._init_interp(b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask, dict(opt or {}))
+            self._sampTimes = sampTimes
+            self._opt = dict(opt or {})
+            self._init_interp(b0, sampTimes, phs_spha, phs_conc, phs_grid, b0mask, self._opt)
             return
         gx = np.asarray(phs_grid["x"], np.float64)
         self.imSize = gx.shape
@@ -448,6 +450,19 @@ class sampHighOrder(LinearOp):
         self.image_shape = tuple(self.imSize)
         self.data_shape = (K,)
         self.nslice = 1
+
+    def updateObj(self, phs_spha, phs_conc, b0map, b0mask, phs_grid):
+        """``updateObj`` (sampHighOrder.m:904-957): new field coefficients, B0
+        map, mask and grid for the same sample times and options (the next
+        slice or volume of a multi-slice acquisition).  For approach 1 the
+        interpolated plan (NUFFT locations, phase ramp, truncated SVD) is
+        rebuilt on the device (prepForInterp, :937); other approaches raise
+        like the reference (:953-955).  Returns the operator."""
+        if self.approach != 1:
+            raise NotImplementedError("this case not yet coded")   # :953-955
+        self._init_interp(b0map, self._sampTimes, phs_spha, phs_conc, phs_grid, b0mask, self._opt)
+        self.adjoint = 0
+        return self
 
     def setPhiDiv(self, ksphaDiv, kconcDiv=None):
         """``setPhiDiv`` (sampHighOrder.m:590-622): the forward becomes
