@@ -1116,80 +1116,110 @@ __global__ __launch_bounds__(256) void nu_grid_csr(const NuSliceDev* __restrict_
 // grid cells; kind 2 (a chunk of a dense rectangle) writes a partial block,
 // which nu_grid_merge adds in chunk order into the grid (deterministic for a
 // given plan).  A unit whose slice has finished is neither read nor written.
+//
+// GridStage holds what one lane loads for an item before its LDS stores:
+// NPRE staged samples (one transform each), the first 1024 entries and the
+// first 512 row pointers (four and two per lane).  Loads use clamped, valid
+// indices, so no load sits behind a branch.
 template <int BCT, int SG>
-__global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__ work,
-                                                    const int* __restrict__ isid, int sid_stride,
-                                                    const int* __restrict__ tptr,
-                                                    const int2* __restrict__ tent,
-                                                    const float2* __restrict__ yt, float2* __restrict__ ws,
-                                                    float2* __restrict__ parts, const int* __restrict__ skip) {
-    constexpr int TT = BCT * SG;   // transforms per staged sample
-    static_assert(256 % TT == 0, "transforms per sample must divide the work-group");
-    constexpr int SPR = 256 / TT;  // staged samples per pass of the work-group
-    constexpr int NPRE = 8;        // sample ids each lane loads up front
-    extern __shared__ float2 lds[];
-    // element i of the staged samples is transform (i mod TT) of sample
-    // i / TT; 256 is a multiple of TT, so a lane's transform (and unit) is
-    // fixed and its samples are q0, q0 + SPR, ...
-    const int jb = (int)threadIdx.x & (TT - 1);
-    const int q0 = (int)threadIdx.x / TT;
-    // the item's sample ids are addressed by the block index alone: they are
-    // loaded together with the item record, not after it
-    const int* sid = isid + (int64_t)blockIdx.x * sid_stride;
-    int m[NPRE];
+struct GridStage {
+    static constexpr int TT = BCT * SG;   // transforms per staged sample
+    static constexpr int SPR = 256 / TT;  // staged samples per pass of the work-group
+    static constexpr int NPRE = 8;        // sample ids each lane loads up front
+    float2 v[NPRE];
+    int2 e[4];
+    int pr[2];
+};
+
+// sample ids of item i that lane (q0) stages, addressed by the item index
+// alone (so they load together with the item record, not after it)
+template <int NPRE, int SPR>
+__device__ __forceinline__ void grid_item_ids(const int* __restrict__ isid, int sid_stride, int i, int q0,
+                                              int (&m)[NPRE]) {
+    const int* sid = isid + (int64_t)i * sid_stride;
 #pragma unroll
-    for (int k = 0; k < NPRE; ++k) m[k] = sid[min(q0 + k * SPR, sid_stride - 1)];   // unconditional loads
-    const GridItem T = work[blockIdx.x];
+    for (int k = 0; k < NPRE; ++k) m[k] = sid[min(q0 + k * SPR, sid_stride - 1)];
+}
+
+// The loads of one item into registers.  act0 / act1: the units whose slice
+// is still active; a lane of a finished unit loads sample 0 (its value is
+// never read), an item with no active unit entry 0 and row pointer 0.
+template <int BCT, int SG>
+__device__ __forceinline__ void grid_item_load(GridStage<BCT, SG>& g, const GridItem& T, bool act0, bool act1,
+                                               const int (&m)[GridStage<BCT, SG>::NPRE],
+                                               const int* __restrict__ tptr, const int2* __restrict__ tent,
+                                               const float2* __restrict__ yt) {
+    constexpr int TT = GridStage<BCT, SG>::TT;
+    const int jb = (int)threadIdx.x & (TT - 1);
+    const bool any = act0 || act1;
     const int cells = T.w * T.h;
-    // the rectangle's first 1024 (local sample, weight) entries and first
-    // 512 row pointers: plan tables addressed by the item record alone, so
-    // their loads (unconditional, from clamped indices) are in flight with
-    // the sample ids and the samples, not after them
     const int2* te = tent + T.ent_off;
     const int* tp = tptr + T.ptr_off;
-    const int ne_last = max(T.n_ent - 1, 0);
-    int2 e[4];
+    const int ne_last = any ? max(T.n_ent - 1, 0) : 0;
+    const int pr_last = any ? cells : 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) e[u] = te[min((int)threadIdx.x + u * 256, ne_last)];
-    int pr[2];
+    for (int u = 0; u < 4; ++u) g.e[u] = te[min((int)threadIdx.x + u * 256, ne_last)];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) pr[u] = tp[min((int)threadIdx.x + u * 256, cells)];
-    const bool act0 = !(skip && skip[T.slice]);
-    const bool act1 = SG > 1 && T.slice2 >= 0 && !(skip && skip[T.slice2]);
-    if (!act0 && !act1) return;
-    const int nsv = T.n_stage * TT;
-    float2* sv = lds;                              // [n_stage][TT]
-    int2* se = (int2*)(lds + nsv);                 // [n_ent]
-    int* sp = (int*)(se + T.n_ent);                // [cells + 1]
-    {
+    for (int u = 0; u < 2; ++u) g.pr[u] = tp[min((int)threadIdx.x + u * 256, pr_last)];
+    const int u1 = SG > 1 ? jb / BCT : 0;
+    const bool act = u1 ? act1 : act0;
+    const float2* ych = yt + (u1 ? T.yt2 : T.yt) + (jb & (BCT - 1));
+#pragma unroll
+    for (int k = 0; k < GridStage<BCT, SG>::NPRE; ++k) g.v[k] = ych[(int64_t)(act ? m[k] : 0) * BCT];
+}
+
+// The LDS stores of one item (registers from grid_item_load; samples,
+// entries and row pointers past the first round load here, straight from
+// global memory: only with caps above 8 samples per lane, 1024 entries or
+// 511 cells).  LDS: sv [n_stage][TT], se [n_ent], sp [cells + 1].
+template <int BCT, int SG>
+__device__ __forceinline__ void grid_item_store(const GridStage<BCT, SG>& g, const GridItem& T, bool act0,
+                                                bool act1, const int* __restrict__ sid,
+                                                const int* __restrict__ tptr, const int2* __restrict__ tent,
+                                                const float2* __restrict__ yt, float2* lds) {
+    constexpr int TT = GridStage<BCT, SG>::TT, SPR = GridStage<BCT, SG>::SPR, NPRE = GridStage<BCT, SG>::NPRE;
+    const int jb = (int)threadIdx.x & (TT - 1);
+    const int q0 = (int)threadIdx.x / TT;
+    const int cells = T.w * T.h;
+    float2* sv = lds;
+    int2* se = (int2*)(lds + T.n_stage * TT);
+    int* sp = (int*)(se + T.n_ent);
+#pragma unroll
+    for (int k = 0; k < NPRE; ++k)
+        if (q0 + k * SPR < T.n_stage) sv[(q0 + k * SPR) * TT + jb] = g.v[k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if ((int)threadIdx.x + u * 256 < T.n_ent) se[threadIdx.x + u * 256] = g.e[u];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        if ((int)threadIdx.x + u * 256 <= cells) sp[threadIdx.x + u * 256] = g.pr[u];
+    if (T.n_stage > NPRE * SPR) {
         const int u1 = SG > 1 ? jb / BCT : 0;
+        const bool act = u1 ? act1 : act0;
         const float2* ych = yt + (u1 ? T.yt2 : T.yt) + (jb & (BCT - 1));
-        // every lane loads (a valid sample: the row's padding ids are 0), so
-        // no load waits on a branch; values of a finished unit are staged but
-        // never read (its lanes do not gather)
-        float2 v[NPRE];
-#pragma unroll
-        for (int k = 0; k < NPRE; ++k) v[k] = ych[(int64_t)m[k] * BCT];
-#pragma unroll
-        for (int k = 0; k < NPRE; ++k)
-            if (q0 + k * SPR < T.n_stage) sv[(q0 + k * SPR) * TT + jb] = v[k];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if ((int)threadIdx.x + u * 256 < T.n_ent) se[threadIdx.x + u * 256] = e[u];
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if ((int)threadIdx.x + u * 256 <= cells) sp[threadIdx.x + u * 256] = pr[u];
-        for (int q = q0 + NPRE * SPR; q < T.n_stage; q += SPR) sv[q * TT + jb] = ych[(int64_t)sid[q] * BCT];
-        for (int i = threadIdx.x + 4 * 256; i < T.n_ent; i += 256) se[i] = te[i];
-        for (int i = threadIdx.x + 2 * 256; i <= cells; i += 256) sp[i] = tp[i];
+        for (int q = q0 + NPRE * SPR; q < T.n_stage; q += SPR)
+            sv[q * TT + jb] = ych[(int64_t)(act ? sid[q] : 0) * BCT];
     }
-    __syncthreads();
-    // gather: CPL consecutive transforms of one unit per lane (one
-    // 8*CPL-byte LDS read of the sample's values per entry), TT/CPL lanes per
-    // cell
+    const int2* te = tent + T.ent_off;
+    for (int i = threadIdx.x + 4 * 256; i < T.n_ent; i += 256) se[i] = te[i];
+    const int* tp = tptr + T.ptr_off;
+    for (int i = threadIdx.x + 2 * 256; i <= cells; i += 256) sp[i] = tp[i];
+}
+
+// The gather of one staged item: CPL consecutive transforms of one unit per
+// lane (one 8*CPL-byte LDS read of the sample's values per entry), TT/CPL
+// lanes per cell.
+template <int BCT, int SG>
+__device__ __forceinline__ void grid_item_gather(const GridItem& T, bool act0, bool act1, const float2* lds,
+                                                 float2* __restrict__ ws, float2* __restrict__ parts) {
+    constexpr int TT = GridStage<BCT, SG>::TT;
     constexpr int CPL = BCT >= 2 ? 2 : 1;
     constexpr int LPC = TT / CPL;
     constexpr int NGC = 256 / LPC;
+    const int cells = T.w * T.h;
+    const float2* sv = lds;
+    const int2* se = (const int2*)(lds + T.n_stage * TT);
+    const int* sp = (const int*)(se + T.n_ent);
     const int bc = (int)(threadIdx.x & (LPC - 1)) * CPL;
     const int gc = (int)threadIdx.x / LPC;
     const int u1 = SG > 1 ? bc / BCT : 0;
@@ -1259,6 +1289,31 @@ __global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__
         else
             dst[0] = acc[0];
     }
+}
+
+// One work-group per item.
+template <int BCT, int SG>
+__global__ __launch_bounds__(256) void nu_grid_tile(const GridItem* __restrict__ work,
+                                                    const int* __restrict__ isid, int sid_stride,
+                                                    const int* __restrict__ tptr,
+                                                    const int2* __restrict__ tent,
+                                                    const float2* __restrict__ yt, float2* __restrict__ ws,
+                                                    float2* __restrict__ parts, const int* __restrict__ skip) {
+    using GS = GridStage<BCT, SG>;
+    static_assert(256 % GS::TT == 0, "transforms per sample must divide the work-group");
+    extern __shared__ float2 lds[];
+    const int q0 = (int)threadIdx.x / GS::TT;
+    int m[GS::NPRE];
+    grid_item_ids<GS::NPRE, GS::SPR>(isid, sid_stride, blockIdx.x, q0, m);
+    const GridItem T = work[blockIdx.x];
+    const bool act0 = !(skip && skip[T.slice]);
+    const bool act1 = SG > 1 && T.slice2 >= 0 && !(skip && skip[T.slice2]);
+    if (!act0 && !act1) return;
+    GS g;
+    grid_item_load<BCT, SG>(g, T, act0, act1, m, tptr, tent, yt);
+    grid_item_store<BCT, SG>(g, T, act0, act1, isid + (int64_t)blockIdx.x * sid_stride, tptr, tent, yt, lds);
+    __syncthreads();
+    grid_item_gather<BCT, SG>(T, act0, act1, lds, ws, parts);
 }
 
 // Sum of the partial blocks of each dense rectangle in chunk order (one item
