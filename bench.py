@@ -52,8 +52,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch", "copy"], default="warmstart",
-                   help="copy: a plain 1 GiB device copy (HBM counter calibration, not a solve)")
+    p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch", "copy", "densemat"],
+                   default="warmstart",
+                   help="copy: a plain 1 GiB device copy (HBM counter calibration, not a solve); densemat: "
+                        "mrSampFuncMat's dense normal matrix at the high-order problem's full size")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--heldout", type=int, default=1000, help="held-out problems of the warm-start workload")
@@ -771,6 +773,103 @@ def run_copy(a, torch, dev):
                                  "bytes_written": nb}}), flush=True)
 
 
+PEAK_FP32_MFMA_TFLOPS = 157.3   # MI355X dense FP32 matrix peak (rocBLAS cherk of the dense normal matrix)
+
+
+def run_densemat(a, mm, torch, dev):
+    """``mrSampFuncMat`` (operators/mrSampFuncMat.m:1-47) at the full size of
+    the high-order problem (220x220 image, 24000 samples, 32 coils, no mask):
+    the single-precision dense encoding matrix S [24000 x 48400], its Gram
+    matrix S'S (rocBLAS cherk, lower triangle, mirrored by a kernel) and the
+    coil fold, AtA [48400 x 48400] complex64 left in HBM.  One step = one
+    mm_hio_dense_normal call (the workspace allocation included).  The
+    roofline is that of the dominant kernel, the cherk, timed with HIP events
+    on the library stream on an operand of the same shape."""
+    import ctypes as C
+    import numpy as np
+    P = mm.workloads.highorder_problem(seed=2)
+    N, K = P["N"], P["K"]
+    V = N * N
+    S = mm.sampHighOrder(P["b0"], P["t"], P["h"], None, P["grid"])
+    R = mm.rcvrOp(P["maps"])
+    L = mm.lib()
+    mask = np.ones(V, np.uint8)
+    AtA = torch.empty((V, V), dtype=torch.complex64, device=dev)
+    vm = C.c_int64()
+
+    def call():
+        mm._lib.check(L.mm_hio_dense_normal(S._plan.h, R.C, R._maps.ctypes.data, mask.ctypes.data, AtA.data_ptr(),
+                                            C.byref(vm), 1))
+    note("densemat: warm-up call")
+    torch.cuda.synchronize()
+    call()
+    for _ in range(max(0, a.warmup - 1)):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        call()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / a.steps
+    Vm = vm.value
+    # the dominant kernel on an operand of the same shape (column-major
+    # [K x Vm] = a row-major (Vm, K) tensor), events on the library stream
+    note("densemat: cherk timing")
+    rb = C.CDLL("/opt/rocm/lib/librocblas.so")
+    h = C.c_void_p()
+    assert rb.rocblas_create_handle(C.byref(h)) == 0
+    assert rb.rocblas_set_stream(h, C.c_void_p(mm._lib.context_stream())) == 0
+    Sx = torch.randn((Vm, K), dtype=torch.complex64, device=dev)
+    one, zero = C.c_float(1.0), C.c_float(0.0)
+
+    def herk():
+        rc = rb.rocblas_cherk(h, 122, 113, C.c_int(Vm), C.c_int(K), C.byref(one), C.c_void_p(Sx.data_ptr()),
+                              C.c_int(K), C.byref(zero), C.c_void_p(AtA.data_ptr()), C.c_int(Vm))
+        assert rc == 0, f"rocblas_cherk status {rc}"
+    herk()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        herk()
+    e1.record()
+    torch.cuda.synchronize()
+    kms = e0.elapsed_time(e1) / reps
+    rb.rocblas_destroy_handle(h)
+    del Sx
+    flops = 4.0 * K * Vm * (Vm + 1)          # one triangle of the Hermitian Gram matrix
+    ach = flops / (kms * 1e-3) / 1e12
+    out = {"metric": "dense normal matrices (mrSampFuncMat AtA) per second", "value": 1.0 / el,
+           "unit": "matrices/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * el,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+           "data": "synthetic (seeded high-order problem: 16 SH + B0, Gaussian-ring coils)",
+           "config": {"workload": "densemat: mrSampFuncMat AtA of the 220x220 / K=24000 / 32-coil high-order "
+                                  "operator, no mask (AtA 48400 x 48400 c64 in HBM)",
+                      "image": [N, N], "samples": K, "coils": R.C, "masked_voxels": Vm},
+           "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                        "frac": ach / PEAK_FP32_MFMA_TFLOPS, "traffic": None, "kernel": "rocblas_cherk",
+                        "ms_per_launch": kms, "algorithmic_flops_per_launch": flops,
+                        "note": "4 K Vm (Vm + 1) flops: the Hermitian product's lower triangle; the full "
+                                "product the reference forms (S'*S) is twice that"}}
+    if not a.no_cpu:
+        # FP64 Gram matrix of a column subset on the host (NumPy/OpenBLAS,
+        # every host thread), scaled by (Vm / Vs)^2 to the full matrix
+        vs = 6000
+        rng = np.random.default_rng(0)
+        Sd = np.exp(1j * rng.uniform(-np.pi, np.pi, (K, vs)))
+        t0 = time.perf_counter()
+        G = Sd.conj().T @ Sd
+        t1 = time.perf_counter() - t0
+        del G, Sd
+        full = t1 * (Vm / vs) ** 2
+        out["cpu_baseline"] = {"value": 1.0 / full, "unit": "matrices/s", "cores": os.cpu_count(), "kind": "port",
+                               "sample": f"NumPy FP64 Gram matrix S'*S of {K} x {vs} (the oracle's mrSampFuncMat.m:37 "
+                                         f"step) in {t1:.1f} s, scaled by (Vm/{vs})^2 to the {Vm}-voxel matrix"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -790,6 +889,10 @@ def main():
     if a.workload == "warmstart":
         with torch.cuda.stream(stream):
             run_warmstart(a, mm, torch, dev)
+        return
+    if a.workload == "densemat":
+        with torch.cuda.stream(stream):
+            run_densemat(a, mm, torch, dev)
         return
     with torch.cuda.stream(stream):
         W = {"highorder": highorder_setup, "nufft": nufft_setup, "batch": batch_setup}[a.workload](mm, torch, dev)

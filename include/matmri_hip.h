@@ -157,7 +157,7 @@ int mm_sumdims_plan_create(mm_ctx* ctx, int nd, const int64_t* dims, const int* 
  * S = exp(1i*single(kbase))/sqrt(Vm) the dense [K x Vm] encoding matrix in
  * single precision, as the reference forms it.  maps: host [V x C] c64;
  * AtA: [Vm x Vm] c64, host or device per on_device; *Vm_out receives Vm
- * (AtA == NULL: size query only).  The Gram product runs on rocBLAS (cgemm);
+ * (AtA == NULL: size query only).  The Gram product runs on rocBLAS (cherk, lower triangle, mirrored);
  * workspace K*Vm + (host output) Vm*Vm complex values.  Aty is the adjoint
  * of the SENSE operator (mm_op_apply) restricted to the mask. */
 int mm_hio_dense_normal(mm_op* op, int64_t C, const float* maps, const unsigned char* mask, void* AtA,

@@ -581,20 +581,51 @@ __global__ __launch_bounds__(256) void hio_dense_s_k(const double* __restrict__ 
 }
 
 // AtA[v, w] = G[v, w] * sum_c conj(m_c(v)) m_c(w)  (mrSampFuncMat.m:38-44),
-// m: masked coil maps [Vm x C]; in place on G (column-major [Vm x Vm])
+// m: masked coil maps [C][Vm]; in place on G (column-major [Vm x Vm]).  A
+// work-group takes a 64 x 64 tile: the coil values of its 64 rows and 64
+// columns are staged in LDS 32 coils at a time, each lane sums the coil
+// products of 16 elements of one row (its row value read once per coil, the
+// column values broadcast across the wave), then scales them in place, G
+// read and written down the tile's columns.
+constexpr int kMapsTile = 64, kMapsCoils = 32;
 __global__ __launch_bounds__(256) void hio_dense_maps_k(float2* __restrict__ G, const float2* __restrict__ m, int64_t Vm,
                                                         int C) {
-    const int64_t n = Vm * Vm;
-    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int64_t v = i % Vm, w = i / Vm;
-        float2 s = make_float2(0.f, 0.f);
-        for (int c = 0; c < C; ++c) {
-            const float2 a = m[(int64_t)c * Vm + v], b = m[(int64_t)c * Vm + w];
-            s.x += a.x * b.x + a.y * b.y;   // conj(a) * b
-            s.y += a.x * b.y - a.y * b.x;
+    __shared__ float2 mv[kMapsCoils][kMapsTile], mw[kMapsCoils][kMapsTile];
+    const int64_t v0 = (int64_t)blockIdx.x * kMapsTile, w0 = (int64_t)blockIdx.y * kMapsTile;
+    const int tx = (int)threadIdx.x & (kMapsTile - 1), ty = (int)threadIdx.x / kMapsTile;   // 4 column groups
+    constexpr int NJ = kMapsTile / 4;
+    float2 s[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) s[j] = make_float2(0.f, 0.f);
+    for (int c0 = 0; c0 < C; c0 += kMapsCoils) {
+        const int nc = min(kMapsCoils, C - c0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kMapsCoils * kMapsTile; i += 256) {
+            const int c = i / kMapsTile, j = i - c * kMapsTile;
+            const int cc = c0 + min(c, nc - 1);
+            mv[c][j] = m[(int64_t)cc * Vm + min(v0 + j, Vm - 1)];
+            mw[c][j] = m[(int64_t)cc * Vm + min(w0 + j, Vm - 1)];
         }
-        const float2 g = G[i];
-        G[i] = make_float2(g.x * s.x - g.y * s.y, g.x * s.y + g.y * s.x);
+        __syncthreads();
+        for (int c = 0; c < nc; ++c) {
+            const float2 a = mv[c][tx];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                const float2 b = mw[c][ty * NJ + j];
+                s[j].x += a.x * b.x + a.y * b.y;   // conj(a) * b
+                s[j].y += a.x * b.y - a.y * b.x;
+            }
+        }
+    }
+    const int64_t v = v0 + tx;
+    if (v >= Vm) return;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const int64_t w = w0 + ty * NJ + j;
+        if (w < Vm) {
+            const float2 g = G[v + w * Vm];
+            G[v + w * Vm] = make_float2(g.x * s[j].x - g.y * s[j].y, g.x * s[j].y + g.y * s[j].x);
+        }
     }
 }
 
@@ -608,8 +639,8 @@ void HioOp::dense_normal(const int* vidx, int64_t Vm, const float2* mmaps, int C
                        (float)(1.0 / std::sqrt((double)Vm)), S);
     LAUNCHCHECK();
     dense_gram(blas, st, S, K_, Vm, AtA);
-    const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>((Vm * Vm + 255) / 256, 65536));
-    hipLaunchKernelGGL(hio_dense_maps_k, dim3(g2), dim3(256), 0, st, AtA, mmaps, Vm, C);
+    const unsigned nt = (unsigned)((Vm + kMapsTile - 1) / kMapsTile);
+    hipLaunchKernelGGL(hio_dense_maps_k, dim3(nt, nt), dim3(256), 0, st, AtA, mmaps, Vm, C);
     LAUNCHCHECK();
 }
 
