@@ -123,7 +123,8 @@ int mm_hio_set_phi_div(mm_op* op, const double* hd, const double* gd);
  * a NUFFT plan whose grid is at least 2N - 1 per side (os = 2) into the
  * Toeplitz form of its normal operator.  psf: host complex64 (interleaved),
  * per slice the point-spread image S2'*(dcf or ones) of a NUFFT on the doubled
- * image size (2 N1 x 2 N2, column-major), slices concatenated.  Afterwards
+ * image size (2 N1 x 2 N2, or the 2 N1 x 2 N2 x 2 N3 volume of a 3-D plan,
+ * column-major), slices concatenated.  Afterwards
  * both directions of mm_op_apply map an image to an image (n_data = n_image):
  * crop(fftnc(convMat .* ifftnc(pad(x)))), convMat = 2^ndim ifftnc(psf).  A
  * SENSE operator built on it maps, like mrSampFunc on such a nufftOp,
