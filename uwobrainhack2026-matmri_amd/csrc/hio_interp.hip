@@ -84,6 +84,64 @@ HioInterpOp::HioInterpOp(mm_ctx* c, mm_op* nufft, int64_t V, int64_t K, int Ns, 
     HIPCHECK(hipStreamSynchronize(ctx->stream));
 }
 
+bool HioInterpOp::profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters, double* ms,
+                                   std::string* name) {
+    if ((adjoint != 0 && adjoint != 1) || nrep < 1 || iters < 1) return false;
+    hipStream_t st = ctx->stream;
+    const int64_t R = nrep;
+    apply(adjoint, in, out, nrep);   // sizes the buffers, first-launch costs
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    auto time_of = [&](auto launch) {
+        HIPCHECK(hipEventRecord(e0, st));
+        for (int i = 0; i < iters; ++i) launch();
+        HIPCHECK(hipEventRecord(e1, st));
+        HIPCHECK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+        return (double)t / iters;
+    };
+    double t_in, t_out;
+    if (!adjoint) {
+        t_in = time_of([&] {
+            hipLaunchKernelGGL(hin_modulate_k, dim3(hin_grid(V_ * R * Ns_)), dim3(256), 0, st, in, sp_.p, imgs_.p, V_,
+                               R, Ns_);
+        });
+        t_out = time_of([&] {
+            hipLaunchKernelGGL(hin_combine_k, dim3(hin_grid(K_ * R)), dim3(256), 0, st, dat_.p, st_.p, ph_.p, out, K_,
+                               R, Ns_);
+        });
+    } else {
+        t_in = time_of([&] {
+            hipLaunchKernelGGL(hin_spread_k, dim3(hin_grid(K_ * R * Ns_)), dim3(256), 0, st, in, st_.p, ph_.p, dat_.p,
+                               K_, R, Ns_);
+        });
+        t_out = time_of([&] {
+            hipLaunchKernelGGL(hin_demodulate_k, dim3(hin_grid(V_ * R)), dim3(256), 0, st, imgs_.p, sp_.p, out, V_,
+                               R, Ns_);
+        });
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    LAUNCHCHECK();
+    double t_nu = 0;
+    std::string nu_name;
+    const bool have_nu = adjoint ? nu_->profile_dominant(1, dat_.p, imgs_.p, R * Ns_, iters, &t_nu, &nu_name)
+                                 : nu_->profile_dominant(0, imgs_.p, dat_.p, R * Ns_, iters, &t_nu, &nu_name);
+    *ms = t_in;
+    *name = adjoint ? "hin_spread_k" : "hin_modulate_k";
+    if (t_out > *ms) {
+        *ms = t_out;
+        *name = adjoint ? "hin_demodulate_k" : "hin_combine_k";
+    }
+    if (have_nu && t_nu > *ms) {
+        *ms = t_nu;
+        *name = nu_name;
+    }
+    return true;
+}
+
 void HioInterpOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     if (nrep < 1) fail(MM_ERR_ARG, "sampHighOrder: nrep must be >= 1");
     hipStream_t st = ctx->stream;

@@ -15,6 +15,11 @@ class HioInterpOp : public mm_op {
     int64_t n_image() const override { return V_; }
     int64_t n_data() const override { return K_; }
     void apply(int adjoint, const float2* in, float2* out, int64_t nrep) override;
+    // The kernel of a forward (adjoint 0) or adjoint (1) application of nrep
+    // images with the largest time: the modulation, the combination, or the
+    // NUFFT's dominant stage over its nrep * Ns transforms.
+    bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters, double* ms,
+                          std::string* name) override;
     uint64_t config_gen() const override { return gen_ + nu_->config_gen(); }
     bool set_option(const std::string& key, int value) override { ++gen_; return nu_->set_option(key, value); }
     bool get_option(const std::string& key, int* value) const override { return nu_->get_option(key, value); }

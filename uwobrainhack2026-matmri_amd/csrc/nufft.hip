@@ -2765,10 +2765,33 @@ void NufftOp::mark(const char* name) {
 void NufftOp::profile_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                              const float* Wt, int iters,
                              std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel) {
+    profile_run(0, 1, in, out, nrep, maps, C, Wt, iters, per_kernel);
+}
+
+bool NufftOp::profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters, double* ms,
+                               std::string* name) {
+    if (adjoint != 0 && adjoint != 1) return false;
+    std::vector<std::pair<std::string, std::pair<double, int>>> k;
+    profile_run(adjoint, 0, in, out, nrep, nullptr, 0, nullptr, iters, &k);
+    double best = -1;
+    for (auto& e : k) {
+        const double tot = e.second.first * e.second.second;
+        if (tot > best) {
+            best = tot;
+            *ms = e.second.first;
+            *name = e.first;
+        }
+    }
+    return best >= 0;
+}
+
+void NufftOp::profile_run(int adjoint, int mode, const float2* in, float2* out, int64_t nrep, const float2* maps,
+                          int C, const float* Wt, int iters,
+                          std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel) {
     if (nrep < 1 || iters < 1) fail(MM_ERR_ARG, "profile: bad argument");
     Cfg& c = config(nrep, maps ? 1 : 0, maps ? C : 0);
     // one untimed application (plan set-up, first-launch costs)
-    run(c, 0, in, out, maps, Wt, 1);
+    run(c, adjoint, in, out, maps, Wt, mode);
     Prof p;
     // marks per application: up to 7 stages per slice group
     const size_t nev = 8 * std::max<size_t>(1, c.goff.size()) + 8;
@@ -2782,7 +2805,7 @@ void NufftOp::profile_normal(const float2* in, float2* out, int64_t nrep, const 
             p.k = 0;
             HIPCHECK(hipEventRecord(p.ev[0], ctx->stream));
             prof_ = &p;
-            run(c, 0, in, out, maps, Wt, 1);
+            run(c, adjoint, in, out, maps, Wt, mode);
             prof_ = nullptr;
             HIPCHECK(hipEventSynchronize(p.ev[p.k]));
             launches_per_app = p.k;

@@ -110,6 +110,10 @@ class NufftOp : public mm_op {
     void profile_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                         const float* W, int iters,
                         std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
+    // The stage of a plain forward (adjoint 0) or adjoint (1) application of
+    // nrep transforms with the largest time per application.
+    bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters, double* ms,
+                          std::string* name) override;
     // Tuning options (mm_op_set_option); the plan-time partitions are rebuilt
     // when their caps change:
     //   nu_stage_samples / nu_stage_entries  staged gridding rectangle caps
@@ -173,6 +177,10 @@ class NufftOp : public mm_op {
              int mode = 0);
     template <int BCT, bool BIG>
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
+    // per-kernel times of `iters` runs of (adjoint, mode) with an event after every kernel
+    void profile_run(int adjoint, int mode, const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
+                     const float* W, int iters,
+                     std::vector<std::pair<std::string, std::pair<double, int>>>* per_kernel);
     bool big_ = false;
     bool toep_ = false;
     bool any_odd_ = false;   // some slice has an odd image side: the ramp stage runs

@@ -253,6 +253,16 @@ bool SenseOp::profile_dominant(int adjoint, const float2* in, float2* out, int64
                                double* ms, std::string* name) {
     RcvrOp* rc = static_cast<RcvrOp*>(R_);
     if (hio_ && adjoint != 2) return hio_->profile_sense(adjoint, in, out, nrep, rc->maps(), C_, iters, ms, name);
+    if (!hio_ && (!nu_ || nu_->is_toeplitz()) && (adjoint == 0 || adjoint == 1)) {
+        // the composition S(R x): the coil operator is elementwise, the
+        // dominant kernel is the sampling operator's over nrep * C images
+        const int64_t nr = nrep * C_;
+        DevBuf<float2> a((size_t)(S_->n_image() * nr)), b((size_t)(S_->n_data() * nr));
+        HIPCHECK(hipMemsetAsync(a.p, 0, sizeof(float2) * a.n, ctx->stream));
+        HIPCHECK(hipMemsetAsync(b.p, 0, sizeof(float2) * b.n, ctx->stream));
+        return adjoint ? S_->profile_dominant(1, b.p, a.p, nr, iters, ms, name)
+                       : S_->profile_dominant(0, a.p, b.p, nr, iters, ms, name);
+    }
     if (nu_ && adjoint == 2) {
         // the dominant kernel of the normal operator cgne iterates
         std::vector<std::pair<std::string, std::pair<double, int>>> k;
