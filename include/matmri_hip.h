@@ -258,7 +258,8 @@ int mm_op_profile_stage(mm_op* op, int idx, double* ms_per_launch, int* launches
  * switches "nu_pair_cols", "nu_xcd_order", "nu_grid_pairs", "nu_grid_region" (with "nu_xcd_order": the
  * staged gridding items of one spatial region of a trajectory's rectangles on one XCD, for every slice),
  * "nu_snake", "nu_degrid_merge", "nu_grid_stage", "nu_degrid_stage", "nu_degrid_t", "nu_toep_fused" (the
- * Toeplitz form's 2-D middle stage in one pass). */
+ * Toeplitz form's 2-D middle stage in one pass), "nu_fft_mid" (FFT kernels that hold 32 values per thread
+ * for grid sides the default ones cannot run in one LDS buffer: 257..512 at 16 transforms). */
 int mm_op_set_option(mm_op* op, const char* key, int value);
 /* Current value of a tuning option of mm_op_set_option (no counterpart in
  * the reference). */

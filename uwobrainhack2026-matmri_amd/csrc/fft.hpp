@@ -279,16 +279,18 @@ __device__ __forceinline__ void fft_pass(const float2* __restrict__ a, float2* _
 
 // Butterflies a thread holds in registers in a single-buffer pass of radix
 // R: 8 complex values at most, 16 in kernels with 16 interleaved transforms
-// (which run at fewer waves per SIMD); at least one butterfly (radix 12 and
-// 16 take one).
-__host__ __device__ constexpr int fft_inplace_maxb(int R, int T) {
-    return (T >= 16 ? 16 : 8) / R > 0 ? (T >= 16 ? 16 : 8) / R : 1;
+// (which run at fewer waves per SIMD), or CAP where the kernel sets it (32
+// in the kernels for transform lengths 257..512, whose passes then fit one
+// LDS buffer); at least one butterfly (radix 12 and 16 take one).
+__host__ __device__ constexpr int fft_reg_cap(int T, int CAP) { return CAP > 0 ? CAP : (T >= 16 ? 16 : 8); }
+__host__ __device__ constexpr int fft_inplace_maxb(int R, int T, int CAP = 0) {
+    return fft_reg_cap(T, CAP) / R > 0 ? fft_reg_cap(T, CAP) / R : 1;
 }
 
 // Whether every pass of the transform (length n, T interleaved transforms)
 // can run in a single buffer: each thread's butterflies (at most cap / R, at
 // least one)
-// fit its registers; cap = 16 for T >= 16, else 8 (fft_inplace_maxb).
+// fit its registers; cap = fft_reg_cap(T, CAP) of the kernel.
 inline bool fft_inplace_ok(const FftFactors& f, int T, int cap) {
     for (int s = 0; s < f.nf; ++s)
         if ((f.n / f.r[s]) * T > (cap / f.r[s] > 0 ? cap / f.r[s] : 1) * kFftThreads) return false;
@@ -298,10 +300,10 @@ inline bool fft_inplace_ok(const FftFactors& f, int T, int cap) {
 // Single-buffer Stockham pass: every thread reads the inputs of its
 // butterflies (at most fft_inplace_maxb(R)) into registers, the block
 // synchronises, and the outputs overwrite the same buffer.
-template <int R, int T, int SGN, bool TW, bool BIG>
+template <int R, int T, int SGN, bool TW, bool BIG, int CAP = 0>
 __device__ __forceinline__ void fft_pass_inplace(float2* __restrict__ a, int n, int p,
                                                  const float2* __restrict__ twp) {
-    constexpr int MAXB = fft_inplace_maxb(R, T);
+    constexpr int MAXB = fft_inplace_maxb(R, T, CAP);
     constexpr int LT = T == 1 ? 0 : T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : T == 16 ? 4 : T == 32 ? 5 : 6;
     const int nb = n / R;
     const int total = nb * T;
@@ -346,7 +348,7 @@ __device__ __forceinline__ void fft_pass_inplace(float2* __restrict__ a, int n, 
 // a (the caller checked fft_inplace_ok), else the passes ping-pong between a
 // and b.  All threads of the block take part; returns the buffer holding the
 // result.  The caller must __syncthreads() after filling `a` (and tw).
-template <int T, int SGN, bool BIG = false, bool WIDE = false>
+template <int T, int SGN, bool BIG = false, bool WIDE = false, int CAP = 0>
 __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const FftFactors& f, const float2* tw, int s0,
                                                  int s1) {
     const int n = f.n;
@@ -358,25 +360,25 @@ __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const Fft
         if (b == nullptr) {
             if (s == 0) {
                 switch (R) {
-                    case 16: if constexpr (WIDE) fft_pass_inplace<16, T, SGN, false, BIG>(a, n, p, twp); break;
-                    case 12: if constexpr (WIDE) fft_pass_inplace<12, T, SGN, false, BIG>(a, n, p, twp); break;
-                    case 8: fft_pass_inplace<8, T, SGN, false, BIG>(a, n, p, twp); break;
-                    case 4: fft_pass_inplace<4, T, SGN, false, BIG>(a, n, p, twp); break;
-                    case 2: fft_pass_inplace<2, T, SGN, false, BIG>(a, n, p, twp); break;
-                    case 3: fft_pass_inplace<3, T, SGN, false, BIG>(a, n, p, twp); break;
-                    case 5: fft_pass_inplace<5, T, SGN, false, BIG>(a, n, p, twp); break;
-                    default: fft_pass_inplace<7, T, SGN, false, BIG>(a, n, p, twp); break;
+                    case 16: if constexpr (WIDE) fft_pass_inplace<16, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
+                    case 12: if constexpr (WIDE) fft_pass_inplace<12, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
+                    case 8: fft_pass_inplace<8, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
+                    case 4: fft_pass_inplace<4, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
+                    case 2: fft_pass_inplace<2, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
+                    case 3: fft_pass_inplace<3, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
+                    case 5: fft_pass_inplace<5, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
+                    default: fft_pass_inplace<7, T, SGN, false, BIG, CAP>(a, n, p, twp); break;
                 }
             } else {
                 switch (R) {
-                    case 16: if constexpr (WIDE) fft_pass_inplace<16, T, SGN, true, BIG>(a, n, p, twp); break;
-                    case 12: if constexpr (WIDE) fft_pass_inplace<12, T, SGN, true, BIG>(a, n, p, twp); break;
-                    case 8: fft_pass_inplace<8, T, SGN, true, BIG>(a, n, p, twp); break;
-                    case 4: fft_pass_inplace<4, T, SGN, true, BIG>(a, n, p, twp); break;
-                    case 2: fft_pass_inplace<2, T, SGN, true, BIG>(a, n, p, twp); break;
-                    case 3: fft_pass_inplace<3, T, SGN, true, BIG>(a, n, p, twp); break;
-                    case 5: fft_pass_inplace<5, T, SGN, true, BIG>(a, n, p, twp); break;
-                    default: fft_pass_inplace<7, T, SGN, true, BIG>(a, n, p, twp); break;
+                    case 16: if constexpr (WIDE) fft_pass_inplace<16, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
+                    case 12: if constexpr (WIDE) fft_pass_inplace<12, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
+                    case 8: fft_pass_inplace<8, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
+                    case 4: fft_pass_inplace<4, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
+                    case 2: fft_pass_inplace<2, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
+                    case 3: fft_pass_inplace<3, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
+                    case 5: fft_pass_inplace<5, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
+                    default: fft_pass_inplace<7, T, SGN, true, BIG, CAP>(a, n, p, twp); break;
                 }
             }
             __syncthreads();
@@ -416,15 +418,17 @@ __device__ __forceinline__ float2* fft_lds_range(float2* a, float2* b, const Fft
 }
 
 // Full transform in LDS (fft_lds_range over every pass).
-template <int T, int SGN, bool BIG = false, bool WIDE = false>
+template <int T, int SGN, bool BIG = false, bool WIDE = false, int CAP = 0>
 __device__ __forceinline__ float2* fft_lds(float2* a, float2* b, const FftFactors& f, const float2* tw) {
-    return fft_lds_range<T, SGN, BIG, WIDE>(a, b, f, tw, 0, f.nf);
+    return fft_lds_range<T, SGN, BIG, WIDE, CAP>(a, b, f, tw, 0, f.nf);
 }
 
 // First pass (radix R, P = 1, no twiddles) with its inputs taken from
 // src(q, t) -- the value at position q of interleaved transform t -- instead
 // of LDS, so the data are loaded straight into the butterflies' registers
-// (every load of a round issued before any use); the outputs go to `a`.
+// (every load of a round issued before any use; rounds of the default
+// register cap, whatever the kernel's single-buffer cap); the outputs go to
+// `a`.
 // The caller synchronises before the next pass reads `a`.
 template <int R, int T, int SGN, class Src>
 __device__ __forceinline__ void fft_first_pass_src(float2* __restrict__ a, int n, Src src) {

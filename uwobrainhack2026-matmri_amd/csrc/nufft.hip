@@ -312,19 +312,42 @@ __host__ __device__ constexpr int rows_T(int BCT, bool BIG) { return BIG || BCT 
 __host__ __device__ constexpr int cols_CL(int BCT) { return BCT < 16 ? (BCT < 8 ? BCT : 8) : 16; }
 __host__ __device__ constexpr int cols_T(int BCT, bool BIG) { return BIG ? cols_CL(BCT) : (BCT >= 8 ? 16 : 8); }
 
+// FFT mode of a configuration's row / column kernels (template FM):
+// kFftSmall -- every pass of every slice fits one LDS buffer with the
+// default register cap (fft_reg_cap: 16 values per thread at 16 transforms,
+// 8 otherwise); kFftMid -- a grid side above that (257..512), where the
+// kernels hold twice as many values per thread (32 at 16 transforms, 16 at
+// 8) so the passes still run in one LDS buffer (3 work-groups per CU
+// instead of 1 at G = 336 and 16 transforms, which is also why those may
+// take 168 registers: fft_waves), and the row passes of 16 transforms may
+// take the radix-16/12 factorization; kFftBig -- a side above kFftSmallMax
+// (wide index modulo).
+constexpr int kFftSmall = 0, kFftMid = 1, kFftBig = 2;
+__host__ __device__ constexpr int fft_mode_cap(int FM, int T) { return FM == kFftMid ? 2 * fft_reg_cap(T, 0) : 0; }
+__host__ __device__ constexpr bool rows_wide(int T, int FM) { return FM == kFftMid && T >= 16; }
+// minimum waves per SIMD of an FFT kernel with T transforms per work-group
+// (the register budget): mid-size kernels of 16 transforms 3 (their LDS
+// allows no more), others 4 at 16 transforms; `dflt` otherwise
+__host__ __device__ constexpr int fft_waves(int FM, int T, int dflt) {
+    return FM == kFftMid ? (T >= 16 ? 3 : 4) : (T >= 16 ? 4 : dflt);
+}
+
 // Forward rows: coil multiply + deapodisation, image written wrapped into
 // the zero-padded row (positions (i1 - N1/2) mod G1), FFT along g1.  Lane
 // index = (position, row, transform) with the transform fastest.  Offsets
 // inside a slice are 32-bit (the plan checks V*BCT and G1*G2*BCT < 2^31).
-template <int BCT, bool BIG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ? 4 : 8, 8))) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
+template <int BCT, int FM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fft_waves(FM, rows_T(BCT, FM == kFftBig), 8), 8))) void nu_fwd_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
                                                    const float2* __restrict__ twp,
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ x,
                                                    float2* __restrict__ ws, NuRun run) {
+    constexpr bool BIG = FM == kFftBig;
     constexpr int T = rows_T(BCT, BIG), LT = clog2(T), LB = clog2(BCT);
+    constexpr int CAP = fft_mode_cap(FM, T);
+    constexpr bool kWide = rows_wide(T, FM);
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
     if (skipped(run, wk.x)) return;
@@ -361,7 +384,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
                 const float f = d1[i1c] * d2[i2c] * ok;
                 return make_float2(v.x * f, v.y * f);
             };
-            fft_first_pass_src<T, 1>(F.r[0], A, G1, src);
+            fft_first_pass_src<T, 1, kWide>(F.r[0], A, G1, src);
         };
         if (run.sense)
             direct([&](int v, int b) { return cmulf(xs[v], mp[b * V + v]); });
@@ -370,10 +393,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
         // the pass twiddles after the data loads, which they then do not delay
         for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
         __syncthreads();
-        const float2* Rb = fft_lds_range<T, 1, BIG>(A, Bb, F, tw, 1, F.nf - 1);
+        const float2* Rb = fft_lds_range<T, 1, BIG, kWide, CAP>(A, Bb, F, tw, 1, F.nf - 1);
         int p = 1;
         for (int k = 0; k + 1 < F.nf; ++k) p *= F.r[k];
-        fft_last_pass_dst<T, 1, BIG>(F.r[F.nf - 1], Rb, G1, p, tw + (p - 1), [&](int q, int t, float2 v) {
+        fft_last_pass_dst<T, 1, BIG, kWide>(F.r[F.nf - 1], Rb, G1, p, tw + (p - 1), [&](int q, int t, float2 v) {
             const int i2 = r0 + (t >> LB);
             if (i2 < N2) out[(i2 * G1 + q) * BCT + (t & (BCT - 1))] = v;
         });
@@ -424,7 +447,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     // the pass twiddles after the data loads, which they then do not delay
     for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     __syncthreads();
-    const float2* R = fft_lds<T, 1, BIG>(A, Bb, F, tw);
+    const float2* R = fft_lds<T, 1, BIG, kWide, CAP>(A, Bb, F, tw);
     const int tot2 = G1 * T;
     for (int idx = threadIdx.x; idx < tot2; idx += kFftThreads) {
         const int t = idx & (T - 1);
@@ -438,14 +461,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
 // rows 0..N2-1 are placed at (i2 - N2/2) mod G2, output G2 rows.  Adjoint:
 // input G2 rows, output rows i2 < N2 read from (i2 - N2/2) mod G2.  In place
 // in `ws`.
-template <int BCT, bool BIG, int ADJ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void nu_cols(const NuSliceDev* __restrict__ sl,
+template <int BCT, int FM, int ADJ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fft_waves(FM, cols_T(BCT, FM == kFftBig), 4), 8))) void nu_cols(const NuSliceDev* __restrict__ sl,
                                                const int4* __restrict__ work,
                                                const FftFactors* __restrict__ facs,
                                                const float2* __restrict__ twp,
                                                float2* __restrict__ ws, const int* __restrict__ skip) {
+    constexpr bool BIG = FM == kFftBig;
     constexpr int T = cols_T(BCT, BIG), LT = clog2(T), CL = cols_CL(BCT), LCL = clog2(CL), CPW = T / CL;
-    // passes of 16 transforms run the radix-16/12 factorization (fac2)
+    constexpr int CAP = fft_mode_cap(FM, T);
+    // passes of 16 transforms may run a radix-16/12 factorization (fac2)
     constexpr bool kWide = T >= 16 && !BIG;
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
@@ -486,7 +511,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         // the pass twiddles after the data loads, which they then do not delay
         for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
         __syncthreads();
-        const float2* Rb = fft_lds_range<T, SG, BIG, kWide>(A, Bb, F, tw, 1, F.nf - 1);
+        const float2* Rb = fft_lds_range<T, SG, BIG, kWide, CAP>(A, Bb, F, tw, 1, F.nf - 1);
         int p = 1;
         for (int k = 0; k + 1 < F.nf; ++k) p *= F.r[k];
         auto dst = [&](int q, int t, float2 v) {
@@ -544,7 +569,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     // the pass twiddles after the data loads, which they then do not delay
     for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
     __syncthreads();
-    const float2* R = fft_lds<T, ADJ ? -1 : 1, BIG, kWide>(A, Bb, facs[S.fac2], tw);
+    const float2* R = fft_lds<T, ADJ ? -1 : 1, BIG, kWide, CAP>(A, Bb, facs[S.fac2], tw);
     if (!ADJ) {
         const int total = G2 * T;
         for (int idx = threadIdx.x; idx < total; idx += kFftThreads) {
@@ -660,12 +685,14 @@ __global__ __launch_bounds__(256) void nu_degrid_t(const NuSliceDev* __restrict_
 // nu_cols), the multiply by the kernel spectrum K[g2][g1] in LDS, the
 // inverse FFT and the store of the N2 rows the adjoint rows read back; the
 // grid rows beyond N2 never reach HBM.  Items and LDS as nu_cols.
-template <int BCT, bool BIG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void nu_cols_toep(
+template <int BCT, int FM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fft_waves(FM, cols_T(BCT, FM == kFftBig), 4), 8))) void nu_cols_toep(
     const NuSliceDev* __restrict__ sl, const int4* __restrict__ work, const FftFactors* __restrict__ facs,
     const float2* __restrict__ twp, const float2* __restrict__ km, const int64_t* __restrict__ koff,
     float2* __restrict__ ws, const int* __restrict__ skip) {
+    constexpr bool BIG = FM == kFftBig;
     constexpr int T = cols_T(BCT, BIG), LT = clog2(T), CL = cols_CL(BCT), LCL = clog2(CL), CPW = T / CL;
+    constexpr int CAP = fft_mode_cap(FM, T);
     constexpr bool kWide = T >= 16 && !BIG;
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
@@ -693,7 +720,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     fft_first_pass_src<T, 1, kWide>(F.r[0], A, G2, src);
     for (int i = threadIdx.x; i < G2 - 1; i += kFftThreads) tw[i] = twp[S.tw2 + i];
     __syncthreads();
-    float2* R = const_cast<float2*>(fft_lds_range<T, 1, BIG, kWide>(A, Bb, F, tw, 1, F.nf));
+    float2* R = const_cast<float2*>(fft_lds_range<T, 1, BIG, kWide, CAP>(A, Bb, F, tw, 1, F.nf));
     __syncthreads();
     for (int idx = threadIdx.x; idx < G2 * T; idx += kFftThreads) {
         const int t = idx & (T - 1);
@@ -702,7 +729,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         R[idx] = make_float2(v.x * k.x - v.y * k.y, v.x * k.y + v.y * k.x);
     }
     __syncthreads();
-    const float2* Rb = fft_lds_range<T, -1, BIG, kWide>(R, R == A ? Bb : A, F, tw, 0, F.nf - 1);
+    const float2* Rb = fft_lds_range<T, -1, BIG, kWide, CAP>(R, R == A ? Bb : A, F, tw, 0, F.nf - 1);
     int p = 1;
     for (int k = 0; k + 1 < F.nf; ++k) p *= F.r[k];
     auto dst = [&](int q, int t, float2 v) {
@@ -1351,15 +1378,18 @@ __global__ __launch_bounds__(256) void nu_grid_merge(const GridItem* __restrict_
 // the conjugate-coil combine.  In the combine two lanes share an output
 // pixel, each summing half of the coils; the halves are added through a
 // lane shuffle.
-template <int BCT, bool BIG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ? 4 : 8, 8))) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
+template <int BCT, int FM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fft_waves(FM, rows_T(BCT, FM == kFftBig), 8), 8))) void nu_adj_rows(const NuSliceDev* __restrict__ sl,
                                                    const int4* __restrict__ work,
                                                    const FftFactors* __restrict__ facs,
                                                    const float2* __restrict__ twp,
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ ws,
                                                    float2* __restrict__ x, NuRun run) {
+    constexpr bool BIG = FM == kFftBig;
     constexpr int T = rows_T(BCT, BIG), LT = clog2(T), LB = clog2(BCT), RPW = T / BCT, LR = clog2(RPW);
+    constexpr int CAP = fft_mode_cap(FM, T);
+    constexpr bool kWide = rows_wide(T, FM);
     extern __shared__ float2 lds[];
     const int4 wk = work[blockIdx.x];
     if (skipped(run, wk.x)) return;
@@ -1378,7 +1408,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
         // the first pass takes its inputs straight from the workspace rows
         // into the butterflies' registers (one LDS round trip fewer); rows
         // past the image are scaled by 0
-        fft_first_pass_src<T, -1>(F.r[0], A, G1, [&](int q, int t) {
+        fft_first_pass_src<T, -1, kWide>(F.r[0], A, G1, [&](int q, int t) {
             const int i2 = r0 + (t >> LB);
             const float ok = i2 < N2 ? 1.f : 0.f;
             const float2 v = g[(min(i2, N2 - 1) * G1 + q) * BCT + (t & (BCT - 1))];
@@ -1387,7 +1417,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
         // the pass twiddles after the data loads, which they then do not delay
         for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
         __syncthreads();
-        R = fft_lds_range<T, -1, BIG>(A, Bb, F, tw, 1, F.nf);
+        R = fft_lds_range<T, -1, BIG, kWide, CAP>(A, Bb, F, tw, 1, F.nf);
     } else {
     const int total = G1 * T;
     // four loads per thread in flight, from clamped (valid) addresses; rows
@@ -1416,7 +1446,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCT >= 16 ?
     // the pass twiddles after the data loads, which they then do not delay
     for (int i = threadIdx.x; i < G1 - 1; i += kFftThreads) tw[i] = twp[S.tw1 + i];
     __syncthreads();
-    R = fft_lds<T, -1, BIG>(A, Bb, F, tw);
+    R = fft_lds<T, -1, BIG, kWide, CAP>(A, Bb, F, tw);
     }
     const int lanes = lanes_of(run, BCT, ch);
     const float* d1 = deap + S.deap1;
@@ -1547,12 +1577,15 @@ NufftOp::NufftOp(mm_ctx* c, int ndim, int nslice, const int64_t* imN, const int6
         auto f1 = fft_for(d.G1, false);
         auto f2 = fft_for(d.G2, false);
         auto f2w = fft_for(d.G2, true);
+        auto f1w = fft_for(d.G1, true);
         d.fac1 = f1.first;
         d.tw1 = f1.second;
         d.fac2 = f2.first;
         d.tw2 = f2.second;
         d.fac2w = f2w.first;
         d.tw2w = f2w.second;
+        d.fac1w = f1w.first;
+        d.tw1w = f1w.second;
         // deapodisation with 1/sqrt(G1*G2) and the real part (-1)^(s1+s2),
         // s = floor(N/2), of the ramp constant exp(i pi (N1+N2)/2) folded in
         // (an odd side's factor i goes with its ramp, nu_odd_ramp)
@@ -1847,6 +1880,7 @@ bool NufftOp::set_option(const std::string& key, int value) {
     };
     if (key == "nu_degrid_t") return flag(degrid_t_, false);
     if (key == "nu_pair_cols") return flag(pair_cols_, true);
+    if (key == "nu_fft_mid") return flag(fft_mid_, true);
     if (key == "nu_degrid_stage") return flag(degrid_stage_, false);
     if (key == "nu_grid_stage") return flag(grid_stage_, false);
     if (key == "nu_xcd_order") return flag(xcd_order_, true);
@@ -1889,6 +1923,7 @@ bool NufftOp::get_option(const std::string& key, int* value) const {
     if (!value) return false;
     if (key == "nu_degrid_t") *value = degrid_t_;
     else if (key == "nu_pair_cols") *value = pair_cols_;
+    else if (key == "nu_fft_mid") *value = fft_mid_;
     else if (key == "nu_degrid_stage") *value = degrid_stage_;
     else if (key == "nu_grid_stage") *value = grid_stage_;
     else if (key == "nu_xcd_order") *value = xcd_order_;
@@ -2006,6 +2041,22 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     };
     std::vector<GridUnit> gunits;
     c.lds_gt = 16;
+    // FFT mode: the small kernels where every slice's passes fit one LDS
+    // buffer at their register cap, else the mid-size kernels (32 values per
+    // thread) below kFftSmallMax
+    {
+        const int TR = rows_T(BCT, big_), TC = cols_T(BCT, big_);
+        bool small_ok = true;
+        for (int s = 0; s < ns; ++s) {
+            const NuSliceDev& d = c.hs[s];
+            FftFactors f1, f2;
+            fft_factorize(d.G1, f1);
+            fft_factorize(d.G2, f2);
+            small_ok = small_ok && fft_inplace_ok(f1, TR, fft_reg_cap(TR, 0)) &&
+                       fft_inplace_ok(f2, TC, fft_reg_cap(TC, 0));
+        }
+        c.fm = big_ ? kFftBig : small_ok || !fft_mid_ ? kFftSmall : kFftMid;
+    }
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
         // rows / columns per work-group (the kernels' compile-time tiling):
@@ -2023,25 +2074,37 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         d.lcl = ilog2(cl);
         {
             // column passes of 16 transforms use the radix-16/12
-            // factorization where it is made of those radices only and
-            // takes fewer passes (192 = 16 x 12: two instead of three;
-            // 0.93 -> 0.77 us per 8-coil slice)
-            bool wide = !big_ && TC >= 16;
-            if (wide) {
+            // factorization where it takes fewer passes -- in the small mode
+            // only where it is made of those radices alone (192 = 16 x 12:
+            // two passes instead of three; 0.93 -> 0.77 us per 8-coil
+            // slice), in the mid-size mode also with others (336 = 16 x 3 x
+            // 7 instead of 8 x 2 x 3 x 7), where row passes of 16 transforms
+            // take it too
+            auto use_wide = [&](int G, bool allowed) {
+                if (!allowed) return false;
                 FftFactors fw, fn;
-                fft_factorize(d.G2, fw, true);
-                fft_factorize(d.G2, fn, false);
-                for (int k = 0; k < fw.nf; ++k) wide = wide && (fw.r[k] == 16 || fw.r[k] == 12);
-                wide = wide && fw.nf < fn.nf;
-            }
+                fft_factorize(G, fw, true);
+                fft_factorize(G, fn, false);
+                bool w = fw.nf < fn.nf;
+                if (c.fm == kFftSmall)
+                    for (int k = 0; k < fw.nf; ++k) w = w && (fw.r[k] == 16 || fw.r[k] == 12);
+                return w;
+            };
+            const bool wide = use_wide(d.G2, !big_ && TC >= 16);
+            const bool wide1 = use_wide(d.G1, rows_wide(TR, c.fm));
             if (wide) {
                 d.fac2 = d.fac2w;
                 d.tw2 = d.tw2w;
             }
+            if (wide1) {
+                d.fac1 = d.fac1w;
+                d.tw1 = d.tw1w;
+            }
             FftFactors f1, f2;
-            fft_factorize(d.G1, f1);
+            fft_factorize(d.G1, f1, wide1);
             fft_factorize(d.G2, f2, wide);
-            const bool ip1 = fft_inplace_ok(f1, TR, TR >= 16 ? 16 : 8), ip2 = fft_inplace_ok(f2, TC, TC >= 16 ? 16 : 8);
+            const bool ip1 = fft_inplace_ok(f1, TR, fft_reg_cap(TR, fft_mode_cap(c.fm, TR))),
+                       ip2 = fft_inplace_ok(f2, TC, fft_reg_cap(TC, fft_mode_cap(c.fm, TC)));
             d.fip = (ip1 ? 1 : 0) | (ip2 ? 2 : 0);
             c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR, ip1));
             c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, TC, ip2));
@@ -2404,19 +2467,19 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     return c;
 }
 
-template <int BCT, bool BIG>
+template <int BCT, int FM>
 void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, float2* out, int mode) {
     hipStream_t st = ctx->stream;
     static bool attr_set = false;
     if (!attr_set) {
         // allow FFT passes above 64 KB of LDS where the device permits it
-        (void)hipFuncSetAttribute((const void*)nu_fwd_rows<BCT, BIG>,
+        (void)hipFuncSetAttribute((const void*)nu_fwd_rows<BCT, FM>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)nu_adj_rows<BCT, BIG>,
+        (void)hipFuncSetAttribute((const void*)nu_adj_rows<BCT, FM>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, BIG, 0>,
+        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, FM, 0>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, BIG, 1>,
+        (void)hipFuncSetAttribute((const void*)nu_cols<BCT, FM, 1>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)nu_degrid_tile<BCT, 4>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -2454,12 +2517,12 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
     auto forward = [&](const float2* x, float2* y, NuRun r, int g) {
         const auto fr = span(kLFrows, g, c.n_frows), fc = span(kLFcols, g, c.n_cols), db = span(kLDb, g, c.n_db);
         if (fr.second > 0)
-            hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(fr.second), dim3(256), c.lds_rows, st, sl,
+            hipLaunchKernelGGL((nu_fwd_rows<BCT, FM>), dim3(fr.second), dim3(256), c.lds_rows, st, sl,
                                c.w_frows.p + fr.first, facs_.p, twid_.p, deap_.p, x, c.ws.p, r);
         LAUNCHCHECK();
         mark("nu_fwd_rows");
         if (ndim_ == 2 && fc.second > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(fc.second), dim3(256), c.lds_cols, st, sl,
+            hipLaunchKernelGGL((nu_cols<BCT, FM, 0>), dim3(fc.second), dim3(256), c.lds_cols, st, sl,
                                c.w_fcols.p + fc.first, facs_.p, twid_.p, c.ws.p, r.skip);
             LAUNCHCHECK();
             mark("nu_cols");
@@ -2520,13 +2583,13 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         mark("nu_grid");
         odd_ramp(true, r.skip);
         if (ndim_ == 2 && cc.second > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(cc.second), dim3(256), c.lds_cols, st, sl,
+            hipLaunchKernelGGL((nu_cols<BCT, FM, 1>), dim3(cc.second), dim3(256), c.lds_cols, st, sl,
                                c.w_cols.p + cc.first, facs_.p, twid_.p, c.ws.p, r.skip);
             LAUNCHCHECK();
             mark("nu_cols");
         }
         if (rw.second > 0)
-            hipLaunchKernelGGL((nu_adj_rows<BCT, BIG>), dim3(rw.second), dim3(256), c.lds_rows, st, sl,
+            hipLaunchKernelGGL((nu_adj_rows<BCT, FM>), dim3(rw.second), dim3(256), c.lds_rows, st, sl,
                                c.w_rows.p + rw.first, facs_.p, twid_.p, deap_.p, (const float2*)c.ws.p, x, r);
         LAUNCHCHECK();
         mark("nu_adj_rows");
@@ -2535,20 +2598,20 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         // Toeplitz normal operator: forward row and column FFTs, the kernel
         // spectrum multiply, adjoint column and row FFTs
         if (c.n_frows > 0)
-            hipLaunchKernelGGL((nu_fwd_rows<BCT, BIG>), dim3(c.n_frows), dim3(256), c.lds_rows, st, sl, c.w_frows.p,
+            hipLaunchKernelGGL((nu_fwd_rows<BCT, FM>), dim3(c.n_frows), dim3(256), c.lds_rows, st, sl, c.w_frows.p,
                                facs_.p, twid_.p, deap_.p, in, c.ws.p, run);
         LAUNCHCHECK();
         mark("nu_fwd_rows");
         if (ndim_ == 2 && toep_fused_ok_) {
             if (c.n_cols > 0)
-                hipLaunchKernelGGL((nu_cols_toep<BCT, BIG>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
+                hipLaunchKernelGGL((nu_cols_toep<BCT, FM>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
                                    facs_.p, twid_.p, (const float2*)toep_m_.p, (const int64_t*)toep_off_.p, c.ws.p,
                                    run.skip);
             LAUNCHCHECK();
             mark("nu_cols_toep");
         } else {
         if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_fcols.p,
+            hipLaunchKernelGGL((nu_cols<BCT, FM, 0>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_fcols.p,
                                facs_.p, twid_.p, c.ws.p, run.skip);
             LAUNCHCHECK();
             mark("nu_cols");
@@ -2562,14 +2625,14 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         LAUNCHCHECK();
         mark("nu_toep_mul");
         if (ndim_ == 2 && c.n_cols > 0) {
-            hipLaunchKernelGGL((nu_cols<BCT, BIG, 1>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
+            hipLaunchKernelGGL((nu_cols<BCT, FM, 1>), dim3(c.n_cols), dim3(256), c.lds_cols, st, sl, c.w_cols.p,
                                facs_.p, twid_.p, c.ws.p, run.skip);
             LAUNCHCHECK();
             mark("nu_cols");
         }
         }
         if (c.n_rows > 0)
-            hipLaunchKernelGGL((nu_adj_rows<BCT, BIG>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p,
+            hipLaunchKernelGGL((nu_adj_rows<BCT, FM>), dim3(c.n_rows), dim3(256), c.lds_rows, st, sl, c.w_rows.p,
                                facs_.p, twid_.p, deap_.p, (const float2*)c.ws.p, out, run);
         LAUNCHCHECK();
         mark("nu_adj_rows");
@@ -2622,23 +2685,22 @@ void NufftOp::run(Cfg& c, int adjoint, const float2* in, float2* out, const floa
     }
     r.skip = mode == 1 || mode == 3 ? skip_ : nullptr;
     if (mode == 3) mode = 2;
-    if (big_) {
+    auto go = [&](auto fm) {
+        constexpr int FM = decltype(fm)::value;
         switch (c.BCT) {
-            case 1: launch<1, true>(c, adjoint, r, in, out, mode); break;
-            case 2: launch<2, true>(c, adjoint, r, in, out, mode); break;
-            case 4: launch<4, true>(c, adjoint, r, in, out, mode); break;
-            case 8: launch<8, true>(c, adjoint, r, in, out, mode); break;
-            default: launch<16, true>(c, adjoint, r, in, out, mode); break;
+            case 1: launch<1, FM>(c, adjoint, r, in, out, mode); break;
+            case 2: launch<2, FM>(c, adjoint, r, in, out, mode); break;
+            case 4: launch<4, FM>(c, adjoint, r, in, out, mode); break;
+            case 8: launch<8, FM>(c, adjoint, r, in, out, mode); break;
+            default: launch<16, FM>(c, adjoint, r, in, out, mode); break;
         }
-        return;
-    }
-    switch (c.BCT) {
-        case 1: launch<1, false>(c, adjoint, r, in, out, mode); break;
-        case 2: launch<2, false>(c, adjoint, r, in, out, mode); break;
-        case 4: launch<4, false>(c, adjoint, r, in, out, mode); break;
-        case 8: launch<8, false>(c, adjoint, r, in, out, mode); break;
-        default: launch<16, false>(c, adjoint, r, in, out, mode); break;
-    }
+    };
+    if (c.fm == kFftBig)
+        go(std::integral_constant<int, kFftBig>());
+    else if (c.fm == kFftMid)
+        go(std::integral_constant<int, kFftMid>());
+    else
+        go(std::integral_constant<int, kFftSmall>());
 }
 
 void NufftOp::set_toeplitz(const float2* psf) {

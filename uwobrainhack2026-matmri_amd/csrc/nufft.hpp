@@ -22,6 +22,8 @@ struct NuSliceDev {
     int fip;            // bit 0: row FFTs, bit 1: column FFTs in a single LDS buffer
     int fac2w, tw2w;    // the column length's radix-16/12 factorization (passes of
                         // 16 transforms): used for fac2/tw2 when the column pass has them
+    int fac1w, tw1w;    // the row length's, used for fac1/tw1 by row passes of 16
+                        // transforms in the mid-size FFT mode
     int odd;            // bit 0: N1 odd, bit 1: N2 odd (the grid takes the odd-size ramp)
     int64_t samp_off;   // first sample of this slice in the sample pools
     int64_t img_off;    // image-side offset (elements) for the current apply
@@ -149,6 +151,7 @@ class NufftOp : public mm_op {
         DevBuf<int4> w_fcols;    // forward column items (w_cols: adjoint)
         int n_rows = 0, n_frows = 0, n_cols = 0, n_deg = 0;
         size_t lds_rows = 0, lds_cols = 0;
+        int fm = 0;   // FFT mode of the row / column kernels (kFftSmall, kFftMid, kFftBig)
         DevBuf<float2> ws, parts, yT;
         DevBuf<int4> w_csr;      // CSR gridding items: (slice, chunk, first cell, cells)
         int n_csr = 0;
@@ -175,7 +178,7 @@ class NufftOp : public mm_op {
     // that the gathering adjoint reads)
     void run(Cfg& c, int adjoint, const float2* in, float2* out, const float2* maps, const float* W,
              int mode = 0);
-    template <int BCT, bool BIG>
+    template <int BCT, int FM>
     void launch(Cfg& c, int adjoint, const struct NuRun& run, const float2* in, float2* out, int mode);
     // per-kernel times of `iters` runs of (adjoint, mode) with an event after every kernel
     void profile_run(int adjoint, int mode, const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
@@ -256,6 +259,7 @@ class NufftOp : public mm_op {
     std::vector<uint32_t> sbase_host_;
     std::vector<float> swt_host_, sdcf_host_;
     bool pair_cols_ = true;   // XCD-paired order of the column-pass items (option "nu_pair_cols")
+    bool fft_mid_ = true;     // mid-size FFT mode for grid sides the small one cannot hold in one LDS buffer ("nu_fft_mid")
     // staged degridding (option "nu_degrid_stage", default on): sample
     // blocks of each distinct 2-D trajectory, (g1 | g2 << 16 of the tile's
     // corner, first entry of dsamp_, samples, tile width along g1)
