@@ -52,11 +52,13 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch", "copy", "densemat", "interp"],
+    p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch", "copy", "densemat", "interp",
+                                          "finddel"],
                    default="warmstart",
                    help="copy: a plain 1 GiB device copy (HBM counter calibration, not a solve); densemat: "
                         "mrSampFuncMat's dense normal matrix at the high-order problem's full size; interp: the "
-                        "high-order solve with sampHighOrder approach 1 (interpolated)")
+                        "high-order solve with sampHighOrder approach 1 (interpolated); finddel: findDelAuto's "
+                        "delay estimate on the high-order problem")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--heldout", type=int, default=1000, help="held-out problems of the warm-start workload")
@@ -67,7 +69,8 @@ def parse():
                    help="skip the 64-slice batch roofline of the warm-start line")
     p.add_argument("--hio-variant", type=int, default=None, help="high-order MFMA kernel variant 0..3")
     p.add_argument("--no-autotune", action="store_true",
-                   help="skip the untimed autotuner of the NUFFT SENSE operator (mm_op_autotune)")
+                   help="skip the untimed autotuner of the NUFFT SENSE operator, or of the interpolated "
+                        "operator's NUFFT (mm_op_autotune)")
     p.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                    help="tuning option of the sampling operator (mm_op_set_option), e.g. nu_degrid_t=0")
     return p.parse_args()
@@ -194,20 +197,20 @@ def cpu_baseline_batch(Ps, seconds):
                       f"solve per slice ({apps} applications)"}
 
 
-def cpu_baseline_highorder(P, seconds):
-    """The C/OpenMP FP64 restatement (oracle/c/mmref.c, direct approach: the
-    phase and exp(i Phi) recomputed every application, as the reference's
-    direct worker evaluates exp(1i*kbase) per call) of the same SENSE
-    operator, forward + adjoint on a bounded sample of time points, scaled
-    to a 30-iteration solve (65 applications)."""
-    import numpy as np
+def cpu_highorder_per_app(b0, t, h, hc, grid, maps, x, seconds):
+    """Seconds per operator application of the C/OpenMP FP64 restatement
+    (oracle/c/mmref.c, direct approach: the phase and exp(i Phi) recomputed
+    every application, as the reference's direct worker evaluates
+    exp(1i*kbase) per call) of a high-order SENSE operator: forward +
+    adjoint on a bounded sample of time points, scaled to all of them.
+    Returns (seconds, samples timed, seconds measured, threads) or None."""
     cref = _cref()
     if cref is None:
         return None
-    K = P["K"]
-    ks, x = 256, P["phantom"]
+    K = len(t)
+    ks = 256
     while True:
-        E = cref.HighOrderSenseRef(P["b0"], P["t"][:ks], P["h"][:, :ks], None, P["grid"], P["maps"])
+        E = cref.HighOrderSenseRef(b0, t[:ks], h[:, :ks], None if hc is None else hc[:, :ks], grid, maps)
         t0 = time.perf_counter()
         y = E.forward(x)
         E.adjoint(y)
@@ -215,9 +218,20 @@ def cpu_baseline_highorder(P, seconds):
         if dt > seconds / 3 or ks >= K:
             break
         ks = min(K, int(ks * max(2.0, (seconds / 3) / max(dt, 1e-3))))
-    per_app = dt / 2 * K / ks
+    return dt / 2 * K / ks, ks, dt, cref.num_threads()
+
+
+def cpu_baseline_highorder(P, seconds):
+    """The C/OpenMP FP64 restatement of the same SENSE operator
+    (cpu_highorder_per_app), scaled to a 30-iteration solve (65
+    applications)."""
+    r = cpu_highorder_per_app(P["b0"], P["t"], P["h"], None, P["grid"], P["maps"], P["phantom"], seconds)
+    if r is None:
+        return None
+    per_app, ks, dt, thr = r
+    K = P["K"]
     apps = 2 * 30 + 5  # cgne with 30 iterations: 32 forward + 33 adjoint applications
-    return {"value": 1.0 / (per_app * apps), "unit": "solves/s", "cores": cref.num_threads(), "kind": "port",
+    return {"value": 1.0 / (per_app * apps), "unit": "solves/s", "cores": thr, "kind": "port",
             "sample": f"C/OpenMP FP64 restatement (oracle/c/mmref.c), one forward + one adjoint over {ks} of "
                       f"{K} samples ({dt:.2f} s), scaled to one 30-iteration solve ({apps} applications)"}
 
@@ -816,6 +830,8 @@ def run_interp(a, mm, torch, dev):
     Si = mm.sampHighOrder(P["b0"], P["t"], P["h"], None, P["grid"], None, 1, 0, 1, {})
     setup_s = time.perf_counter() - t0
     Ei = mm.senseOp(Si, W["R"])
+    # untimed: the NUFFT's options for its batch (32 coils x Ns transforms)
+    tuned = None if a.no_autotune else Ei.autotune(5)
     V = N * N
     xr = torch.randn(V, dtype=torch.complex64, device=dev)
     fd, fi = Ed * xr, Ei * xr
@@ -865,7 +881,7 @@ def run_interp(a, mm, torch, dev):
            "dtype": "c64", "data": "synthetic (seeded phantom, coils, field coefficients, B0, noise)",
            "config": dict(W["cfg"], workload="interp: " + W["cfg"]["workload"] +
                           "; sampHighOrder approach 1 (svdThresh 0.05, subFact 5, NUFFT W=6 os=1.5)"),
-           "iterations": nits, "svd_terms": Si.Ns, "setup_s": setup_s,
+           "iterations": nits, "svd_terms": Si.Ns, "setup_s": setup_s, "autotuned": tuned,
            "forward_rel_l2_vs_direct": rel_fwd, "solution_nrmse_vs_direct_solve": nrmse_vs_direct,
            "normal_operator_applications_per_s": {"interp": rate_i, "direct": rate_d},
            "roofline": roof}
@@ -893,6 +909,125 @@ def run_interp(a, mm, torch, dev):
                                          f"terms): one coil image forward + adjoint in {dt:.1f} s, scaled to "
                                          f"{W['R'].C} coils x {apps} applications"}
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+def finddel_problem(mm, N=160, C=16, seed=5):
+    """The reference's delay demo (demos/demo_autoDel.m) on synthetic images:
+    the reference's own spiral R4 field-probe record (tests/golden fixture:
+    15000 x 16 harmonic and 4 concomitant coefficients at 1 us; 5268 data
+    samples at 2.5 us, acquisition from 50 us), an N x N grid at the
+    trajectory's 1.35 mm resolution, C Gaussian coils, a smooth +-50 Hz B0
+    map, data simulated with the coefficients delayed by 2 us."""
+    import numpy as np
+    d = np.load(os.path.join(ROOT, "tests", "golden", "traj_spiral_R4.npz"))
+    rng = np.random.default_rng(seed)
+    fov = 0.00135 * N
+    gx = (np.arange(N) - N // 2) * (fov / N)
+    X, Y = np.meshgrid(gx, gx, indexing="ij")
+    grid = {"x": X, "y": Y, "z": np.full_like(X, 0.02)}
+    from scipy.ndimage import gaussian_filter
+    b = gaussian_filter(rng.standard_normal((N, N)), 15, mode="wrap")
+    b0 = 2 * np.pi * 50.0 * b / np.max(np.abs(b)) * 1e-6          # rad/us
+    return {"N": N, "C": C, "grid": grid, "b0": b0, "maps": mm.workloads.gaussian_coils(N, N, C, rng),
+            "phantom": mm.workloads.ellipse_phantom(N, N, rng),
+            "spha": np.ascontiguousarray(d["phs_spha"]), "conc": np.ascontiguousarray(d["phs_conc"]),
+            "tdwell": float(np.asarray(d["tdwelltraj"]).reshape(-1)[0]),
+            "datatime": d["datatime"].reshape(-1).astype(np.float64) + 50.0, "delay": 2.0}
+
+
+def run_finddel(a, mm, torch, dev):
+    """findDelAuto (findDel/findDelAuto.m:1-143) as the reference's demo runs
+    it (demos/demo_autoDel.m:48-86): the spiral R4 record, a 160 x 160 image,
+    16 coils, data with the trajectory delayed 2 us plus noise; maxNit_cgne =
+    the iterations a default cgne of at most 20 takes on the undelayed
+    operator, delJumpFact 3, no coarse search, started at 0.  One step = one
+    whole delay estimate: every sampHighOrder plan (host set-up and device
+    rank compression), cgne solve, setPhiDiv application and update, data
+    resident in HBM.  Also reported: the estimate and its steps, the operator
+    applications per estimate and the high-order GEMM's roofline."""
+    import ctypes as C
+    import numpy as np
+    P = finddel_problem(mm)
+    N, C_, dt, tdw = P["N"], P["C"], P["datatime"], P["tdwell"]
+    fd = mm.finddel
+    R = mm.rcvrOp(P["maps"])
+
+    def op(delay):
+        sa, sc = fd.interpTrajTime(P["spha"], P["conc"], tdw, delay, dt)
+        return mm.senseOp(mm.sampHighOrder(P["b0"], dt, sa.T, sc.T, P["grid"]), R)
+    x_true = torch.as_tensor(np.ascontiguousarray(P["phantom"].T), dtype=torch.complex64, device=dev)
+    y = op(P["delay"]) * x_true
+    g = torch.Generator(device=dev).manual_seed(5)
+    sig = float(torch.sqrt(torch.mean(torch.abs(y) ** 2))) / 1000.0
+    noise = torch.randn(y.shape, generator=g, device=dev) + 1j * torch.randn(y.shape, generator=g, device=dev)
+    y = (y + sig / np.sqrt(2) * noise).to(torch.complex64).contiguous()
+    # the demo's iteration count: a default cgne of at most 20 on the undelayed operator
+    _, res0, *_ = mm.cgne(op(0.0), y, None, 20)
+    nit = len(res0) - 1
+    args = (0.0, y, tdw, P["spha"], P["conc"], dt, P["grid"], P["b0"], P["maps"], nit, 3, 0)
+    # untimed warm-up, counting the operator applications of one estimate
+    solves = []
+    real_cgne = fd.cgne
+
+    def counting(*c_args, **kw):
+        out = real_cgne(*c_args, **kw)
+        solves.append(len(out[1]) - 1)
+        return out
+    fd.cgne = counting
+    try:
+        est, per = fd.findDelAuto(*args)
+    finally:
+        fd.cgne = real_cgne
+    apps = sum(2 * n + 5 for n in solves) + 2 * (len(per) - 1)
+    for _ in range(max(0, a.warmup - 1)):
+        fd.findDelAuto(*args)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        est, per = fd.findDelAuto(*args)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / a.steps
+    # the operator at the estimate: its dominant kernel (HIP events)
+    K = len(dt)
+    Ee = op(est)
+    ms = C.c_double()
+    name = C.create_string_buffer(64)
+    xin = torch.zeros(N * N, dtype=torch.complex64, device=dev)
+    yout = torch.empty(K * C_, dtype=torch.complex64, device=dev)
+    rc = mm.lib().mm_op_profile(Ee._plan.h, 0, xin.data_ptr(), yout.data_ptr(), 1, 10, C.byref(ms), name, 64)
+    L = P["spha"].shape[1] + P["conc"].shape[1] + 1
+    roof = None
+    if rc == 0 and ms.value > 0:
+        flops = 2.0 * K * N * N * L + 8.0 * K * N * N * C_
+        ach = flops / (ms.value * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP16_MFMA_TFLOPS, "traffic": None, "kernel": name.value.decode(),
+                "ms_per_launch": ms.value, "mfma_products_per_flop": 3}
+    out = {"metric": "findDelAuto delay estimates/s (the reference demo's setting)",
+           "value": 1.0 / el, "unit": "estimates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64", "data": "the reference's spiral R4 field-probe record; synthetic phantom, coils, B0, "
+                                   "noise; 2 us applied delay",
+           "config": {"workload": f"finddel: {N}x{N} c64 image at 1.35 mm, {C_} coils, {K} samples @2.5 us "
+                                  "(spiral R4, 16 SH + 4 concomitant terms + B0), findDelAuto from 0, "
+                                  f"{nit} cgne it/image, delJumpFact 3, no coarse search",
+                      "image": [N, N], "coils": C_, "samples": K, "phase_terms": L,
+                      "cgne_iterations": nit, "applied_delay_us": P["delay"]},
+           "estimate_us": est, "estimate_steps": [float(v) for v in per], "solves": len(solves),
+           "solve_iterations": solves, "operator_applications": apps, "roofline": roof}
+    if not a.no_cpu:
+        sa, sc = fd.interpTrajTime(P["spha"], P["conc"], tdw, est, dt)
+        r = cpu_highorder_per_app(P["b0"], dt, sa.T, sc.T, P["grid"], P["maps"], P["phantom"], a.cpu_seconds)
+        if r:
+            per_app, ks, dts, thr = r
+            out["cpu_baseline"] = {"value": 1.0 / (per_app * apps), "unit": "estimates/s", "cores": thr,
+                                   "kind": "port",
+                                   "sample": f"C/OpenMP FP64 restatement (oracle/c/mmref.c), one forward + one "
+                                             f"adjoint over {ks} of {K} samples ({dts:.2f} s), scaled to the "
+                                             f"{apps} operator applications of one estimate (plan set-ups not "
+                                             "counted)"}
+            out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 
@@ -1020,6 +1155,10 @@ def main():
     if a.workload == "interp":
         with torch.cuda.stream(stream):
             run_interp(a, mm, torch, dev)
+        return
+    if a.workload == "finddel":
+        with torch.cuda.stream(stream):
+            run_finddel(a, mm, torch, dev)
         return
     with torch.cuda.stream(stream):
         W = {"highorder": highorder_setup, "nufft": nufft_setup, "batch": batch_setup}[a.workload](mm, torch, dev)

@@ -269,7 +269,10 @@ int mm_op_get_option(mm_op* op, const char* key, int* value);
  * operator on a 2-D NUFFT, times candidate staged-gridding rectangle caps,
  * degridding block sizes and work orders on this device with the normal
  * operator's stage timers (`iters` applications each) and keeps the
- * fastest; the choice can be read back with mm_op_get_option.
+ * fastest; the choice can be read back with mm_op_get_option.  For the
+ * interpolated high-order operator (mm_hio_interp_create), or a SENSE
+ * operator on one, the same for its NUFFT at the batch it runs (images x
+ * SVD terms transforms; the SENSE operator's images are its coils).
  * MM_ERR_UNSUPPORTED for other operators. */
 int mm_op_autotune(mm_op* op, int iters);
 /* The hipStream_t the context runs on. */

@@ -54,3 +54,10 @@ def test_find_del_auto_vs_oracle(mm):
     est_o, per_o = o_findDelAuto(0.0, y, 1.0, d["phs_spha"], d["phs_conc"], dt, grid, b0, maps, 12)
     print(f"  device {est:.4f} ({len(per)} steps), oracle {est_o:.4f} ({len(per_o)} steps)", flush=True)
     assert abs(est - est_o) < 0.01 and abs(est - 2.0) < 0.02
+    # the data already on the device (this package's torch layout, [C, K])
+    # take the same path: the same estimate and steps
+    import torch
+    yd = torch.as_tensor(np.ascontiguousarray(np.asarray(y).T), dtype=torch.complex64, device="cuda")
+    est_d, per_d = mm.finddel.findDelAuto(0.0, yd, 1.0, d["phs_spha"], d["phs_conc"], dt, grid, b0, maps, 12)
+    print(f"  device-resident data: {est_d:.4f} ({len(per_d)} steps)", flush=True)
+    assert len(per_d) == len(per) and abs(est_d - est) < 1e-9

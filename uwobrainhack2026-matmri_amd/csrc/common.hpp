@@ -134,6 +134,10 @@ struct mm_op {
     // (`iters` applications each) and keep the fastest; false if the
     // operator has nothing to tune.
     virtual bool autotune(int iters) { (void)iters; return false; }
+    // The same for applications of nrep images at once, as a composite
+    // operator applies this one (a SENSE operator: its coils); false if
+    // there is nothing to tune.
+    virtual bool autotune_reps(int64_t nrep, int iters) { (void)nrep; (void)iters; return false; }
     // Time the operator's dominant kernel: `iters` back-to-back launches on
     // the context stream between two HIP events.  Returns ms per launch and
     // the kernel's name; false if the operator has no single dominant kernel.

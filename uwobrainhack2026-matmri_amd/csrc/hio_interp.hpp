@@ -21,6 +21,9 @@ class HioInterpOp : public mm_op {
     bool profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters, double* ms,
                           std::string* name) override;
     uint64_t config_gen() const override { return gen_ + nu_->config_gen(); }
+    // the NUFFT's options for applications of nrep images (nrep * Ns transforms)
+    bool autotune(int iters) override { return autotune_reps(1, iters); }
+    bool autotune_reps(int64_t nrep, int iters) override { return nu_->autotune_reps(nrep * Ns_, iters); }
     bool set_option(const std::string& key, int value) override { ++gen_; return nu_->set_option(key, value); }
     bool get_option(const std::string& key, int* value) const override { return nu_->get_option(key, value); }
 

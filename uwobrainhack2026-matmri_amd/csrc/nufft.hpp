@@ -127,11 +127,16 @@ class NufftOp : public mm_op {
     //   nu_degrid_t         unstaged degridding with one lane per (sample, coil) (1)
     bool set_option(const std::string& key, int value) override;
     bool get_option(const std::string& key, int* value) const override;
-    // The autotuner of the SENSE normal operator (nrep 1, C coils): times the
-    // candidate gridding caps, degridding block sizes, column-pass order and
-    // staged-item order with profile_normal (`iters` applications each),
-    // one option at a time, and keeps the fastest.
-    bool autotune_sense(const float2* maps, int C, const float* W, int iters);
+    // The autotuner of the normal operator -- SENSE-fused (maps of C coils,
+    // nrep 1) or, with maps NULL, nrep plain transforms (the gridding form):
+    // times the candidate gridding caps, degridding block sizes, column-pass
+    // order and staged-item order with profile_normal (`iters` applications
+    // each), one option at a time, and keeps the fastest.
+    bool autotune_batch(int64_t nrep, const float2* maps, int C, const float* W, int iters);
+    bool autotune_sense(const float2* maps, int C, const float* W, int iters) {
+        return autotune_batch(1, maps, C, W, iters);
+    }
+    bool autotune_reps(int64_t nrep, int iters) override { return autotune_batch(nrep, nullptr, 0, nullptr, iters); }
     void set_skip(const int* flags) override { skip_ = flags; }
     int ndim() const { return ndim_; }
     const std::vector<int64_t>& slice_V() const { return Vs_; }

@@ -283,7 +283,7 @@ bool SenseOp::profile_dominant(int adjoint, const float2* in, float2* out, int64
 }
 
 bool SenseOp::autotune(int iters) {
-    if (!nu_) return false;
+    if (!nu_) return S_->autotune_reps(C_, iters);
     return nu_->autotune_sense(static_cast<RcvrOp*>(R_)->maps(), C_, W, iters);
 }
 

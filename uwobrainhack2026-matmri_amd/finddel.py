@@ -79,15 +79,21 @@ def findDelAuto(delSk0, data_in, tdwell, phs_spha, phs_conc, datatime, phs_sph_g
     """``[delSk, delSk_perIt] = findDelAuto(delSk0, data_in, tdwell, phs_spha,
     phs_conc, datatime, phs_sph_grid, b0map, Crcvr, maxNit_cgne, delJumpFact,
     numCoarseSearch)`` (findDel/findDelAuto.m:1-143).  Times in one unit
-    (tdwell, datatime, b0map in rad/unit); ``data_in`` [K x C]; ``Crcvr`` has
-    the image size plus a coil dimension.  Every operator application and
-    solve runs on the GPU."""
+    (tdwell, datatime, b0map in rad/unit); ``data_in`` [K x C] (a host
+    array, or a GPU tensor in this package's torch layout, [C, K]: then it is
+    used in place); ``Crcvr`` has the image size plus a coil dimension.
+    Every operator application and solve runs on the GPU."""
     R = rcvrOp(Crcvr)
     t = np.asarray(datatime, dtype=np.float64).reshape(-1)
     dts = float(t[1] - t[0])
     dspha, dconc = _deriv(phs_spha, tdwell), _deriv(phs_conc, tdwell)
-    data = torch.as_tensor(np.ascontiguousarray(np.transpose(np.asarray(data_in))).astype(np.complex64),
-                           device="cuda")
+    if torch is not None and torch.is_tensor(data_in):
+        if not data_in.is_cuda:
+            raise ValueError("torch tensors must be on the GPU")
+        data = data_in.to(torch.complex64).contiguous()
+    else:
+        data = torch.as_tensor(np.ascontiguousarray(np.transpose(np.asarray(data_in))).astype(np.complex64),
+                               device="cuda")
 
     def build(dl):
         a, b = interpTrajTime(phs_spha, phs_conc, tdwell, dl, t)
