@@ -14,7 +14,7 @@
 // Phase precision.  |Phi| reaches 1e4 rad, where an FP32 ulp is 1e-3 rad,
 // so the contraction is done in FP64 and reduced modulo one revolution
 // before the FP32 sine/cosine.  The L phase terms are first compressed on
-// the host: B (V x L, FP64) = Q R by Gram-Schmidt with re-orthogonalisation,
+// the host: B (V x L, FP64) = Q R by Gram-Schmidt applied twice,
 // dropping directions whose residual is below 1e-13 of the column norm
 // (on a 2-D slice the 16 third-order harmonics span only the 10 planar
 // monomials), and the coefficients become R*h/(2*pi) (revolutions).
@@ -127,28 +127,12 @@ HioOp::HioOp(mm_ctx* c, int64_t V, int64_t K, int Lh, int Lc, const double* x, c
         for (int l = 0; l < Lc; ++l) H[(size_t)(Lh + l) * K + k] = g[(size_t)k * Lc + l];
         if (hasB0) H[(size_t)Lnb_ * K + k] = t[k];
     }
-    // max |Phi| (FP64, reported for the precision study)
-    {
-        double mx = 0;
-        const int64_t sv = std::max<int64_t>(1, V / 4096), sk = std::max<int64_t>(1, K / 4096);
-        for (int64_t v = 0; v < V; v += sv)
-            for (int64_t k = 0; k < K; k += sk) {
-                double p = 0;
-                for (int l = 0; l < L_; ++l) p += B[(size_t)v * L_ + l] * H[(size_t)l * K + k];
-                mx = std::max(mx, std::fabs(p));
-            }
-        // include the last sample, where the trajectory phase usually peaks
-        for (int64_t v = 0; v < V; v += sv) {
-            double p = 0;
-            for (int l = 0; l < L_; ++l) p += B[(size_t)v * L_ + l] * H[(size_t)l * K + (K - 1)];
-            mx = std::max(mx, std::fabs(p));
-        }
-        maxphi_ = mx;
-    }
-    // Gram-Schmidt (twice) on the columns of B: B = Q R
+    // B = Q R by classical Gram-Schmidt applied twice (as orthogonal as the
+    // modified form with re-orthogonalisation; each pass is two row-major
+    // sweeps over B's V x L layout instead of a strided walk per column)
     std::vector<double> Q((size_t)V * L_, 0.0), R((size_t)L_ * L_, 0.0);
     std::vector<int> keep;
-    std::vector<double> col(V);
+    std::vector<double> col(V), d(L_);
     for (int l = 0; l < L_; ++l) {
         double n0 = 0;
         for (int64_t v = 0; v < V; ++v) {
@@ -156,13 +140,22 @@ HioOp::HioOp(mm_ctx* c, int64_t V, int64_t K, int Lh, int Lc, const double* x, c
             n0 += col[v] * col[v];
         }
         n0 = std::sqrt(n0);
-        for (int pass = 0; pass < 2; ++pass)
-            for (size_t j = 0; j < keep.size(); ++j) {
-                double d = 0;
-                for (int64_t v = 0; v < V; ++v) d += Q[(size_t)v * L_ + j] * col[v];
-                R[j * L_ + l] += d;
-                for (int64_t v = 0; v < V; ++v) col[v] -= d * Q[(size_t)v * L_ + j];
+        const int nk = (int)keep.size();
+        for (int pass = 0; pass < 2 && nk > 0; ++pass) {
+            std::fill(d.begin(), d.begin() + nk, 0.0);
+            for (int64_t v = 0; v < V; ++v) {
+                const double* q = &Q[(size_t)v * L_];
+                const double cv = col[v];
+                for (int j = 0; j < nk; ++j) d[j] += q[j] * cv;
             }
+            for (int64_t v = 0; v < V; ++v) {
+                const double* q = &Q[(size_t)v * L_];
+                double sp = 0;
+                for (int j = 0; j < nk; ++j) sp += q[j] * d[j];
+                col[v] -= sp;
+            }
+            for (int j = 0; j < nk; ++j) R[(size_t)j * L_ + l] += d[j];
+        }
         double nr = 0;
         for (int64_t v = 0; v < V; ++v) nr += col[v] * col[v];
         nr = std::sqrt(nr);

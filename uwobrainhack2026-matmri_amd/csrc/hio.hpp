@@ -34,8 +34,6 @@ class HioOp : public mm_op {
     // mmaps = masked coil maps [Vm x C]; blas = a rocBLAS handle
     void dense_normal(const int* vidx, int64_t Vm, const float2* mmaps, int C, float2* S, float2* AtA, void* blas);
     int64_t samples() const { return K_; }
-    // max |Phi| in radians over the plan (host FP64, computed at creation)
-    double max_abs_phase() const { return maxphi_; }
 
     // the MFMA path is used when the padded rank is <= 16 and there is no phiDiv
     bool use_mfma_ = true;
@@ -58,7 +56,6 @@ class HioOp : public mm_op {
     int Lnb_ = 0;    // terms without the B0 column (phiDiv basis)
     int Lh_ = 0, Lc_ = 0;
     int num_cu_ = 256;   // compute units of the device (work-group slots of the MFMA GEMM)
-    double maxphi_ = 0;
     DevBuf<double> Q_;    // [V][r] compressed voxel basis
     DevBuf<double> Hc_;   // [K][r] compressed coefficients, revolutions
     // phiDiv: dPhi = Bnb * Hd  (FP64 basis without B0, coefficients in rad)
