@@ -913,27 +913,13 @@ def run_interp(a, mm, torch, dev):
 
 
 def finddel_problem(mm, N=160, C=16, seed=5):
-    """The reference's delay demo (demos/demo_autoDel.m) on synthetic images:
-    the reference's own spiral R4 field-probe record (tests/golden fixture:
-    15000 x 16 harmonic and 4 concomitant coefficients at 1 us; 5268 data
-    samples at 2.5 us, acquisition from 50 us), an N x N grid at the
-    trajectory's 1.35 mm resolution, C Gaussian coils, a smooth +-50 Hz B0
-    map, data simulated with the coefficients delayed by 2 us."""
+    """The reference's delay demo (demos/demo_autoDel.m) on synthetic images
+    (workloads.delay_demo_problem) over the reference's own spiral R4
+    field-probe record (tests/golden fixture: 15000 x 16 harmonic and 4
+    concomitant coefficients at 1 us; 5268 data samples at 2.5 us)."""
     import numpy as np
     d = np.load(os.path.join(ROOT, "tests", "golden", "traj_spiral_R4.npz"))
-    rng = np.random.default_rng(seed)
-    fov = 0.00135 * N
-    gx = (np.arange(N) - N // 2) * (fov / N)
-    X, Y = np.meshgrid(gx, gx, indexing="ij")
-    grid = {"x": X, "y": Y, "z": np.full_like(X, 0.02)}
-    from scipy.ndimage import gaussian_filter
-    b = gaussian_filter(rng.standard_normal((N, N)), 15, mode="wrap")
-    b0 = 2 * np.pi * 50.0 * b / np.max(np.abs(b)) * 1e-6          # rad/us
-    return {"N": N, "C": C, "grid": grid, "b0": b0, "maps": mm.workloads.gaussian_coils(N, N, C, rng),
-            "phantom": mm.workloads.ellipse_phantom(N, N, rng),
-            "spha": np.ascontiguousarray(d["phs_spha"]), "conc": np.ascontiguousarray(d["phs_conc"]),
-            "tdwell": float(np.asarray(d["tdwelltraj"]).reshape(-1)[0]),
-            "datatime": d["datatime"].reshape(-1).astype(np.float64) + 50.0, "delay": 2.0}
+    return mm.workloads.delay_demo_problem(d, N=N, C=C, seed=seed)
 
 
 def run_finddel(a, mm, torch, dev):

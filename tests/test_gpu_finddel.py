@@ -61,3 +61,28 @@ def test_find_del_auto_vs_oracle(mm):
     est_d, per_d = mm.finddel.findDelAuto(0.0, yd, 1.0, d["phs_spha"], d["phs_conc"], dt, grid, b0, maps, 12)
     print(f"  device-resident data: {est_d:.4f} ({len(per_d)} steps)", flush=True)
     assert len(per_d) == len(per) and abs(est_d - est) < 1e-9
+
+
+def test_find_del_auto_demo_record_vs_oracle(mm):
+    """findDelAuto in the delay demo's setting (demos/demo_autoDel.m:48-86:
+    the reference's spiral R4 record with its 16 harmonic and 4 concomitant
+    terms, 5268 samples, delJumpFact 3, no coarse search) reduced to 48 x 48
+    and 4 coils, on data the FP64 oracle simulated with a 2 us delay: every
+    step of the device search equals the oracle's (1e-4 us; the oracle's run
+    is kept in tests/golden/finddel_demo_48x4.json by make_finddel_demo.py)
+    and the estimate lands within 0.01 us of the applied delay."""
+    import json
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_finddel_demo import problem
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "finddel_demo_48x4.json")))
+    print("[case] findDelAuto demo record 48x48 C=4, delay 2 us", flush=True)
+    P, y = problem(gold["N"], gold["C"])
+    est, per = mm.finddel.findDelAuto(0.0, y, P["tdwell"], P["spha"], P["conc"], P["datatime"], P["grid"],
+                                      P["b0"], P["maps"], 20, 3, 0)
+    print(f"  device {est:.6f} steps {np.round(per, 4)}; oracle {gold['estimate_us']:.6f}", flush=True)
+    assert len(per) == len(gold["steps_us"])
+    assert np.max(np.abs(np.asarray(per) - np.asarray(gold["steps_us"]))) < 1e-4
+    assert abs(est - gold["applied_delay_us"]) < 0.01
+

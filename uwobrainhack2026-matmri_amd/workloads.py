@@ -148,6 +148,31 @@ def highorder_problem(seed=2, N=220, K=24000, C=32, L=16, fov=0.2, z=0.02, dwell
             "phantom": ph, "rng": rng}
 
 
+def delay_demo_problem(record, N=160, C=16, seed=5, delay=2.0):
+    """Inputs of the trajectory-delay demo (demos/demo_autoDel.m:12-59) on
+    synthetic images: a field-probe record (dict with phs_spha [K_in x 16],
+    phs_conc [K_in x 4], datatime and tdwelltraj, as the reference's
+    data_traj_spiral_R4.mat holds; acquisition from 50 us as the demo's
+    spiral case), an N x N grid at 1.35 mm (the record's resolution), C
+    Gaussian coils, an ellipse phantom and a smooth +-50 Hz B0 map in rad/us;
+    the data are to be simulated with the coefficients delayed by `delay`."""
+    rng = np.random.default_rng(seed)
+    fov = 0.00135 * N
+    gx = (np.arange(N) - N // 2) * (fov / N)
+    X, Y = np.meshgrid(gx, gx, indexing="ij")
+    grid = {"x": X, "y": Y, "z": np.full_like(X, 0.02)}
+    from scipy.ndimage import gaussian_filter
+    b = gaussian_filter(rng.standard_normal((N, N)), 15, mode="wrap")
+    b0 = 2 * np.pi * 50.0 * b / np.max(np.abs(b)) * 1e-6
+    maps = gaussian_coils(N, N, C, rng)
+    ph = ellipse_phantom(N, N, rng)
+    return {"N": N, "C": C, "grid": grid, "b0": b0, "maps": maps, "phantom": ph,
+            "spha": np.ascontiguousarray(np.asarray(record["phs_spha"], np.float64)),
+            "conc": np.ascontiguousarray(np.asarray(record["phs_conc"], np.float64)),
+            "tdwell": float(np.asarray(record["tdwelltraj"]).reshape(-1)[0]),
+            "datatime": np.asarray(record["datatime"], np.float64).reshape(-1) + 50.0, "delay": float(delay)}
+
+
 def batch_problems(seed=4, nslice=64, C=16, Nrange=(96, 160), Mrange=(6000, 20000)):
     """Per-slice inputs of the uneven batch: list of dicts (N, M, kloc, maps, phantom)."""
     out = []
