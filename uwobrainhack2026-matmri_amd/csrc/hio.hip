@@ -20,6 +20,10 @@
 // monomials), and the coefficients become R*h/(2*pi) (revolutions).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
 
 #include "fft.hpp"
 #include "hio.hpp"
@@ -28,6 +32,22 @@
 namespace mm {
 
 constexpr int kHioMaxRank = 24;
+
+// The factorization B = Q R of a voxel basis depends only on the grid, the
+// B0 map, the mask and the term counts, which every plan of a delay search
+// (or of any other sequence of plans over one slice) shares: the last few
+// are kept (up to 64 MB of host memory each), looked up by an exact
+// comparison of those inputs.
+struct HioQr {
+    int64_t V = 0;
+    int Lh = 0, Lc = 0, L = 0, flags = 0;
+    std::vector<double> key;   // x, y, z, then b0 and mask where given
+    std::vector<double> Q, R;  // [V][L], [L][L]
+    std::vector<int> keep;
+};
+static std::mutex g_qr_mu;
+static std::deque<std::shared_ptr<const HioQr>> g_qr;
+constexpr size_t kHioQrKeep = 4;
 constexpr int kHioMaxDiv = 24;
 constexpr int kHioTile = 64;
 
@@ -129,43 +149,87 @@ HioOp::HioOp(mm_ctx* c, int64_t V, int64_t K, int Lh, int Lc, const double* x, c
     }
     // B = Q R by classical Gram-Schmidt applied twice (as orthogonal as the
     // modified form with re-orthogonalisation; each pass is two row-major
-    // sweeps over B's V x L layout instead of a strided walk per column)
-    std::vector<double> Q((size_t)V * L_, 0.0), R((size_t)L_ * L_, 0.0);
-    std::vector<int> keep;
-    std::vector<double> col(V), d(L_);
-    for (int l = 0; l < L_; ++l) {
-        double n0 = 0;
-        for (int64_t v = 0; v < V; ++v) {
-            col[v] = B[(size_t)v * L_ + l];
-            n0 += col[v] * col[v];
-        }
-        n0 = std::sqrt(n0);
-        const int nk = (int)keep.size();
-        for (int pass = 0; pass < 2 && nk > 0; ++pass) {
-            std::fill(d.begin(), d.begin() + nk, 0.0);
-            for (int64_t v = 0; v < V; ++v) {
-                const double* q = &Q[(size_t)v * L_];
-                const double cv = col[v];
-                for (int j = 0; j < nk; ++j) d[j] += q[j] * cv;
+    // sweeps over B's V x L layout instead of a strided walk per column),
+    // unless a recent plan factorized the same basis
+    const int flags = (hasB0 ? 1 : 0) | (mask ? 2 : 0);
+    std::vector<double> key((size_t)V * (3 + (hasB0 ? 1 : 0) + (mask ? 1 : 0)));
+    {
+        double* kp = key.data();
+        for (const double* a : {x, y, z, hasB0 ? b0 : nullptr, mask})
+            if (a) {
+                std::memcpy(kp, a, sizeof(double) * (size_t)V);
+                kp += V;
             }
-            for (int64_t v = 0; v < V; ++v) {
-                const double* q = &Q[(size_t)v * L_];
-                double sp = 0;
-                for (int j = 0; j < nk; ++j) sp += q[j] * d[j];
-                col[v] -= sp;
-            }
-            for (int j = 0; j < nk; ++j) R[(size_t)j * L_ + l] += d[j];
-        }
-        double nr = 0;
-        for (int64_t v = 0; v < V; ++v) nr += col[v] * col[v];
-        nr = std::sqrt(nr);
-        if (n0 > 0 && nr > 1e-13 * n0) {
-            const size_t j = keep.size();
-            for (int64_t v = 0; v < V; ++v) Q[(size_t)v * L_ + j] = col[v] / nr;
-            R[j * L_ + l] = nr;
-            keep.push_back(l);
-        }
     }
+    std::shared_ptr<const HioQr> qr;
+    {
+        std::lock_guard<std::mutex> lk(g_qr_mu);
+        for (const auto& e : g_qr)
+            if (e->V == V && e->Lh == Lh && e->Lc == Lc && e->L == L_ && e->flags == flags &&
+                e->key.size() == key.size() &&
+                std::memcmp(e->key.data(), key.data(), sizeof(double) * key.size()) == 0) {
+                qr = e;
+                break;
+            }
+    }
+    if (!qr) {
+        auto e = std::make_shared<HioQr>();
+        e->V = V;
+        e->Lh = Lh;
+        e->Lc = Lc;
+        e->L = L_;
+        e->flags = flags;
+        e->key = std::move(key);
+        std::vector<double>& Q = e->Q;
+        std::vector<double>& R = e->R;
+        std::vector<int>& keep = e->keep;
+        Q.assign((size_t)V * L_, 0.0);
+        R.assign((size_t)L_ * L_, 0.0);
+        std::vector<double> col(V), d(L_);
+        for (int l = 0; l < L_; ++l) {
+            double n0 = 0;
+            for (int64_t v = 0; v < V; ++v) {
+                col[v] = B[(size_t)v * L_ + l];
+                n0 += col[v] * col[v];
+            }
+            n0 = std::sqrt(n0);
+            const int nk = (int)keep.size();
+            for (int pass = 0; pass < 2 && nk > 0; ++pass) {
+                std::fill(d.begin(), d.begin() + nk, 0.0);
+                for (int64_t v = 0; v < V; ++v) {
+                    const double* q = &Q[(size_t)v * L_];
+                    const double cv = col[v];
+                    for (int j = 0; j < nk; ++j) d[j] += q[j] * cv;
+                }
+                for (int64_t v = 0; v < V; ++v) {
+                    const double* q = &Q[(size_t)v * L_];
+                    double sp = 0;
+                    for (int j = 0; j < nk; ++j) sp += q[j] * d[j];
+                    col[v] -= sp;
+                }
+                for (int j = 0; j < nk; ++j) R[(size_t)j * L_ + l] += d[j];
+            }
+            double nr = 0;
+            for (int64_t v = 0; v < V; ++v) nr += col[v] * col[v];
+            nr = std::sqrt(nr);
+            if (n0 > 0 && nr > 1e-13 * n0) {
+                const size_t j = keep.size();
+                for (int64_t v = 0; v < V; ++v) Q[(size_t)v * L_ + j] = col[v] / nr;
+                R[j * L_ + l] = nr;
+                keep.push_back(l);
+            }
+        }
+        // entries up to 64 MB of host memory each (Q, R and the key)
+        if (sizeof(double) * (e->Q.size() + e->R.size() + e->key.size()) <= ((size_t)64 << 20)) {
+            std::lock_guard<std::mutex> lk(g_qr_mu);
+            g_qr.push_front(e);
+            if (g_qr.size() > kHioQrKeep) g_qr.pop_back();
+        }
+        qr = e;
+    }
+    const std::vector<double>& Q = qr->Q;
+    const std::vector<double>& R = qr->R;
+    const std::vector<int>& keep = qr->keep;
     r_ = (int)keep.size();
     if (r_ == 0) r_ = 1;  // all-zero phase: one zero column
     rp_ = ((r_ + 3) / 4) * 4;
