@@ -10,14 +10,20 @@
 
 namespace mm {
 
-// imgs[v, r + R n] = x[v, r] * sp[v, n]
+// imgs[v, r + R n] = x[v, r] * sp[v, n].  Transform b = r + R n on the
+// grid's y dimension (stride gridDim.y), voxels on x: the index split is one
+// 32-bit division per transform, not a 64-bit one per element (900 -> 739 us
+// at V = 220^2, 5824 transforms: the 2.25 GB of stores at 3.0 TB/s).
 __global__ __launch_bounds__(256) void hin_modulate_k(const float2* __restrict__ x, const float2* __restrict__ sp,
-                                                      float2* __restrict__ imgs, int64_t V, int64_t R, int Ns) {
-    const int64_t n = V * R * Ns;
-    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const int64_t v = i % V, b = i / V;
-        const int64_t r = b % R, t = b / R;
-        imgs[i] = cmulf(x[r * V + v], sp[t * V + v]);
+                                                      float2* __restrict__ imgs, int64_t V, int R, int Ns) {
+    const int nb = R * Ns;
+    for (int b = blockIdx.y; b < nb; b += gridDim.y) {
+        const int r = b % R, t = b / R;
+        const float2* xr = x + (int64_t)r * V;
+        const float2* st = sp + (int64_t)t * V;
+        float2* o = imgs + (int64_t)b * V;
+        for (int64_t v = blockIdx.x * (int64_t)256 + threadIdx.x; v < V; v += (int64_t)gridDim.x * 256)
+            o[v] = cmulf(xr[v], st[v]);
     }
 }
 
@@ -38,7 +44,9 @@ __global__ __launch_bounds__(256) void hin_combine_k(const float2* __restrict__ 
     }
 }
 
-// Y[k, r + R n] = conj(ph[k] * st[k, n]) * y[k, r]
+// Y[k, r + R n] = conj(ph[k] * st[k, n]) * y[k, r] (a flat grid-stride
+// loop measured faster here than the per-transform launch of hin_modulate_k:
+// 378 against 407 us at K = 24000, 5824 transforms)
 __global__ __launch_bounds__(256) void hin_spread_k(const float2* __restrict__ y, const float2* __restrict__ st,
                                                     const float2* __restrict__ ph, float2* __restrict__ Y, int64_t K,
                                                     int64_t R, int Ns) {
@@ -69,6 +77,14 @@ __global__ __launch_bounds__(256) void hin_demodulate_k(const float2* __restrict
 }
 
 static int hin_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+// (voxel or sample blocks, transforms) of the per-transform kernels: two
+// elements per thread along a transform, at most 512 rows of transforms (each
+// block then loops over every 512th), so blocks live long enough to keep
+// their stores streaming
+static dim3 hin_grid2(int64_t n, int64_t nb) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 511) / 512, 65535)),
+                (unsigned)std::max<int64_t>(1, std::min<int64_t>(nb, 512)));
+}
 
 HioInterpOp::HioInterpOp(mm_ctx* c, mm_op* nufft, int64_t V, int64_t K, int Ns, const float* svdSpace,
                          const float* svdTime, const float* phsShft)
@@ -105,8 +121,8 @@ bool HioInterpOp::profile_dominant(int adjoint, const float2* in, float2* out, i
     double t_in, t_out;
     if (!adjoint) {
         t_in = time_of([&] {
-            hipLaunchKernelGGL(hin_modulate_k, dim3(hin_grid(V_ * R * Ns_)), dim3(256), 0, st, in, sp_.p, imgs_.p, V_,
-                               R, Ns_);
+            hipLaunchKernelGGL(hin_modulate_k, hin_grid2(V_, R * Ns_), dim3(256), 0, st, in, sp_.p, imgs_.p, V_,
+                               (int)R, Ns_);
         });
         t_out = time_of([&] {
             hipLaunchKernelGGL(hin_combine_k, dim3(hin_grid(K_ * R)), dim3(256), 0, st, dat_.p, st_.p, ph_.p, out, K_,
@@ -144,12 +160,13 @@ bool HioInterpOp::profile_dominant(int adjoint, const float2* in, float2* out, i
 
 void HioInterpOp::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     if (nrep < 1) fail(MM_ERR_ARG, "sampHighOrder: nrep must be >= 1");
+    if (nrep * Ns_ >= INT32_MAX) fail(MM_ERR_UNSUPPORTED, "sampHighOrder: too many images x SVD terms");
     hipStream_t st = ctx->stream;
     const int64_t R = nrep;
     imgs_.ensure((size_t)(V_ * R * Ns_));
     dat_.ensure((size_t)(K_ * R * Ns_));
     if (!adjoint) {
-        hipLaunchKernelGGL(hin_modulate_k, dim3(hin_grid(V_ * R * Ns_)), dim3(256), 0, st, in, sp_.p, imgs_.p, V_, R,
+        hipLaunchKernelGGL(hin_modulate_k, hin_grid2(V_, R * Ns_), dim3(256), 0, st, in, sp_.p, imgs_.p, V_, (int)R,
                            Ns_);
         LAUNCHCHECK();
         nu_->apply(0, imgs_.p, dat_.p, R * Ns_);
