@@ -53,12 +53,12 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--workload", choices=["warmstart", "highorder", "nufft", "batch", "copy", "densemat", "interp",
-                                          "finddel"],
+                                          "finddel", "nufft3"],
                    default="warmstart",
                    help="copy: a plain 1 GiB device copy (HBM counter calibration, not a solve); densemat: "
                         "mrSampFuncMat's dense normal matrix at the high-order problem's full size; interp: the "
                         "high-order solve with sampHighOrder approach 1 (interpolated); finddel: findDelAuto's "
-                        "delay estimate on the high-order problem")
+                        "delay estimate in the reference demo's setting; nufft3: a 3-D stack-of-spirals SENSE solve")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--heldout", type=int, default=1000, help="held-out problems of the warm-start workload")
@@ -1017,6 +1017,113 @@ def run_finddel(a, mm, torch, dev):
     print(json.dumps(out), flush=True)
 
 
+def nufft3_stage_bytes(N, G, M, T):
+    """Algorithmic HBM bytes per launch of each 3-D NUFFT stage over T
+    transforms (complex64 values in and out; plan tables not counted)."""
+    N1, N2, N3 = N
+    G1, G2, G3 = G
+    V, Gc = N1 * N2 * N3, G1 * G2 * G3
+    r1, r2 = G1 * N2 * N3, G1 * G2 * N3   # after the row pass; after the g2 pass
+    return {
+        "n3_rows_fwd": 8.0 * T * (V + r1), "n3_axis2_fwd": 8.0 * T * (r1 + r2), "n3_axis3_fwd": 8.0 * T * (r2 + Gc),
+        "n3_degrid": 8.0 * (T * Gc + M + M * T), "n3_transpose_y": 2 * 8.0 * M * T,
+        "n3_grid_csr": 8.0 * (M * T + M + T * Gc), "n3_axis3_adj": 8.0 * T * (Gc + r2),
+        "n3_axis2_adj": 8.0 * T * (r2 + r1), "n3_rows_adj": 8.0 * T * (r1 + V), "n3_odd_ramp": 2 * 8.0 * T * Gc,
+    }
+
+
+def run_nufft3(a, mm, torch, dev):
+    """3-D SENSE cgne (nufftOp with a 3-D imN, nufftOp.m:89-274 in three
+    dimensions, composed with rcvrOp as mrSampFunc.m:97-107 does): a 128 x
+    128 x 64 volume, 8 coils, a stack of 64 golden-angle spirals of 4096
+    samples (262144), KB W = 4, os 1.5 (grid 192 x 192 x 96).  One step = one
+    cgne solve from zero to rel. resid 1e-4 (<= 20 iterations), data in HBM.
+    Roofline: the slowest stage of a forward or adjoint application (HIP
+    events after every launch, mm_op_profile)."""
+    import ctypes as C
+    import numpy as np
+    P = mm.workloads.nufft3_problem()
+    N, Cc, M = P["N"], P["C"], P["M"]
+    V = int(np.prod(N))
+    G = tuple(int(mm.workloads.grid_size(n, 1.5)) for n in N)
+    note("nufft3: plan (host set-up of the 3-D gridding tables)")
+    t0 = time.perf_counter()
+    S = mm.nufftOp(list(N), P["kloc"], None, 1, 1.5, 4)
+    setup_s = time.perf_counter() - t0
+    E = mm.senseOp(S, mm.rcvrOp(P["maps"]))
+    x_true = torch.as_tensor(np.ascontiguousarray(P["phantom"].transpose(2, 1, 0)), dtype=torch.complex64,
+                             device=dev)
+    y = E * x_true
+    g = torch.Generator(device=dev).manual_seed(6)
+    sig = float(torch.sqrt(torch.mean(torch.abs(y) ** 2))) / 40.0
+    noise = torch.randn(y.shape, generator=g, device=dev) + 1j * torch.randn(y.shape, generator=g, device=dev)
+    y = (y + sig / np.sqrt(2) * noise).to(torch.complex64).contiguous()
+    NitMax = 20
+    opt = {"resThresh": None, "resThreshRel": 1e-8, "xChangeThresh": None}
+    x0 = torch.zeros(V, dtype=torch.complex64, device=dev)
+    for _ in range(a.warmup):
+        mm.cgne(E, y, x0, NitMax, opt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nits = []
+    for _ in range(a.steps):
+        x, res, *_, st = mm.cgne(E, y, x0, NitMax, opt)
+        nits.append(len(res) - 1)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / a.steps
+    xr = torch.randn(V, dtype=torch.complex64, device=dev)
+    E.normal(xr)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        E.normal(xr)
+    torch.cuda.synchronize()
+    napp = 3 / (time.perf_counter() - t1)
+    # slowest stage of either direction (HIP events on the library stream)
+    byts = nufft3_stage_bytes(N, G, M, Cc)
+    roof, stages = None, {}
+    yo = torch.empty(M * Cc, dtype=torch.complex64, device=dev)
+    xo = torch.empty(V, dtype=torch.complex64, device=dev)
+    for adj, (xi, xo_) in ((0, (xr, yo)), (1, (yo, xo))):
+        ms = C.c_double()
+        name = C.create_string_buffer(64)
+        if mm.lib().mm_op_profile(E._plan.h, adj, xi.data_ptr(), xo_.data_ptr(), 1, 5, C.byref(ms), name, 64) == 0:
+            stages[name.value.decode()] = ms.value
+    if stages:
+        kname, kms = max(stages.items(), key=lambda kv: kv[1])
+        b = byts.get(kname)
+        if b:
+            ach = b / (kms * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                    "traffic": read_traffic(kname, "nufft3"), "kernel": kname, "ms_per_launch": kms,
+                    "algorithmic_bytes_per_launch": b}
+    out = {"metric": "CG-SENSE solves/s at rel. resid 1e-4, 3-D NUFFT (stack of spirals)",
+           "value": 1.0 / el, "unit": "solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64", "data": "synthetic (seeded phantom, coils, trajectory, noise)",
+           "config": {"workload": "nufft3: 128x128x64 c64 volume, 8 coils, stack of 64 golden-angle spirals x 4096 "
+                                  "samples, KB W=4 os=1.5 (grid 192x192x96), cgne to rel. resid 1e-4 (<=20 it), "
+                                  "zero init", "image": list(N), "coils": Cc, "samples": M, "grid": list(G),
+                      "max_iterations": NitMax, "rel_resid": 1e-4},
+           "iterations": nits, "plan_setup_s": setup_s, "normal_operator_applications_per_s": napp,
+           "dominant_stage_ms": stages, "roofline": roof}
+    if not a.no_cpu:
+        # the oracle's FP64 3-D NUFFT and coil operator: one coil image forward
+        # and adjoint, scaled to the coils and the solve's applications
+        from oracle.nufft import NufftPlan
+        So = NufftPlan(list(N), P["kloc"], None, 1.5, 4)
+        xc = P["phantom"] * P["maps"][..., 0]
+        t0 = time.perf_counter()
+        So.adjoint(So.forward(xc))
+        dt = time.perf_counter() - t0
+        apps = 2 * max(nits) + 5
+        out["cpu_baseline"] = {"value": 1.0 / (dt / 2 * Cc * apps), "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": f"NumPy FP64 oracle 3-D NUFFT: one coil volume forward + adjoint in {dt:.1f} "
+                                         f"s, scaled to {Cc} coils x {apps} applications"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
 PEAK_FP32_MFMA_TFLOPS = 157.3   # MI355X dense FP32 matrix peak (rocBLAS cherk of the dense normal matrix)
 
 
@@ -1145,6 +1252,10 @@ def main():
     if a.workload == "finddel":
         with torch.cuda.stream(stream):
             run_finddel(a, mm, torch, dev)
+        return
+    if a.workload == "nufft3":
+        with torch.cuda.stream(stream):
+            run_nufft3(a, mm, torch, dev)
         return
     with torch.cuda.stream(stream):
         W = {"highorder": highorder_setup, "nufft": nufft_setup, "batch": batch_setup}[a.workload](mm, torch, dev)

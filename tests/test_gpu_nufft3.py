@@ -1,6 +1,7 @@
 """GPU parity of the 3-D NUFFT (nufftOp with a 3-D imN) against the FP64
 oracle: forward and adjoint at the per-application bound 1e-5 (even and odd
-sides, W 4/6/8, os 1.5/2, dcf, 1 to 20 transforms: partly filled chunks),
+sides, W 4/6/8, os 1.5/2, dcf, 1 to 20 transforms: partly filled chunks,
+grid sides up to 486 along each axis),
 the reference's on-grid known answer in 3-D (@nufftOp/tests.m:90-101), and a
 3-D SENSE cgne against the FP64 loop.  The oracle's 3-D operator is pinned by
 the brute-force NUDFT (tests/test_oracle_nufft.py::test_vs_nudft) and the
@@ -46,6 +47,15 @@ CASES_3D = [
     ((24, 20, 18), 4000, 8, 2.0, True, 1),
     ((20, 20, 20), 3000, 4, 1.5, False, 16),
     ((20, 18, 9), 2000, 6, 1.375, True, 20),
+    # sides above what the default FFT kernels hold in one LDS buffer: the
+    # doubled-register-cap kernels along each axis, 16 and 8 transforms, and
+    # a side of 486 points where they run two buffers
+    ((176, 16, 12), 3000, 4, 1.5, False, 16),
+    ((16, 176, 12), 3000, 4, 1.5, True, 16),
+    ((12, 10, 200), 2500, 6, 1.5, False, 3),
+    ((200, 12, 10), 2500, 4, 1.5, True, 3),
+    ((324, 8, 8), 2000, 4, 1.5, False, 16),
+    ((12, 176, 10), 2500, 4, 1.5, False, 8),
 ]
 
 
@@ -59,6 +69,25 @@ def test_nufft3_vs_oracle(mm, imN, M, W, os_, use_dcf, nrep):
     Sg = mm.nufftOp(list(imN), kloc, dcf, 1, os_, W)
     x = crand(rng, imN + (nrep,))
     y = crand(rng, (M, nrep))
+    e_f = rel(Sg * x, So.forward(x))
+    e_a = rel(Sg.H * y, So.adjoint(y))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+
+
+@pytest.mark.parametrize("nrep", [1, 8, 16])
+def test_nufft3_dense_centre_vs_oracle(mm, nrep):
+    """Samples concentrated at the k-space centre (rows of several hundred
+    entries: the gridding's one-wave-per-cell path, 64 / 8 / 4 entry phases)
+    against the oracle in both directions (1e-5)."""
+    print(f"[case] 3-D nufft dense centre nrep={nrep}", flush=True)
+    rng = np.random.default_rng(120 + nrep)
+    imN, M = (12, 16, 20), 20000
+    kloc = np.clip(rng.normal(0.0, 0.04, (M, 3)), -0.5, 0.4999)
+    So = NufftPlan(list(imN), kloc, None, 1.5, 6)
+    Sg = mm.nufftOp(list(imN), kloc, None, 1, 1.5, 6)
+    x = crand(rng, imN + ((nrep,) if nrep > 1 else ()))
+    y = crand(rng, (M, nrep) if nrep > 1 else (M,))
     e_f = rel(Sg * x, So.forward(x))
     e_a = rel(Sg.H * y, So.adjoint(y))
     print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)

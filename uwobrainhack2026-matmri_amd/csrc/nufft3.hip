@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "nufft.hpp"
 #include "nufft3.hpp"
@@ -37,10 +38,11 @@
 namespace mm {
 
 // transforms per work-group of the FFT passes: a chunk's BCT transforms of
-// T / BCT pencils (16 for 16-transform chunks, else 8)
+// T / BCT pencils (16 transforms for chunks of 8 or 16: each grid position
+// a whole 128-byte line, one work-group per line; else 8)
 template <int BCT>
 __host__ __device__ constexpr int n3_T() {
-    return BCT >= 16 ? 16 : 8;
+    return BCT >= 8 ? 16 : 8;
 }
 
 __device__ __forceinline__ int wrap_pos(int i, int s, int n) {
@@ -48,18 +50,34 @@ __device__ __forceinline__ int wrap_pos(int i, int s, int n) {
     return p < 0 ? p + n : p;
 }
 
-// LDS of a pass over sides of length n: two buffers of n*T values and the
-// n - 1 pass twiddles
+// Values a thread holds in registers in a single-buffer pass: the default
+// cap (16 at 16 transforms, 8 at 8), or, in the kernels for sides the
+// default cannot hold in one LDS buffer (MID = 1: above 256 points at 16
+// transforms, 512 at 8), twice that, so every pass over a side of up to 512
+// points fits one buffer unless its radices leave the butterflies short of
+// it (420, 486..504: then two buffers).  Those kernels of 16 transforms take
+// up to 168 registers (3 waves per SIMD), of 8 up to 128.
+template <int T, int MID>
+__host__ __device__ constexpr int n3_cap() {
+    return MID ? 2 * fft_reg_cap(T, 0) : fft_reg_cap(T, 0);
+}
+template <int BCT, int MID>
+__host__ __device__ constexpr int n3_waves() {
+    return MID ? (BCT >= 16 ? 3 : 4) : 1;
+}
+
+// LDS of a pass over sides of length n: one (single-buffer passes) or two
+// buffers of n*T values and the n - 1 pass twiddles
 template <int BCT>
-static size_t n3_lds(int n) {
-    return (size_t)(2 * n * n3_T<BCT>() + n) * sizeof(float2);
+static size_t n3_lds(int n, bool ip) {
+    return (size_t)((ip ? 1 : 2) * n * n3_T<BCT>() + n) * sizeof(float2);
 }
 
 // Rows, forward: lines L = i2 + N2 i3 (i2 < N2, i3 < N3), T / BCT of them per
 // work-group.  Image reads run along i1 (coalesced), grid writes along
 // (g1, b).
-template <int BCT>
-__global__ __launch_bounds__(256) void n3_rows_fwd(Nu3Geom g, const FftFactors* __restrict__ fac,
+template <int BCT, int MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BCT, MID>(), 8))) void n3_rows_fwd(Nu3Geom g, const FftFactors* __restrict__ fac,
                                                    const float2* __restrict__ twp,
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ x, float2* __restrict__ ws,
@@ -68,9 +86,10 @@ __global__ __launch_bounds__(256) void n3_rows_fwd(Nu3Geom g, const FftFactors* 
     extern __shared__ float2 lds[];
     const FftFactors& F = *fac;
     const int n = g.G1;
+    const bool ip = g.ip & 1;
     float2* A = lds;
-    float2* B = lds + n * T;
-    float2* tw = lds + 2 * n * T;
+    float2* B = ip ? nullptr : lds + n * T;
+    float2* tw = lds + (ip ? 1 : 2) * n * T;
     const int L0 = (int)blockIdx.x * LPG;
     const int nl = g.N2 * g.N3;
     for (int i = threadIdx.x; i < n * T; i += kFftThreads) A[i] = make_float2(0.f, 0.f);
@@ -89,7 +108,7 @@ __global__ __launch_bounds__(256) void n3_rows_fwd(Nu3Geom g, const FftFactors* 
         A[wrap_pos(i1, g.s1, n) * T + t] = make_float2(v.x * f, v.y * f);
     }
     __syncthreads();
-    const float2* R = fft_lds<T, 1>(A, B, F, tw);
+    const float2* R = fft_lds<T, 1, false, false, n3_cap<T, MID>()>(A, B, F, tw);
     for (int idx = threadIdx.x; idx < n * T; idx += kFftThreads) {
         const int b = idx % BCT, q = (idx / BCT) % n, j = idx / (BCT * n);
         const int L = L0 + j;
@@ -105,8 +124,8 @@ __global__ __launch_bounds__(256) void n3_rows_fwd(Nu3Geom g, const FftFactors* 
 // (q - s) mod G, all G outputs.  Adjoint: all G inputs, outputs q < N_ax
 // read from (q - s) mod G.  A work-group reads its pencils whole (into LDS)
 // before writing any of them.
-template <int BCT, int ADJ>
-__global__ __launch_bounds__(256) void n3_axis(Nu3Geom g, int ax, const FftFactors* __restrict__ fac,
+template <int BCT, int ADJ, int MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BCT, MID>(), 8))) void n3_axis(Nu3Geom g, int ax, const FftFactors* __restrict__ fac,
                                                const float2* __restrict__ twp, float2* __restrict__ ws) {
     constexpr int T = n3_T<BCT>(), LPG = T / BCT, SGN = ADJ ? -1 : 1;
     extern __shared__ float2 lds[];
@@ -126,9 +145,10 @@ __global__ __launch_bounds__(256) void n3_axis(Nu3Geom g, int ax, const FftFacto
         base = ((int64_t)outer * g.G1 + g1_0) * BCT;
         qs = (int64_t)g.G2 * g.G1 * BCT;
     }
+    const bool ip = (g.ip >> (ax - 1)) & 1;
     float2* A = lds;
-    float2* B = lds + n * T;
-    float2* tw = lds + 2 * n * T;
+    float2* B = ip ? nullptr : lds + n * T;
+    float2* tw = lds + (ip ? 1 : 2) * n * T;
     for (int i = threadIdx.x; i < n - 1; i += kFftThreads) tw[i] = twp[i];
     if (!ADJ) {
         for (int i = threadIdx.x; i < n * T; i += kFftThreads) A[i] = make_float2(0.f, 0.f);
@@ -144,7 +164,7 @@ __global__ __launch_bounds__(256) void n3_axis(Nu3Geom g, int ax, const FftFacto
         }
     }
     __syncthreads();
-    const float2* R = fft_lds<T, SGN>(A, B, F, tw);
+    const float2* R = fft_lds<T, SGN, false, false, n3_cap<T, MID>()>(A, B, F, tw);
     if (!ADJ) {
         for (int idx = threadIdx.x; idx < n * T; idx += kFftThreads) {
             const int t = idx % T, q = idx / T;
@@ -160,8 +180,8 @@ __global__ __launch_bounds__(256) void n3_axis(Nu3Geom g, int ax, const FftFacto
 
 // Rows, adjoint: inverse FFT along g1 of rows i2 < N2 of planes i3 < N3,
 // crop (pixel i1 from (i1 - s1) mod G1) and deapodisation.
-template <int BCT>
-__global__ __launch_bounds__(256) void n3_rows_adj(Nu3Geom g, const FftFactors* __restrict__ fac,
+template <int BCT, int MID>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BCT, MID>(), 8))) void n3_rows_adj(Nu3Geom g, const FftFactors* __restrict__ fac,
                                                    const float2* __restrict__ twp,
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ ws, float2* __restrict__ x,
@@ -170,9 +190,10 @@ __global__ __launch_bounds__(256) void n3_rows_adj(Nu3Geom g, const FftFactors* 
     extern __shared__ float2 lds[];
     const FftFactors& F = *fac;
     const int n = g.G1;
+    const bool ip = g.ip & 1;
     float2* A = lds;
-    float2* B = lds + n * T;
-    float2* tw = lds + 2 * n * T;
+    float2* B = ip ? nullptr : lds + n * T;
+    float2* tw = lds + (ip ? 1 : 2) * n * T;
     const int L0 = (int)blockIdx.x * LPG;
     const int nl = g.N2 * g.N3;
     for (int i = threadIdx.x; i < n - 1; i += kFftThreads) tw[i] = twp[i];
@@ -187,7 +208,7 @@ __global__ __launch_bounds__(256) void n3_rows_adj(Nu3Geom g, const FftFactors* 
         A[q * T + j * BCT + b] = v;
     }
     __syncthreads();
-    const float2* R = fft_lds<T, -1>(A, B, F, tw);
+    const float2* R = fft_lds<T, -1, false, false, n3_cap<T, MID>()>(A, B, F, tw);
     const float* d1 = deap;
     const float* d2 = deap + g.N1;
     const float* d3 = d2 + g.N2;
@@ -298,18 +319,44 @@ __global__ __launch_bounds__(256) void n3_transpose_y(Nu3Geom g, const float2* _
     if (m2 < g.M) yT[(int64_t)m2 * BCT + b2] = st[b2 * SPB + ml2];
 }
 
+// Rows longer than this go to n3_grid_long (one wave per cell).
+constexpr int kN3LongRow = 64;
+
 // Gridding as an output-stationary gather: lane = (cell, transform) sums
-// its cell's CSR row in sample order; every cell written once.
+// its cell's CSR row in sample order; every cell written once (here, or by
+// n3_grid_long when its row exceeds kN3LongRow entries).  The row is
+// walked eight entries at a time: their entry loads, then their sample
+// loads, are issued together, so a row waits on two load latencies per eight
+// entries instead of two per entry (same sums, same order; measured faster
+// than predicated batches for the short rows, 325 against 337 us, and than
+// four cells per lane, 448 us, on a 192 x 192 x 96 grid).
 template <int BCT>
 __global__ __launch_bounds__(256) void n3_grid_csr(const int* __restrict__ rp, const int2* __restrict__ ent,
                                                    const float2* __restrict__ yT, float2* __restrict__ ws,
                                                    int64_t ncell) {
+    constexpr int U = 8;
     const int64_t cell = (int64_t)blockIdx.x * (256 / BCT) + threadIdx.x / BCT;
     const int b = threadIdx.x % BCT;
     if (cell >= ncell) return;
     const int e0 = rp[cell], e1 = rp[cell + 1];
+    if (e1 - e0 > kN3LongRow) return;
     float2 acc = make_float2(0.f, 0.f);
-    for (int e = e0; e < e1; ++e) {
+    int e = e0;
+    for (; e + U <= e1; e += U) {
+        int2 en[U];
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) en[u] = ent[e + u];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = yT[(int64_t)en[u].x * BCT + b];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float w = __int_as_float(en[u].y);
+            acc.x += w * v[u].x;
+            acc.y += w * v[u].y;
+        }
+    }
+    for (; e < e1; ++e) {
         const int2 en = ent[e];
         const float w = __int_as_float(en.y);
         const float2 v = yT[(int64_t)en.x * BCT + b];
@@ -317,6 +364,53 @@ __global__ __launch_bounds__(256) void n3_grid_csr(const int* __restrict__ rp, c
         acc.y += w * v.y;
     }
     ws[cell * BCT + b] = acc;
+}
+
+// Gridding of the long rows (the k-space centre): one wave per listed cell,
+// lane = (phase j, transform b) with P = 64 / BCT phases; lane j sums entries
+// e0 + j, e0 + j + P, ... in order (four entries' loads issued together),
+// then the phases are added by a fixed shuffle tree, so every cell's sum is
+// the same from run to run.
+template <int BCT>
+__global__ __launch_bounds__(256) void n3_grid_long(const int* __restrict__ cells, int nlong,
+                                                    const int* __restrict__ rp, const int2* __restrict__ ent,
+                                                    const float2* __restrict__ yT, float2* __restrict__ ws) {
+    constexpr int P = 64 / BCT, U = 4;
+    const int wave = (int)((blockIdx.x * 256 + threadIdx.x) >> 6);
+    if (wave >= nlong) return;   // uniform over the wave
+    const int lane = threadIdx.x & 63;
+    const int j = lane / BCT, b = lane % BCT;
+    const int64_t cell = cells[wave];
+    const int e0 = rp[cell], e1 = rp[cell + 1];
+    float2 acc = make_float2(0.f, 0.f);
+    int e = e0 + j;
+    for (; e + (U - 1) * P < e1; e += U * P) {
+        int2 en[U];
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) en[u] = ent[e + u * P];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = yT[(int64_t)en[u].x * BCT + b];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float w = __int_as_float(en[u].y);
+            acc.x += w * v[u].x;
+            acc.y += w * v[u].y;
+        }
+    }
+    for (; e < e1; e += P) {
+        const int2 en = ent[e];
+        const float w = __int_as_float(en.y);
+        const float2 v = yT[(int64_t)en.x * BCT + b];
+        acc.x += w * v.x;
+        acc.y += w * v.y;
+    }
+#pragma unroll
+    for (int off = BCT; off < 64; off <<= 1) {
+        acc.x += __shfl_xor(acc.x, off);
+        acc.y += __shfl_xor(acc.y, off);
+    }
+    if (j == 0) ws[cell * BCT + b] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -364,6 +458,7 @@ Nufft3Op::Nufft3Op(mm_ctx* c, const int64_t* imN, int64_t M, const double* kloc,
     std::vector<float2> twid;
     for (int d = 0; d < 3; ++d) {
         if (!fft_factorize(G[d], facs[d], false)) fail(MM_ERR_ARG, "nufftOp: grid size with a prime factor > 7");
+        fach_[d] = facs[d];
         tw_off_[d] = (int)twid.size();
         fft_pass_twiddles(facs[d], [&](float cs, float sn) { twid.push_back(make_float2(cs, sn)); });
     }
@@ -448,28 +543,49 @@ Nufft3Op::Nufft3Op(mm_ctx* c, const int64_t* imN, int64_t M, const double* kloc,
     wt_.upload(wt, st);
     dcf_.upload(dcfv, st);
     rp_.upload(rp, st);
+    {
+        std::vector<int> lc;
+        for (int64_t i = 0; i < ncell; ++i)
+            if (rp[(size_t)i + 1] - rp[(size_t)i] > kN3LongRow) lc.push_back((int)i);
+        nlong_ = (int)lc.size();
+        long_.upload(lc.empty() ? std::vector<int>{0} : lc, st);
+    }
     ent_.upload(ent, st);
     HIPCHECK(hipStreamSynchronize(st));
 }
 
-template <int BCT>
+template <int BCT, int MID>
 void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) {
     hipStream_t st = ctx->stream;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)n3_rows_fwd<BCT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)n3_rows_fwd<BCT, MID>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        (void)hipFuncSetAttribute((const void*)n3_rows_adj<BCT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)n3_rows_adj<BCT, MID>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 0, MID>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 1, MID>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         (void)hipGetLastError();
         attr_set = true;
     }
     constexpr int T = n3_T<BCT>(), LPG = T / BCT, SPB = 256 / BCT;
     Nu3Geom g = g_;
+    g.ip = 0;
+    for (int d = 0; d < 3; ++d)
+        if (fft_inplace_ok(fach_[d], T, n3_cap<T, MID>())) g.ip |= 1 << d;
+    const size_t lds1 = n3_lds<BCT>(g.G1, g.ip & 1), lds2 = n3_lds<BCT>(g.G2, (g.ip >> 1) & 1),
+                 lds3 = n3_lds<BCT>(g.G3, (g.ip >> 2) & 1);
+    // after each launch: the launch check and, while profiling, an event
+    auto done = [&](const char* stage) {
+        LAUNCHCHECK();
+        if (marks_ && nmark_ < marks_->size()) {
+            (*marks_)[nmark_].first = stage;
+            HIPCHECK(hipEventRecord((*marks_)[nmark_].second, ctx->stream));
+            ++nmark_;
+        }
+    };
     const int64_t ncell = (int64_t)g.G1 * g.G2 * g.G3;
     g.chunk = ncell * BCT;
     ws_.ensure((size_t)g.chunk);
@@ -484,66 +600,71 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
         const int lanes = (int)std::min<int64_t>(BCT, nrep - ch * BCT);
         if (toep_) {
             // both directions: forward FFTs, spectrum multiply, inverse FFTs
-            hipLaunchKernelGGL(n3_rows_fwd<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+            hipLaunchKernelGGL((n3_rows_fwd<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
                                twid_.p + tw_off_[0], toep_deap_.p, in, ws_.p, (int)ch, lanes);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+            done("n3_rows_fwd");
+            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+            done("n3_axis2_fwd");
+            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
                                facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
-            LAUNCHCHECK();
+            done("n3_axis3_fwd");
             hipLaunchKernelGGL(n3_toep_mul<BCT>, dim3(elem_blocks), dim3(256), 0, st, g, (const float2*)toep_k_.p,
                                ws_.p);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+            done("n3_toep_mul");
+            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
                                facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+            done("n3_axis3_adj");
+            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL(n3_rows_adj<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+            done("n3_axis2_adj");
+            hipLaunchKernelGGL((n3_rows_adj<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
                                twid_.p + tw_off_[0], toep_deap_.p, (const float2*)ws_.p, out, (int)ch, lanes);
-            LAUNCHCHECK();
+            done("n3_rows_adj");
             continue;
         }
         if (!adjoint) {
-            hipLaunchKernelGGL(n3_rows_fwd<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+            hipLaunchKernelGGL((n3_rows_fwd<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
                                twid_.p + tw_off_[0], deap_.p, in, ws_.p, (int)ch, lanes);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+            done("n3_rows_fwd");
+            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL((n3_axis<BCT, 0>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+            done("n3_axis2_fwd");
+            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
                                facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
-            LAUNCHCHECK();
+            done("n3_axis3_fwd");
             if (g.odd) {
                 hipLaunchKernelGGL((n3_odd_ramp<BCT, 0>), dim3(elem_blocks), dim3(256), 0, st, g, ws_.p);
-                LAUNCHCHECK();
+                done("n3_odd_ramp");
             }
             hipLaunchKernelGGL(n3_degrid<BCT>, dim3(samp_blocks), dim3(256), 0, st, g, base_.p, wt_.p, dcf_.p,
                                (const float2*)ws_.p, out, (int)ch, lanes);
-            LAUNCHCHECK();
+            done("n3_degrid");
         } else {
             hipLaunchKernelGGL(n3_transpose_y<BCT>, dim3(samp_blocks), dim3(256), 0, st, g, in, yT_.p, (int)ch,
                                lanes);
-            LAUNCHCHECK();
+            done("n3_transpose_y");
             hipLaunchKernelGGL(n3_grid_csr<BCT>, dim3(cell_blocks), dim3(256), 0, st, rp_.p, ent_.p,
                                (const float2*)yT_.p, ws_.p, ncell);
-            LAUNCHCHECK();
+            done("n3_grid_csr");
+            if (nlong_ > 0) {
+                hipLaunchKernelGGL(n3_grid_long<BCT>, dim3((unsigned)((nlong_ + 3) / 4)), dim3(256), 0, st, long_.p,
+                                   nlong_, rp_.p, ent_.p, (const float2*)yT_.p, ws_.p);
+                done("n3_grid_csr");   // the gridding stage: both launches
+            }
             if (g.odd) {
                 hipLaunchKernelGGL((n3_odd_ramp<BCT, 1>), dim3(elem_blocks), dim3(256), 0, st, g, ws_.p);
-                LAUNCHCHECK();
+                done("n3_odd_ramp");
             }
-            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.G2), dim3(256), n3_lds<BCT>(g.G3), st, g, 3,
+            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
                                facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL((n3_axis<BCT, 1>), dim3(gx * (unsigned)g.N3), dim3(256), n3_lds<BCT>(g.G2), st, g, 2,
+            done("n3_axis3_adj");
+            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
-            LAUNCHCHECK();
-            hipLaunchKernelGGL(n3_rows_adj<BCT>, dim3(rows_blocks), dim3(256), n3_lds<BCT>(g.G1), st, g, facs_.p,
+            done("n3_axis2_adj");
+            hipLaunchKernelGGL((n3_rows_adj<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
                                twid_.p + tw_off_[0], deap_.p, (const float2*)ws_.p, out, (int)ch, lanes);
-            LAUNCHCHECK();
+            done("n3_rows_adj");
         }
     }
 }
@@ -629,17 +750,76 @@ void Nufft3Op::set_toeplitz(const float2* psf) {
     ++gen_;
 }
 
+bool Nufft3Op::profile_dominant(int adjoint, const float2* in, float2* out, int64_t nrep, int iters, double* ms,
+                                std::string* name) {
+    if ((adjoint != 0 && adjoint != 1) || nrep < 1 || iters < 1) return false;
+    hipStream_t st = ctx->stream;
+    apply(adjoint, in, out, nrep);   // workspace, first-launch costs
+    // chunks of apply's transform count, at most seven marked launches each
+    int64_t bct = 1;
+    while (bct < nrep && bct < 16) bct <<= 1;
+    const int64_t nchunk = (nrep + bct - 1) / bct;
+    std::vector<std::pair<const char*, hipEvent_t>> mk((size_t)(8 * nchunk));
+    hipEvent_t e0;
+    HIPCHECK(hipEventCreate(&e0));
+    for (auto& m : mk) HIPCHECK(hipEventCreate(&m.second));
+    std::map<std::string, double> acc;
+    try {
+        for (int it = 0; it < iters; ++it) {
+            nmark_ = 0;
+            HIPCHECK(hipEventRecord(e0, st));
+            marks_ = &mk;
+            apply(adjoint, in, out, nrep);
+            marks_ = nullptr;
+            if (nmark_ == 0) break;
+            HIPCHECK(hipEventSynchronize(mk[nmark_ - 1].second));
+            hipEvent_t prev = e0;
+            for (size_t j = 0; j < nmark_; ++j) {
+                float t = 0.f;
+                HIPCHECK(hipEventElapsedTime(&t, prev, mk[j].second));
+                acc[mk[j].first] += t;
+                prev = mk[j].second;
+            }
+        }
+    } catch (...) {
+        marks_ = nullptr;
+        (void)hipEventDestroy(e0);
+        for (auto& m : mk) (void)hipEventDestroy(m.second);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    for (auto& m : mk) (void)hipEventDestroy(m.second);
+    if (acc.empty()) return false;
+    auto best = std::max_element(acc.begin(), acc.end(),
+                                 [](const auto& a, const auto& b) { return a.second < b.second; });
+    *ms = best->second / iters;
+    *name = best->first;
+    return true;
+}
+
 void Nufft3Op::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");
     int bct = 1;
     while (bct < nrep && bct < 16) bct <<= 1;
-    switch (bct) {
-        case 1: launch<1>(adjoint, in, out, nrep); break;
-        case 2: launch<2>(adjoint, in, out, nrep); break;
-        case 4: launch<4>(adjoint, in, out, nrep); break;
-        case 8: launch<8>(adjoint, in, out, nrep); break;
-        default: launch<16>(adjoint, in, out, nrep); break;
-    }
+    // the kernels with twice the register cap where a side's passes do not
+    // fit one LDS buffer with the default one
+    const int T = bct >= 8 ? 16 : 8;   // n3_T
+    bool mid = false;
+    for (int d = 0; d < 3; ++d) mid = mid || !fft_inplace_ok(fach_[d], T, fft_reg_cap(T, 0));
+    auto go = [&](auto m) {
+        constexpr int MID = decltype(m)::value;
+        switch (bct) {
+            case 1: launch<1, MID>(adjoint, in, out, nrep); break;
+            case 2: launch<2, MID>(adjoint, in, out, nrep); break;
+            case 4: launch<4, MID>(adjoint, in, out, nrep); break;
+            case 8: launch<8, MID>(adjoint, in, out, nrep); break;
+            default: launch<16, MID>(adjoint, in, out, nrep); break;
+        }
+    };
+    if (mid)
+        go(std::integral_constant<int, 1>());
+    else
+        go(std::integral_constant<int, 0>());
 }
 
 }  // namespace mm

@@ -148,6 +148,34 @@ def highorder_problem(seed=2, N=220, K=24000, C=32, L=16, fov=0.2, z=0.02, dwell
             "phantom": ph, "rng": rng}
 
 
+def nufft3_problem(seed=6, N=(128, 128, 64), C=8, per_partition=4096, n_turns=16):
+    """Inputs of the 3-D stack-of-spirals SENSE problem: N3 partitions (kz on
+    the Cartesian grid, cycles/pixel), each a 2-D variable-density spiral of
+    `per_partition` samples (the spiral above, `n_turns` turns: about four
+    times undersampled in-plane at 128 x 128) rotated by the golden angle from
+    one partition to the next; C coils (the 2-D Gaussian-ring maps times a
+    Gaussian z profile centred alternately in the two halves of the volume);
+    a phantom of ellipses extruded over the middle 80 % of the partitions."""
+    rng = np.random.default_rng(seed)
+    N1, N2, N3 = N
+    k2 = spiral(per_partition, n_turns, wrap=False)
+    ga = np.pi * (3.0 - np.sqrt(5.0))
+    kl = []
+    for p in range(N3):
+        c, s_ = np.cos(ga * p), np.sin(ga * p)
+        kx = c * k2[:, 0] - s_ * k2[:, 1]
+        ky = s_ * k2[:, 0] + c * k2[:, 1]
+        kl.append(np.stack([kx, ky, np.full(per_partition, (p - N3 // 2) / N3)], axis=1))
+    kloc = np.concatenate(kl)
+    kloc = np.where(kloc >= 0.5, kloc - 1.0, kloc)
+    z = (np.arange(N3) - N3 // 2) / N3
+    maps2 = gaussian_coils(N1, N2, C, rng)
+    prof = np.stack([np.exp(-(z - (0.25 if c % 2 else -0.25)) ** 2 / (2 * 0.4 ** 2)) for c in range(C)], axis=-1)
+    maps = maps2[:, :, None, :] * prof[None, None, :, :]
+    ph = ellipse_phantom(N1, N2, rng)[:, :, None] * (np.abs(z) < 0.4).astype(float)[None, None, :]
+    return {"N": tuple(N), "C": C, "M": kloc.shape[0], "kloc": kloc, "maps": maps, "phantom": ph}
+
+
 def delay_demo_problem(record, N=160, C=16, seed=5, delay=2.0):
     """Inputs of the trajectory-delay demo (demos/demo_autoDel.m:12-59) on
     synthetic images: a field-probe record (dict with phs_spha [K_in x 16],
