@@ -37,12 +37,18 @@
 
 namespace mm {
 
-// transforms per work-group of the FFT passes: a chunk's BCT transforms of
-// T / BCT pencils (16 transforms for chunks of 8 or 16: each grid position
-// a whole 128-byte line, one work-group per line; else 8)
+// transforms per work-group of the g2 / g3 passes: a chunk's BCT
+// transforms of T / BCT pencils (16 transforms for chunks of 8 or 16: each
+// grid position a whole 128-byte line, one work-group per line; else 8)
 template <int BCT>
 __host__ __device__ constexpr int n3_T() {
     return BCT >= 8 ? 16 : 8;
+}
+// ... and of the row passes (16 only for 16-transform chunks: two rows of 8
+// transforms per work-group measured slower than one, 64 against 55 us)
+template <int BCT>
+__host__ __device__ constexpr int n3_Tr() {
+    return BCT >= 16 ? 16 : 8;
 }
 
 __device__ __forceinline__ int wrap_pos(int i, int s, int n) {
@@ -61,28 +67,27 @@ template <int T, int MID>
 __host__ __device__ constexpr int n3_cap() {
     return MID ? 2 * fft_reg_cap(T, 0) : fft_reg_cap(T, 0);
 }
-template <int BCT, int MID>
+template <int T, int MID>
 __host__ __device__ constexpr int n3_waves() {
-    return MID ? (BCT >= 16 ? 3 : 4) : 1;
+    return MID ? (T >= 16 ? 3 : 4) : 1;
 }
 
 // LDS of a pass over sides of length n: one (single-buffer passes) or two
 // buffers of n*T values and the n - 1 pass twiddles
-template <int BCT>
-static size_t n3_lds(int n, bool ip) {
-    return (size_t)((ip ? 1 : 2) * n * n3_T<BCT>() + n) * sizeof(float2);
+static size_t n3_lds(int n, int T, bool ip) {
+    return (size_t)((ip ? 1 : 2) * n * T + n) * sizeof(float2);
 }
 
 // Rows, forward: lines L = i2 + N2 i3 (i2 < N2, i3 < N3), T / BCT of them per
 // work-group.  Image reads run along i1 (coalesced), grid writes along
 // (g1, b).
 template <int BCT, int MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BCT, MID>(), 8))) void n3_rows_fwd(Nu3Geom g, const FftFactors* __restrict__ fac,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<n3_Tr<BCT>(), MID>(), 8))) void n3_rows_fwd(Nu3Geom g, const FftFactors* __restrict__ fac,
                                                    const float2* __restrict__ twp,
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ x, float2* __restrict__ ws,
                                                    int ch, int lanes) {
-    constexpr int T = n3_T<BCT>(), LPG = T / BCT;
+    constexpr int T = n3_Tr<BCT>(), LPG = T / BCT;
     extern __shared__ float2 lds[];
     const FftFactors& F = *fac;
     const int n = g.G1;
@@ -125,7 +130,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BC
 // read from (q - s) mod G.  A work-group reads its pencils whole (into LDS)
 // before writing any of them.
 template <int BCT, int ADJ, int MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BCT, MID>(), 8))) void n3_axis(Nu3Geom g, int ax, const FftFactors* __restrict__ fac,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<n3_T<BCT>(), MID>(), 8))) void n3_axis(Nu3Geom g, int ax, const FftFactors* __restrict__ fac,
                                                const float2* __restrict__ twp, float2* __restrict__ ws) {
     constexpr int T = n3_T<BCT>(), LPG = T / BCT, SGN = ADJ ? -1 : 1;
     extern __shared__ float2 lds[];
@@ -181,12 +186,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BC
 // Rows, adjoint: inverse FFT along g1 of rows i2 < N2 of planes i3 < N3,
 // crop (pixel i1 from (i1 - s1) mod G1) and deapodisation.
 template <int BCT, int MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<BCT, MID>(), 8))) void n3_rows_adj(Nu3Geom g, const FftFactors* __restrict__ fac,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<n3_Tr<BCT>(), MID>(), 8))) void n3_rows_adj(Nu3Geom g, const FftFactors* __restrict__ fac,
                                                    const float2* __restrict__ twp,
                                                    const float* __restrict__ deap,
                                                    const float2* __restrict__ ws, float2* __restrict__ x,
                                                    int ch, int lanes) {
-    constexpr int T = n3_T<BCT>(), LPG = T / BCT;
+    constexpr int T = n3_Tr<BCT>(), LPG = T / BCT;
     extern __shared__ float2 lds[];
     const FftFactors& F = *fac;
     const int n = g.G1;
@@ -570,13 +575,14 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
         (void)hipGetLastError();
         attr_set = true;
     }
-    constexpr int T = n3_T<BCT>(), LPG = T / BCT, SPB = 256 / BCT;
+    constexpr int T = n3_T<BCT>(), TR = n3_Tr<BCT>(), LPG = T / BCT, LPR = TR / BCT, SPB = 256 / BCT;
     Nu3Geom g = g_;
     g.ip = 0;
-    for (int d = 0; d < 3; ++d)
+    if (fft_inplace_ok(fach_[0], TR, n3_cap<TR, MID>())) g.ip |= 1;
+    for (int d = 1; d < 3; ++d)
         if (fft_inplace_ok(fach_[d], T, n3_cap<T, MID>())) g.ip |= 1 << d;
-    const size_t lds1 = n3_lds<BCT>(g.G1, g.ip & 1), lds2 = n3_lds<BCT>(g.G2, (g.ip >> 1) & 1),
-                 lds3 = n3_lds<BCT>(g.G3, (g.ip >> 2) & 1);
+    const size_t lds1 = n3_lds(g.G1, TR, g.ip & 1), lds2 = n3_lds(g.G2, T, (g.ip >> 1) & 1),
+                 lds3 = n3_lds(g.G3, T, (g.ip >> 2) & 1);
     // after each launch: the launch check and, while profiling, an event
     auto done = [&](const char* stage) {
         LAUNCHCHECK();
@@ -590,7 +596,7 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
     g.chunk = ncell * BCT;
     ws_.ensure((size_t)g.chunk);
     if (adjoint && !toep_) yT_.ensure((size_t)g.M * BCT);
-    const unsigned rows_blocks = (unsigned)((g.N2 * g.N3 + LPG - 1) / LPG);
+    const unsigned rows_blocks = (unsigned)((g.N2 * g.N3 + LPR - 1) / LPR);
     const unsigned gx = (unsigned)((g.G1 + LPG - 1) / LPG);
     const unsigned samp_blocks = (unsigned)((g.M + SPB - 1) / SPB);
     const unsigned elem_blocks = (unsigned)((g.chunk + 255) / 256);
@@ -803,9 +809,9 @@ void Nufft3Op::apply(int adjoint, const float2* in, float2* out, int64_t nrep) {
     while (bct < nrep && bct < 16) bct <<= 1;
     // the kernels with twice the register cap where a side's passes do not
     // fit one LDS buffer with the default one
-    const int T = bct >= 8 ? 16 : 8;   // n3_T
-    bool mid = false;
-    for (int d = 0; d < 3; ++d) mid = mid || !fft_inplace_ok(fach_[d], T, fft_reg_cap(T, 0));
+    const int T = bct >= 8 ? 16 : 8, TR = bct >= 16 ? 16 : 8;   // n3_T, n3_Tr
+    bool mid = !fft_inplace_ok(fach_[0], TR, fft_reg_cap(TR, 0));
+    for (int d = 1; d < 3; ++d) mid = mid || !fft_inplace_ok(fach_[d], T, fft_reg_cap(T, 0));
     auto go = [&](auto m) {
         constexpr int MID = decltype(m)::value;
         switch (bct) {
