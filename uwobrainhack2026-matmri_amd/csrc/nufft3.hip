@@ -260,23 +260,28 @@ __global__ __launch_bounds__(256) void n3_toep_mul(Nu3Geom g, const float2* __re
 }
 
 // Degridding: lane = (sample, transform), transform fastest, so the BCT
-// lanes of a sample read each tap's BCT consecutive values; W^3 taps in
-// (s3, s2, s1) order with weight w3 w2 w1, then dcf.  The block's results
-// go through LDS and are written sample-fastest.
-template <int BCT>
+// lanes of a sample read each tap's BCT consecutive values (four transforms
+// per lane with 16-byte loads measured slower, 173 against 131 us: fewer
+// waves for loads that do not depend on each other); W^3 taps in (s3, s2,
+// s1) order with weight w3 w2 w1, then dcf (WT = W at compile time, 0:
+// runtime W).  The block's results go through LDS and are written
+// sample-fastest.
+template <int BCT, int WT>
 __global__ __launch_bounds__(256) void n3_degrid(Nu3Geom g, const uint32_t* __restrict__ base,
                                                  const float* __restrict__ wt, const float* __restrict__ dcf,
                                                  const float2* __restrict__ ws, float2* __restrict__ y, int ch,
                                                  int lanes) {
-    constexpr int SPB = 256 / BCT;
-    __shared__ float2 st[256];
-    const int b = threadIdx.x % BCT, ml = threadIdx.x / BCT;
+    constexpr int TPL = 1, LPS = BCT / TPL, SPB = 256 / LPS;
+    __shared__ float2 st[SPB * BCT];
+    const int b0 = (int)(threadIdx.x % LPS) * TPL, ml = threadIdx.x / LPS;
     const int m = (int)blockIdx.x * SPB + ml;
-    float2 acc = make_float2(0.f, 0.f);
+    float2 acc[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) acc[t] = make_float2(0.f, 0.f);
     if (m < g.M) {
         const uint32_t bs = base[m];
         const int g10 = (int)(bs & 1023u), g20 = (int)((bs >> 10) & 1023u), g30 = (int)(bs >> 20);
-        const int W = g.W;
+        const int W = WT > 0 ? WT : g.W;
         const float* w = wt + (int64_t)m * 3 * W;
         int c3 = g30;
         for (int s3 = 0; s3 < W; ++s3) {
@@ -284,13 +289,26 @@ __global__ __launch_bounds__(256) void n3_degrid(Nu3Geom g, const uint32_t* __re
             int c2 = g20;
             for (int s2 = 0; s2 < W; ++s2) {
                 const float w23 = w3 * w[W + s2];
-                const float2* row = ws + (((int64_t)c3 * g.G2 + c2) * g.G1) * BCT + b;
+                const float2* row = ws + (((int64_t)c3 * g.G2 + c2) * g.G1) * BCT + b0;
                 int c1 = g10;
-                for (int s1 = 0; s1 < W; ++s1) {
+#pragma unroll
+                for (int s1 = 0; s1 < (WT > 0 ? WT : 8); ++s1) {
+                    if (WT == 0 && s1 >= W) break;
                     const float wk = w23 * w[s1];
-                    const float2 z = row[(int64_t)c1 * BCT];
-                    acc.x += wk * z.x;
-                    acc.y += wk * z.y;
+                    const float2* z = row + (int64_t)c1 * BCT;
+                    if constexpr (TPL >= 2) {
+#pragma unroll
+                        for (int t = 0; t < TPL; t += 2) {
+                            const float4 q = *reinterpret_cast<const float4*>(z + t);
+                            acc[t].x += wk * q.x;
+                            acc[t].y += wk * q.y;
+                            acc[t + 1].x += wk * q.z;
+                            acc[t + 1].y += wk * q.w;
+                        }
+                    } else {
+                        acc[0].x += wk * z[0].x;
+                        acc[0].y += wk * z[0].y;
+                    }
                     if (++c1 == g.G1) c1 = 0;
                 }
                 if (++c2 == g.G2) c2 = 0;
@@ -298,13 +316,19 @@ __global__ __launch_bounds__(256) void n3_degrid(Nu3Geom g, const uint32_t* __re
             if (++c3 == g.G3) c3 = 0;
         }
         const float d = dcf[m];
-        acc.x *= d;
-        acc.y *= d;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+            acc[t].x *= d;
+            acc[t].y *= d;
+        }
     }
-    st[b * SPB + ml] = acc;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) st[(b0 + t) * SPB + ml] = acc[t];
     __syncthreads();
-    const int b2 = threadIdx.x / SPB, m2 = (int)blockIdx.x * SPB + (int)threadIdx.x % SPB;
-    if (m2 < g.M && b2 < lanes) y[(int64_t)(ch * BCT + b2) * g.M + m2] = st[threadIdx.x];
+    for (int i = threadIdx.x; i < SPB * BCT; i += 256) {
+        const int b2 = i / SPB, m2 = (int)blockIdx.x * SPB + i % SPB;
+        if (m2 < g.M && b2 < lanes) y[(int64_t)(ch * BCT + b2) * g.M + m2] = st[i];
+    }
 }
 
 // y of one chunk into the transposed layout yT[m][b] (read sample-fastest,
@@ -327,48 +351,78 @@ __global__ __launch_bounds__(256) void n3_transpose_y(Nu3Geom g, const float2* _
 // Rows longer than this go to n3_grid_long (one wave per cell).
 constexpr int kN3LongRow = 64;
 
-// Gridding as an output-stationary gather: lane = (cell, transform) sums
-// its cell's CSR row in sample order; every cell written once (here, or by
-// n3_grid_long when its row exceeds kN3LongRow entries).  The row is
-// walked eight entries at a time: their entry loads, then their sample
-// loads, are issued together, so a row waits on two load latencies per eight
-// entries instead of two per entry (same sums, same order; measured faster
-// than predicated batches for the short rows, 325 against 337 us, and than
-// four cells per lane, 448 us, on a 192 x 192 x 96 grid).
+// Gridding as an output-stationary gather: each (cell, transform) sum runs
+// over its cell's CSR row in sample order; every cell written once (here, or
+// by n3_grid_long when its row exceeds kN3LongRow entries).  A lane carries
+// TPL = min(BCT, 4) transforms of one cell (16-byte loads and stores), so a
+// wave serves 64 TPL / BCT cells: the kernel waits on the rows' load chains
+// (entry, then sample) once per 16-32 cells instead of once per 8 (806 ->
+// 272 us -> this on a 192^2 x 96 grid).  The row is walked U entries at a
+// time, their entry loads then their sample loads issued together.
 template <int BCT>
 __global__ __launch_bounds__(256) void n3_grid_csr(const int* __restrict__ rp, const int2* __restrict__ ent,
                                                    const float2* __restrict__ yT, float2* __restrict__ ws,
                                                    int64_t ncell) {
-    constexpr int U = 8;
-    const int64_t cell = (int64_t)blockIdx.x * (256 / BCT) + threadIdx.x / BCT;
-    const int b = threadIdx.x % BCT;
+    constexpr int TPL = BCT < 4 ? BCT : 4, LPC = BCT / TPL, CPB = 256 / LPC, U = 4;
+    const int64_t cell = (int64_t)blockIdx.x * CPB + threadIdx.x / LPC;
+    const int b0 = (int)(threadIdx.x % LPC) * TPL;
     if (cell >= ncell) return;
     const int e0 = rp[cell], e1 = rp[cell + 1];
     if (e1 - e0 > kN3LongRow) return;
-    float2 acc = make_float2(0.f, 0.f);
+    float2 acc[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) acc[t] = make_float2(0.f, 0.f);
+    // the TPL values of entry en for this lane
+    auto load = [&](const int2 en, float2* v) {
+        const float2* src = yT + (int64_t)en.x * BCT + b0;
+        if constexpr (TPL >= 2) {
+#pragma unroll
+            for (int t = 0; t < TPL; t += 2) {
+                const float4 q = *reinterpret_cast<const float4*>(src + t);
+                v[t] = make_float2(q.x, q.y);
+                v[t + 1] = make_float2(q.z, q.w);
+            }
+        } else {
+            v[0] = src[0];
+        }
+    };
     int e = e0;
     for (; e + U <= e1; e += U) {
         int2 en[U];
-        float2 v[U];
+        float2 v[U][TPL];
 #pragma unroll
         for (int u = 0; u < U; ++u) en[u] = ent[e + u];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = yT[(int64_t)en[u].x * BCT + b];
+        for (int u = 0; u < U; ++u) load(en[u], v[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const float w = __int_as_float(en[u].y);
-            acc.x += w * v[u].x;
-            acc.y += w * v[u].y;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+                acc[t].x += w * v[u][t].x;
+                acc[t].y += w * v[u][t].y;
+            }
         }
     }
     for (; e < e1; ++e) {
         const int2 en = ent[e];
         const float w = __int_as_float(en.y);
-        const float2 v = yT[(int64_t)en.x * BCT + b];
-        acc.x += w * v.x;
-        acc.y += w * v.y;
+        float2 v[TPL];
+        load(en, v);
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+            acc[t].x += w * v[t].x;
+            acc[t].y += w * v[t].y;
+        }
     }
-    ws[cell * BCT + b] = acc;
+    float2* dst = ws + cell * BCT + b0;
+    if constexpr (TPL >= 2) {
+#pragma unroll
+        for (int t = 0; t < TPL; t += 2)
+            *reinterpret_cast<float4*>(dst + t) = make_float4(acc[t].x, acc[t].y, acc[t + 1].x, acc[t + 1].y);
+    } else {
+        dst[0] = acc[0];
+    }
 }
 
 // Gridding of the long rows (the k-space centre): one wave per listed cell,
@@ -600,7 +654,8 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
     const unsigned gx = (unsigned)((g.G1 + LPG - 1) / LPG);
     const unsigned samp_blocks = (unsigned)((g.M + SPB - 1) / SPB);
     const unsigned elem_blocks = (unsigned)((g.chunk + 255) / 256);
-    const unsigned cell_blocks = (unsigned)((ncell * BCT + 255) / 256);
+    // n3_grid_csr: BCT / min(BCT, 4) lanes per cell
+    const unsigned cell_blocks = (unsigned)((ncell * (BCT / (BCT < 4 ? BCT : 4)) + 255) / 256);
     const int64_t nchunk = (nrep + BCT - 1) / BCT;
     for (int64_t ch = 0; ch < nchunk; ++ch) {
         const int lanes = (int)std::min<int64_t>(BCT, nrep - ch * BCT);
@@ -643,8 +698,16 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
                 hipLaunchKernelGGL((n3_odd_ramp<BCT, 0>), dim3(elem_blocks), dim3(256), 0, st, g, ws_.p);
                 done("n3_odd_ramp");
             }
-            hipLaunchKernelGGL(n3_degrid<BCT>, dim3(samp_blocks), dim3(256), 0, st, g, base_.p, wt_.p, dcf_.p,
-                               (const float2*)ws_.p, out, (int)ch, lanes);
+            {
+                // n3_degrid: BCT lanes per sample
+                const unsigned deg_blocks = (unsigned)((g.M * (int64_t)BCT + 255) / 256);
+                if (g.W == 4)
+                    hipLaunchKernelGGL((n3_degrid<BCT, 4>), dim3(deg_blocks), dim3(256), 0, st, g, base_.p, wt_.p,
+                                       dcf_.p, (const float2*)ws_.p, out, (int)ch, lanes);
+                else
+                    hipLaunchKernelGGL((n3_degrid<BCT, 0>), dim3(deg_blocks), dim3(256), 0, st, g, base_.p, wt_.p,
+                                       dcf_.p, (const float2*)ws_.p, out, (int)ch, lanes);
+            }
             done("n3_degrid");
         } else {
             hipLaunchKernelGGL(n3_transpose_y<BCT>, dim3(samp_blocks), dim3(256), 0, st, g, in, yT_.p, (int)ch,
