@@ -2489,6 +2489,8 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)nu_degrid_tile<BCT, 4>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)nu_degrid_tile<BCT, 6>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)nu_degrid_tile<BCT, 0>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipFuncSetAttribute((const void*)nu_grid_tile<BCT, 1>,
@@ -2499,6 +2501,9 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         attr_set = true;
     }
     const bool wt4 = ndim_ == 2 && W_ == 4;
+    // the staged degridding also has W = 6 at compile time (the interpolated
+    // high-order operator's NUFFT)
+    const bool wt6 = ndim_ == 2 && W_ == 6;
     const NuSliceDev* sl = c.sl.p;
     // (first item, items) of list L in slice group g; g < 0: the whole list
     auto span = [&](int L, int g, int n_all) -> std::pair<int, int> {
@@ -2537,6 +2542,10 @@ void NufftOp::launch(Cfg& c, int adjoint, const NuRun& run, const float2* in, fl
         if (ndim_ == 2 && degrid_stage_ && db.second > 0) {
             if (wt4)
                 hipLaunchKernelGGL((nu_degrid_tile<BCT, 4>), dim3(db.second), dim3(256), c.lds_db, st, sl, c.w_db.p + db.first,
+                                   dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y,
+                                   degrid_block_, r);
+            else if (wt6)
+                hipLaunchKernelGGL((nu_degrid_tile<BCT, 6>), dim3(db.second), dim3(256), c.lds_db, st, sl, c.w_db.p + db.first,
                                    dblk_.p, dsamp_.p, dbase_.p, dwt_.p, ddcf_.p, (const float2*)c.ws.p, y,
                                    degrid_block_, r);
             else
