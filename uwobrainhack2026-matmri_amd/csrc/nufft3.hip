@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void n3_transpose_y(Nu3Geom g, const float2* _
     if (m2 < g.M) yT[(int64_t)m2 * BCT + b2] = st[b2 * SPB + ml2];
 }
 
-// Rows longer than this go to n3_grid_long (one wave per cell).
+// Rows longer than this go to n3_grid_long (one work-group per cell).
 constexpr int kN3LongRow = 64;
 
 // Gridding as an output-stationary gather: each (cell, transform) sum runs
@@ -425,21 +425,22 @@ __global__ __launch_bounds__(256) void n3_grid_csr(const int* __restrict__ rp, c
     }
 }
 
-// Gridding of the long rows (the k-space centre): one wave per listed cell,
-// lane = (phase j, transform b) with P = 64 / BCT phases; lane j sums entries
-// e0 + j, e0 + j + P, ... in order (four entries' loads issued together),
-// then the phases are added by a fixed shuffle tree, so every cell's sum is
-// the same from run to run.
+// Gridding of the long rows (the k-space centre): one work-group per listed
+// cell, lane = (phase j, transform b) with P = 256 / BCT phases; lane j sums
+// entries e0 + j, e0 + j + P, ... in order (four entries' loads issued
+// together), the phases of each wave are added by a fixed shuffle tree and
+// the four waves' sums in wave order, so every cell's sum is the same from
+// run to run (44 us on the nufft3 workload's ~1000 cells; one wave per cell
+// took 55).
 template <int BCT>
 __global__ __launch_bounds__(256) void n3_grid_long(const int* __restrict__ cells, int nlong,
                                                     const int* __restrict__ rp, const int2* __restrict__ ent,
                                                     const float2* __restrict__ yT, float2* __restrict__ ws) {
-    constexpr int P = 64 / BCT, U = 4;
-    const int wave = (int)((blockIdx.x * 256 + threadIdx.x) >> 6);
-    if (wave >= nlong) return;   // uniform over the wave
-    const int lane = threadIdx.x & 63;
-    const int j = lane / BCT, b = lane % BCT;
-    const int64_t cell = cells[wave];
+    constexpr int P = 256 / BCT, U = 4;
+    __shared__ float2 red[4][BCT];
+    if ((int)blockIdx.x >= nlong) return;   // uniform over the block
+    const int j = threadIdx.x / BCT, b = threadIdx.x % BCT;
+    const int64_t cell = cells[blockIdx.x];
     const int e0 = rp[cell], e1 = rp[cell + 1];
     float2 acc = make_float2(0.f, 0.f);
     int e = e0 + j;
@@ -469,7 +470,18 @@ __global__ __launch_bounds__(256) void n3_grid_long(const int* __restrict__ cell
         acc.x += __shfl_xor(acc.x, off);
         acc.y += __shfl_xor(acc.y, off);
     }
-    if (j == 0) ws[cell * BCT + b] = acc;
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) < BCT) red[wv][b] = acc;
+    __syncthreads();
+    if ((int)threadIdx.x < BCT) {
+        float2 s = red[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            s.x += red[w][threadIdx.x].x;
+            s.y += red[w][threadIdx.x].y;
+        }
+        ws[cell * BCT + threadIdx.x] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -717,8 +729,8 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
                                (const float2*)yT_.p, ws_.p, ncell);
             done("n3_grid_csr");
             if (nlong_ > 0) {
-                hipLaunchKernelGGL(n3_grid_long<BCT>, dim3((unsigned)((nlong_ + 3) / 4)), dim3(256), 0, st, long_.p,
-                                   nlong_, rp_.p, ent_.p, (const float2*)yT_.p, ws_.p);
+                hipLaunchKernelGGL(n3_grid_long<BCT>, dim3((unsigned)nlong_), dim3(256), 0, st, long_.p, nlong_,
+                                   rp_.p, ent_.p, (const float2*)yT_.p, ws_.p);
                 done("n3_grid_csr");   // the gridding stage: both launches
             }
             if (g.odd) {
