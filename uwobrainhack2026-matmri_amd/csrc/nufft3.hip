@@ -129,10 +129,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<n3
 // (q - s) mod G, all G outputs.  Adjoint: all G inputs, outputs q < N_ax
 // read from (q - s) mod G.  A work-group reads its pencils whole (into LDS)
 // before writing any of them.
-template <int BCT, int ADJ, int MID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<n3_T<BCT>(), MID>(), 8))) void n3_axis(Nu3Geom g, int ax, const FftFactors* __restrict__ fac,
+// TX = 2: twice the pencils per work-group (the g3 pass where its side fits
+// one LDS buffer at 32 transforms: 256 bytes per grid position, the
+// positions being a whole g1-g2 plane apart)
+template <int BCT, int ADJ, int MID, int TX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<n3_T<BCT>() * TX, MID>(), 8))) void n3_axis(Nu3Geom g, int ax, const FftFactors* __restrict__ fac,
                                                const float2* __restrict__ twp, float2* __restrict__ ws) {
-    constexpr int T = n3_T<BCT>(), LPG = T / BCT, SGN = ADJ ? -1 : 1;
+    constexpr int T = n3_T<BCT>() * TX, LPG = T / BCT, SGN = ADJ ? -1 : 1;
     extern __shared__ float2 lds[];
     const FftFactors& F = *fac;
     const int n = ax == 2 ? g.G2 : g.G3;
@@ -634,9 +637,13 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
                                   160 * 1024);
         (void)hipFuncSetAttribute((const void*)n3_rows_adj<BCT, MID>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 0, MID>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 0, MID, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 1, MID>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 1, MID, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 0, MID, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)n3_axis<BCT, 1, MID, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         (void)hipGetLastError();
         attr_set = true;
@@ -647,8 +654,12 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
     if (fft_inplace_ok(fach_[0], TR, n3_cap<TR, MID>())) g.ip |= 1;
     for (int d = 1; d < 3; ++d)
         if (fft_inplace_ok(fach_[d], T, n3_cap<T, MID>())) g.ip |= 1 << d;
+    // the g3 pass with twice the pencils where its side fits one buffer so
+    constexpr int T3W = 2 * T;
+    const bool wide3 = fft_inplace_ok(fach_[2], T3W, n3_cap<T3W, MID>());
+    if (wide3) g.ip |= 4;
     const size_t lds1 = n3_lds(g.G1, TR, g.ip & 1), lds2 = n3_lds(g.G2, T, (g.ip >> 1) & 1),
-                 lds3 = n3_lds(g.G3, T, (g.ip >> 2) & 1);
+                 lds3 = wide3 ? n3_lds(g.G3, T3W, true) : n3_lds(g.G3, T, (g.ip >> 2) & 1);
     // after each launch: the launch check and, while profiling, an event
     auto done = [&](const char* stage) {
         LAUNCHCHECK();
@@ -664,6 +675,24 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
     if (adjoint && !toep_) yT_.ensure((size_t)g.M * BCT);
     const unsigned rows_blocks = (unsigned)((g.N2 * g.N3 + LPR - 1) / LPR);
     const unsigned gx = (unsigned)((g.G1 + LPG - 1) / LPG);
+    const unsigned gx3 = (unsigned)((g.G1 + 2 * LPG - 1) / (2 * LPG));
+    auto axis3 = [&](int adj) {
+        if (wide3) {
+            if (adj)
+                hipLaunchKernelGGL((n3_axis<BCT, 1, MID, 2>), dim3(gx3 * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
+                                   facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            else
+                hipLaunchKernelGGL((n3_axis<BCT, 0, MID, 2>), dim3(gx3 * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
+                                   facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+        } else {
+            if (adj)
+                hipLaunchKernelGGL((n3_axis<BCT, 1, MID, 1>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
+                                   facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            else
+                hipLaunchKernelGGL((n3_axis<BCT, 0, MID, 1>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
+                                   facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+        }
+    };
     const unsigned samp_blocks = (unsigned)((g.M + SPB - 1) / SPB);
     const unsigned elem_blocks = (unsigned)((g.chunk + 255) / 256);
     // n3_grid_csr: BCT / min(BCT, 4) lanes per cell
@@ -676,19 +705,17 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
             hipLaunchKernelGGL((n3_rows_fwd<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
                                twid_.p + tw_off_[0], toep_deap_.p, in, ws_.p, (int)ch, lanes);
             done("n3_rows_fwd");
-            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
+            hipLaunchKernelGGL((n3_axis<BCT, 0, MID, 1>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
             done("n3_axis2_fwd");
-            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
-                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            axis3(0);
             done("n3_axis3_fwd");
             hipLaunchKernelGGL(n3_toep_mul<BCT>, dim3(elem_blocks), dim3(256), 0, st, g, (const float2*)toep_k_.p,
                                ws_.p);
             done("n3_toep_mul");
-            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
-                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            axis3(1);
             done("n3_axis3_adj");
-            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
+            hipLaunchKernelGGL((n3_axis<BCT, 1, MID, 1>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
             done("n3_axis2_adj");
             hipLaunchKernelGGL((n3_rows_adj<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
@@ -700,11 +727,10 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
             hipLaunchKernelGGL((n3_rows_fwd<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
                                twid_.p + tw_off_[0], deap_.p, in, ws_.p, (int)ch, lanes);
             done("n3_rows_fwd");
-            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
+            hipLaunchKernelGGL((n3_axis<BCT, 0, MID, 1>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
             done("n3_axis2_fwd");
-            hipLaunchKernelGGL((n3_axis<BCT, 0, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
-                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            axis3(0);
             done("n3_axis3_fwd");
             if (g.odd) {
                 hipLaunchKernelGGL((n3_odd_ramp<BCT, 0>), dim3(elem_blocks), dim3(256), 0, st, g, ws_.p);
@@ -737,10 +763,9 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
                 hipLaunchKernelGGL((n3_odd_ramp<BCT, 1>), dim3(elem_blocks), dim3(256), 0, st, g, ws_.p);
                 done("n3_odd_ramp");
             }
-            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.G2), dim3(256), lds3, st, g, 3,
-                               facs_.p + 2, twid_.p + tw_off_[2], ws_.p);
+            axis3(1);
             done("n3_axis3_adj");
-            hipLaunchKernelGGL((n3_axis<BCT, 1, MID>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
+            hipLaunchKernelGGL((n3_axis<BCT, 1, MID, 1>), dim3(gx * (unsigned)g.N3), dim3(256), lds2, st, g, 2,
                                facs_.p + 1, twid_.p + tw_off_[1], ws_.p);
             done("n3_axis2_adj");
             hipLaunchKernelGGL((n3_rows_adj<BCT, MID>), dim3(rows_blocks), dim3(256), lds1, st, g, facs_.p,
