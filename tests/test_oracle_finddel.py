@@ -117,3 +117,23 @@ def test_load_traj_mat_round_trip(tmp_path):
     t = fd.load_traj_mat(str(p))
     assert np.array_equal(t["phs_spha"], d["phs_spha"][:100]) and np.array_equal(t["phs_conc"], d["phs_conc"][:100])
     assert t["datatime"].shape == (40,) and t["tdwelltraj"] == 1.0
+
+
+def test_delay_demo_problem_conventions():
+    """workloads.delay_demo_problem (the setting of demos/demo_autoDel.m:12-59
+    on synthetic images): the record's data times shifted to the spiral
+    case's 50 us start, a 1.35 mm grid, B0 within +-50 Hz in rad/us, and the
+    record's coefficients unchanged."""
+    import importlib
+    w = importlib.import_module("uwobrainhack2026-matmri_amd.workloads")
+    d = np.load(GOLD)
+    P = w.delay_demo_problem(d, N=32, C=4)
+    assert np.array_equal(P["datatime"], d["datatime"].reshape(-1) + 50.0)
+    assert P["tdwell"] == 1.0 and P["delay"] == 2.0
+    assert np.array_equal(P["spha"], d["phs_spha"]) and np.array_equal(P["conc"], d["phs_conc"])
+    gx = P["grid"]["x"]
+    assert gx.shape == (32, 32) and abs((gx[1, 0] - gx[0, 0]) - 0.00135) < 1e-15
+    assert np.max(np.abs(P["b0"])) <= 2 * np.pi * 50e-6 * (1 + 1e-12)
+    assert P["maps"].shape == (32, 32, 4) and P["phantom"].shape == (32, 32)
+    # the record covers every data time for delays within the search's reach
+    assert P["datatime"][-1] + 5.0 < (P["spha"].shape[0] - 1) * P["tdwell"]

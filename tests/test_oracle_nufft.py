@@ -334,3 +334,29 @@ def test_toeplitz_odd_sizes_vs_exact_nudft_normal(imN):
     want = want * (-1.0) ** sum(imN)
     got = ToepPlan(imN, kloc, dcf, 1.5, 6, tos=2, tkwidth=8).apply(x)
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-4
+
+
+def test_nufft3_problem_trajectory():
+    """workloads.nufft3_problem: 64 partitions of 4096 samples in
+    [-0.5, 0.5) cycles/pixel, kz on the partition grid (p - 32) / 64, each
+    partition the same spiral turned by the golden angle (radius kept), the
+    coil profiles along z and the phantom zero outside the middle 80 %."""
+    import importlib
+    w = importlib.import_module("uwobrainhack2026-matmri_amd.workloads")
+    P = w.nufft3_problem()
+    kl = P["kloc"]
+    assert kl.shape == (64 * 4096, 3) and P["M"] == kl.shape[0]
+    assert kl.min() >= -0.5 and kl.max() < 0.5
+    kz = kl[:, 2].reshape(64, 4096)
+    assert np.allclose(kz, ((np.arange(64) - 32) / 64)[:, None])
+    r = np.hypot(kl[:, 0], kl[:, 1]).reshape(64, 4096)
+    k2 = w.spiral(4096, 16, wrap=False)
+    assert np.allclose(r, np.hypot(k2[:, 0], k2[:, 1])[None, :], atol=1e-12)
+    ga = np.pi * (3 - np.sqrt(5))
+    a0 = np.angle(k2[100, 0] + 1j * k2[100, 1])
+    a5 = np.angle(kl[5 * 4096 + 100, 0] + 1j * kl[5 * 4096 + 100, 1])
+    assert abs(np.angle(np.exp(1j * (a5 - a0 - 5 * ga)))) < 1e-9
+    z = (np.arange(64) - 32) / 64
+    assert np.all(P["phantom"][:, :, np.abs(z) >= 0.4] == 0)
+    assert P["maps"].shape == (128, 128, 64, 8)
+
