@@ -505,14 +505,24 @@ static double time_launches(hipStream_t st, int iters, F&& launch) {
 void HioOp::run(int adjoint, const float2* in, float2* out, int64_t ncol64, const float2* maps, int C,
                 const float* W, int prof_iters, double* prof_ms) {
     if (rp_ > kHioMaxRank) fail(MM_ERR_UNSUPPORTED, "sampHighOrder: phase rank above 24");
-    hipStream_t st = ctx->stream;
     const int ncol = (int)ncol64;
     if (!phidiv_ && rp_ <= 16 && (!maps || C <= 32) && use_mfma_) {
         run_mfma(adjoint, in, out, ncol, maps, C, W, prof_iters, prof_ms);
         return;
     }
+    // the simple kernels with 16 columns per group when there are no more
+    // (a 16-coil SENSE operator: no multiply-adds on empty columns)
+    if (ncol <= 16)
+        run_simple<16>(adjoint, in, out, ncol, maps, C, W, prof_iters, prof_ms);
+    else
+        run_simple<32>(adjoint, in, out, ncol, maps, C, W, prof_iters, prof_ms);
+}
+
+template <int NC>
+void HioOp::run_simple(int adjoint, const float2* in, float2* out, int ncol, const float2* maps, int C,
+                       const float* W, int prof_iters, double* prof_ms) {
+    hipStream_t st = ctx->stream;
     const float scale = (float)(1.0 / std::sqrt((double)V_));
-    const int NC = 32;
     const int ncg = (ncol + NC - 1) / NC;
     if (!adjoint) {
         X_.ensure((size_t)V_ * ncol);

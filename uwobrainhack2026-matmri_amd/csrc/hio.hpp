@@ -44,6 +44,9 @@ class HioOp : public mm_op {
     }
 
    private:
+    template <int NC>
+    void run_simple(int adjoint, const float2* in, float2* out, int ncol, const float2* maps, int C,
+                    const float* W, int prof_iters, double* prof_ms);
     void run_mfma(int adjoint, const float2* in, float2* out, int ncol, const float2* maps, int C,
                   const float* W, int prof_iters, double* prof_ms);
     void run(int adjoint, const float2* in, float2* out, int64_t ncol, const float2* maps, int C,
