@@ -25,6 +25,11 @@ class HioOp : public mm_op {
         if (key == "hio_mfma") { use_mfma_ = value != 0; return true; }
         return false;
     }
+    bool get_option(const std::string& key, int* value) const override {
+        if (key == "hio_variant") { *value = variant_; return true; }
+        if (key == "hio_mfma") { *value = use_mfma_ ? 1 : 0; return true; }
+        return false;
+    }
     bool profile_sense(int adjoint, const float2* in, float2* out, int64_t nrep, const float2* maps,
                        int C, int iters, double* ms, std::string* name);
     int phase_terms() const { return L_; }
