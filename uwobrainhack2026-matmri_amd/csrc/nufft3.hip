@@ -353,6 +353,12 @@ __global__ __launch_bounds__(256) void n3_transpose_y(Nu3Geom g, const float2* _
 
 // Rows longer than this go to n3_grid_long (one work-group per cell).
 constexpr int kN3LongRow = 64;
+// transforms per lane of n3_grid_csr (eight measured slower than four: 219
+// against 163 us on a 192^2 x 96 grid at 8 transforms)
+template <int BCT>
+__host__ __device__ constexpr int n3_grid_tpl() {
+    return BCT < 4 ? BCT : 4;
+}
 
 // Gridding as an output-stationary gather: each (cell, transform) sum runs
 // over its cell's CSR row in sample order; every cell written once (here, or
@@ -366,7 +372,7 @@ template <int BCT>
 __global__ __launch_bounds__(256) void n3_grid_csr(const int* __restrict__ rp, const int2* __restrict__ ent,
                                                    const float2* __restrict__ yT, float2* __restrict__ ws,
                                                    int64_t ncell) {
-    constexpr int TPL = BCT < 4 ? BCT : 4, LPC = BCT / TPL, CPB = 256 / LPC, U = 4;
+    constexpr int TPL = n3_grid_tpl<BCT>(), LPC = BCT / TPL, CPB = 256 / LPC, U = 4;
     const int64_t cell = (int64_t)blockIdx.x * CPB + threadIdx.x / LPC;
     const int b0 = (int)(threadIdx.x % LPC) * TPL;
     if (cell >= ncell) return;
@@ -695,8 +701,8 @@ void Nufft3Op::launch(int adjoint, const float2* in, float2* out, int64_t nrep) 
     };
     const unsigned samp_blocks = (unsigned)((g.M + SPB - 1) / SPB);
     const unsigned elem_blocks = (unsigned)((g.chunk + 255) / 256);
-    // n3_grid_csr: BCT / min(BCT, 4) lanes per cell
-    const unsigned cell_blocks = (unsigned)((ncell * (BCT / (BCT < 4 ? BCT : 4)) + 255) / 256);
+    // n3_grid_csr: BCT / n3_grid_tpl lanes per cell
+    const unsigned cell_blocks = (unsigned)((ncell * (BCT / n3_grid_tpl<BCT>()) + 255) / 256);
     const int64_t nchunk = (nrep + BCT - 1) / BCT;
     for (int64_t ch = 0; ch < nchunk; ++ch) {
         const int lanes = (int)std::min<int64_t>(BCT, nrep - ch * BCT);
