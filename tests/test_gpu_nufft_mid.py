@@ -5,7 +5,8 @@ take the radix-16/12 factorization wherever it saves a pass (336 = 16 x 3 x
 7).  Every transform count the plans use (1, 2, 4, 8, 16 per chunk), odd
 sides, unequal grid sides, the sides where even those kernels need two LDS
 buffers (420, 500) and the Toeplitz form are held to the FP64 oracle at the
-per-application bound 1e-5, with the option on and off.
+per-application bound 1e-5, with the option on and off; grid sides above
+620 points (the large-grid kernels) too.
 """
 import numpy as np
 import pytest
@@ -140,3 +141,55 @@ def test_fft_mid_option_roundtrip(mm):
     assert S.get_option("nu_fft_mid") == 0
     b = np.asarray(S * x)
     assert np.array_equal(a, b)
+
+
+LARGE_CASES = [
+    # imN, M, kwidth, dcf, nrep (grid sides at os 1.5 in the comment)
+    ((512, 480), 20000, 4, False, 8),    # 768 x 720: large-grid column passes of 8 transforms
+    ((512, 512), 20000, 4, True, 16),    # 768 x 768: plain transforms in chunks of 8
+    ((451, 300), 12000, 6, False, 3),    # 686 x 450, odd side: two LDS buffers in the mid-size mode
+]
+
+
+@pytest.mark.parametrize("imN,M,W,use_dcf,nrep", LARGE_CASES)
+def test_nufft_large_grid_vs_oracle(mm, imN, M, W, use_dcf, nrep):
+    """Grid sides above what the small and mid-size FFT kernels can hold in
+    LDS at 16 transforms (about 620 points): the large-grid kernels (one
+    column of at most 8 transforms per work-group), plain transforms in
+    chunks of 8, against the oracle (1e-5)."""
+    print(f"[case] large nufft imN={imN} M={M} W={W} dcf={use_dcf} nrep={nrep}", flush=True)
+    rng = np.random.default_rng(600 + sum(imN) + nrep)
+    kloc = rng.random((M, 2)) - 0.5
+    dcf = rng.random(M) + 0.5 if use_dcf else None
+    So = NufftPlan(list(imN), kloc, dcf, 1.5, W)
+    Sg = mm.nufftOp(list(imN), kloc, dcf, 1, 1.5, W)
+    x = crand(rng, imN + ((nrep,) if nrep > 1 else ()))
+    y = crand(rng, (M, nrep) if nrep > 1 else (M,))
+    e_f = rel(Sg * x, So.forward(x))
+    e_a = rel(Sg.H * y, So.adjoint(y))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL
+
+
+def test_sense_large_grid_vs_oracle(mm):
+    """A 512 x 512 SENSE operator with 8 coils (the fused coil chunk on the
+    large-grid kernels) against the oracle in both directions and the normal
+    operator; with 12 coils (a 16-coil chunk the LDS cannot hold at a
+    768-point side) the operator refuses to apply rather than fault."""
+    print("[case] large sense 512x512 C=8", flush=True)
+    rng = np.random.default_rng(650)
+    imN, M, C = (512, 512), 20000, 8
+    kl = rng.random((M, 2)) - 0.5
+    maps = crand(rng, imN + (C,))
+    So, Ro = NufftPlan(list(imN), kl, None, 1.5, 4), RcvrPlan(maps)
+    E = mm.senseOp(mm.nufftOp(list(imN), kl, None, 1, 1.5, 4), mm.rcvrOp(maps))
+    x = crand(rng, imN)
+    y = o_mrSampFunc(crand(rng, imN), "notransp", So, Ro)
+    e_f = rel(E * x, o_mrSampFunc(x, "notransp", So, Ro))
+    e_a = rel(E.H * y, o_mrSampFunc(y, "transp", So, Ro))
+    e_n = rel(E.normal(x), o_mrSampFunc(o_mrSampFunc(x, "notransp", So, Ro), "transp", So, Ro))
+    print(f"  fwd {e_f:.2e} adj {e_a:.2e} normal {e_n:.2e}", flush=True)
+    assert e_f < TOL and e_a < TOL and e_n < TOL
+    E12 = mm.senseOp(mm.nufftOp(list(imN), kl, None, 1, 1.5, 4), mm.rcvrOp(crand(rng, imN + (12,))))
+    with pytest.raises(NotImplementedError):
+        E12 * x

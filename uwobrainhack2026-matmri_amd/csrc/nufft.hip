@@ -2026,9 +2026,96 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         c.BCT = std::min<int>(16, next_pow2(nrep));
         nchunk = (int)((nrep + c.BCT - 1) / c.BCT);
     }
-    const int BCT = c.BCT;
     const int ns = (int)sl_.size();
     c.hs = sl_;
+    // Row / column FFT passes of one slice in mode (bigm, fm) with bct
+    // transforms per chunk: the radix-16/12 factorization where it pays, the
+    // single-buffer flags and the LDS bytes (d's factor and twiddle indices
+    // are set accordingly).  Column passes of 16 transforms use the
+    // radix-16/12 factorization where it takes fewer passes -- in the small
+    // mode only where it is made of those radices alone (192 = 16 x 12: two
+    // passes instead of three; 0.93 -> 0.77 us per 8-coil slice), in the
+    // mid-size mode also with others (336 = 16 x 3 x 7 instead of 8 x 2 x 3 x
+    // 7), where row passes of 16 transforms take it too.
+    auto fft_plan = [&](NuSliceDev& d, bool bigm, int fm, int bct, size_t* lr, size_t* lc) {
+        const int TR = rows_T(bct, bigm), TC = cols_T(bct, bigm);
+        auto use_wide = [&](int G, bool allowed) {
+            if (!allowed) return false;
+            FftFactors fw, fn;
+            fft_factorize(G, fw, true);
+            fft_factorize(G, fn, false);
+            bool w = fw.nf < fn.nf;
+            if (fm == kFftSmall)
+                for (int k = 0; k < fw.nf; ++k) w = w && (fw.r[k] == 16 || fw.r[k] == 12);
+            return w;
+        };
+        const bool wide = use_wide(d.G2, !bigm && TC >= 16);
+        const bool wide1 = use_wide(d.G1, rows_wide(TR, fm));
+        if (wide) {
+            d.fac2 = d.fac2w;
+            d.tw2 = d.tw2w;
+        }
+        if (wide1) {
+            d.fac1 = d.fac1w;
+            d.tw1 = d.tw1w;
+        }
+        FftFactors f1, f2;
+        fft_factorize(d.G1, f1, wide1);
+        fft_factorize(d.G2, f2, wide);
+        const bool ip1 = fft_inplace_ok(f1, TR, fft_reg_cap(TR, fft_mode_cap(fm, TR))),
+                   ip2 = fft_inplace_ok(f2, TC, fft_reg_cap(TC, fft_mode_cap(fm, TC)));
+        d.fip = (ip1 ? 1 : 0) | (ip2 ? 2 : 0);
+        *lr = fft_lds_bytes(d.G1, TR, ip1);
+        *lc = fft_lds_bytes(d.G2, TC, ip2);
+    };
+    // FFT mode: the small kernels where every slice's passes fit one LDS
+    // buffer at their register cap, else the mid-size kernels (32 values per
+    // thread) below kFftSmallMax
+    auto mode_of = [&](bool bigm, int bct) {
+        if (bigm) return kFftBig;
+        const int TR = rows_T(bct, false), TC = cols_T(bct, false);
+        bool small_ok = true;
+        for (int s = 0; s < ns; ++s) {
+            FftFactors f1, f2;
+            fft_factorize(c.hs[s].G1, f1);
+            fft_factorize(c.hs[s].G2, f2);
+            small_ok = small_ok && fft_inplace_ok(f1, TR, fft_reg_cap(TR, 0)) &&
+                       fft_inplace_ok(f2, TC, fft_reg_cap(TC, 0));
+        }
+        return small_ok || !fft_mid_ ? kFftSmall : kFftMid;
+    };
+    // Where those passes would need more than 160 KB of LDS (a side above
+    // about 620 points at 16 transforms, 1240 at 8), the large-grid kernels
+    // (one column of at most 8 transforms per work-group), and for plain
+    // transforms chunks of 8; a fused SENSE chunk of 9..16 coils has no such
+    // fallback (its adjoint combines all coils in one work-group).
+    bool bigc = big_;
+    for (;;) {
+        const int fm = mode_of(bigc, c.BCT);
+        size_t mr = 0, mc = 0;
+        for (int s = 0; s < ns; ++s) {
+            NuSliceDev d = c.hs[s];
+            size_t lr = 0, lc = 0;
+            fft_plan(d, bigc, fm, c.BCT, &lr, &lc);
+            mr = std::max(mr, lr);
+            mc = std::max(mc, lc);
+        }
+        if (mr <= 160 * 1024 && mc <= 160 * 1024) break;
+        if (!bigc) {
+            bigc = true;
+            continue;
+        }
+        if (!sense && c.BCT > 8) {
+            c.BCT = 8;
+            nchunk = (int)((nrep + 7) / 8);
+            continue;
+        }
+        fail(MM_ERR_UNSUPPORTED, sense ? "nufftOp: grid side too large for the LDS FFT passes with more than 8 "
+                                         "coils fused (a SENSE operator on it needs at most 8 coils)"
+                                       : "nufftOp: grid side too large for the LDS FFT passes");
+    }
+    const int BCT = c.BCT;
+    c.fm = mode_of(bigc, BCT);
     // slice groups (2-D staged paths only)
     const int gsz = ndim_ == 2 && grid_stage_ && degrid_stage_ && slice_group_ > 0 &&
                             slice_group_ < ns ? slice_group_ : ns;
@@ -2047,29 +2134,12 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
     };
     std::vector<GridUnit> gunits;
     c.lds_gt = 16;
-    // FFT mode: the small kernels where every slice's passes fit one LDS
-    // buffer at their register cap, else the mid-size kernels (32 values per
-    // thread) below kFftSmallMax
-    {
-        const int TR = rows_T(BCT, big_), TC = cols_T(BCT, big_);
-        bool small_ok = true;
-        for (int s = 0; s < ns; ++s) {
-            const NuSliceDev& d = c.hs[s];
-            FftFactors f1, f2;
-            fft_factorize(d.G1, f1);
-            fft_factorize(d.G2, f2);
-            small_ok = small_ok && fft_inplace_ok(f1, TR, fft_reg_cap(TR, 0)) &&
-                       fft_inplace_ok(f2, TC, fft_reg_cap(TC, 0));
-        }
-        c.fm = big_ ? kFftBig : small_ok || !fft_mid_ ? kFftSmall : kFftMid;
-    }
     for (int s = 0; s < ns; ++s) {
         NuSliceDev& d = c.hs[s];
         // rows / columns per work-group (the kernels' compile-time tiling):
         // rows take all BCT transforms of max(1, 8/BCT) rows, columns 8
         // transforms (8/CL columns of CL = min(BCT, 8) transforms)
-        const int TR = big_ ? rows_T(BCT, true) : rows_T(BCT, false);
-        const int TC = big_ ? cols_T(BCT, true) : cols_T(BCT, false);
+        const int TR = rows_T(BCT, bigc), TC = cols_T(BCT, bigc);
         const int rpw = TR / BCT;
         const int cl = cols_CL(BCT);
         const int cpw = TC / cl;
@@ -2079,41 +2149,10 @@ NufftOp::Cfg& NufftOp::config(int64_t nrep, int sense, int C) {
         d.cpw = cpw;
         d.lcl = ilog2(cl);
         {
-            // column passes of 16 transforms use the radix-16/12
-            // factorization where it takes fewer passes -- in the small mode
-            // only where it is made of those radices alone (192 = 16 x 12:
-            // two passes instead of three; 0.93 -> 0.77 us per 8-coil
-            // slice), in the mid-size mode also with others (336 = 16 x 3 x
-            // 7 instead of 8 x 2 x 3 x 7), where row passes of 16 transforms
-            // take it too
-            auto use_wide = [&](int G, bool allowed) {
-                if (!allowed) return false;
-                FftFactors fw, fn;
-                fft_factorize(G, fw, true);
-                fft_factorize(G, fn, false);
-                bool w = fw.nf < fn.nf;
-                if (c.fm == kFftSmall)
-                    for (int k = 0; k < fw.nf; ++k) w = w && (fw.r[k] == 16 || fw.r[k] == 12);
-                return w;
-            };
-            const bool wide = use_wide(d.G2, !big_ && TC >= 16);
-            const bool wide1 = use_wide(d.G1, rows_wide(TR, c.fm));
-            if (wide) {
-                d.fac2 = d.fac2w;
-                d.tw2 = d.tw2w;
-            }
-            if (wide1) {
-                d.fac1 = d.fac1w;
-                d.tw1 = d.tw1w;
-            }
-            FftFactors f1, f2;
-            fft_factorize(d.G1, f1, wide1);
-            fft_factorize(d.G2, f2, wide);
-            const bool ip1 = fft_inplace_ok(f1, TR, fft_reg_cap(TR, fft_mode_cap(c.fm, TR))),
-                       ip2 = fft_inplace_ok(f2, TC, fft_reg_cap(TC, fft_mode_cap(c.fm, TC)));
-            d.fip = (ip1 ? 1 : 0) | (ip2 ? 2 : 0);
-            c.lds_rows = std::max(c.lds_rows, fft_lds_bytes(d.G1, TR, ip1));
-            c.lds_cols = std::max(c.lds_cols, fft_lds_bytes(d.G2, TC, ip2));
+            size_t lr = 0, lc = 0;
+            fft_plan(d, bigc, c.fm, BCT, &lr, &lc);
+            c.lds_rows = std::max(c.lds_rows, lr);
+            c.lds_cols = std::max(c.lds_cols, lc);
         }
         if (c.lds_rows > 160 * 1024 || c.lds_cols > 160 * 1024)
             fail(MM_ERR_UNSUPPORTED, "nufftOp: grid side too large for the LDS FFT passes");
