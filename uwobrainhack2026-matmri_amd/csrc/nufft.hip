@@ -2863,6 +2863,16 @@ void NufftOp::apply_sense(int adjoint, const float2* in, float2* out, int64_t nr
     run(config(nrep, 1, C), adjoint, in, out, maps, adjoint ? nullptr : Wt);
 }
 
+bool NufftOp::sense_fits(int C) {
+    try {
+        (void)config(1, 1, C);
+        return true;
+    } catch (const Error& e) {
+        if (e.code == MM_ERR_UNSUPPORTED) return false;
+        throw;
+    }
+}
+
 void NufftOp::apply_sense_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                                  const float* Wt) {
     if (nrep < 1) fail(MM_ERR_ARG, "nufftOp: nrep must be >= 1");

@@ -106,6 +106,9 @@ class NufftOp : public mm_op {
     // the transposed [M][coil] layout between the two halves.
     void apply_sense_normal(const float2* in, float2* out, int64_t nrep, const float2* maps, int C,
                             const float* W);
+    // Whether the fused SENSE configuration of C coils can be planned (its
+    // FFT passes fit the LDS); false where it is refused as unsupported.
+    bool sense_fits(int C);
     // Per-kernel time of the normal operator E'(W.*E x): `iters`
     // applications on the context stream with a HIP event after every
     // kernel; per kernel name: (ms per launch, launches per application).

@@ -206,7 +206,10 @@ SenseOp::SenseOp(mm_ctx* c, mm_op* S, mm_op* R, const float* Wh) : S_(S), R_(R) 
             fail(MM_ERR_ARG, "mrSampFunc: slice layouts of S and R differ");
     C_ = rc->coils();
     nu_ = dynamic_cast<NufftOp*>(S);
-    if (nu_ && C_ > 16) nu_ = nullptr;  // unfused path
+    // the unfused path S(R x): more than 16 coils, or a grid on which the
+    // fused coil chunk's FFT passes do not fit the LDS (9..16 coils on sides
+    // above about 620 points); the plain NUFFT then runs chunks of 8
+    if (nu_ && (C_ > 16 || !nu_->sense_fits(C_))) nu_ = nullptr;
     hio_ = dynamic_cast<HioOp*>(S);
     nu3_ = dynamic_cast<Nufft3Op*>(S);
     if (Wh) {

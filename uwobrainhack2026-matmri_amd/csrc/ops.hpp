@@ -55,7 +55,11 @@ class SenseOp : public mm_op {
     void set_skip(const int* flags) override { S_->set_skip(flags); }
     uint64_t config_gen() const override { return gen_ + S_->config_gen() + R_->config_gen(); }
     // the sampling operator is a NUFFT in the Toeplitz form (prepToep)
-    bool toeplitz() const { return (nu_ && nu_->is_toeplitz()) || (nu3_ && nu3_->is_toeplitz()); }
+    // the sampling operator itself (nu_ is NULL on the unfused path)
+    bool toeplitz() const {
+        const auto* n2 = dynamic_cast<const NufftOp*>(S_);
+        return (n2 && n2->is_toeplitz()) || (nu3_ && nu3_->is_toeplitz());
+    }
     mm_op* sampling() const { return S_; }
     int coils() const { return C_; }
     // per-kernel times of the last normal-operator profile
