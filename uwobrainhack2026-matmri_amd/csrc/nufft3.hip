@@ -128,7 +128,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n3_waves<n3
 // of their stores, so a thread does not wait on each load in turn.
 // loads in flight per thread in the axis passes' staging (against one at a
 // time on the nufft3 workload: g3 pass 126 -> 98 us adjoint, 103 -> 86
-// forward; 4 gave 115 / 88, 12 the same as 8)
+// forward; 4 gave 115 / 88, 12 the same as 8).  The row passes keep one
+// load at a time: staged, they measured slower (54 -> 57 us adjoint with
+// eight in flight, 57 -> 59 forward with four).
 constexpr int kN3StageU = 8;
 template <int U, class Ld, class St>
 __device__ inline void n3_stage(int tot, Ld ld, St st) {
