@@ -363,12 +363,14 @@ def test_toeplitz_sense_radix16_columns(mm, N, C):
     assert e < TOL and e_u < TOL
 
 
-def test_toeplitz_sense_forward_adjoint(mm):
+@pytest.mark.parametrize("C", [4, 20])
+def test_toeplitz_sense_forward_adjoint(mm, C):
     """mrSampFunc on a prepToep'd nufftOp (nufftOp.m:348-352): the forward is
-    T(R x) (coil images), the adjoint R'(T y); and cgne refuses the operator."""
-    print("[case] Toeplitz SENSE forward/adjoint N=48 C=4", flush=True)
-    rng = np.random.default_rng(59)
-    N, C = 48, 4
+    T(R x) (coil images), the adjoint R'(T y); and cgne refuses the operator,
+    also with more than 16 coils (the SENSE operator composed of S and R)."""
+    print(f"[case] Toeplitz SENSE forward/adjoint N=48 C={C}", flush=True)
+    rng = np.random.default_rng(59 + C)
+    N = 48
     kl = rng.random((1200, 2)) - 0.5
     maps = crand(rng, (N, N, C))
     T = mm.nufftOp([N, N], kl, None, 1, 1.5, 4).prepToep()
